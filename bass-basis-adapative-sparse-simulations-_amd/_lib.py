@@ -1,0 +1,123 @@
+"""ctypes binding of libbass_hip.so (declared in include/bass_hip.h).
+
+The library is the only compute path: importing this module never falls back
+to a CPU implementation, and every call raises if the library or a GPU is
+missing.  PyTorch supplies device buffers and the current HIP stream.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbass_hip.so")
+
+BASS_OK = 0
+BASS_E_OVERFLOW = -1
+BASS_E_ARG = -2
+BASS_E_WORKSPACE = -3
+BASS_E_HIP = -4
+BASS_E_DEVICE = -5
+
+MODE_GATE = 0
+MODE_ROT = 1
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I = ctypes.c_int
+_D = ctypes.c_double
+_SZ = ctypes.c_size_t
+
+
+class BassError(RuntimeError):
+    """A libbass_hip call failed (message from bass_last_error)."""
+
+
+class BassStats(ctypes.Structure):
+    _fields_ = [
+        ("nnz", ctypes.c_int64),
+        ("total", ctypes.c_double),
+        ("kept", ctypes.c_double),
+        ("norm", ctypes.c_double),
+        ("sum2", ctypes.c_double),
+        ("sum4", ctypes.c_double),
+        ("err", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
+# name -> (restype, argtypes); mirrors include/bass_hip.h
+SIGNATURES = {
+    "bass_abi_version": (_I, []),
+    "bass_last_error": (ctypes.c_char_p, []),
+    "bass_workspace_bytes": (_SZ, [_I64]),
+    "bass_read_stats": (_I, [_P, _SZ, ctypes.POINTER(BassStats), _P]),
+    "bass_gate": (_I, [_P, _P, _I64, _I, _I, _I, _P, _I, _I, _P, _P, _I64, ctypes.POINTER(_I64), _P, _SZ, _P]),
+    "bass_truncate": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _I64, _P, _SZ, _P]),
+    "bass_diag2q": (_I, [_P, _P, _I64, _I, _I, _P, _P]),
+    "bass_moments": (_I, [_P, _I64, _P, _SZ, _P]),
+    "bass_scale": (_I, [_P, _I64, _D, _P]),
+    "bass_gather": (_I, [_P, _P, _P, _I64, _P, _P, _P]),
+    "bass_rdm1q": (_I, [_P, _P, _I64, _I, _P, _P, _SZ, _P]),
+    "bass_ht_clear": (_I, [_P, _P, _I, _P]),
+    "bass_ht_insert": (_I, [_P, _P, _I, _P, _I64, _I64, _P, _SZ, _P]),
+    "bass_ht_lookup": (_I, [_P, _P, _I, _P, _I64, _P, _P]),
+    "bass_radix_sort": (_I, [_P, _P, _I64, _I, _P, _SZ, _P]),
+    "bass_merge_sorted": (_I, [_P, _P, _I64, _P, _P, _P, _SZ, _P]),
+    "bass_timing": (_I, [_I]),
+    "bass_timing_read": (_I, [ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D)]),
+}
+
+
+def timing(enable):
+    """Reset and enable/disable per-operation HIP-event timing."""
+    load().bass_timing(1 if enable else 0)
+
+
+def timing_read(name):
+    """(launch count, summed ms) of one timed operation."""
+    c, t = _I64(0), _D(0.0)
+    load().bass_timing_read(name.encode(), ctypes.byref(c), ctypes.byref(t))
+    return int(c.value), float(t.value)
+
+_lib = None
+
+
+def load():
+    """Load the library (raises ImportError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error():
+    return load().bass_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != BASS_OK:
+        msg = f"{what}: {last_error()} (code {rc})"
+        if rc == BASS_E_OVERFLOW:
+            raise RuntimeError("Temporary buffer overflow: " + msg)
+        raise BassError(msg)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def mat_arg(G):
+    """Complex matrix -> contiguous float64 (re, im) pairs, row-major."""
+    g = np.ascontiguousarray(np.asarray(G, dtype=np.complex128))
+    buf = np.ascontiguousarray(g.view(np.float64).ravel())
+    return buf, buf.ctypes.data_as(ctypes.c_void_p)

@@ -1,0 +1,76 @@
+"""Device operations on state buffers (thin wrappers over the C ABI).
+
+Each function launches on the context's current stream.  Functions that
+return scalars read them back through bass_read_stats (one synchronisation).
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+def gate(ctx, src, n, G, qubits, dst, mode=L.MODE_GATE, dedup=True):
+    """Apply a 1- or 2-qubit gate (or rotation) to src[:n] into dst.
+
+    Returns (n_out, hist_tiles, total) where hist_tiles feeds truncate()."""
+    nq = len(qubits)
+    M = 1 << nq
+    ctx.ensure_ws(M * max(n, 1))
+    dst.ensure(M * n)
+    buf, gp = L.mat_arg(G)
+    q0 = int(qubits[0])
+    q1 = int(qubits[1]) if nq == 2 else 0
+    ht = ctypes.c_int64(0)
+    rc = ctx.lib.bass_gate(src.xp, src.ap, n, nq, q0, q1, gp, mode, 1 if dedup else 0, dst.xp, dst.ap,
+                           dst.cap, ctypes.byref(ht), ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_gate")
+    st = ctx.read_stats("bass_gate")
+    return st.nnz, int(ht.value)
+
+
+def truncate(ctx, src, n, k, dst, renorm=True, hist_tiles=0):
+    """Top-k of src[:n] (n > k) into dst[:k].  Returns stats (kept, total, sum2, sum4, norm)."""
+    ctx.ensure_ws(n)
+    dst.ensure(k)
+    rc = ctx.lib.bass_truncate(src.xp, src.ap, n, k, dst.xp, dst.ap, dst.cap, 1 if renorm else 0,
+                               hist_tiles, ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_truncate")
+    return ctx.read_stats("bass_truncate")
+
+
+def diag2q(ctx, buf, n, q1, q2, diag):
+    d = np.ascontiguousarray(np.asarray(diag, dtype=np.complex128).view(np.float64))
+    rc = ctx.lib.bass_diag2q(buf.xp, buf.ap, n, int(q1), int(q2), d.ctypes.data_as(ctypes.c_void_p),
+                             ctx.stream)
+    L.check(rc, "bass_diag2q")
+
+
+def moments(ctx, buf, n):
+    """(sum |a|^2, sum |a|^4) of buf[:n]."""
+    ctx.ensure_ws(n)
+    rc = ctx.lib.bass_moments(buf.ap, n, ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_moments")
+    st = ctx.read_stats("bass_moments")
+    return st.sum2, st.sum4
+
+
+def scale(ctx, buf, n, div):
+    rc = ctx.lib.bass_scale(buf.ap, n, float(div), ctx.stream)
+    L.check(rc, "bass_scale")
+
+
+def rdm1q(ctx, buf, n, N):
+    """All single-qubit RDMs -> complex128[N, 2, 2] (host)."""
+    ctx.ensure_ws(n)
+    out = np.zeros((N, 4), dtype=np.float64)
+    rc = ctx.lib.bass_rdm1q(buf.xp, buf.ap, n, int(N), out.ctypes.data_as(ctypes.c_void_p), ctx.wsp,
+                            ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_rdm1q")
+    rho = np.zeros((N, 2, 2), dtype=np.complex128)
+    rho[:, 0, 0] = out[:, 0]
+    rho[:, 1, 1] = out[:, 1]
+    rho[:, 0, 1] = out[:, 2] + 1j * out[:, 3]
+    rho[:, 1, 0] = out[:, 2] - 1j * out[:, 3]
+    return rho

@@ -1,0 +1,158 @@
+// Shared device primitives for the BASS sparse-state kernels (gfx950, wave64).
+//
+// State layout in HBM (structure of arrays):
+//   keys : uint64_t[n]   basis-state bit strings, qubit q = bit q
+//   amps : double2[n]    complex128 amplitudes, (re, im) interleaved
+//
+// All floating-point work is FP64 and is compiled with -ffp-contract=off; the
+// only fused operations are the explicit ones in cmul below.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bass {
+
+constexpr uint64_t EMPTY = 0xFFFFFFFFFFFFFFFFull;  // hash sentinel, never a key for N <= 63
+constexpr int NT = 256;                             // threads per block (4 waves)
+constexpr int WAVES = NT / 64;
+
+// Error bits reported through the per-call status word.
+enum : int {
+  ERR_PROBE = 1,      // hash table probe sequence exhausted (table full)
+  ERR_MISSING = 2,    // a key inserted by the build kernel was not found
+  ERR_LOOKBACK = 4,   // decoupled look-back spin bound exceeded
+  ERR_OVERFLOW = 8,   // output capacity exceeded
+  ERR_SELECT = 16,    // top-k selection bookkeeping inconsistent
+};
+
+// Complex product x * y with one fused multiply-add per component:
+//   re = fma(xr, yr, -(xi * yi)),  im = fma(xr, yi, xi * yr).
+// This is the rounding of numpy's complex128 multiply loops (AVX2/AVX-512
+// builds) and of LLVM's fast-math contraction of (ac - bd, ad + bc) in the
+// reference's Numba kernels; the operand order matters for the imaginary part.
+__device__ __forceinline__ double2 cmul(double2 x, double2 y) {
+  return make_double2(__builtin_fma(x.x, y.x, -(x.y * y.y)), __builtin_fma(x.x, y.y, x.y * y.x));
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) {
+  return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ bool cnonzero(double2 a) { return a.x != 0.0 || a.y != 0.0; }
+__device__ __forceinline__ double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+
+// Fibonacci hashing of a 64-bit key into a power-of-two table: the top
+// log2(cap) bits of key * 2^64/phi.  shift = 64 - log2(cap).
+__device__ __forceinline__ uint64_t hslot(uint64_t key, int shift) {
+  return (key * 0x9E3779B97F4A7C15ull) >> shift;
+}
+
+__device__ __forceinline__ void set_err(int* err, int bit) { atomicOr(err, bit); }
+
+// ---------------------------------------------------------------- scans ----
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan for NT threads.  lds must hold WAVES ints.
+// Returns the exclusive prefix; total receives the block sum.
+__device__ __forceinline__ int block_excl_scan(int v, int& total, int* lds) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc = wave_incl_scan(v);
+  if (lane == 63) lds[w] = inc;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < WAVES; i++) {
+    int s = lds[i];
+    pre += (i < w) ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return pre + inc - v;
+}
+
+// Deterministic block sum of doubles (fixed shuffle tree, then waves in order).
+// lds must hold WAVES doubles.  Result valid in every thread.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < WAVES; i++) s += lds[i];
+  __syncthreads();
+  return s;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// ------------------------------------------------------ decoupled look-back --
+// Status word per tile: bits 63..62 flag (0 = not ready, 1 = aggregate,
+// 2 = inclusive prefix), bits 61..0 value.  Tiles take their id from an atomic
+// ticket so every predecessor of a tile is already running when the tile
+// starts; a predecessor publishes its aggregate before it waits on anyone, so
+// each wait below ends.  A spin bound turns a broken invariant into an error
+// bit instead of a hang.
+constexpr unsigned long long LB_MASK = (1ull << 62) - 1;
+
+__device__ __forceinline__ unsigned long long lb_load(unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by ONE thread of the tile.  Returns the exclusive prefix of `count`
+// over all tiles before `tile`.
+__device__ inline unsigned long long lookback_prefix(unsigned long long* status, unsigned int tile,
+                                                     unsigned long long count, int* err) {
+  if (tile == 0) {
+    lb_store(&status[0], (2ull << 62) | count);
+    return 0;
+  }
+  lb_store(&status[tile], (1ull << 62) | count);
+  unsigned long long excl = 0;
+  long t = (long)tile - 1;
+  unsigned int spins = 0;
+  while (t >= 0) {
+    unsigned long long s = lb_load(&status[t]);
+    unsigned int flag = (unsigned int)(s >> 62);
+    if (flag == 0) {
+      if (++spins > (1u << 26)) {
+        set_err(err, ERR_LOOKBACK);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += s & LB_MASK;
+    if (flag == 2) break;
+    --t;
+  }
+  lb_store(&status[tile], (2ull << 62) | (excl + count));
+  return excl;
+}
+
+// Tile id in dispatch order.
+__device__ __forceinline__ unsigned int take_ticket(unsigned int* ticket, unsigned int* lds_slot) {
+  if (threadIdx.x == 0) *lds_slot = atomicAdd(ticket, 1u);
+  __syncthreads();
+  unsigned int t = *lds_slot;
+  __syncthreads();
+  return t;
+}
+
+}  // namespace bass

@@ -1,0 +1,326 @@
+// Gate application on the SoA sparse state.
+//
+// One templated emission kernel covers every gate-like operation of the
+// reference (src/core/sparse_state.py, src/simulation/bass_simulator.py):
+//
+//   DEDUP=false, MODE=GATE : apply_1qubit_gate_kernel_opt / apply_2qubit_gate_kernel_opt
+//                            (each entry emits alone, exact-zero products dropped,
+//                             output in input order, out_idx ascending)
+//   DEDUP=true,  MODE=GATE : the deduplicated result of emit -> sort -> merge
+//                            (1q) and of apply_2qubit_gate_ht (2q)
+//   DEDUP=true,  MODE=ROT  : apply_1qubit_rotation_sparse / apply_2qubit_rotation_sparse
+//                            (absent members read as 0, outputs with |a| <= 1e-30 dropped)
+//
+// Deduplication does not sort.  Entries sharing the bits outside the gate
+// qubits ("rest") form a group of at most 2^QB members.  A group table
+// (open addressing, linear probing, 64-bit CAS insert of the rest key) records
+// each member's input index.  The member with the smallest input index is the
+// group leader: it gathers the other members' amplitudes and emits every
+// output of the group, summing contributions in increasing input index -- the
+// order in which the reference's sequential loop accumulates them.
+//
+// Outputs are compacted in input order of their leaders with a decoupled
+// look-back scan, so results are deterministic run to run.  While writing,
+// each tile adds |a|^2 of its outputs to a 2048-bin histogram of the top 12
+// bits of the IEEE pattern (sign + exponent) -- the first digit of the top-k
+// radix select -- and a per-tile sum of |a|^2.
+#include "common.h"
+#include "kernels.h"
+
+namespace bass {
+
+template <int QB>
+__device__ __forceinline__ int member_index(uint64_t key, int q0, int q1) {
+  if constexpr (QB == 1) {
+    return (int)((key >> q0) & 1ull);
+  } else {
+    return (int)((((key >> q0) & 1ull) << 1) | ((key >> q1) & 1ull));
+  }
+}
+
+template <int QB>
+__device__ __forceinline__ uint64_t out_bits(int o, int q0, int q1) {
+  if constexpr (QB == 1) {
+    return (uint64_t)(o & 1) << q0;
+  } else {
+    return ((uint64_t)((o >> 1) & 1) << q0) | ((uint64_t)(o & 1) << q1);
+  }
+}
+
+template <int QB>
+__global__ __launch_bounds__(NT) void k_group_build(const uint64_t* __restrict__ x, int64_t n, int q0,
+                                                     int q1, uint64_t* __restrict__ gkeys,
+                                                     int32_t* __restrict__ gmem, int shift,
+                                                     uint64_t capmask, int* err) {
+  constexpr int M = 1 << QB;
+  const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t key = x[i];
+  const uint64_t r = key & ~out_bits<QB>(M - 1, q0, q1);
+  const int c = member_index<QB>(key, q0, q1);
+  uint64_t h = hslot(r, shift);
+  for (uint64_t probe = 0; probe <= capmask; ++probe) {
+    uint64_t cur = __hip_atomic_load(&gkeys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == EMPTY) {
+      unsigned long long prev = atomicCAS((unsigned long long*)&gkeys[h],
+                                          (unsigned long long)EMPTY, (unsigned long long)r);
+      cur = (prev == (unsigned long long)EMPTY) ? r : (uint64_t)prev;
+    }
+    if (cur == r) {
+      gmem[h * M + c] = (int32_t)i;
+      return;
+    }
+    h = (h + 1) & capmask;
+  }
+  set_err(err, ERR_PROBE);
+}
+
+__device__ __forceinline__ int64_t table_find(const uint64_t* __restrict__ tkeys, uint64_t key,
+                                              int shift, uint64_t capmask) {
+  uint64_t h = hslot(key, shift);
+  for (uint64_t probe = 0; probe <= capmask; ++probe) {
+    const uint64_t cur = tkeys[h];
+    if (cur == key) return (int64_t)h;
+    if (cur == EMPTY) return -1;
+    h = (h + 1) & capmask;
+  }
+  return -1;
+}
+
+template <int QB, int MODE, bool DEDUP, int ITEMS>
+__global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ x,
+                                              const double2* __restrict__ a, int64_t n, int q0, int q1,
+                                              const uint64_t* __restrict__ gkeys,
+                                              const int32_t* __restrict__ gmem, int shift,
+                                              uint64_t capmask, GateMat G, uint64_t* __restrict__ xo,
+                                              double2* __restrict__ ao, int64_t cap_out,
+                                              unsigned long long* lb_status, unsigned int* ticket,
+                                              unsigned int* hist, double* tile_psum, int64_t* n_out,
+                                              int* err) {
+  constexpr int M = 1 << QB;
+  constexpr int TILE = NT * ITEMS;
+  __shared__ unsigned int s_hist[HIST_BINS];
+  __shared__ int s_scan[WAVES];
+  __shared__ double s_red[WAVES];
+  __shared__ unsigned int s_tile;
+  __shared__ unsigned long long s_base;
+
+  const unsigned int tile = take_ticket(ticket, &s_tile);
+  if (hist) {
+    for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  }
+  __syncthreads();
+
+  const uint64_t gmask = out_bits<QB>(M - 1, q0, q1);
+  uint64_t okey[ITEMS][M];
+  double2 oamp[ITEMS][M];
+  unsigned int omask[ITEMS];
+  int ooff[ITEMS];
+  int rtot[ITEMS];
+
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int64_t i = (int64_t)tile * TILE + (int64_t)it * NT + threadIdx.x;
+    unsigned int mask = 0;
+    if (i < n) {
+      const uint64_t key = x[i];
+      const int c = member_index<QB>(key, q0, q1);
+      const uint64_t r = key & ~gmask;
+      int32_t mem[M];
+#pragma unroll
+      for (int cc = 0; cc < M; ++cc) mem[cc] = -1;
+      if constexpr (DEDUP) {
+        const int64_t h = table_find(gkeys, r, shift, capmask);
+        if (h < 0) {
+          set_err(err, ERR_MISSING);
+          mem[c] = (int32_t)i;
+        } else {
+#pragma unroll
+          for (int cc = 0; cc < M; ++cc) mem[cc] = gmem[h * M + cc];
+        }
+      } else {
+        mem[c] = (int32_t)i;
+      }
+      int32_t lead = 0x7FFFFFFF;
+#pragma unroll
+      for (int cc = 0; cc < M; ++cc)
+        if (mem[cc] >= 0 && mem[cc] < lead) lead = mem[cc];
+      if (lead == (int32_t)i) {
+        double2 av[M];
+#pragma unroll
+        for (int cc = 0; cc < M; ++cc) {
+          if (mem[cc] < 0)
+            av[cc] = make_double2(0.0, 0.0);
+          else if (cc == c)
+            av[cc] = a[i];
+          else
+            av[cc] = a[mem[cc]];
+        }
+        // rank of each present member by input index (summation order)
+        int rank[M];
+#pragma unroll
+        for (int cc = 0; cc < M; ++cc) {
+          int rk = 0;
+#pragma unroll
+          for (int c2 = 0; c2 < M; ++c2) rk += (mem[c2] >= 0 && mem[c2] < mem[cc]) ? 1 : 0;
+          rank[cc] = rk;
+        }
+#pragma unroll
+        for (int o = 0; o < M; ++o) {
+          double2 acc = make_double2(0.0, 0.0);
+          bool any = false;
+          if constexpr (MODE == MODE_GATE) {
+            double2 term[M];
+            bool live[M];
+#pragma unroll
+            for (int cc = 0; cc < M; ++cc) {
+              term[cc] = cmul(G.m[o * M + cc], av[cc]);
+              live[cc] = mem[cc] >= 0 && cnonzero(term[cc]);
+            }
+#pragma unroll
+            for (int rr = 0; rr < M; ++rr) {
+#pragma unroll
+              for (int cc = 0; cc < M; ++cc) {
+                if (live[cc] && rank[cc] == rr) {
+                  acc = any ? cadd(acc, term[cc]) : term[cc];
+                  any = true;
+                }
+              }
+            }
+          } else {
+            acc = cmul(G.m[o * M + 0], av[0]);
+#pragma unroll
+            for (int cc = 1; cc < M; ++cc) acc = cadd(acc, cmul(G.m[o * M + cc], av[cc]));
+            any = hypot(acc.x, acc.y) > 1e-30;
+          }
+          if (any) {
+            mask |= 1u << o;
+            okey[it][o] = r | out_bits<QB>(o, q0, q1);
+            oamp[it][o] = acc;
+          }
+        }
+      }
+    }
+    omask[it] = mask;
+    int tot;
+    ooff[it] = block_excl_scan(__popc(mask), tot, s_scan);
+    rtot[it] = tot;
+  }
+
+  unsigned long long tile_total = 0;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) tile_total += (unsigned long long)rtot[it];
+  if (threadIdx.x == 0) s_base = lookback_prefix(lb_status, tile, tile_total, err);
+  __syncthreads();
+  const unsigned long long base = s_base;
+  const bool fits = base + tile_total <= (unsigned long long)cap_out;
+  if (!fits && threadIdx.x == 0) set_err(err, ERR_OVERFLOW);
+
+  double psum = 0.0;
+  unsigned long long rb = base;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    unsigned long long pos = rb + (unsigned long long)ooff[it];
+#pragma unroll
+    for (int o = 0; o < M; ++o) {
+      if ((omask[it] >> o) & 1u) {
+        const double2 v = oamp[it][o];
+        if (fits) {
+          xo[pos] = okey[it][o];
+          ao[pos] = v;
+        }
+        const double p = cabs2(v);
+        psum += p;
+        if (hist) atomicAdd(&s_hist[(unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1)], 1u);
+        ++pos;
+      }
+    }
+    rb += (unsigned long long)rtot[it];
+  }
+  if (tile == gridDim.x - 1 && threadIdx.x == 0) *n_out = (int64_t)(base + tile_total);
+  if (tile_psum) {
+    const double s = block_sum(psum, s_red);
+    if (threadIdx.x == 0) tile_psum[tile] = s;
+  }
+  if (hist) {
+    __syncthreads();
+    for (int b = threadIdx.x; b < HIST_BINS; b += NT)
+      if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
+  }
+}
+
+// In-place diagonal 2-qubit gate, src/core/sparse_state.py:178-190:
+// alpha[i] *= diag[(b_q1 << 1) | b_q2].
+__global__ __launch_bounds__(NT) void k_diag2q(const uint64_t* __restrict__ x, double2* __restrict__ a,
+                                                int64_t n, int q1, int q2, double2 d0, double2 d1,
+                                                double2 d2, double2 d3) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) {
+    const uint64_t key = x[i];
+    const int idx = (int)((((key >> q1) & 1ull) << 1) | ((key >> q2) & 1ull));
+    const double2 d = idx == 0 ? d0 : idx == 1 ? d1 : idx == 2 ? d2 : d3;
+    a[i] = cmul(a[i], d);
+  }
+}
+
+// ----------------------------------------------------------- host launch ---
+
+int launch_group_build(int qb, const uint64_t* x, int64_t n, int q0, int q1, uint64_t* gkeys,
+                       int32_t* gmem, int log2cap, int* err, hipStream_t s) {
+  if (n <= 0) return 0;
+  const int shift = 64 - log2cap;
+  const uint64_t capmask = (1ull << log2cap) - 1;
+  const dim3 grid((unsigned)((n + NT - 1) / NT));
+  if (qb == 1)
+    hipLaunchKernelGGL(k_group_build<1>, grid, dim3(NT), 0, s, x, n, q0, q1, gkeys, gmem, shift, capmask, err);
+  else
+    hipLaunchKernelGGL(k_group_build<2>, grid, dim3(NT), 0, s, x, n, q0, q1, gkeys, gmem, shift, capmask, err);
+  return (int)hipGetLastError();
+}
+
+int emit_tile_entries(int qb) { return NT * (qb == 1 ? 4 : 2); }
+
+int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* a, int64_t n, int q0,
+                int q1, const uint64_t* gkeys, const int32_t* gmem, int log2cap, const GateMat& G,
+                uint64_t* xo, double2* ao, int64_t cap_out, unsigned long long* lb_status,
+                unsigned int* ticket, unsigned int* hist, double* tile_psum, int64_t* n_out, int* err,
+                hipStream_t s) {
+  if (n <= 0) return 0;
+  const int shift = 64 - log2cap;
+  const uint64_t capmask = log2cap > 0 ? ((1ull << log2cap) - 1) : 0;
+  const int tile = emit_tile_entries(qb);
+  const dim3 grid((unsigned)((n + tile - 1) / tile));
+#define BASS_EMIT(QB_, MODE_, DEDUP_, ITEMS_)                                                        \
+  hipLaunchKernelGGL((k_emit<QB_, MODE_, DEDUP_, ITEMS_>), grid, dim3(NT), 0, s, x, a, n, q0, q1,    \
+                     gkeys, gmem, shift, capmask, G, xo, ao, cap_out, lb_status, ticket, hist,       \
+                     tile_psum, n_out, err)
+  if (qb == 1) {
+    if (!dedup)
+      BASS_EMIT(1, MODE_GATE, false, 4);
+    else if (mode == MODE_GATE)
+      BASS_EMIT(1, MODE_GATE, true, 4);
+    else
+      BASS_EMIT(1, MODE_ROT, true, 4);
+  } else {
+    if (!dedup)
+      BASS_EMIT(2, MODE_GATE, false, 2);
+    else if (mode == MODE_GATE)
+      BASS_EMIT(2, MODE_GATE, true, 2);
+    else
+      BASS_EMIT(2, MODE_ROT, true, 2);
+  }
+#undef BASS_EMIT
+  return (int)hipGetLastError();
+}
+
+int launch_diag2q(const uint64_t* x, double2* a, int64_t n, int q1, int q2, const double2* d,
+                  hipStream_t s) {
+  if (n <= 0) return 0;
+  int64_t blocks = (n + NT - 1) / NT;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_diag2q, dim3((unsigned)blocks), dim3(NT), 0, s, x, a, n, q1, q2, d[0], d[1], d[2],
+                     d[3]);
+  return (int)hipGetLastError();
+}
+
+}  // namespace bass

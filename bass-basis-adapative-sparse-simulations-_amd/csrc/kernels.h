@@ -1,0 +1,108 @@
+// Host-side launch interface shared by the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bass {
+
+constexpr int HIST_BINS = 2048;  // 11-bit digits of the |a|^2 bit pattern
+enum { MODE_GATE = 0, MODE_ROT = 1 };
+
+struct GateMat {
+  double2 m[16];  // row-major (2^QB x 2^QB), index (out, in) with in = (b_q0 << 1) | b_q1
+};
+
+// Top-k selection bookkeeping, device resident.
+struct SelState {
+  unsigned long long need;       // elements still to take inside the current bucket
+  unsigned long long prefix;     // resolved high bits of the threshold pattern
+  unsigned long long pstar;      // |a|^2 bit pattern of the k-th element
+  unsigned long long keystar;    // largest key taken among elements equal to pstar
+  unsigned long long cand1;      // candidates after digit 1
+  unsigned int cand2;            // candidates after digit 2 (atomic append)
+  unsigned int b1, b2;           // threshold buckets of digits 1 and 2
+  unsigned int pad;
+  unsigned long long def_count;  // elements above the digit-1 bucket
+};
+
+// Scalars an operation reports back to the host.
+struct OpStats {
+  int64_t nnz;    // output entry count
+  double total;   // sum |a|^2 of the operation input (truncate) or output (gate)
+  double kept;    // sum |a|^2 of the kept entries (before renormalisation)
+  double norm;    // sqrt(kept) used for renormalisation (1 if not renormalised)
+  double sum2;    // sum |a|^2 of the output as written
+  double sum4;    // sum |a|^4 of the output as written
+  int32_t err;    // error bits (ERR_*)
+  int32_t pad;
+};
+
+int launch_group_build(int qb, const uint64_t* x, int64_t n, int q0, int q1, uint64_t* gkeys,
+                       int32_t* gmem, int log2cap, int* err, hipStream_t s);
+int emit_tile_entries(int qb);
+int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* a, int64_t n, int q0,
+                int q1, const uint64_t* gkeys, const int32_t* gmem, int log2cap, const GateMat& G,
+                uint64_t* xo, double2* ao, int64_t cap_out, unsigned long long* lb_status,
+                unsigned int* ticket, unsigned int* hist, double* tile_psum, int64_t* n_out, int* err,
+                hipStream_t s);
+int launch_diag2q(const uint64_t* x, double2* a, int64_t n, int q1, int q2, const double2* d,
+                  hipStream_t s);
+
+struct SelLaunch {
+  const uint64_t* x;
+  const double2* a;
+  int64_t n;
+  unsigned long long k;
+  uint64_t* xo;
+  double2* ao;
+  int64_t cap_out;
+  int renorm;
+  bool hist_ready;       // hist1 + tile_psum already hold this input's digit-1 data
+  int64_t npsum;         // number of valid tile_psum entries when hist_ready
+  unsigned int* hist1;   // zeroed by the caller unless hist_ready
+  double* tile_psum;
+  unsigned int* hist2;   // zeroed
+  SelState* sel;         // zeroed
+  OpStats* st;
+  unsigned long long* c1p;
+  unsigned int* c1i;
+  unsigned long long* c2p;
+  unsigned int* c2i;
+  double* tile_dsum;
+  double* part_sum;
+  unsigned long long* part_cnt;
+  double* tile_s2;
+  double* tile_s4;
+  unsigned long long* lb1;  // zeroed, sel_tiles(n) + 1 words
+  unsigned int* ticket1;    // zeroed
+  unsigned long long* lb2;  // zeroed
+  unsigned int* ticket2;    // zeroed
+};
+
+int64_t sel_tiles(int64_t n);
+int sel_kept_blocks();
+int launch_select(const SelLaunch& L, hipStream_t s);
+int launch_moments(const double2* a, int64_t n, double* tile_s2, double* tile_s4, OpStats* st,
+                   hipStream_t s);
+int launch_scale(double2* a, int64_t n, double div, hipStream_t s);
+int launch_gather(const uint64_t* x, const double2* a, const int64_t* idx, int64_t m, uint64_t* xo,
+                  double2* ao, hipStream_t s);
+int rdm_blocks();
+int launch_table_build(const uint64_t* x, int64_t n, uint64_t* tkeys, int32_t* tvals, int log2cap,
+                       int overwrite, int* err, hipStream_t s);
+int launch_table_lookup(const uint64_t* tkeys, const int32_t* tvals, int log2cap, const uint64_t* q,
+                        int64_t m, int64_t* out, hipStream_t s);
+int launch_rdm(const uint64_t* x, const double2* a, int64_t n, int N, const uint64_t* tkeys,
+               const int32_t* tvals, int log2cap, double* part, double* rho, hipStream_t s);
+int launch_radix_sort(uint64_t* keys, double2* vals, int64_t n, int key_bits, uint64_t* tk, double2* tv,
+                      unsigned int* ghist, unsigned int* status, unsigned int* tickets, int* err,
+                      hipStream_t s);
+int launch_merge(const uint64_t* x, const double2* a, int64_t n, uint64_t* xo, double2* ao,
+                 unsigned long long* lb, unsigned int* ticket, int64_t* n_out, int* err, hipStream_t s);
+
+// Optional per-operation timing with HIP events recorded on the launch stream
+// (enabled with bass_timing(1)).  ktimer_begin returns -1 when disabled.
+int ktimer_begin(const char* name, hipStream_t s);
+void ktimer_end(int token, hipStream_t s);
+
+}  // namespace bass

@@ -1,0 +1,571 @@
+// Exact top-k truncation by radix select on the IEEE bit pattern of |a|^2.
+//
+// Order: larger |a|^2 first; equal |a|^2 ordered by ascending key (the
+// reference leaves ties to np.argpartition, whose choice is implementation
+// defined; src/core/truncation.py:68-113, src/simulation/bass_simulator.py:899-916).
+// |a|^2 >= 0, so its bit pattern orders like the value.
+//
+// Passes over the full input (n entries, 24 B each):
+//   digit 1 (sign + exponent, 11 bits) histogram: fused into the producing gate
+//     kernel, or k_hist_psum (16 B/entry);
+//   k_extract1: read |a|^2 (16 B/entry); entries above the digit-1 bucket are
+//     kept outright, entries inside it are appended (in order, decoupled
+//     look-back) to a candidate list together with the digit-2 histogram;
+//   k_compact: read key + amplitude (24 B/entry), write the k kept entries in
+//     input order, divided by sqrt(kept) when renormalising (24 B/kept entry).
+// Everything else touches only candidates.
+#include "common.h"
+#include "kernels.h"
+
+namespace bass {
+
+constexpr int SEL_ITEMS = 8;
+constexpr int SEL_TILE = NT * SEL_ITEMS;
+constexpr int KEPT_BLOCKS = 256;
+
+__device__ __forceinline__ unsigned long long pbits_of(double2 v) {
+  return (unsigned long long)__double_as_longlong(cabs2(v));
+}
+
+// Find, in a 2048-bin histogram, the bucket holding the need-th largest element.
+// Block of NT threads, 8 bins per thread.  Results in *b and *above (count in
+// higher buckets).  lds_scan: WAVES ints.
+__device__ void find_from_top(const unsigned int* hist, unsigned long long need, unsigned int* b,
+                              unsigned long long* above, int* lds_scan) {
+  const int t = threadIdx.x;
+  unsigned int c[8];
+  int v = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    c[j] = hist[t * 8 + j];
+    v += (int)c[j];
+  }
+  int tot;
+  const int pre = block_excl_scan(v, tot, lds_scan);
+  unsigned long long ab = (unsigned long long)(tot - pre - v);
+  if (ab < need && need <= ab + (unsigned long long)v) {
+    for (int j = 7; j >= 0; --j) {
+      if (ab + c[j] >= need) {
+        *b = (unsigned int)(t * 8 + j);
+        *above = ab;
+        break;
+      }
+      ab += c[j];
+    }
+  }
+  __syncthreads();
+}
+
+// Same, counting from the smallest bucket (for the key tie-break).
+__device__ void find_from_bottom(const unsigned int* hist, unsigned long long need, unsigned int* b,
+                                 unsigned long long* below, int* lds_scan) {
+  const int t = threadIdx.x;
+  unsigned int c[8];
+  int v = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    c[j] = hist[t * 8 + j];
+    v += (int)c[j];
+  }
+  int tot;
+  unsigned long long bl = (unsigned long long)block_excl_scan(v, tot, lds_scan);
+  if (bl < need && need <= bl + (unsigned long long)v) {
+    for (int j = 0; j < 8; ++j) {
+      if (bl + c[j] >= need) {
+        *b = (unsigned int)(t * 8 + j);
+        *below = bl;
+        break;
+      }
+      bl += c[j];
+    }
+  }
+  __syncthreads();
+}
+
+// Fixed-order sum of a double array by one block.
+__device__ double block_sum_array(const double* v, int64_t m, double* lds) {
+  double s = 0.0;
+  for (int64_t j = threadIdx.x; j < m; j += NT) s += v[j];
+  return block_sum(s, lds);
+}
+
+__global__ __launch_bounds__(NT) void k_hist_psum(const double2* __restrict__ a, int64_t n,
+                                                   unsigned int* hist, double* tile_psum) {
+  __shared__ unsigned int s_hist[HIST_BINS];
+  __shared__ double s_red[WAVES];
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  __syncthreads();
+  double ps = 0.0;
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int64_t i = (int64_t)blockIdx.x * SEL_TILE + (int64_t)it * NT + threadIdx.x;
+    if (i < n) {
+      const double p = cabs2(a[i]);
+      ps += p;
+      atomicAdd(&s_hist[(unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1)], 1u);
+    }
+  }
+  const double s = block_sum(ps, s_red);
+  if (threadIdx.x == 0) tile_psum[blockIdx.x] = s;
+  __syncthreads();
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT)
+    if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
+}
+
+// Digit 1: find the threshold bucket; also reduce the input's sum |a|^2.
+__global__ __launch_bounds__(NT) void k_find1(const unsigned int* hist, unsigned long long k,
+                                               const double* tile_psum, int64_t npsum, SelState* sel,
+                                               OpStats* st) {
+  __shared__ int s_scan[WAVES];
+  __shared__ double s_red[WAVES];
+  __shared__ unsigned int s_b;
+  __shared__ unsigned long long s_above;
+  if (threadIdx.x == 0) {
+    s_b = 0xFFFFFFFFu;
+    s_above = 0;
+  }
+  __syncthreads();
+  find_from_top(hist, k, &s_b, &s_above, s_scan);
+  const double total = block_sum_array(tile_psum, npsum, s_red);
+  if (threadIdx.x == 0) {
+    if (s_b == 0xFFFFFFFFu) set_err(&st->err, ERR_SELECT);
+    sel->b1 = s_b;
+    sel->need = k - s_above;
+    sel->prefix = (unsigned long long)s_b << 52;
+    st->total = total;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_extract1(const double2* __restrict__ a, int64_t n,
+                                                  SelState* sel, unsigned long long* __restrict__ c1p,
+                                                  unsigned int* __restrict__ c1i, unsigned int* hist2,
+                                                  double* tile_dsum, unsigned long long* lb_status,
+                                                  unsigned int* ticket, int* err) {
+  __shared__ unsigned int s_hist[HIST_BINS];
+  __shared__ int s_scan[WAVES];
+  __shared__ double s_red[WAVES];
+  __shared__ unsigned int s_tile;
+  __shared__ unsigned long long s_base;
+  __shared__ int s_dcnt;
+  const unsigned int tile = take_ticket(ticket, &s_tile);
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  if (threadIdx.x == 0) s_dcnt = 0;
+  __syncthreads();
+  const unsigned int b1 = sel->b1;
+  unsigned long long cb[SEL_ITEMS];
+  int off[SEL_ITEMS];
+  int rtot[SEL_ITEMS];
+  bool isc[SEL_ITEMS];
+  double dsum = 0.0;
+  int dcnt = 0;
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int64_t i = (int64_t)tile * SEL_TILE + (int64_t)it * NT + threadIdx.x;
+    isc[it] = false;
+    cb[it] = 0;
+    if (i < n) {
+      const double p = cabs2(a[i]);
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(p);
+      const unsigned int d = (unsigned int)(bits >> 52);
+      if (d > b1) {
+        dsum += p;
+        ++dcnt;
+      } else if (d == b1) {
+        isc[it] = true;
+        cb[it] = bits;
+        atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
+      }
+    }
+    int tot;
+    off[it] = block_excl_scan(isc[it] ? 1 : 0, tot, s_scan);
+    rtot[it] = tot;
+  }
+  unsigned long long ttot = 0;
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) ttot += (unsigned long long)rtot[it];
+  if (threadIdx.x == 0) s_base = lookback_prefix(lb_status, tile, ttot, err);
+  __syncthreads();
+  unsigned long long rb = s_base;
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    if (isc[it]) {
+      const unsigned long long pos = rb + (unsigned long long)off[it];
+      c1p[pos] = cb[it];
+      c1i[pos] = (unsigned int)((int64_t)tile * SEL_TILE + (int64_t)it * NT + threadIdx.x);
+    }
+    rb += (unsigned long long)rtot[it];
+  }
+  const double s = block_sum(dsum, s_red);
+  if (threadIdx.x == 0) tile_dsum[tile] = s;
+  atomicAdd(&s_dcnt, dcnt);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&sel->def_count, (unsigned long long)s_dcnt);
+    if (tile == gridDim.x - 1) sel->cand1 = s_base + ttot;
+  }
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT)
+    if (s_hist[b]) atomicAdd(&hist2[b], s_hist[b]);
+}
+
+__global__ __launch_bounds__(NT) void k_find2(const unsigned int* hist2, SelState* sel, OpStats* st) {
+  __shared__ int s_scan[WAVES];
+  __shared__ unsigned int s_b;
+  __shared__ unsigned long long s_above;
+  if (threadIdx.x == 0) {
+    s_b = 0xFFFFFFFFu;
+    s_above = 0;
+  }
+  __syncthreads();
+  const unsigned long long need = sel->need;
+  find_from_top(hist2, need, &s_b, &s_above, s_scan);
+  if (threadIdx.x == 0) {
+    if (s_b == 0xFFFFFFFFu) set_err(&st->err, ERR_SELECT);
+    sel->b2 = s_b;
+    sel->need = need - s_above;
+    sel->prefix |= (unsigned long long)s_b << 41;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_extract2(const unsigned long long* __restrict__ c1p,
+                                                  const unsigned int* __restrict__ c1i, SelState* sel,
+                                                  unsigned long long* __restrict__ c2p,
+                                                  unsigned int* __restrict__ c2i) {
+  const unsigned long long m1 = sel->cand1;
+  const unsigned int b2 = sel->b2;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long stride = (unsigned long long)gridDim.x * NT;
+  for (unsigned long long j0 = (unsigned long long)blockIdx.x * NT; j0 < m1; j0 += stride) {
+    const unsigned long long j = j0 + threadIdx.x;
+    bool take = false;
+    unsigned long long bits = 0;
+    if (j < m1) {
+      bits = c1p[j];
+      take = ((unsigned int)(bits >> 41) & (HIST_BINS - 1)) == b2;
+    }
+    const unsigned long long bal = __ballot(take);
+    if (bal) {
+      unsigned int base = 0;
+      const int leader = __ffsll((long long)bal) - 1;
+      if (lane == leader) base = atomicAdd(&sel->cand2, (unsigned int)__popcll(bal));
+      base = __shfl(base, leader, 64);
+      if (take) {
+        const unsigned int r = (unsigned int)__popcll(bal & ((1ull << lane) - 1ull));
+        c2p[base + r] = bits;
+        c2i[base + r] = c1i[j];
+      }
+    }
+  }
+}
+
+// Resolve the remaining bits of the threshold from the (small) digit-2
+// candidate list, then the key tie-break among entries equal to it.
+__global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restrict__ c2p,
+                                                const unsigned int* __restrict__ c2i,
+                                                const uint64_t* __restrict__ x, SelState* sel,
+                                                OpStats* st) {
+  __shared__ unsigned int s_hist[HIST_BINS];
+  __shared__ int s_scan[WAVES];
+  __shared__ unsigned int s_b;
+  __shared__ unsigned long long s_ab;
+  const unsigned int m2 = sel->cand2;
+  unsigned long long need = sel->need;
+  unsigned long long prefix = sel->prefix;
+  const int shifts[4] = {30, 19, 8, 0};
+  const int widths[4] = {11, 11, 11, 8};
+  unsigned int cnt_eq = 0;
+  for (int lv = 0; lv < 4; ++lv) {
+    const int sh = shifts[lv], wd = widths[lv];
+    for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+    if (threadIdx.x == 0) {
+      s_b = 0xFFFFFFFFu;
+      s_ab = 0;
+    }
+    __syncthreads();
+    const unsigned long long hi = prefix >> (sh + wd);
+    for (unsigned int j = threadIdx.x; j < m2; j += NT) {
+      const unsigned long long bits = c2p[j];
+      if ((bits >> (sh + wd)) == hi)
+        atomicAdd(&s_hist[(unsigned int)(bits >> sh) & ((1u << wd) - 1u)], 1u);
+    }
+    __syncthreads();
+    find_from_top(s_hist, need, &s_b, &s_ab, s_scan);
+    if (s_b == 0xFFFFFFFFu) {
+      if (threadIdx.x == 0) set_err(&st->err, ERR_SELECT);
+      return;
+    }
+    prefix |= (unsigned long long)s_b << sh;
+    need -= s_ab;
+    cnt_eq = s_hist[s_b];
+    __syncthreads();
+  }
+  const unsigned long long pstar = prefix;
+  unsigned long long keystar = ~0ull;
+  if (need < cnt_eq) {
+    // need-th smallest key among entries whose pattern equals pstar
+    unsigned long long kp = 0;
+    for (int sh = 56; sh >= 0; sh -= 8) {
+      for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+      if (threadIdx.x == 0) {
+        s_b = 0xFFFFFFFFu;
+        s_ab = 0;
+      }
+      __syncthreads();
+      for (unsigned int j = threadIdx.x; j < m2; j += NT) {
+        if (c2p[j] != pstar) continue;
+        const uint64_t key = x[c2i[j]];
+        const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
+        if (match) atomicAdd(&s_hist[(unsigned int)(key >> sh) & 255u], 1u);
+      }
+      __syncthreads();
+      find_from_bottom(s_hist, need, &s_b, &s_ab, s_scan);
+      if (s_b == 0xFFFFFFFFu) {
+        if (threadIdx.x == 0) set_err(&st->err, ERR_SELECT);
+        return;
+      }
+      kp |= (unsigned long long)s_b << sh;
+      need -= s_ab;
+      __syncthreads();
+    }
+    keystar = kp;
+  }
+  if (threadIdx.x == 0) {
+    sel->pstar = pstar;
+    sel->keystar = keystar;
+  }
+}
+
+__device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t key, unsigned long long pstar,
+                                          unsigned long long keystar) {
+  return bits > pstar || (bits == pstar && key <= keystar);
+}
+
+__global__ __launch_bounds__(NT) void k_keptsum(const unsigned long long* __restrict__ c1p,
+                                                 const unsigned int* __restrict__ c1i,
+                                                 const uint64_t* __restrict__ x, const SelState* sel,
+                                                 double* part_sum, unsigned long long* part_cnt) {
+  __shared__ double s_red[WAVES];
+  __shared__ int s_cnt;
+  const unsigned long long m1 = sel->cand1;
+  const unsigned long long pstar = sel->pstar, keystar = sel->keystar;
+  const unsigned long long chunk = (m1 + gridDim.x - 1) / gridDim.x;
+  const unsigned long long lo = (unsigned long long)blockIdx.x * chunk;
+  const unsigned long long hi = lo + chunk < m1 ? lo + chunk : m1;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  double s = 0.0;
+  int c = 0;
+  for (unsigned long long j = lo + threadIdx.x; j < hi; j += NT) {
+    const unsigned long long bits = c1p[j];
+    bool keep = bits > pstar;
+    if (bits == pstar) keep = x[c1i[j]] <= keystar;
+    if (keep) {
+      s += __longlong_as_double((long long)bits);
+      ++c;
+    }
+  }
+  const double tot = block_sum(s, s_red);
+  atomicAdd(&s_cnt, c);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part_sum[blockIdx.x] = tot;
+    part_cnt[blockIdx.x] = (unsigned long long)s_cnt;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_sel_finalize(const double* tile_dsum, int64_t ntiles,
+                                                      const double* part_sum,
+                                                      const unsigned long long* part_cnt, int nparts,
+                                                      const SelState* sel, unsigned long long k,
+                                                      int renorm, OpStats* st) {
+  __shared__ double s_red[WAVES];
+  const double d = block_sum_array(tile_dsum, ntiles, s_red);
+  const double c = block_sum_array(part_sum, nparts, s_red);
+  if (threadIdx.x == 0) {
+    unsigned long long cnt = sel->def_count;
+    for (int j = 0; j < nparts; ++j) cnt += part_cnt[j];
+    if (cnt != k) set_err(&st->err, ERR_SELECT);
+    const double kept = d + c;
+    st->kept = kept;
+    const double norm = sqrt(kept);
+    st->norm = (renorm && norm > 1e-30) ? norm : 1.0;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x,
+                                                 const double2* __restrict__ a, int64_t n,
+                                                 const SelState* sel, uint64_t* __restrict__ xo,
+                                                 double2* __restrict__ ao, int64_t cap_out,
+                                                 OpStats* st, double* tile_s2, double* tile_s4,
+                                                 unsigned long long* lb_status, unsigned int* ticket,
+                                                 int* err) {
+  __shared__ int s_scan[WAVES];
+  __shared__ double s_red[WAVES];
+  __shared__ unsigned int s_tile;
+  __shared__ unsigned long long s_base;
+  const unsigned int tile = take_ticket(ticket, &s_tile);
+  const unsigned long long pstar = sel->pstar, keystar = sel->keystar;
+  // numpy divides complex128 by a real norm as a product with 1/norm
+  const double inv = 1.0 / st->norm;
+  uint64_t kk[SEL_ITEMS];
+  double2 vv[SEL_ITEMS];
+  bool keep[SEL_ITEMS];
+  int off[SEL_ITEMS];
+  int rtot[SEL_ITEMS];
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int64_t i = (int64_t)tile * SEL_TILE + (int64_t)it * NT + threadIdx.x;
+    keep[it] = false;
+    if (i < n) {
+      kk[it] = x[i];
+      vv[it] = a[i];
+      keep[it] = kept_pred(pbits_of(vv[it]), kk[it], pstar, keystar);
+    }
+    int tot;
+    off[it] = block_excl_scan(keep[it] ? 1 : 0, tot, s_scan);
+    rtot[it] = tot;
+  }
+  unsigned long long ttot = 0;
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) ttot += (unsigned long long)rtot[it];
+  if (threadIdx.x == 0) s_base = lookback_prefix(lb_status, tile, ttot, err);
+  __syncthreads();
+  const unsigned long long base = s_base;
+  const bool fits = base + ttot <= (unsigned long long)cap_out;
+  if (!fits && threadIdx.x == 0) set_err(err, ERR_OVERFLOW);
+  unsigned long long rb = base;
+  double s2 = 0.0, s4 = 0.0;
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    if (keep[it]) {
+      const double2 v = make_double2(vv[it].x * inv, vv[it].y * inv);
+      const unsigned long long pos = rb + (unsigned long long)off[it];
+      if (fits) {
+        xo[pos] = kk[it];
+        ao[pos] = v;
+      }
+      const double p = cabs2(v);
+      s2 += p;
+      s4 += p * p;
+    }
+    rb += (unsigned long long)rtot[it];
+  }
+  const double t2 = block_sum(s2, s_red);
+  const double t4 = block_sum(s4, s_red);
+  if (threadIdx.x == 0) {
+    tile_s2[tile] = t2;
+    tile_s4[tile] = t4;
+    if (tile == gridDim.x - 1) st->nnz = (int64_t)(base + ttot);
+  }
+}
+
+// Fixed-order reduction of per-tile sums into the stats block.
+__global__ __launch_bounds__(NT) void k_sums_finalize(const double* tile_s2, const double* tile_s4,
+                                                       int64_t ntiles, OpStats* st) {
+  __shared__ double s_red[WAVES];
+  const double s2 = block_sum_array(tile_s2, ntiles, s_red);
+  const double s4 = block_sum_array(tile_s4, ntiles, s_red);
+  if (threadIdx.x == 0) {
+    st->sum2 = s2;
+    st->sum4 = s4;
+  }
+}
+
+// sum |a|^2 and sum |a|^4 per tile.
+__global__ __launch_bounds__(NT) void k_moments(const double2* __restrict__ a, int64_t n, double* tile_s2,
+                                                 double* tile_s4) {
+  __shared__ double s_red[WAVES];
+  double s2 = 0.0, s4 = 0.0;
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int64_t i = (int64_t)blockIdx.x * SEL_TILE + (int64_t)it * NT + threadIdx.x;
+    if (i < n) {
+      const double p = cabs2(a[i]);
+      s2 += p;
+      s4 += p * p;
+    }
+  }
+  const double t2 = block_sum(s2, s_red);
+  const double t4 = block_sum(s4, s_red);
+  if (threadIdx.x == 0) {
+    tile_s2[blockIdx.x] = t2;
+    tile_s4[blockIdx.x] = t4;
+  }
+}
+
+// a[i] /= div for a real div, rounded as numpy does it: a[i] * (1 / div)
+__global__ __launch_bounds__(NT) void k_scale(double2* __restrict__ a, int64_t n, double div) {
+  const double inv = 1.0 / div;
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) {
+    const double2 v = a[i];
+    a[i] = make_double2(v.x * inv, v.y * inv);
+  }
+}
+
+// Gather x_out[j] = x[idx[j]], a_out[j] = a[idx[j]]; optional sum |a|^2 per block.
+__global__ __launch_bounds__(NT) void k_gather(const uint64_t* __restrict__ x, const double2* __restrict__ a,
+                                                const int64_t* __restrict__ idx, int64_t m,
+                                                uint64_t* __restrict__ xo, double2* __restrict__ ao) {
+  const int64_t stride = (int64_t)gridDim.x * NT;
+  for (int64_t j = (int64_t)blockIdx.x * NT + threadIdx.x; j < m; j += stride) {
+    const int64_t s = idx[j];
+    xo[j] = x[s];
+    ao[j] = a[s];
+  }
+}
+
+// ----------------------------------------------------------- host side ----
+
+int64_t sel_tiles(int64_t n) { return (n + SEL_TILE - 1) / SEL_TILE; }
+int sel_kept_blocks() { return KEPT_BLOCKS; }
+
+int launch_select(const SelLaunch& L, hipStream_t s) {
+  const int64_t tiles = sel_tiles(L.n);
+  int64_t npsum = L.npsum;
+  if (!L.hist_ready) {
+    hipLaunchKernelGGL(k_hist_psum, dim3((unsigned)tiles), dim3(NT), 0, s, L.a, L.n, L.hist1, L.tile_psum);
+    npsum = tiles;
+  }
+  hipLaunchKernelGGL(k_find1, dim3(1), dim3(NT), 0, s, L.hist1, L.k, L.tile_psum, npsum, L.sel, L.st);
+  hipLaunchKernelGGL(k_extract1, dim3((unsigned)tiles), dim3(NT), 0, s, L.a, L.n, L.sel, L.c1p, L.c1i,
+                     L.hist2, L.tile_dsum, L.lb1, L.ticket1, &L.st->err);
+  hipLaunchKernelGGL(k_find2, dim3(1), dim3(NT), 0, s, L.hist2, L.sel, L.st);
+  int64_t eb = (L.n + NT - 1) / NT;
+  if (eb > 1024) eb = 1024;
+  hipLaunchKernelGGL(k_extract2, dim3((unsigned)eb), dim3(NT), 0, s, L.c1p, L.c1i, L.sel, L.c2p, L.c2i);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(NT), 0, s, L.c2p, L.c2i, L.x, L.sel, L.st);
+  hipLaunchKernelGGL(k_keptsum, dim3(KEPT_BLOCKS), dim3(NT), 0, s, L.c1p, L.c1i, L.x, L.sel, L.part_sum,
+                     L.part_cnt);
+  hipLaunchKernelGGL(k_sel_finalize, dim3(1), dim3(NT), 0, s, L.tile_dsum, tiles, L.part_sum, L.part_cnt,
+                     KEPT_BLOCKS, L.sel, L.k, L.renorm, L.st);
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)tiles), dim3(NT), 0, s, L.x, L.a, L.n, L.sel, L.xo, L.ao,
+                     L.cap_out, L.st, L.tile_s2, L.tile_s4, L.lb2, L.ticket2, &L.st->err);
+  hipLaunchKernelGGL(k_sums_finalize, dim3(1), dim3(NT), 0, s, L.tile_s2, L.tile_s4, tiles, L.st);
+  return (int)hipGetLastError();
+}
+
+int launch_moments(const double2* a, int64_t n, double* tile_s2, double* tile_s4, OpStats* st,
+                   hipStream_t s) {
+  const int64_t tiles = sel_tiles(n);
+  if (tiles > 0)
+    hipLaunchKernelGGL(k_moments, dim3((unsigned)tiles), dim3(NT), 0, s, a, n, tile_s2, tile_s4);
+  hipLaunchKernelGGL(k_sums_finalize, dim3(1), dim3(NT), 0, s, tile_s2, tile_s4, tiles, st);
+  return (int)hipGetLastError();
+}
+
+int launch_scale(double2* a, int64_t n, double div, hipStream_t s) {
+  if (n <= 0) return 0;
+  int64_t blocks = (n + NT - 1) / NT;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_scale, dim3((unsigned)blocks), dim3(NT), 0, s, a, n, div);
+  return (int)hipGetLastError();
+}
+
+int launch_gather(const uint64_t* x, const double2* a, const int64_t* idx, int64_t m, uint64_t* xo,
+                  double2* ao, hipStream_t s) {
+  if (m <= 0) return 0;
+  int64_t blocks = (m + NT - 1) / NT;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)blocks), dim3(NT), 0, s, x, a, idx, m, xo, ao);
+  return (int)hipGetLastError();
+}
+
+}  // namespace bass

@@ -1,0 +1,151 @@
+/* C ABI of the MI355X (gfx950) sparse-state core.
+ *
+ * Drop-in boundary for the reference's Numba kernels and drivers:
+ *   src/core/sparse_state.py, src/core/truncation.py,
+ *   src/simulation/bass_simulator.py, src/simulation/simulator.py.
+ * The reference binds these kernels as @njit functions on NumPy arrays; the
+ * Python host layer of this package binds the entry points below with ctypes
+ * and keeps the reference's function names and argument order.
+ *
+ * Conventions
+ *   - keys  : uint64_t[n] device array, qubit q = bit q (reference layout).
+ *   - amps  : double[2 n] device array, complex128 (re, im) interleaved.
+ *   - gate / unitary matrices: host double arrays, row-major complex
+ *     (re, im) pairs, 2x2 (8 doubles) or 4x4 (32 doubles); 2-qubit index
+ *     (b_q1 << 1) | b_q2 as in the reference.
+ *   - ws    : caller-owned device workspace of bass_workspace_bytes(max_n)
+ *     bytes; the library performs no device allocation of its own.
+ *   - stream: hipStream_t (passed as void*); every call is asynchronous on it
+ *     except those documented as synchronous.
+ *   - return: 0 on success, a negative BASS_E* code otherwise;
+ *     bass_last_error() describes the last failure of the calling thread.
+ */
+#ifndef BASS_HIP_H
+#define BASS_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BASS_OK = 0,
+  BASS_E_OVERFLOW = -1, /* output capacity exceeded; the reference returns -1
+                           (src/core/sparse_state.py:117-118) */
+  BASS_E_ARG = -2,      /* invalid argument */
+  BASS_E_WORKSPACE = -3,/* workspace too small for this call */
+  BASS_E_HIP = -4,      /* HIP runtime error */
+  BASS_E_DEVICE = -5    /* device-side failure (probe limit, select bookkeeping) */
+};
+
+/* Scalars a device operation reports (read with bass_read_stats). */
+typedef struct bass_stats {
+  int64_t nnz;  /* output entry count */
+  double total; /* sum |a|^2 of the truncation input (or of the gate output) */
+  double kept;  /* sum |a|^2 of the kept entries before renormalisation */
+  double norm;  /* divisor applied by renormalisation (1 when none) */
+  double sum2;  /* sum |a|^2 of the output as written */
+  double sum4;  /* sum |a|^4 of the output as written */
+  int32_t err;  /* device error bits (0 = none) */
+  int32_t pad;
+} bass_stats;
+
+int bass_abi_version(void);
+const char* bass_last_error(void);
+
+/* Workspace size (bytes) for operations whose inputs and outputs hold at most
+ * max_entries entries. */
+size_t bass_workspace_bytes(int64_t max_entries);
+
+/* Synchronous: waits for the stream, copies the last operation's scalars to
+ * *out and clears the device error bits.  Returns BASS_E_DEVICE when err != 0. */
+int bass_read_stats(void* ws, size_t ws_bytes, bass_stats* out, void* stream);
+
+/* Gate application.  Replaces:
+ *   dedup=0, nq=1: apply_1qubit_gate_kernel_opt   (src/core/sparse_state.py:15-67)
+ *   dedup=0, nq=2: apply_2qubit_gate_kernel_opt   (src/core/sparse_state.py:70-124)
+ *   dedup=1, nq=1, mode=0: apply_1qubit_gate_kernel_opt + radix_sort_uint64 +
+ *                  merge_sorted_kernel (src/core/sparse_state.py:268-285), as a set
+ *   dedup=1, nq=2, mode=0: apply_2qubit_gate_ht   (src/simulation/bass_simulator.py:292-354)
+ *   dedup=1, mode=1: apply_1qubit_rotation_sparse (bass_simulator.py:156-199) /
+ *                  apply_2qubit_rotation_sparse (bass_simulator.py:237-289)
+ * Writes the outputs to (xo, ao) and, in the workspace, the |a|^2 digit
+ * histogram consumed by a following bass_truncate(..., hist_tiles).
+ * *hist_tiles receives the value to pass on (0 when no histogram was made).
+ * Non-deduplicating calls keep the reference's output order (input order,
+ * out index ascending); deduplicating calls order outputs by the first input
+ * entry of each group.  Returns BASS_E_OVERFLOW if cap_out is too small. */
+int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int q1,
+              const double* G, int mode, int dedup, uint64_t* xo, double* ao, int64_t cap_out,
+              int64_t* hist_tiles, void* ws, size_t ws_bytes, void* stream);
+
+/* Top-k truncation: keep the k entries of largest |a|^2 (ties: smaller key),
+ * compacted in input order into (xo, ao); renorm=1 divides them by
+ * sqrt(sum kept |a|^2).  Replaces top_k_truncate_kernel_opt
+ * (src/core/truncation.py:68-113; renorm=0, gamma_sq = stats.kept) and the
+ * driver truncations of src/simulation/bass_simulator.py:565-572, :899-916 and
+ * src/simulation/simulator.py:169-201 (renorm=1).  Requires n > k.
+ * hist_tiles: value reported by the bass_gate call that produced (x, a), or 0. */
+int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo,
+                  double* ao, int64_t cap_out, int renorm, int64_t hist_tiles, void* ws,
+                  size_t ws_bytes, void* stream);
+
+/* In place: a[i] *= diag[(b_q1 << 1) | b_q2]; diag = 4 complex (8 doubles).
+ * Replaces apply_2qubit_diagonal_kernel (src/core/sparse_state.py:178-190). */
+int bass_diag2q(const uint64_t* x, double* a, int64_t n, int q1, int q2, const double* diag,
+                void* stream);
+
+/* stats.sum2 = sum |a|^2, stats.sum4 = sum |a|^4 (fixed-order reduction).
+ * Serves _renorm / _pr (src/simulation/bass_simulator.py:551-554, :574-578). */
+int bass_moments(const double* a, int64_t n, void* ws, size_t ws_bytes, void* stream);
+
+/* In place a[i] /= div (componentwise).  Serves _renorm. */
+int bass_scale(double* a, int64_t n, double div, void* stream);
+
+/* Gather (xo[j], ao[j]) = (x[idx[j]], a[idx[j]]), idx: int64 device array. */
+int bass_gather(const uint64_t* x, const double* a, const int64_t* idx, int64_t m, uint64_t* xo,
+                double* ao, void* stream);
+
+/* All single-qubit RDMs.  Synchronous: rho_out (host) receives N x 4 doubles
+ * (rho00, rho11, Re rho01, Im rho01) per qubit.  Replaces ht_build +
+ * compute_all_1qubit_rdms (src/simulation/bass_simulator.py:108-123, :441-470). */
+int bass_rdm1q(const uint64_t* x, const double* a, int64_t n, int N, double* rho_out, void* ws,
+               size_t ws_bytes, void* stream);
+
+/* Hash table of keys -> int32 index, capacity 2^log2cap slots, EMPTY =
+ * 0xFFFFFFFFFFFFFFFF.  Replaces ht_clear / ht_build / ht_insert / ht_lookup
+ * (src/simulation/bass_simulator.py:100-153) with batched device calls.
+ * The slot hash differs from the reference's key & (cap-1); lookups return the
+ * same indices. */
+int bass_ht_clear(uint64_t* ht_keys, int32_t* ht_vals, int log2cap, void* stream);
+int bass_ht_insert(uint64_t* ht_keys, int32_t* ht_vals, int log2cap, const uint64_t* keys,
+                   int64_t n, int64_t index_base, void* ws, size_t ws_bytes, void* stream);
+int bass_ht_lookup(const uint64_t* ht_keys, const int32_t* ht_vals, int log2cap,
+                   const uint64_t* keys, int64_t n, int64_t* out, void* stream);
+
+/* Stable LSD radix sort of keys[0:n) (low key_bits bits significant, the rest
+ * must be zero) carrying complex128 payload.  Replaces radix_sort_uint64
+ * (src/core/sparse_state.py:150-172).  Uses ws as the ping-pong buffer. */
+int bass_radix_sort(uint64_t* keys, double* vals, int64_t n, int key_bits, void* ws,
+                    size_t ws_bytes, void* stream);
+
+/* Run-length sum of equal adjacent keys of a sorted array into (xo, ao); zero
+ * sums are kept.  Replaces merge_sorted_kernel (src/core/sparse_state.py:127-147). */
+int bass_merge_sorted(const uint64_t* x, const double* a, int64_t n, uint64_t* xo, double* ao,
+                      void* ws, size_t ws_bytes, void* stream);
+
+/* Measurement: bass_timing(1) resets and enables per-operation timing with HIP
+ * events recorded on each call's stream around its kernels ("gate1q", "gate2q",
+ * "rot1q", "rot2q", "emit1q", "emit2q", "select", "diag2q", "rdm", "sort");
+ * bass_timing_read waits for the recorded events and returns the launch count
+ * and summed milliseconds of one operation.  No reference counterpart (the
+ * reference times with perf_counter around simulate(),
+ * src/experimentation/runner.py:833-846). */
+int bass_timing(int enable);
+int bass_timing_read(const char* name, int64_t* count, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BASS_HIP_H */

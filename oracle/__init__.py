@@ -1,0 +1,34 @@
+"""CPU oracle for the BASS sparse-state hot path -- TEST INFRASTRUCTURE ONLY.
+
+This package restates, in numpy (and, under ``oracle/port/``, in single-thread
+C++), the algorithm of the reference's sparse-state core:
+
+* ``kernels.py``  -- the Numba kernels of ``src/core/sparse_state.py``,
+  ``src/core/truncation.py`` and ``src/simulation/bass_simulator.py``;
+* ``drivers.py``  -- ``SparseState``, ``BASS`` and ``FixedBasisSimulator``
+  (``src/core/sparse_state.py:193-415``, ``src/simulation/bass_simulator.py:478-929``,
+  ``src/simulation/simulator.py:61-253``);
+* ``exact.py``    -- the dense ``ExactSimulator`` and the fidelity / participation
+  ratio metrics (``src/simulation/exact_simulator.py``, ``src/benchmarking/fidelity.py``);
+* ``circuits.py`` -- gate matrices and circuit generators (``src/core/gates.py``,
+  ``src/utils/random_circuits.py``, ``src/experimentation/runner.py``).
+
+Every function cites the reference file:line it follows.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import anything from here, and only as the checker.  The product
+(the HIP library and its Python host layer) never imports, links or executes
+this package.
+
+Parity pins (what checks each function; the tests named live in ``tests/``):
+
+* gate matrices / TFIM + RFIM generators / dense simulator: closed forms and the
+  reference-held ``rotation_diag.csv`` PRZ value (``test_oracle_fixtures.py``);
+* 1q / 2q emit, merge, radix sort, diagonal, hash 2q, rotation, RDMs, eigh_2x2:
+  dense brute force at n <= 10, ``numpy.linalg.eigh``, invariants
+  (``test_oracle_kernels.py``);
+* BASS / FixedBasis drivers end to end: the reference-held per-trial fidelity and
+  participation ratio in ``tests/results/fidelity_pr_raw.npz`` (RFIM rows) and
+  ``nscaling_*.npz`` (``test_oracle_fixtures.py``);
+* random-k truncation (LCG partial Fisher-Yates): closed-form LCG sequence.
+"""

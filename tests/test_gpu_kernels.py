@@ -1,0 +1,268 @@
+"""Device kernels vs the CPU oracle on seeded inputs (one op at a time)."""
+
+import numpy as np
+import pytest
+import torch
+
+from _parity import assert_same_entries, assert_topk_sets, numpy_uses_fma_cmul
+from conftest import pkg_module, random_state, random_unitary
+from oracle import drivers as OD
+from oracle import kernels as OK
+
+pytestmark = pytest.mark.gpu
+
+EXACT = None
+
+
+def exact():
+    global EXACT
+    if EXACT is None:
+        EXACT = numpy_uses_fma_cmul()
+    return EXACT
+
+
+def ss():
+    return pkg_module("core.sparse_state")
+
+
+def bs():
+    return pkg_module("simulation.bass_simulator")
+
+
+def tr():
+    return pkg_module("core.truncation")
+
+
+def _gate_with_zero(rng, d):
+    G = random_unitary(d, rng)
+    G[0, d - 1] = 0.0  # exercise the exact-zero emission rule
+    return G
+
+
+@pytest.mark.parametrize("n,nbits,q", [(1, 4, 0), (777, 12, 3), (20000, 20, 19), (5000, 16, 0)])
+def test_emit_1q_reference_order(gpu_ctx, n, nbits, q):
+    print(f"[case] emit1q n={n} nbits={nbits} q={q}", flush=True)
+    rng = np.random.default_rng(n + q)
+    x, a = random_state(n, nbits, seed=n)
+    G = _gate_with_zero(rng, 2)
+    ox, oa = np.zeros(2 * n, np.uint64), np.zeros(2 * n, np.complex128)
+    m = OK.apply_1qubit_gate_kernel_opt(x, a, n, ox, oa, G, q)
+    gx, ga = np.zeros(2 * n, np.uint64), np.zeros(2 * n, np.complex128)
+    g = ss().apply_1qubit_gate_kernel_opt(x, a, n, gx, ga, G, q)
+    assert g == m
+    assert np.array_equal(gx[:m], ox[:m]), "emission order differs"
+    np.testing.assert_allclose(ga[:m], oa[:m], rtol=1e-12, atol=1e-15)
+    if exact():
+        assert np.array_equal(ga[:m].view(np.float64), oa[:m].view(np.float64))
+
+
+def test_emit_2q_reference_order_and_overflow(gpu_ctx):
+    print("[case] emit2q", flush=True)
+    rng = np.random.default_rng(5)
+    n = 3000
+    x, a = random_state(n, 14, seed=11)
+    G = _gate_with_zero(rng, 4)
+    ox, oa = np.zeros(4 * n, np.uint64), np.zeros(4 * n, np.complex128)
+    m = OK.apply_2qubit_gate_kernel_opt(x, a, n, ox, oa, G, 5, 2)
+    gx, ga = np.zeros(4 * n, np.uint64), np.zeros(4 * n, np.complex128)
+    g = ss().apply_2qubit_gate_kernel_opt(x, a, n, gx, ga, G, 5, 2)
+    assert g == m
+    assert np.array_equal(gx[:m], ox[:m])
+    np.testing.assert_allclose(ga[:m], oa[:m], rtol=1e-12, atol=1e-15)
+    small_x, small_a = np.zeros(m - 1, np.uint64), np.zeros(m - 1, np.complex128)
+    assert ss().apply_2qubit_gate_kernel_opt(x, a, n, small_x, small_a, G, 5, 2) == -1
+    assert OK.apply_2qubit_gate_kernel_opt(x, a, n, small_x, small_a, G, 5, 2) == -1
+
+
+def _oracle_1q_dedup(x, a, n, G, q):
+    ox, oa = np.zeros(2 * n, np.uint64), np.zeros(2 * n, np.complex128)
+    m = OK.apply_1qubit_gate_kernel_opt(x, a, n, ox, oa, G, q)
+    OK.radix_sort_uint64(ox, oa, m)
+    m = OK.merge_sorted_kernel(ox, oa, m)
+    return ox[:m], oa[:m]
+
+
+@pytest.mark.parametrize("n,nbits,q", [(2, 3, 1), (4096, 12, 0), (30000, 18, 7), (30000, 15, 14)])
+def test_gate_1q_dedup(gpu_ctx, n, nbits, q):
+    print(f"[case] gate1q-dedup n={n} nbits={nbits} q={q}", flush=True)
+    rng = np.random.default_rng(q)
+    x, a = random_state(n, nbits, seed=3 * n + q)
+    G = _gate_with_zero(rng, 2)
+    okx, oka = _oracle_1q_dedup(x, a, n, G, q)
+    ops = pkg_module("_ops")
+    dev = pkg_module("_device")
+    src = dev.Buffers(n, gpu_ctx.device)
+    src.x[:n] = torch.from_numpy(x.view(np.int64)).to(gpu_ctx.device)
+    src.a[:n] = torch.from_numpy(a).to(gpu_ctx.device)
+    dst = dev.Buffers(2 * n, gpu_ctx.device)
+    m, _ = ops.gate(gpu_ctx, src, n, G, (q,), dst, dedup=True)
+    assert m == len(okx)
+    gx = dev.to_host_keys(dst.x, m)
+    ga = dev.to_host_amps(dst.a, m)
+    assert_same_entries(gx, ga, okx, oka, exact=exact())
+
+
+@pytest.mark.parametrize("n,nbits,q1,q2", [(3, 3, 0, 2), (5000, 13, 4, 1), (40000, 18, 17, 0)])
+def test_gate_2q_ht(gpu_ctx, n, nbits, q1, q2):
+    print(f"[case] gate2q-ht n={n} q=({q1},{q2})", flush=True)
+    rng = np.random.default_rng(n)
+    x, a = random_state(n, nbits, seed=n + 7)
+    G = _gate_with_zero(rng, 4)
+    ox, oa = np.zeros(4 * n, np.uint64), np.zeros(4 * n, np.complex128)
+    m = OK.apply_2qubit_gate_ht(x, a, n, ox, oa, G, q1, q2)
+    gx, ga = np.zeros(4 * n, np.uint64), np.zeros(4 * n, np.complex128)
+    g = bs().apply_2qubit_gate_ht(x, a, n, gx, ga, G, q1, q2, None, None)
+    assert g == m
+    assert_same_entries(gx[:g], ga[:g], ox[:m], oa[:m], exact=exact())
+
+
+@pytest.mark.parametrize("nq", [1, 2])
+def test_rotation(gpu_ctx, nq):
+    print(f"[case] rotation nq={nq}", flush=True)
+    rng = np.random.default_rng(40 + nq)
+    n = 6000
+    x, a = random_state(n, 14, seed=99)
+    U = random_unitary(2 ** nq, rng)
+    ox, oa = np.zeros(4 * n, np.uint64), np.zeros(4 * n, np.complex128)
+    ox[:n], oa[:n] = x, a
+    gx, ga = ox.copy(), oa.copy()
+    if nq == 1:
+        m = OK.apply_1qubit_rotation_sparse(ox, oa, n, 9, U)
+        g = bs().apply_1qubit_rotation_sparse(gx, ga, n, 9, U)
+    else:
+        m = OK.apply_2qubit_rotation_sparse(ox, oa, n, 3, 11, U)
+        g = bs().apply_2qubit_rotation_sparse(gx, ga, n, 3, 11, U)
+    assert g == m
+    assert_same_entries(gx[:g], ga[:g], ox[:m], oa[:m], exact=exact())
+
+
+def test_diagonal(gpu_ctx):
+    print("[case] diag", flush=True)
+    n = 10000
+    x, a = random_state(n, 16, seed=5)
+    d = np.exp(1j * np.array([0.1, -0.4, 0.7, 2.0]))
+    oa = a.copy()
+    OK.apply_2qubit_diagonal_kernel(x, oa, n, 6, 2, d)
+    ga = a.copy()
+    ss().apply_2qubit_diagonal_kernel(x, ga, n, 6, 2, d)
+    np.testing.assert_allclose(ga, oa, rtol=1e-12, atol=0)
+    if exact():
+        assert np.array_equal(ga.view(np.float64), oa.view(np.float64))
+
+
+@pytest.mark.parametrize("n,k,ties", [(10, 3, False), (5000, 2500, False), (70000, 30000, False),
+                                      (4096, 1000, True), (200000, 65536, False)])
+def test_top_k(gpu_ctx, n, k, ties):
+    print(f"[case] topk n={n} k={k} ties={ties}", flush=True)
+    x, a = random_state(n, 24, seed=n + k, ties=ties)
+    gx, ga = x.copy(), a.copy()
+    m, gsq = tr().top_k_truncate_kernel_opt(gx, ga, n, k)
+    ox, oa = x.copy(), a.copy()
+    m2, osq = OK.top_k_truncate_kernel_opt(ox, oa, n, k, selector=OD.key_tiebreak_selector)
+    assert m == m2 == k
+    assert set(gx[:k].tolist()) == set(ox[:k].tolist()), "key tie-break selection differs"
+    assert_same_entries(gx[:k], ga[:k], ox[:k], oa[:k], exact=True)
+    assert abs(gsq - osq) <= 1e-12 * osq
+    # against the reference's argpartition rule: differences only inside the band
+    rx, ra = x.copy(), a.copy()
+    OK.top_k_truncate_kernel_opt(rx, ra, n, k)
+    assert_topk_sets(gx[:k], rx[:k], x, a, k)
+
+
+def test_top_k_all_equal(gpu_ctx):
+    print("[case] topk all-equal", flush=True)
+    n, k = 4000, 1234
+    rng = np.random.default_rng(1)
+    x = rng.choice(2 ** 30, size=n, replace=False).astype(np.uint64)
+    a = np.full(n, 1 / np.sqrt(n), dtype=np.complex128)
+    gx, ga = x.copy(), a.copy()
+    tr().top_k_truncate_kernel_opt(gx, ga, n, k)
+    assert set(gx[:k].tolist()) == set(np.sort(x)[:k].tolist())
+
+
+def test_truncate_renorm_stats(gpu_ctx):
+    print("[case] truncate renorm", flush=True)
+    ops = pkg_module("_ops")
+    dev = pkg_module("_device")
+    n, k = 50000, 12345
+    x, a = random_state(n, 22, seed=8)
+    src = dev.Buffers(n, gpu_ctx.device)
+    src.x[:n] = torch.from_numpy(x.view(np.int64)).to(gpu_ctx.device)
+    src.a[:n] = torch.from_numpy(a).to(gpu_ctx.device)
+    dst = dev.Buffers(k, gpu_ctx.device)
+    st = ops.truncate(gpu_ctx, src, n, k, dst, renorm=True)
+    sel = OD.key_tiebreak_selector(x, a, None, k)
+    kept = np.sum(np.abs(a[sel]) ** 2)
+    assert abs(st.total - np.sum(np.abs(a) ** 2)) < 1e-13
+    assert abs(st.kept - kept) < 1e-13
+    ga = dev.to_host_amps(dst.a, k)
+    gx = dev.to_host_keys(dst.x, k)
+    assert np.array_equal(gx, x[sel]), "compaction must keep input order"
+    np.testing.assert_allclose(ga, a[sel] / np.sqrt(st.kept), rtol=1e-15, atol=0)
+    assert np.array_equal(ga, a[sel] / np.sqrt(st.kept)), "renormalisation must round like numpy"
+    assert abs(st.sum4 - np.sum(np.abs(ga) ** 4)) < 1e-12 * st.sum4
+
+
+def test_rdm(gpu_ctx):
+    print("[case] rdm", flush=True)
+    n, N = 20000, 18
+    x, a = random_state(n, N, seed=21)
+    ref = OK.compute_all_1qubit_rdms(x, a, n, N)
+    got = bs().compute_all_1qubit_rdms(x, a, n, N)
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("n,bits", [(2, 5), (3000, 12), (100000, 40), (70001, 64)])
+def test_radix_sort_and_merge(gpu_ctx, n, bits):
+    print(f"[case] radix n={n} bits={bits}", flush=True)
+    rng = np.random.default_rng(n)
+    hi = 2 ** bits if bits < 64 else 2 ** 64
+    x = rng.integers(0, hi // 2, size=n, dtype=np.uint64) * np.uint64(2) if bits < 64 else \
+        rng.integers(0, 2 ** 63, size=n, dtype=np.uint64) * np.uint64(2)
+    x[: n // 3] = x[n // 3: 2 * (n // 3)]  # duplicates
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    gx, ga = x.copy(), a.copy()
+    ss().radix_sort_uint64(gx, ga, n)
+    order = np.argsort(x, kind="stable")
+    assert np.array_equal(gx, x[order])
+    assert np.array_equal(ga, a[order]), "payload not stable"
+    ox, oa = gx.copy(), ga.copy()
+    m = OK.merge_sorted_kernel(ox, oa, n)
+    g = ss().merge_sorted_kernel(gx, ga, n)
+    assert g == m
+    assert np.array_equal(gx[:m], ox[:m])
+    np.testing.assert_allclose(ga[:m], oa[:m], rtol=1e-15, atol=0)
+
+
+def test_hash_table(gpu_ctx):
+    print("[case] hash", flush=True)
+    n = 50000
+    x, _ = random_state(n, 40, seed=2)
+    cap = 1 << 17
+    hk = torch.empty(cap, dtype=torch.int64, device=gpu_ctx.device)
+    hv = torch.empty(cap, dtype=torch.int32, device=gpu_ctx.device)
+    B = bs()
+    B.ht_clear(hk)
+    xt = torch.from_numpy(x.view(np.int64)).to(gpu_ctx.device)
+    B.ht_build(xt, n, hk, hv)
+    q = np.concatenate([x[::7], x[::11] ^ np.uint64(1 << 41)])
+    got = B.ht_lookup_batch(hk, hv, torch.from_numpy(q.view(np.int64)).to(gpu_ctx.device)).cpu().numpy()
+    pos = {int(k): i for i, k in enumerate(x.tolist())}
+    exp = np.array([pos.get(int(k), -1) for k in q.tolist()])
+    assert np.array_equal(got, exp)
+    assert B.ht_lookup(hk, hv, int(x[123])) == 123
+    assert B.ht_lookup(hk, hv, int(x[123]) ^ (1 << 45)) == -1
+
+
+def test_random_k(gpu_ctx):
+    print("[case] random-k", flush=True)
+    n, k = 3000, 1000
+    x, a = random_state(n, 20, seed=4)
+    gx, ga = x.copy(), a.copy()
+    m, gsq = tr().random_k_truncate_kernel_opt(gx, ga, n, k, 12345)
+    ox, oa = x.copy(), a.copy()
+    m2, osq = OK.random_k_truncate_kernel_opt(ox, oa, n, k, 12345)
+    assert m == m2 == k
+    assert np.array_equal(gx[:k], ox[:k])
+    assert np.array_equal(ga[:k], oa[:k])
+    assert abs(gsq - osq) < 1e-13
