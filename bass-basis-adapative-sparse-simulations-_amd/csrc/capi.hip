@@ -273,7 +273,7 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
     log2cap = log2_ceil(2 * n);
     if (log2cap < 6) log2cap = 6;
     const size_t cap = (size_t)1 << log2cap;
-    if (cap * (8 + 4 * (size_t)M) > (size_t)48 * L.S)
+    if (cap * (8 + 4 * (size_t)M) > L.total - L.big)
       return fail(BASS_E_WORKSPACE, "bass_gate: group table does not fit");
     gkeys = at<uint64_t>(ws, L.big);
     gmem = at<int32_t>(ws, L.big + cap * 8);
@@ -385,7 +385,7 @@ int bass_rdm1q(const uint64_t* x, const double* a, int64_t n, int N, double* rho
     int log2cap = log2_ceil(2 * n);
     if (log2cap < 6) log2cap = 6;
     const size_t cap = (size_t)1 << log2cap;
-    if (cap * 12 > (size_t)48 * L.S) return fail(BASS_E_WORKSPACE, "bass_rdm1q: table does not fit");
+    if (cap * 12 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_rdm1q: table does not fit");
     uint64_t* tk = at<uint64_t>(ws, L.big);
     int32_t* tv = at<int32_t>(ws, L.big + cap * 8);
     const int tm = ktimer_begin("rdm", s);

@@ -106,13 +106,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 // starts; a predecessor publishes its aggregate before it waits on anyone, so
 // each wait below ends.  A spin bound turns a broken invariant into an error
 // bit instead of a hang.
+//
+// The status words carry only counts -- no tile reads data another tile wrote
+// in the same launch -- so they are relaxed agent-scope atomics.  (A release
+// store at agent scope would write back the whole XCD L2 on every publish.)
 constexpr unsigned long long LB_MASK = (1ull << 62) - 1;
 
 __device__ __forceinline__ unsigned long long lb_load(unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void lb_store(unsigned long long* p, unsigned long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Called by ONE thread of the tile.  Returns the exclusive prefix of `count`
@@ -143,6 +147,53 @@ __device__ inline unsigned long long lookback_prefix(unsigned long long* status,
     --t;
   }
   lb_store(&status[tile], (2ull << 62) | (excl + count));
+  return excl;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+// Wave-parallel look-back: all 64 lanes of ONE wave call this together.  Each
+// probe reads the status words of the 64 nearest unresolved predecessors at
+// once; if every word up to the nearest inclusive prefix is published, their
+// values are summed with one wave reduction, otherwise the wave waits and
+// re-reads.  Returns the exclusive prefix of `count` in every lane.
+__device__ inline unsigned long long lookback_prefix_wave(unsigned long long* status, unsigned int tile,
+                                                          unsigned long long count, int* err) {
+  const int lane = threadIdx.x & 63;
+  if (tile == 0) {
+    if (lane == 0) lb_store(&status[0], (2ull << 62) | count);
+    return 0;
+  }
+  if (lane == 0) lb_store(&status[tile], (1ull << 62) | count);
+  unsigned long long excl = 0;
+  long base = (long)tile - 1;
+  unsigned int spins = 0;
+  while (true) {
+    const long t = base - lane;
+    // before tile 0 reads as an inclusive prefix of 0
+    const unsigned long long s = (t >= 0) ? lb_load(&status[t]) : (2ull << 62);
+    const unsigned int flag = (unsigned int)(s >> 62);
+    const unsigned long long incl = __ballot(flag == 2);
+    const unsigned long long zero = __ballot(flag == 0);
+    const int first = incl ? (__ffsll((long long)incl) - 1) : 64;
+    const unsigned long long upto = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+    if (zero & upto) {
+      if (++spins > (1u << 24)) {
+        if (lane == 0) set_err(err, ERR_LOOKBACK);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += wave_sum_u64(lane <= first ? (s & LB_MASK) : 0ull);
+    if (first < 64) break;
+    base -= 64;
+  }
+  if (lane == 0) lb_store(&status[tile], (2ull << 62) | (excl + count));
   return excl;
 }
 

@@ -210,7 +210,10 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ x,
   unsigned long long tile_total = 0;
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) tile_total += (unsigned long long)rtot[it];
-  if (threadIdx.x == 0) s_base = lookback_prefix(lb_status, tile, tile_total, err);
+  if (threadIdx.x < 64) {
+    const unsigned long long e = lookback_prefix_wave(lb_status, tile, tile_total, err);
+    if (threadIdx.x == 0) s_base = e;
+  }
   __syncthreads();
   const unsigned long long base = s_base;
   const bool fits = base + tile_total <= (unsigned long long)cap_out;

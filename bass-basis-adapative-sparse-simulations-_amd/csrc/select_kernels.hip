@@ -182,7 +182,10 @@ __global__ __launch_bounds__(NT) void k_extract1(const double2* __restrict__ a, 
   unsigned long long ttot = 0;
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) ttot += (unsigned long long)rtot[it];
-  if (threadIdx.x == 0) s_base = lookback_prefix(lb_status, tile, ttot, err);
+  if (threadIdx.x < 64) {
+    const unsigned long long e = lookback_prefix_wave(lb_status, tile, ttot, err);
+    if (threadIdx.x == 0) s_base = e;
+  }
   __syncthreads();
   unsigned long long rb = s_base;
 #pragma unroll
@@ -377,11 +380,17 @@ __global__ __launch_bounds__(NT) void k_sel_finalize(const double* tile_dsum, in
                                                       const SelState* sel, unsigned long long k,
                                                       int renorm, OpStats* st) {
   __shared__ double s_red[WAVES];
+  __shared__ unsigned long long s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
   const double d = block_sum_array(tile_dsum, ntiles, s_red);
   const double c = block_sum_array(part_sum, nparts, s_red);
+  unsigned long long pc = 0;
+  for (int j = threadIdx.x; j < nparts; j += NT) pc += part_cnt[j];
+  atomicAdd(&s_cnt, pc);
+  __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long cnt = sel->def_count;
-    for (int j = 0; j < nparts; ++j) cnt += part_cnt[j];
+    const unsigned long long cnt = sel->def_count + s_cnt;
     if (cnt != k) set_err(&st->err, ERR_SELECT);
     const double kept = d + c;
     st->kept = kept;
@@ -426,7 +435,10 @@ __global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x,
   unsigned long long ttot = 0;
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) ttot += (unsigned long long)rtot[it];
-  if (threadIdx.x == 0) s_base = lookback_prefix(lb_status, tile, ttot, err);
+  if (threadIdx.x < 64) {
+    const unsigned long long e = lookback_prefix_wave(lb_status, tile, ttot, err);
+    if (threadIdx.x == 0) s_base = e;
+  }
   __syncthreads();
   const unsigned long long base = s_base;
   const bool fits = base + ttot <= (unsigned long long)cap_out;

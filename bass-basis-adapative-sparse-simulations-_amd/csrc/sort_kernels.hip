@@ -1,13 +1,16 @@
 // Stable LSD radix sort of uint64 keys with complex128 payload, and the
 // run-length merge that follows it (src/core/sparse_state.py:127-172).
 //
-// 8-bit digits, P = ceil(key_bits / 8) passes.  One kernel reads every key
-// once and builds the global histograms of all P digits (8 B/entry).  Each
-// pass is then a single kernel (Onesweep style): a tile of 2048 entries ranks
-// its digits stably in LDS -- wave-level peer masks from 8 ballots, then a
-// running per-digit count across the tile's rounds -- obtains its per-digit
-// global offsets from a decoupled look-back over earlier tiles (one status
-// word per tile and digit) and scatters key + payload (48 B/entry/pass).
+// 8-bit digits, P = ceil(key_bits / 8) passes, each a reduce-then-scan:
+//   k_sort_count   per tile of 2048 entries, LDS histogram of the digit
+//                  -> counts[digit][tile]                      (reads 8 B/entry)
+//   k_sort_rowscan one block per digit: exclusive scan of its row over tiles,
+//                  row total to totals[digit]
+//   k_sort_scatter per tile: stable in-tile rank of every entry (wave peer
+//                  masks from 8 ballots + running per-digit counts across the
+//                  tile's rounds), position = digit start + row prefix + rank,
+//                  scatter key + payload                  (reads 24 B, writes 24 B)
+// No inter-block waits: every kernel finishes independently.
 #include "common.h"
 #include "kernels.h"
 
@@ -16,62 +19,68 @@ namespace bass {
 constexpr int SORT_ITEMS = 8;
 constexpr int SORT_TILE = NT * SORT_ITEMS;
 constexpr int RADIX = 256;
-constexpr unsigned int ST_MASK = (1u << 30) - 1u;
 
 int sort_tiles(int64_t n) { return (int)((n + SORT_TILE - 1) / SORT_TILE); }
 
-__global__ __launch_bounds__(NT) void k_sort_hist(const uint64_t* __restrict__ keys, int64_t n, int passes,
-                                                   unsigned int* ghist) {
-  __shared__ unsigned int s_h[8][RADIX];
-  for (int j = threadIdx.x; j < 8 * RADIX; j += NT) (&s_h[0][0])[j] = 0;
+__global__ __launch_bounds__(NT) void k_sort_count(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                    unsigned int* __restrict__ counts, int tiles) {
+  __shared__ unsigned int s_h[RADIX];
+  s_h[threadIdx.x] = 0;
   __syncthreads();
-  const int64_t stride = (int64_t)gridDim.x * NT;
-  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) {
-    const uint64_t k = keys[i];
-    for (int p = 0; p < passes; ++p) atomicAdd(&s_h[p][(k >> (8 * p)) & 255u], 1u);
+#pragma unroll
+  for (int it = 0; it < SORT_ITEMS; ++it) {
+    const int64_t i = (int64_t)blockIdx.x * SORT_TILE + (int64_t)it * NT + threadIdx.x;
+    if (i < n) atomicAdd(&s_h[(unsigned int)(keys[i] >> shift) & 255u], 1u);
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < passes * RADIX; j += NT) {
-    const unsigned int v = (&s_h[0][0])[j];
-    if (v) atomicAdd(&ghist[j], v);
+  counts[(size_t)threadIdx.x * tiles + blockIdx.x] = s_h[threadIdx.x];
+}
+
+// One block per digit: in-place exclusive scan of counts[digit][0:tiles].
+__global__ __launch_bounds__(NT) void k_sort_rowscan(unsigned int* __restrict__ counts, int tiles,
+                                                      unsigned int* __restrict__ totals) {
+  __shared__ int s_scan[WAVES];
+  unsigned int* row = counts + (size_t)blockIdx.x * tiles;
+  unsigned int carry = 0;
+  for (int t0 = 0; t0 < tiles; t0 += NT) {
+    const int t = t0 + threadIdx.x;
+    const unsigned int v = t < tiles ? row[t] : 0u;
+    int tot;
+    const unsigned int ex = (unsigned int)block_excl_scan((int)v, tot, s_scan);
+    if (t < tiles) row[t] = carry + ex;
+    carry += (unsigned int)tot;
   }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-__device__ __forceinline__ unsigned int st_load(unsigned int* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_store(unsigned int* p, unsigned int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ __launch_bounds__(NT) void k_sort_pass(const uint64_t* __restrict__ kin,
-                                                   const double2* __restrict__ vin, int64_t n, int shift,
-                                                   const unsigned int* __restrict__ ghist_pass,
-                                                   uint64_t* __restrict__ kout, double2* __restrict__ vout,
-                                                   unsigned int* status, unsigned int* ticket, int* err) {
+__global__ __launch_bounds__(NT) void k_sort_scatter(const uint64_t* __restrict__ kin,
+                                                      const double2* __restrict__ vin, int64_t n, int shift,
+                                                      const unsigned int* __restrict__ counts, int tiles,
+                                                      const unsigned int* __restrict__ totals,
+                                                      uint64_t* __restrict__ kout, double2* __restrict__ vout) {
   __shared__ unsigned int s_cnt[RADIX];
   __shared__ unsigned int s_wc[WAVES][RADIX];
   __shared__ unsigned int s_off[RADIX];
   __shared__ int s_scan[WAVES];
-  __shared__ unsigned int s_tile;
-  const unsigned int tile = take_ticket(ticket, &s_tile);
+  const unsigned int tile = blockIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  s_cnt[threadIdx.x] = 0;
-  for (int ww = 0; ww < WAVES; ++ww) s_wc[ww][threadIdx.x] = 0;
+  {
+    // digit start (scan of the totals) + this tile's row prefix
+    int tot;
+    const unsigned int gstart = (unsigned int)block_excl_scan((int)totals[threadIdx.x], tot, s_scan);
+    s_off[threadIdx.x] = gstart + counts[(size_t)threadIdx.x * tiles + tile];
+    s_cnt[threadIdx.x] = 0;
+    for (int ww = 0; ww < WAVES; ++ww) s_wc[ww][threadIdx.x] = 0;
+  }
   __syncthreads();
-  uint64_t kk[SORT_ITEMS];
-  double2 vv[SORT_ITEMS];
-  unsigned int rank[SORT_ITEMS];
-  unsigned int dig[SORT_ITEMS];
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
 #pragma unroll
   for (int it = 0; it < SORT_ITEMS; ++it) {
     const int64_t i = (int64_t)tile * SORT_TILE + (int64_t)it * NT + threadIdx.x;
     const bool ok = i < n;
-    kk[it] = ok ? kin[i] : 0ull;
-    vv[it] = ok ? vin[i] : make_double2(0.0, 0.0);
-    const unsigned int d = (unsigned int)(kk[it] >> shift) & 255u;
-    dig[it] = d;
+    const uint64_t kk = ok ? kin[i] : 0ull;
+    const double2 vv = ok ? vin[i] : make_double2(0.0, 0.0);
+    const unsigned int d = (unsigned int)(kk >> shift) & 255u;
     // lanes of this wave holding the same digit
     unsigned long long peers = __ballot(ok);
 #pragma unroll
@@ -83,9 +92,10 @@ __global__ __launch_bounds__(NT) void k_sort_pass(const uint64_t* __restrict__ k
     if (ok && in_wave == 0) s_wc[w][d] = (unsigned int)__popcll(peers);
     __syncthreads();
     if (ok) {
-      unsigned int r = s_cnt[d] + in_wave;
+      unsigned int r = s_off[d] + s_cnt[d] + in_wave;
       for (int ww = 0; ww < w; ++ww) r += s_wc[ww][d];
-      rank[it] = r;
+      kout[r] = kk;
+      vout[r] = vv;
     }
     __syncthreads();
     {
@@ -99,50 +109,6 @@ __global__ __launch_bounds__(NT) void k_sort_pass(const uint64_t* __restrict__ k
     }
     __syncthreads();
   }
-  // per-digit look-back over earlier tiles; thread t owns digit t
-  {
-    const int d = threadIdx.x;
-    const unsigned int cnt = s_cnt[d];
-    unsigned int* row = status + (size_t)tile * RADIX;
-    unsigned int excl = 0;
-    if (tile == 0) {
-      st_store(&row[d], (2u << 30) | cnt);
-    } else {
-      st_store(&row[d], (1u << 30) | cnt);
-      long t = (long)tile - 1;
-      unsigned int spins = 0;
-      while (t >= 0) {
-        const unsigned int sv = st_load(&status[(size_t)t * RADIX + d]);
-        const unsigned int flag = sv >> 30;
-        if (flag == 0) {
-          if (++spins > (1u << 26)) {
-            set_err(err, ERR_LOOKBACK);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-          continue;
-        }
-        excl += sv & ST_MASK;
-        if (flag == 2) break;
-        --t;
-      }
-      st_store(&row[d], (2u << 30) | (excl + cnt));
-    }
-    // global start of digit d: exclusive scan of the pass histogram
-    int tot;
-    const unsigned int gstart = (unsigned int)block_excl_scan((int)ghist_pass[d], tot, s_scan);
-    s_off[d] = gstart + excl;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int it = 0; it < SORT_ITEMS; ++it) {
-    const int64_t i = (int64_t)tile * SORT_TILE + (int64_t)it * NT + threadIdx.x;
-    if (i < n) {
-      const unsigned int pos = s_off[dig[it]] + rank[it];
-      kout[pos] = kk[it];
-      vout[pos] = vv[it];
-    }
-  }
 }
 
 __global__ __launch_bounds__(NT) void k_copy_kv(const uint64_t* __restrict__ ks, const double2* __restrict__ vs,
@@ -154,32 +120,22 @@ __global__ __launch_bounds__(NT) void k_copy_kv(const uint64_t* __restrict__ ks,
   }
 }
 
-__global__ void k_zero_u32(unsigned int* p, int64_t m) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) p[i] = 0u;
-}
-
+// counts: RADIX * tiles words; totals: RADIX words.
 int launch_radix_sort(uint64_t* keys, double2* vals, int64_t n, int key_bits, uint64_t* tk, double2* tv,
-                      unsigned int* ghist, unsigned int* status, unsigned int* tickets, int* err,
+                      unsigned int* totals, unsigned int* counts, unsigned int* /*unused*/, int* /*err*/,
                       hipStream_t s) {
   if (n <= 1) return 0;
   const int passes = (key_bits + 7) / 8;
   const int tiles = sort_tiles(n);
-  hipLaunchKernelGGL(k_zero_u32, dim3(8), dim3(NT), 0, s, ghist, (int64_t)(8 * RADIX));
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, tickets, (int64_t)16);
-  int64_t hb = (n + NT - 1) / NT;
-  if (hb > 1024) hb = 1024;
-  hipLaunchKernelGGL(k_sort_hist, dim3((unsigned)hb), dim3(NT), 0, s, keys, n, passes, ghist);
   const uint64_t* ki = keys;
   const double2* vi = vals;
   uint64_t* ko = tk;
   double2* vo = tv;
   for (int p = 0; p < passes; ++p) {
-    int64_t zb = ((int64_t)tiles * RADIX + NT - 1) / NT;
-    if (zb > 4096) zb = 4096;
-    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)zb), dim3(NT), 0, s, status, (int64_t)tiles * RADIX);
-    hipLaunchKernelGGL(k_sort_pass, dim3(tiles), dim3(NT), 0, s, ki, vi, n, 8 * p, ghist + p * RADIX, ko, vo,
-                       status, tickets + p, err);
+    hipLaunchKernelGGL(k_sort_count, dim3(tiles), dim3(NT), 0, s, ki, n, 8 * p, counts, tiles);
+    hipLaunchKernelGGL(k_sort_rowscan, dim3(RADIX), dim3(NT), 0, s, counts, tiles, totals);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(tiles), dim3(NT), 0, s, ki, vi, n, 8 * p, counts, tiles, totals, ko,
+                       vo);
     const uint64_t* t0 = ki;
     const double2* t1 = vi;
     ki = ko;
@@ -220,7 +176,10 @@ __global__ __launch_bounds__(NT) void k_merge(const uint64_t* __restrict__ x, co
   unsigned long long ttot = 0;
 #pragma unroll
   for (int it = 0; it < SORT_ITEMS; ++it) ttot += (unsigned long long)rtot[it];
-  if (threadIdx.x == 0) s_base = lookback_prefix(lb, tile, ttot, err);
+  if (threadIdx.x < 64) {
+    const unsigned long long e = lookback_prefix_wave(lb, tile, ttot, err);
+    if (threadIdx.x == 0) s_base = e;
+  }
   __syncthreads();
   unsigned long long rb = s_base;
 #pragma unroll

@@ -269,7 +269,7 @@ class BASS:
         self._count("bytes_" + op, b)
 
     def _moments(self, state):
-        if state._sum4 is None:
+        if state._sum4 is None or state._sum2 is None:
             state._sum2, state._sum4 = _ops.moments(self._ctx, state._buf, state.nnz)
         return state._sum2, state._sum4
 
@@ -339,13 +339,13 @@ class BASS:
                     continue
                 # tentative: rotate -> truncate to k -> renormalise -> PR
                 rot = state._rot
-                m, _ = _ops.gate(ctx, state._buf, state.nnz, V.conj().T, (j,), rot, mode=L.MODE_ROT,
-                                 dedup=True)
+                m, rht = _ops.gate(ctx, state._buf, state.nnz, V.conj().T, (j,), rot, mode=L.MODE_ROT,
+                                   dedup=True)
                 self._count("rotate")
                 self._count("amp_updates", m)
                 self._bytes("rot1q", state.nnz, m)
                 if m > self.k:
-                    st = _ops.truncate(ctx, rot, m, self.k, cand, renorm=True)
+                    st = _ops.truncate(ctx, rot, m, self.k, cand, renorm=True, hist_tiles=rht)
                     self._bytes("select", m, self.k)
                     nn, frac, s4 = self.k, st.kept, st.sum4
                     dst = cand

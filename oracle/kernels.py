@@ -104,8 +104,13 @@ def merge_sorted_kernel(x, alpha, nnz):
         return 0
     xs = x[:n]
     starts = np.flatnonzero(np.concatenate(([True], xs[1:] != xs[:-1])))
-    # sequential left-to-right sums inside each run (runs are short)
-    sums = np.add.reduceat(alpha[:n], starts)
+    lens = np.diff(np.append(starts, n))
+    # strictly left-to-right sums inside each run, as the reference loop adds
+    # (np.add.reduceat does not keep that order for runs longer than 2)
+    sums = alpha[starts].copy()
+    for j in range(1, int(lens.max())):
+        live = lens > j
+        sums[live] = sums[live] + alpha[starts[live] + j]
     keys = xs[starts].copy()
     cnt = len(starts)
     x[:cnt] = keys

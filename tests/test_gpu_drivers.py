@@ -53,9 +53,19 @@ def test_bass_end_to_end(gpu_ctx, idx, K):
     assert st.nnz == rs.nnz
     assert abs(f_dev - f_ora) <= 1e-10
     assert abs(st.gamma ** 2 - rs.gamma ** 2) <= 1e-10
-    for j in range(N):
-        np.testing.assert_allclose(sim.U[j], ref.U[j], rtol=0, atol=1e-10)
-    assert_same_entries(st.x[: st.nnz], st.alpha[: st.nnz], rs.x[: rs.nnz], rs.alpha[: rs.nnz], rtol=1e-10)
+    # The frames U_j are fixed up to a phase per eigenvector column when a
+    # single-qubit RDM is degenerate to rounding (|rho01|^2 at the 1e-28 branch
+    # of eigh_2x2), so compare frame-independent quantities: the kept keys,
+    # |a|, and the reconstructed state vector.
+    gx, ga = st.x[: st.nnz], st.alpha[: st.nnz]
+    assert set(gx.tolist()) == set(rs.x[: rs.nnz].tolist())
+    assert_same_entries(gx, np.abs(ga), rs.x[: rs.nnz], np.abs(rs.alpha[: rs.nnz]), rtol=1e-10)
+    psi_dev = OE.to_statevector(N, gx, ga, sim.U)
+    psi_ora = OE.to_statevector(N, rs.x[: rs.nnz], rs.alpha[: rs.nnz], ref.U)
+    np.testing.assert_allclose(psi_dev, psi_ora, rtol=0, atol=1e-10)
+    same_frame = all(np.allclose(sim.U[j], ref.U[j], rtol=0, atol=1e-10) for j in range(N))
+    if same_frame:
+        assert_same_entries(gx, ga, rs.x[: rs.nnz], rs.alpha[: rs.nnz], rtol=1e-10)
 
 
 @pytest.mark.parametrize("dedup", ["table", "sort"])
