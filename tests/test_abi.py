@@ -1,0 +1,91 @@
+"""The C-ABI library builds, loads and exports every entry point of
+include/bass_hip.h; host-side argument validation works without a GPU."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT, pkg_module
+
+HEADER = os.path.join(ROOT, "include", "bass_hip.h")
+
+
+def header_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bass_[a-z0-9_]+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    L = pkg_module("_lib")
+    if not os.path.exists(L.LIB_PATH):
+        import __graft_entry__
+
+        __graft_entry__.build()
+    return L.load()
+
+
+def test_header_declares_expected_surface():
+    names = header_functions()
+    for must in ["bass_gate", "bass_truncate", "bass_diag2q", "bass_rdm1q", "bass_radix_sort",
+                 "bass_merge_sorted", "bass_ht_insert", "bass_ht_lookup", "bass_workspace_bytes"]:
+        assert must in names
+
+
+def test_every_header_symbol_is_exported(lib):
+    for name in header_functions():
+        assert hasattr(lib, name), f"{name} not exported by libbass_hip.so"
+
+
+def test_ctypes_table_matches_header():
+    L = pkg_module("_lib")
+    assert sorted(L.SIGNATURES) == header_functions()
+
+
+def test_workspace_sizes(lib):
+    a = lib.bass_workspace_bytes(1)
+    b = lib.bass_workspace_bytes(1 << 20)
+    c = lib.bass_workspace_bytes(1 << 22)
+    assert 0 < a < b < c
+    assert c - b >= 48 * ((1 << 22) - (1 << 20))
+
+
+def test_argument_errors_without_gpu(lib):
+    L = pkg_module("_lib")
+    ht = ctypes.c_int64(0)
+    G = (ctypes.c_double * 8)()
+    rc = lib.bass_gate(None, None, 10, 3, 0, 1, G, 0, 1, None, None, 10, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_ARG and "nq" in L.last_error()
+    rc = lib.bass_gate(None, None, 10, 2, 4, 4, G, 0, 1, None, None, 40, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_ARG and "qubits" in L.last_error()
+    rc = lib.bass_gate(None, None, 10, 1, 0, 0, G, 1, 0, None, None, 40, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_ARG and "rotation" in L.last_error()
+    rc = lib.bass_truncate(None, None, 5, 5, None, None, 5, 1, 0, None, 0, None)
+    assert rc == L.BASS_E_ARG
+    rc = lib.bass_truncate(None, None, 10, 5, None, None, 4, 1, 0, None, 0, None)
+    assert rc == L.BASS_E_OVERFLOW
+    rc = lib.bass_radix_sort(None, None, 10, 0, None, 0, None)
+    assert rc == L.BASS_E_ARG
+    rc = lib.bass_gate(None, None, 10, 1, 0, 0, G, 0, 1, None, None, 40, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_WORKSPACE
+
+
+def test_errors_map_to_reference_exceptions():
+    L = pkg_module("_lib")
+    with pytest.raises(RuntimeError, match="Temporary buffer overflow"):
+        L.check(L.BASS_E_OVERFLOW, "x")
+    with pytest.raises(L.BassError):
+        L.check(L.BASS_E_DEVICE, "x")
+
+
+def test_no_gpu_means_loud_failure():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    dev = pkg_module("_device")
+    with pytest.raises(pkg_module("_lib").BassError):
+        dev.context("cuda:0")

@@ -1,0 +1,114 @@
+"""The CPU oracle against results the reference itself stored
+(tests/golden/reference_results, copied from the reference's tests/results).
+
+What each fixture pins:
+  rotation_diag.csv      TFIM generator + RX/RZZ conventions + dense simulator
+                         (Ordered TFIM, N=12: PRZ and mean single-qubit angle);
+  nscaling_fixed_k.npz   N=10, 12 with k=8192 >= 2^N: no truncation, so every
+                         trial must reproduce the stored participation ratio of
+                         the full state (RFIM generator, sparse 1q/2q/diagonal
+                         kernels, both drivers) and F = 1;
+  fidelity_pr_raw.npz    N=20 RFIM, BASS k=100: trials whose trajectory has no
+                         |a|^2 tie at a truncation threshold are reproduced to
+                         1e-10 (drivers, truncation, RDMs, eigh gauge, the
+                         coordinate-descent optimiser with accepted rotations);
+  nscaling_adaptive_k    N=10, k=512 (truncating, tie-prone): the reference
+                         resolves exact ties of product-state amplitudes by
+                         floating-point noise, so per-trial values differ; the
+                         100-trial means must agree.
+"""
+
+import csv
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REF_RESULTS
+from oracle import circuits as OC
+from oracle import drivers as OD
+from oracle import exact as OE
+
+SEED_BASE = 20260520
+
+
+def _load(name):
+    return np.load(os.path.join(REF_RESULTS, name), allow_pickle=False)
+
+
+def _rfim(N, seed):
+    return OC.rfim(N, 5, W=2.0, dt=0.2, rng=np.random.default_rng(seed))
+
+
+def test_rotation_diag_ordered_tfim():
+    row = next(r for r in csv.DictReader(open(os.path.join(REF_RESULTS, "rotation_diag.csv")))
+               if r["family"] == "Ordered TFIM")
+    meta = json.load(open(os.path.join(REF_RESULTS, "rotation_diag_meta.json")))
+    N = meta["N"]
+    # RotationDiagnostics.ipynb c2: make_tfim(N, rng, layers=5, J=1.0, h=0.5, dt=0.1)
+    circ = OC.tfim(N, 5, J=1.0, h=0.5, dt=0.1, rng=np.random.default_rng(SEED_BASE + 10_000))
+    sv = OE.exact_statevector(N, circ)
+    p = np.abs(sv) ** 2
+    assert abs(1.0 / np.sum(p ** 2) - float(row["prz_median"])) < 1e-10
+    # single-qubit diagnostics (c3): angle of the dominant RDM eigenvector to Z
+    psi = sv.reshape([2] * N)
+    phi = []
+    for q in range(N):
+        m = np.moveaxis(psi, q, 0).reshape(2, -1)
+        ev, evec = np.linalg.eigh(m @ m.conj().T)
+        amp0 = min(float(abs(evec[:, np.argmax(ev)][0])), 1.0)
+        th0 = 2.0 * np.arccos(amp0)
+        phi.append(min(th0, np.pi - th0))
+    assert abs(np.degrees(np.mean(phi)) - float(row["mean_phi_deg"])) < 1e-9
+
+
+@pytest.mark.parametrize("ni,N", [(0, 10), (1, 12)])
+def test_nscaling_fixed_k_no_truncation(ni, N):
+    d = _load("nscaling_fixed_k.npz")
+    for t in range(12):
+        circ = _rfim(N, (SEED_BASE + 1) * 10 ** 6 + ni * 1000 + t)
+        sv = OE.exact_statevector(N, circ)
+        fs = OD.FixedBasisSimulator(N, 8192).simulate(circ, seed=0)
+        b = OD.BASS(N, 8192, optimize_every=5, buffer_factor=1)
+        bs = b.simulate(circ)
+        assert abs(OE.compute_fidelity(fs.x[: fs.nnz], fs.alpha[: fs.nnz], sv) - d["RFIM_(Weq2)_f_fixed"][t, ni]) < 1e-10
+        assert abs(OE.fidelity_bass(N, bs.x[: bs.nnz], bs.alpha[: bs.nnz], b.U, sv) - d["RFIM_(Weq2)_f_bass"][t, ni]) < 1e-10
+        pr_f = OE.compute_participation_ratio(fs.alpha[: fs.nnz])
+        pr_b = OE.compute_participation_ratio(bs.alpha[: bs.nnz])
+        assert abs(pr_f / d["RFIM_(Weq2)_pr_fixed"][t, ni] - 1) < 1e-10
+        assert abs(pr_b / d["RFIM_(Weq2)_pr_bass"][t, ni] - 1) < 1e-10
+
+
+@pytest.mark.parametrize("t", [0, 2, 6])
+def test_fidelity_pr_rfim_bass_k100(t):
+    d = _load("fidelity_pr_raw.npz")
+    N, k = 20, 100
+    circ = _rfim(N, SEED_BASE * 10 ** 6 + t)
+    sv = OE.exact_statevector(N, circ)
+    trace = []
+    b = OD.BASS(N, k, optimize_every=5, buffer_factor=1, trace=trace)
+    bs = b.simulate(circ)
+    f = OE.fidelity_bass(N, bs.x[: bs.nnz], bs.alpha[: bs.nnz], b.U, sv)
+    pr = OE.compute_participation_ratio(bs.alpha[: bs.nnz])
+    assert abs(f / d["rfim_f_bass"][t, 0] - 1) < 1e-10
+    assert abs(pr / d["rfim_pr_bass"][t, 0] - 1) < 1e-10
+    assert any(e[0] == "rot" and e[2] for e in trace), "optimiser accepted rotations in this trial"
+
+
+def test_nscaling_adaptive_means():
+    d = _load("nscaling_adaptive_k.npz")
+    N, k, ni, T = 10, 512, 0, 100
+    ff, fb = [], []
+    for t in range(T):
+        circ = _rfim(N, SEED_BASE * 10 ** 6 + ni * 1000 + t)
+        sv = OE.exact_statevector(N, circ)
+        fs = OD.FixedBasisSimulator(N, k).simulate(circ, seed=0)
+        ff.append(OE.compute_fidelity(fs.x[: fs.nnz], fs.alpha[: fs.nnz], sv))
+        b = OD.BASS(N, k, optimize_every=5, buffer_factor=1)
+        bs = b.simulate(circ)
+        fb.append(OE.fidelity_bass(N, bs.x[: bs.nnz], bs.alpha[: bs.nnz], b.U, sv))
+    assert abs(np.mean(ff) - np.mean(d["RFIM_(Weq2)_f_fixed"][:T, ni])) < 2e-3
+    assert abs(np.mean(fb) - np.mean(d["RFIM_(Weq2)_f_bass"][:T, ni])) < 2e-3
+    # per-trial spread is the tie-resolution noise, not a systematic offset
+    assert np.median(np.abs(np.array(fb) - d["RFIM_(Weq2)_f_bass"][:T, ni])) < 5e-3
