@@ -53,8 +53,10 @@ SIGNATURES = {
     "bass_last_error": (ctypes.c_char_p, []),
     "bass_workspace_bytes": (_SZ, [_I64]),
     "bass_read_stats": (_I, [_P, _SZ, ctypes.POINTER(BassStats), _P]),
-    "bass_gate": (_I, [_P, _P, _I64, _I, _I, _I, _P, _I, _I, _P, _P, _I64, ctypes.POINTER(_I64), _P, _SZ, _P]),
-    "bass_truncate": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _I64, _P, _SZ, _P]),
+    "bass_gate": (_I, [_P, _P, _I64, _I, _I, _I, _P, _I, _I, _I, _P, _P, _I64, ctypes.POINTER(_I64), _P, _SZ,
+                       _P]),
+    "bass_densify": (_I, [_P, _P, _I64, _P, _P, _I64, _P, _SZ, _P]),
+    "bass_truncate": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _I64, _I, _P, _SZ, _P]),
     "bass_diag2q": (_I, [_P, _P, _I64, _I, _I, _P, _P]),
     "bass_moments": (_I, [_P, _I64, _P, _SZ, _P]),
     "bass_scale": (_I, [_P, _I64, _D, _P]),

@@ -10,32 +10,49 @@ import numpy as np
 
 from . import _lib as L
 
+SEG = 2048  # output slots per emission tile (segmented gate output)
 
-def gate(ctx, src, n, G, qubits, dst, mode=L.MODE_GATE, dedup=True):
+
+def _tiles(n, nq):
+    per = 1024 if nq == 1 else 512
+    return (n + per - 1) // per
+
+
+def gate(ctx, src, n, G, qubits, dst, mode=L.MODE_GATE, dedup=True, segmented=False):
     """Apply a 1- or 2-qubit gate (or rotation) to src[:n] into dst.
 
-    Returns (n_out, hist_tiles, total) where hist_tiles feeds truncate()."""
+    Returns (n_out, hist_tiles).  With segmented=True the outputs sit in
+    per-tile segments of dst (feed truncate(..., segmented=True) or densify())."""
     nq = len(qubits)
     M = 1 << nq
-    ctx.ensure_ws(M * max(n, 1))
-    dst.ensure(M * n)
+    need = _tiles(n, nq) * SEG if segmented else M * n
+    ctx.ensure_ws(max(need, n, 1))
+    dst.ensure(max(need, 1))
     buf, gp = L.mat_arg(G)
     q0 = int(qubits[0])
     q1 = int(qubits[1]) if nq == 2 else 0
     ht = ctypes.c_int64(0)
-    rc = ctx.lib.bass_gate(src.xp, src.ap, n, nq, q0, q1, gp, mode, 1 if dedup else 0, dst.xp, dst.ap,
-                           dst.cap, ctypes.byref(ht), ctx.wsp, ctx.ws_bytes, ctx.stream)
+    rc = ctx.lib.bass_gate(src.xp, src.ap, n, nq, q0, q1, gp, mode, 1 if dedup else 0, 1 if segmented else 0,
+                           dst.xp, dst.ap, dst.cap, ctypes.byref(ht), ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_gate")
     st = ctx.read_stats("bass_gate")
     return st.nnz, int(ht.value)
 
 
-def truncate(ctx, src, n, k, dst, renorm=True, hist_tiles=0):
+def densify(ctx, src, seg_tiles, n, dst):
+    """Dense copy of the preceding segmented gate output."""
+    dst.ensure(max(n, 1))
+    rc = ctx.lib.bass_densify(src.xp, src.ap, seg_tiles, dst.xp, dst.ap, dst.cap, ctx.wsp, ctx.ws_bytes,
+                              ctx.stream)
+    L.check(rc, "bass_densify")
+
+
+def truncate(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False):
     """Top-k of src[:n] (n > k) into dst[:k].  Returns stats (kept, total, sum2, sum4, norm)."""
-    ctx.ensure_ws(n)
+    ctx.ensure_ws(max(n, hist_tiles * SEG if segmented else 0))
     dst.ensure(k)
     rc = ctx.lib.bass_truncate(src.xp, src.ap, n, k, dst.xp, dst.ap, dst.cap, 1 if renorm else 0,
-                               hist_tiles, ctx.wsp, ctx.ws_bytes, ctx.stream)
+                               hist_tiles, 1 if segmented else 0, ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_truncate")
     return ctx.read_stats("bass_truncate")
 

@@ -42,8 +42,9 @@ struct Layout {
   // header B (zeroed by every truncate / auxiliary call)
   size_t hist2, sel, tick_sel, hdrB_end;
   // persistent
-  size_t st, tile_psum, tile_dsum, tile_s2, tile_s4, part_sum, part_cnt, rdm_part, rho, sort_hist, sort_tick;
-  size_t lb_gate, lb1, lb2, sort_status;
+  size_t st, tile_psum, tile_dsum, tile_s2, tile_s4, rdm_part, rho, sort_hist, sort_tick;
+  size_t lb_gate, lb1, sort_status;
+  size_t seg_cnt, soff, cnt1, tile_dcnt, ksum, kcnt, koff;
   size_t big;
   size_t total;
 };
@@ -67,22 +68,26 @@ static Layout layout(int64_t S) {
   L.tile_dsum = o; o += al(stiles * 8);
   L.tile_s2 = o; o += al(stiles * 8);
   L.tile_s4 = o; o += al(stiles * 8);
-  L.part_sum = o; o += al(1024 * 8);
-  L.part_cnt = o; o += al(1024 * 8);
   L.rdm_part = o; o += al((size_t)1024 * 64 * 4 * 8);
   L.rho = o; o += al(64 * 4 * 8);
   L.sort_hist = o; o += al(8 * 256 * 4);
   L.sort_tick = o; o += al(16 * 4);
   L.lb_gate = o; o += al(etiles * 8);
   L.lb1 = o; o += al(stiles * 8);
-  L.lb2 = o; o += al(stiles * 8);
   L.sort_status = o; o += al((size_t)stiles * 256 * 4);
+  L.seg_cnt = o; o += al(etiles * 4);
+  L.soff = o; o += al(etiles * 8);
+  L.cnt1 = o; o += al(stiles * 4);
+  L.tile_dcnt = o; o += al(stiles * 4);
+  L.ksum = o; o += al(stiles * 8);
+  L.kcnt = o; o += al(stiles * 4);
+  L.koff = o; o += al(stiles * 8);
   L.big = o;
   // big scratch union:
-  //   gate group table: cap <= 4 * (S/4) slots (output <= 4 x input <= S) -> <= 24 S
-  //   select candidates: 24 S;   rdm / ht table: <= 4 S slots x 12 B = 48 S
+  //   gate group table: cap <= 4 n_in slots x <= 32 B, n_in <= S/4 (2q) or S/2 (1q) -> <= 32 S
+  //   select candidates: 24 B per slot of (S + one tile);   rdm / ht table: <= 4 S slots x 12 B = 48 S
   //   radix sort ping-pong: 24 S
-  o += al((size_t)48 * S + 4096);
+  o += al((size_t)48 * S + ((size_t)1 << 17));
   L.total = o;
   return L;
 }
@@ -242,15 +247,18 @@ int bass_read_stats(void* ws, size_t ws_bytes, bass_stats* out, void* stream) {
 }
 
 int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int q1, const double* G,
-              int mode, int dedup, uint64_t* xo, double* ao, int64_t cap_out, int64_t* hist_tiles,
-              void* ws, size_t ws_bytes, void* stream) {
+              int mode, int dedup, int segmented, uint64_t* xo, double* ao, int64_t cap_out,
+              int64_t* hist_tiles, void* ws, size_t ws_bytes, void* stream) {
   if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_gate: nq must be 1 or 2");
   if (q0 < 0 || q0 > 62 || (nq == 2 && (q1 < 0 || q1 > 62 || q1 == q0)))
     return fail(BASS_E_ARG, "bass_gate: bad qubits %d %d", q0, q1);
   if (n < 0 || n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_gate: bad n %lld", (long long)n);
   if (mode == MODE_ROT && !dedup) return fail(BASS_E_ARG, "bass_gate: rotation requires dedup");
   const int M = 1 << nq;
-  const int64_t need_out = (int64_t)M * n;
+  const int64_t tiles = n > 0 ? (n + emit_tile_entries(nq) - 1) / emit_tile_entries(nq) : 0;
+  const int64_t need_out = segmented ? tiles * SEG_ENTRIES : (int64_t)M * n;
+  if (segmented && cap_out < need_out)
+    return fail(BASS_E_OVERFLOW, "bass_gate: segmented output needs %lld slots", (long long)need_out);
   WS_REQUIRE(need_out > n ? need_out : n);
   hipStream_t s = (hipStream_t)stream;
   if (hist_tiles) *hist_tiles = 0;
@@ -258,59 +266,68 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
   static const char* names[2][2][2] = {{{"emit1q", "emit2q"}, {"gate1q", "gate2q"}},
                                        {{"rot1q?", "rot2q?"}, {"rot1q", "rot2q"}}};
   const int tk = ktimer_begin(names[mode == MODE_ROT][dedup != 0][nq - 1], s);
-  // zero header A and the gate's look-back status; stats nnz/sums
+  // zero header A and (dense output) the gate's look-back status; stats nnz/sums
   fill(ws, L.hdrA_end, 0ull, s);
-  const int64_t tiles = n > 0 ? (n + emit_tile_entries(nq) - 1) / emit_tile_entries(nq) : 0;
-  fill(at<void>(ws, L.lb_gate), (size_t)(tiles + 1) * 8, 0ull, s);
+  if (!segmented) fill(at<void>(ws, L.lb_gate), (size_t)(tiles + 1) * 8, 0ull, s);
   fill(st, 48, 0ull, s);  // nnz, total, kept, norm, sum2, sum4 (err untouched)
   GateMat gm;
   memset(&gm, 0, sizeof(gm));
   for (int j = 0; j < M * M; ++j) gm.m[j] = make_double2(G[2 * j], G[2 * j + 1]);
   int log2cap = 0;
-  uint64_t* gkeys = nullptr;
-  int32_t* gmem = nullptr;
+  uint64_t* gslot = nullptr;
   if (dedup && n > 0) {
     log2cap = log2_ceil(2 * n);
     if (log2cap < 6) log2cap = 6;
     const size_t cap = (size_t)1 << log2cap;
-    if (cap * (8 + 4 * (size_t)M) > L.total - L.big)
-      return fail(BASS_E_WORKSPACE, "bass_gate: group table does not fit");
-    gkeys = at<uint64_t>(ws, L.big);
-    gmem = at<int32_t>(ws, L.big + cap * 8);
-    fill(gkeys, cap * 8, ~0ull, s);
-    fill(gmem, cap * 4 * (size_t)M, ~0ull, s);
-    launch_group_build(nq, x, n, q0, q1, gkeys, gmem, log2cap, &st->err, s);
+    const size_t bytes = cap * (size_t)group_slot_bytes(nq);
+    if (bytes > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_gate: group table does not fit");
+    gslot = at<uint64_t>(ws, L.big);
+    fill(gslot, bytes, ~0ull, s);
+    launch_group_build(nq, x, n, q0, q1, gslot, log2cap, &st->err, s);
   }
+  int* seg = segmented ? at<int>(ws, L.seg_cnt) : nullptr;
   if (n > 0) {
-    launch_emit(nq, mode, dedup != 0, x, (const double2*)a, n, q0, nq == 2 ? q1 : 0, gkeys, gmem, log2cap,
-                gm, xo, (double2*)ao, cap_out, at<unsigned long long>(ws, L.lb_gate),
-                at<unsigned int>(ws, L.tick_gate), at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum),
-                &st->nnz, &st->err, s);
+    launch_emit(nq, mode, dedup != 0, x, (const double2*)a, n, q0, nq == 2 ? q1 : 0, gslot, log2cap, gm, xo,
+                (double2*)ao, cap_out, at<unsigned long long>(ws, L.lb_gate), at<unsigned int>(ws, L.tick_gate),
+                at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), &st->nnz, seg, &st->err, s);
+    if (segmented) launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), st, s);
     if (hist_tiles) *hist_tiles = tiles;
   }
   ktimer_end(tk, s);
   return hip_check("bass_gate");
 }
 
+int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t* xo, double* ao,
+                 int64_t cap_out, void* ws, size_t ws_bytes, void* stream) {
+  if (seg_tiles < 0) return fail(BASS_E_ARG, "bass_densify: bad segment count");
+  WS_REQUIRE(seg_tiles * SEG_ENTRIES);
+  (void)cap_out;
+  launch_densify(x, (const double2*)a, at<int>(ws, L.seg_cnt), at<int64_t>(ws, L.soff), seg_tiles, xo,
+                 (double2*)ao, (hipStream_t)stream);
+  return hip_check("bass_densify");
+}
+
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
-                  int64_t cap_out, int renorm, int64_t hist_tiles, void* ws, size_t ws_bytes,
+                  int64_t cap_out, int renorm, int64_t hist_tiles, int segmented, void* ws, size_t ws_bytes,
                   void* stream) {
   if (k < 1 || n <= k) return fail(BASS_E_ARG, "bass_truncate: need n > k >= 1 (n=%lld k=%lld)",
                                    (long long)n, (long long)k);
   if (cap_out < k) return fail(BASS_E_OVERFLOW, "bass_truncate: cap_out < k");
   if (n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_truncate: n too large");
-  WS_REQUIRE(n);
+  if (segmented && hist_tiles <= 0) return fail(BASS_E_ARG, "bass_truncate: segmented input needs hist_tiles");
+  const int64_t tiles = segmented ? hist_tiles : sel_tiles(n);
+  const int64_t slots = tiles * SEG_ENTRIES;
+  WS_REQUIRE(slots > n ? slots : n);
   hipStream_t s = (hipStream_t)stream;
   OpStats* st = at<OpStats>(ws, L.st);
   fill(at<void>(ws, L.hist2), L.hdrB_end - L.hist2, 0ull, s);
-  const int64_t tiles = sel_tiles(n);
-  fill(at<void>(ws, L.lb1), (size_t)(tiles + 1) * 8, 0ull, s);
-  fill(at<void>(ws, L.lb2), (size_t)(tiles + 1) * 8, 0ull, s);
   if (hist_tiles <= 0) fill(at<void>(ws, L.hist1), HIST_BINS * 4, 0ull, s);
   SelLaunch S_;
   S_.x = x;
   S_.a = (const double2*)a;
   S_.n = n;
+  S_.ntiles = tiles;
+  S_.seg_cnt = segmented ? at<int>(ws, L.seg_cnt) : nullptr;
   S_.k = (unsigned long long)k;
   S_.xo = xo;
   S_.ao = (double2*)ao;
@@ -324,19 +341,20 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   S_.sel = at<SelState>(ws, L.sel);
   S_.st = st;
   char* big = at<char>(ws, L.big);
+  const size_t p8 = al((size_t)8 * slots), p4 = al((size_t)4 * slots);
+  if (2 * p8 + 2 * p4 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_truncate: candidates do not fit");
   S_.c1p = (unsigned long long*)big;
-  S_.c1i = (unsigned int*)(big + al((size_t)8 * n));
-  S_.c2p = (unsigned long long*)(big + al((size_t)8 * n) + al((size_t)4 * n));
-  S_.c2i = (unsigned int*)(big + al((size_t)8 * n) * 2 + al((size_t)4 * n));
+  S_.c1i = (unsigned int*)(big + p8);
+  S_.c2p = (unsigned long long*)(big + p8 + p4);
+  S_.c2i = (unsigned int*)(big + 2 * p8 + p4);
+  S_.cnt1 = at<int>(ws, L.cnt1);
   S_.tile_dsum = at<double>(ws, L.tile_dsum);
-  S_.part_sum = at<double>(ws, L.part_sum);
-  S_.part_cnt = at<unsigned long long>(ws, L.part_cnt);
+  S_.tile_dcnt = at<int>(ws, L.tile_dcnt);
+  S_.ksum = at<double>(ws, L.ksum);
+  S_.kcnt = at<int>(ws, L.kcnt);
+  S_.koff = at<int64_t>(ws, L.koff);
   S_.tile_s2 = at<double>(ws, L.tile_s2);
   S_.tile_s4 = at<double>(ws, L.tile_s4);
-  S_.lb1 = at<unsigned long long>(ws, L.lb1);
-  S_.ticket1 = at<unsigned int>(ws, L.tick_sel);
-  S_.lb2 = at<unsigned long long>(ws, L.lb2);
-  S_.ticket2 = at<unsigned int>(ws, L.tick_sel) + 1;
   const int tk = ktimer_begin("select", s);
   launch_select(S_, s);
   ktimer_end(tk, s);

@@ -47,12 +47,19 @@ __device__ __forceinline__ uint64_t out_bits(int o, int q0, int q1) {
   }
 }
 
+// Group table slot: the rest key and the member indices share one slot
+// (16 B for 1 qubit, 32 B for 2) so a probe and its member reads touch one line.
+template <int QB>
+struct GSlot {
+  static constexpr int WORDS = QB == 1 ? 2 : 4;  // u64 words per slot
+};
+
 template <int QB>
 __global__ __launch_bounds__(NT) void k_group_build(const uint64_t* __restrict__ x, int64_t n, int q0,
-                                                     int q1, uint64_t* __restrict__ gkeys,
-                                                     int32_t* __restrict__ gmem, int shift,
+                                                     int q1, uint64_t* __restrict__ gslot, int shift,
                                                      uint64_t capmask, int* err) {
   constexpr int M = 1 << QB;
+  constexpr int W = GSlot<QB>::WORDS;
   const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
   if (i >= n) return;
   const uint64_t key = x[i];
@@ -60,14 +67,15 @@ __global__ __launch_bounds__(NT) void k_group_build(const uint64_t* __restrict__
   const int c = member_index<QB>(key, q0, q1);
   uint64_t h = hslot(r, shift);
   for (uint64_t probe = 0; probe <= capmask; ++probe) {
-    uint64_t cur = __hip_atomic_load(&gkeys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t* slot = gslot + h * W;
+    uint64_t cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (cur == EMPTY) {
-      unsigned long long prev = atomicCAS((unsigned long long*)&gkeys[h],
-                                          (unsigned long long)EMPTY, (unsigned long long)r);
+      unsigned long long prev = atomicCAS((unsigned long long*)slot, (unsigned long long)EMPTY,
+                                          (unsigned long long)r);
       cur = (prev == (unsigned long long)EMPTY) ? r : (uint64_t)prev;
     }
     if (cur == r) {
-      gmem[h * M + c] = (int32_t)i;
+      reinterpret_cast<int32_t*>(slot + 1)[c] = (int32_t)i;
       return;
     }
     h = (h + 1) & capmask;
@@ -75,28 +83,30 @@ __global__ __launch_bounds__(NT) void k_group_build(const uint64_t* __restrict__
   set_err(err, ERR_PROBE);
 }
 
-__device__ __forceinline__ int64_t table_find(const uint64_t* __restrict__ tkeys, uint64_t key,
-                                              int shift, uint64_t capmask) {
-  uint64_t h = hslot(key, shift);
+template <int QB>
+__device__ __forceinline__ const uint64_t* group_find(const uint64_t* __restrict__ gslot, uint64_t r,
+                                                      int shift, uint64_t capmask) {
+  constexpr int W = GSlot<QB>::WORDS;
+  uint64_t h = hslot(r, shift);
   for (uint64_t probe = 0; probe <= capmask; ++probe) {
-    const uint64_t cur = tkeys[h];
-    if (cur == key) return (int64_t)h;
-    if (cur == EMPTY) return -1;
+    const uint64_t* slot = gslot + h * W;
+    const uint64_t cur = slot[0];
+    if (cur == r) return slot;
+    if (cur == EMPTY) return nullptr;
     h = (h + 1) & capmask;
   }
-  return -1;
+  return nullptr;
 }
 
 template <int QB, int MODE, bool DEDUP, int ITEMS>
 __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ x,
                                               const double2* __restrict__ a, int64_t n, int q0, int q1,
-                                              const uint64_t* __restrict__ gkeys,
-                                              const int32_t* __restrict__ gmem, int shift,
+                                              const uint64_t* __restrict__ gslot, int shift,
                                               uint64_t capmask, GateMat G, uint64_t* __restrict__ xo,
                                               double2* __restrict__ ao, int64_t cap_out,
                                               unsigned long long* lb_status, unsigned int* ticket,
                                               unsigned int* hist, double* tile_psum, int64_t* n_out,
-                                              int* err) {
+                                              int* seg_cnt, int* err) {
   constexpr int M = 1 << QB;
   constexpr int TILE = NT * ITEMS;
   __shared__ unsigned int s_hist[HIST_BINS];
@@ -105,7 +115,11 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ x,
   __shared__ unsigned int s_tile;
   __shared__ unsigned long long s_base;
 
-  const unsigned int tile = take_ticket(ticket, &s_tile);
+  // Segmented output (seg_cnt != null): tile t writes its outputs to the fixed
+  // segment [t * M * TILE, ...) and its count to seg_cnt[t]; no inter-tile
+  // coordination.  Dense output: tiles in dispatch order + decoupled look-back.
+  const bool segmented = seg_cnt != nullptr;
+  const unsigned int tile = segmented ? blockIdx.x : take_ticket(ticket, &s_tile);
   if (hist) {
     for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
   }
@@ -130,13 +144,14 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ x,
 #pragma unroll
       for (int cc = 0; cc < M; ++cc) mem[cc] = -1;
       if constexpr (DEDUP) {
-        const int64_t h = table_find(gkeys, r, shift, capmask);
-        if (h < 0) {
+        const uint64_t* slot = group_find<QB>(gslot, r, shift, capmask);
+        if (!slot) {
           set_err(err, ERR_MISSING);
           mem[c] = (int32_t)i;
         } else {
+          const int32_t* sm = reinterpret_cast<const int32_t*>(slot + 1);
 #pragma unroll
-          for (int cc = 0; cc < M; ++cc) mem[cc] = gmem[h * M + cc];
+          for (int cc = 0; cc < M; ++cc) mem[cc] = sm[cc];
         }
       } else {
         mem[c] = (int32_t)i;
@@ -202,15 +217,24 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ x,
       }
     }
     omask[it] = mask;
+  }
+  // scans after every item's loads and probes are issued
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
     int tot;
-    ooff[it] = block_excl_scan(__popc(mask), tot, s_scan);
+    ooff[it] = block_excl_scan(__popc(omask[it]), tot, s_scan);
     rtot[it] = tot;
   }
 
   unsigned long long tile_total = 0;
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) tile_total += (unsigned long long)rtot[it];
-  if (threadIdx.x < 64) {
+  if (segmented) {
+    if (threadIdx.x == 0) {
+      s_base = (unsigned long long)tile * (unsigned long long)(M * TILE);
+      seg_cnt[tile] = (int)tile_total;
+    }
+  } else if (threadIdx.x < 64) {
     const unsigned long long e = lookback_prefix_wave(lb_status, tile, tile_total, err);
     if (threadIdx.x == 0) s_base = e;
   }
@@ -240,7 +264,7 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ x,
     }
     rb += (unsigned long long)rtot[it];
   }
-  if (tile == gridDim.x - 1 && threadIdx.x == 0) *n_out = (int64_t)(base + tile_total);
+  if (!segmented && tile == gridDim.x - 1 && threadIdx.x == 0) *n_out = (int64_t)(base + tile_total);
   if (tile_psum) {
     const double s = block_sum(psum, s_red);
     if (threadIdx.x == 0) tile_psum[tile] = s;
@@ -268,26 +292,28 @@ __global__ __launch_bounds__(NT) void k_diag2q(const uint64_t* __restrict__ x, d
 
 // ----------------------------------------------------------- host launch ---
 
-int launch_group_build(int qb, const uint64_t* x, int64_t n, int q0, int q1, uint64_t* gkeys,
-                       int32_t* gmem, int log2cap, int* err, hipStream_t s) {
+int group_slot_bytes(int qb) { return qb == 1 ? 8 * GSlot<1>::WORDS : 8 * GSlot<2>::WORDS; }
+
+int launch_group_build(int qb, const uint64_t* x, int64_t n, int q0, int q1, uint64_t* gslot, int log2cap,
+                       int* err, hipStream_t s) {
   if (n <= 0) return 0;
   const int shift = 64 - log2cap;
   const uint64_t capmask = (1ull << log2cap) - 1;
   const dim3 grid((unsigned)((n + NT - 1) / NT));
   if (qb == 1)
-    hipLaunchKernelGGL(k_group_build<1>, grid, dim3(NT), 0, s, x, n, q0, q1, gkeys, gmem, shift, capmask, err);
+    hipLaunchKernelGGL(k_group_build<1>, grid, dim3(NT), 0, s, x, n, q0, q1, gslot, shift, capmask, err);
   else
-    hipLaunchKernelGGL(k_group_build<2>, grid, dim3(NT), 0, s, x, n, q0, q1, gkeys, gmem, shift, capmask, err);
+    hipLaunchKernelGGL(k_group_build<2>, grid, dim3(NT), 0, s, x, n, q0, q1, gslot, shift, capmask, err);
   return (int)hipGetLastError();
 }
 
 int emit_tile_entries(int qb) { return NT * (qb == 1 ? 4 : 2); }
 
 int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* a, int64_t n, int q0,
-                int q1, const uint64_t* gkeys, const int32_t* gmem, int log2cap, const GateMat& G,
+                int q1, const uint64_t* gslot, int log2cap, const GateMat& G,
                 uint64_t* xo, double2* ao, int64_t cap_out, unsigned long long* lb_status,
-                unsigned int* ticket, unsigned int* hist, double* tile_psum, int64_t* n_out, int* err,
-                hipStream_t s) {
+                unsigned int* ticket, unsigned int* hist, double* tile_psum, int64_t* n_out, int* seg_cnt,
+                int* err, hipStream_t s) {
   if (n <= 0) return 0;
   const int shift = 64 - log2cap;
   const uint64_t capmask = log2cap > 0 ? ((1ull << log2cap) - 1) : 0;
@@ -295,8 +321,8 @@ int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* 
   const dim3 grid((unsigned)((n + tile - 1) / tile));
 #define BASS_EMIT(QB_, MODE_, DEDUP_, ITEMS_)                                                        \
   hipLaunchKernelGGL((k_emit<QB_, MODE_, DEDUP_, ITEMS_>), grid, dim3(NT), 0, s, x, a, n, q0, q1,    \
-                     gkeys, gmem, shift, capmask, G, xo, ao, cap_out, lb_status, ticket, hist,       \
-                     tile_psum, n_out, err)
+                     gslot, shift, capmask, G, xo, ao, cap_out, lb_status, ticket, hist,             \
+                     tile_psum, n_out, seg_cnt, err)
   if (qb == 1) {
     if (!dedup)
       BASS_EMIT(1, MODE_GATE, false, 4);

@@ -37,21 +37,26 @@ struct OpStats {
   int32_t pad;
 };
 
-int launch_group_build(int qb, const uint64_t* x, int64_t n, int q0, int q1, uint64_t* gkeys,
-                       int32_t* gmem, int log2cap, int* err, hipStream_t s);
+int group_slot_bytes(int qb);
+int launch_group_build(int qb, const uint64_t* x, int64_t n, int q0, int q1, uint64_t* gslot, int log2cap,
+                       int* err, hipStream_t s);
 int emit_tile_entries(int qb);
 int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* a, int64_t n, int q0,
-                int q1, const uint64_t* gkeys, const int32_t* gmem, int log2cap, const GateMat& G,
+                int q1, const uint64_t* gslot, int log2cap, const GateMat& G,
                 uint64_t* xo, double2* ao, int64_t cap_out, unsigned long long* lb_status,
-                unsigned int* ticket, unsigned int* hist, double* tile_psum, int64_t* n_out, int* err,
-                hipStream_t s);
+                unsigned int* ticket, unsigned int* hist, double* tile_psum, int64_t* n_out, int* seg_cnt,
+                int* err, hipStream_t s);
+// Every emission tile covers SEG_ENTRIES output slots (1q: 1024 inputs x 2, 2q: 512 x 4).
+constexpr int SEG_ENTRIES = 2048;
 int launch_diag2q(const uint64_t* x, double2* a, int64_t n, int q1, int q2, const double2* d,
                   hipStream_t s);
 
 struct SelLaunch {
   const uint64_t* x;
   const double2* a;
-  int64_t n;
+  int64_t n;             // dense entry count (used when seg_cnt is null)
+  int64_t ntiles;        // tiles of SEG_ENTRIES slots
+  const int* seg_cnt;    // per-tile entry counts of a segmented input, or null
   unsigned long long k;
   uint64_t* xo;
   double2* ao;
@@ -64,24 +69,25 @@ struct SelLaunch {
   unsigned int* hist2;   // zeroed
   SelState* sel;         // zeroed
   OpStats* st;
-  unsigned long long* c1p;
+  unsigned long long* c1p;  // ntiles * SEG_ENTRIES
   unsigned int* c1i;
+  int* cnt1;
   unsigned long long* c2p;
   unsigned int* c2i;
   double* tile_dsum;
-  double* part_sum;
-  unsigned long long* part_cnt;
+  int* tile_dcnt;
+  double* ksum;
+  int* kcnt;
+  int64_t* koff;
   double* tile_s2;
   double* tile_s4;
-  unsigned long long* lb1;  // zeroed, sel_tiles(n) + 1 words
-  unsigned int* ticket1;    // zeroed
-  unsigned long long* lb2;  // zeroed
-  unsigned int* ticket2;    // zeroed
 };
 
 int64_t sel_tiles(int64_t n);
-int sel_kept_blocks();
 int launch_select(const SelLaunch& L, hipStream_t s);
+int launch_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff, OpStats* st, hipStream_t s);
+int launch_densify(const uint64_t* x, const double2* a, const int* seg_cnt, const int64_t* soff, int64_t ntiles,
+                   uint64_t* xo, double2* ao, hipStream_t s);
 int launch_moments(const double2* a, int64_t n, double* tile_s2, double* tile_s4, OpStats* st,
                    hipStream_t s);
 int launch_scale(double2* a, int64_t n, double div, hipStream_t s);

@@ -5,26 +5,38 @@
 // defined; src/core/truncation.py:68-113, src/simulation/bass_simulator.py:899-916).
 // |a|^2 >= 0, so its bit pattern orders like the value.
 //
-// Passes over the full input (n entries, 24 B each):
-//   digit 1 (sign + exponent, 11 bits) histogram: fused into the producing gate
-//     kernel, or k_hist_psum (16 B/entry);
-//   k_extract1: read |a|^2 (16 B/entry); entries above the digit-1 bucket are
-//     kept outright, entries inside it are appended (in order, decoupled
-//     look-back) to a candidate list together with the digit-2 histogram;
+// Input is a sequence of tiles of at most SEG_ENTRIES entries: tile t holds
+// cnt_t entries at [t * SEG_ENTRIES, t * SEG_ENTRIES + cnt_t).  A dense array
+// is the special case cnt_t = min(SEG_ENTRIES, n - t * SEG_ENTRIES); the
+// segmented output of a gate supplies cnt_t per tile.  No kernel below
+// coordinates with other workgroups while it runs.
+//
+// Passes over the full input (24 B per entry):
+//   digit 1 (sign + exponent) histogram: fused into the producing gate, or
+//     k_hist_psum (16 B/entry);
+//   k_extract1: read |a|^2 (16 B/entry); per tile, entries above the digit-1
+//     bucket are counted and summed, entries inside it are kept as candidates
+//     (in order, per tile) together with the digit-2 histogram;
 //   k_compact: read key + amplitude (24 B/entry), write the k kept entries in
-//     input order, divided by sqrt(kept) when renormalising (24 B/kept entry).
-// Everything else touches only candidates.
+//     input order at offsets scanned from the per-tile kept counts, divided by
+//     sqrt(kept) when renormalising (24 B per kept entry).
+// Everything else touches only candidates or per-tile scalars.
 #include "common.h"
 #include "kernels.h"
 
 namespace bass {
 
-constexpr int SEL_ITEMS = 8;
-constexpr int SEL_TILE = NT * SEL_ITEMS;
-constexpr int KEPT_BLOCKS = 256;
+constexpr int SEL_ITEMS = SEG_ENTRIES / NT;  // 8
+constexpr int SEL_TILE = SEG_ENTRIES;
 
 __device__ __forceinline__ unsigned long long pbits_of(double2 v) {
   return (unsigned long long)__double_as_longlong(cabs2(v));
+}
+
+__device__ __forceinline__ int tile_count(const int* seg_cnt, int64_t n_dense, int64_t t) {
+  if (seg_cnt) return seg_cnt[t];
+  const int64_t r = n_dense - t * SEL_TILE;
+  return r < SEL_TILE ? (int)r : SEL_TILE;
 }
 
 // Find, in a 2048-bin histogram, the bucket holding the need-th largest element.
@@ -89,17 +101,20 @@ __device__ double block_sum_array(const double* v, int64_t m, double* lds) {
   return block_sum(s, lds);
 }
 
-__global__ __launch_bounds__(NT) void k_hist_psum(const double2* __restrict__ a, int64_t n,
-                                                   unsigned int* hist, double* tile_psum) {
+__global__ __launch_bounds__(NT) void k_hist_psum(const double2* __restrict__ a, const int* seg_cnt,
+                                                   int64_t n_dense, unsigned int* hist, double* tile_psum) {
   __shared__ unsigned int s_hist[HIST_BINS];
   __shared__ double s_red[WAVES];
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
   __syncthreads();
+  const int cnt = tile_count(seg_cnt, n_dense, blockIdx.x);
+  const int64_t base = (int64_t)blockIdx.x * SEL_TILE;
   double ps = 0.0;
+#pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
-    const int64_t i = (int64_t)blockIdx.x * SEL_TILE + (int64_t)it * NT + threadIdx.x;
-    if (i < n) {
-      const double p = cabs2(a[i]);
+    const int j = it * NT + threadIdx.x;
+    if (j < cnt) {
+      const double p = cabs2(a[base + j]);
       ps += p;
       atomicAdd(&s_hist[(unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1)], 1u);
     }
@@ -135,35 +150,33 @@ __global__ __launch_bounds__(NT) void k_find1(const unsigned int* hist, unsigned
   }
 }
 
-__global__ __launch_bounds__(NT) void k_extract1(const double2* __restrict__ a, int64_t n,
-                                                  SelState* sel, unsigned long long* __restrict__ c1p,
-                                                  unsigned int* __restrict__ c1i, unsigned int* hist2,
-                                                  double* tile_dsum, unsigned long long* lb_status,
-                                                  unsigned int* ticket, int* err) {
+__global__ __launch_bounds__(NT) void k_extract1(const double2* __restrict__ a, const int* seg_cnt,
+                                                  int64_t n_dense, const SelState* sel,
+                                                  unsigned long long* __restrict__ c1p,
+                                                  unsigned int* __restrict__ c1i, int* __restrict__ cnt1,
+                                                  unsigned int* hist2, double* tile_dsum, int* tile_dcnt) {
   __shared__ unsigned int s_hist[HIST_BINS];
   __shared__ int s_scan[WAVES];
   __shared__ double s_red[WAVES];
-  __shared__ unsigned int s_tile;
-  __shared__ unsigned long long s_base;
   __shared__ int s_dcnt;
-  const unsigned int tile = take_ticket(ticket, &s_tile);
+  const unsigned int tile = blockIdx.x;
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
   if (threadIdx.x == 0) s_dcnt = 0;
   __syncthreads();
   const unsigned int b1 = sel->b1;
+  const int cnt = tile_count(seg_cnt, n_dense, tile);
+  const int64_t base = (int64_t)tile * SEL_TILE;
   unsigned long long cb[SEL_ITEMS];
-  int off[SEL_ITEMS];
-  int rtot[SEL_ITEMS];
   bool isc[SEL_ITEMS];
   double dsum = 0.0;
   int dcnt = 0;
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
-    const int64_t i = (int64_t)tile * SEL_TILE + (int64_t)it * NT + threadIdx.x;
+    const int j = it * NT + threadIdx.x;
     isc[it] = false;
     cb[it] = 0;
-    if (i < n) {
-      const double p = cabs2(a[i]);
+    if (j < cnt) {
+      const double p = cabs2(a[base + j]);
       const unsigned long long bits = (unsigned long long)__double_as_longlong(p);
       const unsigned int d = (unsigned int)(bits >> 52);
       if (d > b1) {
@@ -175,35 +188,25 @@ __global__ __launch_bounds__(NT) void k_extract1(const double2* __restrict__ a, 
         atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
       }
     }
-    int tot;
-    off[it] = block_excl_scan(isc[it] ? 1 : 0, tot, s_scan);
-    rtot[it] = tot;
   }
-  unsigned long long ttot = 0;
-#pragma unroll
-  for (int it = 0; it < SEL_ITEMS; ++it) ttot += (unsigned long long)rtot[it];
-  if (threadIdx.x < 64) {
-    const unsigned long long e = lookback_prefix_wave(lb_status, tile, ttot, err);
-    if (threadIdx.x == 0) s_base = e;
-  }
-  __syncthreads();
-  unsigned long long rb = s_base;
+  int rb = 0;
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
+    int tot;
+    const int off = block_excl_scan(isc[it] ? 1 : 0, tot, s_scan);
     if (isc[it]) {
-      const unsigned long long pos = rb + (unsigned long long)off[it];
-      c1p[pos] = cb[it];
-      c1i[pos] = (unsigned int)((int64_t)tile * SEL_TILE + (int64_t)it * NT + threadIdx.x);
+      c1p[base + rb + off] = cb[it];
+      c1i[base + rb + off] = (unsigned int)(base + it * NT + threadIdx.x);
     }
-    rb += (unsigned long long)rtot[it];
+    rb += tot;
   }
   const double s = block_sum(dsum, s_red);
-  if (threadIdx.x == 0) tile_dsum[tile] = s;
   atomicAdd(&s_dcnt, dcnt);
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(&sel->def_count, (unsigned long long)s_dcnt);
-    if (tile == gridDim.x - 1) sel->cand1 = s_base + ttot;
+    tile_dsum[tile] = s;
+    tile_dcnt[tile] = s_dcnt;
+    cnt1[tile] = rb;
   }
   for (int b = threadIdx.x; b < HIST_BINS; b += NT)
     if (s_hist[b]) atomicAdd(&hist2[b], s_hist[b]);
@@ -228,32 +231,37 @@ __global__ __launch_bounds__(NT) void k_find2(const unsigned int* hist2, SelStat
   }
 }
 
+// Candidates of the digit-2 bucket, appended in any order (the threshold does
+// not depend on their order).
 __global__ __launch_bounds__(NT) void k_extract2(const unsigned long long* __restrict__ c1p,
-                                                  const unsigned int* __restrict__ c1i, SelState* sel,
+                                                  const unsigned int* __restrict__ c1i,
+                                                  const int* __restrict__ cnt1, int64_t ntiles, SelState* sel,
                                                   unsigned long long* __restrict__ c2p,
                                                   unsigned int* __restrict__ c2i) {
-  const unsigned long long m1 = sel->cand1;
   const unsigned int b2 = sel->b2;
   const int lane = threadIdx.x & 63;
-  const unsigned long long stride = (unsigned long long)gridDim.x * NT;
-  for (unsigned long long j0 = (unsigned long long)blockIdx.x * NT; j0 < m1; j0 += stride) {
-    const unsigned long long j = j0 + threadIdx.x;
-    bool take = false;
-    unsigned long long bits = 0;
-    if (j < m1) {
-      bits = c1p[j];
-      take = ((unsigned int)(bits >> 41) & (HIST_BINS - 1)) == b2;
-    }
-    const unsigned long long bal = __ballot(take);
-    if (bal) {
-      unsigned int base = 0;
-      const int leader = __ffsll((long long)bal) - 1;
-      if (lane == leader) base = atomicAdd(&sel->cand2, (unsigned int)__popcll(bal));
-      base = __shfl(base, leader, 64);
-      if (take) {
-        const unsigned int r = (unsigned int)__popcll(bal & ((1ull << lane) - 1ull));
-        c2p[base + r] = bits;
-        c2i[base + r] = c1i[j];
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int cnt = cnt1[t];
+    const int64_t base = t * SEL_TILE;
+    for (int j0 = 0; j0 < cnt; j0 += NT) {
+      const int j = j0 + threadIdx.x;
+      bool take = false;
+      unsigned long long bits = 0;
+      if (j < cnt) {
+        bits = c1p[base + j];
+        take = ((unsigned int)(bits >> 41) & (HIST_BINS - 1)) == b2;
+      }
+      const unsigned long long bal = __ballot(take);
+      if (bal) {
+        unsigned int wb = 0;
+        const int leader = __ffsll((long long)bal) - 1;
+        if (lane == leader) wb = atomicAdd(&sel->cand2, (unsigned int)__popcll(bal));
+        wb = __shfl(wb, leader, 64);
+        if (take) {
+          const unsigned int r = (unsigned int)__popcll(bal & ((1ull << lane) - 1ull));
+          c2p[wb + r] = bits;
+          c2i[wb + r] = c1i[base + j];
+        }
       }
     }
   }
@@ -341,25 +349,25 @@ __device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t key,
   return bits > pstar || (bits == pstar && key <= keystar);
 }
 
+// Per tile: sum and count of the kept candidates (in tile order).
 __global__ __launch_bounds__(NT) void k_keptsum(const unsigned long long* __restrict__ c1p,
                                                  const unsigned int* __restrict__ c1i,
-                                                 const uint64_t* __restrict__ x, const SelState* sel,
-                                                 double* part_sum, unsigned long long* part_cnt) {
+                                                 const int* __restrict__ cnt1, const uint64_t* __restrict__ x,
+                                                 const SelState* sel, double* ksum, int* kcnt) {
   __shared__ double s_red[WAVES];
   __shared__ int s_cnt;
-  const unsigned long long m1 = sel->cand1;
   const unsigned long long pstar = sel->pstar, keystar = sel->keystar;
-  const unsigned long long chunk = (m1 + gridDim.x - 1) / gridDim.x;
-  const unsigned long long lo = (unsigned long long)blockIdx.x * chunk;
-  const unsigned long long hi = lo + chunk < m1 ? lo + chunk : m1;
+  const int64_t t = blockIdx.x;
+  const int cnt = cnt1[t];
+  const int64_t base = t * SEL_TILE;
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
   double s = 0.0;
   int c = 0;
-  for (unsigned long long j = lo + threadIdx.x; j < hi; j += NT) {
-    const unsigned long long bits = c1p[j];
+  for (int j = threadIdx.x; j < cnt; j += NT) {
+    const unsigned long long bits = c1p[base + j];
     bool keep = bits > pstar;
-    if (bits == pstar) keep = x[c1i[j]] <= keystar;
+    if (bits == pstar) keep = x[c1i[base + j]] <= keystar;
     if (keep) {
       s += __longlong_as_double((long long)bits);
       ++c;
@@ -369,103 +377,93 @@ __global__ __launch_bounds__(NT) void k_keptsum(const unsigned long long* __rest
   atomicAdd(&s_cnt, c);
   __syncthreads();
   if (threadIdx.x == 0) {
-    part_sum[blockIdx.x] = tot;
-    part_cnt[blockIdx.x] = (unsigned long long)s_cnt;
+    ksum[t] = tot;
+    kcnt[t] = s_cnt;
   }
 }
 
-__global__ __launch_bounds__(NT) void k_sel_finalize(const double* tile_dsum, int64_t ntiles,
-                                                      const double* part_sum,
-                                                      const unsigned long long* part_cnt, int nparts,
-                                                      const SelState* sel, unsigned long long k,
-                                                      int renorm, OpStats* st) {
+// kept = sum over tiles in order; kept-entry offsets per tile (exclusive scan).
+__global__ __launch_bounds__(NT) void k_sel_finalize(const double* tile_dsum, const double* ksum,
+                                                      const int* tile_dcnt, const int* kcnt, int64_t ntiles,
+                                                      unsigned long long k, int renorm, int64_t* koff,
+                                                      OpStats* st) {
   __shared__ double s_red[WAVES];
-  __shared__ unsigned long long s_cnt;
-  if (threadIdx.x == 0) s_cnt = 0;
-  __syncthreads();
-  const double d = block_sum_array(tile_dsum, ntiles, s_red);
-  const double c = block_sum_array(part_sum, nparts, s_red);
-  unsigned long long pc = 0;
-  for (int j = threadIdx.x; j < nparts; j += NT) pc += part_cnt[j];
-  atomicAdd(&s_cnt, pc);
-  __syncthreads();
+  __shared__ int s_scan[WAVES];
+  double s = 0.0;
+  for (int64_t t = threadIdx.x; t < ntiles; t += NT) s += tile_dsum[t] + ksum[t];
+  const double kept = block_sum(s, s_red);
+  int64_t carry = 0;
+  for (int64_t t0 = 0; t0 < ntiles; t0 += NT) {
+    const int64_t t = t0 + threadIdx.x;
+    const int v = t < ntiles ? tile_dcnt[t] + kcnt[t] : 0;
+    int tot;
+    const int ex = block_excl_scan(v, tot, s_scan);
+    if (t < ntiles) koff[t] = carry + ex;
+    carry += tot;
+  }
   if (threadIdx.x == 0) {
-    const unsigned long long cnt = sel->def_count + s_cnt;
-    if (cnt != k) set_err(&st->err, ERR_SELECT);
-    const double kept = d + c;
+    if ((unsigned long long)carry != k) set_err(&st->err, ERR_SELECT);
+    st->nnz = carry;
     st->kept = kept;
     const double norm = sqrt(kept);
     st->norm = (renorm && norm > 1e-30) ? norm : 1.0;
   }
 }
 
-__global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x,
-                                                 const double2* __restrict__ a, int64_t n,
-                                                 const SelState* sel, uint64_t* __restrict__ xo,
-                                                 double2* __restrict__ ao, int64_t cap_out,
-                                                 OpStats* st, double* tile_s2, double* tile_s4,
-                                                 unsigned long long* lb_status, unsigned int* ticket,
-                                                 int* err) {
+__global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x, const double2* __restrict__ a,
+                                                 const int* seg_cnt, int64_t n_dense, const SelState* sel,
+                                                 const int64_t* __restrict__ koff, uint64_t* __restrict__ xo,
+                                                 double2* __restrict__ ao, int64_t cap_out, const OpStats* st,
+                                                 double* tile_s2, double* tile_s4, int* err) {
   __shared__ int s_scan[WAVES];
   __shared__ double s_red[WAVES];
-  __shared__ unsigned int s_tile;
-  __shared__ unsigned long long s_base;
-  const unsigned int tile = take_ticket(ticket, &s_tile);
+  const unsigned int tile = blockIdx.x;
   const unsigned long long pstar = sel->pstar, keystar = sel->keystar;
   // numpy divides complex128 by a real norm as a product with 1/norm
   const double inv = 1.0 / st->norm;
+  const int cnt = tile_count(seg_cnt, n_dense, tile);
+  const int64_t base = (int64_t)tile * SEL_TILE;
   uint64_t kk[SEL_ITEMS];
   double2 vv[SEL_ITEMS];
   bool keep[SEL_ITEMS];
-  int off[SEL_ITEMS];
-  int rtot[SEL_ITEMS];
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
-    const int64_t i = (int64_t)tile * SEL_TILE + (int64_t)it * NT + threadIdx.x;
+    const int j = it * NT + threadIdx.x;
     keep[it] = false;
-    if (i < n) {
-      kk[it] = x[i];
-      vv[it] = a[i];
+    if (j < cnt) {
+      kk[it] = x[base + j];
+      vv[it] = a[base + j];
       keep[it] = kept_pred(pbits_of(vv[it]), kk[it], pstar, keystar);
     }
-    int tot;
-    off[it] = block_excl_scan(keep[it] ? 1 : 0, tot, s_scan);
-    rtot[it] = tot;
   }
-  unsigned long long ttot = 0;
-#pragma unroll
-  for (int it = 0; it < SEL_ITEMS; ++it) ttot += (unsigned long long)rtot[it];
-  if (threadIdx.x < 64) {
-    const unsigned long long e = lookback_prefix_wave(lb_status, tile, ttot, err);
-    if (threadIdx.x == 0) s_base = e;
-  }
-  __syncthreads();
-  const unsigned long long base = s_base;
-  const bool fits = base + ttot <= (unsigned long long)cap_out;
-  if (!fits && threadIdx.x == 0) set_err(err, ERR_OVERFLOW);
-  unsigned long long rb = base;
+  int64_t rb = koff[tile];
   double s2 = 0.0, s4 = 0.0;
+  bool fits = true;
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
+    int tot;
+    const int off = block_excl_scan(keep[it] ? 1 : 0, tot, s_scan);
     if (keep[it]) {
       const double2 v = make_double2(vv[it].x * inv, vv[it].y * inv);
-      const unsigned long long pos = rb + (unsigned long long)off[it];
-      if (fits) {
+      const int64_t pos = rb + off;
+      if (pos < cap_out) {
         xo[pos] = kk[it];
         ao[pos] = v;
+      } else {
+        fits = false;
       }
       const double p = cabs2(v);
       s2 += p;
       s4 += p * p;
     }
-    rb += (unsigned long long)rtot[it];
+    rb += tot;
   }
+  if (!fits) set_err(err, ERR_OVERFLOW);
   const double t2 = block_sum(s2, s_red);
   const double t4 = block_sum(s4, s_red);
   if (threadIdx.x == 0) {
     tile_s2[tile] = t2;
     tile_s4[tile] = t4;
-    if (tile == gridDim.x - 1) st->nnz = (int64_t)(base + ttot);
   }
 }
 
@@ -478,6 +476,35 @@ __global__ __launch_bounds__(NT) void k_sums_finalize(const double* tile_s2, con
   if (threadIdx.x == 0) {
     st->sum2 = s2;
     st->sum4 = s4;
+  }
+}
+
+// Segment counts -> exclusive offsets and total (nnz of a segmented output).
+__global__ __launch_bounds__(NT) void k_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff,
+                                                  OpStats* st) {
+  __shared__ int s_scan[WAVES];
+  int64_t carry = 0;
+  for (int64_t t0 = 0; t0 < ntiles; t0 += NT) {
+    const int64_t t = t0 + threadIdx.x;
+    const int v = t < ntiles ? seg_cnt[t] : 0;
+    int tot;
+    const int ex = block_excl_scan(v, tot, s_scan);
+    if (t < ntiles) soff[t] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) st->nnz = carry;
+}
+
+// Copy a segmented array to dense positions (soff from k_seg_scan).
+__global__ __launch_bounds__(NT) void k_densify(const uint64_t* __restrict__ x, const double2* __restrict__ a,
+                                                 const int* seg_cnt, const int64_t* soff,
+                                                 uint64_t* __restrict__ xo, double2* __restrict__ ao) {
+  const int64_t t = blockIdx.x;
+  const int cnt = seg_cnt[t];
+  const int64_t src = t * SEL_TILE, dst = soff[t];
+  for (int j = threadIdx.x; j < cnt; j += NT) {
+    xo[dst + j] = x[src + j];
+    ao[dst + j] = a[src + j];
   }
 }
 
@@ -512,7 +539,7 @@ __global__ __launch_bounds__(NT) void k_scale(double2* __restrict__ a, int64_t n
   }
 }
 
-// Gather x_out[j] = x[idx[j]], a_out[j] = a[idx[j]]; optional sum |a|^2 per block.
+// Gather x_out[j] = x[idx[j]], a_out[j] = a[idx[j]].
 __global__ __launch_bounds__(NT) void k_gather(const uint64_t* __restrict__ x, const double2* __restrict__ a,
                                                 const int64_t* __restrict__ idx, int64_t m,
                                                 uint64_t* __restrict__ xo, double2* __restrict__ ao) {
@@ -527,30 +554,42 @@ __global__ __launch_bounds__(NT) void k_gather(const uint64_t* __restrict__ x, c
 // ----------------------------------------------------------- host side ----
 
 int64_t sel_tiles(int64_t n) { return (n + SEL_TILE - 1) / SEL_TILE; }
-int sel_kept_blocks() { return KEPT_BLOCKS; }
 
 int launch_select(const SelLaunch& L, hipStream_t s) {
-  const int64_t tiles = sel_tiles(L.n);
+  const int64_t tiles = L.ntiles;
   int64_t npsum = L.npsum;
   if (!L.hist_ready) {
-    hipLaunchKernelGGL(k_hist_psum, dim3((unsigned)tiles), dim3(NT), 0, s, L.a, L.n, L.hist1, L.tile_psum);
+    hipLaunchKernelGGL(k_hist_psum, dim3((unsigned)tiles), dim3(NT), 0, s, L.a, L.seg_cnt, L.n, L.hist1,
+                       L.tile_psum);
     npsum = tiles;
   }
   hipLaunchKernelGGL(k_find1, dim3(1), dim3(NT), 0, s, L.hist1, L.k, L.tile_psum, npsum, L.sel, L.st);
-  hipLaunchKernelGGL(k_extract1, dim3((unsigned)tiles), dim3(NT), 0, s, L.a, L.n, L.sel, L.c1p, L.c1i,
-                     L.hist2, L.tile_dsum, L.lb1, L.ticket1, &L.st->err);
+  hipLaunchKernelGGL(k_extract1, dim3((unsigned)tiles), dim3(NT), 0, s, L.a, L.seg_cnt, L.n, L.sel, L.c1p,
+                     L.c1i, L.cnt1, L.hist2, L.tile_dsum, L.tile_dcnt);
   hipLaunchKernelGGL(k_find2, dim3(1), dim3(NT), 0, s, L.hist2, L.sel, L.st);
-  int64_t eb = (L.n + NT - 1) / NT;
-  if (eb > 1024) eb = 1024;
-  hipLaunchKernelGGL(k_extract2, dim3((unsigned)eb), dim3(NT), 0, s, L.c1p, L.c1i, L.sel, L.c2p, L.c2i);
+  int64_t eb = tiles < 1024 ? tiles : 1024;
+  hipLaunchKernelGGL(k_extract2, dim3((unsigned)eb), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, tiles, L.sel, L.c2p,
+                     L.c2i);
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(NT), 0, s, L.c2p, L.c2i, L.x, L.sel, L.st);
-  hipLaunchKernelGGL(k_keptsum, dim3(KEPT_BLOCKS), dim3(NT), 0, s, L.c1p, L.c1i, L.x, L.sel, L.part_sum,
-                     L.part_cnt);
-  hipLaunchKernelGGL(k_sel_finalize, dim3(1), dim3(NT), 0, s, L.tile_dsum, tiles, L.part_sum, L.part_cnt,
-                     KEPT_BLOCKS, L.sel, L.k, L.renorm, L.st);
-  hipLaunchKernelGGL(k_compact, dim3((unsigned)tiles), dim3(NT), 0, s, L.x, L.a, L.n, L.sel, L.xo, L.ao,
-                     L.cap_out, L.st, L.tile_s2, L.tile_s4, L.lb2, L.ticket2, &L.st->err);
+  hipLaunchKernelGGL(k_keptsum, dim3((unsigned)tiles), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, L.x, L.sel, L.ksum,
+                     L.kcnt);
+  hipLaunchKernelGGL(k_sel_finalize, dim3(1), dim3(NT), 0, s, L.tile_dsum, L.ksum, L.tile_dcnt, L.kcnt, tiles,
+                     L.k, L.renorm, L.koff, L.st);
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)tiles), dim3(NT), 0, s, L.x, L.a, L.seg_cnt, L.n, L.sel, L.koff,
+                     L.xo, L.ao, L.cap_out, L.st, L.tile_s2, L.tile_s4, &L.st->err);
   hipLaunchKernelGGL(k_sums_finalize, dim3(1), dim3(NT), 0, s, L.tile_s2, L.tile_s4, tiles, L.st);
+  return (int)hipGetLastError();
+}
+
+int launch_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff, OpStats* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(NT), 0, s, seg_cnt, ntiles, soff, st);
+  return (int)hipGetLastError();
+}
+
+int launch_densify(const uint64_t* x, const double2* a, const int* seg_cnt, const int64_t* soff, int64_t ntiles,
+                   uint64_t* xo, double2* ao, hipStream_t s) {
+  if (ntiles <= 0) return 0;
+  hipLaunchKernelGGL(k_densify, dim3((unsigned)ntiles), dim3(NT), 0, s, x, a, seg_cnt, soff, xo, ao);
   return (int)hipGetLastError();
 }
 

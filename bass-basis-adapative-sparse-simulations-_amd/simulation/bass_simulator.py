@@ -340,34 +340,30 @@ class BASS:
                 # tentative: rotate -> truncate to k -> renormalise -> PR
                 rot = state._rot
                 m, rht = _ops.gate(ctx, state._buf, state.nnz, V.conj().T, (j,), rot, mode=L.MODE_ROT,
-                                   dedup=True)
+                                   dedup=True, segmented=True)
                 self._count("rotate")
                 self._count("amp_updates", m)
                 self._bytes("rot1q", state.nnz, m)
                 if m > self.k:
-                    st = _ops.truncate(ctx, rot, m, self.k, cand, renorm=True, hist_tiles=rht)
+                    st = _ops.truncate(ctx, rot, m, self.k, cand, renorm=True, hist_tiles=rht, segmented=True)
                     self._bytes("select", m, self.k)
                     nn, frac, s4 = self.k, st.kept, st.sum4
-                    dst = cand
                 else:
-                    s2, _ = _ops.moments(ctx, rot, m)
+                    _ops.densify(ctx, rot, rht, m, cand)
+                    s2, _ = _ops.moments(ctx, cand, m)
                     frac = s2
                     norm = math.sqrt(s2)
                     if norm > 1e-30:
-                        _ops.scale(ctx, rot, m, norm)
-                    _, s4 = _ops.moments(ctx, rot, m)
+                        _ops.scale(ctx, cand, m, norm)
+                    _, s4 = _ops.moments(ctx, cand, m)
                     nn = m
-                    dst = rot
                 pr_new = 1.0 / s4 if s4 > 1e-60 else float("inf")
                 if pr_new >= pr:
                     self._count("reject")
                     continue
                 # accept: the tentative buffer becomes the state
-                if dst is cand:
-                    state._buf, state._tmp = cand, state._buf
-                    cand = state._tmp
-                else:
-                    state._buf, state._rot = rot, state._buf
+                state._buf, state._tmp = cand, state._buf
+                cand = state._tmp
                 state.nnz = nn
                 state._sum2, state._sum4 = None, s4
                 state._hist_tiles = 0
@@ -432,17 +428,25 @@ class BASS:
         return state
 
     def step(self, i, gate):
+        """One iteration of the reference loop (:861-926).  Gate outputs are
+        written in per-tile segments of state._tmp; the truncation (or, without
+        one, a dense copy) brings the state back to state._buf."""
         state, ctx, M = self.state, self._ctx, self._M
+        seg = None  # (n, tiles) while state._tmp holds a segmented gate output
         if gate.n_qubits == 1:
             q = gate.qubits[0]
             G = self.U[q].conj().T @ gate.matrix @ self.U[q]
-            n, ht = _ops.gate(ctx, state._buf, state.nnz, G, (q,), state._tmp, dedup=True)
+            n, ht = _ops.gate(ctx, state._buf, state.nnz, G, (q,), state._tmp, dedup=True, segmented=True)
             self._bytes("gate1q", state.nnz, n)
-            state._buf, state._tmp = state._tmp, state._buf
-            state.nnz, state._hist_tiles, state._sum4 = n, ht, None
+            seg = (n, ht)
+            state.nnz, state._hist_tiles, state._sum4 = n, 0, None
             self._count("amp_updates", n)
             if state.nnz > state.k:  # SparseState's own budget (:297-301)
-                state.truncate_top_k()
+                st = _ops.truncate(ctx, state._tmp, n, state.k, state._buf, renorm=True, hist_tiles=ht,
+                                   segmented=True)
+                state.gamma *= np.sqrt(st.kept)
+                state.nnz = state.k
+                seg = None
             self._since_trunc += 1
             self._count("gate1q")
         elif gate.n_qubits == 2:
@@ -462,10 +466,10 @@ class BASS:
                         self._pr_at_last_opt = self._pr(state)
                     self._trunc_active = False
                 return
-            n, ht = _ops.gate(ctx, state._buf, state.nnz, G, (q1, q2), state._tmp, dedup=True)
+            n, ht = _ops.gate(ctx, state._buf, state.nnz, G, (q1, q2), state._tmp, dedup=True, segmented=True)
             self._bytes("gate2q", state.nnz, n)
-            state._buf, state._tmp = state._tmp, state._buf
-            state.nnz, state._hist_tiles, state._sum4 = n, ht, None
+            seg = (n, ht)
+            state.nnz, state._hist_tiles, state._sum4 = n, 0, None
             self._count("amp_updates", n)
             self._since_trunc += 1
             self._count("gate2q")
@@ -476,10 +480,15 @@ class BASS:
         over_budget = state.nnz > self.k
         if over_hard or (over_budget and self._since_trunc >= self.truncate_every):
             self._trunc_active = True
-            st = _ops.truncate(ctx, state._buf, state.nnz, self.k, state._tmp, renorm=True,
-                               hist_tiles=state._hist_tiles)
+            if seg is not None:
+                st = _ops.truncate(ctx, state._tmp, seg[0], self.k, state._buf, renorm=True, hist_tiles=seg[1],
+                                   segmented=True)
+                seg = None
+            else:
+                st = _ops.truncate(ctx, state._buf, state.nnz, self.k, state._tmp, renorm=True,
+                                   hist_tiles=state._hist_tiles)
+                state._buf, state._tmp = state._tmp, state._buf
             self._bytes("select", state.nnz, self.k)
-            state._buf, state._tmp = state._tmp, state._buf
             total, kept = st.total, st.kept
             self._gamma_running *= kept / total if total > 1e-30 else 0
             state.nnz = self.k
@@ -487,6 +496,8 @@ class BASS:
             state._hist_tiles = 0
             self._since_trunc = 0
             self._count("truncate")
+        elif seg is not None:
+            _ops.densify(ctx, state._tmp, seg[1], seg[0], state._buf)
 
         if self._check_point(i, M):
             if self._trunc_active and self.optimize_trigger(state):

@@ -75,10 +75,21 @@ int bass_read_stats(void* ws, size_t ws_bytes, bass_stats* out, void* stream);
  * *hist_tiles receives the value to pass on (0 when no histogram was made).
  * Non-deduplicating calls keep the reference's output order (input order,
  * out index ascending); deduplicating calls order outputs by the first input
- * entry of each group.  Returns BASS_E_OVERFLOW if cap_out is too small. */
+ * entry of each group.
+ * segmented=0: dense output, cap_out >= 2^nq n.  segmented=1: tile t of the
+ * input writes its outputs to slots [t*2048, t*2048 + count_t) with no
+ * coordination between tiles (cap_out >= *hist_tiles * 2048); the result feeds
+ * bass_truncate(..., segmented=1) or bass_densify.  stats.nnz is the total.
+ * Returns BASS_E_OVERFLOW if cap_out is too small. */
 int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int q1,
-              const double* G, int mode, int dedup, uint64_t* xo, double* ao, int64_t cap_out,
-              int64_t* hist_tiles, void* ws, size_t ws_bytes, void* stream);
+              const double* G, int mode, int dedup, int segmented, uint64_t* xo, double* ao,
+              int64_t cap_out, int64_t* hist_tiles, void* ws, size_t ws_bytes, void* stream);
+
+/* Copies the segmented output of the preceding bass_gate (seg_tiles = its
+ * *hist_tiles) to dense positions in (xo, ao), preserving order.  No reference
+ * counterpart (the reference's kernels write densely). */
+int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t* xo, double* ao,
+                 int64_t cap_out, void* ws, size_t ws_bytes, void* stream);
 
 /* Top-k truncation: keep the k entries of largest |a|^2 (ties: smaller key),
  * compacted in input order into (xo, ao); renorm=1 divides them by
@@ -86,10 +97,11 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
  * (src/core/truncation.py:68-113; renorm=0, gamma_sq = stats.kept) and the
  * driver truncations of src/simulation/bass_simulator.py:565-572, :899-916 and
  * src/simulation/simulator.py:169-201 (renorm=1).  Requires n > k.
- * hist_tiles: value reported by the bass_gate call that produced (x, a), or 0. */
+ * hist_tiles: value reported by the bass_gate call that produced (x, a), or 0;
+ * segmented=1: (x, a) is that call's segmented output (n = stats.nnz). */
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo,
-                  double* ao, int64_t cap_out, int renorm, int64_t hist_tiles, void* ws,
-                  size_t ws_bytes, void* stream);
+                  double* ao, int64_t cap_out, int renorm, int64_t hist_tiles, int segmented,
+                  void* ws, size_t ws_bytes, void* stream);
 
 /* In place: a[i] *= diag[(b_q1 << 1) | b_q2]; diag = 4 complex (8 doubles).
  * Replaces apply_2qubit_diagonal_kernel (src/core/sparse_state.py:178-190). */

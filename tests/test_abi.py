@@ -57,19 +57,23 @@ def test_argument_errors_without_gpu(lib):
     L = pkg_module("_lib")
     ht = ctypes.c_int64(0)
     G = (ctypes.c_double * 8)()
-    rc = lib.bass_gate(None, None, 10, 3, 0, 1, G, 0, 1, None, None, 10, ctypes.byref(ht), None, 0, None)
+    rc = lib.bass_gate(None, None, 10, 3, 0, 1, G, 0, 1, 0, None, None, 10, ctypes.byref(ht), None, 0, None)
     assert rc == L.BASS_E_ARG and "nq" in L.last_error()
-    rc = lib.bass_gate(None, None, 10, 2, 4, 4, G, 0, 1, None, None, 40, ctypes.byref(ht), None, 0, None)
+    rc = lib.bass_gate(None, None, 10, 2, 4, 4, G, 0, 1, 0, None, None, 40, ctypes.byref(ht), None, 0, None)
     assert rc == L.BASS_E_ARG and "qubits" in L.last_error()
-    rc = lib.bass_gate(None, None, 10, 1, 0, 0, G, 1, 0, None, None, 40, ctypes.byref(ht), None, 0, None)
+    rc = lib.bass_gate(None, None, 10, 1, 0, 0, G, 1, 0, 0, None, None, 40, ctypes.byref(ht), None, 0, None)
     assert rc == L.BASS_E_ARG and "rotation" in L.last_error()
-    rc = lib.bass_truncate(None, None, 5, 5, None, None, 5, 1, 0, None, 0, None)
+    rc = lib.bass_gate(None, None, 10, 1, 0, 0, G, 0, 1, 1, None, None, 40, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_OVERFLOW and "segmented" in L.last_error()
+    rc = lib.bass_truncate(None, None, 5, 5, None, None, 5, 1, 0, 0, None, 0, None)
     assert rc == L.BASS_E_ARG
-    rc = lib.bass_truncate(None, None, 10, 5, None, None, 4, 1, 0, None, 0, None)
+    rc = lib.bass_truncate(None, None, 10, 5, None, None, 4, 1, 0, 0, None, 0, None)
     assert rc == L.BASS_E_OVERFLOW
+    rc = lib.bass_truncate(None, None, 10, 5, None, None, 5, 1, 0, 1, None, 0, None)
+    assert rc == L.BASS_E_ARG
     rc = lib.bass_radix_sort(None, None, 10, 0, None, 0, None)
     assert rc == L.BASS_E_ARG
-    rc = lib.bass_gate(None, None, 10, 1, 0, 0, G, 0, 1, None, None, 40, ctypes.byref(ht), None, 0, None)
+    rc = lib.bass_gate(None, None, 10, 1, 0, 0, G, 0, 1, 0, None, None, 40, ctypes.byref(ht), None, 0, None)
     assert rc == L.BASS_E_WORKSPACE
 
 

@@ -102,6 +102,41 @@ def test_gate_1q_dedup(gpu_ctx, n, nbits, q):
     assert_same_entries(gx, ga, okx, oka, exact=exact())
 
 
+@pytest.mark.parametrize("nq,n,k", [(1, 1, 1), (1, 3000, 2000), (1, 70000, 50000), (2, 5000, 3000),
+                                    (2, 40000, 60000)])
+def test_segmented_gate_matches_dense(gpu_ctx, nq, n, k):
+    print(f"[case] segmented nq={nq} n={n} k={k}", flush=True)
+    ops = pkg_module("_ops")
+    dev = pkg_module("_device")
+    rng = np.random.default_rng(n + nq)
+    x, a = random_state(n, 20, seed=n + 3)
+    G = _gate_with_zero(rng, 2 ** nq)
+    qs = (5,) if nq == 1 else (5, 12)
+    src = dev.Buffers(n, gpu_ctx.device)
+    src.x[:n] = torch.from_numpy(x.view(np.int64)).to(gpu_ctx.device)
+    src.a[:n] = torch.from_numpy(a).to(gpu_ctx.device)
+    dense = dev.Buffers(1, gpu_ctx.device)
+    m, _ = ops.gate(gpu_ctx, src, n, G, qs, dense, dedup=True)
+    segb = dev.Buffers(1, gpu_ctx.device)
+    m2, tiles = ops.gate(gpu_ctx, src, n, G, qs, segb, dedup=True, segmented=True)
+    assert m2 == m
+    out = dev.Buffers(1, gpu_ctx.device)
+    ops.densify(gpu_ctx, segb, tiles, m2, out)
+    assert np.array_equal(dev.to_host_keys(out.x, m), dev.to_host_keys(dense.x, m)), "order must match"
+    assert np.array_equal(dev.to_host_amps(out.a, m), dev.to_host_amps(dense.a, m))
+    if m > k:
+        t1 = dev.Buffers(k, gpu_ctx.device)
+        s1 = ops.truncate(gpu_ctx, dense, m, k, t1, renorm=True)
+        m3, tiles = ops.gate(gpu_ctx, src, n, G, qs, segb, dedup=True, segmented=True)
+        t2 = dev.Buffers(k, gpu_ctx.device)
+        s2 = ops.truncate(gpu_ctx, segb, m3, k, t2, renorm=True, hist_tiles=tiles, segmented=True)
+        assert np.array_equal(dev.to_host_keys(t1.x, k), dev.to_host_keys(t2.x, k))
+        # the two inputs are tiled differently, so the kept sum (and the
+        # renormalisation) may differ in the last bit
+        np.testing.assert_allclose(dev.to_host_amps(t1.a, k), dev.to_host_amps(t2.a, k), rtol=1e-15, atol=0)
+        assert abs(s1.kept - s2.kept) <= 1e-15 * s1.kept and abs(s1.total - s2.total) <= 1e-15 * s1.total
+
+
 @pytest.mark.parametrize("n,nbits,q1,q2", [(3, 3, 0, 2), (5000, 13, 4, 1), (40000, 18, 17, 0)])
 def test_gate_2q_ht(gpu_ctx, n, nbits, q1, q2):
     print(f"[case] gate2q-ht n={n} q=({q1},{q2})", flush=True)
