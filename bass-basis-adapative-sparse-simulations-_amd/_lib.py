@@ -22,6 +22,7 @@ BASS_E_DEVICE = -5
 
 MODE_GATE = 0
 MODE_ROT = 1
+MODE_MOVE = 2
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -56,7 +57,8 @@ SIGNATURES = {
     "bass_gate": (_I, [_P, _P, _I64, _I, _I, _I, _P, _I, _I, _I, _P, _P, _I64, ctypes.POINTER(_I64), _P, _SZ,
                        _P]),
     "bass_densify": (_I, [_P, _P, _I64, _P, _P, _I64, _P, _SZ, _P]),
-    "bass_truncate": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _I64, _I, _P, _SZ, _P]),
+    "bass_lift": (_I, [_P, _P, _I64, _I, _I, _P, _I, _P, _P, _I64, ctypes.POINTER(_I64), _P, _SZ, _P]),
+    "bass_truncate": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _I64, _I, _P, _I, _P, _SZ, _P]),
     "bass_diag2q": (_I, [_P, _P, _I64, _I, _I, _P, _P]),
     "bass_moments": (_I, [_P, _I64, _P, _SZ, _P]),
     "bass_scale": (_I, [_P, _I64, _D, _P]),

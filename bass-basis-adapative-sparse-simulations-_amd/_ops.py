@@ -47,12 +47,50 @@ def densify(ctx, src, seg_tiles, n, dst):
     L.check(rc, "bass_densify")
 
 
-def truncate(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False):
-    """Top-k of src[:n] (n > k) into dst[:k].  Returns stats (kept, total, sum2, sum4, norm)."""
+def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE):
+    """Key-ordered gate (mode GATE / ROT) or bit move (MODE_MOVE) on src[:n].
+
+    Gates: returns (n_out, seg_tiles) with the output segmented in dst.
+    Move: returns (n, 0) with the output dense in dst."""
+    tm = 1024 if mode == L.MODE_MOVE else SEG // (1 << nq)
+    tiles = (n + tm - 1) // tm
+    need = n if mode == L.MODE_MOVE else tiles * SEG
+    ctx.ensure_ws(max(need, n, 1))
+    dst.ensure(max(need, 1))
+    if mode == L.MODE_MOVE:
+        buf, gp = None, None
+    else:
+        buf, gp = L.mat_arg(G)
+    st = ctypes.c_int64(0)
+    rc = ctx.lib.bass_lift(src.xp, src.ap, n, nq, int(j), gp, mode, dst.xp, dst.ap, dst.cap, ctypes.byref(st),
+                           ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_lift")
+    if mode == L.MODE_MOVE:
+        return n, 0
+    return ctx.read_stats("bass_lift").nnz, int(st.value)
+
+
+def key_map_arg(pos):
+    """Stored-bit -> qubit map (int32[N]) of an ordered state whose qubit q sits at bit pos[q]."""
+    pos = np.asarray(pos)
+    km = np.empty(len(pos), dtype=np.int32)
+    km[pos] = np.arange(len(pos), dtype=np.int32)
+    return km
+
+
+def truncate(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False, key_map=None):
+    """Top-k of src[:n] (n > k) into dst[:k].  Returns stats (kept, total, sum2, sum4, norm).
+
+    key_map: stored-bit -> qubit map (key_map_arg) when src holds stored words."""
     ctx.ensure_ws(max(n, hist_tiles * SEG if segmented else 0))
     dst.ensure(k)
+    if key_map is None:
+        kp, kb = None, 0
+    else:
+        km = np.ascontiguousarray(key_map, dtype=np.int32)
+        kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
     rc = ctx.lib.bass_truncate(src.xp, src.ap, n, k, dst.xp, dst.ap, dst.cap, 1 if renorm else 0,
-                               hist_tiles, 1 if segmented else 0, ctx.wsp, ctx.ws_bytes, ctx.stream)
+                               hist_tiles, 1 if segmented else 0, kp, kb, ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_truncate")
     return ctx.read_stats("bass_truncate")
 

@@ -16,6 +16,31 @@ from .._device import Buffers, context, to_host_amps, to_host_keys
 
 
 # --------------------------------------------------------------------------
+# key-ordered states: stored word <-> basis key
+
+
+def lifted_positions(pos, j, t):
+    """Qubit positions after moving stored bit j down to bit t (bits t..j-1 move up by one)."""
+    pos = np.asarray(pos)
+    new = pos.copy()
+    new[(pos >= t) & (pos < j)] += 1
+    new[pos == j] = t
+    return new
+
+
+def stored_to_keys(w, pos):
+    """Basis keys of stored words w (uint64) of a state whose qubit q sits at bit pos[q]."""
+    w = np.asarray(w, dtype=np.uint64)
+    if pos is None or np.array_equal(pos, np.arange(len(pos))):
+        return w.copy()
+    out = np.zeros_like(w)
+    one = np.uint64(1)
+    for q, p in enumerate(pos):
+        out |= ((w >> np.uint64(p)) & one) << np.uint64(q)
+    return out
+
+
+# --------------------------------------------------------------------------
 # argument staging
 
 
@@ -189,12 +214,15 @@ class SparseState:
         self._temp_nnz = 0
         self._hist_tiles = 0  # digit histogram of the current entries, if fresh
         self.dedup = dedup
+        # key-ordered layout (set by the BASS driver): entries sorted by stored
+        # word, qubit q at bit pos[q]; None = keys stored as is
+        self.pos = None
 
     # host views ------------------------------------------------------------
     @property
     def x(self):
         out = np.zeros(self._buf.cap, dtype=np.uint64)
-        out[: self.nnz] = to_host_keys(self._buf.x, self.nnz)
+        out[: self.nnz] = stored_to_keys(to_host_keys(self._buf.x, self.nnz), self.pos)
         return out
 
     @property
@@ -222,14 +250,19 @@ class SparseState:
         s._buf.a[: self.nnz].copy_(self._buf.a[: self.nnz])
         s.nnz = self.nnz
         s.gamma = self.gamma
+        s.pos = None if self.pos is None else self.pos.copy()
         return s
+
+    def keys(self):
+        """Basis keys of the entries (host, uint64[nnz])."""
+        return stored_to_keys(to_host_keys(self._buf.x, self.nnz), self.pos)
 
     def to_dense(self):
         """sparse_state.py:240-255 (stored amplitudes, no division by gamma)."""
         if self.N > 20:
             raise ValueError("Dense conversion only allowed for N <= 20")
         dense = np.zeros(2 ** self.N, dtype=np.complex128)
-        dense[to_host_keys(self._buf.x, self.nnz).astype(np.int64)] = to_host_amps(self._buf.a, self.nnz)
+        dense[self.keys().astype(np.int64)] = to_host_amps(self._buf.a, self.nnz)
         return dense
 
     def probability_distribution(self):
@@ -239,7 +272,7 @@ class SparseState:
         p = self.probability_distribution()
         p /= p.sum()
         idx = np.random.choice(self.nnz, size=num_samples, p=p)
-        return to_host_keys(self._buf.x, self.nnz)[idx]
+        return self.keys()[idx]
 
     def _swap(self):
         self._buf, self._tmp = self._tmp, self._buf
@@ -280,6 +313,8 @@ class SparseState:
 
     def apply_gate(self, gate, method="top-k", seed="None"):
         """sparse_state.py:379-385."""
+        if self.pos is not None:
+            raise ValueError("state is in the BASS driver's key-ordered layout; apply gates through the driver")
         if gate.n_qubits == 1:
             self.apply_single_qubit_gate(gate, method, seed)
         elif gate.n_qubits == 2:

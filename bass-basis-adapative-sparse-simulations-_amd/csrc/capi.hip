@@ -44,7 +44,7 @@ struct Layout {
   // persistent
   size_t st, tile_psum, tile_dsum, tile_s2, tile_s4, rdm_part, rho, sort_hist, sort_tick;
   size_t lb_gate, lb1, sort_status;
-  size_t seg_cnt, soff, cnt1, tile_dcnt, ksum, kcnt, koff;
+  size_t seg_cnt, soff, cnt1, tile_dcnt, ksum, kcnt, koff, lift_b;
   size_t big;
   size_t total;
 };
@@ -82,6 +82,7 @@ static Layout layout(int64_t S) {
   L.ksum = o; o += al(stiles * 8);
   L.kcnt = o; o += al(stiles * 4);
   L.koff = o; o += al(stiles * 8);
+  L.lift_b = o; o += al((etiles + 1) * sizeof(LiftBound));
   L.big = o;
   // big scratch union:
   //   gate group table: cap <= 4 n_in slots x <= 32 B, n_in <= S/4 (2q) or S/2 (1q) -> <= 32 S
@@ -297,6 +298,43 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
   return hip_check("bass_gate");
 }
 
+int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G, int mode,
+              uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws, size_t ws_bytes,
+              void* stream) {
+  if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_lift: nq must be 1 or 2");
+  if (mode != MODE_GATE && mode != MODE_ROT && mode != MODE_MOVE) return fail(BASS_E_ARG, "bass_lift: bad mode");
+  if (mode == MODE_MOVE && nq != 1) return fail(BASS_E_ARG, "bass_lift: a move lifts one bit");
+  if (j < nq - 1 || j > 62) return fail(BASS_E_ARG, "bass_lift: bad bit %d for %d qubit(s)", j, nq);
+  if (n < 0 || n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_lift: bad n %lld", (long long)n);
+  const int tm = lift_tile_entries(nq, mode);
+  const int64_t tiles = n > 0 ? (n + tm - 1) / tm : 0;
+  const int64_t need_out = mode == MODE_MOVE ? n : tiles * SEG_ENTRIES;
+  if (cap_out < need_out)
+    return fail(BASS_E_OVERFLOW, "bass_lift: output needs %lld slots", (long long)need_out);
+  WS_REQUIRE(need_out > n ? need_out : n);
+  hipStream_t s = (hipStream_t)stream;
+  if (seg_tiles) *seg_tiles = mode == MODE_MOVE ? 0 : tiles;
+  OpStats* st = at<OpStats>(ws, L.st);
+  static const char* names[3][2] = {{"gate1q", "gate2q"}, {"rot1q", "rot2q"}, {"move", "move"}};
+  const int tk = ktimer_begin(names[mode][nq - 1], s);
+  GateMat gm;
+  memset(&gm, 0, sizeof(gm));
+  if (mode != MODE_MOVE) {
+    const int M = 1 << nq;
+    for (int e = 0; e < M * M; ++e) gm.m[e] = make_double2(G[2 * e], G[2 * e + 1]);
+    fill(at<void>(ws, L.hist1), HIST_BINS * 4, 0ull, s);
+  }
+  fill(st, 48, 0ull, s);
+  if (n > 0) {
+    int* seg = at<int>(ws, L.seg_cnt);
+    launch_lift(nq, mode, x, (const double2*)a, n, j, gm, xo, (double2*)ao, at<LiftBound>(ws, L.lift_b),
+                at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), seg, s);
+    if (mode != MODE_MOVE) launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), st, s);
+  }
+  ktimer_end(tk, s);
+  return hip_check("bass_lift");
+}
+
 int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t* xo, double* ao,
                  int64_t cap_out, void* ws, size_t ws_bytes, void* stream) {
   if (seg_tiles < 0) return fail(BASS_E_ARG, "bass_densify: bad segment count");
@@ -308,13 +346,27 @@ int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t
 }
 
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
-                  int64_t cap_out, int renorm, int64_t hist_tiles, int segmented, void* ws, size_t ws_bytes,
-                  void* stream) {
+                  int64_t cap_out, int renorm, int64_t hist_tiles, int segmented, const int32_t* key_map,
+                  int key_bits, void* ws, size_t ws_bytes, void* stream) {
   if (k < 1 || n <= k) return fail(BASS_E_ARG, "bass_truncate: need n > k >= 1 (n=%lld k=%lld)",
                                    (long long)n, (long long)k);
   if (cap_out < k) return fail(BASS_E_OVERFLOW, "bass_truncate: cap_out < k");
   if (n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_truncate: n too large");
   if (segmented && hist_tiles <= 0) return fail(BASS_E_ARG, "bass_truncate: segmented input needs hist_tiles");
+  KeyMap km;
+  memset(&km, 0, sizeof(km));
+  if (key_map) {
+    if (key_bits < 1 || key_bits > 63) return fail(BASS_E_ARG, "bass_truncate: key_bits out of range");
+    uint64_t seen = 0;
+    for (int p = 0; p < key_bits; ++p) {
+      const int q = key_map[p];
+      if (q < 0 || q >= key_bits || ((seen >> q) & 1ull))
+        return fail(BASS_E_ARG, "bass_truncate: key_map is not a permutation");
+      seen |= 1ull << q;
+      km.q[p] = (unsigned char)q;
+    }
+    km.nbits = key_bits;
+  }
   const int64_t tiles = segmented ? hist_tiles : sel_tiles(n);
   const int64_t slots = tiles * SEG_ENTRIES;
   WS_REQUIRE(slots > n ? slots : n);
@@ -333,6 +385,7 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   S_.ao = (double2*)ao;
   S_.cap_out = cap_out;
   S_.renorm = renorm;
+  S_.km = km;
   S_.hist_ready = hist_tiles > 0;
   S_.npsum = hist_tiles;
   S_.hist1 = at<unsigned int>(ws, L.hist1);

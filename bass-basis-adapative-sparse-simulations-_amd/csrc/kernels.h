@@ -6,7 +6,7 @@
 namespace bass {
 
 constexpr int HIST_BINS = 2048;  // 11-bit digits of the |a|^2 bit pattern
-enum { MODE_GATE = 0, MODE_ROT = 1 };
+enum { MODE_GATE = 0, MODE_ROT = 1, MODE_MOVE = 2 };
 
 struct GateMat {
   double2 m[16];  // row-major (2^QB x 2^QB), index (out, in) with in = (b_q0 << 1) | b_q1
@@ -51,6 +51,23 @@ constexpr int SEG_ENTRIES = 2048;
 int launch_diag2q(const uint64_t* x, double2* a, int64_t n, int q1, int q2, const double2* d,
                   hipStream_t s);
 
+// Key-ordered state: merge split of the run around one tile boundary.
+struct LiftBound {
+  int s0, sm, s1;  // run [s0, s1); its entries with the lifted bit set start at sm
+  int a, b;        // A / B cursors of the merge at the boundary
+  int pad0, pad1, pad2;
+};
+int lift_tile_entries(int qb, int mode);
+int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
+                uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
+                hipStream_t s);
+
+// Stored-bit -> qubit map of a key-ordered state (nbits = 0: keys are stored as is).
+struct KeyMap {
+  int nbits;
+  unsigned char q[64];
+};
+
 struct SelLaunch {
   const uint64_t* x;
   const double2* a;
@@ -62,6 +79,7 @@ struct SelLaunch {
   double2* ao;
   int64_t cap_out;
   int renorm;
+  KeyMap km;             // ties are broken by the logical key
   bool hist_ready;       // hist1 + tile_psum already hold this input's digit-1 data
   int64_t npsum;         // number of valid tile_psum entries when hist_ready
   unsigned int* hist1;   // zeroed by the caller unless hist_ready

@@ -1,0 +1,381 @@
+// Gate application on a key-ordered state (the simulator's internal layout).
+//
+// The simulator keeps its entries sorted by a stored word w: the basis key
+// with its bits permuted (the host records which qubit sits at which bit; the
+// bit VALUES are the key's).  Applying a gate to the qubit stored at bit j
+// "lifts" bit j down to bit t (t = 0 for a 1-qubit gate; t = 1 for a 2-qubit
+// gate, whose other qubit must already sit at bit 0):
+//
+//     T(w) = hi | mid << 1 | b << t | lo
+//     hi = bits above j,  b = bit j,  mid = bits t..j-1,  lo = bits below t.
+//
+// T keeps the order of entries that agree in bit j.  Entries sharing hi form
+// a run; inside a run the part with b = 0 (A) precedes the part with b = 1
+// (B), and the entries sorted by T are the merge of A and B.  After the merge
+// the members of every gate group (entries that differ only in bits 0..t of
+// T) are adjacent, so the gate is applied to consecutive entries and its
+// outputs come out sorted by T: the output is again an ordered state, with
+// the gated qubit(s) at the lowest bits.
+//
+// Kernels (TM merged positions per tile; runs coincide in input and output):
+//   k_lift_part : one wave per tile boundary d finds the run around d and the
+//                 merge split at d with 64-ary searches.
+//   k_lift_gate : per tile, loads its <= 3 input ranges (jointly sorted by w)
+//                 into LDS, ranks every entry in T order with two LDS binary
+//                 searches, scatters into merged order, applies the gate per
+//                 group and writes the tile's outputs to its own segment
+//                 [tile * SEG_ENTRIES, +count) with the digit-1 histogram and
+//                 the |a|^2 sum that the top-k select consumes.
+//   k_lift_move : the same permutation without a gate (dense output).
+//
+// Group semantics follow the hash-table kernels (gate_kernels.hip): a gate
+// output exists when one of its products is nonzero, products are summed in
+// input order (member index order here, which is input order because the
+// input is sorted); rotations sum every member (absent = 0) and drop outputs
+// with |a| <= 1e-30.
+#include "common.h"
+#include "kernels.h"
+
+namespace bass {
+
+__device__ __forceinline__ uint64_t lift_key(uint64_t w, int j, int t) {
+  const uint64_t lowmask = (1ull << t) - 1ull;
+  const uint64_t hi = w & ~((2ull << j) - 1ull);
+  const uint64_t b = (w >> j) & 1ull;
+  const uint64_t mid = w & ((1ull << j) - 1ull) & ~lowmask;
+  return hi | (mid << 1) | (b << t) | (w & lowmask);
+}
+
+// First index in [lo, hi) where the monotone (true ... false) predicate is
+// false, or hi.  Called by all 64 lanes of a wave with equal arguments.
+template <class P>
+__device__ __forceinline__ int64_t wave_partition_point(int64_t lo, int64_t hi, P pred) {
+  const int lane = threadIdx.x & 63;
+  while (hi - lo > 64) {
+    const int64_t step = (hi - lo + 63) / 64;
+    const int64_t p = lo + (int64_t)lane * step;
+    const bool t = p < hi && pred(p);
+    const int c = __popcll(__ballot(t));
+    if (c == 0) return lo;
+    const int64_t nlo = lo + (int64_t)(c - 1) * step + 1;
+    const int64_t nhi = lo + (int64_t)c * step;
+    lo = nlo;
+    hi = nhi < hi ? nhi : hi;
+  }
+  const int64_t p = lo + lane;
+  const bool t = p < hi && pred(p);
+  return lo + __popcll(__ballot(t));
+}
+
+__global__ __launch_bounds__(NT) void k_lift_part(const uint64_t* __restrict__ w, int64_t n, int j, int t,
+                                                   int tm, int64_t nbound, LiftBound* bounds) {
+  const int lane = threadIdx.x & 63;
+  const int64_t u = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (u >= nbound) return;  // uniform per wave
+  int64_t d = u * (int64_t)tm;
+  if (d > n) d = n;
+  LiftBound r;
+  if (d >= n) {
+    r.s0 = r.sm = r.s1 = r.a = r.b = (int)n;
+  } else {
+    const uint64_t bj = 1ull << j;
+    const uint64_t hi = w[d] & ~((bj << 1) - 1ull);
+    const int64_t s0 = wave_partition_point(0, d, [&](int64_t i) { return w[i] < hi; });
+    const int64_t s1 = wave_partition_point(d + 1, n, [&](int64_t i) { return w[i] < hi + (bj << 1); });
+    const int64_t sm = wave_partition_point(s0, s1, [&](int64_t i) { return w[i] < (hi | bj); });
+    const int64_t ld = d - s0, na = sm - s0, nb = s1 - sm;
+    const int64_t lo = ld - nb > 0 ? ld - nb : 0;
+    const int64_t hh = ld < na ? ld : na;
+    // number of A entries among the first ld merged entries of the run
+    const int64_t i = wave_partition_point(
+        lo, hh, [&](int64_t q) { return lift_key(w[s0 + q], j, t) < lift_key(w[sm + ld - 1 - q], j, t); });
+    r.s0 = (int)s0;
+    r.sm = (int)sm;
+    r.s1 = (int)s1;
+    r.a = (int)(s0 + i);
+    r.b = (int)(sm + ld - i);
+  }
+  r.pad0 = r.pad1 = r.pad2 = 0;
+  if (lane == 0) bounds[u] = r;
+}
+
+__device__ __forceinline__ int lds_lower_bound(const uint64_t* s, int cnt, uint64_t v) {
+  int lo = 0, len = cnt;
+  while (len > 0) {
+    const int half = len >> 1;
+    const bool go = s[lo + half] < v;
+    lo = go ? lo + half + 1 : lo;
+    len = go ? len - half - 1 : half;
+  }
+  return lo;
+}
+
+// Load the tile's entries (merged positions [tile*TM, +cnt)) and place them in
+// merged order: s_t = lifted keys, s_a = amplitudes.  s_w is scratch.
+// Returns cnt.  Ends with a barrier.
+template <int TM>
+__device__ __forceinline__ int lift_stage(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                          int64_t n, int j, int t, const LiftBound& B0, const LiftBound& B1,
+                                          uint64_t* s_w, uint64_t* s_t, double2* s_a) {
+  constexpr int ITEMS = TM / NT;
+  const int64_t d = (int64_t)blockIdx.x * TM;
+  const int cnt = (int)((n - d) < TM ? (n - d) : TM);
+  int64_t p1lo, p1n, p2lo, p2n, p3lo;
+  if (B0.s0 == B1.s0) {  // the tile lies inside one run
+    p1lo = B0.a;
+    p1n = B1.a - B0.a;
+    p2lo = B0.b;
+    p2n = B1.b - B0.b;
+    p3lo = 0;
+  } else {
+    p1lo = B0.a;
+    p1n = B0.sm - B0.a;
+    p2lo = B0.b;
+    p2n = B1.a - B0.b;
+    p3lo = B1.sm;
+  }
+  uint64_t kr[ITEMS];
+  double2 ar[ITEMS];
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int idx = it * NT + threadIdx.x;
+    if (idx < cnt) {
+      const int64_t g = idx < p1n ? p1lo + idx : (idx < p1n + p2n ? p2lo + (idx - p1n) : p3lo + (idx - p1n - p2n));
+      kr[it] = w[g];
+      ar[it] = a[g];
+      s_w[idx] = kr[it];
+    }
+  }
+  __syncthreads();
+  const uint64_t bj = 1ull << j;
+  const uint64_t lowmask = (1ull << t) - 1ull;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int idx = it * NT + threadIdx.x;
+    if (idx < cnt) {
+      const uint64_t x = kr[it];
+      const uint64_t hi = x & ~((bj << 1) - 1ull);
+      const int sp = lds_lower_bound(s_w, cnt, hi | bj);  // first B entry of the run
+      int rank;
+      if (!(x & bj)) {
+        // A entry: entries before it in w order + B entries of its run below it in T order
+        rank = idx + lds_lower_bound(s_w, cnt, (x | bj) & ~lowmask) - sp;
+      } else {
+        // B entry: every entry up to the A entries below it in T order + B entries before it
+        rank = lds_lower_bound(s_w, cnt, ((x & ~bj) | lowmask) + 1ull) + (idx - sp);
+      }
+      s_t[rank] = lift_key(x, j, t);
+      s_a[rank] = ar[it];
+    }
+  }
+  __syncthreads();
+  return cnt;
+}
+
+template <int QB, int MODE>
+__global__ __launch_bounds__(NT) void k_lift_gate(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                                   int64_t n, int j, const LiftBound* __restrict__ bounds,
+                                                   GateMat G, uint64_t* __restrict__ xo, double2* __restrict__ ao,
+                                                   unsigned int* hist, double* tile_psum, int* seg_cnt) {
+  constexpr int M = 1 << QB;
+  constexpr int T = QB - 1;
+  constexpr int TM = SEG_ENTRIES / M;
+  constexpr int ITEMS = TM / NT;
+  constexpr uint64_t NONE = ~0ull;
+  __shared__ uint64_t s_w[TM];
+  __shared__ uint64_t s_t[TM];
+  __shared__ double2 s_a[TM];
+  __shared__ unsigned int s_hist[HIST_BINS];
+  __shared__ int s_scan[WAVES];
+  __shared__ double s_red[WAVES];
+  __shared__ uint64_t s_halo_t[3];
+  __shared__ double2 s_halo_a[3];
+  __shared__ int s_nhalo;
+  __shared__ uint64_t s_prev[2];
+
+  const unsigned int tile = blockIdx.x;
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  const LiftBound B0 = bounds[tile], B1 = bounds[tile + 1];
+  const int cnt = lift_stage<TM>(w, a, n, j, T, B0, B1, s_w, s_t, s_a);
+
+  // group key: the lifted key without the gate bits 0..T
+  auto gk = [](uint64_t tk) { return tk >> (T + 1); };
+  if (threadIdx.x == 0) {
+    // group of the entry merged just before the tile (A or B tail of the run)
+    s_prev[0] = B0.a > B0.s0 ? gk(lift_key(w[B0.a - 1], j, T)) : NONE;
+    s_prev[1] = B0.b > B0.sm ? gk(lift_key(w[B0.b - 1], j, T)) : NONE;
+    // members of the tile's last group merged after the tile
+    int nh = 0;
+    const uint64_t glast = gk(s_t[cnt - 1]);
+    int64_t cand[3] = {B1.a < B1.sm ? (int64_t)B1.a : -1, B1.b < B1.s1 ? (int64_t)B1.b : -1,
+                       B1.b + 1 < B1.s1 ? (int64_t)B1.b + 1 : -1};
+    for (int c = 0; c < 3; ++c) {
+      if (cand[c] < 0) continue;
+      const uint64_t tk = lift_key(w[cand[c]], j, T);
+      if (gk(tk) == glast) {
+        s_halo_t[nh] = tk;
+        s_halo_a[nh] = a[cand[c]];
+        ++nh;
+      }
+    }
+    s_nhalo = nh;
+  }
+  __syncthreads();
+
+  uint64_t okey[ITEMS][M];
+  double2 oamp[ITEMS][M];
+  unsigned int omask[ITEMS];
+  int ooff[ITEMS];
+  int rtot[ITEMS];
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int m = it * NT + threadIdx.x;
+    unsigned int mask = 0;
+    if (m < cnt) {
+      const uint64_t tk = s_t[m];
+      const uint64_t g = gk(tk);
+      const bool head = m == 0 ? (g != s_prev[0] && g != s_prev[1]) : g != gk(s_t[m - 1]);
+      if (head) {
+        double2 av[M];
+        bool pres[M];
+#pragma unroll
+        for (int c = 0; c < M; ++c) {
+          av[c] = make_double2(0.0, 0.0);
+          pres[c] = false;
+        }
+        int k = m;
+#pragma unroll
+        for (int s = 0; s < M; ++s) {
+          if (k < cnt && gk(s_t[k]) == g) {
+            const int c = (int)(s_t[k] & (M - 1));
+            av[c] = s_a[k];
+            pres[c] = true;
+            ++k;
+          }
+        }
+        if (k == cnt) {
+          for (int h = 0; h < s_nhalo; ++h) {
+            const int c = (int)(s_halo_t[h] & (M - 1));
+            av[c] = s_halo_a[h];
+            pres[c] = true;
+          }
+        }
+        const uint64_t base = tk & ~(uint64_t)(M - 1);
+#pragma unroll
+        for (int o = 0; o < M; ++o) {
+          double2 acc = make_double2(0.0, 0.0);
+          bool any = false;
+          if constexpr (MODE == MODE_GATE) {
+#pragma unroll
+            for (int c = 0; c < M; ++c) {
+              const double2 term = cmul(G.m[o * M + c], av[c]);
+              if (pres[c] && cnonzero(term)) {
+                acc = any ? cadd(acc, term) : term;
+                any = true;
+              }
+            }
+          } else {
+            acc = cmul(G.m[o * M + 0], av[0]);
+#pragma unroll
+            for (int c = 1; c < M; ++c) acc = cadd(acc, cmul(G.m[o * M + c], av[c]));
+            any = hypot(acc.x, acc.y) > 1e-30;
+          }
+          if (any) {
+            mask |= 1u << o;
+            okey[it][o] = base | (uint64_t)o;
+            oamp[it][o] = acc;
+          }
+        }
+      }
+    }
+    omask[it] = mask;
+  }
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    int tot;
+    ooff[it] = block_excl_scan(__popc(omask[it]), tot, s_scan);
+    rtot[it] = tot;
+  }
+  int tile_total = 0;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) tile_total += rtot[it];
+  if (threadIdx.x == 0) seg_cnt[tile] = tile_total;
+
+  double psum = 0.0;
+  int64_t rb = (int64_t)tile * SEG_ENTRIES;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    int64_t pos = rb + ooff[it];
+#pragma unroll
+    for (int o = 0; o < M; ++o) {
+      if ((omask[it] >> o) & 1u) {
+        const double2 v = oamp[it][o];
+        xo[pos] = okey[it][o];
+        ao[pos] = v;
+        const double p = cabs2(v);
+        psum += p;
+        atomicAdd(&s_hist[(unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1)], 1u);
+        ++pos;
+      }
+    }
+    rb += rtot[it];
+  }
+  const double s = block_sum(psum, s_red);
+  if (threadIdx.x == 0) tile_psum[tile] = s;
+  __syncthreads();
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT)
+    if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
+}
+
+template <int TM>
+__global__ __launch_bounds__(NT) void k_lift_move(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                                   int64_t n, int j, const LiftBound* __restrict__ bounds,
+                                                   uint64_t* __restrict__ xo, double2* __restrict__ ao) {
+  __shared__ uint64_t s_w[TM];
+  __shared__ uint64_t s_t[TM];
+  __shared__ double2 s_a[TM];
+  const LiftBound B0 = bounds[blockIdx.x], B1 = bounds[blockIdx.x + 1];
+  const int cnt = lift_stage<TM>(w, a, n, j, 0, B0, B1, s_w, s_t, s_a);
+  const int64_t d = (int64_t)blockIdx.x * TM;
+  for (int m = threadIdx.x; m < cnt; m += NT) {
+    xo[d + m] = s_t[m];
+    ao[d + m] = s_a[m];
+  }
+}
+
+// ----------------------------------------------------------- host launch ---
+
+int lift_tile_entries(int qb, int mode) { return mode == MODE_MOVE ? 1024 : SEG_ENTRIES / (1 << qb); }
+
+int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
+                uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
+                hipStream_t s) {
+  if (n <= 0) return 0;
+  const int tm = lift_tile_entries(qb, mode);
+  const int t = (mode == MODE_MOVE) ? 0 : qb - 1;
+  const int64_t tiles = (n + tm - 1) / tm;
+  const int64_t nbound = tiles + 1;
+  hipLaunchKernelGGL(k_lift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES)), dim3(NT), 0, s, x, n, j, t, tm,
+                     nbound, bounds);
+  const dim3 grid((unsigned)tiles);
+  if (mode == MODE_MOVE) {
+    hipLaunchKernelGGL(k_lift_move<1024>, grid, dim3(NT), 0, s, x, a, n, j, bounds, xo, ao);
+  } else if (qb == 1) {
+    if (mode == MODE_GATE)
+      hipLaunchKernelGGL((k_lift_gate<1, MODE_GATE>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
+                         tile_psum, seg_cnt);
+    else
+      hipLaunchKernelGGL((k_lift_gate<1, MODE_ROT>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
+                         tile_psum, seg_cnt);
+  } else {
+    if (mode == MODE_GATE)
+      hipLaunchKernelGGL((k_lift_gate<2, MODE_GATE>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
+                         tile_psum, seg_cnt);
+    else
+      hipLaunchKernelGGL((k_lift_gate<2, MODE_ROT>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
+                         tile_psum, seg_cnt);
+  }
+  return (int)hipGetLastError();
+}
+
+}  // namespace bass

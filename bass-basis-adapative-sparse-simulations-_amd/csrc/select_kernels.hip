@@ -1,8 +1,10 @@
 // Exact top-k truncation by radix select on the IEEE bit pattern of |a|^2.
 //
-// Order: larger |a|^2 first; equal |a|^2 ordered by ascending key (the
-// reference leaves ties to np.argpartition, whose choice is implementation
+// Order: larger |a|^2 first; equal |a|^2 ordered by ascending logical key
+// (the reference leaves ties to np.argpartition, whose choice is implementation
 // defined; src/core/truncation.py:68-113, src/simulation/bass_simulator.py:899-916).
+// Entries of a key-ordered state store permuted keys; a KeyMap maps them back
+// for the tie-break.
 // |a|^2 >= 0, so its bit pattern orders like the value.
 //
 // Input is a sequence of tiles of at most SEG_ENTRIES entries: tile t holds
@@ -31,6 +33,14 @@ constexpr int SEL_TILE = SEG_ENTRIES;
 
 __device__ __forceinline__ unsigned long long pbits_of(double2 v) {
   return (unsigned long long)__double_as_longlong(cabs2(v));
+}
+
+// Logical key of a stored word (ties are ordered by it).
+__device__ __forceinline__ uint64_t logical_key(uint64_t w, const KeyMap& km) {
+  if (km.nbits == 0) return w;
+  uint64_t r = 0;
+  for (int p = 0; p < km.nbits; ++p) r |= ((w >> p) & 1ull) << km.q[p];
+  return r;
 }
 
 __device__ __forceinline__ int tile_count(const int* seg_cnt, int64_t n_dense, int64_t t) {
@@ -271,8 +281,8 @@ __global__ __launch_bounds__(NT) void k_extract2(const unsigned long long* __res
 // candidate list, then the key tie-break among entries equal to it.
 __global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restrict__ c2p,
                                                 const unsigned int* __restrict__ c2i,
-                                                const uint64_t* __restrict__ x, SelState* sel,
-                                                OpStats* st) {
+                                                const uint64_t* __restrict__ x, KeyMap km,
+                                                SelState* sel, OpStats* st) {
   __shared__ unsigned int s_hist[HIST_BINS];
   __shared__ int s_scan[WAVES];
   __shared__ unsigned int s_b;
@@ -322,7 +332,7 @@ __global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restr
       __syncthreads();
       for (unsigned int j = threadIdx.x; j < m2; j += NT) {
         if (c2p[j] != pstar) continue;
-        const uint64_t key = x[c2i[j]];
+        const uint64_t key = logical_key(x[c2i[j]], km);
         const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
         if (match) atomicAdd(&s_hist[(unsigned int)(key >> sh) & 255u], 1u);
       }
@@ -344,16 +354,16 @@ __global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restr
   }
 }
 
-__device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t key, unsigned long long pstar,
-                                          unsigned long long keystar) {
-  return bits > pstar || (bits == pstar && key <= keystar);
+__device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t w, unsigned long long pstar,
+                                          unsigned long long keystar, const KeyMap& km) {
+  return bits > pstar || (bits == pstar && logical_key(w, km) <= keystar);
 }
 
 // Per tile: sum and count of the kept candidates (in tile order).
 __global__ __launch_bounds__(NT) void k_keptsum(const unsigned long long* __restrict__ c1p,
                                                  const unsigned int* __restrict__ c1i,
                                                  const int* __restrict__ cnt1, const uint64_t* __restrict__ x,
-                                                 const SelState* sel, double* ksum, int* kcnt) {
+                                                 KeyMap km, const SelState* sel, double* ksum, int* kcnt) {
   __shared__ double s_red[WAVES];
   __shared__ int s_cnt;
   const unsigned long long pstar = sel->pstar, keystar = sel->keystar;
@@ -367,7 +377,7 @@ __global__ __launch_bounds__(NT) void k_keptsum(const unsigned long long* __rest
   for (int j = threadIdx.x; j < cnt; j += NT) {
     const unsigned long long bits = c1p[base + j];
     bool keep = bits > pstar;
-    if (bits == pstar) keep = x[c1i[base + j]] <= keystar;
+    if (bits == pstar) keep = logical_key(x[c1i[base + j]], km) <= keystar;
     if (keep) {
       s += __longlong_as_double((long long)bits);
       ++c;
@@ -411,7 +421,7 @@ __global__ __launch_bounds__(NT) void k_sel_finalize(const double* tile_dsum, co
 }
 
 __global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x, const double2* __restrict__ a,
-                                                 const int* seg_cnt, int64_t n_dense, const SelState* sel,
+                                                 const int* seg_cnt, int64_t n_dense, KeyMap km, const SelState* sel,
                                                  const int64_t* __restrict__ koff, uint64_t* __restrict__ xo,
                                                  double2* __restrict__ ao, int64_t cap_out, const OpStats* st,
                                                  double* tile_s2, double* tile_s4, int* err) {
@@ -433,7 +443,7 @@ __global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x, 
     if (j < cnt) {
       kk[it] = x[base + j];
       vv[it] = a[base + j];
-      keep[it] = kept_pred(pbits_of(vv[it]), kk[it], pstar, keystar);
+      keep[it] = kept_pred(pbits_of(vv[it]), kk[it], pstar, keystar, km);
     }
   }
   int64_t rb = koff[tile];
@@ -570,12 +580,12 @@ int launch_select(const SelLaunch& L, hipStream_t s) {
   int64_t eb = tiles < 1024 ? tiles : 1024;
   hipLaunchKernelGGL(k_extract2, dim3((unsigned)eb), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, tiles, L.sel, L.c2p,
                      L.c2i);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(NT), 0, s, L.c2p, L.c2i, L.x, L.sel, L.st);
-  hipLaunchKernelGGL(k_keptsum, dim3((unsigned)tiles), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, L.x, L.sel, L.ksum,
-                     L.kcnt);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(NT), 0, s, L.c2p, L.c2i, L.x, L.km, L.sel, L.st);
+  hipLaunchKernelGGL(k_keptsum, dim3((unsigned)tiles), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, L.x, L.km, L.sel,
+                     L.ksum, L.kcnt);
   hipLaunchKernelGGL(k_sel_finalize, dim3(1), dim3(NT), 0, s, L.tile_dsum, L.ksum, L.tile_dcnt, L.kcnt, tiles,
                      L.k, L.renorm, L.koff, L.st);
-  hipLaunchKernelGGL(k_compact, dim3((unsigned)tiles), dim3(NT), 0, s, L.x, L.a, L.seg_cnt, L.n, L.sel, L.koff,
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)tiles), dim3(NT), 0, s, L.x, L.a, L.seg_cnt, L.n, L.km, L.sel, L.koff,
                      L.xo, L.ao, L.cap_out, L.st, L.tile_s2, L.tile_s4, &L.st->err);
   hipLaunchKernelGGL(k_sums_finalize, dim3(1), dim3(NT), 0, s, L.tile_s2, L.tile_s4, tiles, L.st);
   return (int)hipGetLastError();

@@ -11,7 +11,13 @@ Differences from the reference, all results-preserving:
     np.argpartition, whose choice is implementation defined);
   * the optimiser double-buffers the tentative rotated state instead of saving
     and restoring copies (:767-770, :798-803);
-  * the hash tables use a multiplicative hash instead of key & (cap-1).
+  * the hash tables use a multiplicative hash instead of key & (cap-1);
+  * the state is kept sorted by a stored word, the key with its bits permuted
+    (qubit q at bit state.pos[q]).  A gate on the qubit at bit j merges the
+    entries with bit j clear and set (bass_lift), which moves that qubit to
+    bit 0 and makes every gate group adjacent: no hash table, no sort, and
+    the output is sorted again.  The truncations preserve order.  Host views
+    (state.x, to_statevector) map stored words back to keys.
 """
 
 import math
@@ -24,6 +30,10 @@ from .. import _ops
 from .._device import Buffers, context, to_host_amps, to_host_keys
 from ..core.gates import SingleQubitGate, TwoQubitGate  # noqa: F401  (reference surface)
 from ..core.sparse_state import SparseState, _pair, apply_2qubit_diagonal_kernel  # noqa: F401
+from ..core.sparse_state import lifted_positions
+
+# 2-qubit gate with its qubits exchanged: G'[i, k] = G[s(i), s(k)], s swaps the index bits
+_SWAP_IDX = np.array([0, 2, 1, 3])
 
 EMPTY = np.uint64(0xFFFFFFFFFFFFFFFF)
 
@@ -226,13 +236,19 @@ class BASS:
     Constructor keywords are the reference's.  Extensions: ``device``;
     ``optimize_at`` -- optional iterable of 0-based gate indices after which the
     optimiser check runs, replacing ``(i + 1) % optimize_every == 0`` (used to
-    run it at layer boundaries of circuits whose layers differ in length).
+    run it at layer boundaries of circuits whose layers differ in length);
+    ``trace`` -- optional list receiving the decision log ("rot" / "trunc"
+    records); ``accept_rtol`` -- a basis rotation is accepted only when it
+    lowers PR by more than this fraction (the reference accepts any decrease,
+    :785-795; rotations that merely relabel the state leave PR unchanged up to
+    rounding, and the reference's choice there depends on summation order, so
+    they are rejected deterministically; 0 restores the literal rule).
     """
 
     def __init__(self, num_qubits, k, optimize_every=5, buffer_factor=2, truncate_every=1,
                  max_nnz_factor=8, use_2qubit_rotations=False, verbose=False, use_fast_eigh=True,
                  use_diag_gate=True, use_kron_cache=True, pr_opt_threshold=0.90, device=None,
-                 optimize_at=None):
+                 optimize_at=None, trace=None, accept_rtol=1e-10):
         self.N = num_qubits
         self.k = k
         self.optimize_every = optimize_every
@@ -252,6 +268,8 @@ class BASS:
         self.optimize_at = None if optimize_at is None else frozenset(int(i) for i in optimize_at)
         self._ctx = context(device)
         self.counters = {}
+        self.trace = trace  # optional list: decision log, same records as the oracle's
+        self.accept_rtol = float(accept_rtol)
 
     # helpers ----------------------------------------------------------------
     def _count(self, key, v=1):
@@ -305,7 +323,7 @@ class BASS:
         if self.N > 24:
             raise ValueError(f"N={self.N} too large for statevector")
         psi = np.zeros(2 ** self.N, dtype=np.complex128)
-        psi[to_host_keys(state._buf.x, state.nnz).astype(np.int64)] = to_host_amps(state._buf.a, state.nnz)
+        psi[state.keys().astype(np.int64)] = to_host_amps(state._buf.a, state.nnz)
         for j in range(self.N):
             psi = np.einsum("ij,kjl->kil", self.U[j], psi.reshape(-1, 2, 1 << j)).reshape(-1)
         return psi
@@ -322,7 +340,7 @@ class BASS:
         ctx = self._ctx
         cand = state._tmp
         for _pass in range(3):
-            all_rho = _ops.rdm1q(ctx, state._buf, state.nnz, self.N)
+            all_rho = _ops.rdm1q(ctx, state._buf, state.nnz, self.N)[state.pos]
             self._count("rdm")
             pr = self._pr(state)
             improved = False
@@ -339,13 +357,15 @@ class BASS:
                     continue
                 # tentative: rotate -> truncate to k -> renormalise -> PR
                 rot = state._rot
-                m, rht = _ops.gate(ctx, state._buf, state.nnz, V.conj().T, (j,), rot, mode=L.MODE_ROT,
-                                   dedup=True, segmented=True)
+                jb = int(state.pos[j])
+                m, rht = _ops.lift(ctx, state._buf, state.nnz, jb, rot, G=V.conj().T, nq=1, mode=L.MODE_ROT)
+                trial_pos = lifted_positions(state.pos, jb, 0)
                 self._count("rotate")
                 self._count("amp_updates", m)
                 self._bytes("rot1q", state.nnz, m)
                 if m > self.k:
-                    st = _ops.truncate(ctx, rot, m, self.k, cand, renorm=True, hist_tiles=rht, segmented=True)
+                    st = _ops.truncate(ctx, rot, m, self.k, cand, renorm=True, hist_tiles=rht, segmented=True,
+                                       key_map=_ops.key_map_arg(trial_pos))
                     self._bytes("select", m, self.k)
                     nn, frac, s4 = self.k, st.kept, st.sum4
                 else:
@@ -358,13 +378,17 @@ class BASS:
                     _, s4 = _ops.moments(ctx, cand, m)
                     nn = m
                 pr_new = 1.0 / s4 if s4 > 1e-60 else float("inf")
-                if pr_new >= pr:
+                reject = pr_new >= pr * (1.0 - self.accept_rtol)
+                if self.trace is not None:
+                    self.trace.append(("rot", j, not reject, pr, pr_new, frac))
+                if reject:
                     self._count("reject")
                     continue
                 # accept: the tentative buffer becomes the state
                 state._buf, state._tmp = cand, state._buf
                 cand = state._tmp
                 state.nnz = nn
+                state.pos = trial_pos
                 state._sum2, state._sum4 = None, s4
                 state._hist_tiles = 0
                 self.U[j] = self.U[j] @ V
@@ -385,7 +409,7 @@ class BASS:
         if state.nnz <= self.k:
             return
         st = _ops.truncate(self._ctx, state._buf, state.nnz, self.k, state._tmp, renorm=False,
-                           hist_tiles=state._hist_tiles)
+                           hist_tiles=state._hist_tiles, key_map=_ops.key_map_arg(state.pos))
         state._buf, state._tmp = state._tmp, state._buf
         state.nnz = self.k
         state._sum2, state._sum4 = st.sum2, st.sum4
@@ -419,6 +443,7 @@ class BASS:
         internal_k = min(max(max(self.buffer_factor, 2) * self.k, self._hard_cap), 2 ** self.N)
         state = SparseState(self.N, internal_k, initial_state, device=self._ctx.device)
         state._rot = Buffers(2 * self.k + 2, self._ctx.device)
+        state.pos = np.arange(self.N)
         state._sum2 = state._sum4 = None
         self._M = int(num_gates)
         self._gamma_running = 1.0
@@ -436,14 +461,16 @@ class BASS:
         if gate.n_qubits == 1:
             q = gate.qubits[0]
             G = self.U[q].conj().T @ gate.matrix @ self.U[q]
-            n, ht = _ops.gate(ctx, state._buf, state.nnz, G, (q,), state._tmp, dedup=True, segmented=True)
+            jb = int(state.pos[q])
+            n, ht = _ops.lift(ctx, state._buf, state.nnz, jb, state._tmp, G=G, nq=1)
+            state.pos = lifted_positions(state.pos, jb, 0)
             self._bytes("gate1q", state.nnz, n)
             seg = (n, ht)
             state.nnz, state._hist_tiles, state._sum4 = n, 0, None
             self._count("amp_updates", n)
             if state.nnz > state.k:  # SparseState's own budget (:297-301)
                 st = _ops.truncate(ctx, state._tmp, n, state.k, state._buf, renorm=True, hist_tiles=ht,
-                                   segmented=True)
+                                   segmented=True, key_map=_ops.key_map_arg(state.pos))
                 state.gamma *= np.sqrt(st.kept)
                 state.nnz = state.k
                 seg = None
@@ -454,7 +481,8 @@ class BASS:
             Ut = self._get_kron(q1, q2)
             G = Ut.conj().T @ gate.matrix @ Ut
             if self.use_diag_gate and _is_gate_diagonal(G):
-                _ops.diag2q(ctx, state._buf, state.nnz, q1, q2, np.diag(G).astype(np.complex128))
+                _ops.diag2q(ctx, state._buf, state.nnz, int(state.pos[q1]), int(state.pos[q2]),
+                            np.diag(G).astype(np.complex128))
                 self._bytes("diag2q", state.nnz, state.nnz)
                 state._sum4 = None
                 state._hist_tiles = 0
@@ -466,7 +494,7 @@ class BASS:
                         self._pr_at_last_opt = self._pr(state)
                     self._trunc_active = False
                 return
-            n, ht = _ops.gate(ctx, state._buf, state.nnz, G, (q1, q2), state._tmp, dedup=True, segmented=True)
+            n, ht = self._lift_2q(state, G, q1, q2)
             self._bytes("gate2q", state.nnz, n)
             seg = (n, ht)
             state.nnz, state._hist_tiles, state._sum4 = n, 0, None
@@ -482,15 +510,17 @@ class BASS:
             self._trunc_active = True
             if seg is not None:
                 st = _ops.truncate(ctx, state._tmp, seg[0], self.k, state._buf, renorm=True, hist_tiles=seg[1],
-                                   segmented=True)
+                                   segmented=True, key_map=_ops.key_map_arg(state.pos))
                 seg = None
             else:
                 st = _ops.truncate(ctx, state._buf, state.nnz, self.k, state._tmp, renorm=True,
-                                   hist_tiles=state._hist_tiles)
+                                   hist_tiles=state._hist_tiles, key_map=_ops.key_map_arg(state.pos))
                 state._buf, state._tmp = state._tmp, state._buf
             self._bytes("select", state.nnz, self.k)
             total, kept = st.total, st.kept
             self._gamma_running *= kept / total if total > 1e-30 else 0
+            if self.trace is not None:
+                self.trace.append(("trunc", i, float(kept), float(total)))
             state.nnz = self.k
             state._sum2, state._sum4 = st.sum2, st.sum4
             state._hist_tiles = 0
@@ -504,6 +534,25 @@ class BASS:
                 self._optimize_basis(state)
                 self._pr_at_last_opt = self._pr(state)
             self._trunc_active = False
+
+    def _lift_2q(self, state, G, q1, q2):
+        """2-qubit gate on the ordered state: bring q2 to bit 0 (a move, unless it
+        is there already or q1 is), then lift q1's bit to bit 1 with the gate.
+        Returns (n_out, seg_tiles) of the segmented output in state._tmp."""
+        ctx = self._ctx
+        if state.pos[q1] == 0 and state.pos[q2] != 0:
+            G = G[np.ix_(_SWAP_IDX, _SWAP_IDX)]
+            q1, q2 = q2, q1
+        if state.pos[q2] != 0:
+            jb = int(state.pos[q2])
+            _ops.lift(ctx, state._buf, state.nnz, jb, state._tmp, mode=L.MODE_MOVE)
+            state._buf, state._tmp = state._tmp, state._buf
+            state.pos = lifted_positions(state.pos, jb, 0)
+            self._bytes("move", state.nnz, state.nnz)
+        jb = int(state.pos[q1])
+        n, ht = _ops.lift(ctx, state._buf, state.nnz, jb, state._tmp, G=G, nq=2)
+        state.pos = lifted_positions(state.pos, jb, 1)
+        return n, ht
 
     def end(self):
         state = self.state

@@ -127,7 +127,7 @@ def main():
     value = amp / dt
 
     kernels = {}
-    for op in ["gate1q", "gate2q", "rot1q", "select", "diag2q", "rdm"]:
+    for op in ["gate1q", "gate2q", "rot1q", "move", "select", "diag2q", "rdm"]:
         cnt, ms = lib.timing_read(op)
         if cnt == 0:
             continue
@@ -148,7 +148,7 @@ def main():
     if not args.no_cpu:
         dev = importlib.import_module(PKG + "._device")
         st = sim.state
-        keys = dev.to_host_keys(st._buf.x, st.nnz)
+        keys = st.keys()
         amps = dev.to_host_amps(st._buf.a, st.nnz)
         gates = [g for g in circ if g.n_qubits == 1][: 64]
         # the sample applies the frame-rotated gates the driver would see

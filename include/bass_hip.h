@@ -91,6 +91,26 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
 int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t* xo, double* ao,
                  int64_t cap_out, void* ws, size_t ws_bytes, void* stream);
 
+/* Key-ordered gate application (the simulator's internal layout).  (x, a) is
+ * sorted by stored word w, a bit permutation of the basis key that the caller
+ * tracks.  The call moves bit j of w down to bit nq-1 -- bits nq-1..j-1 move
+ * up by one -- and applies the gate to the groups this makes adjacent:
+ *   mode=0 (gate) / 1 (rotation), nq=1: gate on the qubit stored at bit j;
+ *   nq=2: gate on the qubits stored at bit j (first matrix index bit) and at
+ *         bit 0 (second), j >= 1;
+ *   mode=2, nq=1: the bit move alone (G unused), dense output of n entries.
+ * Gate and rotation outputs are written per tile as in bass_gate(segmented=1)
+ * (*seg_tiles receives the tile count for bass_truncate / bass_densify) and
+ * come out sorted by the new stored word.  Output entries and amplitudes
+ * equal those of bass_gate(dedup=1) on the same state with keys mapped back;
+ * a group's products are summed in stored order.  Replaces, inside the
+ * driver, the gate step of src/simulation/bass_simulator.py:861-896 and the
+ * tentative rotation of :768-783 (no reference counterpart as an entry point).
+ * Returns BASS_E_OVERFLOW if cap_out is below the output size. */
+int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G,
+              int mode, uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws,
+              size_t ws_bytes, void* stream);
+
 /* Top-k truncation: keep the k entries of largest |a|^2 (ties: smaller key),
  * compacted in input order into (xo, ao); renorm=1 divides them by
  * sqrt(sum kept |a|^2).  Replaces top_k_truncate_kernel_opt
@@ -98,10 +118,12 @@ int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t
  * driver truncations of src/simulation/bass_simulator.py:565-572, :899-916 and
  * src/simulation/simulator.py:169-201 (renorm=1).  Requires n > k.
  * hist_tiles: value reported by the bass_gate call that produced (x, a), or 0;
- * segmented=1: (x, a) is that call's segmented output (n = stats.nnz). */
+ * segmented=1: (x, a) is that call's segmented output (n = stats.nnz).
+ * key_map (key_bits entries, or NULL): stored bit p of x holds qubit
+ * key_map[p]; ties are then ordered by the key mapped back. */
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo,
                   double* ao, int64_t cap_out, int renorm, int64_t hist_tiles, int segmented,
-                  void* ws, size_t ws_bytes, void* stream);
+                  const int32_t* key_map, int key_bits, void* ws, size_t ws_bytes, void* stream);
 
 /* In place: a[i] *= diag[(b_q1 << 1) | b_q2]; diag = 4 complex (8 doubles).
  * Replaces apply_2qubit_diagonal_kernel (src/core/sparse_state.py:178-190). */

@@ -7,7 +7,11 @@
 Oracle extension: every top-k selection goes through ``select_topk``, whose
 default is the reference's ``np.argpartition(probs, -k)[-k:]``.  A ``selector``
 callable ``(keys, probs, k) -> indices`` may replace it (for instance the
-"largest |a|^2, ties to the smaller key" rule).  Nothing else differs.
+"largest |a|^2, ties to the smaller key" rule).  ``BASS(accept_rtol=r)``
+accepts a basis rotation only when it lowers PR by more than r relative
+(default 0: the reference's ``not (pr_new >= pr)``); r > 0 makes rotations that
+leave PR unchanged up to rounding -- relabelings -- a deterministic reject
+instead of a summation-order coin flip.  Nothing else differs.
 """
 
 import numpy as np
@@ -161,7 +165,7 @@ class BASS:
     def __init__(self, num_qubits, k, optimize_every=5, buffer_factor=2, truncate_every=1,
                  max_nnz_factor=8, use_2qubit_rotations=False, verbose=False,
                  use_fast_eigh=True, use_diag_gate=True, use_kron_cache=True,
-                 pr_opt_threshold=0.90, selector=None, optimize_at=None, trace=None):
+                 pr_opt_threshold=0.90, selector=None, optimize_at=None, trace=None, accept_rtol=0.0):
         self.N = num_qubits
         self.k = k
         self.optimize_every = optimize_every
@@ -183,6 +187,7 @@ class BASS:
         # which the optimiser check runs, instead of (i+1) % optimize_every == 0
         self.optimize_at = None if optimize_at is None else set(int(i) for i in optimize_at)
         self.trace = trace  # optional list: decision log
+        self.accept_rtol = float(accept_rtol)
 
     # -- helpers (:551-593) -------------------------------------------------
     def _renorm(self, state):
@@ -328,7 +333,7 @@ class BASS:
                 frac = float(np.sum(np.abs(state.alpha[: state.nnz]) ** 2))
                 self._renorm(state)
                 pr_new = self._pr(state)
-                accept = not (pr_new >= pr)
+                accept = not (pr_new >= pr * (1.0 - self.accept_rtol))
                 if self.trace is not None:
                     self.trace.append(("rot", j, bool(accept), pr, pr_new, frac))
                 if not accept:

@@ -65,12 +65,21 @@ def test_argument_errors_without_gpu(lib):
     assert rc == L.BASS_E_ARG and "rotation" in L.last_error()
     rc = lib.bass_gate(None, None, 10, 1, 0, 0, G, 0, 1, 1, None, None, 40, ctypes.byref(ht), None, 0, None)
     assert rc == L.BASS_E_OVERFLOW and "segmented" in L.last_error()
-    rc = lib.bass_truncate(None, None, 5, 5, None, None, 5, 1, 0, 0, None, 0, None)
+    rc = lib.bass_truncate(None, None, 5, 5, None, None, 5, 1, 0, 0, None, 0, None, 0, None)
     assert rc == L.BASS_E_ARG
-    rc = lib.bass_truncate(None, None, 10, 5, None, None, 4, 1, 0, 0, None, 0, None)
+    rc = lib.bass_truncate(None, None, 10, 5, None, None, 4, 1, 0, 0, None, 0, None, 0, None)
     assert rc == L.BASS_E_OVERFLOW
-    rc = lib.bass_truncate(None, None, 10, 5, None, None, 5, 1, 0, 1, None, 0, None)
+    rc = lib.bass_truncate(None, None, 10, 5, None, None, 5, 1, 0, 1, None, 0, None, 0, None)
     assert rc == L.BASS_E_ARG
+    bad = (ctypes.c_int32 * 3)(0, 0, 2)
+    rc = lib.bass_truncate(None, None, 10, 5, None, None, 5, 1, 0, 0, bad, 3, None, 0, None)
+    assert rc == L.BASS_E_ARG and "permutation" in L.last_error()
+    rc = lib.bass_lift(None, None, 10, 2, 0, G, 0, None, None, 40, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_ARG and "bit" in L.last_error()
+    rc = lib.bass_lift(None, None, 10, 2, 3, G, 2, None, None, 40, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_ARG and "move" in L.last_error()
+    rc = lib.bass_lift(None, None, 3000, 1, 3, G, 0, None, None, 2048, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_OVERFLOW
     rc = lib.bass_radix_sort(None, None, 10, 0, None, 0, None)
     assert rc == L.BASS_E_ARG
     rc = lib.bass_gate(None, None, 10, 1, 0, 0, G, 0, 1, 0, None, None, 40, ctypes.byref(ht), None, 0, None)

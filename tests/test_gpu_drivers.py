@@ -41,10 +41,16 @@ def test_bass_end_to_end(gpu_ctx, idx, K):
     kw = dict(optimize_every=5, buffer_factor=1)
     if ends is not None:
         kw = dict(optimize_at=ends)
-    sim = B.BASS(N, K, device="cuda:0", **kw)
+    td, to = [], []
+    sim = B.BASS(N, K, device="cuda:0", trace=td, **kw)
     st = sim.simulate(circ)
-    ref = OD.BASS(N, K, selector=OD.key_tiebreak_selector, **kw)
+    ref = OD.BASS(N, K, selector=OD.key_tiebreak_selector, trace=to, accept_rtol=sim.accept_rtol, **kw)
     rs = ref.simulate(circ)
+    # same decisions in the same order: truncation steps, rotations tried and accepted
+    def decisions(tr):
+        return [(e[0], e[1], e[2] if e[0] == "rot" else None) for e in tr]
+
+    assert decisions(td) == decisions(to), "decision logs differ"
     sv = OE.exact_statevector(N, circ)
     f_dev = OE.fidelity_bass(N, st.x[: st.nnz], st.alpha[: st.nnz], sim.U, sv)
     f_ora = OE.fidelity_bass(N, rs.x[: rs.nnz], rs.alpha[: rs.nnz], ref.U, sv)

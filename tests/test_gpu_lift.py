@@ -1,0 +1,173 @@
+"""Key-ordered gates (bass_lift) and the key-map tie-break of bass_truncate
+against the oracle, through the C ABI.
+
+States are given a random qubit -> bit placement, stored as permuted words
+sorted ascending, and the device results are mapped back to keys."""
+
+import numpy as np
+import pytest
+import torch
+
+from _parity import assert_same_entries, numpy_uses_fma_cmul
+from conftest import pkg_module, random_state, random_unitary
+from oracle import drivers as OD
+from oracle import kernels as OK
+
+pytestmark = pytest.mark.gpu
+
+
+def _mods():
+    return pkg_module("_ops"), pkg_module("_device"), pkg_module("core.sparse_state"), pkg_module("_lib")
+
+
+def _to_stored(x, pos):
+    out = np.zeros_like(x)
+    for q, p in enumerate(pos):
+        out |= ((x >> np.uint64(q)) & np.uint64(1)) << np.uint64(p)
+    return out
+
+
+def _ordered(x, a, pos):
+    w = _to_stored(x, pos)
+    o = np.argsort(w)
+    return w[o], a[o]
+
+
+def _upload(dev, ctx, w, a):
+    n = len(w)
+    b = dev.Buffers(max(n, 1), ctx.device)
+    if n:
+        b.x[:n] = torch.from_numpy(w.view(np.int64)).to(ctx.device)
+        b.a[:n] = torch.from_numpy(a).to(ctx.device)
+    return b
+
+
+def _gate_with_zero(rng, d):
+    G = random_unitary(d, rng)
+    G[0, d - 1] = 0.0  # exercise the exact-zero emission rule
+    return G
+
+
+def _oracle_1q(x, a, G, q):
+    n = len(x)
+    ox, oa = np.zeros(2 * n, np.uint64), np.zeros(2 * n, np.complex128)
+    m = OK.apply_1qubit_gate_kernel_opt(x, a, n, ox, oa, G, q)
+    OK.radix_sort_uint64(ox, oa, m)
+    m = OK.merge_sorted_kernel(ox, oa, m)
+    return ox[:m], oa[:m]
+
+
+def _run_lift(ctx, w, a, j, G, nq, mode):
+    ops, dev, _, L = _mods()
+    n = len(w)
+    src = _upload(dev, ctx, w, a)
+    seg = dev.Buffers(1, ctx.device)
+    m, tiles = ops.lift(ctx, src, n, j, seg, G=G, nq=nq, mode=mode)
+    out = dev.Buffers(max(m, 1), ctx.device)
+    if m:
+        ops.densify(ctx, seg, tiles, m, out)
+    return dev.to_host_keys(out.x, m), dev.to_host_amps(out.a, m)
+
+
+@pytest.mark.parametrize("n,nbits,mode", [(1, 5, 0), (2, 3, 0), (777, 12, 0), (30000, 18, 0), (70000, 24, 0),
+                                          (5000, 14, 1), (40000, 20, 1)])
+def test_lift_1q(gpu_ctx, n, nbits, mode):
+    _, _, SS, L = _mods()
+    rng = np.random.default_rng(n + nbits + 100 * mode)
+    x, a = random_state(n, nbits, seed=n + 11)
+    pos = rng.permutation(nbits)
+    for q in sorted({0, nbits - 1, int(rng.integers(nbits))}):
+        print(f"[case] lift1q n={n} nbits={nbits} mode={mode} q={q} bit={pos[q]}", flush=True)
+        w, aw = _ordered(x, a, pos)
+        G = _gate_with_zero(rng, 2) if mode == 0 else random_unitary(2, rng)
+        gw, ga = _run_lift(gpu_ctx, w, aw, int(pos[q]), G, 1, mode)
+        assert np.all(gw[1:] > gw[:-1]), "output not sorted by stored word"
+        npos = SS.lifted_positions(pos, int(pos[q]), 0)
+        assert npos[q] == 0
+        gx = SS.stored_to_keys(gw, npos)
+        if mode == 0:
+            ox, oa = _oracle_1q(x, a, G, q)
+        else:
+            ox, oa = np.zeros(2 * n, np.uint64), np.zeros(2 * n, np.complex128)
+            ox[:n], oa[:n] = x, a
+            m = OK.apply_1qubit_rotation_sparse(ox, oa, n, q, G)
+            ox, oa = ox[:m], oa[:m]
+        # one or two products per output: the sum does not depend on order
+        assert_same_entries(gx, ga, ox, oa, exact=numpy_uses_fma_cmul())
+
+
+@pytest.mark.parametrize("n,nbits,mode", [(3, 3, 0), (5000, 13, 0), (40000, 18, 0), (6000, 14, 1)])
+def test_lift_2q(gpu_ctx, n, nbits, mode):
+    ops, dev, SS, L = _mods()
+    rng = np.random.default_rng(n + 7 * mode)
+    x, a = random_state(n, nbits, seed=n + 7)
+    pos = rng.permutation(nbits)
+    q1, q2 = (int(v) for v in rng.choice(nbits, size=2, replace=False))
+    print(f"[case] lift2q n={n} nbits={nbits} mode={mode} q=({q1},{q2}) bits=({pos[q1]},{pos[q2]})", flush=True)
+    w, aw = _ordered(x, a, pos)
+    # bring q2 to bit 0 with a move, then lift q1's bit to bit 1 with the gate
+    src = _upload(dev, gpu_ctx, w, aw)
+    mv = dev.Buffers(n, gpu_ctx.device)
+    ops.lift(gpu_ctx, src, n, int(pos[q2]), mv, mode=L.MODE_MOVE)
+    p1 = SS.lifted_positions(pos, int(pos[q2]), 0)
+    mw = dev.to_host_keys(mv.x, n)
+    assert np.all(mw[1:] > mw[:-1]), "move output not sorted"
+    assert np.array_equal(np.sort(SS.stored_to_keys(mw, p1)), np.sort(x)), "move lost entries"
+    G = _gate_with_zero(rng, 4) if mode == 0 else random_unitary(4, rng)
+    gw, ga = _run_lift(gpu_ctx, mw, dev.to_host_amps(mv.a, n), int(p1[q1]), G, 2, mode)
+    assert np.all(gw[1:] > gw[:-1]), "output not sorted by stored word"
+    p2 = SS.lifted_positions(p1, int(p1[q1]), 1)
+    assert p2[q1] == 1 and p2[q2] == 0
+    gx = SS.stored_to_keys(gw, p2)
+    ox, oa = np.zeros(4 * n, np.uint64), np.zeros(4 * n, np.complex128)
+    if mode == 0:
+        m = OK.apply_2qubit_gate_ht(x, a, n, ox, oa, G, q1, q2)
+    else:
+        ox[:n], oa[:n] = x, a
+        m = OK.apply_2qubit_rotation_sparse(ox, oa, n, q1, q2, G)
+    # rotations sum in matrix order (bitwise); gate products are summed in
+    # stored order, which may differ from the oracle's input order
+    assert_same_entries(gx, ga, ox[:m], oa[:m], rtol=1e-13, exact=(mode == 1 and numpy_uses_fma_cmul()))
+
+
+@pytest.mark.parametrize("n,nbits", [(1, 4), (5, 3), (3000, 13), (50000, 30)])
+def test_lift_move(gpu_ctx, n, nbits):
+    ops, dev, SS, L = _mods()
+    rng = np.random.default_rng(n)
+    x, a = random_state(n, nbits, seed=n + 5)
+    pos = rng.permutation(nbits)
+    w, aw = _ordered(x, a, pos)
+    for j in sorted({0, nbits - 1, int(rng.integers(nbits))}):
+        print(f"[case] move n={n} nbits={nbits} bit={j}", flush=True)
+        src = _upload(dev, gpu_ctx, w, aw)
+        out = dev.Buffers(n, gpu_ctx.device)
+        ops.lift(gpu_ctx, src, n, j, out, mode=L.MODE_MOVE)
+        npos = SS.lifted_positions(pos, j, 0)
+        exp_w = _to_stored(x, npos)
+        o = np.argsort(exp_w)
+        assert np.array_equal(dev.to_host_keys(out.x, n), exp_w[o])
+        assert np.array_equal(dev.to_host_amps(out.a, n), a[o])
+
+
+@pytest.mark.parametrize("n,k", [(4096, 1000), (30000, 20000)])
+def test_truncate_key_map_ties(gpu_ctx, n, k):
+    """Top-k of an ordered state: ties broken by the key, not the stored word."""
+    ops, dev, SS, L = _mods()
+    print(f"[case] truncate key map n={n} k={k}", flush=True)
+    nbits = 24
+    rng = np.random.default_rng(k)
+    x, a = random_state(n, nbits, seed=n + k, ties=True)
+    a[: n // 2] = a[0]  # a large block of exactly equal |a|^2 around the threshold
+    pos = rng.permutation(nbits)
+    w, aw = _ordered(x, a, pos)
+    src = _upload(dev, gpu_ctx, w, aw)
+    dst = dev.Buffers(k, gpu_ctx.device)
+    st = ops.truncate(gpu_ctx, src, n, k, dst, renorm=False, key_map=ops.key_map_arg(pos))
+    gw = dev.to_host_keys(dst.x, k)
+    assert np.all(gw[1:] > gw[:-1]), "truncation must preserve order"
+    gx = SS.stored_to_keys(gw, pos)
+    ox, oa = x.copy(), a.copy()
+    OK.top_k_truncate_kernel_opt(ox, oa, n, k, selector=OD.key_tiebreak_selector)
+    assert set(gx.tolist()) == set(ox[:k].tolist())
+    assert_same_entries(gx, dev.to_host_amps(dst.a, k), ox[:k], oa[:k], exact=True)
+    assert st.kept > 0
