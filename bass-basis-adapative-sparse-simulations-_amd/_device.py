@@ -48,16 +48,18 @@ class Context:
         rc = self.lib.bass_read_stats(self.wsp, self.ws_bytes, ctypes.byref(self.stats), self.stream)
         L.check(rc, what)
         s = self.stats
-        return _Stats(s.nnz, s.total, s.kept, s.norm, s.sum2, s.sum4)
+        return _Stats(s.nnz, s.total, s.kept, s.norm, s.sum2, s.sum4, s.n_in, s.flags)
 
     def synchronize(self):
         torch.cuda.current_stream(self.device).synchronize()
 
 
 class _Stats:
-    __slots__ = ("nnz", "total", "kept", "norm", "sum2", "sum4")
+    __slots__ = ("nnz", "total", "kept", "norm", "sum2", "sum4", "n_in", "flags")
 
-    def __init__(self, nnz, total, kept, norm, sum2, sum4):
+    def __init__(self, nnz, total, kept, norm, sum2, sum4, n_in=0, flags=0):
+        self.n_in = int(n_in)
+        self.flags = int(flags)
         self.nnz = int(nnz)
         self.total = float(total)
         self.kept = float(kept)

@@ -43,8 +43,9 @@ class BassStats(ctypes.Structure):
         ("norm", ctypes.c_double),
         ("sum2", ctypes.c_double),
         ("sum4", ctypes.c_double),
+        ("n_in", ctypes.c_int64),
         ("err", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
     ]
 
 

@@ -47,10 +47,12 @@ def densify(ctx, src, seg_tiles, n, dst):
     L.check(rc, "bass_densify")
 
 
-def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE):
+def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE, count=True):
     """Key-ordered gate (mode GATE / ROT) or bit move (MODE_MOVE) on src[:n].
 
-    Gates: returns (n_out, seg_tiles) with the output segmented in dst.
+    Gates: returns (n_out, seg_tiles) with the output segmented in dst; with
+    count=False n_out is None and nothing waits for the device (a following
+    truncate(..., segmented=True) reports it as stats.n_in).
     Move: returns (n, 0) with the output dense in dst."""
     tm = 1024 if mode == L.MODE_MOVE else SEG // (1 << nq)
     tiles = (n + tm - 1) // tm
@@ -67,6 +69,8 @@ def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE):
     L.check(rc, "bass_lift")
     if mode == L.MODE_MOVE:
         return n, 0
+    if not count:
+        return None, int(st.value)
     return ctx.read_stats("bass_lift").nnz, int(st.value)
 
 
@@ -81,7 +85,11 @@ def key_map_arg(pos):
 def truncate(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False, key_map=None):
     """Top-k of src[:n] (n > k) into dst[:k].  Returns stats (kept, total, sum2, sum4, norm).
 
+    Segmented input (a gate's output): n may be None and the input may hold
+    <= k entries, which are then all kept, in order, without renormalisation
+    (stats.flags bit 0 clear); stats.n_in is the input count.
     key_map: stored-bit -> qubit map (key_map_arg) when src holds stored words."""
+    n = 0 if n is None else n
     ctx.ensure_ws(max(n, hist_tiles * SEG if segmented else 0))
     dst.ensure(k)
     if key_map is None:

@@ -44,7 +44,7 @@ struct Layout {
   // persistent
   size_t st, tile_psum, tile_dsum, tile_s2, tile_s4, rdm_part, rho, sort_hist, sort_tick;
   size_t lb_gate, lb1, sort_status;
-  size_t seg_cnt, soff, cnt1, tile_dcnt, ksum, kcnt, koff, lift_b;
+  size_t seg_cnt, soff, cnt1, tile_dcnt, ksum, kcnt, koff, lift_b, hist2_rows;
   size_t big;
   size_t total;
 };
@@ -83,6 +83,7 @@ static Layout layout(int64_t S) {
   L.kcnt = o; o += al(stiles * 4);
   L.koff = o; o += al(stiles * 8);
   L.lift_b = o; o += al((etiles + 1) * sizeof(LiftBound));
+  L.hist2_rows = o; o += al((size_t)stiles * HIST_BINS * 2);
   L.big = o;
   // big scratch union:
   //   gate group table: cap <= 4 n_in slots x <= 32 B, n_in <= S/4 (2q) or S/2 (1q) -> <= 32 S
@@ -202,7 +203,7 @@ static int hip_check(const char* where) {
 
 extern "C" {
 
-int bass_abi_version(void) { return 1; }
+int bass_abi_version(void) { return 2; }
 
 const char* bass_last_error(void) { return g_err; }
 
@@ -348,8 +349,8 @@ int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
                   int64_t cap_out, int renorm, int64_t hist_tiles, int segmented, const int32_t* key_map,
                   int key_bits, void* ws, size_t ws_bytes, void* stream) {
-  if (k < 1 || n <= k) return fail(BASS_E_ARG, "bass_truncate: need n > k >= 1 (n=%lld k=%lld)",
-                                   (long long)n, (long long)k);
+  if (k < 1 || (!segmented && n <= k))
+    return fail(BASS_E_ARG, "bass_truncate: need n > k >= 1 (n=%lld k=%lld)", (long long)n, (long long)k);
   if (cap_out < k) return fail(BASS_E_OVERFLOW, "bass_truncate: cap_out < k");
   if (n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_truncate: n too large");
   if (segmented && hist_tiles <= 0) return fail(BASS_E_ARG, "bass_truncate: segmented input needs hist_tiles");
@@ -391,6 +392,7 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   S_.hist1 = at<unsigned int>(ws, L.hist1);
   S_.tile_psum = at<double>(ws, L.tile_psum);
   S_.hist2 = at<unsigned int>(ws, L.hist2);
+  S_.hist2_rows = at<unsigned short>(ws, L.hist2_rows);
   S_.sel = at<SelState>(ws, L.sel);
   S_.st = st;
   char* big = at<char>(ws, L.big);

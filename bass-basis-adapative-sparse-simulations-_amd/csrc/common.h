@@ -15,6 +15,8 @@ namespace bass {
 constexpr uint64_t EMPTY = 0xFFFFFFFFFFFFFFFFull;  // hash sentinel, never a key for N <= 63
 constexpr int NT = 256;                             // threads per block (4 waves)
 constexpr int WAVES = NT / 64;
+constexpr int NTB = 1024;                           // single-block reduction / scan kernels
+constexpr int WAVESB = NTB / 64;
 
 // Error bits reported through the per-call status word.
 enum : int {
@@ -59,8 +61,9 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
   return v;
 }
 
-// Block-wide exclusive scan for NT threads.  lds must hold WAVES ints.
+// Block-wide exclusive scan for NTHR threads.  lds must hold NTHR/64 ints.
 // Returns the exclusive prefix; total receives the block sum.
+template <int NTHR = NT>
 __device__ __forceinline__ int block_excl_scan(int v, int& total, int* lds) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int inc = wave_incl_scan(v);
@@ -68,7 +71,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int& total, int* lds) {
   __syncthreads();
   int pre = 0, tot = 0;
 #pragma unroll
-  for (int i = 0; i < WAVES; i++) {
+  for (int i = 0; i < NTHR / 64; i++) {
     int s = lds[i];
     pre += (i < w) ? s : 0;
     tot += s;
@@ -79,7 +82,8 @@ __device__ __forceinline__ int block_excl_scan(int v, int& total, int* lds) {
 }
 
 // Deterministic block sum of doubles (fixed shuffle tree, then waves in order).
-// lds must hold WAVES doubles.  Result valid in every thread.
+// lds must hold NTHR/64 doubles.  Result valid in every thread.
+template <int NTHR = NT>
 __device__ __forceinline__ double block_sum(double v, double* lds) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
@@ -88,7 +92,7 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   __syncthreads();
   double s = 0.0;
 #pragma unroll
-  for (int i = 0; i < WAVES; i++) s += lds[i];
+  for (int i = 0; i < NTHR / 64; i++) s += lds[i];
   __syncthreads();
   return s;
 }

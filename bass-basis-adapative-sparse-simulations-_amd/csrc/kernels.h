@@ -21,7 +21,7 @@ struct SelState {
   unsigned long long cand1;      // candidates after digit 1
   unsigned int cand2;            // candidates after digit 2 (atomic append)
   unsigned int b1, b2;           // threshold buckets of digits 1 and 2
-  unsigned int pad;
+  unsigned int keepall;          // the input holds <= k entries: keep all, no renormalisation
   unsigned long long def_count;  // elements above the digit-1 bucket
 };
 
@@ -33,8 +33,9 @@ struct OpStats {
   double norm;    // sqrt(kept) used for renormalisation (1 if not renormalised)
   double sum2;    // sum |a|^2 of the output as written
   double sum4;    // sum |a|^4 of the output as written
+  int64_t n_in;   // truncate: input entry count
   int32_t err;    // error bits (ERR_*)
-  int32_t pad;
+  int32_t flags;  // truncate: bit 0 = entries dropped
 };
 
 int group_slot_bytes(int qb);
@@ -85,6 +86,7 @@ struct SelLaunch {
   unsigned int* hist1;   // zeroed by the caller unless hist_ready
   double* tile_psum;
   unsigned int* hist2;   // zeroed
+  unsigned short* hist2_rows;  // ntiles * HIST_BINS
   SelState* sel;         // zeroed
   OpStats* st;
   unsigned long long* c1p;  // ntiles * SEG_ENTRIES

@@ -50,25 +50,27 @@ __device__ __forceinline__ int tile_count(const int* seg_cnt, int64_t n_dense, i
 }
 
 // Find, in a 2048-bin histogram, the bucket holding the need-th largest element.
-// Block of NT threads, 8 bins per thread.  Results in *b and *above (count in
-// higher buckets).  lds_scan: WAVES ints.
+// Block of NTHR threads, HIST_BINS/NTHR bins per thread.  Results in *b and
+// *above (count in higher buckets).  lds_scan: NTHR/64 ints.
+template <int NTHR = NT>
 __device__ void find_from_top(const unsigned int* hist, unsigned long long need, unsigned int* b,
                               unsigned long long* above, int* lds_scan) {
+  constexpr int PER = HIST_BINS / NTHR;
   const int t = threadIdx.x;
-  unsigned int c[8];
+  unsigned int c[PER];
   int v = 0;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    c[j] = hist[t * 8 + j];
+  for (int j = 0; j < PER; ++j) {
+    c[j] = hist[t * PER + j];
     v += (int)c[j];
   }
   int tot;
-  const int pre = block_excl_scan(v, tot, lds_scan);
+  const int pre = block_excl_scan<NTHR>(v, tot, lds_scan);
   unsigned long long ab = (unsigned long long)(tot - pre - v);
   if (ab < need && need <= ab + (unsigned long long)v) {
-    for (int j = 7; j >= 0; --j) {
+    for (int j = PER - 1; j >= 0; --j) {
       if (ab + c[j] >= need) {
-        *b = (unsigned int)(t * 8 + j);
+        *b = (unsigned int)(t * PER + j);
         *above = ab;
         break;
       }
@@ -79,22 +81,24 @@ __device__ void find_from_top(const unsigned int* hist, unsigned long long need,
 }
 
 // Same, counting from the smallest bucket (for the key tie-break).
+template <int NTHR = NT>
 __device__ void find_from_bottom(const unsigned int* hist, unsigned long long need, unsigned int* b,
                                  unsigned long long* below, int* lds_scan) {
+  constexpr int PER = HIST_BINS / NTHR;
   const int t = threadIdx.x;
-  unsigned int c[8];
+  unsigned int c[PER];
   int v = 0;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    c[j] = hist[t * 8 + j];
+  for (int j = 0; j < PER; ++j) {
+    c[j] = hist[t * PER + j];
     v += (int)c[j];
   }
   int tot;
-  unsigned long long bl = (unsigned long long)block_excl_scan(v, tot, lds_scan);
+  unsigned long long bl = (unsigned long long)block_excl_scan<NTHR>(v, tot, lds_scan);
   if (bl < need && need <= bl + (unsigned long long)v) {
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < PER; ++j) {
       if (bl + c[j] >= need) {
-        *b = (unsigned int)(t * 8 + j);
+        *b = (unsigned int)(t * PER + j);
         *below = bl;
         break;
       }
@@ -104,11 +108,21 @@ __device__ void find_from_bottom(const unsigned int* hist, unsigned long long ne
   __syncthreads();
 }
 
-// Fixed-order sum of a double array by one block.
+// Fixed-order sum of a double array by one block (each thread adds its strided
+// elements in order; loads are issued four at a time).
+template <int NTHR = NT>
 __device__ double block_sum_array(const double* v, int64_t m, double* lds) {
   double s = 0.0;
-  for (int64_t j = threadIdx.x; j < m; j += NT) s += v[j];
-  return block_sum(s, lds);
+  int64_t j = threadIdx.x;
+  for (; j + 3 * NTHR < m; j += 4 * NTHR) {
+    const double a0 = v[j], a1 = v[j + NTHR], a2 = v[j + 2 * NTHR], a3 = v[j + 3 * NTHR];
+    s += a0;
+    s += a1;
+    s += a2;
+    s += a3;
+  }
+  for (; j < m; j += NTHR) s += v[j];
+  return block_sum<NTHR>(s, lds);
 }
 
 __global__ __launch_bounds__(NT) void k_hist_psum(const double2* __restrict__ a, const int* seg_cnt,
@@ -137,11 +151,11 @@ __global__ __launch_bounds__(NT) void k_hist_psum(const double2* __restrict__ a,
 }
 
 // Digit 1: find the threshold bucket; also reduce the input's sum |a|^2.
-__global__ __launch_bounds__(NT) void k_find1(const unsigned int* hist, unsigned long long k,
-                                               const double* tile_psum, int64_t npsum, SelState* sel,
-                                               OpStats* st) {
-  __shared__ int s_scan[WAVES];
-  __shared__ double s_red[WAVES];
+__global__ __launch_bounds__(NTB) void k_find1(const unsigned int* hist, unsigned long long k,
+                                                const double* tile_psum, int64_t npsum, SelState* sel,
+                                                OpStats* st) {
+  __shared__ int s_scan[WAVESB];
+  __shared__ double s_red[WAVESB];
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_above;
   if (threadIdx.x == 0) {
@@ -149,77 +163,139 @@ __global__ __launch_bounds__(NT) void k_find1(const unsigned int* hist, unsigned
     s_above = 0;
   }
   __syncthreads();
-  find_from_top(hist, k, &s_b, &s_above, s_scan);
-  const double total = block_sum_array(tile_psum, npsum, s_red);
+  // input count = sum of the digit-1 histogram
+  int cnt = 0;
+  for (int b = threadIdx.x; b < HIST_BINS; b += NTB) cnt += (int)hist[b];
+  int count;
+  (void)block_excl_scan<NTB>(cnt, count, s_scan);
+  const bool keepall = (unsigned long long)count <= k;
+  if (!keepall) find_from_top<NTB>(hist, k, &s_b, &s_above, s_scan);
+  const double total = block_sum_array<NTB>(tile_psum, npsum, s_red);
   if (threadIdx.x == 0) {
-    if (s_b == 0xFFFFFFFFu) set_err(&st->err, ERR_SELECT);
-    sel->b1 = s_b;
-    sel->need = k - s_above;
-    sel->prefix = (unsigned long long)s_b << 52;
     st->total = total;
+    st->n_in = count;
+    if (keepall) {
+      // every entry passes kept_pred: bits > 0, or bits == 0 and key <= ~0
+      sel->keepall = 1;
+      sel->pstar = 0;
+      sel->keystar = ~0ull;
+      st->flags = 0;
+    } else {
+      if (s_b == 0xFFFFFFFFu) set_err(&st->err, ERR_SELECT);
+      sel->b1 = s_b;
+      sel->need = k - s_above;
+      sel->prefix = (unsigned long long)s_b << 52;
+      st->flags = 1;
+    }
   }
 }
 
+constexpr int EX1_TILES = 4;  // tiles per k_extract1 block (one digit-2 row per block)
+
 __global__ __launch_bounds__(NT) void k_extract1(const double2* __restrict__ a, const int* seg_cnt,
-                                                  int64_t n_dense, const SelState* sel,
+                                                  int64_t n_dense, int64_t ntiles, const SelState* sel,
                                                   unsigned long long* __restrict__ c1p,
                                                   unsigned int* __restrict__ c1i, int* __restrict__ cnt1,
-                                                  unsigned int* hist2, double* tile_dsum, int* tile_dcnt) {
+                                                  unsigned short* __restrict__ hist2_rows, double* tile_dsum,
+                                                  int* tile_dcnt) {
   __shared__ unsigned int s_hist[HIST_BINS];
   __shared__ int s_scan[WAVES];
   __shared__ double s_red[WAVES];
   __shared__ int s_dcnt;
-  const unsigned int tile = blockIdx.x;
-  for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
-  if (threadIdx.x == 0) s_dcnt = 0;
-  __syncthreads();
-  const unsigned int b1 = sel->b1;
-  const int cnt = tile_count(seg_cnt, n_dense, tile);
-  const int64_t base = (int64_t)tile * SEL_TILE;
-  unsigned long long cb[SEL_ITEMS];
-  bool isc[SEL_ITEMS];
-  double dsum = 0.0;
-  int dcnt = 0;
-#pragma unroll
-  for (int it = 0; it < SEL_ITEMS; ++it) {
-    const int j = it * NT + threadIdx.x;
-    isc[it] = false;
-    cb[it] = 0;
-    if (j < cnt) {
-      const double p = cabs2(a[base + j]);
-      const unsigned long long bits = (unsigned long long)__double_as_longlong(p);
-      const unsigned int d = (unsigned int)(bits >> 52);
-      if (d > b1) {
-        dsum += p;
-        ++dcnt;
-      } else if (d == b1) {
-        isc[it] = true;
-        cb[it] = bits;
-        atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
+  if (sel->keepall) {
+    // every entry is kept: per-tile kept count = tile count, no candidates
+    for (int sub = threadIdx.x; sub < EX1_TILES; sub += NT) {
+      const int64_t tile = (int64_t)blockIdx.x * EX1_TILES + sub;
+      if (tile < ntiles) {
+        tile_dcnt[tile] = tile_count(seg_cnt, n_dense, tile);
+        tile_dsum[tile] = 0.0;
+        cnt1[tile] = 0;
       }
     }
+    return;
   }
-  int rb = 0;
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  const unsigned int b1 = sel->b1;
+  for (int sub = 0; sub < EX1_TILES; ++sub) {
+    const int64_t tile = (int64_t)blockIdx.x * EX1_TILES + sub;
+    if (tile >= ntiles) break;  // uniform
+    if (threadIdx.x == 0) s_dcnt = 0;
+    __syncthreads();
+    const int cnt = tile_count(seg_cnt, n_dense, tile);
+    const int64_t base = tile * SEL_TILE;
+    unsigned long long cb[SEL_ITEMS];
+    bool isc[SEL_ITEMS];
+    double dsum = 0.0;
+    int dcnt = 0;
 #pragma unroll
-  for (int it = 0; it < SEL_ITEMS; ++it) {
-    int tot;
-    const int off = block_excl_scan(isc[it] ? 1 : 0, tot, s_scan);
-    if (isc[it]) {
-      c1p[base + rb + off] = cb[it];
-      c1i[base + rb + off] = (unsigned int)(base + it * NT + threadIdx.x);
+    for (int it = 0; it < SEL_ITEMS; ++it) {
+      const int j = it * NT + threadIdx.x;
+      isc[it] = false;
+      cb[it] = 0;
+      if (j < cnt) {
+        const double p = cabs2(a[base + j]);
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(p);
+        const unsigned int d = (unsigned int)(bits >> 52);
+        if (d > b1) {
+          dsum += p;
+          ++dcnt;
+        } else if (d == b1) {
+          isc[it] = true;
+          cb[it] = bits;
+          atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
+        }
+      }
     }
-    rb += tot;
+    int rb = 0;
+#pragma unroll
+    for (int it = 0; it < SEL_ITEMS; ++it) {
+      int tot;
+      const int off = block_excl_scan(isc[it] ? 1 : 0, tot, s_scan);
+      if (isc[it]) {
+        c1p[base + rb + off] = cb[it];
+        c1i[base + rb + off] = (unsigned int)(base + it * NT + threadIdx.x);
+      }
+      rb += tot;
+    }
+    const double s = block_sum(dsum, s_red);
+    atomicAdd(&s_dcnt, dcnt);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      tile_dsum[tile] = s;
+      tile_dcnt[tile] = s_dcnt;
+      cnt1[tile] = rb;
+    }
   }
-  const double s = block_sum(dsum, s_red);
-  atomicAdd(&s_dcnt, dcnt);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    tile_dsum[tile] = s;
-    tile_dcnt[tile] = s_dcnt;
-    cnt1[tile] = rb;
+  // the block's digit-2 histogram as one row (counts <= EX1_TILES * SEL_TILE
+  // fit 16 bits); k_hist2_reduce sums the rows.  About half of the bins of a
+  // row are nonzero, so per-bin global atomics would cost ~1000 per block.
+  unsigned short* row = hist2_rows + (int64_t)blockIdx.x * HIST_BINS;
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT) row[b] = (unsigned short)s_hist[b];
+}
+
+// Column sums of the digit-2 rows: thread = 4 adjacent bins (one 8-byte load
+// per row), block = H2_ROWS rows.
+constexpr int H2_ROWS = 32;
+__global__ __launch_bounds__(NT) void k_hist2_reduce(const unsigned short* __restrict__ rows, int64_t nrows,
+                                                      const SelState* sel, unsigned int* hist2) {
+  if (sel->keepall) return;
+  const int b = (blockIdx.x * NT + threadIdx.x) * 4;
+  const int64_t r0 = (int64_t)blockIdx.y * H2_ROWS;
+  const int64_t r1 = r0 + H2_ROWS < nrows ? r0 + H2_ROWS : nrows;
+  unsigned int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll 8
+  for (int64_t r = r0; r < r1; ++r) {
+    const ushort4 v = *reinterpret_cast<const ushort4*>(rows + r * HIST_BINS + b);
+    s0 += v.x;
+    s1 += v.y;
+    s2 += v.z;
+    s3 += v.w;
   }
-  for (int b = threadIdx.x; b < HIST_BINS; b += NT)
-    if (s_hist[b]) atomicAdd(&hist2[b], s_hist[b]);
+  if (s0) atomicAdd(&hist2[b], s0);
+  if (s1) atomicAdd(&hist2[b + 1], s1);
+  if (s2) atomicAdd(&hist2[b + 2], s2);
+  if (s3) atomicAdd(&hist2[b + 3], s3);
 }
 
 __global__ __launch_bounds__(NT) void k_find2(const unsigned int* hist2, SelState* sel, OpStats* st) {
@@ -231,6 +307,7 @@ __global__ __launch_bounds__(NT) void k_find2(const unsigned int* hist2, SelStat
     s_above = 0;
   }
   __syncthreads();
+  if (sel->keepall) return;
   const unsigned long long need = sel->need;
   find_from_top(hist2, need, &s_b, &s_above, s_scan);
   if (threadIdx.x == 0) {
@@ -248,6 +325,7 @@ __global__ __launch_bounds__(NT) void k_extract2(const unsigned long long* __res
                                                   const int* __restrict__ cnt1, int64_t ntiles, SelState* sel,
                                                   unsigned long long* __restrict__ c2p,
                                                   unsigned int* __restrict__ c2i) {
+  if (sel->keepall) return;
   const unsigned int b2 = sel->b2;
   const int lane = threadIdx.x & 63;
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -287,6 +365,7 @@ __global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restr
   __shared__ int s_scan[WAVES];
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_ab;
+  if (sel->keepall) return;  // pstar / keystar set by k_find1
   const unsigned int m2 = sel->cand2;
   unsigned long long need = sel->need;
   unsigned long long prefix = sel->prefix;
@@ -392,31 +471,57 @@ __global__ __launch_bounds__(NT) void k_keptsum(const unsigned long long* __rest
   }
 }
 
-// kept = sum over tiles in order; kept-entry offsets per tile (exclusive scan).
-__global__ __launch_bounds__(NT) void k_sel_finalize(const double* tile_dsum, const double* ksum,
-                                                      const int* tile_dcnt, const int* kcnt, int64_t ntiles,
-                                                      unsigned long long k, int renorm, int64_t* koff,
-                                                      OpStats* st) {
-  __shared__ double s_red[WAVES];
-  __shared__ int s_scan[WAVES];
+// kept = sum over tiles in a fixed order; kept-entry offsets per tile
+// (exclusive scan).  Thread i owns FIN_CH consecutive tiles per round; its
+// loads are issued together.
+constexpr int FIN_CH = 4;
+__global__ __launch_bounds__(NTB) void k_sel_finalize(const double* tile_dsum, const double* ksum,
+                                                       const int* tile_dcnt, const int* kcnt, int64_t ntiles,
+                                                       unsigned long long k, int renorm, const SelState* sel,
+                                                       int64_t* koff, OpStats* st) {
+  __shared__ double s_red[WAVESB];
+  __shared__ int s_scan[WAVESB];
   double s = 0.0;
-  for (int64_t t = threadIdx.x; t < ntiles; t += NT) s += tile_dsum[t] + ksum[t];
-  const double kept = block_sum(s, s_red);
   int64_t carry = 0;
-  for (int64_t t0 = 0; t0 < ntiles; t0 += NT) {
-    const int64_t t = t0 + threadIdx.x;
-    const int v = t < ntiles ? tile_dcnt[t] + kcnt[t] : 0;
+  for (int64_t r0 = 0; r0 < ntiles; r0 += (int64_t)NTB * FIN_CH) {
+    const int64_t t0 = r0 + (int64_t)threadIdx.x * FIN_CH;
+    double ds[FIN_CH], ks[FIN_CH];
+    int c[FIN_CH];
+#pragma unroll
+    for (int q = 0; q < FIN_CH; ++q) {
+      const int64_t t = t0 + q;
+      const bool in = t < ntiles;
+      ds[q] = in ? tile_dsum[t] : 0.0;
+      ks[q] = in ? ksum[t] : 0.0;
+      c[q] = in ? tile_dcnt[t] + kcnt[t] : 0;
+    }
+    int v = 0;
+#pragma unroll
+    for (int q = 0; q < FIN_CH; ++q) {
+      s += ds[q] + ks[q];
+      v += c[q];
+    }
     int tot;
-    const int ex = block_excl_scan(v, tot, s_scan);
-    if (t < ntiles) koff[t] = carry + ex;
+    int64_t run = carry + block_excl_scan<NTB>(v, tot, s_scan);
+#pragma unroll
+    for (int q = 0; q < FIN_CH; ++q) {
+      if (t0 + q < ntiles) koff[t0 + q] = run;
+      run += c[q];
+    }
     carry += tot;
   }
+  const double kept = block_sum<NTB>(s, s_red);
   if (threadIdx.x == 0) {
-    if ((unsigned long long)carry != k) set_err(&st->err, ERR_SELECT);
     st->nnz = carry;
-    st->kept = kept;
-    const double norm = sqrt(kept);
-    st->norm = (renorm && norm > 1e-30) ? norm : 1.0;
+    if (sel->keepall) {
+      st->kept = st->total;
+      st->norm = 1.0;
+    } else {
+      if ((unsigned long long)carry != k) set_err(&st->err, ERR_SELECT);
+      st->kept = kept;
+      const double norm = sqrt(kept);
+      st->norm = (renorm && norm > 1e-30) ? norm : 1.0;
+    }
   }
 }
 
@@ -478,11 +583,11 @@ __global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x, 
 }
 
 // Fixed-order reduction of per-tile sums into the stats block.
-__global__ __launch_bounds__(NT) void k_sums_finalize(const double* tile_s2, const double* tile_s4,
-                                                       int64_t ntiles, OpStats* st) {
-  __shared__ double s_red[WAVES];
-  const double s2 = block_sum_array(tile_s2, ntiles, s_red);
-  const double s4 = block_sum_array(tile_s4, ntiles, s_red);
+__global__ __launch_bounds__(NTB) void k_sums_finalize(const double* tile_s2, const double* tile_s4,
+                                                        int64_t ntiles, OpStats* st) {
+  __shared__ double s_red[WAVESB];
+  const double s2 = block_sum_array<NTB>(tile_s2, ntiles, s_red);
+  const double s4 = block_sum_array<NTB>(tile_s4, ntiles, s_red);
   if (threadIdx.x == 0) {
     st->sum2 = s2;
     st->sum4 = s4;
@@ -490,16 +595,25 @@ __global__ __launch_bounds__(NT) void k_sums_finalize(const double* tile_s2, con
 }
 
 // Segment counts -> exclusive offsets and total (nnz of a segmented output).
-__global__ __launch_bounds__(NT) void k_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff,
-                                                  OpStats* st) {
-  __shared__ int s_scan[WAVES];
+__global__ __launch_bounds__(NTB) void k_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff,
+                                                   OpStats* st) {
+  __shared__ int s_scan[WAVESB];
   int64_t carry = 0;
-  for (int64_t t0 = 0; t0 < ntiles; t0 += NT) {
-    const int64_t t = t0 + threadIdx.x;
-    const int v = t < ntiles ? seg_cnt[t] : 0;
+  for (int64_t r0 = 0; r0 < ntiles; r0 += (int64_t)NTB * FIN_CH) {
+    const int64_t t0 = r0 + (int64_t)threadIdx.x * FIN_CH;
+    int c[FIN_CH];
+#pragma unroll
+    for (int q = 0; q < FIN_CH; ++q) c[q] = t0 + q < ntiles ? seg_cnt[t0 + q] : 0;
+    int v = 0;
+#pragma unroll
+    for (int q = 0; q < FIN_CH; ++q) v += c[q];
     int tot;
-    const int ex = block_excl_scan(v, tot, s_scan);
-    if (t < ntiles) soff[t] = carry + ex;
+    int64_t run = carry + block_excl_scan<NTB>(v, tot, s_scan);
+#pragma unroll
+    for (int q = 0; q < FIN_CH; ++q) {
+      if (t0 + q < ntiles) soff[t0 + q] = run;
+      run += c[q];
+    }
     carry += tot;
   }
   if (threadIdx.x == 0) st->nnz = carry;
@@ -573,9 +687,12 @@ int launch_select(const SelLaunch& L, hipStream_t s) {
                        L.tile_psum);
     npsum = tiles;
   }
-  hipLaunchKernelGGL(k_find1, dim3(1), dim3(NT), 0, s, L.hist1, L.k, L.tile_psum, npsum, L.sel, L.st);
-  hipLaunchKernelGGL(k_extract1, dim3((unsigned)tiles), dim3(NT), 0, s, L.a, L.seg_cnt, L.n, L.sel, L.c1p,
-                     L.c1i, L.cnt1, L.hist2, L.tile_dsum, L.tile_dcnt);
+  hipLaunchKernelGGL(k_find1, dim3(1), dim3(NTB), 0, s, L.hist1, L.k, L.tile_psum, npsum, L.sel, L.st);
+  const int64_t rows = (tiles + EX1_TILES - 1) / EX1_TILES;
+  hipLaunchKernelGGL(k_extract1, dim3((unsigned)rows), dim3(NT), 0, s, L.a, L.seg_cnt, L.n, tiles, L.sel, L.c1p,
+                     L.c1i, L.cnt1, L.hist2_rows, L.tile_dsum, L.tile_dcnt);
+  hipLaunchKernelGGL(k_hist2_reduce, dim3(HIST_BINS / (4 * NT), (unsigned)((rows + H2_ROWS - 1) / H2_ROWS)),
+                     dim3(NT), 0, s, L.hist2_rows, rows, L.sel, L.hist2);
   hipLaunchKernelGGL(k_find2, dim3(1), dim3(NT), 0, s, L.hist2, L.sel, L.st);
   int64_t eb = tiles < 1024 ? tiles : 1024;
   hipLaunchKernelGGL(k_extract2, dim3((unsigned)eb), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, tiles, L.sel, L.c2p,
@@ -583,16 +700,16 @@ int launch_select(const SelLaunch& L, hipStream_t s) {
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(NT), 0, s, L.c2p, L.c2i, L.x, L.km, L.sel, L.st);
   hipLaunchKernelGGL(k_keptsum, dim3((unsigned)tiles), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, L.x, L.km, L.sel,
                      L.ksum, L.kcnt);
-  hipLaunchKernelGGL(k_sel_finalize, dim3(1), dim3(NT), 0, s, L.tile_dsum, L.ksum, L.tile_dcnt, L.kcnt, tiles,
-                     L.k, L.renorm, L.koff, L.st);
+  hipLaunchKernelGGL(k_sel_finalize, dim3(1), dim3(NTB), 0, s, L.tile_dsum, L.ksum, L.tile_dcnt, L.kcnt, tiles,
+                     L.k, L.renorm, L.sel, L.koff, L.st);
   hipLaunchKernelGGL(k_compact, dim3((unsigned)tiles), dim3(NT), 0, s, L.x, L.a, L.seg_cnt, L.n, L.km, L.sel, L.koff,
                      L.xo, L.ao, L.cap_out, L.st, L.tile_s2, L.tile_s4, &L.st->err);
-  hipLaunchKernelGGL(k_sums_finalize, dim3(1), dim3(NT), 0, s, L.tile_s2, L.tile_s4, tiles, L.st);
+  hipLaunchKernelGGL(k_sums_finalize, dim3(1), dim3(NTB), 0, s, L.tile_s2, L.tile_s4, tiles, L.st);
   return (int)hipGetLastError();
 }
 
 int launch_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff, OpStats* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(NT), 0, s, seg_cnt, ntiles, soff, st);
+  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(NTB), 0, s, seg_cnt, ntiles, soff, st);
   return (int)hipGetLastError();
 }
 
@@ -608,7 +725,7 @@ int launch_moments(const double2* a, int64_t n, double* tile_s2, double* tile_s4
   const int64_t tiles = sel_tiles(n);
   if (tiles > 0)
     hipLaunchKernelGGL(k_moments, dim3((unsigned)tiles), dim3(NT), 0, s, a, n, tile_s2, tile_s4);
-  hipLaunchKernelGGL(k_sums_finalize, dim3(1), dim3(NT), 0, s, tile_s2, tile_s4, tiles, st);
+  hipLaunchKernelGGL(k_sums_finalize, dim3(1), dim3(NTB), 0, s, tile_s2, tile_s4, tiles, st);
   return (int)hipGetLastError();
 }
 

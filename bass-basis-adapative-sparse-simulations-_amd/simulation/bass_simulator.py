@@ -358,19 +358,22 @@ class BASS:
                 # tentative: rotate -> truncate to k -> renormalise -> PR
                 rot = state._rot
                 jb = int(state.pos[j])
-                m, rht = _ops.lift(ctx, state._buf, state.nnz, jb, rot, G=V.conj().T, nq=1, mode=L.MODE_ROT)
+                _, rht = _ops.lift(ctx, state._buf, state.nnz, jb, rot, G=V.conj().T, nq=1, mode=L.MODE_ROT,
+                                   count=False)
                 trial_pos = lifted_positions(state.pos, jb, 0)
+                # truncate to k, or (<= k entries) keep all; one stats read
+                st = _ops.truncate(ctx, rot, None, self.k, cand, renorm=True, hist_tiles=rht, segmented=True,
+                                   key_map=_ops.key_map_arg(trial_pos))
+                m = st.n_in
                 self._count("rotate")
                 self._count("amp_updates", m)
                 self._bytes("rot1q", state.nnz, m)
-                if m > self.k:
-                    st = _ops.truncate(ctx, rot, m, self.k, cand, renorm=True, hist_tiles=rht, segmented=True,
-                                       key_map=_ops.key_map_arg(trial_pos))
+                if st.flags & 1:
                     self._bytes("select", m, self.k)
                     nn, frac, s4 = self.k, st.kept, st.sum4
                 else:
-                    _ops.densify(ctx, rot, rht, m, cand)
-                    s2, _ = _ops.moments(ctx, cand, m)
+                    self._count("bytes_select", 48 * m)
+                    s2 = st.sum2
                     frac = s2
                     norm = math.sqrt(s2)
                     if norm > 1e-30:
@@ -455,27 +458,37 @@ class BASS:
     def step(self, i, gate):
         """One iteration of the reference loop (:861-926).  Gate outputs are
         written in per-tile segments of state._tmp; the truncation (or, without
-        one, a dense copy) brings the state back to state._buf."""
+        one, a dense copy) brings the state back to state._buf.
+
+        With truncate_every == 1 the rule ":899-903" is "truncate iff nnz > k",
+        which the device decides by itself (bass_truncate keeps every entry of
+        a segmented input holding <= k), so a gate and its truncation cost one
+        host round trip."""
         state, ctx, M = self.state, self._ctx, self._M
-        seg = None  # (n, tiles) while state._tmp holds a segmented gate output
         if gate.n_qubits == 1:
             q = gate.qubits[0]
             G = self.U[q].conj().T @ gate.matrix @ self.U[q]
+            fast = self.truncate_every == 1 and 2 * state.nnz <= state.k
+            n_in = state.nnz
             jb = int(state.pos[q])
-            n, ht = _ops.lift(ctx, state._buf, state.nnz, jb, state._tmp, G=G, nq=1)
+            n, ht = _ops.lift(ctx, state._buf, n_in, jb, state._tmp, G=G, nq=1, count=not fast)
             state.pos = lifted_positions(state.pos, jb, 0)
-            self._bytes("gate1q", state.nnz, n)
-            seg = (n, ht)
-            state.nnz, state._hist_tiles, state._sum4 = n, 0, None
-            self._count("amp_updates", n)
-            if state.nnz > state.k:  # SparseState's own budget (:297-301)
-                st = _ops.truncate(ctx, state._tmp, n, state.k, state._buf, renorm=True, hist_tiles=ht,
-                                   segmented=True, key_map=_ops.key_map_arg(state.pos))
-                state.gamma *= np.sqrt(st.kept)
-                state.nnz = state.k
-                seg = None
-            self._since_trunc += 1
             self._count("gate1q")
+            if fast:
+                self._fast_truncate(i, state, n_in, ht, "gate1q")
+            else:
+                self._bytes("gate1q", n_in, n)
+                self._count("amp_updates", n)
+                state.nnz, state._hist_tiles, state._sum4 = n, 0, None
+                seg = (n, ht)
+                if state.nnz > state.k:  # SparseState's own budget (:297-301)
+                    st = _ops.truncate(ctx, state._tmp, n, state.k, state._buf, renorm=True, hist_tiles=ht,
+                                       segmented=True, key_map=_ops.key_map_arg(state.pos))
+                    state.gamma *= np.sqrt(st.kept)
+                    state.nnz = state.k
+                    seg = None
+                self._since_trunc += 1
+                self._legacy_truncate(i, state, seg)
         elif gate.n_qubits == 2:
             q1, q2 = gate.qubits
             Ut = self._get_kron(q1, q2)
@@ -494,16 +507,55 @@ class BASS:
                         self._pr_at_last_opt = self._pr(state)
                     self._trunc_active = False
                 return
-            n, ht = self._lift_2q(state, G, q1, q2)
-            self._bytes("gate2q", state.nnz, n)
-            seg = (n, ht)
-            state.nnz, state._hist_tiles, state._sum4 = n, 0, None
-            self._count("amp_updates", n)
-            self._since_trunc += 1
+            fast = self.truncate_every == 1
+            n_in = state.nnz
+            n, ht = self._lift_2q(state, G, q1, q2, count=not fast)
             self._count("gate2q")
+            if fast:
+                self._fast_truncate(i, state, n_in, ht, "gate2q")
+            else:
+                self._bytes("gate2q", n_in, n)
+                self._count("amp_updates", n)
+                state.nnz, state._hist_tiles, state._sum4 = n, 0, None
+                self._since_trunc += 1
+                self._legacy_truncate(i, state, (n, ht))
         else:
             raise ValueError(f"{gate.n_qubits}-qubit gates unsupported")
 
+        if self._check_point(i, M):
+            if self._trunc_active and self.optimize_trigger(state):
+                self._optimize_basis(state)
+                self._pr_at_last_opt = self._pr(state)
+            self._trunc_active = False
+
+    def _fast_truncate(self, i, state, n_in, ht, op):
+        """Truncate-or-keep of the segmented gate output in state._tmp, decided
+        on the device; one stats read."""
+        st = _ops.truncate(self._ctx, state._tmp, None, self.k, state._buf, renorm=True, hist_tiles=ht,
+                           segmented=True, key_map=_ops.key_map_arg(state.pos))
+        n = st.n_in
+        self._bytes(op, n_in, n)
+        self._count("amp_updates", n)
+        self._since_trunc += 1
+        state._hist_tiles = 0
+        state._sum2, state._sum4 = st.sum2, st.sum4
+        if st.flags & 1:
+            self._trunc_active = True
+            self._bytes("select", n, self.k)
+            total, kept = st.total, st.kept
+            self._gamma_running *= kept / total if total > 1e-30 else 0
+            if self.trace is not None:
+                self.trace.append(("trunc", i, float(kept), float(total)))
+            state.nnz = self.k
+            self._since_trunc = 0
+            self._count("truncate")
+        else:
+            self._count("bytes_select", 48 * n)  # kept everything: a dense copy
+            state.nnz = n
+
+    def _legacy_truncate(self, i, state, seg):
+        """The reference's truncation rule with host-side counts (truncate_every > 1)."""
+        ctx = self._ctx
         over_hard = state.nnz > self._hard_cap
         over_budget = state.nnz > self.k
         if over_hard or (over_budget and self._since_trunc >= self.truncate_every):
@@ -511,7 +563,6 @@ class BASS:
             if seg is not None:
                 st = _ops.truncate(ctx, state._tmp, seg[0], self.k, state._buf, renorm=True, hist_tiles=seg[1],
                                    segmented=True, key_map=_ops.key_map_arg(state.pos))
-                seg = None
             else:
                 st = _ops.truncate(ctx, state._buf, state.nnz, self.k, state._tmp, renorm=True,
                                    hist_tiles=state._hist_tiles, key_map=_ops.key_map_arg(state.pos))
@@ -529,16 +580,10 @@ class BASS:
         elif seg is not None:
             _ops.densify(ctx, state._tmp, seg[1], seg[0], state._buf)
 
-        if self._check_point(i, M):
-            if self._trunc_active and self.optimize_trigger(state):
-                self._optimize_basis(state)
-                self._pr_at_last_opt = self._pr(state)
-            self._trunc_active = False
-
-    def _lift_2q(self, state, G, q1, q2):
+    def _lift_2q(self, state, G, q1, q2, count=True):
         """2-qubit gate on the ordered state: bring q2 to bit 0 (a move, unless it
         is there already or q1 is), then lift q1's bit to bit 1 with the gate.
-        Returns (n_out, seg_tiles) of the segmented output in state._tmp."""
+        Returns (n_out or None, seg_tiles) of the segmented output in state._tmp."""
         ctx = self._ctx
         if state.pos[q1] == 0 and state.pos[q2] != 0:
             G = G[np.ix_(_SWAP_IDX, _SWAP_IDX)]
@@ -550,7 +595,7 @@ class BASS:
             state.pos = lifted_positions(state.pos, jb, 0)
             self._bytes("move", state.nnz, state.nnz)
         jb = int(state.pos[q1])
-        n, ht = _ops.lift(ctx, state._buf, state.nnz, jb, state._tmp, G=G, nq=2)
+        n, ht = _ops.lift(ctx, state._buf, state.nnz, jb, state._tmp, G=G, nq=2, count=count)
         state.pos = lifted_positions(state.pos, jb, 1)
         return n, ht
 

@@ -47,8 +47,9 @@ typedef struct bass_stats {
   double norm;  /* divisor applied by renormalisation (1 when none) */
   double sum2;  /* sum |a|^2 of the output as written */
   double sum4;  /* sum |a|^4 of the output as written */
+  int64_t n_in; /* bass_truncate: entries in its input */
   int32_t err;  /* device error bits (0 = none) */
-  int32_t pad;
+  int32_t flags; /* bass_truncate: bit 0 = entries were dropped (0: kept all, no renormalisation) */
 } bass_stats;
 
 int bass_abi_version(void);
@@ -118,7 +119,10 @@ int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, cons
  * driver truncations of src/simulation/bass_simulator.py:565-572, :899-916 and
  * src/simulation/simulator.py:169-201 (renorm=1).  Requires n > k.
  * hist_tiles: value reported by the bass_gate call that produced (x, a), or 0;
- * segmented=1: (x, a) is that call's segmented output (n = stats.nnz).
+ * segmented=1: (x, a) is that call's segmented output; n is not needed
+ * (the count comes from the device) and the input may hold <= k entries:
+ * then all are kept, in order and not renormalised, and stats.flags bit 0 is
+ * clear -- the gate / truncate pair needs no host round trip in between.
  * key_map (key_bits entries, or NULL): stored bit p of x holds qubit
  * key_map[p]; ties are then ordered by the key mapped back. */
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo,

@@ -171,3 +171,35 @@ def test_truncate_key_map_ties(gpu_ctx, n, k):
     assert set(gx.tolist()) == set(ox[:k].tolist())
     assert_same_entries(gx, dev.to_host_amps(dst.a, k), ox[:k], oa[:k], exact=True)
     assert st.kept > 0
+
+
+@pytest.mark.parametrize("n", [1, 3000])
+def test_truncate_segmented_keeps_all_up_to_k(gpu_ctx, n):
+    """A segmented gate output with <= k entries comes back whole, in order and
+    not renormalised (stats.flags bit 0 clear); with > k it is truncated."""
+    ops, dev, SS, L = _mods()
+    print(f"[case] keep-all n={n}", flush=True)
+    rng = np.random.default_rng(n + 1)
+    x, a = random_state(n, 16, seed=n + 2)
+    pos = rng.permutation(16)
+    w, aw = _ordered(x, a, pos)
+    G = random_unitary(2, rng)
+    ew, ea = _run_lift(gpu_ctx, w, aw, int(pos[3]), G, 1, 0)
+    m = len(ew)
+    km = ops.key_map_arg(SS.lifted_positions(pos, int(pos[3]), 0))
+    for k, trunc in [(m + 5, False), (m, False), (max(1, m - 7), m > 1)]:
+        src = _upload(dev, gpu_ctx, w, aw)
+        seg = dev.Buffers(1, gpu_ctx.device)
+        none, tiles = ops.lift(gpu_ctx, src, n, int(pos[3]), seg, G=G, nq=1, count=False)
+        assert none is None
+        dst = dev.Buffers(k, gpu_ctx.device)
+        st = ops.truncate(gpu_ctx, seg, None, k, dst, renorm=True, hist_tiles=tiles, segmented=True, key_map=km)
+        assert st.n_in == m
+        assert bool(st.flags & 1) == trunc
+        if not trunc:
+            assert st.nnz == m and st.norm == 1.0
+            assert np.array_equal(dev.to_host_keys(dst.x, m), ew)
+            assert np.array_equal(dev.to_host_amps(dst.a, m).view(np.float64), ea.view(np.float64))
+        else:
+            assert st.nnz == k
+            assert abs(np.sum(np.abs(dev.to_host_amps(dst.a, k)) ** 2) - 1.0) < 1e-12
