@@ -124,12 +124,13 @@ def scale(ctx, buf, n, div):
     L.check(rc, "bass_scale")
 
 
-def rdm1q(ctx, buf, n, N):
-    """All single-qubit RDMs -> complex128[N, 2, 2] (host)."""
+def rdm1q(ctx, buf, n, N, ordered=False):
+    """All single-qubit RDMs -> complex128[N, 2, 2] (host).  ordered=True: buf
+    is a key-ordered state and row p is the RDM of stored bit p."""
     ctx.ensure_ws(n)
     out = np.zeros((N, 4), dtype=np.float64)
-    rc = ctx.lib.bass_rdm1q(buf.xp, buf.ap, n, int(N), out.ctypes.data_as(ctypes.c_void_p), ctx.wsp,
-                            ctx.ws_bytes, ctx.stream)
+    fn = ctx.lib.bass_rdm1q_ordered if ordered else ctx.lib.bass_rdm1q
+    rc = fn(buf.xp, buf.ap, n, int(N), out.ctypes.data_as(ctypes.c_void_p), ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_rdm1q")
     rho = np.zeros((N, 2, 2), dtype=np.complex128)
     rho[:, 0, 0] = out[:, 0]

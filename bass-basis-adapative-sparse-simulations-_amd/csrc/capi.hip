@@ -475,6 +475,30 @@ int bass_rdm1q(const uint64_t* x, const double* a, int64_t n, int N, double* rho
   return BASS_OK;
 }
 
+int bass_rdm1q_ordered(const uint64_t* x, const double* a, int64_t n, int N, double* rho_out, void* ws,
+                       size_t ws_bytes, void* stream) {
+  if (N < 1 || N > 63) return fail(BASS_E_ARG, "bass_rdm1q_ordered: N out of range");
+  if (n < 0 || n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_rdm1q_ordered: bad n");
+  WS_REQUIRE(n);
+  hipStream_t s = (hipStream_t)stream;
+  double* rho = at<double>(ws, L.rho);
+  if (n <= 0) {
+    fill(rho, (size_t)N * 4 * 8, 0ull, s);
+  } else {
+    if (rdm_ordered_scratch(n, N) > L.total - L.big)
+      return fail(BASS_E_WORKSPACE, "bass_rdm1q_ordered: scratch does not fit");
+    const int tm = ktimer_begin("rdm", s);
+    launch_rdm_ordered(x, (const double2*)a, n, N, at<void>(ws, L.big), rho, s);
+    ktimer_end(tm, s);
+  }
+  int rc = hip_check("bass_rdm1q_ordered");
+  if (rc) return rc;
+  if (hipMemcpyAsync(rho_out, rho, (size_t)N * 4 * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(BASS_E_HIP, "bass_rdm1q_ordered copy: %s", hipGetErrorString(hipGetLastError()));
+  return BASS_OK;
+}
+
 int bass_ht_clear(uint64_t* ht_keys, int32_t* ht_vals, int log2cap, void* stream) {
   if (log2cap < 1 || log2cap > 40) return fail(BASS_E_ARG, "bass_ht_clear: bad capacity");
   hipStream_t s = (hipStream_t)stream;

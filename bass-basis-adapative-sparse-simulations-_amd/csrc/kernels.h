@@ -63,6 +63,10 @@ int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
                 hipStream_t s);
 
+size_t rdm_ordered_scratch(int64_t n, int nbits);
+int launch_rdm_ordered(const uint64_t* x, const double2* a, int64_t n, int nbits, void* scratch, double* rho,
+                       hipStream_t s);
+
 // Stored-bit -> qubit map of a key-ordered state (nbits = 0: keys are stored as is).
 struct KeyMap {
   int nbits;

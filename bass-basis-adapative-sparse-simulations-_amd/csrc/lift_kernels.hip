@@ -67,9 +67,13 @@ __device__ __forceinline__ int64_t wave_partition_point(int64_t lo, int64_t hi, 
   return lo + __popcll(__ballot(t));
 }
 
-__global__ __launch_bounds__(NT) void k_lift_part(const uint64_t* __restrict__ w, int64_t n, int j, int t,
+// Tile boundaries of the lift of bit j0 + blockIdx.y (one bit per grid row;
+// row r writes bounds[r * nbound + u]).
+__global__ __launch_bounds__(NT) void k_lift_part(const uint64_t* __restrict__ w, int64_t n, int j0, int t,
                                                    int tm, int64_t nbound, LiftBound* bounds) {
   const int lane = threadIdx.x & 63;
+  const int j = j0 + (int)blockIdx.y;
+  bounds += (int64_t)blockIdx.y * nbound;
   const int64_t u = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
   if (u >= nbound) return;  // uniform per wave
   int64_t d = u * (int64_t)tm;
@@ -341,6 +345,181 @@ __global__ __launch_bounds__(NT) void k_lift_move(const uint64_t* __restrict__ w
     xo[d + m] = s_t[m];
     ao[d + m] = s_a[m];
   }
+}
+
+// ------------------------------------------------ single-qubit RDMs ------
+// RDMs of every stored bit of a key-ordered state (src/simulation/bass_simulator.py:441-470):
+//   rho_p[0][0] = sum |a_x|^2 over bit p of x clear, rho_p[1][1] over bit p set,
+//   rho_p[0][1] = sum a_x * conj(a_{x | 2^p}) over pairs present.
+// Pairs: the lift of bit p (t = 0) makes x and x | 2^p adjacent in merged
+// order, so every pair lies in the tile holding x, or its partner is the first
+// entry merged after the tile (B cursor of the next boundary).  A tile loads
+// only keys (and their indices), finds each partner with one LDS search and
+// gathers the two amplitudes of each pair.
+constexpr int RDM_TM = 1024;
+
+__global__ __launch_bounds__(NT) void k_rdm_pairs(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                                   int64_t n, int64_t nbound, const LiftBound* __restrict__ bounds,
+                                                   double2* __restrict__ part) {
+  constexpr int ITEMS = RDM_TM / NT;
+  __shared__ uint64_t s_w[RDM_TM];
+  __shared__ int s_g[RDM_TM];
+  __shared__ double s_red[WAVES];
+  __shared__ uint64_t s_hw;
+  __shared__ int s_hg;
+  const int p = blockIdx.y;
+  const int64_t u = blockIdx.x;
+  const LiftBound B0 = bounds[(int64_t)p * nbound + u], B1 = bounds[(int64_t)p * nbound + u + 1];
+  const int64_t d = u * RDM_TM;
+  const int cnt = (int)((n - d) < RDM_TM ? (n - d) : RDM_TM);
+  int64_t p1lo, p1n, p2lo, p2n, p3lo;
+  if (B0.s0 == B1.s0) {
+    p1lo = B0.a;
+    p1n = B1.a - B0.a;
+    p2lo = B0.b;
+    p2n = B1.b - B0.b;
+    p3lo = 0;
+  } else {
+    p1lo = B0.a;
+    p1n = B0.sm - B0.a;
+    p2lo = B0.b;
+    p2n = B1.a - B0.b;
+    p3lo = B1.sm;
+  }
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int idx = it * NT + threadIdx.x;
+    if (idx < cnt) {
+      const int64_t g = idx < p1n ? p1lo + idx : (idx < p1n + p2n ? p2lo + (idx - p1n) : p3lo + (idx - p1n - p2n));
+      s_w[idx] = w[g];
+      s_g[idx] = (int)g;
+    }
+  }
+  if (threadIdx.x == 0) {
+    const bool h = B1.b < B1.s1;
+    s_hw = h ? w[B1.b] : ~0ull;  // ~0 never equals x | 2^p for keys < 2^63
+    s_hg = h ? B1.b : -1;
+  }
+  __syncthreads();
+  const uint64_t bp = 1ull << p;
+  double re = 0.0, im = 0.0;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int idx = it * NT + threadIdx.x;
+    if (idx < cnt) {
+      const uint64_t x = s_w[idx];
+      if (!(x & bp)) {
+        const uint64_t y = x | bp;
+        const int q = lds_lower_bound(s_w, cnt, y);
+        const int gb = (q < cnt && s_w[q] == y) ? s_g[q] : (s_hw == y ? s_hg : -1);
+        if (gb >= 0) {
+          const double2 v = cmul(a[s_g[idx]], cconj(a[gb]));
+          re += v.x;
+          im += v.y;
+        }
+      }
+    }
+  }
+  const double tr = block_sum(re, s_red);
+  const double ti = block_sum(im, s_red);
+  if (threadIdx.x == 0) part[(int64_t)p * gridDim.x + u] = make_double2(tr, ti);
+}
+
+// Diagonal RDM elements of every bit in one pass: per tile of SEG_ENTRIES
+// entries, lane b of each wave owns bit b and walks the wave's entries in
+// order (LDS broadcast reads).
+__global__ __launch_bounds__(NT) void k_rdm_diag(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                                  int64_t n, int nbits, double2* __restrict__ dpart) {
+  constexpr int TILE = SEG_ENTRIES;
+  constexpr int PER_WAVE = TILE / WAVES;
+  __shared__ uint64_t s_w[TILE];
+  __shared__ double s_p[TILE];
+  __shared__ double2 s_acc[WAVES][64];
+  const int64_t d = (int64_t)blockIdx.x * TILE;
+  const int cnt = (int)((n - d) < TILE ? (n - d) : TILE);
+  for (int j = threadIdx.x; j < cnt; j += NT) {
+    s_w[j] = w[d + j];
+    s_p[j] = cabs2(a[d + j]);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double z = 0.0, o = 0.0;
+  const int e0 = wv * PER_WAVE;
+  const int e1 = e0 + PER_WAVE < cnt ? e0 + PER_WAVE : cnt;
+  for (int e = e0; e < e1; ++e) {
+    const double pe = s_p[e];
+    if ((s_w[e] >> lane) & 1ull)
+      o += pe;
+    else
+      z += pe;
+  }
+  s_acc[wv][lane] = make_double2(z, o);
+  __syncthreads();
+  if (threadIdx.x < nbits) {
+    double2 t = s_acc[0][threadIdx.x];
+    for (int k = 1; k < WAVES; ++k) {
+      t.x += s_acc[k][threadIdx.x].x;
+      t.y += s_acc[k][threadIdx.x].y;
+    }
+    dpart[(int64_t)blockIdx.x * 64 + threadIdx.x] = t;
+  }
+}
+
+// rho[p] = (rho00, rho11, re rho01, im rho01): one wave per bit, partials
+// summed in a fixed order.
+__global__ __launch_bounds__(NT) void k_rdm_ordered_fin(const double2* __restrict__ part, int64_t ptiles,
+                                                         const double2* __restrict__ dpart, int64_t dtiles, int nbits,
+                                                         double* rho) {
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (p >= nbits) return;
+  double z = 0.0, o = 0.0, re = 0.0, im = 0.0;
+  for (int64_t u = lane; u < dtiles; u += 64) {
+    const double2 v = dpart[u * 64 + p];
+    z += v.x;
+    o += v.y;
+  }
+  for (int64_t u = lane; u < ptiles; u += 64) {
+    const double2 v = part[(int64_t)p * ptiles + u];
+    re += v.x;
+    im += v.y;
+  }
+  z = wave_sum(z);
+  o = wave_sum(o);
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if (lane == 0) {
+    rho[4 * p + 0] = z;
+    rho[4 * p + 1] = o;
+    rho[4 * p + 2] = re;
+    rho[4 * p + 3] = im;
+  }
+}
+
+size_t rdm_ordered_scratch(int64_t n, int nbits) {
+  const int64_t ptiles = (n + RDM_TM - 1) / RDM_TM, dtiles = (n + SEG_ENTRIES - 1) / SEG_ENTRIES;
+  return (size_t)nbits * (ptiles + 1) * sizeof(LiftBound) + (size_t)nbits * ptiles * 16 + (size_t)dtiles * 64 * 16 +
+         3 * 256;
+}
+
+int launch_rdm_ordered(const uint64_t* x, const double2* a, int64_t n, int nbits, void* scratch, double* rho,
+                       hipStream_t s) {
+  const int64_t ptiles = (n + RDM_TM - 1) / RDM_TM, dtiles = (n + SEG_ENTRIES - 1) / SEG_ENTRIES;
+  const int64_t nbound = ptiles + 1;
+  char* base = (char*)scratch;
+  LiftBound* bounds = (LiftBound*)base;
+  base += ((size_t)nbits * nbound * sizeof(LiftBound) + 255) & ~(size_t)255;
+  double2* part = (double2*)base;
+  base += ((size_t)nbits * ptiles * 16 + 255) & ~(size_t)255;
+  double2* dpart = (double2*)base;
+  hipLaunchKernelGGL(k_lift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES), (unsigned)nbits), dim3(NT), 0, s,
+                     x, n, 0, 0, RDM_TM, nbound, bounds);
+  hipLaunchKernelGGL(k_rdm_pairs, dim3((unsigned)ptiles, (unsigned)nbits), dim3(NT), 0, s, x, a, n, nbound, bounds,
+                     part);
+  hipLaunchKernelGGL(k_rdm_diag, dim3((unsigned)dtiles), dim3(NT), 0, s, x, a, n, nbits, dpart);
+  hipLaunchKernelGGL(k_rdm_ordered_fin, dim3((unsigned)((nbits + WAVES - 1) / WAVES)), dim3(NT), 0, s, part, ptiles,
+                     dpart, dtiles, nbits, rho);
+  return (int)hipGetLastError();
 }
 
 // ----------------------------------------------------------- host launch ---

@@ -340,7 +340,7 @@ class BASS:
         ctx = self._ctx
         cand = state._tmp
         for _pass in range(3):
-            all_rho = _ops.rdm1q(ctx, state._buf, state.nnz, self.N)[state.pos]
+            all_rho = _ops.rdm1q(ctx, state._buf, state.nnz, self.N, ordered=True)[state.pos]
             self._count("rdm")
             pr = self._pr(state)
             improved = False

@@ -156,6 +156,14 @@ int bass_rdm1q(const uint64_t* x, const double* a, int64_t n, int N, double* rho
  * (src/simulation/bass_simulator.py:100-153) with batched device calls.
  * The slot hash differs from the reference's key & (cap-1); lookups return the
  * same indices. */
+/* Single-qubit RDMs of a key-ordered state (x sorted ascending by stored
+ * word; see bass_lift), one per stored bit p < N, same output layout as
+ * bass_rdm1q.  Pairs (x, x | 2^p) are found by the merge that lifts bit p, so
+ * no hash table is built.  Replaces compute_all_1qubit_rdms
+ * (src/simulation/bass_simulator.py:441-470) inside the driver.  Synchronous. */
+int bass_rdm1q_ordered(const uint64_t* x, const double* a, int64_t n, int N, double* rho_out, void* ws,
+                       size_t ws_bytes, void* stream);
+
 int bass_ht_clear(uint64_t* ht_keys, int32_t* ht_vals, int log2cap, void* stream);
 int bass_ht_insert(uint64_t* ht_keys, int32_t* ht_vals, int log2cap, const uint64_t* keys,
                    int64_t n, int64_t index_base, void* ws, size_t ws_bytes, void* stream);

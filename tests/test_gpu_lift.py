@@ -203,3 +203,28 @@ def test_truncate_segmented_keeps_all_up_to_k(gpu_ctx, n):
         else:
             assert st.nnz == k
             assert abs(np.sum(np.abs(dev.to_host_amps(dst.a, k)) ** 2) - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("n,nbits", [(1, 5), (7, 4), (5000, 14), (70000, 40)])
+def test_rdm_ordered(gpu_ctx, n, nbits):
+    """RDMs of every stored bit of an ordered state vs the oracle's (hash) RDMs."""
+    ops, dev, SS, L = _mods()
+    print(f"[case] rdm ordered n={n} nbits={nbits}", flush=True)
+    rng = np.random.default_rng(n + nbits)
+    if nbits <= 16:
+        x, a = random_state(n, nbits, seed=n + 3)
+    else:
+        # clustered keys so that many pairs exist on every bit
+        base = rng.choice(2 ** 20, size=n // 8, replace=False).astype(np.uint64)
+        hi = rng.choice(2 ** (nbits - 20), size=8, replace=False).astype(np.uint64) << np.uint64(20)
+        x = np.unique((base[:, None] | hi[None, :]).ravel())[:n]
+        n = len(x)
+        a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        a /= np.linalg.norm(a)
+    pos = rng.permutation(nbits)
+    w, aw = _ordered(x, a, pos)
+    src = _upload(dev, gpu_ctx, w, aw)
+    rho_st = ops.rdm1q(gpu_ctx, src, n, nbits, ordered=True)
+    rho = rho_st[pos]
+    ref = OK.compute_all_1qubit_rdms(x, a, n, nbits)
+    np.testing.assert_allclose(rho, ref, rtol=1e-12, atol=1e-15)
