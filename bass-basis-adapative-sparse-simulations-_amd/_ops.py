@@ -97,10 +97,18 @@ def truncate(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False, ke
     else:
         km = np.ascontiguousarray(key_map, dtype=np.int32)
         kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
+    flags = 1 if segmented else 0
     rc = ctx.lib.bass_truncate(src.xp, src.ap, n, k, dst.xp, dst.ap, dst.cap, 1 if renorm else 0,
-                               hist_tiles, 1 if segmented else 0, kp, kb, ctx.wsp, ctx.ws_bytes, ctx.stream)
+                               hist_tiles, flags, kp, kb, ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_truncate")
-    return ctx.read_stats("bass_truncate")
+    st = ctx.read_stats("bass_truncate")
+    if st.flags & 2:
+        # very many entries tie at the threshold: repeat with the multi-block path
+        rc = ctx.lib.bass_truncate(src.xp, src.ap, n, k, dst.xp, dst.ap, dst.cap, 1 if renorm else 0,
+                                   hist_tiles, flags | 2, kp, kb, ctx.wsp, ctx.ws_bytes, ctx.stream)
+        L.check(rc, "bass_truncate")
+        st = ctx.read_stats("bass_truncate")
+    return st
 
 
 def diag2q(ctx, buf, n, q1, q2, diag):

@@ -199,12 +199,15 @@ class SparseState:
 
     RADIX_THRESHOLD = 2000
 
-    def __init__(self, num_qubits, k, init_state=None, device=None, dedup="table"):
+    def __init__(self, num_qubits, k, init_state=None, device=None, dedup="table", capacity=None):
         self.N = num_qubits
         self.k = k
         self._ctx = context(device)
-        self._buf = Buffers(max(k, 1), self._ctx.device)
-        self._tmp = Buffers(max(4 * k, 4), self._ctx.device)
+        # capacity: initial entries of the state buffer (default k); both
+        # buffers grow on demand
+        cap = k if capacity is None else min(k, capacity)
+        self._buf = Buffers(max(cap, 1), self._ctx.device)
+        self._tmp = Buffers(max(4 * cap, 4), self._ctx.device)
         self.gamma = 1.0
         if init_state is None:
             init_state = 0

@@ -40,7 +40,7 @@ struct Layout {
   size_t hist1, tick_gate, nout;
   size_t hdrA_end;
   // header B (zeroed by every truncate / auxiliary call)
-  size_t hist2, sel, tick_sel, hdrB_end;
+  size_t hist2, sel, tick_sel, hist_w, hdrB_end;
   // persistent
   size_t st, tile_psum, tile_dsum, tile_s2, tile_s4, rdm_part, rho, sort_hist, sort_tick;
   size_t lb_gate, lb1, sort_status;
@@ -62,6 +62,7 @@ static Layout layout(int64_t S) {
   L.hist2 = o; o += al(HIST_BINS * 4);
   L.sel = o; o += al(sizeof(SelState));
   L.tick_sel = o; o += al(16 * 4);
+  L.hist_w = o; o += al((size_t)WIDE_LEVELS * HIST_BINS * 4);
   L.hdrB_end = o;
   L.st = o; o += al(sizeof(OpStats));
   L.tile_psum = o; o += al(etiles * 8);
@@ -347,8 +348,10 @@ int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t
 }
 
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
-                  int64_t cap_out, int renorm, int64_t hist_tiles, int segmented, const int32_t* key_map,
+                  int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags, const int32_t* key_map,
                   int key_bits, void* ws, size_t ws_bytes, void* stream) {
+  const bool segmented = (in_flags & 1) != 0;
+  const bool wide = (in_flags & 2) != 0;
   if (k < 1 || (!segmented && n <= k))
     return fail(BASS_E_ARG, "bass_truncate: need n > k >= 1 (n=%lld k=%lld)", (long long)n, (long long)k);
   if (cap_out < k) return fail(BASS_E_OVERFLOW, "bass_truncate: cap_out < k");
@@ -397,11 +400,14 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   S_.st = st;
   char* big = at<char>(ws, L.big);
   const size_t p8 = al((size_t)8 * slots), p4 = al((size_t)4 * slots);
-  if (2 * p8 + 2 * p4 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_truncate: candidates do not fit");
+  if (3 * p8 + 2 * p4 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_truncate: candidates do not fit");
   S_.c1p = (unsigned long long*)big;
   S_.c1i = (unsigned int*)(big + p8);
   S_.c2p = (unsigned long long*)(big + p8 + p4);
   S_.c2i = (unsigned int*)(big + 2 * p8 + p4);
+  S_.tk = (uint64_t*)(big + 2 * p8 + 2 * p4);
+  S_.wide = wide;
+  S_.hist_w = at<unsigned int>(ws, L.hist_w);
   S_.cnt1 = at<int>(ws, L.cnt1);
   S_.tile_dsum = at<double>(ws, L.tile_dsum);
   S_.tile_dcnt = at<int>(ws, L.tile_dcnt);

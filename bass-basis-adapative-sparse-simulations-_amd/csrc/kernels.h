@@ -23,7 +23,17 @@ struct SelState {
   unsigned int b1, b2;           // threshold buckets of digits 1 and 2
   unsigned int keepall;          // the input holds <= k entries: keep all, no renormalisation
   unsigned long long def_count;  // elements above the digit-1 bucket
+  unsigned long long cnt_eq;     // wide path: entries equal to the resolved prefix
+  unsigned long long kp;         // wide path: resolved high bits of the key threshold
+  unsigned int tcount;           // wide path: tied candidates collected
+  unsigned int tie;              // wide path: the key tie-break is needed
 };
+
+// k_finish resolves at most FIN_MAX digit-2 candidates in one block; beyond
+// that the call reports OpStats.flags bit 1 and is repeated with the wide
+// (multi-block) path.
+constexpr unsigned int FIN_MAX = 1u << 16;
+constexpr int WIDE_LEVELS = 10;  // 4 |a|^2 digits + 6 key digits
 
 // Scalars an operation reports back to the host.
 struct OpStats {
@@ -91,6 +101,9 @@ struct SelLaunch {
   double* tile_psum;
   unsigned int* hist2;   // zeroed
   unsigned short* hist2_rows;  // ntiles * HIST_BINS
+  bool wide;             // resolve the digit-2 candidates with multi-block passes
+  unsigned int* hist_w;  // WIDE_LEVELS * HIST_BINS, zeroed
+  uint64_t* tk;          // tied logical keys (wide path), one per slot
   SelState* sel;         // zeroed
   OpStats* st;
   unsigned long long* c1p;  // ntiles * SEG_ENTRIES

@@ -367,6 +367,15 @@ __global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restr
   __shared__ unsigned long long s_ab;
   if (sel->keepall) return;  // pstar / keystar set by k_find1
   const unsigned int m2 = sel->cand2;
+  if (m2 > FIN_MAX) {
+    // too many for one block: keep nothing, ask for the wide path
+    if (threadIdx.x == 0) {
+      sel->pstar = ~0ull;
+      sel->keystar = 0;
+      atomicOr(&st->flags, 2);
+    }
+    return;
+  }
   unsigned long long need = sel->need;
   unsigned long long prefix = sel->prefix;
   const int shifts[4] = {30, 19, 8, 0};
@@ -436,6 +445,117 @@ __global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restr
 __device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t w, unsigned long long pstar,
                                           unsigned long long keystar, const KeyMap& km) {
   return bits > pstar || (bits == pstar && logical_key(w, km) <= keystar);
+}
+
+// ------------------------------------------------------------- wide path --
+// Multi-block resolution of the digit-2 candidates (c2p / c2i, count
+// sel->cand2), for inputs with very many equal |a|^2 near the threshold.
+// |a|^2 digits: bits 30..40, 19..29, 8..18, 0..7; then, if entries equal to
+// the threshold must be split, a radix select of the smallest logical keys
+// over the collected tied keys: bits 53..63, 42..52, 31..41, 20..30, 9..19, 0..8.
+constexpr int WIDE_BLOCKS = 1024;
+__device__ __constant__ int kWideShift[WIDE_LEVELS] = {30, 19, 8, 0, 53, 42, 31, 20, 9, 0};
+__device__ __constant__ int kWideWidth[WIDE_LEVELS] = {11, 11, 11, 8, 11, 11, 11, 11, 11, 9};
+
+__device__ __forceinline__ void flush_hist(unsigned int* s_hist, unsigned int* hist) {
+  __syncthreads();
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT)
+    if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
+}
+
+__global__ __launch_bounds__(NT) void k_wide_hist(const unsigned long long* __restrict__ c2p, const SelState* sel,
+                                                   unsigned int* hist_w, int lv) {
+  __shared__ unsigned int s_hist[HIST_BINS];
+  if (sel->keepall) return;
+  const int sh = kWideShift[lv], wd = kWideWidth[lv];
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  __syncthreads();
+  const unsigned int m2 = sel->cand2;
+  const unsigned long long hi = sel->prefix >> (sh + wd);
+  for (unsigned int j = blockIdx.x * NT + threadIdx.x; j < m2; j += gridDim.x * NT) {
+    const unsigned long long bits = c2p[j];
+    if ((bits >> (sh + wd)) == hi) atomicAdd(&s_hist[(unsigned int)(bits >> sh) & ((1u << wd) - 1u)], 1u);
+  }
+  flush_hist(s_hist, hist_w + (int64_t)lv * HIST_BINS);
+}
+
+__global__ __launch_bounds__(NTB) void k_wide_find(const unsigned int* hist_w, SelState* sel, OpStats* st, int lv) {
+  __shared__ int s_scan[WAVESB];
+  __shared__ unsigned int s_b;
+  __shared__ unsigned long long s_ab;
+  if (sel->keepall) return;
+  if (lv >= 4 && !sel->tie) return;
+  const int sh = kWideShift[lv];
+  const unsigned int* hist = hist_w + (int64_t)lv * HIST_BINS;
+  if (threadIdx.x == 0) {
+    s_b = 0xFFFFFFFFu;
+    s_ab = 0;
+  }
+  __syncthreads();
+  const unsigned long long need = sel->need;
+  if (lv < 4)
+    find_from_top<NTB>(hist, need, &s_b, &s_ab, s_scan);
+  else
+    find_from_bottom<NTB>(hist, need, &s_b, &s_ab, s_scan);
+  if (threadIdx.x != 0) return;
+  if (s_b == 0xFFFFFFFFu) {
+    set_err(&st->err, ERR_SELECT);
+    return;
+  }
+  sel->need = need - s_ab;
+  if (lv < 4) {
+    sel->prefix |= (unsigned long long)s_b << sh;
+    sel->cnt_eq = hist[s_b];
+    if (lv == 3) {
+      sel->pstar = sel->prefix;
+      sel->tie = sel->need < sel->cnt_eq ? 1u : 0u;
+      sel->keystar = ~0ull;
+      sel->kp = 0;
+    }
+  } else {
+    sel->kp |= (unsigned long long)s_b << sh;
+    if (lv == WIDE_LEVELS - 1) sel->keystar = sel->kp;
+  }
+}
+
+// Logical keys of the candidates equal to the threshold (any order).
+__global__ __launch_bounds__(NT) void k_wide_collect(const unsigned long long* __restrict__ c2p,
+                                                      const unsigned int* __restrict__ c2i,
+                                                      const uint64_t* __restrict__ x, KeyMap km, SelState* sel,
+                                                      uint64_t* __restrict__ tk) {
+  if (sel->keepall || !sel->tie) return;
+  const unsigned int m2 = sel->cand2;
+  const unsigned long long pstar = sel->pstar;
+  const int lane = threadIdx.x & 63;
+  for (unsigned int j0 = blockIdx.x * NT; j0 < m2; j0 += gridDim.x * NT) {
+    const unsigned int j = j0 + threadIdx.x;
+    const bool take = j < m2 && c2p[j] == pstar;
+    const unsigned long long bal = __ballot(take);
+    if (!bal) continue;
+    unsigned int wb = 0;
+    const int leader = __ffsll((long long)bal) - 1;
+    if (lane == leader) wb = atomicAdd(&sel->tcount, (unsigned int)__popcll(bal));
+    wb = __shfl(wb, leader, 64);
+    if (take) tk[wb + __popcll(bal & ((1ull << lane) - 1ull))] = logical_key(x[c2i[j]], km);
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_wide_key_hist(const uint64_t* __restrict__ tk, const SelState* sel,
+                                                       unsigned int* hist_w, int lv) {
+  __shared__ unsigned int s_hist[HIST_BINS];
+  if (sel->keepall || !sel->tie) return;
+  const int sh = kWideShift[lv], wd = kWideWidth[lv];
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  __syncthreads();
+  const unsigned int m = sel->tcount;
+  const unsigned long long kp = sel->kp;
+  const int top = sh + wd;  // 64 at the first key level
+  for (unsigned int j = blockIdx.x * NT + threadIdx.x; j < m; j += gridDim.x * NT) {
+    const uint64_t key = tk[j];
+    if (top >= 64 || (key >> top) == (kp >> top))
+      atomicAdd(&s_hist[(unsigned int)(key >> sh) & ((1u << wd) - 1u)], 1u);
+  }
+  flush_hist(s_hist, hist_w + (int64_t)lv * HIST_BINS);
 }
 
 // Per tile: sum and count of the kept candidates (in tile order).
@@ -515,6 +635,9 @@ __global__ __launch_bounds__(NTB) void k_sel_finalize(const double* tile_dsum, c
     st->nnz = carry;
     if (sel->keepall) {
       st->kept = st->total;
+      st->norm = 1.0;
+    } else if (st->flags & 2) {
+      st->kept = 0.0;  // result discarded; the caller repeats with the wide path
       st->norm = 1.0;
     } else {
       if ((unsigned long long)carry != k) set_err(&st->err, ERR_SELECT);
@@ -697,7 +820,19 @@ int launch_select(const SelLaunch& L, hipStream_t s) {
   int64_t eb = tiles < 1024 ? tiles : 1024;
   hipLaunchKernelGGL(k_extract2, dim3((unsigned)eb), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, tiles, L.sel, L.c2p,
                      L.c2i);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(NT), 0, s, L.c2p, L.c2i, L.x, L.km, L.sel, L.st);
+  if (!L.wide) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(NT), 0, s, L.c2p, L.c2i, L.x, L.km, L.sel, L.st);
+  } else {
+    for (int lv = 0; lv < 4; ++lv) {
+      hipLaunchKernelGGL(k_wide_hist, dim3(WIDE_BLOCKS), dim3(NT), 0, s, L.c2p, L.sel, L.hist_w, lv);
+      hipLaunchKernelGGL(k_wide_find, dim3(1), dim3(NTB), 0, s, L.hist_w, L.sel, L.st, lv);
+    }
+    hipLaunchKernelGGL(k_wide_collect, dim3(WIDE_BLOCKS), dim3(NT), 0, s, L.c2p, L.c2i, L.x, L.km, L.sel, L.tk);
+    for (int lv = 4; lv < WIDE_LEVELS; ++lv) {
+      hipLaunchKernelGGL(k_wide_key_hist, dim3(WIDE_BLOCKS), dim3(NT), 0, s, L.tk, L.sel, L.hist_w, lv);
+      hipLaunchKernelGGL(k_wide_find, dim3(1), dim3(NTB), 0, s, L.hist_w, L.sel, L.st, lv);
+    }
+  }
   hipLaunchKernelGGL(k_keptsum, dim3((unsigned)tiles), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, L.x, L.km, L.sel,
                      L.ksum, L.kcnt);
   hipLaunchKernelGGL(k_sel_finalize, dim3(1), dim3(NTB), 0, s, L.tile_dsum, L.ksum, L.tile_dcnt, L.kcnt, tiles,

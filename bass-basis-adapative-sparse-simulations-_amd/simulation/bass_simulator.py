@@ -444,7 +444,9 @@ class BASS:
         self.counters = {"amp_updates": 0}
         self._hard_cap = min(self.max_nnz_factor * self.k, 2 ** self.N)
         internal_k = min(max(max(self.buffer_factor, 2) * self.k, self._hard_cap), 2 ** self.N)
-        state = SparseState(self.N, internal_k, initial_state, device=self._ctx.device)
+        # buffers start at the truncation size and grow on demand (internal_k
+        # only bounds the rarely-used SparseState budget)
+        state = SparseState(self.N, internal_k, initial_state, device=self._ctx.device, capacity=self.k)
         state._rot = Buffers(2 * self.k + 2, self._ctx.device)
         state.pos = np.arange(self.N)
         state._sum2 = state._sum4 = None

@@ -1,10 +1,20 @@
 #!/usr/bin/env python3
 """Plain-path benchmark of the BASS sparse-state core on one MI355X.
 
-Workload: brickwork random circuit, n=40 qubits, each layer a Haar-random U(2)
-on every qubit then CZ on alternating even/odd bonds, K=2^22, complex128
-amplitudes, uint64 keys, top-k truncation after every non-diagonal gate and the
-adaptive single-qubit basis optimisation at every layer boundary.
+Workloads (--workload):
+  brickwork     (default) random circuit, n=40 qubits, each layer a Haar-random
+                U(2) on every qubit then CZ on alternating even/odd bonds, 30
+                layers, K=2^22;
+  kicked-ising  periodic chain, n=56, each step RZZ(pi/4) on every bond then
+                RX(0.3 + 0.01 i) on every qubit, 60 steps, K=2^26 (a window of
+                steps is timed after the state saturates).
+Both: complex128 amplitudes, uint64 keys, top-k truncation after every
+non-diagonal gate, the adaptive single-qubit basis optimisation at every
+layer boundary.
+
+--calibrate-copy MIB times a plain device-to-device copy of MIB MiB (torch
+copy_) --steps times and prints its bytes; it is the reference kernel that
+scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes.
 
 A step is one circuit layer.  --warmup layers run untimed (the state saturates
 at K during the first layer), then --steps layers are timed between two
@@ -72,13 +82,28 @@ def cpu_baseline(keys, amps, n_qubits, k, gates, budget_s):
     return amp_updates / dt, done, dt
 
 
+def measured_traffic(workload, n, k, op):
+    """HBM bytes per launch of `op` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (line "traffic <op> <bytes>"), or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}{n}_K{k.bit_length() - 1}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic" and f[1] == op:
+            return float(f[2])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=25, help="timed layers")
     ap.add_argument("--warmup", type=int, default=5, help="untimed layers")
-    ap.add_argument("--qubits", type=int, default=40)
-    ap.add_argument("--log2k", type=int, default=22)
+    ap.add_argument("--workload", choices=["brickwork", "kicked-ising"], default="brickwork")
+    ap.add_argument("--qubits", type=int, default=None)
+    ap.add_argument("--log2k", type=int, default=None)
+    ap.add_argument("--calibrate-copy", type=float, default=None, metavar="MIB")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -90,23 +115,44 @@ def main():
     import numpy as np
     import torch
 
+    if args.calibrate_copy is not None:
+        nbytes = int(args.calibrate_copy * 2 ** 20)
+        src = torch.ones(nbytes, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": args.steps}), flush=True)
+        return
+
     lib = importlib.import_module(PKG + "._lib")
     rc = importlib.import_module(PKG + ".utils.random_circuits")
     B = importlib.import_module(PKG + ".simulation.bass_simulator")
     lib.load()
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
 
-    N, K = args.qubits, 1 << args.log2k
-    L = args.warmup + args.steps
-    circ, ends = rc.brickwork_haar_circuit(N, L, np.random.default_rng(SEED))
+    if args.workload == "brickwork":
+        N = args.qubits or 40
+        K = 1 << (args.log2k or 22)
+        L = args.warmup + args.steps
+        circ, ends = rc.brickwork_haar_circuit(N, L, np.random.default_rng(SEED))
+        wname = f"brickwork-haar n={N} layers={L} K=2^{K.bit_length() - 1} (step = 1 layer)"
+    else:
+        N = args.qubits or 56
+        K = 1 << (args.log2k or 26)
+        L = args.warmup + args.steps
+        circ, ends = rc.kicked_ising_circuit(N, L)
+        wname = f"kicked-ising n={N} periodic steps={L} K=2^{K.bit_length() - 1} (step = RZZ layer + RX layer)"
     starts = [0] + [e + 1 for e in ends[:-1]]
     sim = B.BASS(N, K, optimize_at=ends, device="cuda:0")
     sim.begin(len(circ))
-    log(f"workload brickwork-haar n={N} layers={L} K=2^{args.log2k} gates={len(circ)}")
+    log(f"workload {wname} gates={len(circ)}")
 
     for layer in range(args.warmup):
         for i in range(starts[layer], ends[layer] + 1):
             sim.step(i, circ[i])
+        log(f"warmup layer {layer} done nnz={sim.state.nnz}")
     torch.cuda.synchronize()
     log(f"warmup done: nnz={sim.state.nnz} counters={sim.counters}")
 
@@ -141,7 +187,8 @@ def main():
     dom = max((k for k in kernels if "achieved_GBps" in kernels[k]), key=lambda k: kernels[k]["total_ms"])
     d = kernels[dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": d["achieved_GBps"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": d["achieved_GBps"] / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": d["achieved_GBps"] / HBM_PEAK_GBS,
+                "traffic": measured_traffic(args.workload, N, K, dom),
                 "bytes_per_launch": d["bytes_per_launch"], "avg_ms": d["avg_ms"]}
 
     cpu = None
@@ -150,6 +197,11 @@ def main():
         st = sim.state
         keys = st.keys()
         amps = dev.to_host_amps(st._buf.a, st.nnz)
+        cpu_k = min(K, 1 << 22)  # larger states: the sample works on their largest 2^22 entries
+        if cpu_k < len(keys):
+            top = np.argsort(-np.abs(amps))[:cpu_k]
+            keys, amps = keys[top], amps[top]
+            amps = amps / np.linalg.norm(amps)
         gates = [g for g in circ if g.n_qubits == 1][: 64]
         # the sample applies the frame-rotated gates the driver would see
         gl = []
@@ -157,11 +209,13 @@ def main():
             q = g.qubits[0]
             G = sim.U[q].conj().T @ g.matrix @ sim.U[q]
             gl.append(type("G", (), {"qubits": (q,), "matrix": G, "n_qubits": 1})())
-        v, ng, cdt = cpu_baseline(keys, amps, N, K, gl, args.cpu_seconds)
+        v, ng, cdt = cpu_baseline(keys, amps, N, min(K, cpu_k), gl, args.cpu_seconds)
         cpu = {"value": v, "unit": "amp-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{ng} Haar 1q gates (emit+sort+merge+argpartition top-k+renorm) on the saturated "
-                         f"n={N} K=2^{args.log2k} state, numpy oracle, {cdt:.1f} s, "
-                         f"host affinity {len(os.sched_getaffinity(0))} cpus"}
+               "sample": f"{ng} 1q gates of the circuit (emit+sort+merge+argpartition top-k+renorm) on the "
+                         f"saturated {args.workload} n={N} state"
+                         + (f" (largest {len(keys)} entries, K=2^{cpu_k.bit_length() - 1})" if cpu_k < K else
+                            f" K=2^{K.bit_length() - 1}")
+                         + f", numpy oracle, {cdt:.1f} s, host affinity {len(os.sched_getaffinity(0))} cpus"}
 
     out = {
         "metric": "sparse amp-updates/s",
@@ -176,7 +230,7 @@ def main():
         "vs_baseline": None,
         "dtype": "complex128",
         "data": "synthetic",
-        "config": {"workload": f"brickwork-haar n={N} layers={L} K=2^{args.log2k} (step = 1 layer)",
+        "config": {"workload": wname,
                    "qubits": N, "k": K, "layers_timed": args.steps, "optimize": "every layer",
                    "gate_counts": {k: v for k, v in delta.items() if not k.startswith("bytes_")}},
         "roofline": roofline,

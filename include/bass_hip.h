@@ -119,14 +119,18 @@ int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, cons
  * driver truncations of src/simulation/bass_simulator.py:565-572, :899-916 and
  * src/simulation/simulator.py:169-201 (renorm=1).  Requires n > k.
  * hist_tiles: value reported by the bass_gate call that produced (x, a), or 0;
- * segmented=1: (x, a) is that call's segmented output; n is not needed
+ * in_flags bit 0 (segmented): (x, a) is that call's segmented output; n is not needed
  * (the count comes from the device) and the input may hold <= k entries:
  * then all are kept, in order and not renormalised, and stats.flags bit 0 is
  * clear -- the gate / truncate pair needs no host round trip in between.
+ * in_flags bit 1 (wide): resolve the threshold with multi-block passes.  A
+ * call without it whose threshold bucket holds more than 65536 entries
+ * (massive |a|^2 ties) keeps nothing and sets stats.flags bit 1; repeat it
+ * with bit 1 set (the ctypes layer does).
  * key_map (key_bits entries, or NULL): stored bit p of x holds qubit
  * key_map[p]; ties are then ordered by the key mapped back. */
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo,
-                  double* ao, int64_t cap_out, int renorm, int64_t hist_tiles, int segmented,
+                  double* ao, int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags,
                   const int32_t* key_map, int key_bits, void* ws, size_t ws_bytes, void* stream);
 
 /* In place: a[i] *= diag[(b_q1 << 1) | b_q2]; diag = 4 complex (8 doubles).

@@ -228,3 +228,34 @@ def test_rdm_ordered(gpu_ctx, n, nbits):
     rho = rho_st[pos]
     ref = OK.compute_all_1qubit_rdms(x, a, n, nbits)
     np.testing.assert_allclose(rho, ref, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("n,k,ordered", [(300000, 123457, False), (200000, 150001, True)])
+def test_truncate_massive_ties(gpu_ctx, n, k, ordered):
+    """|a|^2 with three distinct values: the threshold bucket holds far more
+    than one block resolves, so the call repeats with the multi-block path."""
+    ops, dev, SS, L = _mods()
+    print(f"[case] massive ties n={n} k={k} ordered={ordered}", flush=True)
+    nbits = 40
+    rng = np.random.default_rng(n)
+    x = np.unique(rng.integers(0, 2 ** nbits, size=n + 1000, dtype=np.uint64))[:n]
+    rng.shuffle(x)
+    vals = np.array([0.5, 0.25 + 0.25j, 0.125], dtype=np.complex128)
+    a = vals[rng.integers(0, 3, size=n)]
+    a[: n // 2] = vals[1]  # the threshold lands inside this tie class
+    if ordered:
+        pos = rng.permutation(nbits)
+        w, aw = _ordered(x, a, pos)
+        km = ops.key_map_arg(pos)
+    else:
+        w, aw, km = x, a, None
+    src = _upload(dev, gpu_ctx, w, aw)
+    dst = dev.Buffers(k, gpu_ctx.device)
+    st = ops.truncate(gpu_ctx, src, n, k, dst, renorm=False, key_map=km)
+    assert st.nnz == k and st.flags & 1
+    gw = dev.to_host_keys(dst.x, k)
+    gx = SS.stored_to_keys(gw, pos) if ordered else gw
+    ox, oa = x.copy(), a.copy()
+    OK.top_k_truncate_kernel_opt(ox, oa, n, k, selector=OD.key_tiebreak_selector)
+    assert set(gx.tolist()) == set(ox[:k].tolist())
+    assert_same_entries(gx, dev.to_host_amps(dst.a, k), ox[:k], oa[:k], exact=True)
