@@ -113,6 +113,23 @@ static int log2_ceil(int64_t v) {
   return l;
 }
 
+static SelTiles tiles_view(void* ws, const Layout& L, unsigned long long* c1p, unsigned int* c1i) {
+  SelTiles T;
+  T.tile_psum = at<double>(ws, L.tile_psum);
+  T.c1p = c1p;
+  T.c1i = c1i;
+  T.cnt1 = at<int>(ws, L.cnt1);
+  T.tile_dsum = at<double>(ws, L.tile_dsum);
+  T.tile_dcnt = at<int>(ws, L.tile_dcnt);
+  T.ksum = at<double>(ws, L.ksum);
+  T.kcnt = at<int>(ws, L.kcnt);
+  T.koff = at<int64_t>(ws, L.koff);
+  T.tile_s2 = at<double>(ws, L.tile_s2);
+  T.tile_s4 = at<double>(ws, L.tile_s4);
+  T.hist2_rows = at<unsigned short>(ws, L.hist2_rows);
+  return T;
+}
+
 // ------------------------------------------------------------- timing ----
 struct TimerSlot {
   std::string name;
@@ -378,46 +395,37 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   OpStats* st = at<OpStats>(ws, L.st);
   fill(at<void>(ws, L.hist2), L.hdrB_end - L.hist2, 0ull, s);
   if (hist_tiles <= 0) fill(at<void>(ws, L.hist1), HIST_BINS * 4, 0ull, s);
-  SelLaunch S_;
-  S_.x = x;
-  S_.a = (const double2*)a;
-  S_.n = n;
-  S_.ntiles = tiles;
-  S_.seg_cnt = segmented ? at<int>(ws, L.seg_cnt) : nullptr;
-  S_.k = (unsigned long long)k;
-  S_.xo = xo;
-  S_.ao = (double2*)ao;
-  S_.cap_out = cap_out;
-  S_.renorm = renorm;
-  S_.km = km;
-  S_.hist_ready = hist_tiles > 0;
-  S_.npsum = hist_tiles;
-  S_.hist1 = at<unsigned int>(ws, L.hist1);
-  S_.tile_psum = at<double>(ws, L.tile_psum);
-  S_.hist2 = at<unsigned int>(ws, L.hist2);
-  S_.hist2_rows = at<unsigned short>(ws, L.hist2_rows);
-  S_.sel = at<SelState>(ws, L.sel);
-  S_.st = st;
   char* big = at<char>(ws, L.big);
   const size_t p8 = al((size_t)8 * slots), p4 = al((size_t)4 * slots);
   if (3 * p8 + 2 * p4 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_truncate: candidates do not fit");
-  S_.c1p = (unsigned long long*)big;
-  S_.c1i = (unsigned int*)(big + p8);
-  S_.c2p = (unsigned long long*)(big + p8 + p4);
-  S_.c2i = (unsigned int*)(big + 2 * p8 + p4);
-  S_.tk = (uint64_t*)(big + 2 * p8 + 2 * p4);
-  S_.wide = wide;
-  S_.hist_w = at<unsigned int>(ws, L.hist_w);
-  S_.cnt1 = at<int>(ws, L.cnt1);
-  S_.tile_dsum = at<double>(ws, L.tile_dsum);
-  S_.tile_dcnt = at<int>(ws, L.tile_dcnt);
-  S_.ksum = at<double>(ws, L.ksum);
-  S_.kcnt = at<int>(ws, L.kcnt);
-  S_.koff = at<int64_t>(ws, L.koff);
-  S_.tile_s2 = at<double>(ws, L.tile_s2);
-  S_.tile_s4 = at<double>(ws, L.tile_s4);
+  SelProblem P;
+  memset(&P, 0, sizeof(P));
+  P.x = x;
+  P.a = (const double2*)a;
+  P.seg_cnt = segmented ? at<int>(ws, L.seg_cnt) : nullptr;
+  P.n_dense = n;
+  P.tile0 = 0;
+  P.ntiles = tiles;
+  P.row0 = 0;
+  P.nrows = select_rows(tiles, false);
+  P.npsum = hist_tiles > 0 ? hist_tiles : tiles;
+  P.k = (unsigned long long)k;
+  P.xo = xo;
+  P.ao = (double2*)ao;
+  P.cap_out = cap_out;
+  P.renorm = renorm;
+  P.hist1 = at<unsigned int>(ws, L.hist1);
+  P.hist2 = at<unsigned int>(ws, L.hist2);
+  P.hist_w = at<unsigned int>(ws, L.hist_w);
+  P.sel = at<SelState>(ws, L.sel);
+  P.st = st;
+  P.c2p = (unsigned long long*)(big + p8 + p4);
+  P.c2i = (unsigned int*)(big + 2 * p8 + p4);
+  P.tk = (uint64_t*)(big + 2 * p8 + 2 * p4);
+  P.km = km;
+  const SelTiles T = tiles_view(ws, L, (unsigned long long*)big, (unsigned int*)(big + p8));
   const int tk = ktimer_begin("select", s);
-  launch_select(S_, s);
+  launch_select(P, T, hist_tiles > 0, wide, s);
   ktimer_end(tk, s);
   return hip_check("bass_truncate");
 }

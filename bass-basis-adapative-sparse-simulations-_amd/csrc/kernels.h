@@ -83,34 +83,40 @@ struct KeyMap {
   unsigned char q[64];
 };
 
-struct SelLaunch {
-  const uint64_t* x;
+// One top-k truncation problem: its input tiles [tile0, tile0 + ntiles) of the
+// per-tile scratch (SelTiles), its output and its own bookkeeping.
+struct SelProblem {
+  const uint64_t* x;     // input keys (segmented: the gate's tile buffer, indexed by global slot)
   const double2* a;
-  int64_t n;             // dense entry count (used when seg_cnt is null)
-  int64_t ntiles;        // tiles of SEG_ENTRIES slots
-  const int* seg_cnt;    // per-tile entry counts of a segmented input, or null
+  const int* seg_cnt;    // per global tile counts of a segmented input, or null (dense)
+  int64_t n_dense;       // dense input entries
+  int64_t tile0, ntiles; // global tiles of the input
+  int64_t row0, nrows;   // digit-2 histogram rows of the input
+  int64_t npsum;         // tile_psum entries from tile0 holding this input's |a|^2 sums
   unsigned long long k;
   uint64_t* xo;
   double2* ao;
   int64_t cap_out;
   int renorm;
-  KeyMap km;             // ties are broken by the logical key
-  bool hist_ready;       // hist1 + tile_psum already hold this input's digit-1 data
-  int64_t npsum;         // number of valid tile_psum entries when hist_ready
-  unsigned int* hist1;   // zeroed by the caller unless hist_ready
-  double* tile_psum;
-  unsigned int* hist2;   // zeroed
-  unsigned short* hist2_rows;  // ntiles * HIST_BINS
-  bool wide;             // resolve the digit-2 candidates with multi-block passes
-  unsigned int* hist_w;  // WIDE_LEVELS * HIST_BINS, zeroed
-  uint64_t* tk;          // tied logical keys (wide path), one per slot
+  int pad;
+  unsigned int* hist1;   // HIST_BINS, zeroed by the caller unless filled by the producing gate
+  unsigned int* hist2;   // HIST_BINS, zeroed
+  unsigned int* hist_w;  // HIST_BINS, zeroed (wide path)
   SelState* sel;         // zeroed
   OpStats* st;
-  unsigned long long* c1p;  // ntiles * SEG_ENTRIES
+  unsigned long long* c2p;  // >= ntiles * SEG_ENTRIES slots
+  unsigned int* c2i;
+  uint64_t* tk;             // >= ntiles * SEG_ENTRIES
+  KeyMap km;             // ties are broken by the logical key
+};
+
+// Per-tile scratch shared by all problems of a launch (indexed by global tile;
+// c1p / c1i by global slot).
+struct SelTiles {
+  double* tile_psum;
+  unsigned long long* c1p;
   unsigned int* c1i;
   int* cnt1;
-  unsigned long long* c2p;
-  unsigned int* c2i;
   double* tile_dsum;
   int* tile_dcnt;
   double* ksum;
@@ -118,10 +124,14 @@ struct SelLaunch {
   int64_t* koff;
   double* tile_s2;
   double* tile_s4;
+  unsigned short* hist2_rows;
 };
 
+int select_rows(int64_t ntiles, bool batch);
+int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s);
+int launch_select_batch(const SelProblem* d_probs, int nprob, const SelTiles& T, int64_t tiles_end,
+                        int64_t max_rows, bool hist_ready, bool wide, hipStream_t s);
 int64_t sel_tiles(int64_t n);
-int launch_select(const SelLaunch& L, hipStream_t s);
 int launch_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff, OpStats* st, hipStream_t s);
 int launch_densify(const uint64_t* x, const double2* a, const int* seg_cnt, const int64_t* soff, int64_t ntiles,
                    uint64_t* xo, double2* ao, hipStream_t s);

@@ -4,14 +4,17 @@
 // (the reference leaves ties to np.argpartition, whose choice is implementation
 // defined; src/core/truncation.py:68-113, src/simulation/bass_simulator.py:899-916).
 // Entries of a key-ordered state store permuted keys; a KeyMap maps them back
-// for the tie-break.
-// |a|^2 >= 0, so its bit pattern orders like the value.
+// for the tie-break.  |a|^2 >= 0, so its bit pattern orders like the value.
 //
-// Input is a sequence of tiles of at most SEG_ENTRIES entries: tile t holds
-// cnt_t entries at [t * SEG_ENTRIES, t * SEG_ENTRIES + cnt_t).  A dense array
-// is the special case cnt_t = min(SEG_ENTRIES, n - t * SEG_ENTRIES); the
-// segmented output of a gate supplies cnt_t per tile.  No kernel below
-// coordinates with other workgroups while it runs.
+// A truncation problem (SelProblem) reads a sequence of tiles of at most
+// SEG_ENTRIES entries: tile t holds cnt_t entries.  A dense array is the
+// special case cnt_t = min(SEG_ENTRIES, n - t * SEG_ENTRIES); the segmented
+// output of a gate supplies cnt_t per tile.  Per-tile scratch (SelTiles) is
+// indexed by global tile, so several problems -- a batch of circuits -- run in
+// one launch of every kernel: per-tile kernels find their problem from the
+// tile index, per-problem kernels take one block each.  One problem by value
+// (BATCH = false) or a device table of problems (BATCH = true) run the same
+// code.  No kernel coordinates with other workgroups while it runs.
 //
 // Passes over the full input (24 B per entry):
 //   digit 1 (sign + exponent) histogram: fused into the producing gate, or
@@ -23,6 +26,8 @@
 //     input order at offsets scanned from the per-tile kept counts, divided by
 //     sqrt(kept) when renormalising (24 B per kept entry).
 // Everything else touches only candidates or per-tile scalars.
+#include <string.h>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -41,12 +46,6 @@ __device__ __forceinline__ uint64_t logical_key(uint64_t w, const KeyMap& km) {
   uint64_t r = 0;
   for (int p = 0; p < km.nbits; ++p) r |= ((w >> p) & 1ull) << km.q[p];
   return r;
-}
-
-__device__ __forceinline__ int tile_count(const int* seg_cnt, int64_t n_dense, int64_t t) {
-  if (seg_cnt) return seg_cnt[t];
-  const int64_t r = n_dense - t * SEL_TILE;
-  return r < SEL_TILE ? (int)r : SEL_TILE;
 }
 
 // Find, in a 2048-bin histogram, the bucket holding the need-th largest element.
@@ -125,39 +124,87 @@ __device__ double block_sum_array(const double* v, int64_t m, double* lds) {
   return block_sum<NTHR>(s, lds);
 }
 
-__global__ __launch_bounds__(NT) void k_hist_psum(const double2* __restrict__ a, const int* seg_cnt,
-                                                   int64_t n_dense, unsigned int* hist, double* tile_psum) {
+constexpr int EX1_TILES = 4;  // tiles per k_extract1 block of a single problem
+
+// ------------------------------------------------------ problem lookup ----
+// Problem of a global tile: the last problem whose tile0 <= gt (problems are
+// stored in tile order).  Uniform per block.
+__device__ __forceinline__ int prob_of_tile(const SelProblem* probs, int nprob, int64_t gt) {
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].tile0 <= gt)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool BATCH>
+__device__ __forceinline__ const SelProblem& tile_problem(const SelProblem* probs, int nprob, const SelProblem& one,
+                                                          int64_t gt) {
+  if constexpr (BATCH)
+    return probs[prob_of_tile(probs, nprob, gt)];
+  else
+    return one;
+}
+
+template <bool BATCH>
+__device__ __forceinline__ const SelProblem& block_problem(const SelProblem* probs, const SelProblem& one,
+                                                           unsigned int b) {
+  if constexpr (BATCH)
+    return probs[b];
+  else
+    return one;
+}
+
+// entries of global tile gt of problem P, and their first input slot
+__device__ __forceinline__ int tile_count(const SelProblem& P, int64_t gt) {
+  if (P.seg_cnt) return P.seg_cnt[gt];
+  const int64_t r = P.n_dense - (gt - P.tile0) * SEL_TILE;
+  return r < SEL_TILE ? (int)r : SEL_TILE;
+}
+__device__ __forceinline__ int64_t tile_base(const SelProblem& P, int64_t gt) {
+  return (P.seg_cnt ? gt : gt - P.tile0) * SEL_TILE;
+}
+
+// ------------------------------------------------------------ kernels -----
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_hist_psum(const SelProblem* probs, int nprob, SelProblem one, SelTiles T) {
   __shared__ unsigned int s_hist[HIST_BINS];
   __shared__ double s_red[WAVES];
+  const int64_t gt = blockIdx.x;
+  const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt);
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
   __syncthreads();
-  const int cnt = tile_count(seg_cnt, n_dense, blockIdx.x);
-  const int64_t base = (int64_t)blockIdx.x * SEL_TILE;
+  const int cnt = tile_count(P, gt);
+  const int64_t base = tile_base(P, gt);
   double ps = 0.0;
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
     const int j = it * NT + threadIdx.x;
     if (j < cnt) {
-      const double p = cabs2(a[base + j]);
+      const double p = cabs2(P.a[base + j]);
       ps += p;
       atomicAdd(&s_hist[(unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1)], 1u);
     }
   }
   const double s = block_sum(ps, s_red);
-  if (threadIdx.x == 0) tile_psum[blockIdx.x] = s;
+  if (threadIdx.x == 0) T.tile_psum[gt] = s;
   __syncthreads();
   for (int b = threadIdx.x; b < HIST_BINS; b += NT)
-    if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
+    if (s_hist[b]) atomicAdd(&P.hist1[b], s_hist[b]);
 }
 
 // Digit 1: find the threshold bucket; also reduce the input's sum |a|^2.
-__global__ __launch_bounds__(NTB) void k_find1(const unsigned int* hist, unsigned long long k,
-                                                const double* tile_psum, int64_t npsum, SelState* sel,
-                                                OpStats* st) {
+template <bool BATCH>
+__global__ __launch_bounds__(NTB) void k_find1(const SelProblem* probs, SelProblem one, SelTiles T) {
   __shared__ int s_scan[WAVESB];
   __shared__ double s_red[WAVESB];
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_above;
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
   if (threadIdx.x == 0) {
     s_b = 0xFFFFFFFFu;
     s_above = 0;
@@ -165,13 +212,15 @@ __global__ __launch_bounds__(NTB) void k_find1(const unsigned int* hist, unsigne
   __syncthreads();
   // input count = sum of the digit-1 histogram
   int cnt = 0;
-  for (int b = threadIdx.x; b < HIST_BINS; b += NTB) cnt += (int)hist[b];
+  for (int b = threadIdx.x; b < HIST_BINS; b += NTB) cnt += (int)P.hist1[b];
   int count;
   (void)block_excl_scan<NTB>(cnt, count, s_scan);
-  const bool keepall = (unsigned long long)count <= k;
-  if (!keepall) find_from_top<NTB>(hist, k, &s_b, &s_above, s_scan);
-  const double total = block_sum_array<NTB>(tile_psum, npsum, s_red);
+  const bool keepall = (unsigned long long)count <= P.k;
+  if (!keepall) find_from_top<NTB>(P.hist1, P.k, &s_b, &s_above, s_scan);
+  const double total = block_sum_array<NTB>(T.tile_psum + P.tile0, P.npsum, s_red);
   if (threadIdx.x == 0) {
+    SelState* sel = P.sel;
+    OpStats* st = P.st;
     st->total = total;
     st->n_in = count;
     if (keepall) {
@@ -183,46 +232,48 @@ __global__ __launch_bounds__(NTB) void k_find1(const unsigned int* hist, unsigne
     } else {
       if (s_b == 0xFFFFFFFFu) set_err(&st->err, ERR_SELECT);
       sel->b1 = s_b;
-      sel->need = k - s_above;
+      sel->need = P.k - s_above;
       sel->prefix = (unsigned long long)s_b << 52;
       st->flags = 1;
     }
   }
 }
 
-constexpr int EX1_TILES = 4;  // tiles per k_extract1 block (one digit-2 row per block)
-
-__global__ __launch_bounds__(NT) void k_extract1(const double2* __restrict__ a, const int* seg_cnt,
-                                                  int64_t n_dense, int64_t ntiles, const SelState* sel,
-                                                  unsigned long long* __restrict__ c1p,
-                                                  unsigned int* __restrict__ c1i, int* __restrict__ cnt1,
-                                                  unsigned short* __restrict__ hist2_rows, double* tile_dsum,
-                                                  int* tile_dcnt) {
+// Per tile (EXT consecutive tiles per block, all of one problem): count / sum
+// above the digit-1 bucket, candidates inside it, and the block's digit-2
+// histogram as one row of hist2_rows.
+template <bool BATCH, int EXT>
+__global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
+                                                 int64_t tiles_end) {
   __shared__ unsigned int s_hist[HIST_BINS];
   __shared__ int s_scan[WAVES];
   __shared__ double s_red[WAVES];
   __shared__ int s_dcnt;
+  const int64_t gt0 = (int64_t)blockIdx.x * EXT;
+  const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt0);
+  const SelState* sel = P.sel;
   if (sel->keepall) {
     // every entry is kept: per-tile kept count = tile count, no candidates
-    for (int sub = threadIdx.x; sub < EX1_TILES; sub += NT) {
-      const int64_t tile = (int64_t)blockIdx.x * EX1_TILES + sub;
-      if (tile < ntiles) {
-        tile_dcnt[tile] = tile_count(seg_cnt, n_dense, tile);
-        tile_dsum[tile] = 0.0;
-        cnt1[tile] = 0;
+    for (int sub = threadIdx.x; sub < EXT; sub += NT) {
+      const int64_t gt = gt0 + sub;
+      if (gt < tiles_end) {
+        T.tile_dcnt[gt] = tile_count(P, gt);
+        T.tile_dsum[gt] = 0.0;
+        T.cnt1[gt] = 0;
       }
     }
     return;
   }
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
   const unsigned int b1 = sel->b1;
-  for (int sub = 0; sub < EX1_TILES; ++sub) {
-    const int64_t tile = (int64_t)blockIdx.x * EX1_TILES + sub;
-    if (tile >= ntiles) break;  // uniform
+  for (int sub = 0; sub < EXT; ++sub) {
+    const int64_t gt = gt0 + sub;
+    if (gt >= tiles_end) break;  // uniform
     if (threadIdx.x == 0) s_dcnt = 0;
     __syncthreads();
-    const int cnt = tile_count(seg_cnt, n_dense, tile);
-    const int64_t base = tile * SEL_TILE;
+    const int cnt = tile_count(P, gt);
+    const int64_t base = tile_base(P, gt);
+    const int64_t cbase = gt * SEL_TILE;
     unsigned long long cb[SEL_ITEMS];
     bool isc[SEL_ITEMS];
     double dsum = 0.0;
@@ -233,7 +284,7 @@ __global__ __launch_bounds__(NT) void k_extract1(const double2* __restrict__ a, 
       isc[it] = false;
       cb[it] = 0;
       if (j < cnt) {
-        const double p = cabs2(a[base + j]);
+        const double p = cabs2(P.a[base + j]);
         const unsigned long long bits = (unsigned long long)__double_as_longlong(p);
         const unsigned int d = (unsigned int)(bits >> 52);
         if (d > b1) {
@@ -252,8 +303,8 @@ __global__ __launch_bounds__(NT) void k_extract1(const double2* __restrict__ a, 
       int tot;
       const int off = block_excl_scan(isc[it] ? 1 : 0, tot, s_scan);
       if (isc[it]) {
-        c1p[base + rb + off] = cb[it];
-        c1i[base + rb + off] = (unsigned int)(base + it * NT + threadIdx.x);
+        T.c1p[cbase + rb + off] = cb[it];
+        T.c1i[cbase + rb + off] = (unsigned int)(base + it * NT + threadIdx.x);
       }
       rb += tot;
     }
@@ -261,47 +312,51 @@ __global__ __launch_bounds__(NT) void k_extract1(const double2* __restrict__ a, 
     atomicAdd(&s_dcnt, dcnt);
     __syncthreads();
     if (threadIdx.x == 0) {
-      tile_dsum[tile] = s;
-      tile_dcnt[tile] = s_dcnt;
-      cnt1[tile] = rb;
+      T.tile_dsum[gt] = s;
+      T.tile_dcnt[gt] = s_dcnt;
+      T.cnt1[gt] = rb;
     }
   }
   __syncthreads();
-  // the block's digit-2 histogram as one row (counts <= EX1_TILES * SEL_TILE
-  // fit 16 bits); k_hist2_reduce sums the rows.  About half of the bins of a
-  // row are nonzero, so per-bin global atomics would cost ~1000 per block.
-  unsigned short* row = hist2_rows + (int64_t)blockIdx.x * HIST_BINS;
+  // counts <= EXT * SEL_TILE fit 16 bits; k_hist2_reduce sums the rows (about
+  // half of a row's bins are nonzero, too many for per-bin global atomics)
+  unsigned short* row = T.hist2_rows + (int64_t)blockIdx.x * HIST_BINS;
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) row[b] = (unsigned short)s_hist[b];
 }
 
-// Column sums of the digit-2 rows: thread = 4 adjacent bins (one 8-byte load
-// per row), block = H2_ROWS rows.
+// Column sums of a problem's digit-2 rows: thread = 4 adjacent bins (one
+// 8-byte load per row), block = H2_ROWS rows; blockIdx.z = problem.
 constexpr int H2_ROWS = 32;
-__global__ __launch_bounds__(NT) void k_hist2_reduce(const unsigned short* __restrict__ rows, int64_t nrows,
-                                                      const SelState* sel, unsigned int* hist2) {
-  if (sel->keepall) return;
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_hist2_reduce(const SelProblem* probs, SelProblem one, SelTiles T) {
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.z);
+  if (P.sel->keepall) return;
   const int b = (blockIdx.x * NT + threadIdx.x) * 4;
   const int64_t r0 = (int64_t)blockIdx.y * H2_ROWS;
-  const int64_t r1 = r0 + H2_ROWS < nrows ? r0 + H2_ROWS : nrows;
+  if (r0 >= P.nrows) return;
+  const int64_t r1 = r0 + H2_ROWS < P.nrows ? r0 + H2_ROWS : P.nrows;
   unsigned int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll 8
   for (int64_t r = r0; r < r1; ++r) {
-    const ushort4 v = *reinterpret_cast<const ushort4*>(rows + r * HIST_BINS + b);
+    const ushort4 v = *reinterpret_cast<const ushort4*>(T.hist2_rows + (P.row0 + r) * HIST_BINS + b);
     s0 += v.x;
     s1 += v.y;
     s2 += v.z;
     s3 += v.w;
   }
-  if (s0) atomicAdd(&hist2[b], s0);
-  if (s1) atomicAdd(&hist2[b + 1], s1);
-  if (s2) atomicAdd(&hist2[b + 2], s2);
-  if (s3) atomicAdd(&hist2[b + 3], s3);
+  if (s0) atomicAdd(&P.hist2[b], s0);
+  if (s1) atomicAdd(&P.hist2[b + 1], s1);
+  if (s2) atomicAdd(&P.hist2[b + 2], s2);
+  if (s3) atomicAdd(&P.hist2[b + 3], s3);
 }
 
-__global__ __launch_bounds__(NT) void k_find2(const unsigned int* hist2, SelState* sel, OpStats* st) {
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_find2(const SelProblem* probs, SelProblem one) {
   __shared__ int s_scan[WAVES];
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_above;
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
+  SelState* sel = P.sel;
   if (threadIdx.x == 0) {
     s_b = 0xFFFFFFFFu;
     s_above = 0;
@@ -309,47 +364,45 @@ __global__ __launch_bounds__(NT) void k_find2(const unsigned int* hist2, SelStat
   __syncthreads();
   if (sel->keepall) return;
   const unsigned long long need = sel->need;
-  find_from_top(hist2, need, &s_b, &s_above, s_scan);
+  find_from_top(P.hist2, need, &s_b, &s_above, s_scan);
   if (threadIdx.x == 0) {
-    if (s_b == 0xFFFFFFFFu) set_err(&st->err, ERR_SELECT);
+    if (s_b == 0xFFFFFFFFu) set_err(&P.st->err, ERR_SELECT);
     sel->b2 = s_b;
     sel->need = need - s_above;
     sel->prefix |= (unsigned long long)s_b << 41;
   }
 }
 
-// Candidates of the digit-2 bucket, appended in any order (the threshold does
-// not depend on their order).
-__global__ __launch_bounds__(NT) void k_extract2(const unsigned long long* __restrict__ c1p,
-                                                  const unsigned int* __restrict__ c1i,
-                                                  const int* __restrict__ cnt1, int64_t ntiles, SelState* sel,
-                                                  unsigned long long* __restrict__ c2p,
-                                                  unsigned int* __restrict__ c2i) {
+// Candidates of the digit-2 bucket, appended to the problem's list in any
+// order (the threshold does not depend on their order).  One tile per block.
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_extract2(const SelProblem* probs, int nprob, SelProblem one, SelTiles T) {
+  const int64_t gt = blockIdx.x;
+  const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt);
+  SelState* sel = P.sel;
   if (sel->keepall) return;
   const unsigned int b2 = sel->b2;
   const int lane = threadIdx.x & 63;
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const int cnt = cnt1[t];
-    const int64_t base = t * SEL_TILE;
-    for (int j0 = 0; j0 < cnt; j0 += NT) {
-      const int j = j0 + threadIdx.x;
-      bool take = false;
-      unsigned long long bits = 0;
-      if (j < cnt) {
-        bits = c1p[base + j];
-        take = ((unsigned int)(bits >> 41) & (HIST_BINS - 1)) == b2;
-      }
-      const unsigned long long bal = __ballot(take);
-      if (bal) {
-        unsigned int wb = 0;
-        const int leader = __ffsll((long long)bal) - 1;
-        if (lane == leader) wb = atomicAdd(&sel->cand2, (unsigned int)__popcll(bal));
-        wb = __shfl(wb, leader, 64);
-        if (take) {
-          const unsigned int r = (unsigned int)__popcll(bal & ((1ull << lane) - 1ull));
-          c2p[wb + r] = bits;
-          c2i[wb + r] = c1i[base + j];
-        }
+  const int cnt = T.cnt1[gt];
+  const int64_t base = gt * SEL_TILE;
+  for (int j0 = 0; j0 < cnt; j0 += NT) {
+    const int j = j0 + threadIdx.x;
+    bool take = false;
+    unsigned long long bits = 0;
+    if (j < cnt) {
+      bits = T.c1p[base + j];
+      take = ((unsigned int)(bits >> 41) & (HIST_BINS - 1)) == b2;
+    }
+    const unsigned long long bal = __ballot(take);
+    if (bal) {
+      unsigned int wb = 0;
+      const int leader = __ffsll((long long)bal) - 1;
+      if (lane == leader) wb = atomicAdd(&sel->cand2, (unsigned int)__popcll(bal));
+      wb = __shfl(wb, leader, 64);
+      if (take) {
+        const unsigned int r = (unsigned int)__popcll(bal & ((1ull << lane) - 1ull));
+        P.c2p[wb + r] = bits;
+        P.c2i[wb + r] = T.c1i[base + j];
       }
     }
   }
@@ -357,14 +410,15 @@ __global__ __launch_bounds__(NT) void k_extract2(const unsigned long long* __res
 
 // Resolve the remaining bits of the threshold from the (small) digit-2
 // candidate list, then the key tie-break among entries equal to it.
-__global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restrict__ c2p,
-                                                const unsigned int* __restrict__ c2i,
-                                                const uint64_t* __restrict__ x, KeyMap km,
-                                                SelState* sel, OpStats* st) {
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_finish(const SelProblem* probs, SelProblem one) {
   __shared__ unsigned int s_hist[HIST_BINS];
   __shared__ int s_scan[WAVES];
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_ab;
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
+  SelState* sel = P.sel;
+  OpStats* st = P.st;
   if (sel->keepall) return;  // pstar / keystar set by k_find1
   const unsigned int m2 = sel->cand2;
   if (m2 > FIN_MAX) {
@@ -376,6 +430,8 @@ __global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restr
     }
     return;
   }
+  const unsigned long long* c2p = P.c2p;
+  const unsigned int* c2i = P.c2i;
   unsigned long long need = sel->need;
   unsigned long long prefix = sel->prefix;
   const int shifts[4] = {30, 19, 8, 0};
@@ -420,7 +476,7 @@ __global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restr
       __syncthreads();
       for (unsigned int j = threadIdx.x; j < m2; j += NT) {
         if (c2p[j] != pstar) continue;
-        const uint64_t key = logical_key(x[c2i[j]], km);
+        const uint64_t key = logical_key(P.x[c2i[j]], P.km);
         const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
         if (match) atomicAdd(&s_hist[(unsigned int)(key >> sh) & 255u], 1u);
       }
@@ -442,18 +498,14 @@ __global__ __launch_bounds__(NT) void k_finish(const unsigned long long* __restr
   }
 }
 
-__device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t w, unsigned long long pstar,
-                                          unsigned long long keystar, const KeyMap& km) {
-  return bits > pstar || (bits == pstar && logical_key(w, km) <= keystar);
-}
-
 // ------------------------------------------------------------- wide path --
-// Multi-block resolution of the digit-2 candidates (c2p / c2i, count
+// Multi-block resolution of the digit-2 candidates (P.c2p / P.c2i, count
 // sel->cand2), for inputs with very many equal |a|^2 near the threshold.
 // |a|^2 digits: bits 30..40, 19..29, 8..18, 0..7; then, if entries equal to
 // the threshold must be split, a radix select of the smallest logical keys
 // over the collected tied keys: bits 53..63, 42..52, 31..41, 20..30, 9..19, 0..8.
-constexpr int WIDE_BLOCKS = 1024;
+// One histogram per problem (P.hist_w): each find clears it for the next level.
+constexpr int WIDE_BLOCKS = 256;
 __device__ __constant__ int kWideShift[WIDE_LEVELS] = {30, 19, 8, 0, 53, 42, 31, 20, 9, 0};
 __device__ __constant__ int kWideWidth[WIDE_LEVELS] = {11, 11, 11, 8, 11, 11, 11, 11, 11, 9};
 
@@ -463,9 +515,11 @@ __device__ __forceinline__ void flush_hist(unsigned int* s_hist, unsigned int* h
     if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
 }
 
-__global__ __launch_bounds__(NT) void k_wide_hist(const unsigned long long* __restrict__ c2p, const SelState* sel,
-                                                   unsigned int* hist_w, int lv) {
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_wide_hist(const SelProblem* probs, SelProblem one, int lv) {
   __shared__ unsigned int s_hist[HIST_BINS];
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.y);
+  const SelState* sel = P.sel;
   if (sel->keepall) return;
   const int sh = kWideShift[lv], wd = kWideWidth[lv];
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
@@ -473,20 +527,23 @@ __global__ __launch_bounds__(NT) void k_wide_hist(const unsigned long long* __re
   const unsigned int m2 = sel->cand2;
   const unsigned long long hi = sel->prefix >> (sh + wd);
   for (unsigned int j = blockIdx.x * NT + threadIdx.x; j < m2; j += gridDim.x * NT) {
-    const unsigned long long bits = c2p[j];
+    const unsigned long long bits = P.c2p[j];
     if ((bits >> (sh + wd)) == hi) atomicAdd(&s_hist[(unsigned int)(bits >> sh) & ((1u << wd) - 1u)], 1u);
   }
-  flush_hist(s_hist, hist_w + (int64_t)lv * HIST_BINS);
+  flush_hist(s_hist, P.hist_w);
 }
 
-__global__ __launch_bounds__(NTB) void k_wide_find(const unsigned int* hist_w, SelState* sel, OpStats* st, int lv) {
+template <bool BATCH>
+__global__ __launch_bounds__(NTB) void k_wide_find(const SelProblem* probs, SelProblem one, int lv) {
   __shared__ int s_scan[WAVESB];
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_ab;
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
+  SelState* sel = P.sel;
   if (sel->keepall) return;
   if (lv >= 4 && !sel->tie) return;
   const int sh = kWideShift[lv];
-  const unsigned int* hist = hist_w + (int64_t)lv * HIST_BINS;
+  unsigned int* hist = P.hist_w;
   if (threadIdx.x == 0) {
     s_b = 0xFFFFFFFFu;
     s_ab = 0;
@@ -497,15 +554,19 @@ __global__ __launch_bounds__(NTB) void k_wide_find(const unsigned int* hist_w, S
     find_from_top<NTB>(hist, need, &s_b, &s_ab, s_scan);
   else
     find_from_bottom<NTB>(hist, need, &s_b, &s_ab, s_scan);
+  const unsigned int b = s_b;
+  const unsigned int cnt_b = b != 0xFFFFFFFFu ? hist[b] : 0u;
+  __syncthreads();
+  for (int i = threadIdx.x; i < HIST_BINS; i += NTB) hist[i] = 0;
   if (threadIdx.x != 0) return;
-  if (s_b == 0xFFFFFFFFu) {
-    set_err(&st->err, ERR_SELECT);
+  if (b == 0xFFFFFFFFu) {
+    set_err(&P.st->err, ERR_SELECT);
     return;
   }
   sel->need = need - s_ab;
   if (lv < 4) {
-    sel->prefix |= (unsigned long long)s_b << sh;
-    sel->cnt_eq = hist[s_b];
+    sel->prefix |= (unsigned long long)b << sh;
+    sel->cnt_eq = cnt_b;
     if (lv == 3) {
       sel->pstar = sel->prefix;
       sel->tie = sel->need < sel->cnt_eq ? 1u : 0u;
@@ -513,36 +574,38 @@ __global__ __launch_bounds__(NTB) void k_wide_find(const unsigned int* hist_w, S
       sel->kp = 0;
     }
   } else {
-    sel->kp |= (unsigned long long)s_b << sh;
+    sel->kp |= (unsigned long long)b << sh;
     if (lv == WIDE_LEVELS - 1) sel->keystar = sel->kp;
   }
 }
 
 // Logical keys of the candidates equal to the threshold (any order).
-__global__ __launch_bounds__(NT) void k_wide_collect(const unsigned long long* __restrict__ c2p,
-                                                      const unsigned int* __restrict__ c2i,
-                                                      const uint64_t* __restrict__ x, KeyMap km, SelState* sel,
-                                                      uint64_t* __restrict__ tk) {
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_wide_collect(const SelProblem* probs, SelProblem one) {
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.y);
+  SelState* sel = P.sel;
   if (sel->keepall || !sel->tie) return;
   const unsigned int m2 = sel->cand2;
   const unsigned long long pstar = sel->pstar;
   const int lane = threadIdx.x & 63;
   for (unsigned int j0 = blockIdx.x * NT; j0 < m2; j0 += gridDim.x * NT) {
     const unsigned int j = j0 + threadIdx.x;
-    const bool take = j < m2 && c2p[j] == pstar;
+    const bool take = j < m2 && P.c2p[j] == pstar;
     const unsigned long long bal = __ballot(take);
     if (!bal) continue;
     unsigned int wb = 0;
     const int leader = __ffsll((long long)bal) - 1;
     if (lane == leader) wb = atomicAdd(&sel->tcount, (unsigned int)__popcll(bal));
     wb = __shfl(wb, leader, 64);
-    if (take) tk[wb + __popcll(bal & ((1ull << lane) - 1ull))] = logical_key(x[c2i[j]], km);
+    if (take) P.tk[wb + __popcll(bal & ((1ull << lane) - 1ull))] = logical_key(P.x[P.c2i[j]], P.km);
   }
 }
 
-__global__ __launch_bounds__(NT) void k_wide_key_hist(const uint64_t* __restrict__ tk, const SelState* sel,
-                                                       unsigned int* hist_w, int lv) {
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_wide_key_hist(const SelProblem* probs, SelProblem one, int lv) {
   __shared__ unsigned int s_hist[HIST_BINS];
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.y);
+  const SelState* sel = P.sel;
   if (sel->keepall || !sel->tie) return;
   const int sh = kWideShift[lv], wd = kWideWidth[lv];
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
@@ -551,32 +614,36 @@ __global__ __launch_bounds__(NT) void k_wide_key_hist(const uint64_t* __restrict
   const unsigned long long kp = sel->kp;
   const int top = sh + wd;  // 64 at the first key level
   for (unsigned int j = blockIdx.x * NT + threadIdx.x; j < m; j += gridDim.x * NT) {
-    const uint64_t key = tk[j];
+    const uint64_t key = P.tk[j];
     if (top >= 64 || (key >> top) == (kp >> top))
       atomicAdd(&s_hist[(unsigned int)(key >> sh) & ((1u << wd) - 1u)], 1u);
   }
-  flush_hist(s_hist, hist_w + (int64_t)lv * HIST_BINS);
+  flush_hist(s_hist, P.hist_w);
+}
+
+__device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t w, unsigned long long pstar,
+                                          unsigned long long keystar, const KeyMap& km) {
+  return bits > pstar || (bits == pstar && logical_key(w, km) <= keystar);
 }
 
 // Per tile: sum and count of the kept candidates (in tile order).
-__global__ __launch_bounds__(NT) void k_keptsum(const unsigned long long* __restrict__ c1p,
-                                                 const unsigned int* __restrict__ c1i,
-                                                 const int* __restrict__ cnt1, const uint64_t* __restrict__ x,
-                                                 KeyMap km, const SelState* sel, double* ksum, int* kcnt) {
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_keptsum(const SelProblem* probs, int nprob, SelProblem one, SelTiles T) {
   __shared__ double s_red[WAVES];
   __shared__ int s_cnt;
-  const unsigned long long pstar = sel->pstar, keystar = sel->keystar;
-  const int64_t t = blockIdx.x;
-  const int cnt = cnt1[t];
-  const int64_t base = t * SEL_TILE;
+  const int64_t gt = blockIdx.x;
+  const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt);
+  const unsigned long long pstar = P.sel->pstar, keystar = P.sel->keystar;
+  const int cnt = T.cnt1[gt];
+  const int64_t base = gt * SEL_TILE;
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
   double s = 0.0;
   int c = 0;
   for (int j = threadIdx.x; j < cnt; j += NT) {
-    const unsigned long long bits = c1p[base + j];
+    const unsigned long long bits = T.c1p[base + j];
     bool keep = bits > pstar;
-    if (bits == pstar) keep = logical_key(x[c1i[base + j]], km) <= keystar;
+    if (bits == pstar) keep = logical_key(P.x[T.c1i[base + j]], P.km) <= keystar;
     if (keep) {
       s += __longlong_as_double((long long)bits);
       ++c;
@@ -586,34 +653,34 @@ __global__ __launch_bounds__(NT) void k_keptsum(const unsigned long long* __rest
   atomicAdd(&s_cnt, c);
   __syncthreads();
   if (threadIdx.x == 0) {
-    ksum[t] = tot;
-    kcnt[t] = s_cnt;
+    T.ksum[gt] = tot;
+    T.kcnt[gt] = s_cnt;
   }
 }
 
-// kept = sum over tiles in a fixed order; kept-entry offsets per tile
-// (exclusive scan).  Thread i owns FIN_CH consecutive tiles per round; its
-// loads are issued together.
+// kept = sum over the problem's tiles in a fixed order; kept-entry offsets
+// per tile (exclusive scan).  Thread i owns FIN_CH consecutive tiles per
+// round; its loads are issued together.
 constexpr int FIN_CH = 4;
-__global__ __launch_bounds__(NTB) void k_sel_finalize(const double* tile_dsum, const double* ksum,
-                                                       const int* tile_dcnt, const int* kcnt, int64_t ntiles,
-                                                       unsigned long long k, int renorm, const SelState* sel,
-                                                       int64_t* koff, OpStats* st) {
+template <bool BATCH>
+__global__ __launch_bounds__(NTB) void k_sel_finalize(const SelProblem* probs, SelProblem one, SelTiles T) {
   __shared__ double s_red[WAVESB];
   __shared__ int s_scan[WAVESB];
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
+  const int64_t t_end = P.tile0 + P.ntiles;
   double s = 0.0;
   int64_t carry = 0;
-  for (int64_t r0 = 0; r0 < ntiles; r0 += (int64_t)NTB * FIN_CH) {
+  for (int64_t r0 = P.tile0; r0 < t_end; r0 += (int64_t)NTB * FIN_CH) {
     const int64_t t0 = r0 + (int64_t)threadIdx.x * FIN_CH;
     double ds[FIN_CH], ks[FIN_CH];
     int c[FIN_CH];
 #pragma unroll
     for (int q = 0; q < FIN_CH; ++q) {
       const int64_t t = t0 + q;
-      const bool in = t < ntiles;
-      ds[q] = in ? tile_dsum[t] : 0.0;
-      ks[q] = in ? ksum[t] : 0.0;
-      c[q] = in ? tile_dcnt[t] + kcnt[t] : 0;
+      const bool in = t < t_end;
+      ds[q] = in ? T.tile_dsum[t] : 0.0;
+      ks[q] = in ? T.ksum[t] : 0.0;
+      c[q] = in ? T.tile_dcnt[t] + T.kcnt[t] : 0;
     }
     int v = 0;
 #pragma unroll
@@ -625,42 +692,41 @@ __global__ __launch_bounds__(NTB) void k_sel_finalize(const double* tile_dsum, c
     int64_t run = carry + block_excl_scan<NTB>(v, tot, s_scan);
 #pragma unroll
     for (int q = 0; q < FIN_CH; ++q) {
-      if (t0 + q < ntiles) koff[t0 + q] = run;
+      if (t0 + q < t_end) T.koff[t0 + q] = run;
       run += c[q];
     }
     carry += tot;
   }
   const double kept = block_sum<NTB>(s, s_red);
   if (threadIdx.x == 0) {
+    OpStats* st = P.st;
     st->nnz = carry;
-    if (sel->keepall) {
+    if (P.sel->keepall) {
       st->kept = st->total;
       st->norm = 1.0;
     } else if (st->flags & 2) {
       st->kept = 0.0;  // result discarded; the caller repeats with the wide path
       st->norm = 1.0;
     } else {
-      if ((unsigned long long)carry != k) set_err(&st->err, ERR_SELECT);
+      if ((unsigned long long)carry != P.k) set_err(&st->err, ERR_SELECT);
       st->kept = kept;
       const double norm = sqrt(kept);
-      st->norm = (renorm && norm > 1e-30) ? norm : 1.0;
+      st->norm = (P.renorm && norm > 1e-30) ? norm : 1.0;
     }
   }
 }
 
-__global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x, const double2* __restrict__ a,
-                                                 const int* seg_cnt, int64_t n_dense, KeyMap km, const SelState* sel,
-                                                 const int64_t* __restrict__ koff, uint64_t* __restrict__ xo,
-                                                 double2* __restrict__ ao, int64_t cap_out, const OpStats* st,
-                                                 double* tile_s2, double* tile_s4, int* err) {
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_compact(const SelProblem* probs, int nprob, SelProblem one, SelTiles T) {
   __shared__ int s_scan[WAVES];
   __shared__ double s_red[WAVES];
-  const unsigned int tile = blockIdx.x;
-  const unsigned long long pstar = sel->pstar, keystar = sel->keystar;
+  const int64_t gt = blockIdx.x;
+  const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt);
+  const unsigned long long pstar = P.sel->pstar, keystar = P.sel->keystar;
   // numpy divides complex128 by a real norm as a product with 1/norm
-  const double inv = 1.0 / st->norm;
-  const int cnt = tile_count(seg_cnt, n_dense, tile);
-  const int64_t base = (int64_t)tile * SEL_TILE;
+  const double inv = 1.0 / P.st->norm;
+  const int cnt = tile_count(P, gt);
+  const int64_t base = tile_base(P, gt);
   uint64_t kk[SEL_ITEMS];
   double2 vv[SEL_ITEMS];
   bool keep[SEL_ITEMS];
@@ -669,12 +735,12 @@ __global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x, 
     const int j = it * NT + threadIdx.x;
     keep[it] = false;
     if (j < cnt) {
-      kk[it] = x[base + j];
-      vv[it] = a[base + j];
-      keep[it] = kept_pred(pbits_of(vv[it]), kk[it], pstar, keystar, km);
+      kk[it] = P.x[base + j];
+      vv[it] = P.a[base + j];
+      keep[it] = kept_pred(pbits_of(vv[it]), kk[it], pstar, keystar, P.km);
     }
   }
-  int64_t rb = koff[tile];
+  int64_t rb = T.koff[gt];
   double s2 = 0.0, s4 = 0.0;
   bool fits = true;
 #pragma unroll
@@ -684,9 +750,9 @@ __global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x, 
     if (keep[it]) {
       const double2 v = make_double2(vv[it].x * inv, vv[it].y * inv);
       const int64_t pos = rb + off;
-      if (pos < cap_out) {
-        xo[pos] = kk[it];
-        ao[pos] = v;
+      if (pos < P.cap_out) {
+        P.xo[pos] = kk[it];
+        P.ao[pos] = v;
       } else {
         fits = false;
       }
@@ -696,18 +762,30 @@ __global__ __launch_bounds__(NT) void k_compact(const uint64_t* __restrict__ x, 
     }
     rb += tot;
   }
-  if (!fits) set_err(err, ERR_OVERFLOW);
+  if (!fits) set_err(&P.st->err, ERR_OVERFLOW);
   const double t2 = block_sum(s2, s_red);
   const double t4 = block_sum(s4, s_red);
   if (threadIdx.x == 0) {
-    tile_s2[tile] = t2;
-    tile_s4[tile] = t4;
+    T.tile_s2[gt] = t2;
+    T.tile_s4[gt] = t4;
   }
 }
 
-// Fixed-order reduction of per-tile sums into the stats block.
-__global__ __launch_bounds__(NTB) void k_sums_finalize(const double* tile_s2, const double* tile_s4,
-                                                        int64_t ntiles, OpStats* st) {
+// Fixed-order reduction of a problem's per-tile sums into its stats.
+template <bool BATCH>
+__global__ __launch_bounds__(NTB) void k_sums_finalize(const SelProblem* probs, SelProblem one, SelTiles T) {
+  __shared__ double s_red[WAVESB];
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
+  const double s2 = block_sum_array<NTB>(T.tile_s2 + P.tile0, P.ntiles, s_red);
+  const double s4 = block_sum_array<NTB>(T.tile_s4 + P.tile0, P.ntiles, s_red);
+  if (threadIdx.x == 0) {
+    P.st->sum2 = s2;
+    P.st->sum4 = s4;
+  }
+}
+
+__global__ __launch_bounds__(NTB) void k_sums_reduce(const double* tile_s2, const double* tile_s4,
+                                                      int64_t ntiles, OpStats* st) {
   __shared__ double s_red[WAVESB];
   const double s2 = block_sum_array<NTB>(tile_s2, ntiles, s_red);
   const double s4 = block_sum_array<NTB>(tile_s4, ntiles, s_red);
@@ -800,48 +878,62 @@ __global__ __launch_bounds__(NT) void k_gather(const uint64_t* __restrict__ x, c
 
 // ----------------------------------------------------------- host side ----
 
-int64_t sel_tiles(int64_t n) { return (n + SEL_TILE - 1) / SEL_TILE; }
-
-int launch_select(const SelLaunch& L, hipStream_t s) {
-  const int64_t tiles = L.ntiles;
-  int64_t npsum = L.npsum;
-  if (!L.hist_ready) {
-    hipLaunchKernelGGL(k_hist_psum, dim3((unsigned)tiles), dim3(NT), 0, s, L.a, L.seg_cnt, L.n, L.hist1,
-                       L.tile_psum);
-    npsum = tiles;
-  }
-  hipLaunchKernelGGL(k_find1, dim3(1), dim3(NTB), 0, s, L.hist1, L.k, L.tile_psum, npsum, L.sel, L.st);
-  const int64_t rows = (tiles + EX1_TILES - 1) / EX1_TILES;
-  hipLaunchKernelGGL(k_extract1, dim3((unsigned)rows), dim3(NT), 0, s, L.a, L.seg_cnt, L.n, tiles, L.sel, L.c1p,
-                     L.c1i, L.cnt1, L.hist2_rows, L.tile_dsum, L.tile_dcnt);
-  hipLaunchKernelGGL(k_hist2_reduce, dim3(HIST_BINS / (4 * NT), (unsigned)((rows + H2_ROWS - 1) / H2_ROWS)),
-                     dim3(NT), 0, s, L.hist2_rows, rows, L.sel, L.hist2);
-  hipLaunchKernelGGL(k_find2, dim3(1), dim3(NT), 0, s, L.hist2, L.sel, L.st);
-  int64_t eb = tiles < 1024 ? tiles : 1024;
-  hipLaunchKernelGGL(k_extract2, dim3((unsigned)eb), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, tiles, L.sel, L.c2p,
-                     L.c2i);
-  if (!L.wide) {
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(NT), 0, s, L.c2p, L.c2i, L.x, L.km, L.sel, L.st);
+// The select sequence for nprob problems covering tiles [0, tiles_end).
+// BATCH = false: one problem by value (probs unused); extract1 groups 4 tiles
+// per block.
+template <bool BATCH>
+static void select_sequence(const SelProblem* probs, int nprob, const SelProblem& one, const SelTiles& T,
+                            int64_t tiles_end, int64_t max_rows, bool hist_ready, bool wide, hipStream_t s) {
+  constexpr int EXT = BATCH ? 1 : EX1_TILES;
+  if (!hist_ready)
+    hipLaunchKernelGGL((k_hist_psum<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
+  hipLaunchKernelGGL((k_find1<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
+  const int64_t rows = (tiles_end + EXT - 1) / EXT;
+  hipLaunchKernelGGL((k_extract1<BATCH, EXT>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one, T,
+                     tiles_end);
+  hipLaunchKernelGGL((k_hist2_reduce<BATCH>),
+                     dim3(HIST_BINS / (4 * NT), (unsigned)((max_rows + H2_ROWS - 1) / H2_ROWS), (unsigned)nprob),
+                     dim3(NT), 0, s, probs, one, T);
+  hipLaunchKernelGGL((k_find2<BATCH>), dim3((unsigned)nprob), dim3(NT), 0, s, probs, one);
+  hipLaunchKernelGGL((k_extract2<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
+  if (!wide) {
+    hipLaunchKernelGGL((k_finish<BATCH>), dim3((unsigned)nprob), dim3(NT), 0, s, probs, one);
   } else {
+    const dim3 wg(WIDE_BLOCKS, (unsigned)nprob);
     for (int lv = 0; lv < 4; ++lv) {
-      hipLaunchKernelGGL(k_wide_hist, dim3(WIDE_BLOCKS), dim3(NT), 0, s, L.c2p, L.sel, L.hist_w, lv);
-      hipLaunchKernelGGL(k_wide_find, dim3(1), dim3(NTB), 0, s, L.hist_w, L.sel, L.st, lv);
+      hipLaunchKernelGGL((k_wide_hist<BATCH>), wg, dim3(NT), 0, s, probs, one, lv);
+      hipLaunchKernelGGL((k_wide_find<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, lv);
     }
-    hipLaunchKernelGGL(k_wide_collect, dim3(WIDE_BLOCKS), dim3(NT), 0, s, L.c2p, L.c2i, L.x, L.km, L.sel, L.tk);
+    hipLaunchKernelGGL((k_wide_collect<BATCH>), wg, dim3(NT), 0, s, probs, one);
     for (int lv = 4; lv < WIDE_LEVELS; ++lv) {
-      hipLaunchKernelGGL(k_wide_key_hist, dim3(WIDE_BLOCKS), dim3(NT), 0, s, L.tk, L.sel, L.hist_w, lv);
-      hipLaunchKernelGGL(k_wide_find, dim3(1), dim3(NTB), 0, s, L.hist_w, L.sel, L.st, lv);
+      hipLaunchKernelGGL((k_wide_key_hist<BATCH>), wg, dim3(NT), 0, s, probs, one, lv);
+      hipLaunchKernelGGL((k_wide_find<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, lv);
     }
   }
-  hipLaunchKernelGGL(k_keptsum, dim3((unsigned)tiles), dim3(NT), 0, s, L.c1p, L.c1i, L.cnt1, L.x, L.km, L.sel,
-                     L.ksum, L.kcnt);
-  hipLaunchKernelGGL(k_sel_finalize, dim3(1), dim3(NTB), 0, s, L.tile_dsum, L.ksum, L.tile_dcnt, L.kcnt, tiles,
-                     L.k, L.renorm, L.sel, L.koff, L.st);
-  hipLaunchKernelGGL(k_compact, dim3((unsigned)tiles), dim3(NT), 0, s, L.x, L.a, L.seg_cnt, L.n, L.km, L.sel, L.koff,
-                     L.xo, L.ao, L.cap_out, L.st, L.tile_s2, L.tile_s4, &L.st->err);
-  hipLaunchKernelGGL(k_sums_finalize, dim3(1), dim3(NTB), 0, s, L.tile_s2, L.tile_s4, tiles, L.st);
+  hipLaunchKernelGGL((k_keptsum<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
+  hipLaunchKernelGGL((k_sel_finalize<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
+  hipLaunchKernelGGL((k_compact<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
+  hipLaunchKernelGGL((k_sums_finalize<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
+}
+
+int select_rows(int64_t ntiles, bool batch) { return (int)((ntiles + (batch ? 1 : EX1_TILES) - 1) / (batch ? 1 : EX1_TILES)); }
+
+int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s) {
+  select_sequence<false>(nullptr, 1, P, T, P.ntiles, P.nrows, hist_ready, wide, s);
   return (int)hipGetLastError();
 }
+
+int launch_select_batch(const SelProblem* d_probs, int nprob, const SelTiles& T, int64_t tiles_end,
+                        int64_t max_rows, bool hist_ready, bool wide, hipStream_t s) {
+  SelProblem none;
+  memset(&none, 0, sizeof(none));
+  select_sequence<true>(d_probs, nprob, none, T, tiles_end, max_rows, hist_ready, wide, s);
+  return (int)hipGetLastError();
+}
+
+
+
+int64_t sel_tiles(int64_t n) { return (n + SEL_TILE - 1) / SEL_TILE; }
 
 int launch_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff, OpStats* st, hipStream_t s) {
   hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(NTB), 0, s, seg_cnt, ntiles, soff, st);
@@ -860,7 +952,7 @@ int launch_moments(const double2* a, int64_t n, double* tile_s2, double* tile_s4
   const int64_t tiles = sel_tiles(n);
   if (tiles > 0)
     hipLaunchKernelGGL(k_moments, dim3((unsigned)tiles), dim3(NT), 0, s, a, n, tile_s2, tile_s4);
-  hipLaunchKernelGGL(k_sums_finalize, dim3(1), dim3(NTB), 0, s, tile_s2, tile_s4, tiles, st);
+  hipLaunchKernelGGL(k_sums_reduce, dim3(1), dim3(NTB), 0, s, tile_s2, tile_s4, tiles, st);
   return (int)hipGetLastError();
 }
 
