@@ -49,6 +49,13 @@ class BassStats(ctypes.Structure):
     ]
 
 
+# bass_job (include/bass_hip.h) as a numpy record, so a batch's table is
+# filled with vector assignments
+JOB_DTYPE = np.dtype([("x", "<u8"), ("a", "<u8"), ("xo", "<u8"), ("ao", "<u8"), ("n", "<i8"), ("k", "<i8"),
+                      ("cap_out", "<i8"), ("j", "<i4"), ("j2", "<i4"), ("renorm", "<i4"), ("key_bits", "<i4"),
+                      ("g", "<f8", (32,)), ("key_map", "<i4", (64,)), ("scale", "<f8")])
+assert JOB_DTYPE.itemsize == 592
+
 # name -> (restype, argtypes); mirrors include/bass_hip.h
 SIGNATURES = {
     "bass_abi_version": (_I, []),
@@ -71,6 +78,13 @@ SIGNATURES = {
     "bass_ht_lookup": (_I, [_P, _P, _I, _P, _I64, _P, _P]),
     "bass_radix_sort": (_I, [_P, _P, _I64, _I, _P, _SZ, _P]),
     "bass_merge_sorted": (_I, [_P, _P, _I64, _P, _P, _P, _SZ, _P]),
+    "bass_batch_step": (_I, [_P, _I, _I, _I, _I, _P, _P, _I64, _P, _SZ, _P]),
+    "bass_batch_move": (_I, [_P, _I, _P, _SZ, _P]),
+    "bass_batch_diag2q": (_I, [_P, _I, _P, _SZ, _P]),
+    "bass_batch_scale": (_I, [_P, _I, _P, _SZ, _P]),
+    "bass_batch_moments": (_I, [_P, _I, _P, _SZ, _P]),
+    "bass_batch_rdm1q": (_I, [_P, _I, _P, _P, _SZ, _P]),
+    "bass_batch_stats": (_I, [_P, _SZ, _I, _P, _P]),
     "bass_timing": (_I, [_I]),
     "bass_timing_read": (_I, [ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D)]),
 }

@@ -146,3 +146,93 @@ def rdm1q(ctx, buf, n, N, ordered=False):
     rho[:, 0, 1] = out[:, 2] + 1j * out[:, 3]
     rho[:, 1, 0] = out[:, 2] - 1j * out[:, 3]
     return rho
+
+
+# ---------------------------------------------------------------- batches --
+# A batch is a numpy array of L.JOB_DTYPE records (one per state).
+
+STATS_DTYPE = np.dtype([("nnz", "<i8"), ("total", "<f8"), ("kept", "<f8"), ("norm", "<f8"), ("sum2", "<f8"),
+                        ("sum4", "<f8"), ("n_in", "<i8"), ("err", "<i4"), ("flags", "<i4")])
+
+
+def new_jobs(count):
+    return np.zeros(count, dtype=L.JOB_DTYPE)
+
+
+def batch_stats(ctx, njobs, what="bass_batch_stats"):
+    out = np.zeros(njobs, dtype=STATS_DTYPE)
+    rc = ctx.lib.bass_batch_stats(ctx.wsp, ctx.ws_bytes, int(njobs), out.ctypes.data_as(ctypes.c_void_p), ctx.stream)
+    L.check(rc, what)
+    return out
+
+
+def batch_tiles(jobs, nq, mode=L.MODE_GATE):
+    tm = 1024 if mode == L.MODE_MOVE else SEG // (1 << nq)
+    return int(np.sum((jobs["n"] + tm - 1) // tm))
+
+
+def batch_step(ctx, jobs, nq, mode, tiles):
+    """Gate / rotation + truncate-or-keep for every job; returns per-job stats.
+    tiles: Buffers for the gate outputs (grown as needed)."""
+    T = batch_tiles(jobs, nq, mode)
+    slots = max(T * SEG, 1)
+    ctx.ensure_ws(max(slots, len(jobs) * 1024))
+    tiles.ensure(slots)
+    jp = jobs.ctypes.data_as(ctypes.c_void_p)
+    rc = ctx.lib.bass_batch_step(jp, len(jobs), nq, mode, 0, tiles.xp, tiles.ap, tiles.cap, ctx.wsp, ctx.ws_bytes,
+                                 ctx.stream)
+    L.check(rc, "bass_batch_step")
+    st = batch_stats(ctx, len(jobs))
+    wide = np.flatnonzero(st["flags"] & 2)
+    if len(wide):
+        # very many entries tie at some thresholds: those jobs' states are
+        # untouched (nothing was kept); repeat them with the multi-block path
+        sub = np.ascontiguousarray(jobs[wide])
+        rc = ctx.lib.bass_batch_step(sub.ctypes.data_as(ctypes.c_void_p), len(sub), nq, mode, 1, tiles.xp, tiles.ap,
+                                     tiles.cap, ctx.wsp, ctx.ws_bytes, ctx.stream)
+        L.check(rc, "bass_batch_step")
+        st[wide] = batch_stats(ctx, len(sub))
+    return st
+
+
+def batch_move(ctx, jobs):
+    ctx.ensure_ws(max(batch_tiles(jobs, 1, L.MODE_MOVE) * 1024, len(jobs) * 1024))
+    rc = ctx.lib.bass_batch_move(jobs.ctypes.data_as(ctypes.c_void_p), len(jobs), ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_batch_move")
+
+
+def _batch_ew(ctx, fn, jobs, what):
+    ctx.ensure_ws(len(jobs) * 1024 + int(np.sum(jobs["n"])) // 64)
+    rc = fn(jobs.ctypes.data_as(ctypes.c_void_p), len(jobs), ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, what)
+
+
+def batch_diag2q(ctx, jobs):
+    _batch_ew(ctx, ctx.lib.bass_batch_diag2q, jobs, "bass_batch_diag2q")
+
+
+def batch_scale(ctx, jobs):
+    _batch_ew(ctx, ctx.lib.bass_batch_scale, jobs, "bass_batch_scale")
+
+
+def batch_moments(ctx, jobs):
+    """Per-job (sum |a|^2, sum |a|^4)."""
+    _batch_ew(ctx, ctx.lib.bass_batch_moments, jobs, "bass_batch_moments")
+    st = batch_stats(ctx, len(jobs))
+    return st["sum2"], st["sum4"]
+
+
+def batch_rdm1q(ctx, jobs):
+    """RDMs of stored bits p < key_bits per job -> complex128[njobs, 64, 2, 2]."""
+    n = len(jobs)
+    ctx.ensure_ws(n * 2048 + int(np.sum(jobs["n"])) * 2)
+    out = np.zeros((n, 64, 4), dtype=np.float64)
+    rc = ctx.lib.bass_batch_rdm1q(jobs.ctypes.data_as(ctypes.c_void_p), n, out.ctypes.data_as(ctypes.c_void_p),
+                                  ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_batch_rdm1q")
+    rho = np.zeros((n, 64, 2, 2), dtype=np.complex128)
+    rho[:, :, 0, 0] = out[:, :, 0]
+    rho[:, :, 1, 1] = out[:, :, 1]
+    rho[:, :, 0, 1] = out[:, :, 2] + 1j * out[:, :, 3]
+    rho[:, :, 1, 0] = out[:, :, 2] - 1j * out[:, :, 3]
+    return rho

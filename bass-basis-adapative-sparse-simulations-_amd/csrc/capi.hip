@@ -513,6 +513,306 @@ int bass_rdm1q_ordered(const uint64_t* x, const double* a, int64_t n, int N, dou
   return BASS_OK;
 }
 
+// ----------------------------------------------------------- batches ----
+// Per-job region at the start of the workspace's big area (offsets depend on
+// njobs only), then the select's candidate arrays (32 B per tile slot), then
+// scratch of the RDM / moments calls.
+struct BatchRegion {
+  size_t st, sel, hist1, hist2, histw, ljobs, probs, rjobs, ewjobs, end;
+};
+
+static BatchRegion batch_region(const Layout& L, int njobs) {
+  BatchRegion R;
+  size_t o = L.big;
+  R.st = o; o += al((size_t)njobs * sizeof(OpStats));
+  R.sel = o; o += al((size_t)njobs * sizeof(SelState));
+  R.hist1 = o; o += al((size_t)njobs * HIST_BINS * 4);
+  R.hist2 = o; o += al((size_t)njobs * HIST_BINS * 4);
+  R.histw = o; o += al((size_t)njobs * HIST_BINS * 4);
+  R.ljobs = o; o += al((size_t)njobs * sizeof(LiftJob));
+  R.probs = o; o += al((size_t)njobs * sizeof(SelProblem));
+  R.rjobs = o; o += al((size_t)njobs * sizeof(RdmJob));
+  R.ewjobs = o; o += al((size_t)njobs * sizeof(EwJob));
+  R.end = o;
+  return R;
+}
+
+// Pinned staging ring for host -> device job tables: a slot is reused only
+// after the copy that last read it has completed.
+struct StageSlot {
+  void* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t done = nullptr;
+};
+static std::mutex g_stage_mu;
+static StageSlot g_stage[8];
+static int g_stage_next = 0;
+
+static int stage_copy(void* dev, const void* src, size_t bytes, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  StageSlot& S = g_stage[g_stage_next];
+  g_stage_next = (g_stage_next + 1) % 8;
+  if (S.done) (void)hipEventSynchronize(S.done);
+  if (S.cap < bytes) {
+    if (S.host) (void)hipHostFree(S.host);
+    S.host = nullptr;
+    S.cap = 0;
+    if (hipHostMalloc(&S.host, bytes) != hipSuccess) return fail(BASS_E_HIP, "pinned staging allocation failed");
+    S.cap = bytes;
+  }
+  if (!S.done && hipEventCreateWithFlags(&S.done, hipEventDisableTiming) != hipSuccess)
+    return fail(BASS_E_HIP, "staging event creation failed");
+  memcpy(S.host, src, bytes);
+  if (hipMemcpyAsync(dev, S.host, bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipEventRecord(S.done, s) != hipSuccess)
+    return fail(BASS_E_HIP, "job table copy: %s", hipGetErrorString(hipGetLastError()));
+  return BASS_OK;
+}
+
+static int check_job(const bass_job& J, int c, bool need_k, int jmin, const char* who, KeyMap* km) {
+  if (J.n < 0 || J.n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "%s: job %d: bad n %lld", who, c, (long long)J.n);
+  if (J.j < jmin || J.j > 62) return fail(BASS_E_ARG, "%s: job %d: bad bit %d", who, c, J.j);
+  if (need_k && (J.k < 1 || J.cap_out < J.k))
+    return fail(BASS_E_OVERFLOW, "%s: job %d: need 1 <= k <= cap_out", who, c);
+  if (km) {
+    memset(km, 0, sizeof(*km));
+    if (J.key_bits) {
+      if (J.key_bits < 1 || J.key_bits > 63) return fail(BASS_E_ARG, "%s: job %d: key_bits out of range", who, c);
+      uint64_t seen = 0;
+      for (int p = 0; p < J.key_bits; ++p) {
+        const int q = J.key_map[p];
+        if (q < 0 || q >= J.key_bits || ((seen >> q) & 1ull))
+          return fail(BASS_E_ARG, "%s: job %d: key_map is not a permutation", who, c);
+        seen |= 1ull << q;
+        km->q[p] = (unsigned char)q;
+      }
+      km->nbits = J.key_bits;
+    }
+  }
+  return BASS_OK;
+}
+
+int bass_batch_step(const bass_job* jobs, int njobs, int nq, int mode, int wide, uint64_t* tiles_x, double* tiles_a,
+                    int64_t tiles_cap, void* ws, size_t ws_bytes, void* stream) {
+  if (njobs < 1) return fail(BASS_E_ARG, "bass_batch_step: no jobs");
+  if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_batch_step: nq must be 1 or 2");
+  if (mode != MODE_GATE && mode != MODE_ROT) return fail(BASS_E_ARG, "bass_batch_step: bad mode");
+  const int tm = lift_tile_entries(nq, mode);
+  std::vector<LiftJob> lj(njobs);
+  std::vector<SelProblem> pr(njobs);
+  std::vector<KeyMap> kms(njobs);
+  int64_t T = 0, max_rows = 1;
+  for (int c = 0; c < njobs; ++c) {
+    const int rc = check_job(jobs[c], c, true, nq - 1, "bass_batch_step", &kms[c]);
+    if (rc) return rc;
+    const int64_t nt = (jobs[c].n + tm - 1) / tm;
+    memset(&lj[c], 0, sizeof(LiftJob));
+    lj[c].x = jobs[c].x;
+    lj[c].a = (const double2*)jobs[c].a;
+    lj[c].n = jobs[c].n;
+    lj[c].tile0 = T;
+    lj[c].j = jobs[c].j;
+    const int M = 1 << nq;
+    for (int e = 0; e < M * M; ++e) lj[c].g.m[e] = make_double2(jobs[c].g[2 * e], jobs[c].g[2 * e + 1]);
+    pr[c].ntiles = nt;
+    T += nt;
+    if (nt > max_rows) max_rows = nt;
+  }
+  const int64_t slots = T * SEG_ENTRIES;
+  if (tiles_cap < slots)
+    return fail(BASS_E_OVERFLOW, "bass_batch_step: tiles need %lld slots", (long long)slots);
+  WS_REQUIRE(slots > njobs ? slots : njobs);
+  const BatchRegion R = batch_region(L, njobs);
+  const size_t p8 = al((size_t)8 * slots), p4 = al((size_t)4 * slots);
+  if (R.end + 3 * p8 + 2 * p4 > L.total) return fail(BASS_E_WORKSPACE, "bass_batch_step: batch does not fit");
+  hipStream_t s = (hipStream_t)stream;
+  char* c1p = at<char>(ws, R.end);
+  char* c1i = c1p + p8;
+  char* c2p = c1i + p4;
+  char* c2i = c2p + p8;
+  char* tkb = c2i + p4;
+  for (int c = 0; c < njobs; ++c) {
+    SelProblem& P = pr[c];
+    const int64_t nt = P.ntiles;
+    memset(&P, 0, sizeof(P));
+    P.x = tiles_x;
+    P.a = (const double2*)tiles_a;
+    P.seg_cnt = at<int>(ws, L.seg_cnt);
+    P.tile0 = lj[c].tile0;
+    P.ntiles = nt;
+    P.row0 = lj[c].tile0;
+    P.nrows = nt;
+    P.npsum = nt;
+    P.k = (unsigned long long)jobs[c].k;
+    P.xo = jobs[c].xo;
+    P.ao = (double2*)jobs[c].ao;
+    P.cap_out = jobs[c].cap_out;
+    P.renorm = jobs[c].renorm;
+    P.hist1 = at<unsigned int>(ws, R.hist1) + (size_t)c * HIST_BINS;
+    P.hist2 = at<unsigned int>(ws, R.hist2) + (size_t)c * HIST_BINS;
+    P.hist_w = at<unsigned int>(ws, R.histw) + (size_t)c * HIST_BINS;
+    P.sel = at<SelState>(ws, R.sel) + c;
+    P.st = at<OpStats>(ws, R.st) + c;
+    P.c2p = (unsigned long long*)c2p + P.tile0 * SEG_ENTRIES;
+    P.c2i = (unsigned int*)c2i + P.tile0 * SEG_ENTRIES;
+    P.tk = (uint64_t*)tkb + P.tile0 * SEG_ENTRIES;
+    P.km = kms[c];
+  }
+  static const char* names[2][2] = {{"batch_gate1q", "batch_gate2q"}, {"batch_rot1q", "batch_rot2q"}};
+  const int tk = ktimer_begin(names[mode][nq - 1], s);
+  // per-job headers: stats, selection state, histograms
+  fill(at<void>(ws, R.st), R.ljobs - R.st, 0ull, s);
+  int rc = stage_copy(at<void>(ws, R.ljobs), lj.data(), sizeof(LiftJob) * njobs, s);
+  if (!rc) rc = stage_copy(at<void>(ws, R.probs), pr.data(), sizeof(SelProblem) * njobs, s);
+  if (rc) return rc;
+  if (T > 0) {
+    launch_lift_batch(nq, mode, at<LiftJob>(ws, R.ljobs), njobs, T, at<LiftBound>(ws, L.lift_b), tiles_x,
+                      (double2*)tiles_a, at<unsigned int>(ws, R.hist1), at<double>(ws, L.tile_psum),
+                      at<int>(ws, L.seg_cnt), s);
+  }
+  launch_select_batch(at<SelProblem>(ws, R.probs), njobs,
+                      tiles_view(ws, L, (unsigned long long*)c1p, (unsigned int*)c1i), T, max_rows, true,
+                      wide != 0, s);
+  ktimer_end(tk, s);
+  return hip_check("bass_batch_step");
+}
+
+int bass_batch_move(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream) {
+  if (njobs < 1) return fail(BASS_E_ARG, "bass_batch_move: no jobs");
+  std::vector<LiftJob> lj(njobs);
+  int64_t T = 0, nmax = 1;
+  for (int c = 0; c < njobs; ++c) {
+    const int rc = check_job(jobs[c], c, false, 0, "bass_batch_move", nullptr);
+    if (rc) return rc;
+    if (jobs[c].cap_out < jobs[c].n) return fail(BASS_E_OVERFLOW, "bass_batch_move: job %d: cap_out < n", c);
+    memset(&lj[c], 0, sizeof(LiftJob));
+    lj[c].x = jobs[c].x;
+    lj[c].a = (const double2*)jobs[c].a;
+    lj[c].xo = jobs[c].xo;
+    lj[c].ao = (double2*)jobs[c].ao;
+    lj[c].n = jobs[c].n;
+    lj[c].tile0 = T;
+    lj[c].j = jobs[c].j;
+    T += (jobs[c].n + 1023) / 1024;
+    if (jobs[c].n > nmax) nmax = jobs[c].n;
+  }
+  WS_REQUIRE(T * 1024 + njobs);
+  const BatchRegion R = batch_region(L, njobs);
+  if (R.end > L.total) return fail(BASS_E_WORKSPACE, "bass_batch_move: batch does not fit");
+  hipStream_t s = (hipStream_t)stream;
+  const int tk = ktimer_begin("batch_move", s);
+  int rc = stage_copy(at<void>(ws, R.ljobs), lj.data(), sizeof(LiftJob) * njobs, s);
+  if (rc) return rc;
+  if (T > 0)
+    launch_lift_batch(1, MODE_MOVE, at<LiftJob>(ws, R.ljobs), njobs, T, at<LiftBound>(ws, L.lift_b), nullptr,
+                      nullptr, nullptr, nullptr, nullptr, s);
+  ktimer_end(tk, s);
+  return hip_check("bass_batch_move");
+}
+
+static int batch_ew(int op, const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream,
+                    const char* who) {
+  if (njobs < 1) return fail(BASS_E_ARG, "%s: no jobs", who);
+  std::vector<EwJob> ej(njobs);
+  int64_t ntiles = 0;
+  for (int c = 0; c < njobs; ++c) {
+    const bass_job& J = jobs[c];
+    if (J.n < 0) return fail(BASS_E_ARG, "%s: job %d: bad n", who, c);
+    if (op == 0 && (J.j < 0 || J.j > 62 || J.j2 < 0 || J.j2 > 62 || J.j == J.j2))
+      return fail(BASS_E_ARG, "%s: job %d: bad bits", who, c);
+    memset(&ej[c], 0, sizeof(EwJob));
+    ej[c].x = J.x;
+    ej[c].a = (double2*)J.a;
+    ej[c].n = J.n;
+    ej[c].j1 = J.j;
+    ej[c].j2 = J.j2;
+    for (int e = 0; e < 4; ++e) ej[c].d[e] = make_double2(J.g[2 * e], J.g[2 * e + 1]);
+    ej[c].div = J.scale;
+    ej[c].toff = ntiles;
+    ntiles += (J.n + SEG_ENTRIES - 1) / SEG_ENTRIES;
+  }
+  WS_REQUIRE(0);
+  const BatchRegion R = batch_region(L, njobs);
+  if (R.end + ew_batch_scratch(ntiles) > L.total) return fail(BASS_E_WORKSPACE, "%s: batch does not fit", who);
+  hipStream_t s = (hipStream_t)stream;
+  if (op == 2) fill(at<void>(ws, R.st), (size_t)njobs * sizeof(OpStats), 0ull, s);
+  int rc = stage_copy(at<void>(ws, R.ewjobs), ej.data(), sizeof(EwJob) * njobs, s);
+  if (rc) return rc;
+  launch_ew_batch(op, at<EwJob>(ws, R.ewjobs), njobs, ntiles, at<double>(ws, R.end), at<OpStats>(ws, R.st), s);
+  return hip_check(who);
+}
+
+int bass_batch_diag2q(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream) {
+  return batch_ew(0, jobs, njobs, ws, ws_bytes, stream, "bass_batch_diag2q");
+}
+int bass_batch_scale(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream) {
+  return batch_ew(1, jobs, njobs, ws, ws_bytes, stream, "bass_batch_scale");
+}
+int bass_batch_moments(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream) {
+  return batch_ew(2, jobs, njobs, ws, ws_bytes, stream, "bass_batch_moments");
+}
+
+int bass_batch_rdm1q(const bass_job* jobs, int njobs, double* rho_out, void* ws, size_t ws_bytes, void* stream) {
+  if (njobs < 1) return fail(BASS_E_ARG, "bass_batch_rdm1q: no jobs");
+  const int tm = rdm_tile_entries();
+  std::vector<RdmJob> rj(njobs);
+  int64_t nb = 0, np = 0, nd = 0;
+  for (int c = 0; c < njobs; ++c) {
+    const bass_job& J = jobs[c];
+    if (J.n < 0 || J.n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_batch_rdm1q: job %d: bad n", c);
+    if (J.key_bits < 1 || J.key_bits > 63) return fail(BASS_E_ARG, "bass_batch_rdm1q: job %d: bad key_bits", c);
+    memset(&rj[c], 0, sizeof(RdmJob));
+    rj[c].x = J.x;
+    rj[c].a = (const double2*)J.a;
+    rj[c].n = J.n;
+    rj[c].nbits = J.key_bits;
+    rj[c].ptiles = (J.n + tm - 1) / tm;
+    rj[c].dtiles = (J.n + SEG_ENTRIES - 1) / SEG_ENTRIES;
+    rj[c].bound_off = nb;
+    rj[c].ptile_off = np;
+    rj[c].dtile_off = nd;
+    nb += (int64_t)J.key_bits * (rj[c].ptiles + 1);
+    np += (int64_t)J.key_bits * rj[c].ptiles;
+    nd += rj[c].dtiles;
+  }
+  WS_REQUIRE(0);
+  const BatchRegion R = batch_region(L, njobs);
+  const size_t rho_bytes = (size_t)njobs * 64 * 4 * 8;
+  if (R.end + al(rho_bytes) + rdm_batch_scratch(nb, np, nd) > L.total)
+    return fail(BASS_E_WORKSPACE, "bass_batch_rdm1q: batch does not fit");
+  hipStream_t s = (hipStream_t)stream;
+  double* rho = at<double>(ws, R.end);
+  const int tm2 = ktimer_begin("batch_rdm", s);
+  fill(rho, rho_bytes, 0ull, s);
+  int rc = stage_copy(at<void>(ws, R.rjobs), rj.data(), sizeof(RdmJob) * njobs, s);
+  if (rc) return rc;
+  launch_rdm_batch(at<RdmJob>(ws, R.rjobs), njobs, nb, np, nd, at<void>(ws, R.end + al(rho_bytes)), rho, s);
+  ktimer_end(tm2, s);
+  rc = hip_check("bass_batch_rdm1q");
+  if (rc) return rc;
+  if (hipMemcpyAsync(rho_out, rho, rho_bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(BASS_E_HIP, "bass_batch_rdm1q copy: %s", hipGetErrorString(hipGetLastError()));
+  return BASS_OK;
+}
+
+int bass_batch_stats(void* ws, size_t ws_bytes, int njobs, bass_stats* out, void* stream) {
+  if (njobs < 1) return fail(BASS_E_ARG, "bass_batch_stats: no jobs");
+  WS_REQUIRE(0);
+  const BatchRegion R = batch_region(L, njobs);
+  hipStream_t s = (hipStream_t)stream;
+  OpStats* st = at<OpStats>(ws, R.st);
+  if (hipMemcpyAsync(out, st, sizeof(bass_stats) * njobs, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(BASS_E_HIP, "bass_batch_stats: %s", hipGetErrorString(hipGetLastError()));
+  for (int c = 0; c < njobs; ++c)
+    if (out[c].err) {
+      fill(st, sizeof(OpStats) * njobs, 0ull, s);
+      return fail(BASS_E_DEVICE, "job %d: device error bits 0x%x", c, out[c].err);
+    }
+  return BASS_OK;
+}
+
 int bass_ht_clear(uint64_t* ht_keys, int32_t* ht_vals, int log2cap, void* stream) {
   if (log2cap < 1 || log2cap > 40) return fail(BASS_E_ARG, "bass_ht_clear: bad capacity");
   hipStream_t s = (hipStream_t)stream;

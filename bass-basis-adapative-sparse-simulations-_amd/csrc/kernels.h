@@ -68,11 +68,56 @@ struct LiftBound {
   int a, b;        // A / B cursors of the merge at the boundary
   int pad0, pad1, pad2;
 };
+// One state of a batched lift (see launch_lift_batch).
+struct LiftJob {
+  const uint64_t* x;  // input (sorted by stored word)
+  const double2* a;
+  uint64_t* xo;       // move: dense output of this job
+  double2* ao;
+  int64_t n;
+  int64_t tile0;      // first global output tile
+  int j;              // bit to lift
+  int pad;
+  GateMat g;
+};
+int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, int njobs, int64_t ntiles_total, LiftBound* bounds,
+                      uint64_t* xo, double2* ao, unsigned int* hist, double* tile_psum, int* seg_cnt,
+                      hipStream_t s);
 int lift_tile_entries(int qb, int mode);
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
                 hipStream_t s);
 
+// One state of a batched elementwise operation (see launch_ew_batch):
+// op 0 diagonal 2-qubit gate a *= d[(b_j1 << 1) | b_j2], op 1 a /= div,
+// op 2 moments into st[job].sum2 / sum4.
+struct EwJob {
+  const uint64_t* x;
+  double2* a;
+  int64_t n;
+  int j1, j2;
+  double2 d[4];
+  double div;
+  int64_t toff;  // moments: first of the job's SEG_ENTRIES tiles in the scratch arrays
+};
+int launch_ew_batch(int op, const EwJob* d_jobs, int njobs, int64_t ntiles_total, double* part, OpStats* st,
+                    hipStream_t s);
+size_t ew_batch_scratch(int64_t ntiles_total);
+
+// One state of a batched RDM computation (see launch_rdm_batch).
+struct RdmJob {
+  const uint64_t* x;
+  const double2* a;
+  int64_t n;
+  int64_t bound_off, ptile_off, dtile_off;  // offsets in the flattened index spaces
+  int64_t ptiles, dtiles;                   // pair tiles per bit, diagonal tiles
+  int nbits;
+  int pad;
+};
+int launch_rdm_batch(const RdmJob* d_jobs, int njobs, int64_t nbound, int64_t nptiles, int64_t ndtiles, void* scratch,
+                     double* rho, hipStream_t s);
+size_t rdm_batch_scratch(int64_t nbound, int64_t nptiles, int64_t ndtiles);
+int rdm_tile_entries();
 size_t rdm_ordered_scratch(int64_t n, int nbits);
 int launch_rdm_ordered(const uint64_t* x, const double2* a, int64_t n, int nbits, void* scratch, double* rho,
                        hipStream_t s);

@@ -67,16 +67,9 @@ __device__ __forceinline__ int64_t wave_partition_point(int64_t lo, int64_t hi, 
   return lo + __popcll(__ballot(t));
 }
 
-// Tile boundaries of the lift of bit j0 + blockIdx.y (one bit per grid row;
-// row r writes bounds[r * nbound + u]).
-__global__ __launch_bounds__(NT) void k_lift_part(const uint64_t* __restrict__ w, int64_t n, int j0, int t,
-                                                   int tm, int64_t nbound, LiftBound* bounds) {
-  const int lane = threadIdx.x & 63;
-  const int j = j0 + (int)blockIdx.y;
-  bounds += (int64_t)blockIdx.y * nbound;
-  const int64_t u = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
-  if (u >= nbound) return;  // uniform per wave
-  int64_t d = u * (int64_t)tm;
+// The run around merged position d (clamped to n) and the merge split there.
+// Called by all 64 lanes of a wave.
+__device__ LiftBound lift_boundary(const uint64_t* __restrict__ w, int64_t n, int j, int t, int64_t d) {
   if (d > n) d = n;
   LiftBound r;
   if (d >= n) {
@@ -100,6 +93,19 @@ __global__ __launch_bounds__(NT) void k_lift_part(const uint64_t* __restrict__ w
     r.b = (int)(sm + ld - i);
   }
   r.pad0 = r.pad1 = r.pad2 = 0;
+  return r;
+}
+
+// Tile boundaries of the lift of bit j0 + blockIdx.y (one bit per grid row;
+// row r writes bounds[r * nbound + u]).
+__global__ __launch_bounds__(NT) void k_lift_part(const uint64_t* __restrict__ w, int64_t n, int j0, int t,
+                                                   int tm, int64_t nbound, LiftBound* bounds) {
+  const int lane = threadIdx.x & 63;
+  const int j = j0 + (int)blockIdx.y;
+  bounds += (int64_t)blockIdx.y * nbound;
+  const int64_t u = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (u >= nbound) return;  // uniform per wave
+  const LiftBound r = lift_boundary(w, n, j, t, u * (int64_t)tm);
   if (lane == 0) bounds[u] = r;
 }
 
@@ -120,9 +126,9 @@ __device__ __forceinline__ int lds_lower_bound(const uint64_t* s, int cnt, uint6
 template <int TM>
 __device__ __forceinline__ int lift_stage(const uint64_t* __restrict__ w, const double2* __restrict__ a,
                                           int64_t n, int j, int t, const LiftBound& B0, const LiftBound& B1,
-                                          uint64_t* s_w, uint64_t* s_t, double2* s_a) {
+                                          int64_t tile, uint64_t* s_w, uint64_t* s_t, double2* s_a) {
   constexpr int ITEMS = TM / NT;
-  const int64_t d = (int64_t)blockIdx.x * TM;
+  const int64_t d = tile * TM;
   const int cnt = (int)((n - d) < TM ? (n - d) : TM);
   int64_t p1lo, p1n, p2lo, p2n, p3lo;
   if (B0.s0 == B1.s0) {  // the tile lies inside one run
@@ -176,11 +182,14 @@ __device__ __forceinline__ int lift_stage(const uint64_t* __restrict__ w, const 
   return cnt;
 }
 
+// One tile of a gate: local tile `tile` of the input (merged positions
+// [tile*TM, ...)), outputs to segment `otile` of (xo, ao), seg_cnt[otile],
+// tile_psum[otile]; digit-1 histogram into hist.
 template <int QB, int MODE>
-__global__ __launch_bounds__(NT) void k_lift_gate(const uint64_t* __restrict__ w, const double2* __restrict__ a,
-                                                   int64_t n, int j, const LiftBound* __restrict__ bounds,
-                                                   GateMat G, uint64_t* __restrict__ xo, double2* __restrict__ ao,
-                                                   unsigned int* hist, double* tile_psum, int* seg_cnt) {
+__device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int j,
+                               const LiftBound& B0, const LiftBound& B1, const GateMat& G, int64_t tile,
+                               int64_t otile, uint64_t* __restrict__ xo, double2* __restrict__ ao,
+                               unsigned int* hist, double* tile_psum, int* seg_cnt) {
   constexpr int M = 1 << QB;
   constexpr int T = QB - 1;
   constexpr int TM = SEG_ENTRIES / M;
@@ -197,10 +206,8 @@ __global__ __launch_bounds__(NT) void k_lift_gate(const uint64_t* __restrict__ w
   __shared__ int s_nhalo;
   __shared__ uint64_t s_prev[2];
 
-  const unsigned int tile = blockIdx.x;
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
-  const LiftBound B0 = bounds[tile], B1 = bounds[tile + 1];
-  const int cnt = lift_stage<TM>(w, a, n, j, T, B0, B1, s_w, s_t, s_a);
+  const int cnt = lift_stage<TM>(w, a, n, j, T, B0, B1, tile, s_w, s_t, s_a);
 
   // group key: the lifted key without the gate bits 0..T
   auto gk = [](uint64_t tk) { return tk >> (T + 1); };
@@ -303,10 +310,10 @@ __global__ __launch_bounds__(NT) void k_lift_gate(const uint64_t* __restrict__ w
   int tile_total = 0;
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) tile_total += rtot[it];
-  if (threadIdx.x == 0) seg_cnt[tile] = tile_total;
+  if (threadIdx.x == 0) seg_cnt[otile] = tile_total;
 
   double psum = 0.0;
-  int64_t rb = (int64_t)tile * SEG_ENTRIES;
+  int64_t rb = otile * SEG_ENTRIES;
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
     int64_t pos = rb + ooff[it];
@@ -325,26 +332,102 @@ __global__ __launch_bounds__(NT) void k_lift_gate(const uint64_t* __restrict__ w
     rb += rtot[it];
   }
   const double s = block_sum(psum, s_red);
-  if (threadIdx.x == 0) tile_psum[tile] = s;
+  if (threadIdx.x == 0) tile_psum[otile] = s;
   __syncthreads();
   for (int b = threadIdx.x; b < HIST_BINS; b += NT)
     if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
+}
+
+template <int QB, int MODE>
+__global__ __launch_bounds__(NT) void k_lift_gate(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                                   int64_t n, int j, const LiftBound* __restrict__ bounds,
+                                                   GateMat G, uint64_t* __restrict__ xo, double2* __restrict__ ao,
+                                                   unsigned int* hist, double* tile_psum, int* seg_cnt) {
+  const int64_t tile = blockIdx.x;
+  lift_gate_tile<QB, MODE>(w, a, n, j, bounds[tile], bounds[tile + 1], G, tile, tile, xo, ao, hist, tile_psum,
+                           seg_cnt);
+}
+
+template <int TM>
+__device__ void lift_move_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int j,
+                               const LiftBound& B0, const LiftBound& B1, int64_t tile, uint64_t* __restrict__ xo,
+                               double2* __restrict__ ao) {
+  __shared__ uint64_t s_w[TM];
+  __shared__ uint64_t s_t[TM];
+  __shared__ double2 s_a[TM];
+  const int cnt = lift_stage<TM>(w, a, n, j, 0, B0, B1, tile, s_w, s_t, s_a);
+  const int64_t d = tile * TM;
+  for (int m = threadIdx.x; m < cnt; m += NT) {
+    xo[d + m] = s_t[m];
+    ao[d + m] = s_a[m];
+  }
 }
 
 template <int TM>
 __global__ __launch_bounds__(NT) void k_lift_move(const uint64_t* __restrict__ w, const double2* __restrict__ a,
                                                    int64_t n, int j, const LiftBound* __restrict__ bounds,
                                                    uint64_t* __restrict__ xo, double2* __restrict__ ao) {
-  __shared__ uint64_t s_w[TM];
-  __shared__ uint64_t s_t[TM];
-  __shared__ double2 s_a[TM];
-  const LiftBound B0 = bounds[blockIdx.x], B1 = bounds[blockIdx.x + 1];
-  const int cnt = lift_stage<TM>(w, a, n, j, 0, B0, B1, s_w, s_t, s_a);
-  const int64_t d = (int64_t)blockIdx.x * TM;
-  for (int m = threadIdx.x; m < cnt; m += NT) {
-    xo[d + m] = s_t[m];
-    ao[d + m] = s_a[m];
+  lift_move_tile<TM>(w, a, n, j, bounds[blockIdx.x], bounds[blockIdx.x + 1], blockIdx.x, xo, ao);
+}
+
+// ------------------------------------------------------------- batches ----
+// A batch of independent states (one job each) in one launch: job c owns
+// output tiles [tile0_c, tile0_c + ntiles_c) and boundaries [b0_c, b0_c +
+// ntiles_c] (b0_c = tile0_c + c); blocks find their job by binary search.
+__device__ __forceinline__ int job_of_tile(const LiftJob* jobs, int njobs, int64_t gt) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].tile0 <= gt)
+      lo = mid;
+    else
+      hi = mid - 1;
   }
+  return lo;
+}
+
+__global__ __launch_bounds__(NT) void k_blift_part(const LiftJob* __restrict__ jobs, int njobs, int t, int tm,
+                                                    int64_t nbound, LiftBound* bounds) {
+  const int lane = threadIdx.x & 63;
+  const int64_t gb = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (gb >= nbound) return;  // uniform per wave
+  // job: the last one with b0 = tile0 + index <= gb
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].tile0 + mid <= gb)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const LiftJob& J = jobs[lo];
+  const int64_t u = gb - (J.tile0 + lo);
+  const LiftBound r = lift_boundary(J.x, J.n, J.j, t, u * (int64_t)tm);
+  if (lane == 0) bounds[gb] = r;
+}
+
+template <int QB, int MODE>
+__global__ __launch_bounds__(NT) void k_blift_gate(const LiftJob* __restrict__ jobs, int njobs,
+                                                    const LiftBound* __restrict__ bounds, uint64_t* __restrict__ xo,
+                                                    double2* __restrict__ ao, unsigned int* hist, double* tile_psum,
+                                                    int* seg_cnt) {
+  const int64_t gt = blockIdx.x;
+  const int c = job_of_tile(jobs, njobs, gt);
+  const LiftJob& J = jobs[c];
+  const int64_t u = gt - J.tile0;
+  const int64_t b = J.tile0 + c + u;
+  lift_gate_tile<QB, MODE>(J.x, J.a, J.n, J.j, bounds[b], bounds[b + 1], J.g, u, gt, xo, ao,
+                           hist + (int64_t)c * HIST_BINS, tile_psum, seg_cnt);
+}
+
+__global__ __launch_bounds__(NT) void k_blift_move(const LiftJob* __restrict__ jobs, int njobs,
+                                                    const LiftBound* __restrict__ bounds) {
+  const int64_t gt = blockIdx.x;
+  const int c = job_of_tile(jobs, njobs, gt);
+  const LiftJob& J = jobs[c];
+  const int64_t u = gt - J.tile0;
+  const int64_t b = J.tile0 + c + u;
+  lift_move_tile<1024>(J.x, J.a, J.n, J.j, bounds[b], bounds[b + 1], u, J.xo, J.ao);
 }
 
 // ------------------------------------------------ single-qubit RDMs ------
@@ -358,18 +441,15 @@ __global__ __launch_bounds__(NT) void k_lift_move(const uint64_t* __restrict__ w
 // gathers the two amplitudes of each pair.
 constexpr int RDM_TM = 1024;
 
-__global__ __launch_bounds__(NT) void k_rdm_pairs(const uint64_t* __restrict__ w, const double2* __restrict__ a,
-                                                   int64_t n, int64_t nbound, const LiftBound* __restrict__ bounds,
-                                                   double2* __restrict__ part) {
+// rho01 partial of stored bit p over merged tile u (bounds B0, B1).
+__device__ double2 rdm_pairs_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int p,
+                                  int64_t u, const LiftBound& B0, const LiftBound& B1) {
   constexpr int ITEMS = RDM_TM / NT;
   __shared__ uint64_t s_w[RDM_TM];
   __shared__ int s_g[RDM_TM];
   __shared__ double s_red[WAVES];
   __shared__ uint64_t s_hw;
   __shared__ int s_hg;
-  const int p = blockIdx.y;
-  const int64_t u = blockIdx.x;
-  const LiftBound B0 = bounds[(int64_t)p * nbound + u], B1 = bounds[(int64_t)p * nbound + u + 1];
   const int64_t d = u * RDM_TM;
   const int cnt = (int)((n - d) < RDM_TM ? (n - d) : RDM_TM);
   int64_t p1lo, p1n, p2lo, p2n, p3lo;
@@ -422,20 +502,29 @@ __global__ __launch_bounds__(NT) void k_rdm_pairs(const uint64_t* __restrict__ w
   }
   const double tr = block_sum(re, s_red);
   const double ti = block_sum(im, s_red);
-  if (threadIdx.x == 0) part[(int64_t)p * gridDim.x + u] = make_double2(tr, ti);
+  return make_double2(tr, ti);
+}
+
+__global__ __launch_bounds__(NT) void k_rdm_pairs(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                                   int64_t n, int64_t nbound, const LiftBound* __restrict__ bounds,
+                                                   double2* __restrict__ part) {
+  const int p = blockIdx.y;
+  const int64_t u = blockIdx.x;
+  const double2 v = rdm_pairs_tile(w, a, n, p, u, bounds[(int64_t)p * nbound + u], bounds[(int64_t)p * nbound + u + 1]);
+  if (threadIdx.x == 0) part[(int64_t)p * gridDim.x + u] = v;
 }
 
 // Diagonal RDM elements of every bit in one pass: per tile of SEG_ENTRIES
 // entries, lane b of each wave owns bit b and walks the wave's entries in
 // order (LDS broadcast reads).
-__global__ __launch_bounds__(NT) void k_rdm_diag(const uint64_t* __restrict__ w, const double2* __restrict__ a,
-                                                  int64_t n, int nbits, double2* __restrict__ dpart) {
+__device__ void rdm_diag_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int nbits,
+                              int64_t tile, double2* __restrict__ out) {
   constexpr int TILE = SEG_ENTRIES;
   constexpr int PER_WAVE = TILE / WAVES;
   __shared__ uint64_t s_w[TILE];
   __shared__ double s_p[TILE];
   __shared__ double2 s_acc[WAVES][64];
-  const int64_t d = (int64_t)blockIdx.x * TILE;
+  const int64_t d = tile * TILE;
   const int cnt = (int)((n - d) < TILE ? (n - d) : TILE);
   for (int j = threadIdx.x; j < cnt; j += NT) {
     s_w[j] = w[d + j];
@@ -461,8 +550,13 @@ __global__ __launch_bounds__(NT) void k_rdm_diag(const uint64_t* __restrict__ w,
       t.x += s_acc[k][threadIdx.x].x;
       t.y += s_acc[k][threadIdx.x].y;
     }
-    dpart[(int64_t)blockIdx.x * 64 + threadIdx.x] = t;
+    out[threadIdx.x] = t;
   }
+}
+
+__global__ __launch_bounds__(NT) void k_rdm_diag(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                                  int64_t n, int nbits, double2* __restrict__ dpart) {
+  rdm_diag_tile(w, a, n, nbits, blockIdx.x, dpart + (int64_t)blockIdx.x * 64);
 }
 
 // rho[p] = (rho00, rho11, re rho01, im rho01): one wave per bit, partials
@@ -496,6 +590,110 @@ __global__ __launch_bounds__(NT) void k_rdm_ordered_fin(const double2* __restric
   }
 }
 
+// ---- batched RDMs: job c covers nbits_c bits; flattened index spaces
+// boundaries [bound_off_c, + nbits_c (ptiles_c + 1)), pair tiles
+// [ptile_off_c, + nbits_c ptiles_c), diagonal tiles [dtile_off_c, + dtiles_c).
+template <int FIELD>
+__device__ __forceinline__ int rdm_job_of(const RdmJob* jobs, int njobs, int64_t g) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int64_t off = FIELD == 0 ? jobs[mid].bound_off : (FIELD == 1 ? jobs[mid].ptile_off : jobs[mid].dtile_off);
+    if (off <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(NT) void k_brdm_part(const RdmJob* __restrict__ jobs, int njobs, int64_t nbound,
+                                                   LiftBound* bounds) {
+  const int lane = threadIdx.x & 63;
+  const int64_t gb = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (gb >= nbound) return;  // uniform per wave
+  const RdmJob& J = jobs[rdm_job_of<0>(jobs, njobs, gb)];
+  const int64_t r = gb - J.bound_off;
+  const int p = (int)(r / (J.ptiles + 1));
+  const int64_t u = r % (J.ptiles + 1);
+  const LiftBound b = lift_boundary(J.x, J.n, p, 0, u * RDM_TM);
+  if (lane == 0) bounds[gb] = b;
+}
+
+__global__ __launch_bounds__(NT) void k_brdm_pairs(const RdmJob* __restrict__ jobs, int njobs,
+                                                    const LiftBound* __restrict__ bounds, double2* __restrict__ part) {
+  const int64_t gt = blockIdx.x;
+  const RdmJob& J = jobs[rdm_job_of<1>(jobs, njobs, gt)];
+  const int64_t r = gt - J.ptile_off;
+  const int p = (int)(r / J.ptiles);
+  const int64_t u = r % J.ptiles;
+  const int64_t b = J.bound_off + (int64_t)p * (J.ptiles + 1) + u;
+  const double2 v = rdm_pairs_tile(J.x, J.a, J.n, p, u, bounds[b], bounds[b + 1]);
+  if (threadIdx.x == 0) part[gt] = v;
+}
+
+__global__ __launch_bounds__(NT) void k_brdm_diag(const RdmJob* __restrict__ jobs, int njobs,
+                                                   double2* __restrict__ dpart) {
+  const int64_t gt = blockIdx.x;
+  const RdmJob& J = jobs[rdm_job_of<2>(jobs, njobs, gt)];
+  rdm_diag_tile(J.x, J.a, J.n, J.nbits, gt - J.dtile_off, dpart + gt * 64);
+}
+
+// rho[c][p] for p < nbits_c: one wave per (job, bit); blockIdx.y = job.
+__global__ __launch_bounds__(NT) void k_brdm_fin(const RdmJob* __restrict__ jobs, const double2* __restrict__ part,
+                                                  const double2* __restrict__ dpart, double* rho) {
+  const int lane = threadIdx.x & 63;
+  const RdmJob& J = jobs[blockIdx.y];
+  const int p = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (p >= J.nbits) return;
+  double z = 0.0, o = 0.0, re = 0.0, im = 0.0;
+  for (int64_t u = lane; u < J.dtiles; u += 64) {
+    const double2 v = dpart[(J.dtile_off + u) * 64 + p];
+    z += v.x;
+    o += v.y;
+  }
+  for (int64_t u = lane; u < J.ptiles; u += 64) {
+    const double2 v = part[J.ptile_off + (int64_t)p * J.ptiles + u];
+    re += v.x;
+    im += v.y;
+  }
+  z = wave_sum(z);
+  o = wave_sum(o);
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if (lane == 0) {
+    double* out = rho + ((int64_t)blockIdx.y * 64 + p) * 4;
+    out[0] = z;
+    out[1] = o;
+    out[2] = re;
+    out[3] = im;
+  }
+}
+
+int launch_rdm_batch(const RdmJob* d_jobs, int njobs, int64_t nbound, int64_t nptiles, int64_t ndtiles, void* scratch,
+                     double* rho, hipStream_t s) {
+  char* base = (char*)scratch;
+  LiftBound* bounds = (LiftBound*)base;
+  base += ((size_t)nbound * sizeof(LiftBound) + 255) & ~(size_t)255;
+  double2* part = (double2*)base;
+  base += ((size_t)nptiles * 16 + 255) & ~(size_t)255;
+  double2* dpart = (double2*)base;
+  if (nbound > 0)
+    hipLaunchKernelGGL(k_brdm_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES)), dim3(NT), 0, s, d_jobs, njobs,
+                       nbound, bounds);
+  if (nptiles > 0)
+    hipLaunchKernelGGL(k_brdm_pairs, dim3((unsigned)nptiles), dim3(NT), 0, s, d_jobs, njobs, bounds, part);
+  if (ndtiles > 0) hipLaunchKernelGGL(k_brdm_diag, dim3((unsigned)ndtiles), dim3(NT), 0, s, d_jobs, njobs, dpart);
+  hipLaunchKernelGGL(k_brdm_fin, dim3(64 / WAVES, (unsigned)njobs), dim3(NT), 0, s, d_jobs, part, dpart, rho);
+  return (int)hipGetLastError();
+}
+
+size_t rdm_batch_scratch(int64_t nbound, int64_t nptiles, int64_t ndtiles) {
+  return (size_t)nbound * sizeof(LiftBound) + (size_t)nptiles * 16 + (size_t)ndtiles * 64 * 16 + 3 * 256;
+}
+
+int rdm_tile_entries() { return RDM_TM; }
+
 size_t rdm_ordered_scratch(int64_t n, int nbits) {
   const int64_t ptiles = (n + RDM_TM - 1) / RDM_TM, dtiles = (n + SEG_ENTRIES - 1) / SEG_ENTRIES;
   return (size_t)nbits * (ptiles + 1) * sizeof(LiftBound) + (size_t)nbits * ptiles * 16 + (size_t)dtiles * 64 * 16 +
@@ -525,6 +723,38 @@ int launch_rdm_ordered(const uint64_t* x, const double2* a, int64_t n, int nbits
 // ----------------------------------------------------------- host launch ---
 
 int lift_tile_entries(int qb, int mode) { return mode == MODE_MOVE ? 1024 : SEG_ENTRIES / (1 << qb); }
+
+// Batched lift: jobs (device) with tile0 assigned in order; ntiles_total output
+// tiles, njobs + ntiles_total boundaries.
+int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, int njobs, int64_t ntiles_total, LiftBound* bounds,
+                      uint64_t* xo, double2* ao, unsigned int* hist, double* tile_psum, int* seg_cnt,
+                      hipStream_t s) {
+  if (njobs <= 0 || ntiles_total <= 0) return 0;
+  const int tm = lift_tile_entries(qb, mode);
+  const int t = (mode == MODE_MOVE) ? 0 : qb - 1;
+  const int64_t nbound = ntiles_total + njobs;
+  hipLaunchKernelGGL(k_blift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES)), dim3(NT), 0, s, d_jobs, njobs, t,
+                     tm, nbound, bounds);
+  const dim3 grid((unsigned)ntiles_total);
+  if (mode == MODE_MOVE) {
+    hipLaunchKernelGGL(k_blift_move, grid, dim3(NT), 0, s, d_jobs, njobs, bounds);
+  } else if (qb == 1) {
+    if (mode == MODE_GATE)
+      hipLaunchKernelGGL((k_blift_gate<1, MODE_GATE>), grid, dim3(NT), 0, s, d_jobs, njobs, bounds, xo, ao, hist,
+                         tile_psum, seg_cnt);
+    else
+      hipLaunchKernelGGL((k_blift_gate<1, MODE_ROT>), grid, dim3(NT), 0, s, d_jobs, njobs, bounds, xo, ao, hist,
+                         tile_psum, seg_cnt);
+  } else {
+    if (mode == MODE_GATE)
+      hipLaunchKernelGGL((k_blift_gate<2, MODE_GATE>), grid, dim3(NT), 0, s, d_jobs, njobs, bounds, xo, ao, hist,
+                         tile_psum, seg_cnt);
+    else
+      hipLaunchKernelGGL((k_blift_gate<2, MODE_ROT>), grid, dim3(NT), 0, s, d_jobs, njobs, bounds, xo, ao, hist,
+                         tile_psum, seg_cnt);
+  }
+  return (int)hipGetLastError();
+}
 
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,

@@ -52,6 +52,25 @@ typedef struct bass_stats {
   int32_t flags; /* bass_truncate: bit 0 = entries were dropped (0: kept all, no renormalisation) */
 } bass_stats;
 
+/* One state of a batched call (bass_batch_*).  Device pointers; the table
+ * itself is host memory, staged by the library. */
+typedef struct bass_job {
+  const uint64_t* x;  /* input keys (stored words of a key-ordered state) */
+  const double* a;    /* input amplitudes (re, im) */
+  uint64_t* xo;       /* output keys: truncation / move destination */
+  double* ao;
+  int64_t n;          /* input entries */
+  int64_t k;          /* truncation size */
+  int64_t cap_out;    /* entries xo / ao hold */
+  int32_t j;          /* stored bit lifted; first bit of a diagonal gate */
+  int32_t j2;         /* second bit of a diagonal gate */
+  int32_t renorm;     /* truncation: divide kept entries by sqrt(kept) */
+  int32_t key_bits;   /* entries of key_map (0: keys stored as is) */
+  double g[32];       /* gate matrix (re, im) row-major, 2x2 or 4x4; diagonal gate: 4 values */
+  int32_t key_map[64];/* stored bit -> qubit */
+  double scale;       /* bass_batch_scale divisor */
+} bass_job;
+
 int bass_abi_version(void);
 const char* bass_last_error(void);
 
@@ -167,6 +186,40 @@ int bass_rdm1q(const uint64_t* x, const double* a, int64_t n, int N, double* rho
  * (src/simulation/bass_simulator.py:441-470) inside the driver.  Synchronous. */
 int bass_rdm1q_ordered(const uint64_t* x, const double* a, int64_t n, int N, double* rho_out, void* ws,
                        size_t ws_bytes, void* stream);
+
+/* ---- batches: many independent key-ordered states per launch ----------
+ * Each call runs one launch sequence for all njobs states (per-job scalars via
+ * bass_batch_stats).  Job tables are staged through pinned host buffers.
+ *
+ * bass_batch_step: for every job, the gate (mode 0) or rotation (mode 1) of
+ * bass_lift on (x, a, n, j, g), then bass_truncate of its output into (xo, ao)
+ * with k / renorm / key_map, keeping all entries when there are <= k (stats
+ * flags bit 0 says whether entries were dropped).  tiles_x / tiles_a (capacity
+ * tiles_cap) hold the gate outputs: sum over jobs of ceil(n / (2048 >> nq)) *
+ * 2048 slots.  wide=1: multi-block threshold resolution (repeat a call whose
+ * stats report flags bit 1).  Replaces, per circuit, the gate + truncation
+ * step of src/simulation/bass_simulator.py:861-916 and the tentative rotation
+ * of :768-783. */
+int bass_batch_step(const bass_job* jobs, int njobs, int nq, int mode, int wide, uint64_t* tiles_x, double* tiles_a,
+                    int64_t tiles_cap, void* ws, size_t ws_bytes, void* stream);
+
+/* bass_lift(mode=2) per job: move stored bit j to bit 0, dense output in (xo, ao). */
+int bass_batch_move(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream);
+
+/* Per job in place: diagonal 2-qubit gate on stored bits (j, j2) with the 4
+ * values g[0..7] (src/core/sparse_state.py:178-190); division by `scale`
+ * (:551-554); moments sum |a|^2, sum |a|^4 into the job's stats. */
+int bass_batch_diag2q(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream);
+int bass_batch_scale(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream);
+int bass_batch_moments(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream);
+
+/* Single-qubit RDMs of every stored bit p < key_bits of every job (as
+ * bass_rdm1q_ordered); rho_out: njobs x 64 x 4 doubles.  Synchronous. */
+int bass_batch_rdm1q(const bass_job* jobs, int njobs, double* rho_out, void* ws, size_t ws_bytes, void* stream);
+
+/* Synchronous: the per-job stats of the last batch call (njobs entries);
+ * clears device error bits.  BASS_E_DEVICE if any job reports one. */
+int bass_batch_stats(void* ws, size_t ws_bytes, int njobs, bass_stats* out, void* stream);
 
 int bass_ht_clear(uint64_t* ht_keys, int32_t* ht_vals, int log2cap, void* stream);
 int bass_ht_insert(uint64_t* ht_keys, int32_t* ht_vals, int log2cap, const uint64_t* keys,
