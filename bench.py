@@ -95,12 +95,95 @@ def measured_traffic(workload, n, k, op):
     return None
 
 
+def bench_batched(args):
+    """Config 3: independent TFIM circuits (n ~ U[16,48], steps ~ U[10,60],
+    h ~ U[0.2,2], J = 1, dt = 0.05, K = 2^U{10..18}) run as one lock-step
+    batch; baseline: the same circuits one at a time through the plain driver
+    (a --single-sample subset, timed, as circuits/s)."""
+    import numpy as np
+    import torch
+
+    S = importlib.import_module(PKG + ".serving.server")
+    Bm = importlib.import_module(PKG + ".simulation.batched")
+    Bs = importlib.import_module(PKG + ".simulation.bass_simulator")
+    importlib.import_module(PKG + "._lib").load()
+    reqs = S.make_batched_requests(args.circuits)
+    specs = []
+    for r in reqs:
+        circ, kw = S.build_circuit(r)
+        specs.append((r.n, 1 << r.log2k, circ, kw.get("optimize_at")))
+    log(f"batched: {len(specs)} circuits, {sum(len(s[2]) for s in specs)} gates, "
+        f"sum K = {sum(s[1] for s in specs)}")
+    # one at a time (plain driver), on a sample of the same stream
+    sample = specs[: args.single_sample]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    single_amp = 0
+    for n, k, circ, oat in sample:
+        sim = Bs.BASS(n, k, device="cuda:0", **({"optimize_at": oat} if oat else {}))
+        sim.simulate(circ)
+        single_amp += sim.counters.get("amp_updates", 0)
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    single_rate = len(sample) / t_single
+    log(f"one at a time: {len(sample)} circuits in {t_single:.2f} s = {single_rate:.2f} circuits/s")
+    # the batch
+    bat = Bm.BatchedBASS(specs, device="cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = bat.run()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    rate = len(specs) / t_batch
+    log(f"batched: {len(specs)} circuits in {t_batch:.2f} s = {rate:.2f} circuits/s")
+    out = {
+        "metric": "batched circuits/s", "value": rate, "unit": "circuits/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex128", "data": "synthetic",
+        "config": {"workload": f"batched TFIM circuits={len(specs)} n~U[16,48] steps~U[10,60] h~U[0.2,2] "
+                               f"K=2^U{{10..18}} (step = the whole batch)",
+                   "circuits": len(specs), "gates": int(sum(len(s[2]) for s in specs)),
+                   "amp_updates": int(bat.counters.get("amp_updates", 0))},
+        "one_at_a_time": {"value": single_rate, "unit": "circuits/s", "sample": len(sample),
+                          "seconds": t_single, "driver": "plain BASS, one circuit after another"},
+        "speedup_vs_one_at_a_time": rate / single_rate,
+        "amp_updates_per_s": bat.counters.get("amp_updates", 0) / t_batch,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_served(args):
+    """Config 4: Poisson stream (--rate) of 50/50 TFIM and brickwork circuits
+    served by a worker pool with HBM-budget admission; p50/p99 latency."""
+    S = importlib.import_module(PKG + ".serving.server")
+    reqs = S.make_served_requests(args.circuits, args.rate)
+    with S.BassServer(workers=args.workers, hbm_budget=200e9) as srv:
+        recs = srv.run(reqs, progress=lambda d, t: log(f"served {d}/{t}") if d % 16 == 0 else None)
+    summ = S.summarize(recs)
+    out = {
+        "metric": "served circuits/s", "value": summ["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": summ["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
+        "config": {"workload": f"served Poisson {args.rate:g} req/s, {args.circuits} requests, 50/50 TFIM "
+                               f"(K=2^U{{12..20}}) / brickwork n~U[24,44] depth~U[10,30]",
+                   "workers": args.workers, "hbm_budget_bytes": 200e9},
+        "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"],
+        "mean_service_s": summ["mean_service_s"], "amp_updates": summ["amp_updates"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=25, help="timed layers")
     ap.add_argument("--warmup", type=int, default=5, help="untimed layers")
-    ap.add_argument("--workload", choices=["brickwork", "kicked-ising"], default="brickwork")
+    ap.add_argument("--workload", choices=["brickwork", "kicked-ising", "batched", "served"], default="brickwork")
+    ap.add_argument("--circuits", type=int, default=512, help="batched / served: circuits in the stream")
+    ap.add_argument("--single-sample", type=int, default=24,
+                    help="batched / served: circuits timed one at a time for the baseline")
+    ap.add_argument("--rate", type=float, default=200.0, help="served: Poisson arrival rate (req/s)")
+    ap.add_argument("--workers", type=int, default=8, help="served: worker processes")
     ap.add_argument("--qubits", type=int, default=None)
     ap.add_argument("--log2k", type=int, default=None)
     ap.add_argument("--calibrate-copy", type=float, default=None, metavar="MIB")
@@ -125,6 +208,11 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": args.steps}), flush=True)
         return
+
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "served":
+        return bench_served(args)
 
     lib = importlib.import_module(PKG + "._lib")
     rc = importlib.import_module(PKG + ".utils.random_circuits")
