@@ -111,6 +111,11 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    # torch first: the library must bind to the HIP runtime torch loads (the
+    # one that owns the device memory and streams it is handed).  Loaded
+    # first, it would pull in a second runtime that sees no device.
+    import torch  # noqa: F401
+
     L = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(L, name)

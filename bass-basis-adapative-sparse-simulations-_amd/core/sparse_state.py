@@ -33,10 +33,19 @@ def stored_to_keys(w, pos):
     w = np.asarray(w, dtype=np.uint64)
     if pos is None or np.array_equal(pos, np.arange(len(pos))):
         return w.copy()
+    # one 256-entry table per stored byte: the basis bits that byte carries
+    pos = np.asarray(pos, dtype=np.int64)
+    inv = np.full(64, -1, dtype=np.int64)
+    inv[pos] = np.arange(len(pos))
+    v = np.arange(256, dtype=np.uint64)
     out = np.zeros_like(w)
-    one = np.uint64(1)
-    for q, p in enumerate(pos):
-        out |= ((w >> np.uint64(p)) & one) << np.uint64(q)
+    for b in range((int(pos.max()) >> 3) + 1):
+        table = np.zeros(256, dtype=np.uint64)
+        for i in range(8):
+            q = inv[8 * b + i]
+            if q >= 0:
+                table |= ((v >> np.uint64(i)) & np.uint64(1)) << np.uint64(q)
+        out |= table[((w >> np.uint64(8 * b)) & np.uint64(255)).astype(np.intp)]
     return out
 
 

@@ -557,8 +557,15 @@ static int stage_copy(void* dev, const void* src, size_t bytes, hipStream_t s) {
     if (S.host) (void)hipHostFree(S.host);
     S.host = nullptr;
     S.cap = 0;
-    if (hipHostMalloc(&S.host, bytes) != hipSuccess) return fail(BASS_E_HIP, "pinned staging allocation failed");
-    S.cap = bytes;
+    // grow geometrically from 256 KiB so a slot is reallocated a few times at most
+    size_t want = (size_t)1 << 18;
+    while (want < bytes) want <<= 1;
+    const hipError_t e = hipHostMalloc(&S.host, want);
+    if (e != hipSuccess) {
+      S.host = nullptr;
+      return fail(BASS_E_HIP, "pinned staging allocation of %zu bytes failed: %s", want, hipGetErrorString(e));
+    }
+    S.cap = want;
   }
   if (!S.done && hipEventCreateWithFlags(&S.done, hipEventDisableTiming) != hipSuccess)
     return fail(BASS_E_HIP, "staging event creation failed");

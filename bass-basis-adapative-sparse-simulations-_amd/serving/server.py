@@ -1,25 +1,29 @@
-"""Circuit server: many independent BASS circuits on one MI355X.
+"""Circuit servers: many independent BASS circuits on one MI355X.
 
-A pool of worker processes shares the GPU.  Each worker owns a HIP stream,
-a library workspace and its state buffers, and runs whole circuits through
-the plain device path (BASS driver, key-ordered kernels); independent
-circuits overlap on the GPU because every worker submits to its own queues,
-and the per-gate host work of different circuits runs in parallel instead of
-behind one interpreter.
+BatchServer (the default) runs every admitted circuit in a continuous batch
+(simulation.batched.BatchEngine): each tick advances every running circuit by
+one operation (a gate or an optimiser step) with one launch sequence per
+operation kind, arrivals join between ticks, and a circuit leaves (its result
+copied to the host) as soon as it is done.  Admission is FIFO while the
+engine has a free slot and slab region and the summed HBM footprint of the
+circuits in flight stays within the budget (footprint_bytes).  With several
+workers, each worker process runs its own engine and the dispatcher balances
+arrivals over them.
 
-The dispatcher admits a request when a worker is idle and the summed HBM
-footprint of the requests in flight stays within the budget (footprint_bytes:
-state, gate-output and rotation buffers, workspace).  Requests carry the
-parameters of their circuit, not the gates; the worker builds the circuit.
+BassServer runs whole circuits one per worker process through the plain
+device path (BASS driver), each worker with its own HIP stream, workspace and
+buffers; it is kept as the one-circuit-per-stream comparison.
+
+Requests carry the parameters of their circuit; build_circuit makes its gate
+list when the request arrives (part of its latency) in both servers.
 
 Timing uses time.monotonic() (CLOCK_MONOTONIC, shared by all processes):
 latency = completion - arrival, so it includes queueing.
 """
 
+import collections
 import dataclasses
-import math
 import multiprocessing as mp
-import os
 import queue
 import time
 
@@ -136,6 +140,221 @@ def _worker(wid, device, inq, outq, blocking_sync):
         outq.put(("done", wid, {"rid": req.rid, "start": t0, "end": t1, "result": res, "error": err}))
 
 
+def _request_cost(req):
+    """Work estimate used to balance engine workers: K x qubits x depth."""
+    return float(1 << req.log2k) * req.n * req.depth
+
+
+class _EngineLoop:
+    """Admission + ticking of one BatchEngine (FIFO, HBM budget)."""
+
+    def __init__(self, engine, budget):
+        self.eng = engine
+        self.budget = float(budget)
+        self.pending = collections.deque()
+        self.used = 0.0
+        self.inflight = {}  # rid -> (request, footprint, start, gates)
+
+    def add(self, req):
+        circ, kw = build_circuit(req)
+        self.pending.append((req, circ, kw))
+
+    def admit(self):
+        while self.pending:
+            r, circ, kw = self.pending[0]
+            fp = footprint_bytes(r)
+            if self.eng.active and self.used + fp > self.budget:
+                return
+            if not self.eng.admit(r.n, 1 << r.log2k, circ, kw.get("optimize_at"), tag=r.rid):
+                return
+            self.pending.popleft()
+            self.used += fp
+            self.inflight[r.rid] = (r, fp, time.monotonic(), len(circ))
+
+    def tick(self):
+        """One engine tick; returns completion records (start, end, result)."""
+        out = []
+        for rid, res in self.eng.tick():
+            r, fp, t0, gates = self.inflight.pop(rid)
+            self.used -= fp
+            out.append({"rid": rid, "start": t0, "end": time.monotonic(), "nnz": int(res["nnz"]),
+                        "gamma2": res["gamma"] ** 2, "gates": gates})
+        return out
+
+    @property
+    def busy(self):
+        return bool(self.eng.active or self.pending)
+
+
+def _engine_worker(wid, device, cfg, inq, outq):
+    import importlib
+    import traceback
+
+    try:
+        import torch
+
+        importlib.import_module(PKG + "._lib").load()
+        torch.cuda.set_device(device)
+        Bm = importlib.import_module(PKG + ".simulation.batched")
+        loop = _EngineLoop(Bm.BatchEngine(cfg["max_circuits"], cfg["slab_entries"], device=device, **cfg["kw"]),
+                           cfg["budget"])
+        outq.put(("ready", wid, None))
+        closing = False
+        while True:
+            block = not loop.busy and not closing
+            try:
+                while True:
+                    msg = inq.get(timeout=0.5) if block else inq.get_nowait()
+                    block = False
+                    if msg is None:
+                        closing = True
+                        break
+                    loop.add(msg)
+            except queue.Empty:
+                pass
+            loop.admit()
+            if loop.eng.active:
+                for rec in loop.tick():
+                    outq.put(("done", wid, rec))
+            elif closing and not loop.pending:
+                break
+        outq.put(("counters", wid, dict(loop.eng.counters)))
+    except Exception:  # reported to the dispatcher, which raises
+        outq.put(("error", wid, traceback.format_exc()))
+
+
+class BatchServer:
+    """Continuous-batching server on one GPU.
+
+    workers = 1 runs one engine in this process; workers > 1 runs one engine
+    per worker process (each with its own HIP stream and workspace, so one
+    engine's host work overlaps the others' kernels) and a dispatcher that
+    sends each arrival to the worker with the least outstanding work.
+    max_circuits / slab_entries (24 B entries, two slabs) are per engine;
+    hbm_budget is shared evenly.  Engine keywords (optimize_every, ...) pass
+    through.  Use as a context manager."""
+
+    def __init__(self, workers=1, max_circuits=512, slab_entries=1 << 27, hbm_budget=200e9, device="cuda:0",
+                 **kw):
+        self.workers = int(workers)
+        self.hbm_budget = float(hbm_budget)
+        self.device = device
+        self.counters = {}
+        self._cfg = {"max_circuits": int(max_circuits), "slab_entries": int(slab_entries),
+                     "budget": self.hbm_budget / max(1, self.workers), "kw": kw}
+        self._loop = None
+        self._procs = []
+        self._inqs = []
+
+    def __enter__(self):
+        if self.workers <= 1:
+            import importlib
+
+            Bm = importlib.import_module(PKG + ".simulation.batched")
+            self._loop = _EngineLoop(Bm.BatchEngine(self._cfg["max_circuits"], self._cfg["slab_entries"],
+                                                    device=self.device, **self._cfg["kw"]), self._cfg["budget"])
+            return self
+        ctx = mp.get_context("spawn")
+        self._outq = ctx.Queue()
+        for w in range(self.workers):
+            q = ctx.Queue()
+            p = ctx.Process(target=_engine_worker, args=(w, self.device, self._cfg, q, self._outq), daemon=True)
+            p.start()
+            self._inqs.append(q)
+            self._procs.append(p)
+        ready = 0
+        while ready < self.workers:
+            kind, w, msg = self._outq.get(timeout=600)
+            if kind == "error":
+                raise RuntimeError(f"engine worker {w} failed to start:\n{msg}")
+            ready += kind == "ready"
+        return self
+
+    def __exit__(self, *exc):
+        if self._loop is not None:
+            self.counters = dict(self._loop.eng.counters)
+            return False
+        for q in self._inqs:
+            q.put(None)
+        left = len(self._procs)
+        tot = collections.Counter()
+        while left and exc[0] is None:
+            try:
+                kind, w, msg = self._outq.get(timeout=120)
+            except queue.Empty:
+                break
+            if kind == "counters":
+                tot.update(msg)
+                left -= 1
+        self.counters = dict(tot)
+        for p in self._procs:
+            p.join(timeout=60)
+            if p.is_alive():
+                p.terminate()
+        return False
+
+    @property
+    def engine(self):
+        """The in-process engine (workers = 1)."""
+        return self._loop.eng if self._loop is not None else None
+
+    def run(self, requests, progress=None):
+        """Serve `requests` (arrival times relative to the start of the
+        stream).  Returns one record per request (arrival, start, end, latency,
+        service, result summary) in request order."""
+        import torch
+
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        by_id = {r.rid: r for r in reqs}
+        recs = {}
+        torch.cuda.synchronize(self.device)
+        t_base = time.monotonic()
+
+        def record(w, m):
+            r = by_id[m["rid"]]
+            arrival = t_base + r.arrival
+            recs[r.rid] = {"kind": r.kind, "n": r.n, "depth": r.depth, "log2k": r.log2k, "arrival": arrival,
+                           "latency": m["end"] - arrival, "service": m["end"] - m["start"], "worker": w, **m}
+            if progress:
+                progress(len(recs), len(reqs))
+
+        head = 0
+        if self._loop is not None:
+            loop = self._loop
+            while len(recs) < len(reqs):
+                now = time.monotonic() - t_base
+                while head < len(reqs) and reqs[head].arrival <= now:
+                    loop.add(reqs[head])
+                    head += 1
+                loop.admit()
+                if loop.eng.active:
+                    for m in loop.tick():
+                        record(0, m)
+                elif head < len(reqs):
+                    time.sleep(max(0.0, min(0.01, reqs[head].arrival - (time.monotonic() - t_base))))
+            return [recs[r.rid] for r in reqs]
+        load = [0.0] * self.workers
+        while len(recs) < len(reqs):
+            now = time.monotonic() - t_base
+            while head < len(reqs) and reqs[head].arrival <= now:
+                r = reqs[head]
+                w = int(np.argmin(load))
+                load[w] += _request_cost(r)
+                self._inqs[w].put(r)
+                head += 1
+            wait = 0.5 if head >= len(reqs) else max(1e-3, min(0.5, reqs[head].arrival - now))
+            try:
+                kind, w, msg = self._outq.get(timeout=wait)
+            except queue.Empty:
+                continue
+            if kind == "error":
+                raise RuntimeError(f"engine worker {w} failed:\n{msg}")
+            if kind == "done":
+                load[w] -= _request_cost(by_id[msg["rid"]])
+                record(w, msg)
+        return [recs[r.rid] for r in reqs]
+
+
 class BassServer:
     """Worker pool + dispatcher.  Use as a context manager."""
 
@@ -220,12 +439,14 @@ class BassServer:
         return [recs[r.rid] for r in reqs]
 
 
-def summarize(records):
-    """circuits/s over the stream, p50 / p99 latency (s)."""
+def summarize(records, amp_updates=None):
+    """circuits/s over the stream, p50 / p99 latency (s); amp_updates: the
+    total when records do not carry per-request counts."""
     t0 = min(r["arrival"] for r in records)
     t1 = max(r["end"] for r in records)
     lat = np.array([r["latency"] for r in records])
     return {"circuits": len(records), "wall_s": t1 - t0, "circuits_per_s": len(records) / (t1 - t0),
             "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_service_s": float(np.mean([r["service"] for r in records])),
-            "amp_updates": int(sum(r["amp_updates"] for r in records))}
+            "amp_updates": int(amp_updates if amp_updates is not None
+                               else sum(r.get("amp_updates", 0) for r in records))}

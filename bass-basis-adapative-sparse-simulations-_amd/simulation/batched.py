@@ -1,20 +1,23 @@
-"""BASS over many independent circuits in lock-step on one GPU.
+"""BASS over many independent circuits at once on one GPU.
 
 Every circuit follows the BASS driver's rules (bass_simulator.BASS with
 truncate_every = 1: truncate iff nnz > k, optimiser checks at the same gate
-indices, the same accept rule); the batch advances one gate index ("slot") at a
-time and runs each kind of operation for all circuits of the slot as one
+indices, the same accept rule).  The engine advances every running circuit by
+one gate per tick and runs each kind of operation for all of them as one
 launch sequence over a job table (bass_batch_*), so the launch and
-synchronisation cost of a slot is shared by the whole batch.  The optimiser
-is batched the same way: one RDM launch per pass for every circuit in it, and
-one rotation launch per qubit index for every circuit that tries it.  The host
+synchronisation cost of a tick is shared by the whole batch.  The optimiser is
+batched the same way: one RDM launch per pass for every circuit in it, and one
+rotation launch per qubit index for every circuit that tries it.  The host
 bookkeeping (frames, qubit positions, key maps, decisions) is vectorised over
-the circuits of a slot.
+the circuits of a tick.
 
-States live in two slabs; circuit c owns entries [off_c, off_c + k_c) of both
-and a flag telling which slab holds its state (moves and tentative rotations
-write the other one, accepting flips the flag).  Per circuit the device
-operations and their order are those of the single-circuit driver.
+Circuits keep their own gate counters, so a circuit can join between ticks
+(continuous batching, BatchEngine.admit) and leaves as soon as its last gate
+is done.  States live in two pooled slabs; circuit c owns entries
+[off_c, off_c + k_c) of both and a flag telling which slab holds its state
+(moves and tentative rotations write the other one, accepting flips the flag).
+Per circuit the device operations and their order are those of the
+single-circuit driver, so results do not depend on what else is in the batch.
 """
 
 import numpy as np
@@ -90,65 +93,130 @@ def _eigh2x2_rows(rho):
     return V, trivial
 
 
-class BatchedBASS:
-    """specs: list of (num_qubits, k, circuit, optimize_at or None).
+class BatchEngine:
+    """Continuous batch of BASS circuits on one GPU.
+
+    Up to max_circuits circuits run at once; each keeps its own gate counter,
+    and one tick() applies the next gate of every running circuit (and the
+    optimiser of those at a check point) with one launch sequence per
+    operation kind.  admit() places a circuit in a free slot and a free region
+    of the two pooled state slabs (first fit, slab_entries entries each);
+    tick() returns the circuits that finished, whose regions are released.
 
     Keywords as bass_simulator.BASS (optimize_every, pr_opt_threshold,
-    use_diag_gate, accept_rtol, trace); truncate_every must be 1.
+    use_diag_gate, accept_rtol); truncate_every = 1 (truncate iff nnz > k).
     """
 
-    def __init__(self, specs, device=None, optimize_every=5, pr_opt_threshold=0.90, use_diag_gate=True,
-                 accept_rtol=1e-10, truncate_every=1, trace=False):
-        if truncate_every != 1:
-            raise NotImplementedError("the batched driver implements truncate_every = 1")
+    def __init__(self, max_circuits, slab_entries, device=None, optimize_every=5, pr_opt_threshold=0.90,
+                 use_diag_gate=True, accept_rtol=1e-10, trace=False):
         self._ctx = context(device)
-        self.B = len(specs)
-        self.N = np.array([int(s[0]) for s in specs], dtype=np.int64)
-        if self.B and self.N.max() > 63:
-            raise ValueError("at most 63 qubits")
-        self.k = np.array([int(s[1]) for s in specs], dtype=np.int64)
-        self.circuits = [list(s[2]) for s in specs]
-        self.optimize_at = [None if s[3] is None else frozenset(int(i) for i in s[3]) for s in specs]
+        C = int(max_circuits)
+        self.C = C
         self.optimize_every = optimize_every
         self.pr_opt_threshold = pr_opt_threshold
         self.use_diag_gate = use_diag_gate
         self.accept_rtol = float(accept_rtol)
-        self.traces = [[] for _ in range(self.B)] if trace else None
+        self.trace = trace
         self.counters = {"amp_updates": 0}
-
-    # --------------------------------------------------------------- state --
-    def _setup(self):
-        B = self.B
-        self.off = np.concatenate([[0], np.cumsum(self.k)[:-1]]).astype(np.int64)
-        total = int(np.sum(self.k))
-        self.slabs = [Buffers(total, self._ctx.device), Buffers(total, self._ctx.device)]
+        self.slab_entries = int(slab_entries)
+        self.slabs = [Buffers(self.slab_entries, self._ctx.device), Buffers(self.slab_entries, self._ctx.device)]
         self.tiles = Buffers(1 << 16, self._ctx.device)
-        for s in self.slabs:
-            s.x[self.off] = 0
-            s.a[self.off] = 1.0 + 0.0j
         self.xbase = np.array([s.x.data_ptr() for s in self.slabs], dtype=np.uint64)
         self.abase = np.array([s.a.data_ptr() for s in self.slabs], dtype=np.uint64)
-        self.cur = np.zeros(B, dtype=np.int64)
-        self.nnz = np.ones(B, dtype=np.int64)
+        self.free = [(0, self.slab_entries)]  # free regions (start, length), sorted
+        self.alive = np.zeros(C, dtype=bool)
+        self.N = np.ones(C, dtype=np.int64)
+        self.k = np.ones(C, dtype=np.int64)
+        self.off = np.zeros(C, dtype=np.int64)
+        self.cur = np.zeros(C, dtype=np.int64)
+        self.nnz = np.ones(C, dtype=np.int64)
+        self.pos = np.zeros((C, NMAX), dtype=np.int64)
+        self.U = np.zeros((C, NMAX, 2, 2), dtype=np.complex128)
+        self.gamma_run = np.ones(C)
+        self.trunc_active = np.zeros(C, dtype=bool)
+        self.pr_last = np.zeros(C)
+        self.sum2 = np.full(C, np.nan)
+        self.sum4 = np.full(C, np.nan)
+        self.gidx = np.zeros(C, dtype=np.int64)
+        self.lens = np.zeros(C, dtype=np.int64)
+        self.circuits = [None] * C
+        self.chk = [set() for _ in range(C)]
+        self.tags = [None] * C
+        self.traces = [[] for _ in range(C)] if trace else None
+        # optimiser state machine (phase 1): stage 0 = RDM due (pass start),
+        # 1 = rotation trials along the pass's chain, 2 = final renormalisation
+        self.phase = np.zeros(C, dtype=np.int64)
+        self.ostage = np.zeros(C, dtype=np.int64)
+        self.opass = np.zeros(C, dtype=np.int64)
+        self.oentered = np.zeros(C, dtype=bool)
+        self.oimproved = np.zeros(C, dtype=bool)
+        self.opr = np.zeros(C)
+        self.ochain = np.zeros((C, NMAX), dtype=np.int64)
+        self.oclen = np.zeros(C, dtype=np.int64)
+        self.optr = np.zeros(C, dtype=np.int64)
+        self.oV = np.zeros((C, NMAX, 2, 2), dtype=np.complex128)
+
+    @property
+    def active(self):
+        return int(self.alive.sum())
+
+    def _alloc(self, n):
+        for r, (st, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(r)
+                else:
+                    self.free[r] = (st + n, ln - n)
+                return st
+        return None
+
+    def _release(self, st, n):
+        self.free.append((st, n))
+        self.free.sort()
+        merged = []
+        for a, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == a:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((a, ln))
+        self.free = merged
+
+    def admit(self, num_qubits, k, circuit, optimize_at=None, tag=None):
+        """Start a circuit from |0...0>; False if no slot or slab region is free."""
+        if not (1 <= num_qubits <= 63):
+            raise ValueError("1 to 63 qubits")
+        slots = np.flatnonzero(~self.alive)
+        if not len(slots):
+            return False
+        st = self._alloc(int(k))
+        if st is None:
+            return False
+        c = int(slots[0])
+        M = len(circuit)
+        self.alive[c] = True
+        self.N[c], self.k[c], self.off[c] = num_qubits, k, st
+        self.cur[c], self.nnz[c], self.gidx[c], self.lens[c] = 0, 1, 0, M
         q = np.arange(NMAX)
-        self.pos = np.where(q[None, :] < self.N[:, None], q[None, :], _BAD + q[None, :])
-        self.U = np.zeros((B, NMAX, 2, 2), dtype=np.complex128)
-        self.U[:, :, 0, 0] = 1.0
-        self.U[:, :, 1, 1] = 1.0
-        self.gamma_run = np.ones(B)
-        self.trunc_active = np.zeros(B, dtype=bool)
-        self.pr_last = np.zeros(B)
-        self.sum2 = np.full(B, np.nan)
-        self.sum4 = np.full(B, np.nan)
-        self.lens = np.array([len(c) for c in self.circuits], dtype=np.int64)
-        # check-point slots per circuit
-        self.chk = [set() for _ in range(B)]
-        for c in range(B):
-            M = int(self.lens[c])
-            if self.optimize_at[c] is not None:
-                self.chk[c] = {i for i in self.optimize_at[c] if i + 1 < M}
-            elif self.optimize_every > 0:
-                self.chk[c] = set(range(self.optimize_every - 1, M - 1, self.optimize_every))
+        self.pos[c] = np.where(q < num_qubits, q, _BAD + q)
+        self.U[c] = 0
+        self.U[c, :, 0, 0] = 1.0
+        self.U[c, :, 1, 1] = 1.0
+        self.gamma_run[c], self.trunc_active[c], self.pr_last[c] = 1.0, False, 0.0
+        self.phase[c] = 0
+        self.sum2[c] = self.sum4[c] = np.nan
+        self.circuits[c] = list(circuit)
+        if optimize_at is not None:
+            self.chk[c] = {int(i) for i in optimize_at if int(i) + 1 < M}
+        elif self.optimize_every > 0:
+            self.chk[c] = set(range(self.optimize_every - 1, M - 1, self.optimize_every))
+        else:
+            self.chk[c] = set()
+        self.tags[c] = tag
+        if self.traces is not None:
+            self.traces[c] = []
+        self.slabs[0].x[st] = 0
+        self.slabs[0].a[st] = 1.0 + 0.0j
+        return True
 
     def _jobs(self, cs, out_other=False):
         """Job table for circuits cs: input = current slab, output = the same
@@ -171,7 +239,7 @@ class BatchedBASS:
     def _count(self, key, v=1):
         self.counters[key] = self.counters.get(key, 0) + int(v)
 
-    def _apply_trunc_stats(self, i, cs, st):
+    def _apply_trunc_stats(self, ii, cs, st):
         """The fast-truncation bookkeeping of BASS._fast_truncate, per circuit."""
         n_in = st["n_in"]
         tr = (st["flags"] & 1) != 0
@@ -183,13 +251,13 @@ class BatchedBASS:
             self.trunc_active[t] = True
             self.gamma_run[t] *= np.where(total > 1e-30, kept / np.where(total > 1e-30, total, 1.0), 0.0)
             if self.traces is not None:
-                for c, kp, to in zip(t, kept, total):
-                    self.traces[c].append(("trunc", i, float(kp), float(to)))
+                for c, g, kp, to in zip(t, ii[tr], kept, total):
+                    self.traces[c].append(("trunc", int(g), float(kp), float(to)))
             self._count("truncate", tr.sum())
         self.nnz[cs] = np.where(tr, self.k[cs], n_in)
 
     # ------------------------------------------------------------- gates ----
-    def _gates_1q(self, i, cs, gates):
+    def _gates_1q(self, ii, cs, gates):
         q = np.fromiter((g.qubits[0] for g in gates), dtype=np.int64, count=len(gates))
         gm = np.stack([g.matrix for g in gates])
         U = self.U[cs, q]
@@ -202,9 +270,9 @@ class BatchedBASS:
         J["key_map"] = _key_maps(self.pos[cs])
         st = _ops.batch_step(self._ctx, J, 1, L.MODE_GATE, self.tiles)
         self._count("gate1q", len(cs))
-        self._apply_trunc_stats(i, cs, st)
+        self._apply_trunc_stats(ii, cs, st)
 
-    def _gates_2q(self, i, cs, gates):
+    def _gates_2q(self, ii, cs, gates):
         q1 = np.fromiter((g.qubits[0] for g in gates), dtype=np.int64, count=len(gates))
         q2 = np.fromiter((g.qubits[1] for g in gates), dtype=np.int64, count=len(gates))
         gm = np.stack([g.matrix for g in gates])
@@ -227,7 +295,7 @@ class BatchedBASS:
         gen = ~diag
         if not gen.any():
             return
-        cs, q1, q2, G = cs[gen], q1[gen], q2[gen], G[gen]
+        cs, q1, q2, G, ii = cs[gen], q1[gen], q2[gen], G[gen], ii[gen]
         # the second qubit goes to bit 0 (a move) unless it is there, or the
         # first one is (then the qubits swap roles)
         sw = (self.pos[cs, q1] == 0) & (self.pos[cs, q2] != 0)
@@ -251,7 +319,7 @@ class BatchedBASS:
         J["key_map"] = _key_maps(self.pos[cs])
         st = _ops.batch_step(self._ctx, J, 2, L.MODE_GATE, self.tiles)
         self._count("gate2q", len(cs))
-        self._apply_trunc_stats(i, cs, st)
+        self._apply_trunc_stats(ii, cs, st)
 
     def _ensure_sums(self, cs):
         need = cs[np.isnan(self.sum4[cs]) | np.isnan(self.sum2[cs])]
@@ -264,130 +332,216 @@ class BatchedBASS:
         pr = np.where(s4 > 1e-60, 1.0 / np.where(s4 > 1e-60, s4, 1.0), np.inf)
         return np.where(self.nnz[cs] == 0, np.inf, pr)
 
-    def run(self):
-        """Simulate every circuit; returns per-circuit results (keys, amps,
-        nnz, gamma, U)."""
-        self._setup()
-        for i in range(int(self.lens.max()) if self.B else 0):
-            act = np.flatnonzero(self.lens > i)
-            gates = [self.circuits[c][i] for c in act]
-            nq = np.fromiter((g.n_qubits for g in gates), dtype=np.int64, count=len(gates))
-            if np.any((nq != 1) & (nq != 2)):
-                raise ValueError("only 1- and 2-qubit gates are supported")
-            one = np.flatnonzero(nq == 1)
-            two = np.flatnonzero(nq == 2)
-            if len(two):
-                self._gates_2q(i, act[two], [gates[r] for r in two])
-            if len(one):
-                self._gates_1q(i, act[one], [gates[r] for r in one])
-            chk = np.array([c for c in act if i in self.chk[c]], dtype=np.int64)
-            if len(chk):
-                ta = chk[self.trunc_active[chk]]
-                if len(ta):
-                    self._ensure_sums(ta)
-                    pr = self._pr(ta)
-                    trig = np.ones(len(ta), bool) if self.pr_opt_threshold >= 1.0 else \
-                        pr > self.pr_last[ta] / self.pr_opt_threshold
-                    opt = ta[trig]
-                    if len(opt):
-                        self._optimize(opt)
-                        self._ensure_sums(opt)
-                        self.pr_last[opt] = self._pr(opt)
-                self.trunc_active[chk] = False
-        return self._results()
+    def tick(self):
+        """One operation of every running circuit: its next gate (with the
+        check-point test after it), or its next optimiser step (an RDM at a
+        pass start, one rotation trial, or the final renormalisation).
+        Returns [(tag, result)] of the circuits that finished."""
+        g = np.flatnonzero(self.alive & (self.phase == 0) & (self.gidx < self.lens))
+        if len(g):
+            self._gate_step(g)
+        if (self.alive & (self.phase == 1)).any():
+            self._opt_step()
+        done = np.flatnonzero(self.alive & (self.phase == 0) & (self.gidx >= self.lens))
+        out = []
+        for c in done:
+            out.append((self.tags[c], self._result(c)))
+            self.alive[c] = False
+            self._release(int(self.off[c]), int(self.k[c]))
+            self.circuits[c] = None
+        return out
+
+    def _gate_step(self, act):
+        ii = self.gidx[act]
+        gates = [self.circuits[c][i] for c, i in zip(act, ii)]
+        nq = np.fromiter((g.n_qubits for g in gates), dtype=np.int64, count=len(gates))
+        if np.any((nq != 1) & (nq != 2)):
+            raise ValueError("only 1- and 2-qubit gates are supported")
+        one = np.flatnonzero(nq == 1)
+        two = np.flatnonzero(nq == 2)
+        if len(two):
+            self._gates_2q(ii[two], act[two], [gates[r] for r in two])
+        if len(one):
+            self._gates_1q(ii[one], act[one], [gates[r] for r in one])
+        self.gidx[act] += 1
+        # check point after gate i: optimise if truncation happened since the
+        # last one and the participation ratio grew past the trigger
+        chk = np.array([c for c, i in zip(act, ii) if i in self.chk[c]], dtype=np.int64)
+        if not len(chk):
+            return
+        ta = chk[self.trunc_active[chk]]
+        self.trunc_active[chk] = False
+        if not len(ta):
+            return
+        self._ensure_sums(ta)
+        pr = self._pr(ta)
+        trig = np.ones(len(ta), bool) if self.pr_opt_threshold >= 1.0 else pr > self.pr_last[ta] / self.pr_opt_threshold
+        opt = ta[trig]
+        if not len(opt):
+            return
+        # BASS._optimize_basis (:730-827): skipped while PR < k/4
+        enter = ~(self._pr(opt) < self.k[opt] / 4)
+        e = opt[enter]
+        self.phase[e] = 1
+        self.ostage[e] = 0
+        self.opass[e] = 0
+        self.oentered[opt] = enter
+        skip = opt[~enter]
+        if len(skip):
+            self._opt_leave(skip)
 
     # ------------------------------------------------------------ optimiser --
-    def _optimize(self, cs):
-        """BASS._optimize_basis (:730-827) for every circuit of cs at once."""
+    def _opt_step(self):
+        """Advance every circuit in the optimiser by one step (RDM pass starts
+        first, so a circuit starting a pass also makes its first trial)."""
+        ph = self.alive & (self.phase == 1)
+        s0 = np.flatnonzero(ph & (self.ostage == 0))
+        if len(s0):
+            self._opt_rdm(s0)
+        s1 = np.flatnonzero(ph & (self.ostage == 1) & (self.optr < self.oclen))
+        if len(s1):
+            self._opt_trial(s1)
+        # pass ends: another pass after an improvement (at most 3), else finish
+        end = np.flatnonzero(ph & (self.ostage == 1) & (self.optr >= self.oclen))
+        if len(end):
+            again = self.oimproved[end] & (self.opass[end] < 2)
+            a = end[again]
+            self.opass[a] += 1
+            self.ostage[a] = 0
+            self.ostage[end[~again]] = 2
+        s2 = np.flatnonzero(ph & (self.ostage == 2))
+        if len(s2):
+            self._opt_finish(s2)
+
+    def _opt_rdm(self, cs):
+        rho_st = _ops.batch_rdm1q(self._ctx, self._jobs(cs))
+        self._count("rdm", len(cs))
+        P = np.minimum(self.pos[cs], NMAX - 1)
+        rho = np.take_along_axis(rho_st, P[:, :, None, None], axis=1)
+        V, trivial = _eigh2x2_rows(rho)
+        trivial |= np.arange(NMAX)[None, :] >= self.N[cs][:, None]
+        self.oV[cs] = V
+        # the pass tries the qubits with a non-trivial eigenbasis in index order
+        nt = ~trivial
+        self.oclen[cs] = nt.sum(axis=1)
+        order = np.argsort(trivial, axis=1, kind="stable")
+        self.ochain[cs] = order
+        self.optr[cs] = 0
+        self.opr[cs] = self._pr(cs)
+        self.oimproved[cs] = False
+        self.ostage[cs] = 1
+
+    def _opt_trial(self, trial):
+        """One rotation trial per circuit (its next qubit), accept / reject."""
         ctx = self._ctx
+        j = self.ochain[trial, self.optr[trial]]
+        self.optr[trial] += 1
+        Vj = self.oV[trial, j]
+        jb = self.pos[trial, j]
+        tpos = _lift_rows(self.pos[trial], jb, 0)
+        J = self._jobs(trial, out_other=True)
+        J["j"] = jb
+        J["g"][:, :8] = np.ascontiguousarray(_dagger(Vj)).reshape(len(trial), 4).view(np.float64)
+        J["key_map"] = _key_maps(tpos)
+        st = _ops.batch_step(ctx, J, 1, L.MODE_ROT, self.tiles)
+        self._count("rotate", len(trial))
+        m = st["n_in"]
+        tr = (st["flags"] & 1) != 0
+        self._count("amp_updates", m.sum())
+        frac = np.where(tr, st["kept"], st["sum2"])
+        s4 = st["sum4"].copy()
+        keep = np.flatnonzero(~tr)
+        if len(keep):
+            # kept whole: renormalise the candidates, then their moments
+            sub = np.ascontiguousarray(J[keep])
+            sub["x"], sub["a"] = sub["xo"], sub["ao"]
+            sub["n"] = m[keep]
+            norm = np.sqrt(st["sum2"][keep])
+            sub["scale"] = np.where(norm > 1e-30, norm, 1.0)
+            _ops.batch_scale(ctx, sub)
+            _, s4k = _ops.batch_moments(ctx, sub)
+            s4[keep] = s4k
+        pr_new = np.where(s4 > 1e-60, 1.0 / np.where(s4 > 1e-60, s4, 1.0), np.inf)
+        pr = self.opr[trial]
+        reject = pr_new >= pr * (1.0 - self.accept_rtol)
+        if self.traces is not None:
+            for r, c in enumerate(trial):
+                self.traces[c].append(("rot", int(j[r]), bool(not reject[r]), float(pr[r]),
+                                       float(pr_new[r]), float(frac[r])))
+        acc = np.flatnonzero(~reject)
+        self._count("reject", int(reject.sum()))
+        self._count("accept", len(acc))
+        if not len(acc):
+            return
+        a = trial[acc]
+        ja = j[acc]
+        self.cur[a] ^= 1
+        self.nnz[a] = np.where(tr[acc], self.k[a], m[acc])
+        self.pos[a] = tpos[acc]
+        self.sum2[a], self.sum4[a] = np.nan, s4[acc]
+        self.U[a, ja] = self.U[a, ja] @ Vj[acc]
+        self.opr[a] = pr_new[acc]
+        f = frac[acc]
+        self.gamma_run[a] *= np.sqrt(np.where(f > 1e-30, f, 0.0))
+        self.oimproved[a] = True
+
+    def _opt_finish(self, cs):
+        """Final renormalisation (:551-554; nothing to truncate when nnz <= k)."""
         self._ensure_sums(cs)
-        cs = cs[~(self._pr(cs) < self.k[cs] / 4)]
-        entered = cs.copy()
-        active = cs
-        for _pass in range(3):
-            if not len(active):
-                break
-            rho_st = _ops.batch_rdm1q(ctx, self._jobs(active))
-            self._count("rdm", len(active))
-            P = np.minimum(self.pos[active], NMAX - 1)
-            rho = np.take_along_axis(rho_st, P[:, :, None, None], axis=1)
-            V, trivial = _eigh2x2_rows(rho)
-            trivial |= np.arange(NMAX)[None, :] >= self.N[active][:, None]
-            pr = self._pr(active)
-            improved = np.zeros(len(active), dtype=bool)
-            for j in range(int(self.N[active].max())):
-                rows = np.flatnonzero(~trivial[:, j])
-                if not len(rows):
-                    continue
-                trial = active[rows]
-                Vj = V[rows, j]
-                jb = self.pos[trial, j]
-                tpos = _lift_rows(self.pos[trial], jb, 0)
-                J = self._jobs(trial, out_other=True)
-                J["j"] = jb
-                J["g"][:, :8] = np.ascontiguousarray(_dagger(Vj)).reshape(len(rows), 4).view(np.float64)
-                J["key_map"] = _key_maps(tpos)
-                st = _ops.batch_step(ctx, J, 1, L.MODE_ROT, self.tiles)
-                self._count("rotate", len(rows))
-                m = st["n_in"]
-                tr = (st["flags"] & 1) != 0
-                self._count("amp_updates", m.sum())
-                frac = np.where(tr, st["kept"], st["sum2"])
-                s4 = st["sum4"].copy()
-                keep = np.flatnonzero(~tr)
-                if len(keep):
-                    # kept whole: renormalise the candidates, then their moments
-                    sub = np.ascontiguousarray(J[keep])
-                    sub["x"], sub["a"] = sub["xo"], sub["ao"]
-                    sub["n"] = m[keep]
-                    norm = np.sqrt(st["sum2"][keep])
-                    sub["scale"] = np.where(norm > 1e-30, norm, 1.0)
-                    _ops.batch_scale(ctx, sub)
-                    _, s4k = _ops.batch_moments(ctx, sub)
-                    s4[keep] = s4k
-                pr_new = np.where(s4 > 1e-60, 1.0 / np.where(s4 > 1e-60, s4, 1.0), np.inf)
-                reject = pr_new >= pr[rows] * (1.0 - self.accept_rtol)
-                if self.traces is not None:
-                    for r, c in enumerate(trial):
-                        self.traces[c].append(("rot", j, bool(not reject[r]), float(pr[rows][r]),
-                                               float(pr_new[r]), float(frac[r])))
-                acc = np.flatnonzero(~reject)
-                self._count("reject", int(reject.sum()))
-                self._count("accept", len(acc))
-                if not len(acc):
-                    continue
-                a = trial[acc]
-                self.cur[a] ^= 1
-                self.nnz[a] = np.where(tr[acc], self.k[a], m[acc])
-                self.pos[a] = tpos[acc]
-                self.sum2[a], self.sum4[a] = np.nan, s4[acc]
-                self.U[a, j] = self.U[a, j] @ Vj[acc]
-                pr[rows[acc]] = pr_new[acc]
-                f = frac[acc]
-                self.gamma_run[a] *= np.sqrt(np.where(f > 1e-30, f, 0.0))
-                improved[rows[acc]] = True
-            active = active[improved]
-        # final renormalisation (:551-554; nothing to truncate when nnz <= k)
-        if len(entered):
-            self._ensure_sums(entered)
-            norm = np.sqrt(self.sum2[entered])
-            do = norm > 1e-30
-            if do.any():
-                J = self._jobs(entered[do])
-                J["scale"] = norm[do]
-                _ops.batch_scale(ctx, J)
-            self.sum4[entered] = np.nan
+        norm = np.sqrt(self.sum2[cs])
+        do = norm > 1e-30
+        if do.any():
+            J = self._jobs(cs[do])
+            J["scale"] = norm[do]
+            _ops.batch_scale(self._ctx, J)
+        self.sum4[cs] = np.nan
+        self._opt_leave(cs)
+
+    def _opt_leave(self, cs):
+        self._ensure_sums(cs)
+        self.pr_last[cs] = self._pr(cs)
+        self.phase[cs] = 0
 
     # -------------------------------------------------------------- results --
-    def _results(self):
-        out = []
-        for c in range(self.B):
-            n = int(self.nnz[c])
-            buf = self.slabs[int(self.cur[c])]
-            o = int(self.off[c])
-            w = to_host_keys(buf.x[o: o + n], n)
-            pos = self.pos[c, : self.N[c]]
-            out.append({"keys": stored_to_keys(w, pos), "amps": to_host_amps(buf.a[o: o + n], n), "nnz": n,
-                        "gamma": float(np.sqrt(max(self.gamma_run[c], 0))),
-                        "U": [self.U[c, j].copy() for j in range(self.N[c])]})
+    def _result(self, c):
+        n = int(self.nnz[c])
+        buf = self.slabs[int(self.cur[c])]
+        o = int(self.off[c])
+        w = to_host_keys(buf.x[o: o + n], n)
+        pos = self.pos[c, : self.N[c]]
+        res = {"keys": stored_to_keys(w, pos), "amps": to_host_amps(buf.a[o: o + n], n), "nnz": n,
+               "gamma": float(np.sqrt(max(self.gamma_run[c], 0))),
+               "U": [self.U[c, j].copy() for j in range(self.N[c])]}
+        if self.traces is not None:
+            res["trace"] = self.traces[c]
+        return res
+
+
+class BatchedBASS:
+    """A fixed batch: every circuit starts at once (specs: list of
+    (num_qubits, k, circuit, optimize_at or None)); run() returns the results
+    in spec order.  Keywords as BatchEngine."""
+
+    def __init__(self, specs, device=None, trace=False, **kw):
+        if kw.pop("truncate_every", 1) != 1:
+            raise NotImplementedError("the batched driver implements truncate_every = 1")
+        self.specs = list(specs)
+        self.engine = BatchEngine(max(1, len(self.specs)), max(1, sum(int(s[1]) for s in self.specs)),
+                                  device=device, trace=trace, **kw)
+        self.traces = None
+
+    @property
+    def counters(self):
+        return self.engine.counters
+
+    def run(self):
+        for r, (n, k, circ, oat) in enumerate(self.specs):
+            if not self.engine.admit(n, k, circ, oat, tag=r):
+                raise RuntimeError("batch does not fit")
+        out = [None] * len(self.specs)
+        while self.engine.active:
+            for tag, res in self.engine.tick():
+                out[tag] = res
+        if self.engine.trace:
+            self.traces = [r.pop("trace") for r in out]
         return out

@@ -97,8 +97,7 @@ def measured_traffic(workload, n, k, op):
 
 def bench_batched(args):
     """Config 3: independent TFIM circuits (n ~ U[16,48], steps ~ U[10,60],
-    h ~ U[0.2,2], J = 1, dt = 0.05, K = 2^U{10..18}) run as one lock-step
-    batch; baseline: the same circuits one at a time through the plain driver
+    h ~ U[0.2,2], J = 1, dt = 0.05, K = 2^U{10..18}) run as one batch; baseline: the same circuits one at a time through the plain driver
     (a --single-sample subset, timed, as circuits/s)."""
     import numpy as np
     import torch
@@ -128,9 +127,9 @@ def bench_batched(args):
     single_rate = len(sample) / t_single
     log(f"one at a time: {len(sample)} circuits in {t_single:.2f} s = {single_rate:.2f} circuits/s")
     # the batch
-    bat = Bm.BatchedBASS(specs, device="cuda:0")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    bat = Bm.BatchedBASS(specs, device="cuda:0")
     res = bat.run()
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
@@ -153,22 +152,48 @@ def bench_batched(args):
 
 
 def bench_served(args):
-    """Config 4: Poisson stream (--rate) of 50/50 TFIM and brickwork circuits
-    served by a worker pool with HBM-budget admission; p50/p99 latency."""
+    """Config 4: Poisson stream (--rate) of 50/50 TFIM and brickwork circuits,
+    HBM-budget admission (200 GB), p50/p99 latency.  --server engine: one
+    continuous batch (BatchServer); pool: worker processes running one circuit
+    each (BassServer).  Baseline: a --single-sample subset of the same requests
+    one at a time through the plain driver, as circuits/s."""
+    import torch
+
     S = importlib.import_module(PKG + ".serving.server")
+    importlib.import_module(PKG + "._lib").load()
     reqs = S.make_served_requests(args.circuits, args.rate)
-    with S.BassServer(workers=args.workers, hbm_budget=200e9) as srv:
-        recs = srv.run(reqs, progress=lambda d, t: log(f"served {d}/{t}") if d % 16 == 0 else None)
-    summ = S.summarize(recs)
+    single_rate = None
+    if args.single_sample:
+        sample = reqs[: args.single_sample]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in sample:
+            S.run_request(r, "cuda:0")
+        t_single = time.perf_counter() - t0
+        single_rate = len(sample) / t_single
+        log(f"one at a time: {len(sample)} circuits in {t_single:.2f} s = {single_rate:.2f} circuits/s")
+    prog = lambda d, t: log(f"served {d}/{t}") if d % 32 == 0 else None  # noqa: E731
+    if args.server == "engine":
+        with S.BatchServer(workers=args.workers, max_circuits=args.max_circuits, hbm_budget=200e9) as srv:
+            recs = srv.run(reqs, progress=prog)
+        summ = S.summarize(recs, amp_updates=srv.counters.get("amp_updates", 0))
+        cfg = {"server": "continuous batch", "engines": args.workers, "max_circuits_per_engine": args.max_circuits}
+    else:
+        with S.BassServer(workers=args.workers, hbm_budget=200e9) as srv:
+            recs = srv.run(reqs, progress=prog)
+        summ = S.summarize(recs)
+        cfg = {"server": "worker pool", "workers": args.workers}
     out = {
         "metric": "served circuits/s", "value": summ["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1,
         "steps": 1, "warmup": 0, "ms_per_step": summ["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
         "config": {"workload": f"served Poisson {args.rate:g} req/s, {args.circuits} requests, 50/50 TFIM "
                                f"(K=2^U{{12..20}}) / brickwork n~U[24,44] depth~U[10,30]",
-                   "workers": args.workers, "hbm_budget_bytes": 200e9},
+                   "hbm_budget_bytes": 200e9, **cfg},
         "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"],
         "mean_service_s": summ["mean_service_s"], "amp_updates": summ["amp_updates"],
+        "one_at_a_time_circuits_per_s": single_rate,
+        "speedup_vs_one_at_a_time": (summ["circuits_per_s"] / single_rate) if single_rate else None,
     }
     print(json.dumps(out), flush=True)
 
@@ -183,7 +208,10 @@ def main():
     ap.add_argument("--single-sample", type=int, default=24,
                     help="batched / served: circuits timed one at a time for the baseline")
     ap.add_argument("--rate", type=float, default=200.0, help="served: Poisson arrival rate (req/s)")
-    ap.add_argument("--workers", type=int, default=8, help="served: worker processes")
+    ap.add_argument("--server", choices=["engine", "pool"], default="engine", help="served: server kind")
+    ap.add_argument("--max-circuits", type=int, default=512, help="served (engine): circuits running at once")
+    ap.add_argument("--workers", type=int, default=None,
+                    help="served: worker processes (engine: one batch engine each, default 4; pool: default 8)")
     ap.add_argument("--qubits", type=int, default=None)
     ap.add_argument("--log2k", type=int, default=None)
     ap.add_argument("--calibrate-copy", type=float, default=None, metavar="MIB")
@@ -212,6 +240,8 @@ def main():
     if args.workload == "batched":
         return bench_batched(args)
     if args.workload == "served":
+        if args.workers is None:
+            args.workers = 4 if args.server == "engine" else 8
         return bench_served(args)
 
     lib = importlib.import_module(PKG + "._lib")

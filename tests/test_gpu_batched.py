@@ -47,3 +47,81 @@ def test_batched_matches_single(gpu_ctx):
         assert r["gamma"] == st.gamma
         for j in range(n):
             assert np.array_equal(r["U"][j], sim.U[j])
+
+
+def _single(spec):
+    S = pkg_module("simulation.bass_simulator")
+    n, k, circ, ends = spec
+    kw = {"optimize_at": ends} if ends is not None else {}
+    sim = S.BASS(n, k, device="cuda:0", **kw)
+    st = sim.simulate(circ)
+    return st.keys(), st.alpha[: st.nnz].copy(), st.nnz, st.gamma
+
+
+def test_engine_staggered_admission(gpu_ctx):
+    """Circuits joining a running batch between ticks, into slots and slab
+    regions released by finished ones (two slots, a slab that fits the two
+    largest states only), give the single-circuit results bit for bit."""
+    print("[case] engine staggered admission", flush=True)
+    Bm = pkg_module("simulation.batched")
+    specs = _specs()
+    want = [_single(s) for s in specs]
+    ks = sorted(s[1] for s in specs)
+    eng = Bm.BatchEngine(2, ks[-1] + ks[-2], device="cuda:0")
+    queue = list(range(len(specs)))
+    got = {}
+    ticks = 0
+    while queue or eng.active:
+        # at most one admission per tick, so circuits start at different ticks
+        if queue:
+            c = queue[0]
+            n, k, circ, ends = specs[c]
+            if eng.admit(n, k, circ, ends, tag=c):
+                queue.pop(0)
+        for tag, res in eng.tick():
+            got[tag] = res
+        ticks += 1
+        assert ticks < 10000
+    assert not eng.alive.any() and eng.free == [(0, ks[-1] + ks[-2])]
+    for c, (keys, amps, nnz, gamma) in enumerate(want):
+        r = got[c]
+        print(f"  circuit {c}: nnz {r['nnz']} / {nnz}", flush=True)
+        assert r["nnz"] == nnz and r["gamma"] == gamma
+        o1, o2 = np.argsort(r["keys"]), np.argsort(keys)
+        assert np.array_equal(r["keys"][o1], keys[o2])
+        assert np.array_equal(r["amps"][o1].view(np.float64), amps[o2].view(np.float64))
+
+
+def test_batch_server_stream(gpu_ctx):
+    """BatchServer over a short Poisson stream of small requests: every request
+    completes, latency >= service time, and results match the plain driver."""
+    print("[case] batch server stream", flush=True)
+    Srv = pkg_module("serving.server")
+    reqs = [Srv.CircuitRequest(rid=i, kind=("tfim" if i % 2 else "brickwork"), n=10 + i % 5, depth=3 + i % 4,
+                               log2k=6 + i % 3, h=0.7, seed=50 + i, arrival=0.002 * i) for i in range(10)]
+    with Srv.BatchServer(max_circuits=4, slab_entries=1 << 12, device="cuda:0") as srv:
+        recs = srv.run(reqs)
+    assert [r["rid"] for r in recs] == list(range(10))
+    for q, r in zip(reqs, recs):
+        assert r["latency"] >= r["service"] - 1e-9 and r["end"] >= r["start"]
+        res = Srv.run_request(q, "cuda:0")
+        assert res["nnz"] == r["nnz"] and res["gamma2"] == r["gamma2"]
+    summ = Srv.summarize(recs, amp_updates=srv.engine.counters["amp_updates"])
+    assert summ["circuits"] == 10 and summ["p99_latency_s"] >= summ["p50_latency_s"]
+
+
+def test_batch_server_two_engines(gpu_ctx):
+    """Two engine worker processes: every request is served once, by one of
+    them, with the plain driver's result."""
+    print("[case] batch server, two engine processes", flush=True)
+    Srv = pkg_module("serving.server")
+    reqs = [Srv.CircuitRequest(rid=i, kind=("brickwork" if i % 3 else "tfim"), n=9 + i % 4, depth=2 + i % 3,
+                               log2k=5 + i % 3, h=1.1, seed=80 + i, arrival=0.001 * i) for i in range(8)]
+    with Srv.BatchServer(workers=2, max_circuits=3, slab_entries=1 << 10, device="cuda:0") as srv:
+        recs = srv.run(reqs)
+    assert sorted(r["rid"] for r in recs) == list(range(8))
+    assert {r["worker"] for r in recs} <= {0, 1}
+    for q, r in zip(reqs, recs):
+        res = Srv.run_request(q, "cuda:0")
+        assert res["nnz"] == r["nnz"] and res["gamma2"] == r["gamma2"]
+    assert srv.counters["amp_updates"] > 0

@@ -78,3 +78,53 @@ def test_gate_classes_match_reference_conventions():
     np.testing.assert_array_equal(G.RZGate(0, 0.3).matrix, OC.RZ(0, 0.3).matrix)
     np.testing.assert_array_equal(G.CZGate(0, 1).matrix, OC.CZ(0, 1).matrix)
     assert G.CNOTGate(3, 1).qubits == (3, 1)
+
+
+def test_stored_to_keys_inverts_the_bit_placement():
+    """Stored word: basis bit q placed at bit pos[q]; stored_to_keys must give
+    the basis key back (placement done here one key at a time)."""
+    ss = pkg_module("core.sparse_state")
+    rng = np.random.default_rng(3)
+    for N in (1, 7, 8, 9, 33, 63):
+        pos = rng.permutation(N)
+        keys = rng.integers(0, 1 << min(N, 62), size=200, dtype=np.uint64)
+        if N == 63:
+            keys |= np.uint64(1) << np.uint64(62)
+        w = np.array([sum(((int(k) >> q) & 1) << int(pos[q]) for q in range(N)) for k in keys], dtype=np.uint64)
+        assert np.array_equal(ss.stored_to_keys(w, pos), keys)
+    assert np.array_equal(ss.stored_to_keys(keys, None), keys)
+
+
+def test_batched_host_helpers():
+    """Row-wise position lifts and key maps of the batched driver agree with
+    the single-circuit helpers; the slab allocator is first fit and merges
+    neighbours on release."""
+    Bm = pkg_module("simulation.batched")
+    ss = pkg_module("core.sparse_state")
+    rng = np.random.default_rng(4)
+    P = np.stack([np.concatenate([rng.permutation(20), Bm._BAD + np.arange(20, Bm.NMAX)]) for _ in range(6)])
+    jb = rng.integers(0, 20, size=6)
+    for t in (0, 1):
+        jj = np.maximum(jb, t)
+        L = Bm._lift_rows(P, jj, t)
+        for r in range(6):
+            assert np.array_equal(L[r, :20], ss.lifted_positions(P[r, :20], int(jj[r]), t))
+            assert np.array_equal(L[r, 20:], P[r, 20:])
+        km = Bm._key_maps(L)
+        for r in range(6):
+            assert all(km[r, L[r, q]] == q for q in range(20))
+    A = rng.normal(size=(5, 2, 2)) + 1j * rng.normal(size=(5, 2, 2))
+    B = rng.normal(size=(5, 2, 2)) + 1j * rng.normal(size=(5, 2, 2))
+    K = Bm._kron2(A, B)
+    for r in range(5):
+        assert np.allclose(K[r], np.kron(A[r], B[r]), rtol=0, atol=1e-14)
+    eng = Bm.BatchEngine.__new__(Bm.BatchEngine)
+    eng.free = [(0, 100)]
+    a, b, c = eng._alloc(30), eng._alloc(50), eng._alloc(20)
+    assert (a, b, c) == (0, 30, 80) and eng._alloc(1) is None
+    eng._release(30, 50)
+    assert eng._alloc(60) is None and eng._alloc(40) == 30
+    eng._release(0, 30)
+    eng._release(30, 40)
+    eng._release(80, 20)
+    assert eng.free == [(0, 100)]
