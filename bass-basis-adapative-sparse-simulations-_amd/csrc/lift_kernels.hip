@@ -144,15 +144,21 @@ __device__ __forceinline__ int lift_stage(const uint64_t* __restrict__ w, const 
     p2n = B1.a - B0.b;
     p3lo = B1.sm;
   }
+  // private arrays as plain doubles: arrays of the vector type double2 stay
+  // in scratch memory
   uint64_t kr[ITEMS];
-  double2 ar[ITEMS];
+  double ar_re[ITEMS], ar_im[ITEMS];
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
     const int idx = it * NT + threadIdx.x;
+    kr[it] = 0;
+    ar_re[it] = ar_im[it] = 0.0;
     if (idx < cnt) {
       const int64_t g = idx < p1n ? p1lo + idx : (idx < p1n + p2n ? p2lo + (idx - p1n) : p3lo + (idx - p1n - p2n));
       kr[it] = w[g];
-      ar[it] = a[g];
+      const double2 v = a[g];
+      ar_re[it] = v.x;
+      ar_im[it] = v.y;
       s_w[idx] = kr[it];
     }
   }
@@ -175,7 +181,7 @@ __device__ __forceinline__ int lift_stage(const uint64_t* __restrict__ w, const 
         rank = lds_lower_bound(s_w, cnt, ((x & ~bj) | lowmask) + 1ull) + (idx - sp);
       }
       s_t[rank] = lift_key(x, j, t);
-      s_a[rank] = ar[it];
+      s_a[rank] = make_double2(ar_re[it], ar_im[it]);
     }
   }
   __syncthreads();
@@ -198,7 +204,10 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
   __shared__ uint64_t s_w[TM];
   __shared__ uint64_t s_t[TM];
   __shared__ double2 s_a[TM];
-  __shared__ unsigned int s_hist[HIST_BINS];
+  // digit-1 bins below 1024 (|a|^2 < 2, every normalised state) count in LDS;
+  // higher bins go straight to the global histogram
+  constexpr int LBINS = HIST_BINS / 2;
+  __shared__ unsigned int s_hist[LBINS];
   __shared__ int s_scan[WAVES];
   __shared__ double s_red[WAVES];
   __shared__ uint64_t s_halo_t[3];
@@ -206,7 +215,7 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
   __shared__ int s_nhalo;
   __shared__ uint64_t s_prev[2];
 
-  for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  for (int b = threadIdx.x; b < LBINS; b += NT) s_hist[b] = 0;
   const int cnt = lift_stage<TM>(w, a, n, j, T, B0, B1, tile, s_w, s_t, s_a);
 
   // group key: the lifted key without the gate bits 0..T
@@ -234,7 +243,7 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
   __syncthreads();
 
   uint64_t okey[ITEMS][M];
-  double2 oamp[ITEMS][M];
+  double ore[ITEMS][M], oim[ITEMS][M];
   unsigned int omask[ITEMS];
   int ooff[ITEMS];
   int rtot[ITEMS];
@@ -247,11 +256,13 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
       const uint64_t g = gk(tk);
       const bool head = m == 0 ? (g != s_prev[0] && g != s_prev[1]) : g != gk(s_t[m - 1]);
       if (head) {
-        double2 av[M];
+        // member amplitudes by member index c; written through unrolled
+        // selects so the arrays stay in registers
+        double avr[M], avi[M];
         bool pres[M];
 #pragma unroll
         for (int c = 0; c < M; ++c) {
-          av[c] = make_double2(0.0, 0.0);
+          avr[c] = avi[c] = 0.0;
           pres[c] = false;
         }
         int k = m;
@@ -259,16 +270,33 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
         for (int s = 0; s < M; ++s) {
           if (k < cnt && gk(s_t[k]) == g) {
             const int c = (int)(s_t[k] & (M - 1));
-            av[c] = s_a[k];
-            pres[c] = true;
+            const double2 v = s_a[k];
+#pragma unroll
+            for (int cc = 0; cc < M; ++cc) {
+              if (c == cc) {
+                avr[cc] = v.x;
+                avi[cc] = v.y;
+                pres[cc] = true;
+              }
+            }
             ++k;
           }
         }
         if (k == cnt) {
-          for (int h = 0; h < s_nhalo; ++h) {
-            const int c = (int)(s_halo_t[h] & (M - 1));
-            av[c] = s_halo_a[h];
-            pres[c] = true;
+#pragma unroll
+          for (int h = 0; h < 3; ++h) {
+            if (h < s_nhalo) {
+              const int c = (int)(s_halo_t[h] & (M - 1));
+              const double2 v = s_halo_a[h];
+#pragma unroll
+              for (int cc = 0; cc < M; ++cc) {
+                if (c == cc) {
+                  avr[cc] = v.x;
+                  avi[cc] = v.y;
+                  pres[cc] = true;
+                }
+              }
+            }
           }
         }
         const uint64_t base = tk & ~(uint64_t)(M - 1);
@@ -279,23 +307,22 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
           if constexpr (MODE == MODE_GATE) {
 #pragma unroll
             for (int c = 0; c < M; ++c) {
-              const double2 term = cmul(G.m[o * M + c], av[c]);
+              const double2 term = cmul(G.m[o * M + c], make_double2(avr[c], avi[c]));
               if (pres[c] && cnonzero(term)) {
                 acc = any ? cadd(acc, term) : term;
                 any = true;
               }
             }
           } else {
-            acc = cmul(G.m[o * M + 0], av[0]);
+            acc = cmul(G.m[o * M + 0], make_double2(avr[0], avi[0]));
 #pragma unroll
-            for (int c = 1; c < M; ++c) acc = cadd(acc, cmul(G.m[o * M + c], av[c]));
+            for (int c = 1; c < M; ++c) acc = cadd(acc, cmul(G.m[o * M + c], make_double2(avr[c], avi[c])));
             any = hypot(acc.x, acc.y) > 1e-30;
           }
-          if (any) {
-            mask |= 1u << o;
-            okey[it][o] = base | (uint64_t)o;
-            oamp[it][o] = acc;
-          }
+          okey[it][o] = base | (uint64_t)o;
+          ore[it][o] = acc.x;
+          oim[it][o] = acc.y;
+          if (any) mask |= 1u << o;
         }
       }
     }
@@ -320,12 +347,16 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
 #pragma unroll
     for (int o = 0; o < M; ++o) {
       if ((omask[it] >> o) & 1u) {
-        const double2 v = oamp[it][o];
+        const double2 v = make_double2(ore[it][o], oim[it][o]);
         xo[pos] = okey[it][o];
         ao[pos] = v;
         const double p = cabs2(v);
         psum += p;
-        atomicAdd(&s_hist[(unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1)], 1u);
+        const unsigned int bin = (unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1);
+        if (bin < LBINS)
+          atomicAdd(&s_hist[bin], 1u);
+        else
+          atomicAdd(&hist[bin], 1u);
         ++pos;
       }
     }
@@ -334,7 +365,7 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
   const double s = block_sum(psum, s_red);
   if (threadIdx.x == 0) tile_psum[otile] = s;
   __syncthreads();
-  for (int b = threadIdx.x; b < HIST_BINS; b += NT)
+  for (int b = threadIdx.x; b < LBINS; b += NT)
     if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
 }
 

@@ -95,6 +95,14 @@ def measured_traffic(workload, n, k, op):
     return None
 
 
+def _spread(items, m):
+    """m items evenly spaced over the list (the one-at-a-time sample)."""
+    if not m or not items:
+        return []
+    step = max(1, len(items) // m)
+    return items[::step][:m]
+
+
 def bench_batched(args):
     """Config 3: independent TFIM circuits (n ~ U[16,48], steps ~ U[10,60],
     h ~ U[0.2,2], J = 1, dt = 0.05, K = 2^U{10..18}) run as one batch; baseline: the same circuits one at a time through the plain driver
@@ -113,19 +121,19 @@ def bench_batched(args):
         specs.append((r.n, 1 << r.log2k, circ, kw.get("optimize_at")))
     log(f"batched: {len(specs)} circuits, {sum(len(s[2]) for s in specs)} gates, "
         f"sum K = {sum(s[1] for s in specs)}")
-    # one at a time (plain driver), on a sample of the same stream
-    sample = specs[: args.single_sample]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    single_amp = 0
-    for n, k, circ, oat in sample:
-        sim = Bs.BASS(n, k, device="cuda:0", **({"optimize_at": oat} if oat else {}))
-        sim.simulate(circ)
-        single_amp += sim.counters.get("amp_updates", 0)
-    torch.cuda.synchronize()
-    t_single = time.perf_counter() - t0
-    single_rate = len(sample) / t_single
-    log(f"one at a time: {len(sample)} circuits in {t_single:.2f} s = {single_rate:.2f} circuits/s")
+    # one at a time (plain driver), on an evenly spaced sample of the same stream
+    sample = _spread(specs, args.single_sample)
+    single_rate, t_single = None, 0.0
+    if sample:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for n, k, circ, oat in sample:
+            sim = Bs.BASS(n, k, device="cuda:0", **({"optimize_at": oat} if oat else {}))
+            sim.simulate(circ)
+        torch.cuda.synchronize()
+        t_single = time.perf_counter() - t0
+        single_rate = len(sample) / t_single
+        log(f"one at a time: {len(sample)} circuits in {t_single:.2f} s = {single_rate:.2f} circuits/s")
     # the batch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -144,9 +152,69 @@ def bench_batched(args):
                    "circuits": len(specs), "gates": int(sum(len(s[2]) for s in specs)),
                    "amp_updates": int(bat.counters.get("amp_updates", 0))},
         "one_at_a_time": {"value": single_rate, "unit": "circuits/s", "sample": len(sample),
-                          "seconds": t_single, "driver": "plain BASS, one circuit after another"},
-        "speedup_vs_one_at_a_time": rate / single_rate,
+                          "seconds": t_single, "driver": "plain BASS, one circuit after another, "
+                                                          "every (circuits/sample)-th circuit of the stream"},
+        "speedup_vs_one_at_a_time": (rate / single_rate) if single_rate else None,
         "amp_updates_per_s": bat.counters.get("amp_updates", 0) / t_batch,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_tfim20(args):
+    """Config 0: uniform TFIM n=20 (open chain, J=h=1, dt=0.05, 40 steps =
+    1,560 gates), K=2^14, optimize_every=39.  Times simulate() on the GPU
+    (median of --steps runs after --warmup) and the numpy oracle's simulate()
+    on one host core (one run); cache-resident sizes, so no roofline point."""
+    import numpy as np
+    import torch
+
+    rc = importlib.import_module(PKG + ".utils.random_circuits")
+    B = importlib.import_module(PKG + ".simulation.bass_simulator")
+    importlib.import_module(PKG + "._lib").load()
+    circ = rc.generate_tfim_circuit(20, 40, J=1.0, h=1.0, dt=0.05, rng=np.random.default_rng(0))
+    K = 1 << 14
+
+    def run():
+        sim = B.BASS(20, K, device="cuda:0", optimize_every=39)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = sim.simulate(circ)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, sim, st
+
+    for _ in range(args.warmup):
+        run()
+    times = []
+    for _ in range(max(1, args.steps)):
+        dt, sim, st = run()
+        times.append(dt)
+        log(f"gpu simulate {dt * 1e3:.1f} ms")
+    t_gpu = float(np.median(times))
+    amp = sim.counters.get("amp_updates", 0)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import drivers as OD
+
+        ref = OD.BASS(20, K, selector=OD.key_tiebreak_selector, accept_rtol=sim.accept_rtol, optimize_every=39)
+        t0 = time.perf_counter()
+        rs = ref.simulate(circ)
+        t_cpu = time.perf_counter() - t0
+        same = bool(rs.nnz == st.nnz and abs(rs.gamma ** 2 - st.gamma ** 2) <= 1e-12)
+        cpu = {"value": 1.0 / t_cpu, "unit": "circuits/s", "cores": 1, "kind": "port",
+               "sample": f"the whole circuit, numpy oracle simulate() (key tie-break), {t_cpu:.2f} s; "
+                         f"same nnz and gamma^2 as the GPU run: {same}; "
+                         f"host affinity {len(os.sched_getaffinity(0))} cpus"}
+        log(f"cpu oracle simulate {t_cpu:.2f} s")
+    out = {
+        "metric": "circuit simulations/s", "value": 1.0 / t_gpu, "unit": "circuits/s", "n_gpus": 1,
+        "steps": len(times), "warmup": args.warmup, "ms_per_step": t_gpu * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
+        "config": {"workload": "TFIM n=20 open J=h=1 dt=0.05 40 steps (1,560 gates) K=2^14 optimize_every=39 "
+                               "(step = one simulate())", "amp_updates": amp,
+                   "gamma2": st.gamma ** 2, "nnz": st.nnz},
+        "amp_updates_per_s": amp / t_gpu,
+        "roofline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
@@ -164,7 +232,7 @@ def bench_served(args):
     reqs = S.make_served_requests(args.circuits, args.rate)
     single_rate = None
     if args.single_sample:
-        sample = reqs[: args.single_sample]
+        sample = _spread(reqs, args.single_sample)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for r in sample:
@@ -203,7 +271,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=25, help="timed layers")
     ap.add_argument("--warmup", type=int, default=5, help="untimed layers")
-    ap.add_argument("--workload", choices=["brickwork", "kicked-ising", "batched", "served"], default="brickwork")
+    ap.add_argument("--workload", choices=["brickwork", "kicked-ising", "batched", "served", "tfim20"], default="brickwork")
     ap.add_argument("--circuits", type=int, default=512, help="batched / served: circuits in the stream")
     ap.add_argument("--single-sample", type=int, default=24,
                     help="batched / served: circuits timed one at a time for the baseline")
@@ -239,6 +307,8 @@ def main():
 
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "tfim20":
+        return bench_tfim20(args)
     if args.workload == "served":
         if args.workers is None:
             args.workers = 4 if args.server == "engine" else 8
