@@ -81,6 +81,37 @@ __device__ __forceinline__ int block_excl_scan(int v, int& total, int* lds) {
   return pre + inc - v;
 }
 
+// Exclusive scan of R values per thread in item-major order (all threads'
+// item 0, then item 1, ...: the order of R consecutive block_excl_scan calls)
+// with one barrier pair.  lds must hold R * NTHR/64 ints.
+template <int R, int NTHR = NT>
+__device__ __forceinline__ void block_excl_scan_items(const int (&v)[R], int (&off)[R], int& total, int* lds) {
+  constexpr int NW = NTHR / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    inc[r] = wave_incl_scan(v[r]);
+    if (lane == 63) lds[r * NW + w] = inc[r];
+  }
+  __syncthreads();
+  int base = 0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+      const int s = lds[r * NW + i];
+      pre += (i < w) ? s : 0;
+      tot += s;
+    }
+    off[r] = base + pre + inc[r] - v[r];
+    base += tot;
+  }
+  __syncthreads();
+  total = base;
+}
+
 // Deterministic block sum of doubles (fixed shuffle tree, then waves in order).
 // lds must hold NTHR/64 doubles.  Result valid in every thread.
 template <int NTHR = NT>

@@ -20,9 +20,9 @@
 // Kernels (TM merged positions per tile; runs coincide in input and output):
 //   k_lift_part : one wave per tile boundary d finds the run around d and the
 //                 merge split at d with 64-ary searches.
-//   k_lift_gate : per tile, loads its <= 3 input ranges (jointly sorted by w)
-//                 into LDS, ranks every entry in T order with two LDS binary
-//                 searches, scatters into merged order, applies the gate per
+//   k_lift_gate : per tile, loads its <= 3 input ranges (jointly sorted by w),
+//                 merges the entries with bit j clear and set (merge path in
+//                 LDS) into T order, applies the gate per
 //                 group and writes the tile's outputs to its own segment
 //                 [tile * SEG_ENTRIES, +count) with the digit-1 histogram and
 //                 the |a|^2 sum that the top-k select consumes.
@@ -121,12 +121,22 @@ __device__ __forceinline__ int lds_lower_bound(const uint64_t* s, int cnt, uint6
 }
 
 // Load the tile's entries (merged positions [tile*TM, +cnt)) and place them in
-// merged order: s_t = lifted keys, s_a = amplitudes.  s_w is scratch.
-// Returns cnt.  Ends with a barrier.
+// merged order: s_t = lifted keys, s_a = amplitudes.  Returns cnt.  Ends with
+// a barrier.
+//
+// The tile's entries are loaded in w order.  Those with bit j clear (A) and
+// those with it set (B) are each already ordered by the lifted key T (T keeps
+// the order of entries that agree in bit j), so the merged order is the merge
+// of the two lists: a stable partition (one scan) writes A then B into s_t,
+// each thread finds the split of its ITEMS merged positions on the merge path
+// (one binary search) and merges them, recording every entry's merged rank in
+// s_rank; the loading threads then scatter keys and amplitudes to their ranks.
+// s_scan holds ITEMS * WAVES ints.
 template <int TM>
 __device__ __forceinline__ int lift_stage(const uint64_t* __restrict__ w, const double2* __restrict__ a,
                                           int64_t n, int j, int t, const LiftBound& B0, const LiftBound& B1,
-                                          int64_t tile, uint64_t* s_w, uint64_t* s_t, double2* s_a) {
+                                          int64_t tile, unsigned short* s_rank, int* s_scan, uint64_t* s_t,
+                                          double2* s_a) {
   constexpr int ITEMS = TM / NT;
   const int64_t d = tile * TM;
   const int cnt = (int)((n - d) < TM ? (n - d) : TM);
@@ -146,42 +156,70 @@ __device__ __forceinline__ int lift_stage(const uint64_t* __restrict__ w, const 
   }
   // private arrays as plain doubles: arrays of the vector type double2 stay
   // in scratch memory
-  uint64_t kr[ITEMS];
+  uint64_t tk[ITEMS];
   double ar_re[ITEMS], ar_im[ITEMS];
+  int fb[ITEMS], pb[ITEMS], e[ITEMS];
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
     const int idx = it * NT + threadIdx.x;
-    kr[it] = 0;
+    tk[it] = 0;
     ar_re[it] = ar_im[it] = 0.0;
+    fb[it] = 0;
     if (idx < cnt) {
       const int64_t g = idx < p1n ? p1lo + idx : (idx < p1n + p2n ? p2lo + (idx - p1n) : p3lo + (idx - p1n - p2n));
-      kr[it] = w[g];
+      const uint64_t x = w[g];
       const double2 v = a[g];
       ar_re[it] = v.x;
       ar_im[it] = v.y;
-      s_w[idx] = kr[it];
+      fb[it] = (int)((x >> j) & 1ull);
+      tk[it] = lift_key(x, j, t);
+    }
+  }
+  int nb;
+  block_excl_scan_items<ITEMS>(fb, pb, nb, s_scan);
+  const int na = cnt - nb;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int idx = it * NT + threadIdx.x;
+    e[it] = fb[it] ? na + pb[it] : idx - pb[it];
+    if (idx < cnt) s_t[e[it]] = tk[it];
+  }
+  __syncthreads();
+  {
+    const uint64_t* A = s_t;
+    const uint64_t* B = s_t + na;
+    const int d0 = threadIdx.x * ITEMS;
+    if (d0 < cnt) {
+      int lo = d0 - nb > 0 ? d0 - nb : 0;
+      int hi = d0 < na ? d0 : na;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (A[mid] < B[d0 - 1 - mid])
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      int ia = lo, ib = d0 - lo;
+#pragma unroll
+      for (int k = 0; k < ITEMS; ++k) {
+        const int m = d0 + k;
+        if (m < cnt) {
+          if (ib >= nb || (ia < na && A[ia] < B[ib]))
+            s_rank[ia++] = (unsigned short)m;
+          else
+            s_rank[na + ib++] = (unsigned short)m;
+        }
+      }
     }
   }
   __syncthreads();
-  const uint64_t bj = 1ull << j;
-  const uint64_t lowmask = (1ull << t) - 1ull;
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
     const int idx = it * NT + threadIdx.x;
     if (idx < cnt) {
-      const uint64_t x = kr[it];
-      const uint64_t hi = x & ~((bj << 1) - 1ull);
-      const int sp = lds_lower_bound(s_w, cnt, hi | bj);  // first B entry of the run
-      int rank;
-      if (!(x & bj)) {
-        // A entry: entries before it in w order + B entries of its run below it in T order
-        rank = idx + lds_lower_bound(s_w, cnt, (x | bj) & ~lowmask) - sp;
-      } else {
-        // B entry: every entry up to the A entries below it in T order + B entries before it
-        rank = lds_lower_bound(s_w, cnt, ((x & ~bj) | lowmask) + 1ull) + (idx - sp);
-      }
-      s_t[rank] = lift_key(x, j, t);
-      s_a[rank] = make_double2(ar_re[it], ar_im[it]);
+      const int r = s_rank[e[it]];
+      s_t[r] = tk[it];
+      s_a[r] = make_double2(ar_re[it], ar_im[it]);
     }
   }
   __syncthreads();
@@ -201,14 +239,14 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
   constexpr int TM = SEG_ENTRIES / M;
   constexpr int ITEMS = TM / NT;
   constexpr uint64_t NONE = ~0ull;
-  __shared__ uint64_t s_w[TM];
+  __shared__ unsigned short s_rank[TM];
   __shared__ uint64_t s_t[TM];
   __shared__ double2 s_a[TM];
   // digit-1 bins below 1024 (|a|^2 < 2, every normalised state) count in LDS;
   // higher bins go straight to the global histogram
   constexpr int LBINS = HIST_BINS / 2;
   __shared__ unsigned int s_hist[LBINS];
-  __shared__ int s_scan[WAVES];
+  __shared__ int s_scan[ITEMS * WAVES];
   __shared__ double s_red[WAVES];
   __shared__ uint64_t s_halo_t[3];
   __shared__ double2 s_halo_a[3];
@@ -216,7 +254,7 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
   __shared__ uint64_t s_prev[2];
 
   for (int b = threadIdx.x; b < LBINS; b += NT) s_hist[b] = 0;
-  const int cnt = lift_stage<TM>(w, a, n, j, T, B0, B1, tile, s_w, s_t, s_a);
+  const int cnt = lift_stage<TM>(w, a, n, j, T, B0, B1, tile, s_rank, s_scan, s_t, s_a);
 
   // group key: the lifted key without the gate bits 0..T
   auto gk = [](uint64_t tk) { return tk >> (T + 1); };
@@ -246,7 +284,6 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
   double ore[ITEMS][M], oim[ITEMS][M];
   unsigned int omask[ITEMS];
   int ooff[ITEMS];
-  int rtot[ITEMS];
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
     const int m = it * NT + threadIdx.x;
@@ -328,19 +365,16 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
     }
     omask[it] = mask;
   }
+  // output offsets in merged (item-major) order: one scan for all items
+  int ocnt[ITEMS];
 #pragma unroll
-  for (int it = 0; it < ITEMS; ++it) {
-    int tot;
-    ooff[it] = block_excl_scan(__popc(omask[it]), tot, s_scan);
-    rtot[it] = tot;
-  }
-  int tile_total = 0;
-#pragma unroll
-  for (int it = 0; it < ITEMS; ++it) tile_total += rtot[it];
+  for (int it = 0; it < ITEMS; ++it) ocnt[it] = __popc(omask[it]);
+  int tile_total;
+  block_excl_scan_items<ITEMS>(ocnt, ooff, tile_total, s_scan);
   if (threadIdx.x == 0) seg_cnt[otile] = tile_total;
 
   double psum = 0.0;
-  int64_t rb = otile * SEG_ENTRIES;
+  const int64_t rb = otile * SEG_ENTRIES;
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
     int64_t pos = rb + ooff[it];
@@ -360,7 +394,6 @@ __device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __
         ++pos;
       }
     }
-    rb += rtot[it];
   }
   const double s = block_sum(psum, s_red);
   if (threadIdx.x == 0) tile_psum[otile] = s;
@@ -383,10 +416,11 @@ template <int TM>
 __device__ void lift_move_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int j,
                                const LiftBound& B0, const LiftBound& B1, int64_t tile, uint64_t* __restrict__ xo,
                                double2* __restrict__ ao) {
-  __shared__ uint64_t s_w[TM];
+  __shared__ unsigned short s_rank[TM];
+  __shared__ int s_scan[(TM / NT) * WAVES];
   __shared__ uint64_t s_t[TM];
   __shared__ double2 s_a[TM];
-  const int cnt = lift_stage<TM>(w, a, n, j, 0, B0, B1, tile, s_w, s_t, s_a);
+  const int cnt = lift_stage<TM>(w, a, n, j, 0, B0, B1, tile, s_rank, s_scan, s_t, s_a);
   const int64_t d = tile * TM;
   for (int m = threadIdx.x; m < cnt; m += NT) {
     xo[d + m] = s_t[m];

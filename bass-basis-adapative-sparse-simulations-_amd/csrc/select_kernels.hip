@@ -21,7 +21,7 @@
 //     k_hist_psum (16 B/entry);
 //   k_extract1: read |a|^2 (16 B/entry); per tile, entries above the digit-1
 //     bucket are counted and summed, entries inside it are kept as candidates
-//     (in order, per tile) together with the digit-2 histogram;
+//     (in input order, per tile) together with the digit-2 histogram;
 //   k_compact: read key + amplitude (24 B/entry), write the k kept entries in
 //     input order at offsets scanned from the per-tile kept counts, divided by
 //     sqrt(kept) when renormalising (24 B per kept entry).
@@ -246,7 +246,7 @@ template <bool BATCH, int EXT>
 __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
                                                  int64_t tiles_end) {
   __shared__ unsigned int s_hist[HIST_BINS];
-  __shared__ int s_scan[WAVES];
+  __shared__ int s_scan[SEL_ITEMS * WAVES];
   __shared__ double s_red[WAVES];
   __shared__ int s_dcnt;
   const int64_t gt0 = (int64_t)blockIdx.x * EXT;
@@ -278,6 +278,7 @@ __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int np
     bool isc[SEL_ITEMS];
     double dsum = 0.0;
     int dcnt = 0;
+    int ci[SEL_ITEMS], coff[SEL_ITEMS];
 #pragma unroll
     for (int it = 0; it < SEL_ITEMS; ++it) {
       const int j = it * NT + threadIdx.x;
@@ -296,17 +297,17 @@ __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int np
           atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
         }
       }
+      ci[it] = isc[it] ? 1 : 0;
     }
-    int rb = 0;
+    // candidates in input order (item-major), one scan for all items
+    int rb;
+    block_excl_scan_items<SEL_ITEMS>(ci, coff, rb, s_scan);
 #pragma unroll
     for (int it = 0; it < SEL_ITEMS; ++it) {
-      int tot;
-      const int off = block_excl_scan(isc[it] ? 1 : 0, tot, s_scan);
       if (isc[it]) {
-        T.c1p[cbase + rb + off] = cb[it];
-        T.c1i[cbase + rb + off] = (unsigned int)(base + it * NT + threadIdx.x);
+        T.c1p[cbase + coff[it]] = cb[it];
+        T.c1i[cbase + coff[it]] = (unsigned int)(base + it * NT + threadIdx.x);
       }
-      rb += tot;
     }
     const double s = block_sum(dsum, s_red);
     atomicAdd(&s_dcnt, dcnt);
