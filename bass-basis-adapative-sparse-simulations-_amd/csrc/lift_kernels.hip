@@ -69,7 +69,7 @@ __device__ __forceinline__ int64_t wave_partition_point(int64_t lo, int64_t hi, 
 
 // The run around merged position d (clamped to n) and the merge split there.
 // Called by all 64 lanes of a wave.
-__device__ LiftBound lift_boundary(const uint64_t* __restrict__ w, int64_t n, int j, int t, int64_t d) {
+__device__ __forceinline__ LiftBound lift_boundary(const uint64_t* __restrict__ w, int64_t n, int j, int t, int64_t d) {
   if (d > n) d = n;
   LiftBound r;
   if (d >= n) {
@@ -230,7 +230,7 @@ __device__ __forceinline__ int lift_stage(const uint64_t* __restrict__ w, const 
 // [tile*TM, ...)), outputs to segment `otile` of (xo, ao), seg_cnt[otile],
 // tile_psum[otile]; digit-1 histogram into hist.
 template <int QB, int MODE>
-__device__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int j,
+__device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int j,
                                const LiftBound& B0, const LiftBound& B1, const GateMat& G, int64_t tile,
                                int64_t otile, uint64_t* __restrict__ xo, double2* __restrict__ ao,
                                unsigned int* hist, double* tile_psum, int* seg_cnt) {
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(NT) void k_lift_gate(const uint64_t* __restrict__ w
 }
 
 template <int TM>
-__device__ void lift_move_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int j,
+__device__ __forceinline__ void lift_move_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int j,
                                const LiftBound& B0, const LiftBound& B1, int64_t tile, uint64_t* __restrict__ xo,
                                double2* __restrict__ ao) {
   __shared__ unsigned short s_rank[TM];
@@ -439,6 +439,9 @@ __global__ __launch_bounds__(NT) void k_lift_move(const uint64_t* __restrict__ w
 // A batch of independent states (one job each) in one launch: job c owns
 // output tiles [tile0_c, tile0_c + ntiles_c) and boundaries [b0_c, b0_c +
 // ntiles_c] (b0_c = tile0_c + c); blocks find their job by binary search.
+// Job of a block-uniform tile index.  The result is made explicitly
+// wave-uniform so the job's fields (pointers, gate matrix) are read with
+// scalar loads instead of occupying vector registers.
 __device__ __forceinline__ int job_of_tile(const LiftJob* jobs, int njobs, int64_t gt) {
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
@@ -448,7 +451,7 @@ __device__ __forceinline__ int job_of_tile(const LiftJob* jobs, int njobs, int64
     else
       hi = mid - 1;
   }
-  return lo;
+  return __builtin_amdgcn_readfirstlane(lo);
 }
 
 __global__ __launch_bounds__(NT) void k_blift_part(const LiftJob* __restrict__ jobs, int njobs, int t, int tm,
@@ -465,6 +468,7 @@ __global__ __launch_bounds__(NT) void k_blift_part(const LiftJob* __restrict__ j
     else
       hi = mid - 1;
   }
+  lo = __builtin_amdgcn_readfirstlane(lo);
   const LiftJob& J = jobs[lo];
   const int64_t u = gb - (J.tile0 + lo);
   const LiftBound r = lift_boundary(J.x, J.n, J.j, t, u * (int64_t)tm);
@@ -507,7 +511,7 @@ __global__ __launch_bounds__(NT) void k_blift_move(const LiftJob* __restrict__ j
 constexpr int RDM_TM = 1024;
 
 // rho01 partial of stored bit p over merged tile u (bounds B0, B1).
-__device__ double2 rdm_pairs_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int p,
+__device__ __forceinline__ double2 rdm_pairs_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int p,
                                   int64_t u, const LiftBound& B0, const LiftBound& B1) {
   constexpr int ITEMS = RDM_TM / NT;
   __shared__ uint64_t s_w[RDM_TM];
@@ -582,7 +586,7 @@ __global__ __launch_bounds__(NT) void k_rdm_pairs(const uint64_t* __restrict__ w
 // Diagonal RDM elements of every bit in one pass: per tile of SEG_ENTRIES
 // entries, lane b of each wave owns bit b and walks the wave's entries in
 // order (LDS broadcast reads).
-__device__ void rdm_diag_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int nbits,
+__device__ __forceinline__ void rdm_diag_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int nbits,
                               int64_t tile, double2* __restrict__ out) {
   constexpr int TILE = SEG_ENTRIES;
   constexpr int PER_WAVE = TILE / WAVES;
