@@ -73,6 +73,7 @@ SIGNATURES = {
     "bass_gather": (_I, [_P, _P, _P, _I64, _P, _P, _P]),
     "bass_rdm1q": (_I, [_P, _P, _I64, _I, _P, _P, _SZ, _P]),
     "bass_rdm1q_ordered": (_I, [_P, _P, _I64, _I, _P, _P, _SZ, _P]),
+    "bass_rdm2q_ordered": (_I, [_P, _P, _I64, _I, _P, _P, _SZ, _P]),
     "bass_ht_clear": (_I, [_P, _P, _I, _P]),
     "bass_ht_insert": (_I, [_P, _P, _I, _P, _I64, _I64, _P, _SZ, _P]),
     "bass_ht_lookup": (_I, [_P, _P, _I, _P, _I64, _P, _P]),

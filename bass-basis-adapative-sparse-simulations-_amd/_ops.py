@@ -148,6 +148,17 @@ def rdm1q(ctx, buf, n, N, ordered=False):
     return rho
 
 
+def rdm2q(ctx, buf, n, j):
+    """4x4 RDM (host complex128) of the pair stored at bits (j, 0) of a
+    key-ordered state, index (b_j << 1) | b_0."""
+    ctx.ensure_ws(n)
+    out = np.zeros(32, dtype=np.float64)
+    rc = ctx.lib.bass_rdm2q_ordered(buf.xp, buf.ap, n, int(j), out.ctypes.data_as(ctypes.c_void_p), ctx.wsp,
+                                    ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_rdm2q_ordered")
+    return out.view(np.complex128).reshape(4, 4).copy()
+
+
 # ---------------------------------------------------------------- batches --
 # A batch is a numpy array of L.JOB_DTYPE records (one per state).
 

@@ -513,6 +513,42 @@ int bass_rdm1q_ordered(const uint64_t* x, const double* a, int64_t n, int N, dou
   return BASS_OK;
 }
 
+int bass_rdm2q_ordered(const uint64_t* x, const double* a, int64_t n, int j, double* rho_out, void* ws,
+                       size_t ws_bytes, void* stream) {
+  if (j < 1 || j > 62) return fail(BASS_E_ARG, "bass_rdm2q_ordered: bit j must be in [1, 62]");
+  if (n < 0 || n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_rdm2q_ordered: bad n");
+  WS_REQUIRE(n);
+  hipStream_t s = (hipStream_t)stream;
+  double* v = at<double>(ws, L.rho);  // 16 doubles
+  if (n <= 0) {
+    fill(v, 16 * 8, 0ull, s);
+  } else {
+    if (rdm2_scratch(n) > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_rdm2q_ordered: scratch does not fit");
+    const int tm = ktimer_begin("rdm2", s);
+    launch_rdm2(x, (const double2*)a, n, j, at<void>(ws, L.big), v, s);
+    ktimer_end(tm, s);
+  }
+  int rc = hip_check("bass_rdm2q_ordered");
+  if (rc) return rc;
+  double h[16];
+  if (hipMemcpyAsync(h, v, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return fail(BASS_E_HIP, "bass_rdm2q_ordered copy: %s", hipGetErrorString(hipGetLastError()));
+  // full 4x4 complex matrix, row-major (re, im): diagonal real, lower triangle
+  // the conjugate of the upper
+  for (int e = 0; e < 32; ++e) rho_out[e] = 0.0;
+  for (int p = 0; p < 4; ++p) rho_out[2 * (p * 4 + p)] = h[p];
+  int k = 4;
+  for (int p = 0; p < 4; ++p)
+    for (int q = p + 1; q < 4; ++q) {
+      rho_out[2 * (p * 4 + q)] = h[k];
+      rho_out[2 * (p * 4 + q) + 1] = h[k + 1];
+      rho_out[2 * (q * 4 + p)] = h[k];
+      rho_out[2 * (q * 4 + p) + 1] = -h[k + 1];
+      k += 2;
+    }
+  return BASS_OK;
+}
+
 // ----------------------------------------------------------- batches ----
 // Per-job region at the start of the workspace's big area (offsets depend on
 // njobs only), then the select's candidate arrays (32 B per tile slot), then

@@ -226,19 +226,116 @@ __device__ __forceinline__ int lift_stage(const uint64_t* __restrict__ w, const 
   return cnt;
 }
 
+// Groups of a staged tile (QB gate bits at the bottom of the lifted key):
+// the groups of the entries merged just before the tile, whose members the
+// tile does not own, and the members of the tile's last group merged after
+// the tile.  Written by thread 0 of the block.
+struct LiftHalo {
+  uint64_t prev[2];
+  uint64_t t[3];
+  double2 a[3];
+  int n;
+};
+
+template <int QB>
+__device__ __forceinline__ uint64_t lift_group_of(uint64_t tk) {
+  return tk >> QB;
+}
+
+// Thread 0 fills H; the caller synchronises.
+template <int QB>
+__device__ __forceinline__ void lift_halo_setup(const uint64_t* __restrict__ w, const double2* __restrict__ a, int j,
+                                                const LiftBound& B0, const LiftBound& B1, const uint64_t* s_t,
+                                                int cnt, LiftHalo& H) {
+  constexpr int T = QB - 1;
+  constexpr uint64_t NONE = ~0ull;
+  if (threadIdx.x != 0) return;
+  // group of the entry merged just before the tile (A or B tail of the run)
+  H.prev[0] = B0.a > B0.s0 ? lift_group_of<QB>(lift_key(w[B0.a - 1], j, T)) : NONE;
+  H.prev[1] = B0.b > B0.sm ? lift_group_of<QB>(lift_key(w[B0.b - 1], j, T)) : NONE;
+  // members of the tile's last group merged after the tile
+  int nh = 0;
+  const uint64_t glast = lift_group_of<QB>(s_t[cnt - 1]);
+  const int64_t cand[3] = {B1.a < B1.sm ? (int64_t)B1.a : -1, B1.b < B1.s1 ? (int64_t)B1.b : -1,
+                           B1.b + 1 < B1.s1 ? (int64_t)B1.b + 1 : -1};
+  for (int c = 0; c < 3; ++c) {
+    if (cand[c] < 0) continue;
+    const uint64_t tk = lift_key(w[cand[c]], j, T);
+    if (lift_group_of<QB>(tk) == glast) {
+      H.t[nh] = tk;
+      H.a[nh] = a[cand[c]];
+      ++nh;
+    }
+  }
+  H.n = nh;
+}
+
+// If merged position m heads its group (the tile owns the group), gathers the
+// member amplitudes by member index (absent members are 0) and their presence
+// mask (bit c set if member c exists) and returns true.  The amplitudes are
+// written through unrolled selects and presence is a bit mask, so everything
+// stays in registers.
+template <int QB>
+__device__ __forceinline__ bool lift_group_gather(int m, int cnt, const uint64_t* s_t, const double2* s_a,
+                                                  const LiftHalo& H, double (&avr)[1 << QB], double (&avi)[1 << QB],
+                                                  unsigned int& pres) {
+  constexpr int M = 1 << QB;
+  const uint64_t g = lift_group_of<QB>(s_t[m]);
+  const bool head = m == 0 ? (g != H.prev[0] && g != H.prev[1]) : g != lift_group_of<QB>(s_t[m - 1]);
+  if (!head) return false;
+#pragma unroll
+  for (int c = 0; c < M; ++c) avr[c] = avi[c] = 0.0;
+  pres = 0;
+  int k = m;
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
+    if (k < cnt && lift_group_of<QB>(s_t[k]) == g) {
+      const int c = (int)(s_t[k] & (M - 1));
+      const double2 v = s_a[k];
+#pragma unroll
+      for (int cc = 0; cc < M; ++cc) {
+        if (c == cc) {
+          avr[cc] = v.x;
+          avi[cc] = v.y;
+        }
+      }
+      pres |= 1u << c;
+      ++k;
+    }
+  }
+  if (k == cnt) {
+#pragma unroll
+    for (int h = 0; h < 3; ++h) {
+      if (h < H.n) {
+        const int c = (int)(H.t[h] & (M - 1));
+        const double2 v = H.a[h];
+#pragma unroll
+        for (int cc = 0; cc < M; ++cc) {
+          if (c == cc) {
+            avr[cc] = v.x;
+            avi[cc] = v.y;
+          }
+        }
+        pres |= 1u << c;
+      }
+    }
+  }
+  return true;
+}
+
 // One tile of a gate: local tile `tile` of the input (merged positions
 // [tile*TM, ...)), outputs to segment `otile` of (xo, ao), seg_cnt[otile],
 // tile_psum[otile]; digit-1 histogram into hist.
 template <int QB, int MODE>
-__device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int j,
-                               const LiftBound& B0, const LiftBound& B1, const GateMat& G, int64_t tile,
-                               int64_t otile, uint64_t* __restrict__ xo, double2* __restrict__ ao,
-                               unsigned int* hist, double* tile_psum, int* seg_cnt) {
+__device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                               int64_t n, int j, const LiftBound& B0, const LiftBound& B1,
+                                               const GateMat& G, int64_t tile, int64_t otile,
+                                               uint64_t* __restrict__ xo, double2* __restrict__ ao,
+                                               unsigned int* hist, double* tile_psum, int* seg_cnt) {
   constexpr int M = 1 << QB;
   constexpr int T = QB - 1;
   constexpr int TM = SEG_ENTRIES / M;
   constexpr int ITEMS = TM / NT;
-  constexpr uint64_t NONE = ~0ull;
   __shared__ unsigned short s_rank[TM];
   __shared__ uint64_t s_t[TM];
   __shared__ double2 s_a[TM];
@@ -248,36 +345,11 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
   __shared__ unsigned int s_hist[LBINS];
   __shared__ int s_scan[ITEMS * WAVES];
   __shared__ double s_red[WAVES];
-  __shared__ uint64_t s_halo_t[3];
-  __shared__ double2 s_halo_a[3];
-  __shared__ int s_nhalo;
-  __shared__ uint64_t s_prev[2];
+  __shared__ LiftHalo s_halo;
 
   for (int b = threadIdx.x; b < LBINS; b += NT) s_hist[b] = 0;
   const int cnt = lift_stage<TM>(w, a, n, j, T, B0, B1, tile, s_rank, s_scan, s_t, s_a);
-
-  // group key: the lifted key without the gate bits 0..T
-  auto gk = [](uint64_t tk) { return tk >> (T + 1); };
-  if (threadIdx.x == 0) {
-    // group of the entry merged just before the tile (A or B tail of the run)
-    s_prev[0] = B0.a > B0.s0 ? gk(lift_key(w[B0.a - 1], j, T)) : NONE;
-    s_prev[1] = B0.b > B0.sm ? gk(lift_key(w[B0.b - 1], j, T)) : NONE;
-    // members of the tile's last group merged after the tile
-    int nh = 0;
-    const uint64_t glast = gk(s_t[cnt - 1]);
-    int64_t cand[3] = {B1.a < B1.sm ? (int64_t)B1.a : -1, B1.b < B1.s1 ? (int64_t)B1.b : -1,
-                       B1.b + 1 < B1.s1 ? (int64_t)B1.b + 1 : -1};
-    for (int c = 0; c < 3; ++c) {
-      if (cand[c] < 0) continue;
-      const uint64_t tk = lift_key(w[cand[c]], j, T);
-      if (gk(tk) == glast) {
-        s_halo_t[nh] = tk;
-        s_halo_a[nh] = a[cand[c]];
-        ++nh;
-      }
-    }
-    s_nhalo = nh;
-  }
+  lift_halo_setup<QB>(w, a, j, B0, B1, s_t, cnt, s_halo);
   __syncthreads();
 
   uint64_t okey[ITEMS][M];
@@ -288,79 +360,33 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
   for (int it = 0; it < ITEMS; ++it) {
     const int m = it * NT + threadIdx.x;
     unsigned int mask = 0;
-    if (m < cnt) {
-      const uint64_t tk = s_t[m];
-      const uint64_t g = gk(tk);
-      const bool head = m == 0 ? (g != s_prev[0] && g != s_prev[1]) : g != gk(s_t[m - 1]);
-      if (head) {
-        // member amplitudes by member index c; written through unrolled
-        // selects so the arrays stay in registers
-        double avr[M], avi[M];
-        bool pres[M];
+    double avr[M], avi[M];
+    unsigned int pres;
+    if (m < cnt && lift_group_gather<QB>(m, cnt, s_t, s_a, s_halo, avr, avi, pres)) {
+      const uint64_t base = s_t[m] & ~(uint64_t)(M - 1);
 #pragma unroll
-        for (int c = 0; c < M; ++c) {
-          avr[c] = avi[c] = 0.0;
-          pres[c] = false;
-        }
-        int k = m;
+      for (int o = 0; o < M; ++o) {
+        double2 acc = make_double2(0.0, 0.0);
+        bool any = false;
+        if constexpr (MODE == MODE_GATE) {
 #pragma unroll
-        for (int s = 0; s < M; ++s) {
-          if (k < cnt && gk(s_t[k]) == g) {
-            const int c = (int)(s_t[k] & (M - 1));
-            const double2 v = s_a[k];
-#pragma unroll
-            for (int cc = 0; cc < M; ++cc) {
-              if (c == cc) {
-                avr[cc] = v.x;
-                avi[cc] = v.y;
-                pres[cc] = true;
-              }
-            }
-            ++k;
-          }
-        }
-        if (k == cnt) {
-#pragma unroll
-          for (int h = 0; h < 3; ++h) {
-            if (h < s_nhalo) {
-              const int c = (int)(s_halo_t[h] & (M - 1));
-              const double2 v = s_halo_a[h];
-#pragma unroll
-              for (int cc = 0; cc < M; ++cc) {
-                if (c == cc) {
-                  avr[cc] = v.x;
-                  avi[cc] = v.y;
-                  pres[cc] = true;
-                }
-              }
+          for (int c = 0; c < M; ++c) {
+            const double2 term = cmul(G.m[o * M + c], make_double2(avr[c], avi[c]));
+            if (((pres >> c) & 1u) && cnonzero(term)) {
+              acc = any ? cadd(acc, term) : term;
+              any = true;
             }
           }
+        } else {
+          acc = cmul(G.m[o * M + 0], make_double2(avr[0], avi[0]));
+#pragma unroll
+          for (int c = 1; c < M; ++c) acc = cadd(acc, cmul(G.m[o * M + c], make_double2(avr[c], avi[c])));
+          any = hypot(acc.x, acc.y) > 1e-30;
         }
-        const uint64_t base = tk & ~(uint64_t)(M - 1);
-#pragma unroll
-        for (int o = 0; o < M; ++o) {
-          double2 acc = make_double2(0.0, 0.0);
-          bool any = false;
-          if constexpr (MODE == MODE_GATE) {
-#pragma unroll
-            for (int c = 0; c < M; ++c) {
-              const double2 term = cmul(G.m[o * M + c], make_double2(avr[c], avi[c]));
-              if (pres[c] && cnonzero(term)) {
-                acc = any ? cadd(acc, term) : term;
-                any = true;
-              }
-            }
-          } else {
-            acc = cmul(G.m[o * M + 0], make_double2(avr[0], avi[0]));
-#pragma unroll
-            for (int c = 1; c < M; ++c) acc = cadd(acc, cmul(G.m[o * M + c], make_double2(avr[c], avi[c])));
-            any = hypot(acc.x, acc.y) > 1e-30;
-          }
-          okey[it][o] = base | (uint64_t)o;
-          ore[it][o] = acc.x;
-          oim[it][o] = acc.y;
-          if (any) mask |= 1u << o;
-        }
+        okey[it][o] = base | (uint64_t)o;
+        ore[it][o] = acc.x;
+        oim[it][o] = acc.y;
+        if (any) mask |= 1u << o;
       }
     }
     omask[it] = mask;
@@ -410,6 +436,108 @@ __global__ __launch_bounds__(NT) void k_lift_gate(const uint64_t* __restrict__ w
   const int64_t tile = blockIdx.x;
   lift_gate_tile<QB, MODE>(w, a, n, j, bounds[tile], bounds[tile + 1], G, tile, tile, xo, ao, hist, tile_psum,
                            seg_cnt);
+}
+
+// ------------------------------------------------- two-qubit RDM ------
+// RDM of the qubit pair stored at bits (1, 0) after the lift of bit j to bit 1
+// (the second qubit already at bit 0; src/simulation/bass_simulator.py:202-234):
+// rho[p][q] = sum over groups (entries equal outside the pair) of
+// a_p conj(a_q), member index p = (bit 1 << 1) | bit 0, absent members 0.
+// Per tile the 16 real values of the upper triangle (4 diagonal, 6 complex
+// off-diagonal (p < q) in row order) are summed in a fixed order; a finishing
+// kernel reduces the tiles, also in a fixed order.
+constexpr int RDM2_VALS = 16;
+
+__device__ __forceinline__ void lift_rdm2_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                               int64_t n, int j, const LiftBound& B0, const LiftBound& B1,
+                                               int64_t tile, double* __restrict__ part) {
+  constexpr int QB = 2, M = 4, T = 1;
+  constexpr int TM = SEG_ENTRIES / M;
+  constexpr int ITEMS = TM / NT;
+  __shared__ unsigned short s_rank[TM];
+  __shared__ uint64_t s_t[TM];
+  __shared__ double2 s_a[TM];
+  __shared__ int s_scan[ITEMS * WAVES];
+  __shared__ double s_red[RDM2_VALS * WAVES];
+  __shared__ LiftHalo s_halo;
+  const int cnt = lift_stage<TM>(w, a, n, j, T, B0, B1, tile, s_rank, s_scan, s_t, s_a);
+  lift_halo_setup<QB>(w, a, j, B0, B1, s_t, cnt, s_halo);
+  __syncthreads();
+  double acc[RDM2_VALS];
+#pragma unroll
+  for (int v = 0; v < RDM2_VALS; ++v) acc[v] = 0.0;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int m = it * NT + threadIdx.x;
+    double avr[M], avi[M];
+    unsigned int pres;
+    if (m < cnt && lift_group_gather<QB>(m, cnt, s_t, s_a, s_halo, avr, avi, pres)) {
+#pragma unroll
+      for (int p = 0; p < M; ++p) acc[p] += cabs2(make_double2(avr[p], avi[p]));
+      int v = M;
+#pragma unroll
+      for (int p = 0; p < M; ++p) {
+#pragma unroll
+        for (int q = p + 1; q < M; ++q) {
+          const double2 z = cmul(make_double2(avr[p], avi[p]), make_double2(avr[q], -avi[q]));
+          acc[v] += z.x;
+          acc[v + 1] += z.y;
+          v += 2;
+        }
+      }
+    }
+  }
+  // block sums of the 16 values: shuffle tree per wave, then waves in order
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < RDM2_VALS; ++v) {
+    const double t = wave_sum(acc[v]);
+    if (lane == 0) s_red[v * WAVES + wv] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < RDM2_VALS) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < WAVES; ++i) t += s_red[threadIdx.x * WAVES + i];
+    part[tile * RDM2_VALS + threadIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_lift_rdm2(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                                   int64_t n, int j, const LiftBound* __restrict__ bounds,
+                                                   double* __restrict__ part) {
+  const int64_t tile = blockIdx.x;
+  lift_rdm2_tile(w, a, n, j, bounds[tile], bounds[tile + 1], tile, part);
+}
+
+// Value v (block v) of the tiles' partials, summed in a fixed order.
+__global__ __launch_bounds__(NTB) void k_rdm2_fin(const double* __restrict__ part, int64_t ntiles, double* out) {
+  __shared__ double s_red[WAVESB];
+  const int v = blockIdx.x;
+  double s = 0.0;
+  for (int64_t t = threadIdx.x; t < ntiles; t += NTB) s += part[t * RDM2_VALS + v];
+  const double tot = block_sum<NTB>(s, s_red);
+  if (threadIdx.x == 0) out[v] = tot;
+}
+
+size_t rdm2_scratch(int64_t n) {
+  const int64_t tiles = (n + SEG_ENTRIES / 4 - 1) / (SEG_ENTRIES / 4);
+  return (((size_t)(tiles + 1) * sizeof(LiftBound) + 255) & ~(size_t)255) + (size_t)tiles * RDM2_VALS * 8 + 256;
+}
+
+// out: RDM2_VALS doubles (device).
+int launch_rdm2(const uint64_t* x, const double2* a, int64_t n, int j, void* scratch, double* out, hipStream_t s) {
+  constexpr int TM = SEG_ENTRIES / 4;
+  const int64_t tiles = (n + TM - 1) / TM, nbound = tiles + 1;
+  char* base = (char*)scratch;
+  LiftBound* bounds = (LiftBound*)base;
+  base += ((size_t)nbound * sizeof(LiftBound) + 255) & ~(size_t)255;
+  double* part = (double*)base;
+  hipLaunchKernelGGL(k_lift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES), 1), dim3(NT), 0, s, x, n, j, 1, TM,
+                     nbound, bounds);
+  hipLaunchKernelGGL(k_lift_rdm2, dim3((unsigned)tiles), dim3(NT), 0, s, x, a, n, j, bounds, part);
+  hipLaunchKernelGGL(k_rdm2_fin, dim3(RDM2_VALS), dim3(NTB), 0, s, part, tiles, out);
+  return (int)hipGetLastError();
 }
 
 template <int TM>

@@ -403,9 +403,92 @@ class BASS:
             if not improved:
                 break
         if self.use_2qubit_rotations and self.N >= 2:
-            raise NotImplementedError("use_2qubit_rotations: 2-qubit brick-wall pass not built yet")
+            self._optimize_2q_pass(state, [(i, i + 1) for i in range(0, self.N - 1, 2)])
+            self._optimize_2q_pass(state, [(i, i + 1) for i in range(1, self.N - 1, 2)])
         self._truncate_state(state)
         self._renorm(state)
+
+    def _spare(self, state, name, n):
+        buf = getattr(state, name, None)
+        if buf is None:
+            buf = Buffers(max(n, 1), self._ctx.device)
+            setattr(state, name, buf)
+        buf.ensure(max(n, 1))
+        return buf
+
+    def _rotate2_truncate(self, state, src, n, j, V4, pos, dst):
+        """Rotation of the pair stored at bits (j, 0) by V4 (lifting bit j to
+        bit 1), then _truncate_state + _renorm (:565-572, :551-554) into dst.
+        Returns (nnz, sum |a|^2 before the renormalisation, sum |a|^4 after,
+        positions)."""
+        ctx = self._ctx
+        _, ht = _ops.lift(ctx, src, n, j, state._rot, G=V4, nq=2, mode=L.MODE_ROT, count=False)
+        tpos = lifted_positions(pos, j, 1)
+        st = _ops.truncate(ctx, state._rot, None, self.k, dst, renorm=True, hist_tiles=ht, segmented=True,
+                           key_map=_ops.key_map_arg(tpos))
+        m = st.n_in
+        self._count("rotate2")
+        self._count("amp_updates", m)
+        self._bytes("rot2q", n, m)
+        if st.flags & 1:
+            self._bytes("select", m, self.k)
+            return self.k, st.kept, st.sum4, tpos
+        self._count("bytes_select", 48 * m)
+        norm = math.sqrt(st.sum2)
+        if norm > 1e-30:
+            _ops.scale(ctx, dst, m, norm)
+        _, s4 = _ops.moments(ctx, dst, m)
+        return m, st.sum2, s4, tpos
+
+    def _optimize_2q_pass(self, state, pairs):
+        """:679-728 -- per pair (q1, q2): eigenbasis V of the 2-qubit RDM
+        (np.linalg.eigh, eigenvalues descending); rotate by V^dagger, truncate,
+        renormalise, rotate back by V, truncate, renormalise; the result stays
+        unless PR grew (by more than accept_rtol), else the saved state is
+        restored.  The pair is worked on at stored bits (1, 0): the second
+        qubit is moved to bit 0 (into a spare buffer; the state itself is the
+        saved copy), unless the first one is there already (then the roles and
+        the matrix index order swap)."""
+        ctx = self._ctx
+        for q1, q2 in pairs:
+            pr_before = self._pr(state)
+            swap = state.pos[q1] == 0 and state.pos[q2] != 0
+            hi_q, lo_q = (q2, q1) if swap else (q1, q2)
+            src, pos, n = state._buf, state.pos, state.nnz
+            if pos[lo_q] != 0:
+                jb = int(pos[lo_q])
+                mv = self._spare(state, "_m2", n)
+                _ops.lift(ctx, src, n, jb, mv, mode=L.MODE_MOVE)
+                self._bytes("move", n, n)
+                src, pos = mv, lifted_positions(pos, jb, 0)
+            j = int(pos[hi_q])
+            rho = _ops.rdm2q(ctx, src, n, j)
+            self._count("rdm2")
+            if swap:
+                rho = rho[np.ix_(_SWAP_IDX, _SWAP_IDX)]
+            evals, evecs = np.linalg.eigh(rho)
+            V = evecs[:, np.argsort(-evals)]
+            Vd = V.conj().T
+            if swap:
+                V, Vd = V[np.ix_(_SWAP_IDX, _SWAP_IDX)], Vd[np.ix_(_SWAP_IDX, _SWAP_IDX)]
+            c1 = self._spare(state, "_c1", self.k)
+            n1, frac1, _, pos1 = self._rotate2_truncate(state, src, n, j, Vd, pos, c1)
+            c2 = self._spare(state, "_c2", self.k)
+            n2, frac2, s4, _ = self._rotate2_truncate(state, c1, n1, 1, V, pos1, c2)
+            pr_after = 1.0 / s4 if (n2 > 0 and s4 > 1e-60) else float("inf")
+            accept = not (pr_after > pr_before * (1.0 - self.accept_rtol))
+            if self.trace is not None:
+                self.trace.append(("rot2", (q1, q2), accept, pr_before, pr_after, frac1 * frac2))
+            if not accept:
+                self._count("reject2")
+                continue
+            self._count("accept2")
+            state._buf, state._c2 = c2, state._buf
+            state.nnz = n2
+            state.pos = pos1
+            state._sum2, state._sum4 = None, s4
+            state._hist_tiles = 0
+            self._gamma_running *= np.sqrt(frac1 * frac2)
 
     def _truncate_state(self, state):
         """:565-572 (no renormalisation)."""

@@ -187,6 +187,16 @@ int bass_rdm1q(const uint64_t* x, const double* a, int64_t n, int N, double* rho
 int bass_rdm1q_ordered(const uint64_t* x, const double* a, int64_t n, int N, double* rho_out, void* ws,
                        size_t ws_bytes, void* stream);
 
+/* Two-qubit RDM of a key-ordered state for the pair (first qubit stored at
+ * bit j >= 1, second at bit 0): rho[p][q] = sum over entries equal outside the
+ * pair of a_p conj(a_q), index (b_first << 1) | b_second.  rho_out: 4x4
+ * complex, row-major, 32 doubles (re, im).  The pair's groups are made
+ * adjacent by the merge that lifts bit j to bit 1, as for a 2-qubit gate.
+ * Replaces compute_2qubit_rdm (src/simulation/bass_simulator.py:202-234) in
+ * the 2-qubit pass (:679-728).  Synchronous. */
+int bass_rdm2q_ordered(const uint64_t* x, const double* a, int64_t n, int j, double* rho_out, void* ws,
+                       size_t ws_bytes, void* stream);
+
 /* ---- batches: many independent key-ordered states per launch ----------
  * Each call runs one launch sequence for all njobs states (per-job scalars via
  * bass_batch_stats).  Job tables are staged through pinned host buffers.

@@ -298,7 +298,11 @@ class BASS:
             frac2 = float(np.sum(np.abs(state.alpha[: state.nnz]) ** 2))
             self._renorm(state)
             pr_after = self._pr(state)
-            if pr_after > pr_before:
+            # accept_rtol = 0: the reference's "revert iff pr_after > pr_before"
+            accept = not (pr_after > pr_before * (1.0 - self.accept_rtol))
+            if self.trace is not None:
+                self.trace.append(("rot2", (q1, q2), bool(accept), pr_before, pr_after, frac1 * frac2))
+            if not accept:
                 state.x[:nnz_save] = x_save
                 state.alpha[:nnz_save] = a_save
                 state.nnz = nnz_save

@@ -82,6 +82,11 @@ def test_argument_errors_without_gpu(lib):
     assert rc == L.BASS_E_OVERFLOW
     rc = lib.bass_radix_sort(None, None, 10, 0, None, 0, None)
     assert rc == L.BASS_E_ARG
+    rho = (ctypes.c_double * 32)()
+    rc = lib.bass_rdm2q_ordered(None, None, 10, 0, rho, None, 0, None)
+    assert rc == L.BASS_E_ARG and "bit j" in L.last_error()
+    rc = lib.bass_rdm2q_ordered(None, None, 10, 3, rho, None, 0, None)
+    assert rc == L.BASS_E_WORKSPACE
     rc = lib.bass_gate(None, None, 10, 1, 0, 0, G, 0, 1, 0, None, None, 40, ctypes.byref(ht), None, 0, None)
     assert rc == L.BASS_E_WORKSPACE
 

@@ -90,6 +90,45 @@ def test_fixed_basis_end_to_end(gpu_ctx, dedup):
     np.testing.assert_allclose(sim.get_metrics()["gamma_squared"], ref.gamma_history ** 2, rtol=0, atol=1e-10)
 
 
+@pytest.mark.parametrize("idx", [0, 1])
+def test_bass_2qubit_pass(gpu_ctx, idx):
+    """use_2qubit_rotations=True (the brick-wall pass after the 1-qubit
+    passes, bass_simulator.py:679-728, :820-824): the same decisions as the
+    oracle (1-qubit rotations and 2-qubit pairs, accepted or reverted), F and
+    gamma^2 within 1e-10."""
+    name, N, circ, ends = _circuits()[idx]
+    K = 128
+    print(f"[case] bass 2q pass {name} N={N} K={K} gates={len(circ)}", flush=True)
+    B = pkg_module("simulation.bass_simulator")
+    kw = dict(optimize_every=5, use_2qubit_rotations=True)
+    if ends is not None:
+        kw = dict(optimize_at=ends, use_2qubit_rotations=True)
+    td, to = [], []
+    sim = B.BASS(N, K, device="cuda:0", trace=td, **kw)
+    st = sim.simulate(circ)
+    ref = OD.BASS(N, K, selector=OD.key_tiebreak_selector, trace=to, accept_rtol=sim.accept_rtol, **kw)
+    rs = ref.simulate(circ)
+
+    def decisions(tr):
+        return [(e[0], e[1], e[2] if e[0] in ("rot", "rot2") else None) for e in tr]
+
+    n2 = sum(e[0] == "rot2" for e in to)
+    acc2 = sum(e[0] == "rot2" and e[2] for e in to)
+    print(f"  2-qubit pairs tried {n2}, accepted {acc2}, counters={sim.counters}", flush=True)
+    assert n2 > 0
+    assert decisions(td) == decisions(to), "decision logs differ"
+    sv = OE.exact_statevector(N, circ)
+    f_dev = OE.fidelity_bass(N, st.x[: st.nnz], st.alpha[: st.nnz], sim.U, sv)
+    f_ora = OE.fidelity_bass(N, rs.x[: rs.nnz], rs.alpha[: rs.nnz], ref.U, sv)
+    print(f"  F dev={f_dev:.15f} oracle={f_ora:.15f} gamma2 {st.gamma ** 2:.15f} {rs.gamma ** 2:.15f}", flush=True)
+    assert st.nnz == rs.nnz
+    assert abs(f_dev - f_ora) <= 1e-10
+    assert abs(st.gamma ** 2 - rs.gamma ** 2) <= 1e-10
+    psi_dev = OE.to_statevector(N, st.x[: st.nnz], st.alpha[: st.nnz], sim.U)
+    psi_ora = OE.to_statevector(N, rs.x[: rs.nnz], rs.alpha[: rs.nnz], ref.U)
+    np.testing.assert_allclose(psi_dev, psi_ora, rtol=0, atol=1e-10)
+
+
 def test_tfim_config0_lockstep(gpu_ctx):
     """Uniform TFIM n=20, J=h=1, dt=0.05, 40 steps, K=2^14, optimize_every=39.
     From |0...0> with uniform couplings the |a|^2 values tie exactly across

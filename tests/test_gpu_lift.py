@@ -259,3 +259,28 @@ def test_truncate_massive_ties(gpu_ctx, n, k, ordered):
     OK.top_k_truncate_kernel_opt(ox, oa, n, k, selector=OD.key_tiebreak_selector)
     assert set(gx.tolist()) == set(ox[:k].tolist())
     assert_same_entries(gx, dev.to_host_amps(dst.a, k), ox[:k], oa[:k], exact=True)
+
+
+@pytest.mark.parametrize("n,nbits", [(1, 3), (7, 3), (5000, 13), (60000, 20)])
+def test_rdm2q_ordered(gpu_ctx, n, nbits):
+    """4x4 RDM of a qubit pair of a key-ordered state (second qubit moved to
+    bit 0, first at bit j) against compute_2qubit_rdm, both index orders."""
+    ops, dev, SS, L = _mods()
+    rng = np.random.default_rng(n + 3 * nbits)
+    x, a = random_state(n, nbits, seed=n + 31)
+    pos = rng.permutation(nbits)
+    q1, q2 = (int(v) for v in rng.choice(nbits, size=2, replace=False))
+    for f, s in ((q1, q2), (q2, q1)):
+        print(f"[case] rdm2q n={n} nbits={nbits} pair=({f},{s}) bits=({pos[f]},{pos[s]})", flush=True)
+        w, aw = _ordered(x, a, pos)
+        src = _upload(dev, gpu_ctx, w, aw)
+        p1 = pos
+        if pos[s] != 0:
+            mv = dev.Buffers(n, gpu_ctx.device)
+            ops.lift(gpu_ctx, src, n, int(pos[s]), mv, mode=L.MODE_MOVE)
+            src, p1 = mv, SS.lifted_positions(pos, int(pos[s]), 0)
+        assert p1[s] == 0 and p1[f] >= 1
+        rho = ops.rdm2q(gpu_ctx, src, n, int(p1[f]))
+        ref = OK.compute_2qubit_rdm(x, a, n, f, s)
+        np.testing.assert_allclose(rho, ref, rtol=0, atol=1e-13 * max(1.0, np.abs(ref).max()))
+        assert np.allclose(rho, rho.conj().T, rtol=0, atol=0)

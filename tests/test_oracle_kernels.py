@@ -139,6 +139,18 @@ def test_rdms_equal_partial_trace():
         np.testing.assert_allclose(rho[q], m @ m.conj().T, atol=1e-14)
 
 
+@pytest.mark.parametrize("q1,q2", [(0, 1), (5, 2), (3, 6)])
+def test_2qubit_rdm_equals_partial_trace(q1, q2):
+    """rho[p][q] = sum_rest psi[p, rest] conj(psi[q, rest]), index (b_q1 << 1) | b_q2."""
+    N = 7
+    x, a = random_state(70, N, seed=q1 * 7 + q2)
+    rho = OK.compute_2qubit_rdm(x, a, 70, q1, q2)
+    psi = sparse_to_dense(x, a, N).reshape([2] * N)
+    m = np.moveaxis(psi, (N - 1 - q1, N - 1 - q2), (0, 1)).reshape(4, -1)
+    np.testing.assert_allclose(rho, m @ m.conj().T, atol=1e-14)
+    assert abs(np.trace(rho) - np.sum(np.abs(a) ** 2)) < 1e-13
+
+
 def test_eigh_2x2_against_numpy_and_gauge():
     rng = np.random.default_rng(4)
     for _ in range(50):
