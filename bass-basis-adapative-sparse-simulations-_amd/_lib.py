@@ -79,6 +79,9 @@ SIGNATURES = {
     "bass_ht_lookup": (_I, [_P, _P, _I, _P, _I64, _P, _P]),
     "bass_radix_sort": (_I, [_P, _P, _I64, _I, _P, _SZ, _P]),
     "bass_merge_sorted": (_I, [_P, _P, _I64, _P, _P, _P, _SZ, _P]),
+    "bass_dense_scatter": (_I, [_P, _P, _I64, _P, _I, _P, _I, _P]),
+    "bass_dense_gate": (_I, [_P, _I, _I, _I, _I, _P, _P]),
+    "bass_dense_vdot": (_I, [_P, _P, _I64, _P, _P, _SZ, _P]),
     "bass_batch_step": (_I, [_P, _I, _I, _I, _I, _P, _P, _I64, _P, _SZ, _P]),
     "bass_batch_move": (_I, [_P, _I, _P, _SZ, _P]),
     "bass_batch_diag2q": (_I, [_P, _I, _P, _SZ, _P]),

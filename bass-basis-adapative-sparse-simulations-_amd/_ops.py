@@ -7,6 +7,7 @@ return scalars read them back through bass_read_stats (one synchronisation).
 import ctypes
 
 import numpy as np
+import torch
 
 from . import _lib as L
 
@@ -157,6 +158,45 @@ def rdm2q(ctx, buf, n, j):
                                     ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_rdm2q_ordered")
     return out.view(np.complex128).reshape(4, 4).copy()
+
+
+# ------------------------------------------------------------------ dense --
+def dense_state(N, device):
+    """A zeroed complex128[2^N] device vector."""
+    return torch.zeros(1 << int(N), dtype=torch.complex128, device=device)
+
+
+def dense_scatter(ctx, buf, n, N, psi, key_map=None):
+    """psi = 0, psi[key(x_i)] = a_i for the n entries of buf (key_map: stored
+    bit -> qubit of a key-ordered state, None = keys stored as they are)."""
+    if key_map is None:
+        kp, kb = None, 0
+    else:
+        km = np.ascontiguousarray(key_map, dtype=np.int32)
+        kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
+    rc = ctx.lib.bass_dense_scatter(buf.xp, buf.ap, int(n), kp, kb, L.ptr(psi), int(N), ctx.stream)
+    L.check(rc, "bass_dense_scatter")
+
+
+def dense_gate(ctx, psi, N, G, qubits):
+    """In place: gate G (2x2 or 4x4, index (b_q0 << 1) | b_q1) on qubits."""
+    buf, gp = L.mat_arg(G)
+    q0 = int(qubits[0])
+    q1 = int(qubits[1]) if len(qubits) == 2 else 0
+    rc = ctx.lib.bass_dense_gate(L.ptr(psi), int(N), len(qubits), q0, q1, gp, ctx.stream)
+    L.check(rc, "bass_dense_gate")
+
+
+def dense_vdot(ctx, u, v):
+    """sum conj(u_i) v_i of two device vectors (host complex)."""
+    if u.numel() != v.numel():
+        raise ValueError("dense_vdot: lengths differ")
+    ctx.ensure_ws(max(1, u.numel() >> 12))  # 16 B of partials per 4096 entries
+    out = np.zeros(2, dtype=np.float64)
+    rc = ctx.lib.bass_dense_vdot(L.ptr(u), L.ptr(v), int(u.numel()), out.ctypes.data_as(ctypes.c_void_p), ctx.wsp,
+                                 ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_dense_vdot")
+    return complex(out[0], out[1])
 
 
 # ---------------------------------------------------------------- batches --

@@ -1,17 +1,19 @@
 """Metrics on device states (reference: src/benchmarking/fidelity.py:48-89)."""
 
 import numpy as np
+import torch
 
 from .. import _ops
-from .._device import to_host_amps, to_host_keys
+from .._device import to_host_amps
 
 
 def compute_fidelity(sparse_state, exact_state):
-    """fidelity.py:48-65: |<psi_exact | phi>|^2 over the stored keys (stored
-    amplitudes are already normalised)."""
-    xs = to_host_keys(sparse_state._buf.x, sparse_state.nnz).astype(np.int64)
+    """fidelity.py:48-65: |<psi_exact | phi>|^2 over the stored entries (the
+    stored amplitudes are already normalised).  Keys of a key-ordered state
+    are mapped back from the stored words."""
+    xs = sparse_state.keys().astype(np.int64)
     al = to_host_amps(sparse_state._buf.a, sparse_state.nnz)
-    return float(np.abs(np.dot(exact_state[xs].conj(), al)) ** 2)
+    return float(np.abs(np.dot(np.asarray(exact_state)[xs].conj(), al)) ** 2)
 
 
 def compute_fidelity_bounds(sparse_state):
@@ -26,7 +28,13 @@ def compute_participation_ratio(sparse_state):
 
 
 def fidelity_bass(sim, state, exact_sv):
-    """src/experimentation/runner.py:620-626."""
+    """src/experimentation/runner.py:620-626: |<psi_exact | (U_0 x ... x
+    U_{N-1}) phi>|^2, reconstruction and overlap on the device.  exact_sv: a
+    device tensor (kept there) or a host array (uploaded)."""
     if sim.N > 24:
         return 0.0
-    return float(np.abs(np.vdot(exact_sv, sim.to_statevector(state))) ** 2)
+    psi = sim.to_statevector_device(state)
+    ex = exact_sv if isinstance(exact_sv, torch.Tensor) else torch.from_numpy(
+        np.ascontiguousarray(exact_sv, dtype=np.complex128))
+    ex = ex.to(device=psi.device, dtype=torch.complex128)
+    return float(abs(_ops.dense_vdot(sim._ctx, ex, psi)) ** 2)

@@ -15,7 +15,7 @@ PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "libbass_hip.so")
 OBJ = os.path.join(HERE, "_build")
 SOURCES = ["capi.hip", "gate_kernels.hip", "lift_kernels.hip", "select_kernels.hip", "rdm_kernels.hip",
-           "sort_kernels.hip", "batch_kernels.hip"]
+           "sort_kernels.hip", "batch_kernels.hip", "dense_kernels.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BASS_OFFLOAD_ARCH", "gfx950")
 

@@ -549,6 +549,62 @@ int bass_rdm2q_ordered(const uint64_t* x, const double* a, int64_t n, int j, dou
   return BASS_OK;
 }
 
+// ------------------------------------------------------------ dense ----
+int bass_dense_scatter(const uint64_t* x, const double* a, int64_t n, const int32_t* key_map, int key_bits,
+                       double* psi, int N, void* stream) {
+  if (N < 1 || N > 40) return fail(BASS_E_ARG, "bass_dense_scatter: N out of range");
+  if (n < 0 || n > (1ll << N)) return fail(BASS_E_ARG, "bass_dense_scatter: bad n");
+  KeyMap km;
+  memset(&km, 0, sizeof(km));
+  if (key_bits) {
+    if (key_bits != N || !key_map) return fail(BASS_E_ARG, "bass_dense_scatter: key_map must have N entries");
+    uint64_t seen = 0;
+    for (int p = 0; p < key_bits; ++p) {
+      const int q = key_map[p];
+      if (q < 0 || q >= N || ((seen >> q) & 1ull)) return fail(BASS_E_ARG, "bass_dense_scatter: key_map is not a permutation");
+      seen |= 1ull << q;
+      km.q[p] = (unsigned char)q;
+    }
+    km.nbits = key_bits;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  launch_dense_scatter(x, (const double2*)a, n, km, (double2*)psi, 1ll << N, s);
+  return hip_check("bass_dense_scatter");
+}
+
+int bass_dense_gate(double* psi, int N, int nq, int q0, int q1, const double* G, void* stream) {
+  if (N < 1 || N > 40) return fail(BASS_E_ARG, "bass_dense_gate: N out of range");
+  if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_dense_gate: nq must be 1 or 2");
+  if (q0 < 0 || q0 >= N || (nq == 2 && (q1 < 0 || q1 >= N || q1 == q0)))
+    return fail(BASS_E_ARG, "bass_dense_gate: bad qubits");
+  if (nq == 2 && N < 2) return fail(BASS_E_ARG, "bass_dense_gate: 2-qubit gate needs N >= 2");
+  GateMat g;
+  memset(&g, 0, sizeof(g));
+  const int M = 1 << nq;
+  for (int e = 0; e < M * M; ++e) g.m[e] = make_double2(G[2 * e], G[2 * e + 1]);
+  launch_dense_gate((double2*)psi, N, nq, q0, q1, g, (hipStream_t)stream);
+  return hip_check("bass_dense_gate");
+}
+
+int bass_dense_vdot(const double* u, const double* v, int64_t len, double* out, void* ws, size_t ws_bytes,
+                    void* stream) {
+  if (len < 0) return fail(BASS_E_ARG, "bass_dense_vdot: bad length");
+  WS_REQUIRE(0);
+  if (dense_vdot_scratch(len) + 16 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_dense_vdot: scratch does not fit");
+  hipStream_t s = (hipStream_t)stream;
+  double2* res = at<double2>(ws, L.rho);
+  if (len == 0) {
+    fill(res, 16, 0ull, s);
+  } else {
+    launch_dense_vdot((const double2*)u, (const double2*)v, len, at<void>(ws, L.big), res, s);
+  }
+  int rc = hip_check("bass_dense_vdot");
+  if (rc) return rc;
+  if (hipMemcpyAsync(out, res, 16, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return fail(BASS_E_HIP, "bass_dense_vdot copy: %s", hipGetErrorString(hipGetLastError()));
+  return BASS_OK;
+}
+
 // ----------------------------------------------------------- batches ----
 // Per-job region at the start of the workspace's big area (offsets depend on
 // njobs only), then the select's candidate arrays (32 B per tile slot), then

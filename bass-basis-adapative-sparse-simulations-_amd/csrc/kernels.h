@@ -205,4 +205,11 @@ int launch_merge(const uint64_t* x, const double2* a, int64_t n, uint64_t* xo, d
 int ktimer_begin(const char* name, hipStream_t s);
 void ktimer_end(int token, hipStream_t s);
 
+// dense state vectors (dense_kernels.hip)
+int launch_dense_scatter(const uint64_t* x, const double2* a, int64_t n, const KeyMap& km, double2* psi, int64_t len,
+                         hipStream_t s);
+int launch_dense_gate(double2* psi, int N, int nq, int q0, int q1, const GateMat& G, hipStream_t s);
+size_t dense_vdot_scratch(int64_t len);
+int launch_dense_vdot(const double2* u, const double2* v, int64_t len, void* scratch, double2* out, hipStream_t s);
+
 }  // namespace bass

@@ -318,15 +318,22 @@ class BASS:
         for kk in [kk for kk in self._kron_cache if qubit in kk]:
             del self._kron_cache[kk]
 
-    def to_statevector(self, state):
-        """:608-621 (host: dense reconstruction for N <= 24)."""
-        if self.N > 24:
+    def to_statevector_device(self, state, max_qubits=24):
+        """:608-621 on the device: scatter the stored entries to their keys,
+        then rotate every qubit j by its frame U_j (dense, in place)."""
+        if self.N > max_qubits:
             raise ValueError(f"N={self.N} too large for statevector")
-        psi = np.zeros(2 ** self.N, dtype=np.complex128)
-        psi[state.keys().astype(np.int64)] = to_host_amps(state._buf.a, state.nnz)
+        ctx = self._ctx
+        psi = _ops.dense_state(self.N, ctx.device)
+        km = _ops.key_map_arg(state.pos) if getattr(state, "pos", None) is not None else None
+        _ops.dense_scatter(ctx, state._buf, state.nnz, self.N, psi, key_map=km)
         for j in range(self.N):
-            psi = np.einsum("ij,kjl->kil", self.U[j], psi.reshape(-1, 2, 1 << j)).reshape(-1)
+            _ops.dense_gate(ctx, psi, self.N, self.U[j], (j,))
         return psi
+
+    def to_statevector(self, state):
+        """:608-621 (host array of the device reconstruction, N <= 24)."""
+        return self.to_statevector_device(state).cpu().numpy()
 
     def _truncate_into(self, state, src, n, dst):
         """Top-k of src[:n] into dst with renormalisation; returns (nnz, kept, sum4)."""

@@ -197,6 +197,28 @@ int bass_rdm1q_ordered(const uint64_t* x, const double* a, int64_t n, int N, dou
 int bass_rdm2q_ordered(const uint64_t* x, const double* a, int64_t n, int j, double* rho_out, void* ws,
                        size_t ws_bytes, void* stream);
 
+/* ---- dense state vectors (exact simulation, fidelities) ---------------
+ * psi: complex128[2^N] on the device, index = basis key. */
+
+/* psi = 0, then psi[key(x_i)] = a_i for the n entries; key_map (key_bits = N
+ * entries, stored bit -> qubit) maps the stored words of a key-ordered state
+ * back to keys, key_bits = 0 takes x as keys.  The scatter half of
+ * BASS.to_statevector (src/simulation/bass_simulator.py:608-621). */
+int bass_dense_scatter(const uint64_t* x, const double* a, int64_t n, const int32_t* key_map, int key_bits,
+                       double* psi, int N, void* stream);
+
+/* In place: 1- or 2-qubit gate G (row-major complex, index (b_q0 << 1) | b_q1)
+ * on a dense vector.  Replaces ExactSimulator._apply_1qubit_dense /
+ * _apply_2qubit_dense (src/simulation/exact_simulator.py:109-144) and the
+ * per-qubit frame rotation of BASS.to_statevector. */
+int bass_dense_gate(double* psi, int N, int nq, int q0, int q1, const double* G, void* stream);
+
+/* out[0..1] = sum_i conj(u_i) v_i (np.vdot), fixed-order reduction.  For
+ * fidelity_bass (src/experimentation/runner.py:620-626) and compute_fidelity
+ * (src/benchmarking/fidelity.py:48-65).  Synchronous. */
+int bass_dense_vdot(const double* u, const double* v, int64_t len, double* out, void* ws, size_t ws_bytes,
+                    void* stream);
+
 /* ---- batches: many independent key-ordered states per launch ----------
  * Each call runs one launch sequence for all njobs states (per-job scalars via
  * bass_batch_stats).  Job tables are staged through pinned host buffers.

@@ -82,6 +82,17 @@ def test_argument_errors_without_gpu(lib):
     assert rc == L.BASS_E_OVERFLOW
     rc = lib.bass_radix_sort(None, None, 10, 0, None, 0, None)
     assert rc == L.BASS_E_ARG
+    rc = lib.bass_dense_gate(None, 0, 1, 0, 0, G, None)
+    assert rc == L.BASS_E_ARG and "N out of range" in L.last_error()
+    rc = lib.bass_dense_gate(None, 4, 3, 0, 1, G, None)
+    assert rc == L.BASS_E_ARG and "nq" in L.last_error()
+    rc = lib.bass_dense_gate(None, 4, 2, 2, 2, G, None)
+    assert rc == L.BASS_E_ARG and "qubits" in L.last_error()
+    bad3 = (ctypes.c_int32 * 3)(0, 0, 1)
+    rc = lib.bass_dense_scatter(None, None, 4, bad3, 3, None, 3, None)
+    assert rc == L.BASS_E_ARG and "permutation" in L.last_error()
+    rc = lib.bass_dense_scatter(None, None, 9, None, 0, None, 3, None)
+    assert rc == L.BASS_E_ARG and "bad n" in L.last_error()
     rho = (ctypes.c_double * 32)()
     rc = lib.bass_rdm2q_ordered(None, None, 10, 0, rho, None, 0, None)
     assert rc == L.BASS_E_ARG and "bit j" in L.last_error()
