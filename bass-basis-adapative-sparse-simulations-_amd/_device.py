@@ -37,7 +37,10 @@ class Context:
         if need > self.ws_bytes:
             grow = max(need, int(self.ws_bytes * 1.5))
             self.ws = None
-            self.ws = torch.empty(grow, dtype=torch.uint8, device=self.device)
+            # zeroed: the stats block keeps device error bits between calls
+            # (cleared when read), so a new workspace must not start with
+            # leftover bytes there
+            self.ws = torch.zeros(grow, dtype=torch.uint8, device=self.device)
             self.ws_bytes = grow
 
     @property

@@ -201,9 +201,10 @@ class SparseState:
 
     ``x`` / ``alpha`` return host copies (length = capacity) for reference
     call sites; ``x_dev`` / ``alpha_dev`` are the device tensors.  ``dedup``
-    selects how 1-qubit gates merge partner outputs: "table" (group hash
-    table, outputs in first-input order) or "sort" (emit, radix sort, merge:
-    the reference's key-ascending order).  Both give the same entries.
+    selects how gates merge the outputs of a group: "table" (group hash
+    table, outputs in first-input order) or "sort" (emit, stable radix sort,
+    merge: the reference's path and key-ascending order).  Both give the same
+    entries and sums.
     """
 
     RADIX_THRESHOLD = 2000
@@ -307,9 +308,18 @@ class SparseState:
         self._after_gate(method, seed)
 
     def apply_two_qubit_gate(self, gate, method, seed):
-        """sparse_state.py:303-377 (deduplicated through the group table)."""
+        """sparse_state.py:303-377: emit <= 4 outputs per entry, sort by key
+        (stable radix sort), merge equal keys (dedup="sort", the reference's
+        path); dedup="table" deduplicates through the group table instead
+        (same entries and sums, no sort)."""
         q1, q2 = gate.qubits
-        n, ht = _ops.gate(self._ctx, self._buf, self.nnz, gate.matrix, (q1, q2), self._tmp, dedup=True)
+        if self.dedup == "sort":
+            n, _ = _ops.gate(self._ctx, self._buf, self.nnz, gate.matrix, (q1, q2), self._tmp, dedup=False)
+            radix_sort_uint64(self._tmp.x, self._tmp.a, n, key_bits=max(1, self.N))
+            n = merge_sorted_kernel(self._tmp.x, self._tmp.a, n)
+            ht = 0
+        else:
+            n, ht = _ops.gate(self._ctx, self._buf, self.nnz, gate.matrix, (q1, q2), self._tmp, dedup=True)
         self._hist_tiles = ht
         self._swap()
         self._tmp.ensure(4 * max(n, 1))

@@ -191,6 +191,15 @@ def bench_tfim20(args):
         log(f"gpu simulate {dt * 1e3:.1f} ms")
     t_gpu = float(np.median(times))
     amp = sim.counters.get("amp_updates", 0)
+    # fidelity against the exact state, both computed on the device
+    E = importlib.import_module(PKG + ".simulation.exact_simulator")
+    Fm = importlib.import_module(PKG + ".benchmarking.fidelity")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sv = E.exact_statevector_device(20, circ, device="cuda:0")
+    fid = Fm.fidelity_bass(sim, st, sv)
+    t_exact = time.perf_counter() - t0
+    log(f"exact state vector + fidelity on the device {t_exact * 1e3:.1f} ms, F = {fid:.12f}")
     cpu = None
     if not args.no_cpu:
         from oracle import drivers as OD
@@ -212,6 +221,7 @@ def bench_tfim20(args):
         "config": {"workload": "TFIM n=20 open J=h=1 dt=0.05 40 steps (1,560 gates) K=2^14 optimize_every=39 "
                                "(step = one simulate())", "amp_updates": amp,
                    "gamma2": st.gamma ** 2, "nnz": st.nnz},
+        "fidelity": fid, "exact_plus_fidelity_ms": t_exact * 1e3,
         "amp_updates_per_s": amp / t_gpu,
         "roofline": None,
         "cpu_baseline": cpu,

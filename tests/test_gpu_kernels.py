@@ -301,3 +301,37 @@ def test_random_k(gpu_ctx):
     assert np.array_equal(gx[:k], ox[:k])
     assert np.array_equal(ga[:k], oa[:k])
     assert abs(gsq - osq) < 1e-13
+
+
+@pytest.mark.parametrize("dedup", ["sort", "table"])
+@pytest.mark.parametrize("n,q1,q2", [(40, 5, 1), (3000, 2, 9), (20000, 11, 0)])
+def test_sparse_state_2q_paths(gpu_ctx, dedup, n, q1, q2):
+    """SparseState.apply_gate for a 2-qubit gate (sparse_state.py:303-377):
+    emit, stable radix sort, merge (dedup="sort"; the reference's path, key
+    order and sums) or the group table (dedup="table", same entries and sums)
+    against the oracle's SparseState, with and without truncation."""
+    print(f"[case] sparse state 2q dedup={dedup} n={n} q=({q1},{q2})", flush=True)
+    G = pkg_module("core.gates")
+    rng = np.random.default_rng(n + q1)
+    N = 16
+    x, a = random_state(n, N, seed=n + 5)
+    U = _gate_with_zero(rng, 4)
+    for k in (4 * n, n // 2 + 1):
+        st = ss().SparseState(N, k, device="cuda:0", dedup=dedup, capacity=4 * n)
+        st._buf.ensure(n)
+        st._buf.x[:n] = torch.from_numpy(x.view(np.int64)).to("cuda:0")
+        st._buf.a[:n] = torch.from_numpy(a).to("cuda:0")
+        st.nnz = n
+        st.apply_gate(G.QuantumGate([q1, q2], U), "top-k")
+        o = OD.SparseState(N, k, selector=OD.key_tiebreak_selector)
+        o._ensure_temp(4 * n)
+        o.x, o.alpha = np.zeros(4 * n, np.uint64), np.zeros(4 * n, np.complex128)
+        o.x[:n], o.alpha[:n], o.nnz = x, a, n
+        o.apply_gate(OD._G((q1, q2), U), "top-k")
+        gx, ga = st.x[: st.nnz], st.alpha[: st.nnz]
+        assert st.nnz == o.nnz
+        if dedup == "sort" and k >= 4 * n:
+            assert np.array_equal(gx, o.x[: o.nnz]), "sort path: reference key order"
+        assert_same_entries(gx, ga, o.x[: o.nnz], o.alpha[: o.nnz],
+                            exact=(k >= 4 * n and exact()), rtol=1e-13)
+        assert abs(st.gamma - o.gamma) <= 1e-13
