@@ -83,7 +83,7 @@ static Layout layout(int64_t S) {
   L.ksum = o; o += al(stiles * 8);
   L.kcnt = o; o += al(stiles * 4);
   L.koff = o; o += al(stiles * 8);
-  L.lift_b = o; o += al((etiles + 1) * sizeof(LiftBound));
+  L.lift_b = o; o += al((size_t)lift_bounds_entries(etiles + 1) * sizeof(LiftBound));
   L.hist2_rows = o; o += al((size_t)stiles * HIST_BINS * 2);
   L.big = o;
   // big scratch union:

@@ -120,6 +120,7 @@ size_t rdm_batch_scratch(int64_t nbound, int64_t nptiles, int64_t ndtiles);
 int rdm_tile_entries();
 size_t rdm_ordered_scratch(int64_t n, int nbits);
 size_t rdm2_scratch(int64_t n);
+int64_t lift_bounds_entries(int64_t nbound);
 int launch_rdm2(const uint64_t* x, const double2* a, int64_t n, int j, void* scratch, double* out, hipStream_t s);
 int launch_rdm_ordered(const uint64_t* x, const double2* a, int64_t n, int nbits, void* scratch, double* rho,
                        hipStream_t s);

@@ -109,6 +109,105 @@ __global__ __launch_bounds__(NT) void k_lift_part(const uint64_t* __restrict__ w
   if (lane == 0) bounds[u] = r;
 }
 
+// The same boundary when the boundaries D0 (at a merged position <= d) and
+// D1 (>= d) are known: run start and end, A/B split and both merge cursors
+// are monotone in the merged position, so every search is confined between
+// the two (empty when d lies in the run of both).
+__device__ __forceinline__ LiftBound lift_boundary_between(const uint64_t* __restrict__ w, int64_t n, int j, int t,
+                                                           int64_t d, const LiftBound& D0, const LiftBound& D1) {
+  if (d > n) d = n;
+  LiftBound r;
+  if (d >= n) {
+    r.s0 = r.sm = r.s1 = r.a = r.b = (int)n;
+  } else {
+    const uint64_t bj = 1ull << j;
+    const uint64_t hi = w[d] & ~((bj << 1) - 1ull);
+    const int64_t top = d < (int64_t)D1.s0 ? d : (int64_t)D1.s0;
+    const int64_t s0 = wave_partition_point(D0.s0, top, [&](int64_t i) { return w[i] < hi; });
+    const int64_t lo1 = d + 1 > (int64_t)D0.s1 ? d + 1 : (int64_t)D0.s1;
+    const int64_t s1 = wave_partition_point(lo1, D1.s1, [&](int64_t i) { return w[i] < hi + (bj << 1); });
+    const int64_t lom = s0 > (int64_t)D0.sm ? s0 : (int64_t)D0.sm;
+    const int64_t him = s1 < (int64_t)D1.sm ? s1 : (int64_t)D1.sm;
+    const int64_t sm = wave_partition_point(lom, him, [&](int64_t i) { return w[i] < (hi | bj); });
+    const int64_t ld = d - s0, na = sm - s0, nb = s1 - sm;
+    int64_t lo = ld - nb > 0 ? ld - nb : 0;
+    int64_t hh = ld < na ? ld : na;
+    if ((int64_t)D0.a - s0 > lo) lo = (int64_t)D0.a - s0;
+    if ((int64_t)D1.a - s0 < hh) hh = (int64_t)D1.a - s0;
+    const int64_t i = wave_partition_point(
+        lo, hh, [&](int64_t q) { return lift_key(w[s0 + q], j, t) < lift_key(w[sm + ld - 1 - q], j, t); });
+    r.s0 = (int)s0;
+    r.sm = (int)sm;
+    r.s1 = (int)s1;
+    r.a = (int)(s0 + i);
+    r.b = (int)(sm + ld - i);
+  }
+  r.pad0 = r.pad1 = r.pad2 = 0;
+  return r;
+}
+
+// Tile boundaries in two passes: every LIFT_CS-th boundary (and the last) by
+// full searches, then every boundary between its two coarse neighbours.  The
+// full 64-ary searches touch a cache line per probe; confining them cuts the
+// boundary pass's memory traffic about LIFT_CS-fold.  Grid row y = bit j0 + y.
+constexpr int LIFT_CS = 16;
+
+__host__ __device__ inline int64_t lift_coarse_count(int64_t nbound) { return (nbound - 1 + LIFT_CS - 1) / LIFT_CS + 1; }
+
+__global__ __launch_bounds__(NT) void k_lift_part_coarse(const uint64_t* __restrict__ w, int64_t n, int j0, int t,
+                                                          int tm, int64_t nbound, LiftBound* coarse) {
+  const int lane = threadIdx.x & 63;
+  const int j = j0 + (int)blockIdx.y;
+  const int64_t nc = lift_coarse_count(nbound);
+  coarse += (int64_t)blockIdx.y * nc;
+  const int64_t c = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (c >= nc) return;  // uniform per wave
+  int64_t b = c * LIFT_CS;
+  if (b > nbound - 1) b = nbound - 1;
+  const LiftBound r = lift_boundary(w, n, j, t, b * (int64_t)tm);
+  if (lane == 0) coarse[c] = r;
+}
+
+__global__ __launch_bounds__(NT) void k_lift_part_fine(const uint64_t* __restrict__ w, int64_t n, int j0, int t,
+                                                        int tm, int64_t nbound, const LiftBound* __restrict__ coarse,
+                                                        LiftBound* bounds) {
+  const int lane = threadIdx.x & 63;
+  const int j = j0 + (int)blockIdx.y;
+  const int64_t nc = lift_coarse_count(nbound);
+  coarse += (int64_t)blockIdx.y * nc;
+  bounds += (int64_t)blockIdx.y * nbound;
+  const int64_t u = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (u >= nbound) return;  // uniform per wave
+  const int64_t c0 = u / LIFT_CS;
+  const int64_t c1 = c0 + 1 < nc ? c0 + 1 : nc - 1;
+  const int64_t b1 = c1 * LIFT_CS < nbound - 1 ? c1 * LIFT_CS : nbound - 1;
+  LiftBound r;
+  if (u == c0 * LIFT_CS)
+    r = coarse[c0];
+  else if (u == b1)
+    r = coarse[c1];
+  else
+    r = lift_boundary_between(w, n, j, t, u * (int64_t)tm, coarse[c0], coarse[c1]);
+  if (lane == 0) bounds[u] = r;
+}
+
+// Boundaries of rows j0 .. j0 + rows - 1 (nbound per row) into bounds; the
+// coarse pass uses `coarse` (lift_coarse_count(nbound) per row).  Small
+// lifts take the direct pass.
+static void lift_partition(const uint64_t* x, int64_t n, int j0, int rows, int t, int tm, int64_t nbound,
+                           LiftBound* bounds, LiftBound* coarse, hipStream_t s) {
+  if (nbound <= 4 * LIFT_CS || coarse == nullptr) {
+    hipLaunchKernelGGL(k_lift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES), (unsigned)rows), dim3(NT), 0, s,
+                       x, n, j0, t, tm, nbound, bounds);
+    return;
+  }
+  const int64_t nc = lift_coarse_count(nbound);
+  hipLaunchKernelGGL(k_lift_part_coarse, dim3((unsigned)((nc + WAVES - 1) / WAVES), (unsigned)rows), dim3(NT), 0, s,
+                     x, n, j0, t, tm, nbound, coarse);
+  hipLaunchKernelGGL(k_lift_part_fine, dim3((unsigned)((nbound + WAVES - 1) / WAVES), (unsigned)rows), dim3(NT), 0,
+                     s, x, n, j0, t, tm, nbound, (const LiftBound*)coarse, bounds);
+}
+
 __device__ __forceinline__ int lds_lower_bound(const uint64_t* s, int cnt, uint64_t v) {
   int lo = 0, len = cnt;
   while (len > 0) {
@@ -522,7 +621,8 @@ __global__ __launch_bounds__(NTB) void k_rdm2_fin(const double* __restrict__ par
 
 size_t rdm2_scratch(int64_t n) {
   const int64_t tiles = (n + SEG_ENTRIES / 4 - 1) / (SEG_ENTRIES / 4);
-  return (((size_t)(tiles + 1) * sizeof(LiftBound) + 255) & ~(size_t)255) + (size_t)tiles * RDM2_VALS * 8 + 256;
+  return (((size_t)(tiles + 1 + lift_coarse_count(tiles + 1)) * sizeof(LiftBound) + 255) & ~(size_t)255) +
+         (size_t)tiles * RDM2_VALS * 8 + 256;
 }
 
 // out: RDM2_VALS doubles (device).
@@ -531,10 +631,9 @@ int launch_rdm2(const uint64_t* x, const double2* a, int64_t n, int j, void* scr
   const int64_t tiles = (n + TM - 1) / TM, nbound = tiles + 1;
   char* base = (char*)scratch;
   LiftBound* bounds = (LiftBound*)base;
-  base += ((size_t)nbound * sizeof(LiftBound) + 255) & ~(size_t)255;
+  base += ((size_t)(nbound + lift_coarse_count(nbound)) * sizeof(LiftBound) + 255) & ~(size_t)255;
   double* part = (double*)base;
-  hipLaunchKernelGGL(k_lift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES), 1), dim3(NT), 0, s, x, n, j, 1, TM,
-                     nbound, bounds);
+  lift_partition(x, n, j, 1, 1, TM, nbound, bounds, bounds + nbound, s);
   hipLaunchKernelGGL(k_lift_rdm2, dim3((unsigned)tiles), dim3(NT), 0, s, x, a, n, j, bounds, part);
   hipLaunchKernelGGL(k_rdm2_fin, dim3(RDM2_VALS), dim3(NTB), 0, s, part, tiles, out);
   return (int)hipGetLastError();
@@ -893,8 +992,8 @@ int rdm_tile_entries() { return RDM_TM; }
 
 size_t rdm_ordered_scratch(int64_t n, int nbits) {
   const int64_t ptiles = (n + RDM_TM - 1) / RDM_TM, dtiles = (n + SEG_ENTRIES - 1) / SEG_ENTRIES;
-  return (size_t)nbits * (ptiles + 1) * sizeof(LiftBound) + (size_t)nbits * ptiles * 16 + (size_t)dtiles * 64 * 16 +
-         3 * 256;
+  return (size_t)nbits * (ptiles + 1 + lift_coarse_count(ptiles + 1)) * sizeof(LiftBound) +
+         (size_t)nbits * ptiles * 16 + (size_t)dtiles * 64 * 16 + 3 * 256;
 }
 
 int launch_rdm_ordered(const uint64_t* x, const double2* a, int64_t n, int nbits, void* scratch, double* rho,
@@ -903,12 +1002,12 @@ int launch_rdm_ordered(const uint64_t* x, const double2* a, int64_t n, int nbits
   const int64_t nbound = ptiles + 1;
   char* base = (char*)scratch;
   LiftBound* bounds = (LiftBound*)base;
-  base += ((size_t)nbits * nbound * sizeof(LiftBound) + 255) & ~(size_t)255;
+  LiftBound* coarse = bounds + (size_t)nbits * nbound;
+  base += ((size_t)nbits * (nbound + lift_coarse_count(nbound)) * sizeof(LiftBound) + 255) & ~(size_t)255;
   double2* part = (double2*)base;
   base += ((size_t)nbits * ptiles * 16 + 255) & ~(size_t)255;
   double2* dpart = (double2*)base;
-  hipLaunchKernelGGL(k_lift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES), (unsigned)nbits), dim3(NT), 0, s,
-                     x, n, 0, 0, RDM_TM, nbound, bounds);
+  lift_partition(x, n, 0, nbits, 0, RDM_TM, nbound, bounds, coarse, s);
   hipLaunchKernelGGL(k_rdm_pairs, dim3((unsigned)ptiles, (unsigned)nbits), dim3(NT), 0, s, x, a, n, nbound, bounds,
                      part);
   hipLaunchKernelGGL(k_rdm_diag, dim3((unsigned)dtiles), dim3(NT), 0, s, x, a, n, nbits, dpart);
@@ -918,6 +1017,9 @@ int launch_rdm_ordered(const uint64_t* x, const double2* a, int64_t n, int nbits
 }
 
 // ----------------------------------------------------------- host launch ---
+
+// LiftBound entries a lift of n entries needs (boundaries + coarse boundaries).
+int64_t lift_bounds_entries(int64_t nbound) { return nbound + lift_coarse_count(nbound); }
 
 int lift_tile_entries(int qb, int mode) { return mode == MODE_MOVE ? 1024 : SEG_ENTRIES / (1 << qb); }
 
@@ -961,8 +1063,8 @@ int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n
   const int t = (mode == MODE_MOVE) ? 0 : qb - 1;
   const int64_t tiles = (n + tm - 1) / tm;
   const int64_t nbound = tiles + 1;
-  hipLaunchKernelGGL(k_lift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES)), dim3(NT), 0, s, x, n, j, t, tm,
-                     nbound, bounds);
+  // the coarse boundaries follow the nbound boundaries (lift_bounds_entries)
+  lift_partition(x, n, j, 1, t, tm, nbound, bounds, bounds + nbound, s);
   const dim3 grid((unsigned)tiles);
   if (mode == MODE_MOVE) {
     hipLaunchKernelGGL(k_lift_move<1024>, grid, dim3(NT), 0, s, x, a, n, j, bounds, xo, ao);

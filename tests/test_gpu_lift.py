@@ -70,6 +70,7 @@ def _run_lift(ctx, w, a, j, G, nq, mode):
 
 
 @pytest.mark.parametrize("n,nbits,mode", [(1, 5, 0), (2, 3, 0), (777, 12, 0), (30000, 18, 0), (70000, 24, 0),
+                                          (300000, 22, 0), (300000, 40, 1),
                                           (5000, 14, 1), (40000, 20, 1)])
 def test_lift_1q(gpu_ctx, n, nbits, mode):
     _, _, SS, L = _mods()
@@ -96,7 +97,8 @@ def test_lift_1q(gpu_ctx, n, nbits, mode):
         assert_same_entries(gx, ga, ox, oa, exact=numpy_uses_fma_cmul())
 
 
-@pytest.mark.parametrize("n,nbits,mode", [(3, 3, 0), (5000, 13, 0), (40000, 18, 0), (6000, 14, 1)])
+@pytest.mark.parametrize("n,nbits,mode", [(3, 3, 0), (5000, 13, 0), (40000, 18, 0), (6000, 14, 1),
+                                          (200000, 20, 0), (200000, 36, 1)])
 def test_lift_2q(gpu_ctx, n, nbits, mode):
     ops, dev, SS, L = _mods()
     rng = np.random.default_rng(n + 7 * mode)
@@ -130,7 +132,7 @@ def test_lift_2q(gpu_ctx, n, nbits, mode):
     assert_same_entries(gx, ga, ox[:m], oa[:m], rtol=1e-13, exact=(mode == 1 and numpy_uses_fma_cmul()))
 
 
-@pytest.mark.parametrize("n,nbits", [(1, 4), (5, 3), (3000, 13), (50000, 30)])
+@pytest.mark.parametrize("n,nbits", [(1, 4), (5, 3), (3000, 13), (50000, 30), (400000, 21)])
 def test_lift_move(gpu_ctx, n, nbits):
     ops, dev, SS, L = _mods()
     rng = np.random.default_rng(n)
@@ -205,7 +207,7 @@ def test_truncate_segmented_keeps_all_up_to_k(gpu_ctx, n):
             assert abs(np.sum(np.abs(dev.to_host_amps(dst.a, k)) ** 2) - 1.0) < 1e-12
 
 
-@pytest.mark.parametrize("n,nbits", [(1, 5), (7, 4), (5000, 14), (70000, 40)])
+@pytest.mark.parametrize("n,nbits", [(1, 5), (7, 4), (5000, 14), (70000, 40), (300000, 28)])
 def test_rdm_ordered(gpu_ctx, n, nbits):
     """RDMs of every stored bit of an ordered state vs the oracle's (hash) RDMs."""
     ops, dev, SS, L = _mods()
@@ -261,7 +263,7 @@ def test_truncate_massive_ties(gpu_ctx, n, k, ordered):
     assert_same_entries(gx, dev.to_host_amps(dst.a, k), ox[:k], oa[:k], exact=True)
 
 
-@pytest.mark.parametrize("n,nbits", [(1, 3), (7, 3), (5000, 13), (60000, 20)])
+@pytest.mark.parametrize("n,nbits", [(1, 3), (7, 3), (5000, 13), (60000, 20), (300000, 22)])
 def test_rdm2q_ordered(gpu_ctx, n, nbits):
     """4x4 RDM of a qubit pair of a key-ordered state (second qubit moved to
     bit 0, first at bit j) against compute_2qubit_rdm, both index orders."""
