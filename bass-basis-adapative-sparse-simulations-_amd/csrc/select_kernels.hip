@@ -375,10 +375,13 @@ __global__ __launch_bounds__(NT) void k_find2(const SelProblem* probs, SelProble
 }
 
 // Candidates of the digit-2 bucket, appended to the problem's list in any
-// order (the threshold does not depend on their order).  One tile per block.
+// order (the threshold does not depend on their order).  One wave per tile (a
+// tile holds a few hundred candidates; whole blocks would mostly wait).
 template <bool BATCH>
-__global__ __launch_bounds__(NT) void k_extract2(const SelProblem* probs, int nprob, SelProblem one, SelTiles T) {
-  const int64_t gt = blockIdx.x;
+__global__ __launch_bounds__(NT) void k_extract2(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
+                                                 int64_t tiles_end) {
+  const int64_t gt = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (gt >= tiles_end) return;  // uniform per wave
   const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt);
   SelState* sel = P.sel;
   if (sel->keepall) return;
@@ -386,8 +389,8 @@ __global__ __launch_bounds__(NT) void k_extract2(const SelProblem* probs, int np
   const int lane = threadIdx.x & 63;
   const int cnt = T.cnt1[gt];
   const int64_t base = gt * SEL_TILE;
-  for (int j0 = 0; j0 < cnt; j0 += NT) {
-    const int j = j0 + threadIdx.x;
+  for (int j0 = 0; j0 < cnt; j0 += 64) {
+    const int j = j0 + lane;
     bool take = false;
     unsigned long long bits = 0;
     if (j < cnt) {
@@ -627,21 +630,21 @@ __device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t w, u
   return bits > pstar || (bits == pstar && logical_key(w, km) <= keystar);
 }
 
-// Per tile: sum and count of the kept candidates (in tile order).
+// Per tile: sum and count of the kept candidates (lane-strided, then a
+// fixed shuffle tree).  One wave per tile.
 template <bool BATCH>
-__global__ __launch_bounds__(NT) void k_keptsum(const SelProblem* probs, int nprob, SelProblem one, SelTiles T) {
-  __shared__ double s_red[WAVES];
-  __shared__ int s_cnt;
-  const int64_t gt = blockIdx.x;
+__global__ __launch_bounds__(NT) void k_keptsum(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
+                                                int64_t tiles_end) {
+  const int64_t gt = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  if (gt >= tiles_end) return;  // uniform per wave
   const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt);
   const unsigned long long pstar = P.sel->pstar, keystar = P.sel->keystar;
+  const int lane = threadIdx.x & 63;
   const int cnt = T.cnt1[gt];
   const int64_t base = gt * SEL_TILE;
-  if (threadIdx.x == 0) s_cnt = 0;
-  __syncthreads();
   double s = 0.0;
-  int c = 0;
-  for (int j = threadIdx.x; j < cnt; j += NT) {
+  unsigned long long c = 0;
+  for (int j = lane; j < cnt; j += 64) {
     const unsigned long long bits = T.c1p[base + j];
     bool keep = bits > pstar;
     if (bits == pstar) keep = logical_key(P.x[T.c1i[base + j]], P.km) <= keystar;
@@ -650,12 +653,11 @@ __global__ __launch_bounds__(NT) void k_keptsum(const SelProblem* probs, int npr
       ++c;
     }
   }
-  const double tot = block_sum(s, s_red);
-  atomicAdd(&s_cnt, c);
-  __syncthreads();
-  if (threadIdx.x == 0) {
+  const double tot = wave_sum(s);
+  const unsigned long long cc = wave_sum_u64(c);
+  if (lane == 0) {
     T.ksum[gt] = tot;
-    T.kcnt[gt] = s_cnt;
+    T.kcnt[gt] = (int)cc;
   }
 }
 
@@ -896,7 +898,8 @@ static void select_sequence(const SelProblem* probs, int nprob, const SelProblem
                      dim3(HIST_BINS / (4 * NT), (unsigned)((max_rows + H2_ROWS - 1) / H2_ROWS), (unsigned)nprob),
                      dim3(NT), 0, s, probs, one, T);
   hipLaunchKernelGGL((k_find2<BATCH>), dim3((unsigned)nprob), dim3(NT), 0, s, probs, one);
-  hipLaunchKernelGGL((k_extract2<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
+  const unsigned wave_tiles = (unsigned)((tiles_end + WAVES - 1) / WAVES);
+  hipLaunchKernelGGL((k_extract2<BATCH>), dim3(wave_tiles), dim3(NT), 0, s, probs, nprob, one, T, tiles_end);
   if (!wide) {
     hipLaunchKernelGGL((k_finish<BATCH>), dim3((unsigned)nprob), dim3(NT), 0, s, probs, one);
   } else {
@@ -911,7 +914,7 @@ static void select_sequence(const SelProblem* probs, int nprob, const SelProblem
       hipLaunchKernelGGL((k_wide_find<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, lv);
     }
   }
-  hipLaunchKernelGGL((k_keptsum<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
+  hipLaunchKernelGGL((k_keptsum<BATCH>), dim3(wave_tiles), dim3(NT), 0, s, probs, nprob, one, T, tiles_end);
   hipLaunchKernelGGL((k_sel_finalize<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
   hipLaunchKernelGGL((k_compact<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
   hipLaunchKernelGGL((k_sums_finalize<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
