@@ -413,13 +413,18 @@ __global__ __launch_bounds__(NT) void k_extract2(const SelProblem* probs, int np
 }
 
 // Resolve the remaining bits of the threshold from the (small) digit-2
-// candidate list, then the key tie-break among entries equal to it.
+// candidate list, then the key tie-break among entries equal to it.  One block
+// of NTB threads per problem; the logical keys of the tied entries are
+// gathered once into LDS when they fit (FIN_TIES), else re-read per level.
+constexpr int FIN_TIES = 2048;
 template <bool BATCH>
-__global__ __launch_bounds__(NT) void k_finish(const SelProblem* probs, SelProblem one) {
+__global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProblem one) {
   __shared__ unsigned int s_hist[HIST_BINS];
-  __shared__ int s_scan[WAVES];
+  __shared__ int s_scan[WAVESB];
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_ab;
+  __shared__ uint64_t s_keys[FIN_TIES];
+  __shared__ unsigned int s_nk;
   const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
   SelState* sel = P.sel;
   OpStats* st = P.st;
@@ -443,20 +448,20 @@ __global__ __launch_bounds__(NT) void k_finish(const SelProblem* probs, SelProbl
   unsigned int cnt_eq = 0;
   for (int lv = 0; lv < 4; ++lv) {
     const int sh = shifts[lv], wd = widths[lv];
-    for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+    for (int b = threadIdx.x; b < HIST_BINS; b += NTB) s_hist[b] = 0;
     if (threadIdx.x == 0) {
       s_b = 0xFFFFFFFFu;
       s_ab = 0;
     }
     __syncthreads();
     const unsigned long long hi = prefix >> (sh + wd);
-    for (unsigned int j = threadIdx.x; j < m2; j += NT) {
+    for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
       const unsigned long long bits = c2p[j];
       if ((bits >> (sh + wd)) == hi)
         atomicAdd(&s_hist[(unsigned int)(bits >> sh) & ((1u << wd) - 1u)], 1u);
     }
     __syncthreads();
-    find_from_top(s_hist, need, &s_b, &s_ab, s_scan);
+    find_from_top<NTB>(s_hist, need, &s_b, &s_ab, s_scan);
     if (s_b == 0xFFFFFFFFu) {
       if (threadIdx.x == 0) set_err(&st->err, ERR_SELECT);
       return;
@@ -470,22 +475,42 @@ __global__ __launch_bounds__(NT) void k_finish(const SelProblem* probs, SelProbl
   unsigned long long keystar = ~0ull;
   if (need < cnt_eq) {
     // need-th smallest key among entries whose pattern equals pstar
+    const bool in_lds = cnt_eq <= (unsigned int)FIN_TIES;
+    if (in_lds) {
+      if (threadIdx.x == 0) s_nk = 0;
+      __syncthreads();
+      for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
+        if (c2p[j] != pstar) continue;
+        const unsigned int slot = atomicAdd(&s_nk, 1u);
+        if (slot < (unsigned int)FIN_TIES) s_keys[slot] = logical_key(P.x[c2i[j]], P.km);
+      }
+      __syncthreads();
+    }
+    const unsigned int nk = in_lds ? s_nk : 0u;
     unsigned long long kp = 0;
     for (int sh = 56; sh >= 0; sh -= 8) {
-      for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+      for (int b = threadIdx.x; b < HIST_BINS; b += NTB) s_hist[b] = 0;
       if (threadIdx.x == 0) {
         s_b = 0xFFFFFFFFu;
         s_ab = 0;
       }
       __syncthreads();
-      for (unsigned int j = threadIdx.x; j < m2; j += NT) {
-        if (c2p[j] != pstar) continue;
-        const uint64_t key = logical_key(P.x[c2i[j]], P.km);
-        const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
-        if (match) atomicAdd(&s_hist[(unsigned int)(key >> sh) & 255u], 1u);
+      if (in_lds) {
+        for (unsigned int e = threadIdx.x; e < nk; e += NTB) {
+          const uint64_t key = s_keys[e];
+          const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
+          if (match) atomicAdd(&s_hist[(unsigned int)(key >> sh) & 255u], 1u);
+        }
+      } else {
+        for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
+          if (c2p[j] != pstar) continue;
+          const uint64_t key = logical_key(P.x[c2i[j]], P.km);
+          const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
+          if (match) atomicAdd(&s_hist[(unsigned int)(key >> sh) & 255u], 1u);
+        }
       }
       __syncthreads();
-      find_from_bottom(s_hist, need, &s_b, &s_ab, s_scan);
+      find_from_bottom<NTB>(s_hist, need, &s_b, &s_ab, s_scan);
       if (s_b == 0xFFFFFFFFu) {
         if (threadIdx.x == 0) set_err(&st->err, ERR_SELECT);
         return;
@@ -901,7 +926,7 @@ static void select_sequence(const SelProblem* probs, int nprob, const SelProblem
   const unsigned wave_tiles = (unsigned)((tiles_end + WAVES - 1) / WAVES);
   hipLaunchKernelGGL((k_extract2<BATCH>), dim3(wave_tiles), dim3(NT), 0, s, probs, nprob, one, T, tiles_end);
   if (!wide) {
-    hipLaunchKernelGGL((k_finish<BATCH>), dim3((unsigned)nprob), dim3(NT), 0, s, probs, one);
+    hipLaunchKernelGGL((k_finish<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one);
   } else {
     const dim3 wg(WIDE_BLOCKS, (unsigned)nprob);
     for (int lv = 0; lv < 4; ++lv) {
