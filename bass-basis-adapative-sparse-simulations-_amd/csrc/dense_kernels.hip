@@ -87,13 +87,24 @@ __global__ __launch_bounds__(NT) void k_dense_vdot(const double2* __restrict__ u
   __shared__ double s_re[WAVES], s_im[WAVES];
   const int64_t b0 = (int64_t)blockIdx.x * DOT_CHUNK;
   double re = 0.0, im = 0.0;
-#pragma unroll 4
-  for (int k = 0; k < DOT_CHUNK / NT; ++k) {
-    const int64_t i = b0 + (int64_t)k * NT + threadIdx.x;
-    if (i < len) {
-      const double2 z = cmul(cconj(u[i]), v[i]);
-      re += z.x;
-      im += z.y;
+  constexpr int G = 4;  // loads of G items issued together, clamped to valid indices
+  for (int k0 = 0; k0 < DOT_CHUNK / NT; k0 += G) {
+    double2 uu[G], vv[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int64_t i = b0 + (int64_t)(k0 + g) * NT + threadIdx.x;
+      const int64_t c = i < len ? i : 0;
+      uu[g] = u[c];
+      vv[g] = v[c];
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int64_t i = b0 + (int64_t)(k0 + g) * NT + threadIdx.x;
+      if (i < len) {
+        const double2 z = cmul(cconj(uu[g]), vv[g]);
+        re += z.x;
+        im += z.y;
+      }
     }
   }
   const double tr = block_sum(re, s_re);

@@ -181,11 +181,20 @@ __global__ __launch_bounds__(NT) void k_hist_psum(const SelProblem* probs, int n
   const int cnt = tile_count(P, gt);
   const int64_t base = tile_base(P, gt);
   double ps = 0.0;
+  // all loads first (clamped to a valid slot), so they are in flight together
+  double vr[SEL_ITEMS], vi[SEL_ITEMS];
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int j = it * NT + threadIdx.x;
+    const double2 v = P.a[base + (j < cnt ? j : 0)];
+    vr[it] = v.x;
+    vi[it] = v.y;
+  }
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
     const int j = it * NT + threadIdx.x;
     if (j < cnt) {
-      const double p = cabs2(P.a[base + j]);
+      const double p = cabs2(make_double2(vr[it], vi[it]));
       ps += p;
       atomicAdd(&s_hist[(unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1)], 1u);
     }
@@ -279,13 +288,22 @@ __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int np
     double dsum = 0.0;
     int dcnt = 0;
     int ci[SEL_ITEMS], coff[SEL_ITEMS];
+    // all loads first (clamped to a valid slot), so they are in flight together
+    double vr[SEL_ITEMS], vi[SEL_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SEL_ITEMS; ++it) {
+      const int j = it * NT + threadIdx.x;
+      const double2 v = P.a[base + (j < cnt ? j : 0)];
+      vr[it] = v.x;
+      vi[it] = v.y;
+    }
 #pragma unroll
     for (int it = 0; it < SEL_ITEMS; ++it) {
       const int j = it * NT + threadIdx.x;
       isc[it] = false;
       cb[it] = 0;
       if (j < cnt) {
-        const double p = cabs2(P.a[base + j]);
+        const double p = cabs2(make_double2(vr[it], vi[it]));
         const unsigned long long bits = (unsigned long long)__double_as_longlong(p);
         const unsigned int d = (unsigned int)(bits >> 52);
         if (d > b1) {
@@ -866,10 +884,20 @@ __global__ __launch_bounds__(NT) void k_moments(const double2* __restrict__ a, i
                                                  double* tile_s4) {
   __shared__ double s_red[WAVES];
   double s2 = 0.0, s4 = 0.0;
+  // all loads first (clamped to a valid index), so they are in flight together
+  double vr[SEL_ITEMS], vi[SEL_ITEMS];
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int64_t i = (int64_t)blockIdx.x * SEL_TILE + (int64_t)it * NT + threadIdx.x;
+    const double2 v = a[i < n ? i : 0];
+    vr[it] = v.x;
+    vi[it] = v.y;
+  }
+#pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
     const int64_t i = (int64_t)blockIdx.x * SEL_TILE + (int64_t)it * NT + threadIdx.x;
     if (i < n) {
-      const double p = cabs2(a[i]);
+      const double p = cabs2(make_double2(vr[it], vi[it]));
       s2 += p;
       s4 += p * p;
     }
