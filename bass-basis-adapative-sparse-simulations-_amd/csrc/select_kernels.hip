@@ -407,24 +407,30 @@ __global__ __launch_bounds__(NT) void k_extract2(const SelProblem* probs, int np
   const int lane = threadIdx.x & 63;
   const int cnt = T.cnt1[gt];
   const int64_t base = gt * SEL_TILE;
-  for (int j0 = 0; j0 < cnt; j0 += 64) {
-    const int j = j0 + lane;
-    bool take = false;
-    unsigned long long bits = 0;
-    if (j < cnt) {
-      bits = T.c1p[base + j];
-      take = ((unsigned int)(bits >> 41) & (HIST_BINS - 1)) == b2;
+  constexpr int U = 8;  // candidate loads in flight per lane (clamped to a valid slot)
+  for (int j0 = 0; j0 < cnt; j0 += 64 * U) {
+    unsigned long long bv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u * 64 + lane;
+      bv[u] = T.c1p[base + (j < cnt ? j : 0)];
     }
-    const unsigned long long bal = __ballot(take);
-    if (bal) {
-      unsigned int wb = 0;
-      const int leader = __ffsll((long long)bal) - 1;
-      if (lane == leader) wb = atomicAdd(&sel->cand2, (unsigned int)__popcll(bal));
-      wb = __shfl(wb, leader, 64);
-      if (take) {
-        const unsigned int r = (unsigned int)__popcll(bal & ((1ull << lane) - 1ull));
-        P.c2p[wb + r] = bits;
-        P.c2i[wb + r] = T.c1i[base + j];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u * 64 + lane;
+      const unsigned long long bits = bv[u];
+      const bool take = j < cnt && ((unsigned int)(bits >> 41) & (HIST_BINS - 1)) == b2;
+      const unsigned long long bal = __ballot(take);
+      if (bal) {
+        unsigned int wb = 0;
+        const int leader = __ffsll((long long)bal) - 1;
+        if (lane == leader) wb = atomicAdd(&sel->cand2, (unsigned int)__popcll(bal));
+        wb = __shfl(wb, leader, 64);
+        if (take) {
+          const unsigned int r = (unsigned int)__popcll(bal & ((1ull << lane) - 1ull));
+          P.c2p[wb + r] = bits;
+          P.c2i[wb + r] = T.c1i[base + j];
+        }
       }
     }
   }
@@ -687,13 +693,25 @@ __global__ __launch_bounds__(NT) void k_keptsum(const SelProblem* probs, int npr
   const int64_t base = gt * SEL_TILE;
   double s = 0.0;
   unsigned long long c = 0;
-  for (int j = lane; j < cnt; j += 64) {
-    const unsigned long long bits = T.c1p[base + j];
-    bool keep = bits > pstar;
-    if (bits == pstar) keep = logical_key(P.x[T.c1i[base + j]], P.km) <= keystar;
-    if (keep) {
-      s += __longlong_as_double((long long)bits);
-      ++c;
+  constexpr int U = 8;  // candidate loads in flight per lane (clamped to a valid slot)
+  for (int j0 = 0; j0 < cnt; j0 += 64 * U) {
+    unsigned long long bv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u * 64 + lane;
+      bv[u] = T.c1p[base + (j < cnt ? j : 0)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = j0 + u * 64 + lane;
+      if (j >= cnt) continue;
+      const unsigned long long bits = bv[u];
+      bool keep = bits > pstar;
+      if (bits == pstar) keep = logical_key(P.x[T.c1i[base + j]], P.km) <= keystar;
+      if (keep) {
+        s += __longlong_as_double((long long)bits);
+        ++c;
+      }
     }
   }
   const double tot = wave_sum(s);
