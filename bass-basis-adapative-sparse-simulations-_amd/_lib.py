@@ -78,6 +78,8 @@ SIGNATURES = {
     "bass_ht_insert": (_I, [_P, _P, _I, _P, _I64, _I64, _P, _SZ, _P]),
     "bass_ht_lookup": (_I, [_P, _P, _I, _P, _I64, _P, _P]),
     "bass_radix_sort": (_I, [_P, _P, _I64, _I, _P, _SZ, _P]),
+    "bass_sort_digit_bits": (_I, [_I]),
+    "bass_sort_plan": (_I, [_I, _P]),
     "bass_merge_sorted": (_I, [_P, _P, _I64, _P, _P, _P, _SZ, _P]),
     "bass_dense_scatter": (_I, [_P, _P, _I64, _P, _I, _P, _I, _P]),
     "bass_dense_gate": (_I, [_P, _I, _I, _I, _I, _P, _P]),

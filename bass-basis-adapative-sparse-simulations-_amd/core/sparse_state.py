@@ -174,8 +174,12 @@ def radix_sort_uint64(keys, values, n, key_bits=None):
     if n <= 1:
         return
     if key_bits is None:
-        kmax = int(np.int64(buf.x[:n].max().item()))
-        key_bits = 64 if kmax < 0 else max(1, kmax.bit_length())
+        # keys live in an int64 tensor: any key with bit 63 set reads negative
+        xs_ = buf.x[:n]
+        if int(xs_.min().item()) < 0:
+            key_bits = 64
+        else:
+            key_bits = max(1, int(xs_.max().item()).bit_length())
     ctx.ensure_ws(n)
     rc = ctx.lib.bass_radix_sort(buf.xp, buf.ap, n, int(key_bits), ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_radix_sort")

@@ -71,11 +71,11 @@ static Layout layout(int64_t S) {
   L.tile_s4 = o; o += al(stiles * 8);
   L.rdm_part = o; o += al((size_t)1024 * 64 * 4 * 8);
   L.rho = o; o += al(64 * 4 * 8);
-  L.sort_hist = o; o += al(8 * 256 * 4);
-  L.sort_tick = o; o += al(16 * 4);
+  L.sort_hist = o; o += al(SORT_MAX_PASSES * 1024 * 4);
+  L.sort_tick = o; o += al(64 * 4);
+  L.sort_status = o; o += al(sort_status_bytes(S));
   L.lb_gate = o; o += al(etiles * 8);
   L.lb1 = o; o += al(stiles * 8);
-  L.sort_status = o; o += al((size_t)stiles * 256 * 4);
   L.seg_cnt = o; o += al(etiles * 4);
   L.soff = o; o += al(etiles * 8);
   L.cnt1 = o; o += al(stiles * 4);
@@ -940,6 +940,7 @@ int bass_ht_lookup(const uint64_t* ht_keys, const int32_t* ht_vals, int log2cap,
 int bass_radix_sort(uint64_t* keys, double* vals, int64_t n, int key_bits, void* ws, size_t ws_bytes,
                     void* stream) {
   if (key_bits < 1 || key_bits > 64) return fail(BASS_E_ARG, "bass_radix_sort: key_bits out of range");
+  if (n > SORT_MAX_N) return fail(BASS_E_ARG, "bass_radix_sort: n = %lld exceeds 2^30 - 1", (long long)n);
   if (n <= 1) return BASS_OK;
   WS_REQUIRE(n);
   hipStream_t s = (hipStream_t)stream;
@@ -948,10 +949,23 @@ int bass_radix_sort(uint64_t* keys, double* vals, int64_t n, int key_bits, void*
   uint64_t* tk = (uint64_t*)big;
   double2* tv = (double2*)(big + al((size_t)8 * n));
   const int tm = ktimer_begin("sort", s);
+  // histograms, tickets and the first pass's look-back words start at zero
+  // (contiguous in the layout)
+  fill(at<void>(ws, L.sort_hist), (L.sort_status - L.sort_hist) + (size_t)sweep_tiles(n) * 1024 * 4, 0ull, s);
   launch_radix_sort(keys, (double2*)vals, n, key_bits, tk, tv, at<unsigned int>(ws, L.sort_hist),
                     at<unsigned int>(ws, L.sort_status), at<unsigned int>(ws, L.sort_tick), &st->err, s);
   ktimer_end(tm, s);
   return hip_check("bass_radix_sort");
+}
+
+int bass_sort_digit_bits(int bits) {
+  if (bits != 0 && bits != 8 && bits != 10) return fail(BASS_E_ARG, "bass_sort_digit_bits: %d is not 0, 8 or 10", bits);
+  return sort_digit_bits_set(bits);
+}
+
+int bass_sort_plan(int key_bits, int* passes) {
+  if (key_bits < 1 || key_bits > 64 || !passes) return fail(BASS_E_ARG, "bass_sort_plan: key_bits out of range");
+  return sort_plan(key_bits, passes);
 }
 
 int bass_merge_sorted(const uint64_t* x, const double* a, int64_t n, uint64_t* xo, double* ao, void* ws,

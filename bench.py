@@ -226,6 +226,107 @@ def bench_tfim20(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_sort(args):
+    """bass_radix_sort on the records a 1-qubit gate emits from a saturated
+    state: K random N-bit keys k, each emitted as (k & ~m, k | m) with m the
+    middle qubit (2K records, the reference's dedup input,
+    src/core/sparse_state.py:268-285), complex normal amplitudes.  Every step
+    restores the unsorted input (a device copy outside the timed region) and
+    sorts it; time = HIP events around the sort.  Algorithmic bytes: one key
+    read for the histograms (8 B) + 48 B per digit pass + 48 B for the copy
+    back after an odd pass count."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    L = importlib.import_module(PKG + "._lib")
+    dev = importlib.import_module(PKG + "._device")
+    lib = L.load()
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    N = args.qubits or 40
+    K = 1 << (args.log2k or 22)
+    if N > 62:
+        raise SystemExit("--workload sort: N <= 62")
+    n = 2 * K
+    ctx = dev.context("cuda:0")
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(SEED)
+    k = torch.randint(0, 1 << N, (K,), device="cuda:0", dtype=torch.int64, generator=g)
+    m = 1 << (N // 2)
+    x0 = torch.stack((k & ~m, k | m), 1).reshape(-1).contiguous()
+    del k
+    a0 = torch.randn(n, dtype=torch.complex128, device="cuda:0", generator=g)
+    x, a = x0.clone(), a0.clone()
+    ctx.ensure_ws(n)
+    if args.sort_digits:
+        L.check(lib.bass_sort_digit_bits(args.sort_digits), "bass_sort_digit_bits")
+    passes = ctypes.c_int(0)
+    rb = lib.bass_sort_plan(N, ctypes.byref(passes))
+    P = passes.value
+    byt = 8 * n + 48 * n * (P + (P & 1))
+
+    def one():
+        x.copy_(x0)
+        a.copy_(a0)
+        rc_ = lib.bass_radix_sort(x.data_ptr(), a.data_ptr(), n, N, ctx.wsp, ctx.ws_bytes, ctx.stream)
+        L.check(rc_, "bass_radix_sort")
+
+    for _ in range(args.warmup):
+        one()
+    torch.cuda.synchronize()
+    L.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cnt, ms = L.timing_read("sort")
+    L.timing(False)
+    ctx.read_stats("sort")
+    avg = ms / cnt
+    # the last sort against torch's stable sort
+    ref_x, idx = torch.sort(x0, stable=True)
+    ok = bool(torch.equal(ref_x, x) and torch.equal(a0[idx], a))
+    log(f"sort n={n} bits={N} digits={rb}x{P}: {avg:.4f} ms per sort, matches torch stable sort: {ok}")
+    if not ok:
+        raise SystemExit("bass_radix_sort disagrees with torch.sort(stable=True)")
+    gbs = byt / (avg * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "sort", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic("sort", N, K, "sort"),
+                "bytes_per_launch": byt, "avg_ms": avg}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import port
+
+        cn = min(n, 1 << 22)
+        hx = x0[:cn].cpu().numpy().view(np.uint64).copy()
+        ha = a0[:cn].cpu().numpy().copy()
+        port.load()
+        reps, t_cpu = 0, 0.0
+        while t_cpu < args.cpu_seconds and reps < 64:
+            cx, ca = hx.copy(), ha.copy()
+            c0 = time.perf_counter()
+            port.radix_sort(cx, ca, cn)
+            t_cpu += time.perf_counter() - c0
+            reps += 1
+        cpu = {"value": cn * reps / t_cpu, "unit": "records/s", "cores": 1, "kind": "port",
+               "sample": f"{reps} sorts of the first {cn} records (reference radix_sort_uint64: 4 passes of "
+                         f"16 bits), oracle C++ -O3 single-thread restatement, {t_cpu:.1f} s, "
+                         f"host affinity {len(os.sched_getaffinity(0))} cpus"}
+    out = {
+        "metric": "radix sort records/s", "value": n / (avg * 1e-3), "unit": "records/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": avg, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64 keys + complex128 payload", "data": "synthetic",
+        "config": {"workload": f"radix sort of 2K={n} emitted records, {N}-bit keys (step = one sort; "
+                               f"input restore outside the timed region)",
+                   "digit_bits": rb, "passes": P, "wall_s_including_restores": wall},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_served(args):
     """Config 4: Poisson stream (--rate) of 50/50 TFIM and brickwork circuits,
     HBM-budget admission (200 GB), p50/p99 latency.  --server engine: one
@@ -278,7 +379,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=25, help="timed layers")
     ap.add_argument("--warmup", type=int, default=5, help="untimed layers")
-    ap.add_argument("--workload", choices=["brickwork", "kicked-ising", "batched", "served", "tfim20"], default="brickwork")
+    ap.add_argument("--workload", choices=["brickwork", "kicked-ising", "batched", "served", "tfim20", "sort"],
+                    default="brickwork")
     ap.add_argument("--circuits", type=int, default=512, help="batched / served: circuits in the stream")
     ap.add_argument("--single-sample", type=int, default=24,
                     help="batched / served: circuits timed one at a time for the baseline")
@@ -291,6 +393,8 @@ def main():
     ap.add_argument("--log2k", type=int, default=None)
     ap.add_argument("--calibrate-copy", type=float, default=None, metavar="MIB")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--sort-digits", type=int, default=0, choices=[0, 8, 10],
+                    help="sort: force the digit width (0 = automatic)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -316,6 +420,8 @@ def main():
         return bench_batched(args)
     if args.workload == "tfim20":
         return bench_tfim20(args)
+    if args.workload == "sort":
+        return bench_sort(args)
     if args.workload == "served":
         if args.workers is None:
             args.workers = 4 if args.server == "engine" else 8

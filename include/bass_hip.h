@@ -261,9 +261,23 @@ int bass_ht_lookup(const uint64_t* ht_keys, const int32_t* ht_vals, int log2cap,
 
 /* Stable LSD radix sort of keys[0:n) (low key_bits bits significant, the rest
  * must be zero) carrying complex128 payload.  Replaces radix_sort_uint64
- * (src/core/sparse_state.py:150-172).  Uses ws as the ping-pong buffer. */
+ * (src/core/sparse_state.py:150-172), which takes 4 passes of 16 bits; here
+ * one histogram pass plus one pass per 8-bit digit (plus a copy back after an
+ * odd pass count).  Uses ws as the ping-pong buffer.  n < 2^30 (BASS_E_ARG
+ * otherwise). */
 int bass_radix_sort(uint64_t* keys, double* vals, int64_t n, int key_bits, void* ws,
                     size_t ws_bytes, void* stream);
+
+/* Digit width of bass_radix_sort: 10 selects 10-bit digits (fewer passes,
+ * slower ones on MI355X), 8 or 0 the default 8-bit digits.  Returns the
+ * previous setting.  Process-wide tuning knob; no counterpart in the reference. */
+int bass_sort_digit_bits(int bits);
+
+/* The plan bass_radix_sort follows for keys of key_bits bits: returns the
+ * digit width (8 or 10) and stores the number of digit passes in *passes (an
+ * odd count adds one copy back into keys / vals).  No counterpart in the
+ * reference. */
+int bass_sort_plan(int key_bits, int* passes);
 
 /* Run-length sum of equal adjacent keys of a sorted array into (xo, ao); zero
  * sums are kept.  Replaces merge_sorted_kernel (src/core/sparse_state.py:127-147). */

@@ -18,6 +18,8 @@ def load():
         P = ctypes.c_void_p
         L.cpu_step_1q.restype = ctypes.c_int64
         L.cpu_step_1q.argtypes = [P, P, ctypes.c_int64, ctypes.c_int, P, ctypes.c_int64, P, P, P, P]
+        L.cpu_radix_sort.restype = None
+        L.cpu_radix_sort.argtypes = [P, P, ctypes.c_int64]
         _LIB = L
     return _LIB
 
@@ -33,3 +35,9 @@ def step_1q(x, a, n, q, G, k, tx, ta):
     m = L.cpu_step_1q(x.ctypes.data, a.ctypes.data, int(n), int(q), g.ctypes.data, int(k), tx.ctypes.data,
                       ta.ctypes.data, ctypes.byref(gf), ctypes.byref(mg))
     return int(m), gf.value, int(mg.value)
+
+
+def radix_sort(x, a, n):
+    """The reference's 16-bit LSD radix sort of x[:n] (uint64) carrying a
+    (complex128), in place."""
+    load().cpu_radix_sort(x.ctypes.data, a.ctypes.data, int(n))

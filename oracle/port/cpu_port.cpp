@@ -149,4 +149,12 @@ int64_t cpu_step_1q(uint64_t* x, double* a, int64_t n, int q, const double* G, i
   return m;
 }
 
+// The reference's radix_sort_uint64 alone (src/core/sparse_state.py:150-172),
+// in place on (keys, a: re, im interleaved).
+void cpu_radix_sort(uint64_t* keys, double* a, int64_t n) {
+  static thread_local std::vector<uint64_t> sk;
+  static thread_local std::vector<double> sv;
+  radix_sort(keys, a, n, sk, sv);
+}
+
 }  // extern "C"

@@ -82,6 +82,24 @@ def test_argument_errors_without_gpu(lib):
     assert rc == L.BASS_E_OVERFLOW
     rc = lib.bass_radix_sort(None, None, 10, 0, None, 0, None)
     assert rc == L.BASS_E_ARG
+    rc = lib.bass_radix_sort(None, None, 1 << 30, 40, None, 0, None)
+    assert rc == L.BASS_E_ARG and "2^30" in L.last_error()
+    rc = lib.bass_sort_digit_bits(9)
+    assert rc == L.BASS_E_ARG and "0, 8 or 10" in L.last_error()
+    assert lib.bass_sort_digit_bits(10) == 0 and lib.bass_sort_digit_bits(0) == 10
+    # digit plan: 8-bit digits unless 10 were requested
+    P = ctypes.c_int(0)
+    for bits, want in [(40, (8, 5)), (56, (8, 7)), (64, (8, 8)), (5, (8, 1))]:
+        rb = lib.bass_sort_plan(bits, ctypes.byref(P))
+        assert (rb, P.value) == want, bits
+    lib.bass_sort_digit_bits(10)
+    try:
+        for bits, want in [(40, (10, 4)), (56, (10, 6)), (64, (10, 7))]:
+            rb = lib.bass_sort_plan(bits, ctypes.byref(P))
+            assert (rb, P.value) == want, bits
+    finally:
+        lib.bass_sort_digit_bits(0)
+    assert lib.bass_sort_plan(0, ctypes.byref(P)) == L.BASS_E_ARG
     rc = lib.bass_dense_gate(None, 0, 1, 0, 0, G, None)
     assert rc == L.BASS_E_ARG and "N out of range" in L.last_error()
     rc = lib.bass_dense_gate(None, 4, 3, 0, 1, G, None)

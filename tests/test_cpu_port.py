@@ -79,3 +79,15 @@ def test_port_exact_ties_keep_smaller_keys():
     assert (m, merged) == (k, n)
     assert set(cx[:m].tolist()) == set(ox.tolist()) == set(x[:k].tolist())
     assert abs(gf - og) <= 1e-15
+
+
+def test_port_radix_sort_is_stable():
+    rng = np.random.default_rng(3)
+    n = 50000
+    x = rng.integers(0, 1 << 40, size=n, dtype=np.uint64)
+    x[: n // 4] = x[n // 4: n // 2]
+    a = (np.arange(n) + 1j * rng.standard_normal(n)).astype(np.complex128)
+    gx, ga = x.copy(), a.copy()
+    port.radix_sort(gx, ga, n)
+    order = np.argsort(x, kind="stable")
+    assert np.array_equal(gx, x[order]) and np.array_equal(ga, a[order])

@@ -269,6 +269,47 @@ def test_radix_sort_and_merge(gpu_ctx, n, bits):
     np.testing.assert_allclose(ga[:m], oa[:m], rtol=1e-15, atol=0)
 
 
+@pytest.mark.parametrize("n,bits,digit,kind", [
+    (4095, 24, 0, "random"), (4096, 20, 0, "random"), (4097, 40, 0, "random"),
+    (300000, 40, 8, "random"), (300000, 40, 10, "random"), (300000, 56, 0, "random"),
+    (250001, 64, 10, "random"), (250001, 64, 0, "random"), (123457, 33, 0, "random"),
+    (200000, 40, 0, "few"), (200000, 40, 10, "equal"), (200000, 48, 0, "sorted"),
+    (200000, 40, 0, "reversed"), (1 << 23, 40, 0, "pairs")])
+def test_radix_sort_one_sweep(gpu_ctx, n, bits, digit, kind):
+    """Stable sort against numpy's stable argsort, for both digit widths, tile
+    edges (4096 entries per tile), long runs of one digit, presorted input and
+    the emitted pair layout of a 1-qubit gate (k & ~m, k | m)."""
+    print(f"[case] one-sweep sort n={n} bits={bits} digit={digit} kind={kind}", flush=True)
+    rng = np.random.default_rng(n + bits)
+    top = (1 << bits) if bits < 64 else (1 << 64)
+    if kind == "few":
+        x = rng.choice(rng.integers(0, top, size=7, dtype=np.uint64), size=n)
+    elif kind == "equal":
+        x = np.full(n, top - 1, np.uint64)
+    elif kind == "pairs":
+        k = rng.integers(0, top, size=n // 2, dtype=np.uint64)
+        m = np.uint64(1 << 17)
+        x = np.stack([k & ~m, k | m], axis=1).reshape(-1)
+    else:
+        x = rng.integers(0, top - 1, size=n, dtype=np.uint64, endpoint=True) if bits == 64 else \
+            rng.integers(0, top, size=n, dtype=np.uint64)
+        if kind == "sorted":
+            x = np.sort(x)
+        elif kind == "reversed":
+            x = np.sort(x)[::-1].copy()
+    a = (np.arange(n) + 1j * rng.standard_normal(n)).astype(np.complex128)  # real part = input index
+    lib = gpu_ctx.lib
+    old = lib.bass_sort_digit_bits(digit)
+    try:
+        gx, ga = x.copy(), a.copy()
+        ss().radix_sort_uint64(gx, ga, n, bits)
+    finally:
+        lib.bass_sort_digit_bits(old)
+    order = np.argsort(x, kind="stable")
+    assert np.array_equal(gx, x[order])
+    assert np.array_equal(ga, a[order]), "payload not stable"
+
+
 def test_hash_table(gpu_ctx):
     print("[case] hash", flush=True)
     n = 50000
