@@ -946,13 +946,16 @@ int bass_radix_sort(uint64_t* keys, double* vals, int64_t n, int key_bits, void*
   hipStream_t s = (hipStream_t)stream;
   OpStats* st = at<OpStats>(ws, L.st);
   char* big = at<char>(ws, L.big);
+  // two record buffers of n entries: 2 x (8 + 16) n bytes of the 48 S big region
   uint64_t* tk = (uint64_t*)big;
   double2* tv = (double2*)(big + al((size_t)8 * n));
+  uint64_t* tk2 = (uint64_t*)(big + al((size_t)8 * n) + al((size_t)16 * n));
+  double2* tv2 = (double2*)((char*)tk2 + al((size_t)8 * n));
   const int tm = ktimer_begin("sort", s);
   // histograms, tickets and the first pass's look-back words start at zero
   // (contiguous in the layout)
   fill(at<void>(ws, L.sort_hist), (L.sort_status - L.sort_hist) + (size_t)sweep_tiles(n) * 1024 * 4, 0ull, s);
-  launch_radix_sort(keys, (double2*)vals, n, key_bits, tk, tv, at<unsigned int>(ws, L.sort_hist),
+  launch_radix_sort(keys, (double2*)vals, n, key_bits, tk, tv, tk2, tv2, at<unsigned int>(ws, L.sort_hist),
                     at<unsigned int>(ws, L.sort_status), at<unsigned int>(ws, L.sort_tick), &st->err, s);
   ktimer_end(tm, s);
   return hip_check("bass_radix_sort");

@@ -197,11 +197,12 @@ int launch_rdm(const uint64_t* x, const double2* a, int64_t n, int N, const uint
                const int32_t* tvals, int log2cap, double* part, double* rho, hipStream_t s);
 constexpr int SORT_MAX_PASSES = 8;
 constexpr int64_t SORT_MAX_N = (1ll << 30) - 1;  // 32-bit positions, 30-bit look-back counts
-// hist: SORT_MAX_PASSES x 1024 words, tickets: SORT_MAX_PASSES words, both zero
-// on entry; status: sort_status_bytes(n), its first sweep_tiles(n) x 1024 words zero.
+// (tk, tv), (tk2, tv2): two scratch record buffers of n entries; hist:
+// SORT_MAX_PASSES x 1024 words, tickets: SORT_MAX_PASSES words, both zero on
+// entry; status: sort_status_bytes(n), its first sweep_tiles(n) x 1024 words zero.
 int launch_radix_sort(uint64_t* keys, double2* vals, int64_t n, int key_bits, uint64_t* tk, double2* tv,
-                      unsigned int* hist, unsigned int* status, unsigned int* tickets, int* err,
-                      hipStream_t s);
+                      uint64_t* tk2, double2* tv2, unsigned int* hist, unsigned int* status,
+                      unsigned int* tickets, int* err, hipStream_t s);
 int sweep_tiles(int64_t n);
 size_t sort_status_bytes(int64_t n);
 int sort_plan(int key_bits, int* passes);
