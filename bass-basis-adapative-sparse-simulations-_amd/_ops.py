@@ -91,7 +91,9 @@ def truncate(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False, ke
     (stats.flags bit 0 clear); stats.n_in is the input count.
     key_map: stored-bit -> qubit map (key_map_arg) when src holds stored words."""
     n = 0 if n is None else n
-    ctx.ensure_ws(max(n, hist_tiles * SEG if segmented else 0))
+    # the select reads whole tiles of SEG slots (and hist_tiles of per-tile sums)
+    tiles = hist_tiles if segmented else -(-int(n) // SEG)
+    ctx.ensure_ws(max(n, tiles * SEG, hist_tiles * SEG))
     dst.ensure(k)
     if key_map is None:
         kp, kb = None, 0

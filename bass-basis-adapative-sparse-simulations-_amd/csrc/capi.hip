@@ -44,7 +44,7 @@ struct Layout {
   // persistent
   size_t st, tile_psum, tile_dsum, tile_s2, tile_s4, rdm_part, rho, sort_hist, sort_tick;
   size_t lb_gate, lb1, sort_status;
-  size_t seg_cnt, soff, cnt1, tile_dcnt, ksum, kcnt, koff, lift_b, hist2_rows;
+  size_t seg_cnt, soff, cnt1, tile_dcnt, ksum, kcnt, koff, lift_b, hist2_rows, chunks;
   size_t big;
   size_t total;
 };
@@ -85,6 +85,9 @@ static Layout layout(int64_t S) {
   L.koff = o; o += al(stiles * 8);
   L.lift_b = o; o += al((size_t)lift_bounds_entries(etiles + 1) * sizeof(LiftBound));
   L.hist2_rows = o; o += al((size_t)stiles * HIST_BINS * 2);
+  // per-chunk values of the chunked reductions: 6 arrays for a select, 2 for
+  // a gate's segment scan (read later by bass_densify)
+  L.chunks = o; o += al((size_t)8 * 8 * (sel_chunks(stiles) + 1));
   L.big = o;
   // big scratch union:
   //   gate group table: cap <= 4 n_in slots x <= 32 B, n_in <= S/4 (2q) or S/2 (1q) -> <= 32 S
@@ -113,6 +116,14 @@ static int log2_ceil(int64_t v) {
   return l;
 }
 
+// chunk counts / offsets of a gate's segment scan (after the select's six arrays)
+static int64_t* seg_chunk_cnt(void* ws, const Layout& L) {
+  return at<int64_t>(ws, L.chunks) + 6 * ((size_t)sel_chunks(L.S / 2048 + 2) + 1);
+}
+static int64_t* seg_chunk_off(void* ws, const Layout& L) {
+  return seg_chunk_cnt(ws, L) + ((size_t)sel_chunks(L.S / 2048 + 2) + 1);
+}
+
 static SelTiles tiles_view(void* ws, const Layout& L, unsigned long long* c1p, unsigned int* c1i) {
   SelTiles T;
   T.tile_psum = at<double>(ws, L.tile_psum);
@@ -127,6 +138,14 @@ static SelTiles tiles_view(void* ws, const Layout& L, unsigned long long* c1p, u
   T.tile_s2 = at<double>(ws, L.tile_s2);
   T.tile_s4 = at<double>(ws, L.tile_s4);
   T.hist2_rows = at<unsigned short>(ws, L.hist2_rows);
+  const size_t nc = (size_t)sel_chunks(L.S / 2048 + 2) + 1;
+  int64_t* ch = at<int64_t>(ws, L.chunks);
+  T.ch_cnt = ch;
+  T.ch_off = ch + nc;
+  T.ch_sum = (double*)(ch + 2 * nc);
+  T.ch_psum = (double*)(ch + 3 * nc);
+  T.ch_s2 = (double*)(ch + 4 * nc);
+  T.ch_s4 = (double*)(ch + 5 * nc);
   return T;
 }
 
@@ -310,7 +329,8 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
     launch_emit(nq, mode, dedup != 0, x, (const double2*)a, n, q0, nq == 2 ? q1 : 0, gslot, log2cap, gm, xo,
                 (double2*)ao, cap_out, at<unsigned long long>(ws, L.lb_gate), at<unsigned int>(ws, L.tick_gate),
                 at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), &st->nnz, seg, &st->err, s);
-    if (segmented) launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), st, s);
+    if (segmented)
+      launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), seg_chunk_cnt(ws, L), seg_chunk_off(ws, L), st, s);
     if (hist_tiles) *hist_tiles = tiles;
   }
   ktimer_end(tk, s);
@@ -348,7 +368,8 @@ int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, cons
     int* seg = at<int>(ws, L.seg_cnt);
     launch_lift(nq, mode, x, (const double2*)a, n, j, gm, xo, (double2*)ao, at<LiftBound>(ws, L.lift_b),
                 at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), seg, s);
-    if (mode != MODE_MOVE) launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), st, s);
+    if (mode != MODE_MOVE)
+      launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), seg_chunk_cnt(ws, L), seg_chunk_off(ws, L), st, s);
   }
   ktimer_end(tk, s);
   return hip_check("bass_lift");
@@ -359,7 +380,8 @@ int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t
   if (seg_tiles < 0) return fail(BASS_E_ARG, "bass_densify: bad segment count");
   WS_REQUIRE(seg_tiles * SEG_ENTRIES);
   (void)cap_out;
-  launch_densify(x, (const double2*)a, at<int>(ws, L.seg_cnt), at<int64_t>(ws, L.soff), seg_tiles, xo,
+  launch_densify(x, (const double2*)a, at<int>(ws, L.seg_cnt), at<int64_t>(ws, L.soff), seg_chunk_off(ws, L),
+                 seg_tiles, xo,
                  (double2*)ao, (hipStream_t)stream);
   return hip_check("bass_densify");
 }
@@ -390,7 +412,9 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   }
   const int64_t tiles = segmented ? hist_tiles : sel_tiles(n);
   const int64_t slots = tiles * SEG_ENTRIES;
-  WS_REQUIRE(slots > n ? slots : n);
+  const int64_t psum_slots = hist_tiles > 0 ? hist_tiles * SEG_ENTRIES : 0;  // per-tile sums read
+  const int64_t need_ws = slots > n ? slots : n;
+  WS_REQUIRE(need_ws > psum_slots ? need_ws : psum_slots);
   hipStream_t s = (hipStream_t)stream;
   OpStats* st = at<OpStats>(ws, L.st);
   fill(at<void>(ws, L.hist2), L.hdrB_end - L.hist2, 0ull, s);
@@ -409,6 +433,7 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   P.row0 = 0;
   P.nrows = select_rows(tiles, false);
   P.npsum = hist_tiles > 0 ? hist_tiles : tiles;
+  P.chunked = sel_chunks(P.npsum > tiles ? P.npsum : tiles) > 1 ? 1 : 0;
   P.k = (unsigned long long)k;
   P.xo = xo;
   P.ao = (double2*)ao;

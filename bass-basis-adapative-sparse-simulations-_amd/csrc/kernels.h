@@ -146,7 +146,7 @@ struct SelProblem {
   double2* ao;
   int64_t cap_out;
   int renorm;
-  int pad;
+  int chunked;           // per-tile reductions run by chunk (single large problems)
   unsigned int* hist1;   // HIST_BINS, zeroed by the caller unless filled by the producing gate
   unsigned int* hist2;   // HIST_BINS, zeroed
   unsigned int* hist_w;  // HIST_BINS, zeroed (wide path)
@@ -173,16 +173,30 @@ struct SelTiles {
   double* tile_s2;
   double* tile_s4;
   unsigned short* hist2_rows;
+  // per chunk of SEL_CHUNK tiles (chunked problems): kept count, kept-entry
+  // offset, kept / input / output sums
+  int64_t* ch_cnt;
+  int64_t* ch_off;
+  double* ch_sum;
+  double* ch_psum;
+  double* ch_s2;
+  double* ch_s4;
 };
+
+// Tiles per chunk of the chunked per-tile reductions (one block of NTB threads
+// handling 4 tiles each); inputs of more tiles than this reduce in two levels.
+constexpr int64_t SEL_CHUNK = 4096;
+inline int64_t sel_chunks(int64_t ntiles) { return (ntiles + SEL_CHUNK - 1) / SEL_CHUNK; }
 
 int select_rows(int64_t ntiles, bool batch);
 int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s);
 int launch_select_batch(const SelProblem* d_probs, int nprob, const SelTiles& T, int64_t tiles_end,
                         int64_t max_rows, bool hist_ready, bool wide, hipStream_t s);
 int64_t sel_tiles(int64_t n);
-int launch_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff, OpStats* st, hipStream_t s);
-int launch_densify(const uint64_t* x, const double2* a, const int* seg_cnt, const int64_t* soff, int64_t ntiles,
-                   uint64_t* xo, double2* ao, hipStream_t s);
+int launch_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff, int64_t* ch_cnt, int64_t* ch_off,
+                    OpStats* st, hipStream_t s);
+int launch_densify(const uint64_t* x, const double2* a, const int* seg_cnt, const int64_t* soff,
+                   const int64_t* ch_off, int64_t ntiles, uint64_t* xo, double2* ao, hipStream_t s);
 int launch_moments(const double2* a, int64_t n, double* tile_s2, double* tile_s4, OpStats* st,
                    hipStream_t s);
 int launch_scale(double2* a, int64_t n, double div, hipStream_t s);

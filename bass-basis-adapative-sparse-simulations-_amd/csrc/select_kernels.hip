@@ -124,7 +124,9 @@ __device__ double block_sum_array(const double* v, int64_t m, double* lds) {
   return block_sum<NTHR>(s, lds);
 }
 
-constexpr int EX1_TILES = 4;  // tiles per k_extract1 block of a single problem
+constexpr int EX1_TILES = 4;         // tiles per k_extract1 block of a single problem
+constexpr int EX1_TILES_LARGE = 16;  // the same for large problems (row counts still fit 16 bits)
+int extract1_tiles(int64_t ntiles);
 
 // ------------------------------------------------------ problem lookup ----
 // Problem of a global tile: the last problem whose tile0 <= gt (problems are
@@ -226,11 +228,12 @@ __global__ __launch_bounds__(NTB) void k_find1(const SelProblem* probs, SelProbl
   (void)block_excl_scan<NTB>(cnt, count, s_scan);
   const bool keepall = (unsigned long long)count <= P.k;
   if (!keepall) find_from_top<NTB>(P.hist1, P.k, &s_b, &s_above, s_scan);
-  const double total = block_sum_array<NTB>(T.tile_psum + P.tile0, P.npsum, s_red);
+  // a chunked problem sums its input |a|^2 by chunk (k_sel_chunks, k_sel_final)
+  const double total = P.chunked ? 0.0 : block_sum_array<NTB>(T.tile_psum + P.tile0, P.npsum, s_red);
   if (threadIdx.x == 0) {
     SelState* sel = P.sel;
     OpStats* st = P.st;
-    st->total = total;
+    if (!P.chunked) st->total = total;
     st->n_in = count;
     if (keepall) {
       // every entry passes kept_pred: bits > 0, or bits == 0 and key <= ~0
@@ -275,33 +278,49 @@ __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int np
   }
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
   const unsigned int b1 = sel->b1;
-  for (int sub = 0; sub < EXT; ++sub) {
-    const int64_t gt = gt0 + sub;
-    if (gt >= tiles_end) break;  // uniform
-    if (threadIdx.x == 0) s_dcnt = 0;
-    __syncthreads();
-    const int cnt = tile_count(P, gt);
-    const int64_t base = tile_base(P, gt);
-    const int64_t cbase = gt * SEL_TILE;
-    unsigned long long cb[SEL_ITEMS];
-    bool isc[SEL_ITEMS];
-    double dsum = 0.0;
-    int dcnt = 0;
-    int ci[SEL_ITEMS], coff[SEL_ITEMS];
-    // all loads first (clamped to a valid slot), so they are in flight together
-    double vr[SEL_ITEMS], vi[SEL_ITEMS];
+  // amplitudes of the current tile; the next tile's are loaded (clamped to a
+  // valid slot) before the current one is processed, so one tile's loads are
+  // always in flight (block barriers wait on LDS only)
+  double vr[SEL_ITEMS], vi[SEL_ITEMS];
+  int cnt = gt0 < tiles_end ? tile_count(P, gt0) : 0;
+  if (gt0 < tiles_end) {
+    const int64_t base0 = tile_base(P, gt0);
 #pragma unroll
     for (int it = 0; it < SEL_ITEMS; ++it) {
       const int j = it * NT + threadIdx.x;
-      const double2 v = P.a[base + (j < cnt ? j : 0)];
+      const double2 v = P.a[base0 + (j < cnt ? j : 0)];
       vr[it] = v.x;
       vi[it] = v.y;
     }
+  }
+  for (int sub = 0; sub < EXT; ++sub) {
+    const int64_t gt = gt0 + sub;
+    if (gt >= tiles_end) break;  // uniform
+    const bool more = sub + 1 < EXT && gt + 1 < tiles_end;  // uniform
+    double nr[SEL_ITEMS], ni[SEL_ITEMS];
+    int ncnt = 0;
+    if (more) {
+      ncnt = tile_count(P, gt + 1);
+      const int64_t nbase = tile_base(P, gt + 1);
+#pragma unroll
+      for (int it = 0; it < SEL_ITEMS; ++it) {
+        const int j = it * NT + threadIdx.x;
+        const double2 v = P.a[nbase + (j < ncnt ? j : 0)];
+        nr[it] = v.x;
+        ni[it] = v.y;
+      }
+    }
+    if (threadIdx.x == 0) s_dcnt = 0;
+    __syncthreads();
+    const int64_t base = tile_base(P, gt);
+    const int64_t cbase = gt * SEL_TILE;
+    double dsum = 0.0;
+    int dcnt = 0;
+    int ci[SEL_ITEMS], coff[SEL_ITEMS];
 #pragma unroll
     for (int it = 0; it < SEL_ITEMS; ++it) {
       const int j = it * NT + threadIdx.x;
-      isc[it] = false;
-      cb[it] = 0;
+      ci[it] = 0;
       if (j < cnt) {
         const double p = cabs2(make_double2(vr[it], vi[it]));
         const unsigned long long bits = (unsigned long long)__double_as_longlong(p);
@@ -310,30 +329,46 @@ __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int np
           dsum += p;
           ++dcnt;
         } else if (d == b1) {
-          isc[it] = true;
-          cb[it] = bits;
+          ci[it] = 1;
           atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
         }
       }
-      ci[it] = isc[it] ? 1 : 0;
     }
-    // candidates in input order (item-major), one scan for all items
+    // candidates in input order (item-major), one scan for all items; their
+    // bit patterns are recomputed from the amplitudes (same operations, same
+    // rounding) instead of being held through the scan
     int rb;
     block_excl_scan_items<SEL_ITEMS>(ci, coff, rb, s_scan);
 #pragma unroll
     for (int it = 0; it < SEL_ITEMS; ++it) {
-      if (isc[it]) {
-        T.c1p[cbase + coff[it]] = cb[it];
-        T.c1i[cbase + coff[it]] = (unsigned int)(base + it * NT + threadIdx.x);
+      const int j = it * NT + threadIdx.x;
+      if (j < cnt) {
+        const unsigned long long bits =
+            (unsigned long long)__double_as_longlong(cabs2(make_double2(vr[it], vi[it])));
+        if ((unsigned int)(bits >> 52) == b1) {
+          T.c1p[cbase + coff[it]] = bits;
+          T.c1i[cbase + coff[it]] = (unsigned int)(base + j);
+        }
       }
     }
     const double s = block_sum(dsum, s_red);
-    atomicAdd(&s_dcnt, dcnt);
+    // count above the bucket: wave sums, one LDS add per wave
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) dcnt += __shfl_xor(dcnt, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&s_dcnt, dcnt);
     __syncthreads();
     if (threadIdx.x == 0) {
       T.tile_dsum[gt] = s;
       T.tile_dcnt[gt] = s_dcnt;
       T.cnt1[gt] = rb;
+    }
+    if (more) {
+      cnt = ncnt;
+#pragma unroll
+      for (int it = 0; it < SEL_ITEMS; ++it) {
+        vr[it] = nr[it];
+        vi[it] = ni[it];
+      }
     }
   }
   __syncthreads();
@@ -780,6 +815,87 @@ __global__ __launch_bounds__(NTB) void k_sel_finalize(const SelProblem* probs, S
   }
 }
 
+// Chunked form of k_sel_finalize for one large problem, first level: block c
+// scans the kept counts of tiles [c * SEL_CHUNK, (c+1) * SEL_CHUNK) into
+// chunk-local offsets and sums, in a fixed order, their kept and input |a|^2.
+__global__ __launch_bounds__(NTB) void k_sel_chunks(SelProblem P, SelTiles T) {
+  static_assert(SEL_CHUNK == NTB * FIN_CH, "one round per chunk");
+  __shared__ double s_red[WAVESB];
+  __shared__ int s_scan[WAVESB];
+  const int64_t c = blockIdx.x;
+  const int64_t t_end = P.tile0 + P.ntiles, p_end = P.tile0 + P.npsum;
+  const int64_t t0 = P.tile0 + c * SEL_CHUNK + (int64_t)threadIdx.x * FIN_CH;
+  double ds[FIN_CH], ks[FIN_CH], ps[FIN_CH];
+  int cn[FIN_CH];
+#pragma unroll
+  for (int q = 0; q < FIN_CH; ++q) {
+    const int64_t t = t0 + q;
+    const bool in = t < t_end;
+    ds[q] = in ? T.tile_dsum[t] : 0.0;
+    ks[q] = in ? T.ksum[t] : 0.0;
+    cn[q] = in ? T.tile_dcnt[t] + T.kcnt[t] : 0;
+    ps[q] = t < p_end ? T.tile_psum[t] : 0.0;
+  }
+  double sk = 0.0, sp = 0.0;
+  int v = 0;
+#pragma unroll
+  for (int q = 0; q < FIN_CH; ++q) {
+    sk += ds[q] + ks[q];
+    sp += ps[q];
+    v += cn[q];
+  }
+  int tot;
+  int64_t run = block_excl_scan<NTB>(v, tot, s_scan);
+#pragma unroll
+  for (int q = 0; q < FIN_CH; ++q) {
+    if (t0 + q < t_end) T.koff[t0 + q] = run;
+    run += cn[q];
+  }
+  const double kept = block_sum<NTB>(sk, s_red);
+  const double inp = block_sum<NTB>(sp, s_red);
+  if (threadIdx.x == 0) {
+    T.ch_cnt[c] = tot;
+    T.ch_sum[c] = kept;
+    T.ch_psum[c] = inp;
+  }
+}
+
+// Second level: chunk offsets (exclusive scan of the chunk counts), kept and
+// input totals summed over the chunks in a fixed order; the stats of
+// k_sel_finalize.
+__global__ __launch_bounds__(NTB) void k_sel_final(SelProblem P, SelTiles T, int64_t nch) {
+  __shared__ double s_red[WAVESB];
+  __shared__ int s_scan[WAVESB];
+  int64_t carry = 0;
+  for (int64_t r0 = 0; r0 < nch; r0 += NTB) {
+    const int64_t c = r0 + threadIdx.x;
+    const int v = c < nch ? (int)T.ch_cnt[c] : 0;
+    int tot;
+    const int ex = block_excl_scan<NTB>(v, tot, s_scan);
+    if (c < nch) T.ch_off[c] = carry + ex;
+    carry += tot;
+  }
+  const double kept = block_sum_array<NTB>(T.ch_sum, nch, s_red);
+  const double total = block_sum_array<NTB>(T.ch_psum, nch, s_red);
+  if (threadIdx.x == 0) {
+    OpStats* st = P.st;
+    st->total = total;
+    st->nnz = carry;
+    if (P.sel->keepall) {
+      st->kept = total;
+      st->norm = 1.0;
+    } else if (st->flags & 2) {
+      st->kept = 0.0;  // result discarded; the caller repeats with the wide path
+      st->norm = 1.0;
+    } else {
+      if ((unsigned long long)carry != P.k) set_err(&st->err, ERR_SELECT);
+      st->kept = kept;
+      const double norm = sqrt(kept);
+      st->norm = (P.renorm && norm > 1e-30) ? norm : 1.0;
+    }
+  }
+}
+
 template <bool BATCH>
 __global__ __launch_bounds__(NT) void k_compact(const SelProblem* probs, int nprob, SelProblem one, SelTiles T) {
   __shared__ int s_scan[WAVES];
@@ -804,7 +920,7 @@ __global__ __launch_bounds__(NT) void k_compact(const SelProblem* probs, int npr
       keep[it] = kept_pred(pbits_of(vv[it]), kk[it], pstar, keystar, P.km);
     }
   }
-  int64_t rb = T.koff[gt];
+  int64_t rb = T.koff[gt] + (P.chunked ? T.ch_off[(gt - P.tile0) / SEL_CHUNK] : 0);
   double s2 = 0.0, s4 = 0.0;
   bool fits = true;
 #pragma unroll
@@ -848,6 +964,43 @@ __global__ __launch_bounds__(NTB) void k_sums_finalize(const SelProblem* probs, 
   }
 }
 
+// Chunked form of k_sums_finalize: per-chunk sums of the output's |a|^2 and
+// |a|^4 in a fixed order, then their fixed-order totals.
+__global__ __launch_bounds__(NTB) void k_sums_chunks(SelProblem P, SelTiles T) {
+  __shared__ double s_red[WAVESB];
+  const int64_t t_end = P.tile0 + P.ntiles;
+  const int64_t t0 = P.tile0 + (int64_t)blockIdx.x * SEL_CHUNK + (int64_t)threadIdx.x * FIN_CH;
+  double a2[FIN_CH], a4[FIN_CH];
+#pragma unroll
+  for (int q = 0; q < FIN_CH; ++q) {
+    const bool in = t0 + q < t_end;
+    a2[q] = in ? T.tile_s2[t0 + q] : 0.0;
+    a4[q] = in ? T.tile_s4[t0 + q] : 0.0;
+  }
+  double s2 = 0.0, s4 = 0.0;
+#pragma unroll
+  for (int q = 0; q < FIN_CH; ++q) {
+    s2 += a2[q];
+    s4 += a4[q];
+  }
+  const double t2 = block_sum<NTB>(s2, s_red);
+  const double t4 = block_sum<NTB>(s4, s_red);
+  if (threadIdx.x == 0) {
+    T.ch_s2[blockIdx.x] = t2;
+    T.ch_s4[blockIdx.x] = t4;
+  }
+}
+
+__global__ __launch_bounds__(NTB) void k_sums_final(SelProblem P, SelTiles T, int64_t nch) {
+  __shared__ double s_red[WAVESB];
+  const double s2 = block_sum_array<NTB>(T.ch_s2, nch, s_red);
+  const double s4 = block_sum_array<NTB>(T.ch_s4, nch, s_red);
+  if (threadIdx.x == 0) {
+    P.st->sum2 = s2;
+    P.st->sum4 = s4;
+  }
+}
+
 __global__ __launch_bounds__(NTB) void k_sums_reduce(const double* tile_s2, const double* tile_s4,
                                                       int64_t ntiles, OpStats* st) {
   __shared__ double s_red[WAVESB];
@@ -860,8 +1013,9 @@ __global__ __launch_bounds__(NTB) void k_sums_reduce(const double* tile_s2, cons
 }
 
 // Segment counts -> exclusive offsets and total (nnz of a segmented output).
+// Single block; ch_off[0] = 0 so readers add ch_off[t / SEL_CHUNK] in both forms.
 __global__ __launch_bounds__(NTB) void k_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff,
-                                                   OpStats* st) {
+                                                   int64_t* ch_off, OpStats* st) {
   __shared__ int s_scan[WAVESB];
   int64_t carry = 0;
   for (int64_t r0 = 0; r0 < ntiles; r0 += (int64_t)NTB * FIN_CH) {
@@ -881,16 +1035,56 @@ __global__ __launch_bounds__(NTB) void k_seg_scan(const int* seg_cnt, int64_t nt
     }
     carry += tot;
   }
+  if (threadIdx.x == 0) {
+    st->nnz = carry;
+    ch_off[0] = 0;
+  }
+}
+
+// Chunked segment scan, first level: chunk-local offsets and chunk totals.
+__global__ __launch_bounds__(NTB) void k_seg_chunks(const int* seg_cnt, int64_t ntiles, int64_t* soff,
+                                                     int64_t* ch_cnt) {
+  __shared__ int s_scan[WAVESB];
+  const int64_t t0 = (int64_t)blockIdx.x * SEL_CHUNK + (int64_t)threadIdx.x * FIN_CH;
+  int c[FIN_CH];
+#pragma unroll
+  for (int q = 0; q < FIN_CH; ++q) c[q] = t0 + q < ntiles ? seg_cnt[t0 + q] : 0;
+  int v = 0;
+#pragma unroll
+  for (int q = 0; q < FIN_CH; ++q) v += c[q];
+  int tot;
+  int64_t run = block_excl_scan<NTB>(v, tot, s_scan);
+#pragma unroll
+  for (int q = 0; q < FIN_CH; ++q) {
+    if (t0 + q < ntiles) soff[t0 + q] = run;
+    run += c[q];
+  }
+  if (threadIdx.x == 0) ch_cnt[blockIdx.x] = tot;
+}
+
+// Second level: chunk offsets and the total.
+__global__ __launch_bounds__(NTB) void k_seg_final(const int64_t* ch_cnt, int64_t nch, int64_t* ch_off,
+                                                    OpStats* st) {
+  __shared__ int s_scan[WAVESB];
+  int64_t carry = 0;
+  for (int64_t r0 = 0; r0 < nch; r0 += NTB) {
+    const int64_t c = r0 + threadIdx.x;
+    const int v = c < nch ? (int)ch_cnt[c] : 0;
+    int tot;
+    const int ex = block_excl_scan<NTB>(v, tot, s_scan);
+    if (c < nch) ch_off[c] = carry + ex;
+    carry += tot;
+  }
   if (threadIdx.x == 0) st->nnz = carry;
 }
 
 // Copy a segmented array to dense positions (soff from k_seg_scan).
 __global__ __launch_bounds__(NT) void k_densify(const uint64_t* __restrict__ x, const double2* __restrict__ a,
-                                                 const int* seg_cnt, const int64_t* soff,
+                                                 const int* seg_cnt, const int64_t* soff, const int64_t* ch_off,
                                                  uint64_t* __restrict__ xo, double2* __restrict__ ao) {
   const int64_t t = blockIdx.x;
   const int cnt = seg_cnt[t];
-  const int64_t src = t * SEL_TILE, dst = soff[t];
+  const int64_t src = t * SEL_TILE, dst = soff[t] + ch_off[t / SEL_CHUNK];
   for (int j = threadIdx.x; j < cnt; j += NT) {
     xo[dst + j] = x[src + j];
     ao[dst + j] = a[src + j];
@@ -958,13 +1152,20 @@ __global__ __launch_bounds__(NT) void k_gather(const uint64_t* __restrict__ x, c
 template <bool BATCH>
 static void select_sequence(const SelProblem* probs, int nprob, const SelProblem& one, const SelTiles& T,
                             int64_t tiles_end, int64_t max_rows, bool hist_ready, bool wide, hipStream_t s) {
-  constexpr int EXT = BATCH ? 1 : EX1_TILES;
   if (!hist_ready)
     hipLaunchKernelGGL((k_hist_psum<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
   hipLaunchKernelGGL((k_find1<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
-  const int64_t rows = (tiles_end + EXT - 1) / EXT;
-  hipLaunchKernelGGL((k_extract1<BATCH, EXT>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one, T,
-                     tiles_end);
+  const int ext = BATCH ? 1 : extract1_tiles(tiles_end);
+  const int64_t rows = (tiles_end + ext - 1) / ext;
+  if (ext == EX1_TILES_LARGE)
+    hipLaunchKernelGGL((k_extract1<BATCH, EX1_TILES_LARGE>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob,
+                       one, T, tiles_end);
+  else if (ext == EX1_TILES)
+    hipLaunchKernelGGL((k_extract1<BATCH, EX1_TILES>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one,
+                       T, tiles_end);
+  else
+    hipLaunchKernelGGL((k_extract1<BATCH, 1>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one, T,
+                       tiles_end);
   hipLaunchKernelGGL((k_hist2_reduce<BATCH>),
                      dim3(HIST_BINS / (4 * NT), (unsigned)((max_rows + H2_ROWS - 1) / H2_ROWS), (unsigned)nprob),
                      dim3(NT), 0, s, probs, one, T);
@@ -986,12 +1187,30 @@ static void select_sequence(const SelProblem* probs, int nprob, const SelProblem
     }
   }
   hipLaunchKernelGGL((k_keptsum<BATCH>), dim3(wave_tiles), dim3(NT), 0, s, probs, nprob, one, T, tiles_end);
-  hipLaunchKernelGGL((k_sel_finalize<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
+  const int64_t nch = sel_chunks(one.npsum > one.ntiles ? one.npsum : one.ntiles);
+  if (!BATCH && one.chunked) {
+    hipLaunchKernelGGL(k_sel_chunks, dim3((unsigned)nch), dim3(NTB), 0, s, one, T);
+    hipLaunchKernelGGL(k_sel_final, dim3(1), dim3(NTB), 0, s, one, T, nch);
+  } else {
+    hipLaunchKernelGGL((k_sel_finalize<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
+  }
   hipLaunchKernelGGL((k_compact<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
-  hipLaunchKernelGGL((k_sums_finalize<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
+  if (!BATCH && one.chunked) {
+    hipLaunchKernelGGL(k_sums_chunks, dim3((unsigned)sel_chunks(one.ntiles)), dim3(NTB), 0, s, one, T);
+    hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(NTB), 0, s, one, T, sel_chunks(one.ntiles));
+  } else {
+    hipLaunchKernelGGL((k_sums_finalize<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
+  }
 }
 
-int select_rows(int64_t ntiles, bool batch) { return (int)((ntiles + (batch ? 1 : EX1_TILES) - 1) / (batch ? 1 : EX1_TILES)); }
+// Tiles per k_extract1 block of a single problem: 16 from 16384 tiles on (a
+// quarter of the digit-2 rows for k_hist2_reduce to sum), else 4.
+int extract1_tiles(int64_t ntiles) { return ntiles >= 16384 ? EX1_TILES_LARGE : EX1_TILES; }
+
+int select_rows(int64_t ntiles, bool batch) {
+  const int ext = batch ? 1 : extract1_tiles(ntiles);
+  return (int)((ntiles + ext - 1) / ext);
+}
 
 int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s) {
   select_sequence<false>(nullptr, 1, P, T, P.ntiles, P.nrows, hist_ready, wide, s);
@@ -1010,15 +1229,22 @@ int launch_select_batch(const SelProblem* d_probs, int nprob, const SelTiles& T,
 
 int64_t sel_tiles(int64_t n) { return (n + SEL_TILE - 1) / SEL_TILE; }
 
-int launch_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff, OpStats* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(NTB), 0, s, seg_cnt, ntiles, soff, st);
+int launch_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff, int64_t* ch_cnt, int64_t* ch_off,
+                    OpStats* st, hipStream_t s) {
+  const int64_t nch = sel_chunks(ntiles);
+  if (nch <= 1) {
+    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(NTB), 0, s, seg_cnt, ntiles, soff, ch_off, st);
+  } else {
+    hipLaunchKernelGGL(k_seg_chunks, dim3((unsigned)nch), dim3(NTB), 0, s, seg_cnt, ntiles, soff, ch_cnt);
+    hipLaunchKernelGGL(k_seg_final, dim3(1), dim3(NTB), 0, s, ch_cnt, nch, ch_off, st);
+  }
   return (int)hipGetLastError();
 }
 
-int launch_densify(const uint64_t* x, const double2* a, const int* seg_cnt, const int64_t* soff, int64_t ntiles,
-                   uint64_t* xo, double2* ao, hipStream_t s) {
+int launch_densify(const uint64_t* x, const double2* a, const int* seg_cnt, const int64_t* soff,
+                   const int64_t* ch_off, int64_t ntiles, uint64_t* xo, double2* ao, hipStream_t s) {
   if (ntiles <= 0) return 0;
-  hipLaunchKernelGGL(k_densify, dim3((unsigned)ntiles), dim3(NT), 0, s, x, a, seg_cnt, soff, xo, ao);
+  hipLaunchKernelGGL(k_densify, dim3((unsigned)ntiles), dim3(NT), 0, s, x, a, seg_cnt, soff, ch_off, xo, ao);
   return (int)hipGetLastError();
 }
 

@@ -233,8 +233,8 @@ def bench_sort(args):
     src/core/sparse_state.py:268-285), complex normal amplitudes.  Every step
     restores the unsorted input (a device copy outside the timed region) and
     sorts it; time = HIP events around the sort.  Algorithmic bytes: one key
-    read for the histograms (8 B) + 48 B per digit pass + 48 B for the copy
-    back after an odd pass count."""
+    read for the histograms (8 B) + 48 B per digit pass (+ 48 B for the copy
+    back of a single pass)."""
     import ctypes
 
     import numpy as np
@@ -264,7 +264,7 @@ def bench_sort(args):
     passes = ctypes.c_int(0)
     rb = lib.bass_sort_plan(N, ctypes.byref(passes))
     P = passes.value
-    byt = 8 * n + 48 * n * (P + (P & 1))
+    byt = 8 * n + 48 * n * (P + (1 if P == 1 else 0))  # one pass: copied back; odd counts >= 3 rotate
 
     def one():
         x.copy_(x0)

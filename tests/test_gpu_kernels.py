@@ -238,6 +238,67 @@ def test_truncate_renorm_stats(gpu_ctx):
     assert abs(st.sum4 - np.sum(np.abs(ga) ** 4)) < 1e-12 * st.sum4
 
 
+@pytest.mark.parametrize("n,k", [(9_000_001, 4_000_000)])
+def test_truncate_chunked(gpu_ctx, n, k):
+    """More than 4096 tiles of 2048 entries: the per-tile bookkeeping (kept
+    offsets, kept / input / output sums) reduces by chunk in two levels."""
+    print(f"[case] truncate chunked n={n} k={k}", flush=True)
+    ops = pkg_module("_ops")
+    dev = pkg_module("_device")
+    x, a = random_state(n, 40, seed=9)
+    src = dev.Buffers(n, gpu_ctx.device)
+    src.x[:n] = torch.from_numpy(x.view(np.int64)).to(gpu_ctx.device)
+    src.a[:n] = torch.from_numpy(a).to(gpu_ctx.device)
+    dst = dev.Buffers(k, gpu_ctx.device)
+    st = ops.truncate(gpu_ctx, src, n, k, dst, renorm=True)
+    sel = OD.key_tiebreak_selector(x, a, None, k)
+    p = a.real * a.real + a.imag * a.imag
+    assert st.nnz == k
+    assert abs(st.total - p.sum()) < 1e-12 and abs(st.kept - p[sel].sum()) < 1e-12
+    gx = dev.to_host_keys(dst.x, k)
+    ga = dev.to_host_amps(dst.a, k)
+    assert np.array_equal(gx, x[sel]), "compaction must keep input order"
+    assert np.array_equal(ga, a[sel] / np.sqrt(st.kept)), "renormalisation must round like numpy"
+    q = ga.real * ga.real + ga.imag * ga.imag
+    assert abs(st.sum2 - q.sum()) < 1e-12 and abs(st.sum4 - np.sum(q * q)) < 1e-12 * st.sum4
+
+
+def test_segmented_gate_chunked(gpu_ctx):
+    """A 1-qubit gate on 5M entries: more than 4096 output tiles, so the
+    segment scan, densify and the truncation of the segmented output run their
+    chunked forms; all must agree with the dense path."""
+    n, k = 5_000_000, 4_500_000
+    print(f"[case] segmented chunked n={n} k={k}", flush=True)
+    ops = pkg_module("_ops")
+    dev = pkg_module("_device")
+    rng = np.random.default_rng(5)
+    x, a = random_state(n, 36, seed=11)
+    G = _gate_with_zero(rng, 2)
+    src = dev.Buffers(n, gpu_ctx.device)
+    src.x[:n] = torch.from_numpy(x.view(np.int64)).to(gpu_ctx.device)
+    src.a[:n] = torch.from_numpy(a).to(gpu_ctx.device)
+    dense = dev.Buffers(1, gpu_ctx.device)
+    m, _ = ops.gate(gpu_ctx, src, n, G, (7,), dense, dedup=True)
+    segb = dev.Buffers(1, gpu_ctx.device)
+    m2, tiles = ops.gate(gpu_ctx, src, n, G, (7,), segb, dedup=True, segmented=True)
+    assert tiles > 4096 and m2 == m
+    out = dev.Buffers(1, gpu_ctx.device)
+    ops.densify(gpu_ctx, segb, tiles, m2, out)
+    assert torch.equal(out.x[:m], dense.x[:m]) and torch.equal(out.a[:m], dense.a[:m])
+    t1 = dev.Buffers(k, gpu_ctx.device)
+    s1 = ops.truncate(gpu_ctx, dense, m, k, t1, renorm=True)
+    m3, tiles = ops.gate(gpu_ctx, src, n, G, (7,), segb, dedup=True, segmented=True)
+    t2 = dev.Buffers(k, gpu_ctx.device)
+    s2 = ops.truncate(gpu_ctx, segb, m3, k, t2, renorm=True, hist_tiles=tiles, segmented=True)
+    assert torch.equal(t1.x[:k], t2.x[:k])
+    np.testing.assert_allclose(dev.to_host_amps(t1.a, k), dev.to_host_amps(t2.a, k), rtol=1e-15, atol=0)
+    assert abs(s1.kept - s2.kept) <= 1e-14 * s1.kept and abs(s1.total - s2.total) <= 1e-14 * s1.total
+    # against the oracle's selection on the dense gate output
+    hx, ha = dev.to_host_keys(dense.x, m), dev.to_host_amps(dense.a, m)
+    sel = OD.key_tiebreak_selector(hx, ha, None, k)
+    assert np.array_equal(dev.to_host_keys(t2.x, k), hx[sel])
+
+
 def test_rdm(gpu_ctx):
     print("[case] rdm", flush=True)
     n, N = 20000, 18
