@@ -24,16 +24,28 @@ class Context:
         require_gpu()
         self.lib = L.load()
         self.device = torch.device(device if device is not None else "cuda:0")
+        self._dev_index = self.device.index if self.device.index is not None else 0
+        # the raw handle of torch's current stream without building a Stream
+        # object (several microseconds per call otherwise)
+        self._raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
         self.ws = None
         self.ws_bytes = 0
+        self._wsp = None
+        # bass_workspace_bytes(n) = fixed + per * n (n >= 1): read once
+        b1 = int(self.lib.bass_workspace_bytes(1))
+        self._ws_per = int(self.lib.bass_workspace_bytes(2)) - b1
+        self._ws_fixed = b1 - self._ws_per
         self.stats = L.BassStats()
 
     @property
     def stream(self):
+        """torch's current stream on this device (the caller's stream)."""
+        if self._raw_stream is not None:
+            return ctypes.c_void_p(self._raw_stream(self._dev_index))
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def ensure_ws(self, max_entries):
-        need = int(self.lib.bass_workspace_bytes(max(1, int(max_entries))))
+        need = self._ws_fixed + self._ws_per * max(1, int(max_entries))
         if need > self.ws_bytes:
             grow = max(need, int(self.ws_bytes * 1.5))
             self.ws = None
@@ -42,10 +54,11 @@ class Context:
             # leftover bytes there
             self.ws = torch.zeros(grow, dtype=torch.uint8, device=self.device)
             self.ws_bytes = grow
+            self._wsp = ctypes.c_void_p(self.ws.data_ptr())
 
     @property
     def wsp(self):
-        return ctypes.c_void_p(self.ws.data_ptr())
+        return self._wsp
 
     def read_stats(self, what="read_stats"):
         rc = self.lib.bass_read_stats(self.wsp, self.ws_bytes, ctypes.byref(self.stats), self.stream)
@@ -86,13 +99,14 @@ def context(device=None):
 class Buffers:
     """A (keys, amplitudes) pair of device buffers with a capacity."""
 
-    __slots__ = ("x", "a", "cap", "device")
+    __slots__ = ("x", "a", "cap", "device", "_xt", "_xp", "_at", "_ap")
 
     def __init__(self, cap, device):
         self.cap = max(1, int(cap))
         self.device = device
         self.x = torch.empty(self.cap, dtype=torch.int64, device=device)
         self.a = torch.empty(self.cap, dtype=torch.complex128, device=device)
+        self._xt = self._at = None
 
     def ensure(self, cap, keep=0):
         if cap <= self.cap:
@@ -105,13 +119,18 @@ class Buffers:
             a[:keep].copy_(self.a[:keep])
         self.x, self.a, self.cap = x, a, new
 
+    # device pointers, rebuilt only when a tensor is replaced
     @property
     def xp(self):
-        return ctypes.c_void_p(self.x.data_ptr())
+        if self._xt is not self.x:
+            self._xt, self._xp = self.x, ctypes.c_void_p(self.x.data_ptr())
+        return self._xp
 
     @property
     def ap(self):
-        return ctypes.c_void_p(self.a.data_ptr())
+        if self._at is not self.a:
+            self._at, self._ap = self.a, ctypes.c_void_p(self.a.data_ptr())
+        return self._ap
 
 
 def to_host_keys(t, n):

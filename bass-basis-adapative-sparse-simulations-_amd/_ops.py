@@ -75,6 +75,59 @@ def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE, count=True):
     return ctx.read_stats("bass_lift").nnz, int(st.value)
 
 
+def lift_counted(ctx, src, n_max, j, dst, G, nq=1, mode=L.MODE_GATE):
+    """lift() whose input count is the workspace's state-count slot (<= n_max),
+    read on the device: nothing waits for the device.  Returns the output's
+    tile count (its segmented output goes to a counted truncate)."""
+    tm = SEG // (1 << nq)
+    tiles = (n_max + tm - 1) // tm
+    need = tiles * SEG
+    ctx.ensure_ws(max(need, n_max, 1))
+    dst.ensure(max(need, 1))
+    buf, gp = L.mat_arg(G)
+    st = ctypes.c_int64(0)
+    rc = ctx.lib.bass_lift_counted(src.xp, src.ap, n_max, nq, int(j), gp, mode, dst.xp, dst.ap, dst.cap,
+                                   ctypes.byref(st), ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_lift_counted")
+    return int(st.value)
+
+
+def truncate_counted(ctx, src, k, dst, hist_tiles, key_map=None):
+    """truncate(..., segmented=True) in counted mode: resolves every threshold
+    itself, appends its stats to the op log and sets the state-count slot;
+    nothing waits for the device."""
+    ctx.ensure_ws(max(hist_tiles * SEG, 1))
+    dst.ensure(k)
+    if key_map is None:
+        kp, kb = None, 0
+    else:
+        km = np.ascontiguousarray(key_map, dtype=np.int32)
+        kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
+    rc = ctx.lib.bass_truncate(src.xp, src.ap, 0, k, dst.xp, dst.ap, dst.cap, 1, hist_tiles, 1 | 4, kp, kb,
+                               ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_truncate")
+
+
+def counted_begin(ctx, n, n_max, nq=1):
+    """Start a run of counted gates on a state of n entries (<= n_max): size the
+    workspace for every gate of the run first (growing it later would drop the
+    count slot and the op log), then write n into the state-count slot."""
+    tm = SEG // (1 << nq)
+    ctx.ensure_ws(max(((n_max + tm - 1) // tm) * SEG, n_max, 1))
+    L.check(ctx.lib.bass_set_count(ctx.wsp, ctx.ws_bytes, int(n), ctx.stream), "bass_set_count")
+
+
+def counted_read(ctx, max_entries):
+    """The op log of counted truncations since the last read, as a list of
+    bass_stats records (synchronous); empties it."""
+    out = (L.BassStats * max(1, int(max_entries)))()
+    n = ctypes.c_int64(0)
+    rc = ctx.lib.bass_counted_read(ctx.wsp, ctx.ws_bytes, ctypes.cast(out, ctypes.c_void_p), int(max_entries),
+                                   ctypes.byref(n), ctx.stream)
+    L.check(rc, "bass_counted_read")
+    return list(out[: n.value])
+
+
 def key_map_arg(pos):
     """Stored-bit -> qubit map (int32[N]) of an ordered state whose qubit q sits at bit pos[q]."""
     pos = np.asarray(pos)

@@ -87,6 +87,7 @@ def _pair(x, a, ctx):
     xs, as_ = _Arr(x, "x", ctx), _Arr(a, "a", ctx)
     b = Buffers.__new__(Buffers)
     b.x, b.a, b.device = xs.t, as_.t, ctx.device
+    b._xt = b._at = None
     b.cap = min(xs.t.shape[0], as_.t.shape[0])
     return b, xs, as_
 

@@ -18,12 +18,45 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t words, unsigned long l
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = v;
 }
 
+// Zero two regions (8-byte multiples) in one launch.
+__global__ void k_zero2(unsigned long long* p, int64_t wp, unsigned long long* q, int64_t wq) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < wp + wq; i += stride) {
+    if (i < wp)
+      p[i] = 0ull;
+    else
+      q[i - wp] = 0ull;
+  }
+}
+
+static void zero2(void* p, size_t pbytes, void* q, size_t qbytes, hipStream_t s) {
+  const int64_t wp = (int64_t)(pbytes / 8), wq = (int64_t)(qbytes / 8);
+  int64_t blocks = (wp + wq + NT - 1) / NT;
+  if (blocks > 64) blocks = 64;
+  if (blocks < 1) return;
+  hipLaunchKernelGGL(k_zero2, dim3((unsigned)blocks), dim3(NT), 0, s, (unsigned long long*)p, wp,
+                     (unsigned long long*)q, wq);
+}
+
 static void fill(void* p, size_t bytes, unsigned long long v, hipStream_t s) {
   const int64_t words = (int64_t)(bytes / 8);
   if (words <= 0) return;
   int64_t blocks = (words + NT - 1) / NT;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(NT), 0, s, (unsigned long long*)p, words, v);
+}
+
+// Counted mode (bass_lift_counted / bass_truncate flag 4): entries of the op
+// log, one per truncation, read back by bass_counted_read.
+constexpr int64_t OPLOG_CAP = 4096;
+
+// Append the stats block to the op log and set the state count slot to the
+// operation's output count (one thread; stream order serialises appends).
+__global__ void k_oplog_append(const OpStats* st, unsigned long long* log_n, OpStats* log, int64_t* count) {
+  const unsigned long long i = *log_n;
+  if (i < (unsigned long long)OPLOG_CAP) log[i] = *st;
+  *log_n = i + 1;
+  *count = st->nnz;
 }
 
 // ----------------------------------------------------------- workspace ----
@@ -45,6 +78,7 @@ struct Layout {
   size_t st, tile_psum, tile_dsum, tile_s2, tile_s4, rdm_part, rho, sort_hist, sort_tick;
   size_t lb_gate, lb1, sort_status;
   size_t seg_cnt, soff, cnt1, tile_dcnt, ksum, kcnt, koff, lift_b, hist2_rows, chunks;
+  size_t count_slot, oplog;  // counted mode: state entry count; op log (counter + entries)
   size_t big;
   size_t total;
 };
@@ -88,6 +122,8 @@ static Layout layout(int64_t S) {
   // per-chunk values of the chunked reductions: 6 arrays for a select, 2 for
   // a gate's segment scan (read later by bass_densify)
   L.chunks = o; o += al((size_t)8 * 8 * (sel_chunks(stiles) + 1));
+  L.count_slot = o; o += al(8);
+  L.oplog = o; o += al(8 + (size_t)OPLOG_CAP * sizeof(OpStats));
   L.big = o;
   // big scratch union:
   //   gate group table: cap <= 4 n_in slots x <= 32 B, n_in <= S/4 (2q) or S/2 (1q) -> <= 32 S
@@ -275,9 +311,17 @@ int bass_read_stats(void* ws, size_t ws_bytes, bass_stats* out, void* stream) {
   WS_REQUIRE(0);
   hipStream_t s = (hipStream_t)stream;
   OpStats* st = at<OpStats>(ws, L.st);
-  if (hipMemcpyAsync(out, st, sizeof(bass_stats), hipMemcpyDeviceToHost, s) != hipSuccess ||
+  // copy through a pinned per-thread buffer: a device-to-pageable copy is
+  // staged by the runtime and costs a second hop before the host sees it
+  static thread_local bass_stats* pinned = nullptr;
+  if (!pinned && hipHostMalloc((void**)&pinned, sizeof(bass_stats), hipHostMallocDefault) != hipSuccess) {
+    pinned = nullptr;
+    return fail(BASS_E_HIP, "bass_read_stats: hipHostMalloc: %s", hipGetErrorString(hipGetLastError()));
+  }
+  if (hipMemcpyAsync(pinned, st, sizeof(bass_stats), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return fail(BASS_E_HIP, "bass_read_stats: %s", hipGetErrorString(hipGetLastError()));
+  memcpy(out, pinned, sizeof(bass_stats));
   if (out->err) {
     fill(&st->err, 8, 0ull, s);
     return fail(BASS_E_DEVICE, "device error bits 0x%x", out->err);
@@ -337,9 +381,52 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
   return hip_check("bass_gate");
 }
 
+static int lift_impl(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G, int mode,
+                     uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws, size_t ws_bytes,
+                     void* stream, bool counted);
+
 int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G, int mode,
               uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws, size_t ws_bytes,
               void* stream) {
+  return lift_impl(x, a, n, nq, j, G, mode, xo, ao, cap_out, seg_tiles, ws, ws_bytes, stream, false);
+}
+
+int bass_lift_counted(const uint64_t* x, const double* a, int64_t n_max, int nq, int j, const double* G, int mode,
+                      uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws, size_t ws_bytes,
+                      void* stream) {
+  if (mode == MODE_MOVE) return fail(BASS_E_ARG, "bass_lift_counted: gates and rotations only");
+  return lift_impl(x, a, n_max, nq, j, G, mode, xo, ao, cap_out, seg_tiles, ws, ws_bytes, stream, true);
+}
+
+int bass_set_count(void* ws, size_t ws_bytes, int64_t n, void* stream) {
+  if (n < 0) return fail(BASS_E_ARG, "bass_set_count: bad n %lld", (long long)n);
+  WS_REQUIRE(0);
+  fill(at<void>(ws, L.count_slot), 8, (unsigned long long)n, (hipStream_t)stream);
+  return hip_check("bass_set_count");
+}
+
+int bass_counted_read(void* ws, size_t ws_bytes, bass_stats* out, int64_t max_entries, int64_t* n_entries,
+                      void* stream) {
+  WS_REQUIRE(0);
+  hipStream_t s = (hipStream_t)stream;
+  unsigned long long n = 0;
+  if (hipMemcpyAsync(&n, at<void>(ws, L.oplog), 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail(BASS_E_HIP, "bass_counted_read: %s", hipGetErrorString(hipGetLastError()));
+  *n_entries = (int64_t)n;
+  if ((int64_t)n > OPLOG_CAP) return fail(BASS_E_OVERFLOW, "bass_counted_read: op log overflowed (%llu entries)", n);
+  if ((int64_t)n > max_entries) return fail(BASS_E_OVERFLOW, "bass_counted_read: %llu entries to read", n);
+  if (n > 0 && (hipMemcpyAsync(out, at<char>(ws, L.oplog) + 8, n * sizeof(OpStats), hipMemcpyDeviceToHost, s) !=
+                    hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+    return fail(BASS_E_HIP, "bass_counted_read: %s", hipGetErrorString(hipGetLastError()));
+  fill(at<void>(ws, L.oplog), 8, 0ull, s);
+  return BASS_OK;
+}
+
+static int lift_impl(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G, int mode,
+                     uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws, size_t ws_bytes,
+                     void* stream, bool counted) {
   if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_lift: nq must be 1 or 2");
   if (mode != MODE_GATE && mode != MODE_ROT && mode != MODE_MOVE) return fail(BASS_E_ARG, "bass_lift: bad mode");
   if (mode == MODE_MOVE && nq != 1) return fail(BASS_E_ARG, "bass_lift: a move lifts one bit");
@@ -361,13 +448,15 @@ int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, cons
   if (mode != MODE_MOVE) {
     const int M = 1 << nq;
     for (int e = 0; e < M * M; ++e) gm.m[e] = make_double2(G[2 * e], G[2 * e + 1]);
-    fill(at<void>(ws, L.hist1), HIST_BINS * 4, 0ull, s);
+    zero2(at<void>(ws, L.hist1), HIST_BINS * 4, st, 48, s);  // histogram + stats (err untouched)
+  } else {
+    fill(st, 48, 0ull, s);
   }
-  fill(st, 48, 0ull, s);
   if (n > 0) {
     int* seg = at<int>(ws, L.seg_cnt);
     launch_lift(nq, mode, x, (const double2*)a, n, j, gm, xo, (double2*)ao, at<LiftBound>(ws, L.lift_b),
-                at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), seg, s);
+                at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), seg, s,
+                counted ? at<int64_t>(ws, L.count_slot) : nullptr);
     if (mode != MODE_MOVE)
       launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), seg_chunk_cnt(ws, L), seg_chunk_off(ws, L), st, s);
   }
@@ -391,6 +480,7 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
                   int key_bits, void* ws, size_t ws_bytes, void* stream) {
   const bool segmented = (in_flags & 1) != 0;
   const bool wide = (in_flags & 2) != 0;
+  const bool counted = (in_flags & 4) != 0;
   if (k < 1 || (!segmented && n <= k))
     return fail(BASS_E_ARG, "bass_truncate: need n > k >= 1 (n=%lld k=%lld)", (long long)n, (long long)k);
   if (cap_out < k) return fail(BASS_E_OVERFLOW, "bass_truncate: cap_out < k");
@@ -434,6 +524,7 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   P.nrows = select_rows(tiles, false);
   P.npsum = hist_tiles > 0 ? hist_tiles : tiles;
   P.chunked = sel_chunks(P.npsum > tiles ? P.npsum : tiles) > 1 ? 1 : 0;
+  P.finish_any = counted ? 1 : 0;
   P.k = (unsigned long long)k;
   P.xo = xo;
   P.ao = (double2*)ao;
@@ -452,6 +543,10 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   const int tk = ktimer_begin("select", s);
   launch_select(P, T, hist_tiles > 0, wide, s);
   ktimer_end(tk, s);
+  if (counted)
+    hipLaunchKernelGGL(k_oplog_append, dim3(1), dim3(1), 0, s, (const OpStats*)st,
+                       at<unsigned long long>(ws, L.oplog), (OpStats*)(at<char>(ws, L.oplog) + 8),
+                       at<int64_t>(ws, L.count_slot));
   return hip_check("bass_truncate");
 }
 

@@ -86,7 +86,7 @@ int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, int njobs, int64_
 int lift_tile_entries(int qb, int mode);
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
-                hipStream_t s);
+                hipStream_t s, const int64_t* n_dev = nullptr);
 
 // One state of a batched elementwise operation (see launch_ew_batch):
 // op 0 diagonal 2-qubit gate a *= d[(b_j1 << 1) | b_j2], op 1 a /= div,
@@ -147,6 +147,8 @@ struct SelProblem {
   int64_t cap_out;
   int renorm;
   int chunked;           // per-tile reductions run by chunk (single large problems)
+  int finish_any;        // k_finish resolves any candidate count itself (no wide-path hand-off)
+  int pad;
   unsigned int* hist1;   // HIST_BINS, zeroed by the caller unless filled by the producing gate
   unsigned int* hist2;   // HIST_BINS, zeroed
   unsigned int* hist_w;  // HIST_BINS, zeroed (wide path)

@@ -69,6 +69,15 @@ __device__ __forceinline__ int64_t wave_partition_point(int64_t lo, int64_t hi, 
 
 // The run around merged position d (clamped to n) and the merge split there.
 // Called by all 64 lanes of a wave.
+// Input count of a counted lift: the device slot, clamped to [0, n_max] (the
+// host's bound, which the grid and the buffers are sized for).  n_dev null:
+// n_max is the count.
+__device__ __forceinline__ int64_t dev_count(const int64_t* n_dev, int64_t n_max) {
+  if (!n_dev) return n_max;
+  const int64_t v = *n_dev;
+  return v < 0 ? 0 : (v < n_max ? v : n_max);
+}
+
 __device__ __forceinline__ LiftBound lift_boundary(const uint64_t* __restrict__ w, int64_t n, int j, int t, int64_t d) {
   if (d > n) d = n;
   LiftBound r;
@@ -99,7 +108,9 @@ __device__ __forceinline__ LiftBound lift_boundary(const uint64_t* __restrict__ 
 // Tile boundaries of the lift of bit j0 + blockIdx.y (one bit per grid row;
 // row r writes bounds[r * nbound + u]).
 __global__ __launch_bounds__(NT) void k_lift_part(const uint64_t* __restrict__ w, int64_t n, int j0, int t,
-                                                   int tm, int64_t nbound, LiftBound* bounds) {
+                                                   int tm, int64_t nbound, LiftBound* bounds,
+                                                   const int64_t* __restrict__ n_dev) {
+  n = dev_count(n_dev, n);
   const int lane = threadIdx.x & 63;
   const int j = j0 + (int)blockIdx.y;
   bounds += (int64_t)blockIdx.y * nbound;
@@ -155,7 +166,9 @@ constexpr int LIFT_CS = 16;
 __host__ __device__ inline int64_t lift_coarse_count(int64_t nbound) { return (nbound - 1 + LIFT_CS - 1) / LIFT_CS + 1; }
 
 __global__ __launch_bounds__(NT) void k_lift_part_coarse(const uint64_t* __restrict__ w, int64_t n, int j0, int t,
-                                                          int tm, int64_t nbound, LiftBound* coarse) {
+                                                          int tm, int64_t nbound, LiftBound* coarse,
+                                                          const int64_t* __restrict__ n_dev) {
+  n = dev_count(n_dev, n);
   const int lane = threadIdx.x & 63;
   const int j = j0 + (int)blockIdx.y;
   const int64_t nc = lift_coarse_count(nbound);
@@ -170,7 +183,8 @@ __global__ __launch_bounds__(NT) void k_lift_part_coarse(const uint64_t* __restr
 
 __global__ __launch_bounds__(NT) void k_lift_part_fine(const uint64_t* __restrict__ w, int64_t n, int j0, int t,
                                                         int tm, int64_t nbound, const LiftBound* __restrict__ coarse,
-                                                        LiftBound* bounds) {
+                                                        LiftBound* bounds, const int64_t* __restrict__ n_dev) {
+  n = dev_count(n_dev, n);
   const int lane = threadIdx.x & 63;
   const int j = j0 + (int)blockIdx.y;
   const int64_t nc = lift_coarse_count(nbound);
@@ -195,17 +209,17 @@ __global__ __launch_bounds__(NT) void k_lift_part_fine(const uint64_t* __restric
 // coarse pass uses `coarse` (lift_coarse_count(nbound) per row).  Small
 // lifts take the direct pass.
 static void lift_partition(const uint64_t* x, int64_t n, int j0, int rows, int t, int tm, int64_t nbound,
-                           LiftBound* bounds, LiftBound* coarse, hipStream_t s) {
+                           LiftBound* bounds, LiftBound* coarse, hipStream_t s, const int64_t* n_dev = nullptr) {
   if (nbound <= 4 * LIFT_CS || coarse == nullptr) {
     hipLaunchKernelGGL(k_lift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES), (unsigned)rows), dim3(NT), 0, s,
-                       x, n, j0, t, tm, nbound, bounds);
+                       x, n, j0, t, tm, nbound, bounds, n_dev);
     return;
   }
   const int64_t nc = lift_coarse_count(nbound);
   hipLaunchKernelGGL(k_lift_part_coarse, dim3((unsigned)((nc + WAVES - 1) / WAVES), (unsigned)rows), dim3(NT), 0, s,
-                     x, n, j0, t, tm, nbound, coarse);
+                     x, n, j0, t, tm, nbound, coarse, n_dev);
   hipLaunchKernelGGL(k_lift_part_fine, dim3((unsigned)((nbound + WAVES - 1) / WAVES), (unsigned)rows), dim3(NT), 0,
-                     s, x, n, j0, t, tm, nbound, (const LiftBound*)coarse, bounds);
+                     s, x, n, j0, t, tm, nbound, (const LiftBound*)coarse, bounds, n_dev);
 }
 
 __device__ __forceinline__ int lds_lower_bound(const uint64_t* s, int cnt, uint64_t v) {
@@ -531,7 +545,11 @@ template <int QB, int MODE>
 __global__ __launch_bounds__(NT) void k_lift_gate(const uint64_t* __restrict__ w, const double2* __restrict__ a,
                                                    int64_t n, int j, const LiftBound* __restrict__ bounds,
                                                    GateMat G, uint64_t* __restrict__ xo, double2* __restrict__ ao,
-                                                   unsigned int* hist, double* tile_psum, int* seg_cnt) {
+                                                   unsigned int* hist, double* tile_psum, int* seg_cnt,
+                                                   const int64_t* __restrict__ n_dev) {
+  // tiles past a counted input's end have empty bounds: they write a zero
+  // count and sum
+  n = dev_count(n_dev, n);
   const int64_t tile = blockIdx.x;
   lift_gate_tile<QB, MODE>(w, a, n, j, bounds[tile], bounds[tile + 1], G, tile, tile, xo, ao, hist, tile_psum,
                            seg_cnt);
@@ -1057,31 +1075,31 @@ int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, int njobs, int64_
 
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
-                hipStream_t s) {
+                hipStream_t s, const int64_t* n_dev) {
   if (n <= 0) return 0;
   const int tm = lift_tile_entries(qb, mode);
   const int t = (mode == MODE_MOVE) ? 0 : qb - 1;
   const int64_t tiles = (n + tm - 1) / tm;
   const int64_t nbound = tiles + 1;
   // the coarse boundaries follow the nbound boundaries (lift_bounds_entries)
-  lift_partition(x, n, j, 1, t, tm, nbound, bounds, bounds + nbound, s);
+  lift_partition(x, n, j, 1, t, tm, nbound, bounds, bounds + nbound, s, n_dev);
   const dim3 grid((unsigned)tiles);
   if (mode == MODE_MOVE) {
     hipLaunchKernelGGL(k_lift_move<1024>, grid, dim3(NT), 0, s, x, a, n, j, bounds, xo, ao);
   } else if (qb == 1) {
     if (mode == MODE_GATE)
       hipLaunchKernelGGL((k_lift_gate<1, MODE_GATE>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
-                         tile_psum, seg_cnt);
+                         tile_psum, seg_cnt, n_dev);
     else
       hipLaunchKernelGGL((k_lift_gate<1, MODE_ROT>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
-                         tile_psum, seg_cnt);
+                         tile_psum, seg_cnt, n_dev);
   } else {
     if (mode == MODE_GATE)
       hipLaunchKernelGGL((k_lift_gate<2, MODE_GATE>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
-                         tile_psum, seg_cnt);
+                         tile_psum, seg_cnt, n_dev);
     else
       hipLaunchKernelGGL((k_lift_gate<2, MODE_ROT>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
-                         tile_psum, seg_cnt);
+                         tile_psum, seg_cnt, n_dev);
   }
   return (int)hipGetLastError();
 }

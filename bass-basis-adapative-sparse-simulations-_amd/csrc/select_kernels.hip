@@ -489,7 +489,7 @@ __global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProb
   OpStats* st = P.st;
   if (sel->keepall) return;  // pstar / keystar set by k_find1
   const unsigned int m2 = sel->cand2;
-  if (m2 > FIN_MAX) {
+  if (m2 > FIN_MAX && !P.finish_any) {
     // too many for one block: keep nothing, ask for the wide path
     if (threadIdx.x == 0) {
       sel->pstar = ~0ull;

@@ -248,7 +248,7 @@ class BASS:
     def __init__(self, num_qubits, k, optimize_every=5, buffer_factor=2, truncate_every=1,
                  max_nnz_factor=8, use_2qubit_rotations=False, verbose=False, use_fast_eigh=True,
                  use_diag_gate=True, use_kron_cache=True, pr_opt_threshold=0.90, device=None,
-                 optimize_at=None, trace=None, accept_rtol=1e-10):
+                 optimize_at=None, trace=None, accept_rtol=1e-10, device_counts=True):
         self.N = num_qubits
         self.k = k
         self.optimize_every = optimize_every
@@ -267,13 +267,58 @@ class BASS:
         self._pr_at_last_opt = 0.0
         self.optimize_at = None if optimize_at is None else frozenset(int(i) for i in optimize_at)
         self._ctx = context(device)
-        self.counters = {}
+        self._counters = {}
         self.trace = trace  # optional list: decision log, same records as the oracle's
         self.accept_rtol = float(accept_rtol)
+        # counted 1-qubit gates: the device keeps the state's entry count and an
+        # op log; the host reads them at its next decision point (no round trip
+        # per gate).  Off with a trace (which records every truncation as it
+        # happens) and for truncate_every > 1.
+        self.device_counts = bool(device_counts)
+        self._cpending = []
+        self._cstart_n = 0
+
+    @property
+    def counters(self):
+        """Operation counters (gates, truncations, amp-updates, algorithmic bytes)."""
+        self._counted_flush()
+        return self._counters
 
     # helpers ----------------------------------------------------------------
     def _count(self, key, v=1):
-        self.counters[key] = self.counters.get(key, 0) + v
+        self._counters[key] = self._counters.get(key, 0) + v
+
+    def _counted_flush(self):
+        """Read the op log of the pending counted gates and book them exactly as
+        the per-gate path (_fast_truncate) does, in order."""
+        if not self._cpending:
+            return
+        state, pend = self.state, self._cpending
+        self._cpending = []
+        recs = _ops.counted_read(self._ctx, len(pend))
+        if len(recs) != len(pend):
+            raise L.BassError(f"counted gates: {len(pend)} launched, {len(recs)} logged")
+        if any(r.err for r in recs):
+            self._ctx.read_stats("counted gates")  # raises with the device error bits
+        n_prev = self._cstart_n
+        for (i, op), r in zip(pend, recs):
+            n = int(r.n_in)
+            self._bytes(op, n_prev, n)
+            self._count("amp_updates", n)
+            self._since_trunc += 1
+            if r.flags & 1:
+                self._trunc_active = True
+                self._bytes("select", n, self.k)
+                total, kept = r.total, r.kept
+                self._gamma_running *= kept / total if total > 1e-30 else 0
+                self._since_trunc = 0
+                self._count("truncate")
+            else:
+                self._count("bytes_select", 48 * n)
+            n_prev = int(r.nnz)
+        state.nnz = n_prev
+        state._hist_tiles = 0
+        state._sum2, state._sum4 = recs[-1].sum2, recs[-1].sum4
 
     def _bytes(self, op, n_in, n_out):
         """Algorithmic HBM bytes of one device operation (24 B per key + complex128 record):
@@ -531,7 +576,8 @@ class BASS:
         self.U = [np.eye(2, dtype=np.complex128) for _ in range(self.N)]
         self._kron_cache = {}
         self._pr_at_last_opt = 0.0
-        self.counters = {"amp_updates": 0}
+        self._counters = {"amp_updates": 0}
+        self._cpending = []
         self._hard_cap = min(self.max_nnz_factor * self.k, 2 ** self.N)
         internal_k = min(max(max(self.buffer_factor, 2) * self.k, self._hard_cap), 2 ** self.N)
         # buffers start at the truncation size and grow on demand (internal_k
@@ -557,6 +603,34 @@ class BASS:
         a segmented input holding <= k), so a gate and its truncation cost one
         host round trip."""
         state, ctx, M = self.state, self._ctx, self._M
+        counted = (gate.n_qubits == 1 and self.device_counts and self.trace is None and self.truncate_every == 1
+                   and 2 * self.k <= state.k)
+        if counted:
+            # a counted gate: the state holds <= k entries after every gate
+            # (truncate iff more than k), so k bounds the input count
+            q = gate.qubits[0]
+            G = self.U[q].conj().T @ gate.matrix @ self.U[q]
+            if not self._cpending:
+                self._cstart_n = state.nnz
+                _ops.counted_begin(ctx, state.nnz, self.k, nq=1)
+            jb = int(state.pos[q])
+            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=1)
+            state.pos = lifted_positions(state.pos, jb, 0)
+            self._count("gate1q")
+            _ops.truncate_counted(ctx, state._tmp, self.k, state._buf, ht, key_map=_ops.key_map_arg(state.pos))
+            self._cpending.append((i, "gate1q"))
+            state.nnz = None  # known at the next flush
+            state._sum2 = state._sum4 = None
+            if len(self._cpending) >= 4000:
+                self._counted_flush()
+            if self._check_point(i, M):
+                self._counted_flush()
+                if self._trunc_active and self.optimize_trigger(state):
+                    self._optimize_basis(state)
+                    self._pr_at_last_opt = self._pr(state)
+                self._trunc_active = False
+            return
+        self._counted_flush()
         if gate.n_qubits == 1:
             q = gate.qubits[0]
             G = self.U[q].conj().T @ gate.matrix @ self.U[q]
@@ -692,6 +766,7 @@ class BASS:
         return n, ht
 
     def end(self):
+        self._counted_flush()
         state = self.state
         state.gamma = np.sqrt(max(self._gamma_running, 0))
         return state

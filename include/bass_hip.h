@@ -131,6 +131,27 @@ int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, cons
               int mode, uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws,
               size_t ws_bytes, void* stream);
 
+/* bass_lift, with the input count read on the device from the workspace's
+ * state-count slot (written by a counted bass_truncate or bass_set_count) and
+ * clamped to [0, n_max]; grids, output tiles (*seg_tiles) and buffer checks
+ * follow n_max, tiles past the actual count come out empty.  Gates and
+ * rotations only.  With counted truncations this runs a gate loop without a
+ * host round trip per gate (the driver reads the op log at its decision
+ * points).  No counterpart in the reference. */
+int bass_lift_counted(const uint64_t* x, const double* a, int64_t n_max, int nq, int j, const double* G,
+                      int mode, uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws,
+                      size_t ws_bytes, void* stream);
+
+/* Write n into the workspace's state-count slot (asynchronous). */
+int bass_set_count(void* ws, size_t ws_bytes, int64_t n, void* stream);
+
+/* Copy the op log of counted truncations since the last read (one bass_stats
+ * per truncation, in call order; at most 4096 between reads) into out
+ * (max_entries), its length into *n_entries, and empty it.  Synchronous.
+ * BASS_E_OVERFLOW if the log overflowed or does not fit out. */
+int bass_counted_read(void* ws, size_t ws_bytes, bass_stats* out, int64_t max_entries, int64_t* n_entries,
+                      void* stream);
+
 /* Top-k truncation: keep the k entries of largest |a|^2 (ties: smaller key),
  * compacted in input order into (xo, ao); renorm=1 divides them by
  * sqrt(sum kept |a|^2).  Replaces top_k_truncate_kernel_opt
@@ -146,6 +167,10 @@ int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, cons
  * call without it whose threshold bucket holds more than 65536 entries
  * (massive |a|^2 ties) keeps nothing and sets stats.flags bit 1; repeat it
  * with bit 1 set (the ctypes layer does).
+ * in_flags bit 2 (counted): resolve any threshold bucket in the call itself
+ * (one block; never asks for a repeat), then append the stats block to the
+ * workspace's op log and set its state-count slot to the output count, for
+ * bass_lift_counted and bass_counted_read.
  * key_map (key_bits entries, or NULL): stored bit p of x holds qubit
  * key_map[p]; ties are then ordered by the key mapped back. */
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo,

@@ -189,3 +189,28 @@ def test_tfim_config0_lockstep(gpu_ctx):
         np.testing.assert_allclose(ga[gi], o.alpha[: o.nnz][oi], rtol=1e-12, atol=1e-15)
     print(f"  truncations {truncs}, steps with a near-tie swap at the threshold {flips}", flush=True)
     assert truncs == sum(e[0] == "trunc" for e in td) and truncs > 700
+
+
+@pytest.mark.parametrize("idx,K", [(0, 64), (1, 256), (1, 64), (2, 32)])
+def test_device_counts_match_host_counts(gpu_ctx, idx, K):
+    """Counted 1-qubit gates (input count kept on the device, op log read at
+    the driver's decision points) and the per-gate host round trip produce the
+    same state bit for bit, the same frame, gamma and counters."""
+    name, N, circ, ends = _circuits()[idx]
+    print(f"[case] device counts {name} N={N} K={K}", flush=True)
+    B = pkg_module("simulation.bass_simulator")
+    dev = pkg_module("_device")
+    res = []
+    for counted in (False, True):
+        kw = {"optimize_at": ends} if ends is not None else {"optimize_every": 5}
+        # start from an empty workspace: the first counted gate grows it
+        gpu_ctx.ws, gpu_ctx.ws_bytes = None, 0
+        sim = B.BASS(N, K, device_counts=counted, **kw)
+        st = sim.simulate(circ)
+        res.append((dev.to_host_keys(st._buf.x, st.nnz).copy(), dev.to_host_amps(st._buf.a, st.nnz).copy(),
+                    st.gamma, [u.copy() for u in sim.U], st.pos.copy(), dict(sim.counters)))
+    (x0, a0, g0, u0, p0, c0), (x1, a1, g1, u1, p1, c1) = res
+    assert np.array_equal(x0, x1) and np.array_equal(a0.view(np.float64), a1.view(np.float64))
+    assert g0 == g1 and np.array_equal(p0, p1) and all(np.array_equal(a, b) for a, b in zip(u0, u1))
+    assert c0 == c1, (c0, c1)
+    assert c1.get("truncate", 0) > 0 and c1.get("gate1q", 0) > 0
