@@ -75,16 +75,17 @@ def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE, count=True):
     return ctx.read_stats("bass_lift").nnz, int(st.value)
 
 
-def lift_counted(ctx, src, n_max, j, dst, G, nq=1, mode=L.MODE_GATE):
+def lift_counted(ctx, src, n_max, j, dst, G=None, nq=1, mode=L.MODE_GATE):
     """lift() whose input count is the workspace's state-count slot (<= n_max),
-    read on the device: nothing waits for the device.  Returns the output's
-    tile count (its segmented output goes to a counted truncate)."""
-    tm = SEG // (1 << nq)
+    read on the device: nothing waits for the device.  Gates: returns the
+    output's tile count (its segmented output goes to a counted truncate);
+    a move (MODE_MOVE) copies the state densely and returns 0."""
+    tm = 1024 if mode == L.MODE_MOVE else SEG // (1 << nq)
     tiles = (n_max + tm - 1) // tm
-    need = tiles * SEG
+    need = n_max if mode == L.MODE_MOVE else tiles * SEG
     ctx.ensure_ws(max(need, n_max, 1))
     dst.ensure(max(need, 1))
-    buf, gp = L.mat_arg(G)
+    buf, gp = (None, None) if mode == L.MODE_MOVE else L.mat_arg(G)
     st = ctypes.c_int64(0)
     rc = ctx.lib.bass_lift_counted(src.xp, src.ap, n_max, nq, int(j), gp, mode, dst.xp, dst.ap, dst.cap,
                                    ctypes.byref(st), ctx.wsp, ctx.ws_bytes, ctx.stream)
@@ -108,11 +109,20 @@ def truncate_counted(ctx, src, k, dst, hist_tiles, key_map=None):
     L.check(rc, "bass_truncate")
 
 
-def counted_begin(ctx, n, n_max, nq=1):
+def diag2q_counted(ctx, buf, n_max, q1, q2, diag):
+    """diag2q() on the state's count from the state-count slot (<= n_max)."""
+    d = np.ascontiguousarray(np.asarray(diag, dtype=np.complex128)).view(np.float64)
+    rc = ctx.lib.bass_diag2q_counted(buf.xp, buf.ap, int(n_max), int(q1), int(q2), d.ctypes.data_as(ctypes.c_void_p),
+                                     ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_diag2q_counted")
+
+
+def counted_begin(ctx, n, n_max):
     """Start a run of counted gates on a state of n entries (<= n_max): size the
-    workspace for every gate of the run first (growing it later would drop the
-    count slot and the op log), then write n into the state-count slot."""
-    tm = SEG // (1 << nq)
+    workspace for every operation of the run first (the largest is a 2-qubit
+    gate's 4 n_max output slots; growing it later would drop the count slot
+    and the op log), then write n into the state-count slot."""
+    tm = SEG // 4
     ctx.ensure_ws(max(((n_max + tm - 1) // tm) * SEG, n_max, 1))
     L.check(ctx.lib.bass_set_count(ctx.wsp, ctx.ws_bytes, int(n), ctx.stream), "bass_set_count")
 

@@ -394,8 +394,20 @@ int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, cons
 int bass_lift_counted(const uint64_t* x, const double* a, int64_t n_max, int nq, int j, const double* G, int mode,
                       uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws, size_t ws_bytes,
                       void* stream) {
-  if (mode == MODE_MOVE) return fail(BASS_E_ARG, "bass_lift_counted: gates and rotations only");
   return lift_impl(x, a, n_max, nq, j, G, mode, xo, ao, cap_out, seg_tiles, ws, ws_bytes, stream, true);
+}
+
+int bass_diag2q_counted(const uint64_t* x, double* a, int64_t n_max, int q1, int q2, const double* diag, void* ws,
+                        size_t ws_bytes, void* stream) {
+  if (q1 < 0 || q1 > 62 || q2 < 0 || q2 > 62 || q1 == q2) return fail(BASS_E_ARG, "bass_diag2q_counted: bad qubits");
+  if (n_max < 0 || n_max > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_diag2q_counted: bad n %lld", (long long)n_max);
+  WS_REQUIRE(0);
+  double2 d[4];
+  for (int e = 0; e < 4; ++e) d[e] = make_double2(diag[2 * e], diag[2 * e + 1]);
+  const int tk = ktimer_begin("diag2q", (hipStream_t)stream);
+  launch_diag2q(x, (double2*)a, n_max, q1, q2, d, (hipStream_t)stream, at<int64_t>(ws, L.count_slot));
+  ktimer_end(tk, (hipStream_t)stream);
+  return hip_check("bass_diag2q_counted");
 }
 
 int bass_set_count(void* ws, size_t ws_bytes, int64_t n, void* stream) {

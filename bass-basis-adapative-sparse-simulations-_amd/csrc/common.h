@@ -232,6 +232,15 @@ __device__ inline unsigned long long lookback_prefix_wave(unsigned long long* st
   return excl;
 }
 
+// Input count of a counted operation: the device slot, clamped to [0, n_max]
+// (the host's bound, which the grid and the buffers are sized for).  n_dev
+// null: n_max is the count.
+__device__ __forceinline__ int64_t dev_count(const int64_t* n_dev, int64_t n_max) {
+  if (!n_dev) return n_max;
+  const int64_t v = *n_dev;
+  return v < 0 ? 0 : (v < n_max ? v : n_max);
+}
+
 // Tile id in dispatch order.
 __device__ __forceinline__ unsigned int take_ticket(unsigned int* ticket, unsigned int* lds_slot) {
   if (threadIdx.x == 0) *lds_slot = atomicAdd(ticket, 1u);

@@ -280,7 +280,8 @@ __global__ __launch_bounds__(NT) void k_emit(const uint64_t* __restrict__ x,
 // alpha[i] *= diag[(b_q1 << 1) | b_q2].
 __global__ __launch_bounds__(NT) void k_diag2q(const uint64_t* __restrict__ x, double2* __restrict__ a,
                                                 int64_t n, int q1, int q2, double2 d0, double2 d1,
-                                                double2 d2, double2 d3) {
+                                                double2 d2, double2 d3, const int64_t* __restrict__ n_dev) {
+  n = dev_count(n_dev, n);
   const int64_t stride = (int64_t)gridDim.x * NT;
   for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) {
     const uint64_t key = x[i];
@@ -343,12 +344,12 @@ int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* 
 }
 
 int launch_diag2q(const uint64_t* x, double2* a, int64_t n, int q1, int q2, const double2* d,
-                  hipStream_t s) {
+                  hipStream_t s, const int64_t* n_dev) {
   if (n <= 0) return 0;
   int64_t blocks = (n + NT - 1) / NT;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(k_diag2q, dim3((unsigned)blocks), dim3(NT), 0, s, x, a, n, q1, q2, d[0], d[1], d[2],
-                     d[3]);
+                     d[3], n_dev);
   return (int)hipGetLastError();
 }
 

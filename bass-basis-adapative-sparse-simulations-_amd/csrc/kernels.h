@@ -60,7 +60,7 @@ int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* 
 // Every emission tile covers SEG_ENTRIES output slots (1q: 1024 inputs x 2, 2q: 512 x 4).
 constexpr int SEG_ENTRIES = 2048;
 int launch_diag2q(const uint64_t* x, double2* a, int64_t n, int q1, int q2, const double2* d,
-                  hipStream_t s);
+                  hipStream_t s, const int64_t* n_dev = nullptr);
 
 // Key-ordered state: merge split of the run around one tile boundary.
 struct LiftBound {

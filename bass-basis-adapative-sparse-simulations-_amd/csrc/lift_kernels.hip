@@ -69,15 +69,6 @@ __device__ __forceinline__ int64_t wave_partition_point(int64_t lo, int64_t hi, 
 
 // The run around merged position d (clamped to n) and the merge split there.
 // Called by all 64 lanes of a wave.
-// Input count of a counted lift: the device slot, clamped to [0, n_max] (the
-// host's bound, which the grid and the buffers are sized for).  n_dev null:
-// n_max is the count.
-__device__ __forceinline__ int64_t dev_count(const int64_t* n_dev, int64_t n_max) {
-  if (!n_dev) return n_max;
-  const int64_t v = *n_dev;
-  return v < 0 ? 0 : (v < n_max ? v : n_max);
-}
-
 __device__ __forceinline__ LiftBound lift_boundary(const uint64_t* __restrict__ w, int64_t n, int j, int t, int64_t d) {
   if (d > n) d = n;
   LiftBound r;
@@ -676,7 +667,9 @@ __device__ __forceinline__ void lift_move_tile(const uint64_t* __restrict__ w, c
 template <int TM>
 __global__ __launch_bounds__(NT) void k_lift_move(const uint64_t* __restrict__ w, const double2* __restrict__ a,
                                                    int64_t n, int j, const LiftBound* __restrict__ bounds,
-                                                   uint64_t* __restrict__ xo, double2* __restrict__ ao) {
+                                                   uint64_t* __restrict__ xo, double2* __restrict__ ao,
+                                                   const int64_t* __restrict__ n_dev) {
+  n = dev_count(n_dev, n);
   lift_move_tile<TM>(w, a, n, j, bounds[blockIdx.x], bounds[blockIdx.x + 1], blockIdx.x, xo, ao);
 }
 
@@ -1085,7 +1078,7 @@ int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n
   lift_partition(x, n, j, 1, t, tm, nbound, bounds, bounds + nbound, s, n_dev);
   const dim3 grid((unsigned)tiles);
   if (mode == MODE_MOVE) {
-    hipLaunchKernelGGL(k_lift_move<1024>, grid, dim3(NT), 0, s, x, a, n, j, bounds, xo, ao);
+    hipLaunchKernelGGL(k_lift_move<1024>, grid, dim3(NT), 0, s, x, a, n, j, bounds, xo, ao, n_dev);
   } else if (qb == 1) {
     if (mode == MODE_GATE)
       hipLaunchKernelGGL((k_lift_gate<1, MODE_GATE>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,

@@ -295,13 +295,27 @@ class BASS:
             return
         state, pend = self.state, self._cpending
         self._cpending = []
-        recs = _ops.counted_read(self._ctx, len(pend))
-        if len(recs) != len(pend):
-            raise L.BassError(f"counted gates: {len(pend)} launched, {len(recs)} logged")
+        nlog = sum(1 for e in pend if e[1] != "diag2q")  # diagonal gates truncate nothing
+        recs = _ops.counted_read(self._ctx, max(nlog, 1))
+        if len(recs) != nlog:
+            raise L.BassError(f"counted gates: {nlog} truncations launched, {len(recs)} logged")
         if any(r.err for r in recs):
             self._ctx.read_stats("counted gates")  # raises with the device error bits
         n_prev = self._cstart_n
-        for (i, op), r in zip(pend, recs):
+        sums = (state._sum2, state._sum4)
+        it = iter(recs)
+        for i, op, moved in pend:
+            if op == "diag2q":
+                # in place, count unchanged; moments recomputed on demand, as after
+                # a per-gate diagonal gate
+                self._bytes("diag2q", n_prev, n_prev)
+                self._count("amp_updates", n_prev)
+                sums = (None, None)
+                continue
+            if moved:
+                self._bytes("move", n_prev, n_prev)
+            r = next(it)
+            sums = (r.sum2, r.sum4)
             n = int(r.n_in)
             self._bytes(op, n_prev, n)
             self._count("amp_updates", n)
@@ -318,7 +332,7 @@ class BASS:
             n_prev = int(r.nnz)
         state.nnz = n_prev
         state._hist_tiles = 0
-        state._sum2, state._sum4 = recs[-1].sum2, recs[-1].sum4
+        state._sum2, state._sum4 = sums
 
     def _bytes(self, op, n_in, n_out):
         """Algorithmic HBM bytes of one device operation (24 B per key + complex128 record):
@@ -593,6 +607,57 @@ class BASS:
         self.state = state
         return state
 
+    def _counted_step(self, i, gate):
+        """One gate of a counted run: the state holds <= k entries after every
+        gate (truncate iff more than k), so k bounds every input count, which
+        the device reads from its count slot; the host books the run when it
+        next needs the state (_counted_flush)."""
+        state, ctx = self.state, self._ctx
+        if not self._cpending:
+            self._cstart_n = state.nnz
+            _ops.counted_begin(ctx, state.nnz, self.k)
+        km = None
+        if gate.n_qubits == 1:
+            q = gate.qubits[0]
+            G = self.U[q].conj().T @ gate.matrix @ self.U[q]
+            jb = int(state.pos[q])
+            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=1)
+            state.pos = lifted_positions(state.pos, jb, 0)
+            self._count("gate1q")
+            entry = (i, "gate1q", False)
+        else:
+            q1, q2 = gate.qubits
+            Ut = self._get_kron(q1, q2)
+            G = Ut.conj().T @ gate.matrix @ Ut
+            if self.use_diag_gate and _is_gate_diagonal(G):
+                _ops.diag2q_counted(ctx, state._buf, self.k, int(state.pos[q1]), int(state.pos[q2]),
+                                    np.diag(G).astype(np.complex128))
+                self._count("diag2q")
+                self._cpending.append((i, "diag2q", False))
+                state.nnz = None
+                state._sum2 = state._sum4 = None
+                return
+            # as _lift_2q: bring q2 to bit 0 (a move), lift q1's bit to bit 1 with the gate
+            if state.pos[q1] == 0 and state.pos[q2] != 0:
+                G = G[np.ix_(_SWAP_IDX, _SWAP_IDX)]
+                q1, q2 = q2, q1
+            moved = state.pos[q2] != 0
+            if moved:
+                jb = int(state.pos[q2])
+                _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, mode=L.MODE_MOVE)
+                state._buf, state._tmp = state._tmp, state._buf
+                state.pos = lifted_positions(state.pos, jb, 0)
+            jb = int(state.pos[q1])
+            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=2)
+            state.pos = lifted_positions(state.pos, jb, 1)
+            self._count("gate2q")
+            entry = (i, "gate2q", moved)
+        km = _ops.key_map_arg(state.pos)
+        _ops.truncate_counted(ctx, state._tmp, self.k, state._buf, ht, key_map=km)
+        self._cpending.append(entry)
+        state.nnz = None  # known at the next flush
+        state._sum2 = state._sum4 = None
+
     def step(self, i, gate):
         """One iteration of the reference loop (:861-926).  Gate outputs are
         written in per-tile segments of state._tmp; the truncation (or, without
@@ -603,24 +668,9 @@ class BASS:
         a segmented input holding <= k), so a gate and its truncation cost one
         host round trip."""
         state, ctx, M = self.state, self._ctx, self._M
-        counted = (gate.n_qubits == 1 and self.device_counts and self.trace is None and self.truncate_every == 1
-                   and 2 * self.k <= state.k)
-        if counted:
-            # a counted gate: the state holds <= k entries after every gate
-            # (truncate iff more than k), so k bounds the input count
-            q = gate.qubits[0]
-            G = self.U[q].conj().T @ gate.matrix @ self.U[q]
-            if not self._cpending:
-                self._cstart_n = state.nnz
-                _ops.counted_begin(ctx, state.nnz, self.k, nq=1)
-            jb = int(state.pos[q])
-            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=1)
-            state.pos = lifted_positions(state.pos, jb, 0)
-            self._count("gate1q")
-            _ops.truncate_counted(ctx, state._tmp, self.k, state._buf, ht, key_map=_ops.key_map_arg(state.pos))
-            self._cpending.append((i, "gate1q"))
-            state.nnz = None  # known at the next flush
-            state._sum2 = state._sum4 = None
+        if (self.device_counts and self.trace is None and self.truncate_every == 1 and 2 * self.k <= state.k
+                and gate.n_qubits in (1, 2)):
+            self._counted_step(i, gate)
             if len(self._cpending) >= 4000:
                 self._counted_flush()
             if self._check_point(i, M):

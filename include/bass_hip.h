@@ -134,13 +134,19 @@ int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, cons
 /* bass_lift, with the input count read on the device from the workspace's
  * state-count slot (written by a counted bass_truncate or bass_set_count) and
  * clamped to [0, n_max]; grids, output tiles (*seg_tiles) and buffer checks
- * follow n_max, tiles past the actual count come out empty.  Gates and
- * rotations only.  With counted truncations this runs a gate loop without a
- * host round trip per gate (the driver reads the op log at its decision
- * points).  No counterpart in the reference. */
+ * follow n_max, tiles past the actual count come out empty (a move copies
+ * the actual count and leaves the slot valid).  With counted truncations this
+ * runs a gate loop without a host round trip per gate (the driver reads the
+ * op log at its decision points).  No counterpart in the reference. */
 int bass_lift_counted(const uint64_t* x, const double* a, int64_t n_max, int nq, int j, const double* G,
                       int mode, uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws,
                       size_t ws_bytes, void* stream);
+
+/* bass_diag2q on the first min(count slot, n_max) entries (count from the
+ * workspace's state-count slot; the count is unchanged).  No counterpart in
+ * the reference. */
+int bass_diag2q_counted(const uint64_t* x, double* a, int64_t n_max, int q1, int q2, const double* diag,
+                        void* ws, size_t ws_bytes, void* stream);
 
 /* Write n into the workspace's state-count slot (asynchronous). */
 int bass_set_count(void* ws, size_t ws_bytes, int64_t n, void* stream);

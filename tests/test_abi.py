@@ -31,7 +31,7 @@ def lib():
 def test_header_declares_expected_surface():
     names = header_functions()
     for must in ["bass_gate", "bass_truncate", "bass_diag2q", "bass_rdm1q", "bass_radix_sort", "bass_lift_counted",
-                 "bass_set_count", "bass_counted_read",
+                 "bass_set_count", "bass_counted_read", "bass_diag2q_counted",
                  "bass_merge_sorted", "bass_ht_insert", "bass_ht_lookup", "bass_workspace_bytes"]:
         assert must in names
 
@@ -83,8 +83,10 @@ def test_argument_errors_without_gpu(lib):
     assert rc == L.BASS_E_OVERFLOW
     rc = lib.bass_radix_sort(None, None, 10, 0, None, 0, None)
     assert rc == L.BASS_E_ARG
-    rc = lib.bass_lift_counted(None, None, 10, 1, 3, G, 2, None, None, 40, ctypes.byref(ht), None, 0, None)
-    assert rc == L.BASS_E_ARG and "gates and rotations" in L.last_error()
+    rc = lib.bass_lift_counted(None, None, 10, 3, 3, G, 0, None, None, 40, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_ARG and "nq" in L.last_error()
+    rc = lib.bass_diag2q_counted(None, None, 10, 2, 2, G, None, 0, None)
+    assert rc == L.BASS_E_ARG and "qubits" in L.last_error()
     rc = lib.bass_set_count(None, 0, -1, None)
     assert rc == L.BASS_E_ARG and "bad n" in L.last_error()
     rc = lib.bass_radix_sort(None, None, 1 << 30, 40, None, 0, None)
