@@ -67,6 +67,29 @@ __device__ __forceinline__ int64_t wave_partition_point(int64_t lo, int64_t hi, 
   return lo + __popcll(__ballot(t));
 }
 
+// As wave_partition_point, but with 8 probes per step while the range exceeds
+// 64 entries: a step then touches 8 cache lines instead of 64, for about twice
+// the steps.  For searches whose range is already narrow (bounded by known
+// neighbours), where memory traffic rather than latency sets the cost.
+template <class P>
+__device__ __forceinline__ int64_t wave_partition_point8(int64_t lo, int64_t hi, P pred) {
+  const int lane = threadIdx.x & 63;
+  while (hi - lo > 64) {
+    const int64_t step = (hi - lo + 7) / 8;
+    const int64_t p = lo + (int64_t)lane * step;
+    const bool t = lane < 8 && p < hi && pred(p);
+    const int c = __popcll(__ballot(t));
+    if (c == 0) return lo;
+    const int64_t nlo = lo + (int64_t)(c - 1) * step + 1;
+    const int64_t nhi = lo + (int64_t)c * step;
+    lo = nlo;
+    hi = nhi < hi ? nhi : hi;
+  }
+  const int64_t p = lo + lane;
+  const bool t = p < hi && pred(p);
+  return lo + __popcll(__ballot(t));
+}
+
 // The run around merged position d (clamped to n) and the merge split there.
 // Called by all 64 lanes of a wave.
 __device__ __forceinline__ LiftBound lift_boundary(const uint64_t* __restrict__ w, int64_t n, int j, int t, int64_t d) {
@@ -77,8 +100,11 @@ __device__ __forceinline__ LiftBound lift_boundary(const uint64_t* __restrict__ 
   } else {
     const uint64_t bj = 1ull << j;
     const uint64_t hi = w[d] & ~((bj << 1) - 1ull);
-    const int64_t s0 = wave_partition_point(0, d, [&](int64_t i) { return w[i] < hi; });
-    const int64_t s1 = wave_partition_point(d + 1, n, [&](int64_t i) { return w[i] < hi + (bj << 1); });
+    // a run that starts at 0 or ends at n (one run over everything when bit j
+    // is the top stored bit) is seen from the end entries, without a search
+    const int64_t s0 = w[0] < hi ? wave_partition_point(0, d, [&](int64_t i) { return w[i] < hi; }) : 0;
+    const int64_t s1 =
+        w[n - 1] < hi + (bj << 1) ? n : wave_partition_point(d + 1, n, [&](int64_t i) { return w[i] < hi + (bj << 1); });
     const int64_t sm = wave_partition_point(s0, s1, [&](int64_t i) { return w[i] < (hi | bj); });
     const int64_t ld = d - s0, na = sm - s0, nb = s1 - sm;
     const int64_t lo = ld - nb > 0 ? ld - nb : 0;
@@ -114,7 +140,8 @@ __global__ __launch_bounds__(NT) void k_lift_part(const uint64_t* __restrict__ w
 // The same boundary when the boundaries D0 (at a merged position <= d) and
 // D1 (>= d) are known: run start and end, A/B split and both merge cursors
 // are monotone in the merged position, so every search is confined between
-// the two (empty when d lies in the run of both).
+// the two (empty when d lies in the run of both).  The confined searches take
+// 8 probes per step (wave_partition_point8).
 __device__ __forceinline__ LiftBound lift_boundary_between(const uint64_t* __restrict__ w, int64_t n, int j, int t,
                                                            int64_t d, const LiftBound& D0, const LiftBound& D1) {
   if (d > n) d = n;
@@ -125,18 +152,22 @@ __device__ __forceinline__ LiftBound lift_boundary_between(const uint64_t* __res
     const uint64_t bj = 1ull << j;
     const uint64_t hi = w[d] & ~((bj << 1) - 1ull);
     const int64_t top = d < (int64_t)D1.s0 ? d : (int64_t)D1.s0;
-    const int64_t s0 = wave_partition_point(D0.s0, top, [&](int64_t i) { return w[i] < hi; });
+    const int64_t s0 = wave_partition_point8(D0.s0, top, [&](int64_t i) { return w[i] < hi; });
     const int64_t lo1 = d + 1 > (int64_t)D0.s1 ? d + 1 : (int64_t)D0.s1;
-    const int64_t s1 = wave_partition_point(lo1, D1.s1, [&](int64_t i) { return w[i] < hi + (bj << 1); });
+    const int64_t s1 = wave_partition_point8(lo1, D1.s1, [&](int64_t i) { return w[i] < hi + (bj << 1); });
     const int64_t lom = s0 > (int64_t)D0.sm ? s0 : (int64_t)D0.sm;
     const int64_t him = s1 < (int64_t)D1.sm ? s1 : (int64_t)D1.sm;
-    const int64_t sm = wave_partition_point(lom, him, [&](int64_t i) { return w[i] < (hi | bj); });
+    const int64_t sm = wave_partition_point8(lom, him, [&](int64_t i) { return w[i] < (hi | bj); });
     const int64_t ld = d - s0, na = sm - s0, nb = s1 - sm;
     int64_t lo = ld - nb > 0 ? ld - nb : 0;
     int64_t hh = ld < na ? ld : na;
+    // A cursor a = s0 + i and B cursor b = sm + ld - i both lie between the
+    // neighbours' cursors (a neighbour in another run bounds nothing here).
     if ((int64_t)D0.a - s0 > lo) lo = (int64_t)D0.a - s0;
     if ((int64_t)D1.a - s0 < hh) hh = (int64_t)D1.a - s0;
-    const int64_t i = wave_partition_point(
+    if (ld - ((int64_t)D1.b - sm) > lo) lo = ld - ((int64_t)D1.b - sm);
+    if (ld - ((int64_t)D0.b - sm) < hh) hh = ld - ((int64_t)D0.b - sm);
+    const int64_t i = wave_partition_point8(
         lo, hh, [&](int64_t q) { return lift_key(w[s0 + q], j, t) < lift_key(w[sm + ld - 1 - q], j, t); });
     r.s0 = (int)s0;
     r.sm = (int)sm;

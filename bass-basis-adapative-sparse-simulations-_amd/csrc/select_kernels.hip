@@ -125,7 +125,8 @@ __device__ double block_sum_array(const double* v, int64_t m, double* lds) {
 }
 
 constexpr int EX1_TILES = 4;         // tiles per k_extract1 block of a single problem
-constexpr int EX1_TILES_LARGE = 16;  // the same for large problems (row counts still fit 16 bits)
+constexpr int EX1_TILES_MIN = 12;    // range for large problems (row counts <= 31 * SEL_TILE fit 16 bits)
+constexpr int EX1_TILES_MAX = 31;
 int extract1_tiles(int64_t ntiles);
 
 // ------------------------------------------------------ problem lookup ----
@@ -251,22 +252,103 @@ __global__ __launch_bounds__(NTB) void k_find1(const SelProblem* probs, SelProbl
   }
 }
 
-// Per tile (EXT consecutive tiles per block, all of one problem): count / sum
-// above the digit-1 bucket, candidates inside it, and the block's digit-2
-// histogram as one row of hist2_rows.
+// Amplitudes of tile gt (slots past its count read its first entry, unused).
+__device__ __forceinline__ void ex1_load(const SelProblem& P, int64_t gt, double (&re)[SEL_ITEMS],
+                                         double (&im)[SEL_ITEMS], int& cnt) {
+  cnt = tile_count(P, gt);
+  const int64_t base = tile_base(P, gt);
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int j = it * NT + threadIdx.x;
+    const double2 v = P.a[base + (j < cnt ? j : 0)];
+    re[it] = v.x;
+    im[it] = v.y;
+  }
+}
+
+// One tile of k_extract1 from its |a|^2 values vp: count / sum above the
+// digit-1 bucket b1, candidates inside it (in input order: item-major, then
+// thread order) and their digit-2 counts into s_hist.  Candidate flags are
+// per-item wave ballots (scalar registers); a candidate's offset is its
+// item's and wave's base (one wave scans the (item, wave) counts) plus its
+// rank among the wave's set lanes.
+__device__ __forceinline__ void ex1_tile(const SelProblem& P, const SelTiles& T, int64_t gt, int cnt,
+                                         const double (&vp)[SEL_ITEMS], unsigned int b1, unsigned int* s_hist,
+                                         int* s_wcnt, double* s_red, int* s_dcnt) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) *s_dcnt = 0;
+  __syncthreads();
+  const int64_t base = tile_base(P, gt);
+  const int64_t cbase = gt * SEL_TILE;
+  double dsum = 0.0;
+  int dcnt = 0;
+  unsigned long long m[SEL_ITEMS];
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int j = it * NT + threadIdx.x;
+    bool cand = false;
+    if (j < cnt) {
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(vp[it]);
+      const unsigned int d = (unsigned int)(bits >> 52);
+      if (d > b1) {
+        dsum += vp[it];
+        ++dcnt;
+      } else if (d == b1) {
+        cand = true;
+        atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
+      }
+    }
+    m[it] = __ballot(cand);
+    if (lane == 0) s_wcnt[it * WAVES + w] = __popcll(m[it]);
+  }
+  __syncthreads();
+  const int cw = lane < SEL_ITEMS * WAVES ? s_wcnt[lane] : 0;
+  const int cx = wave_incl_scan(cw) - cw;
+  const int rb = __builtin_amdgcn_readlane(cx + cw, SEL_ITEMS * WAVES - 1);
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int o = __builtin_amdgcn_readlane(cx, it * WAVES + wu);
+    const unsigned long long mk = m[it];
+    if ((mk >> lane) & 1ull) {
+      const int off =
+          o + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mk, 0u));
+      T.c1p[cbase + off] = (unsigned long long)__double_as_longlong(vp[it]);
+      T.c1i[cbase + off] = (unsigned int)(base + it * NT + threadIdx.x);
+    }
+  }
+  const double s = block_sum(dsum, s_red);
+  // count above the bucket: wave sums, one LDS add per wave
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) dcnt += __shfl_xor(dcnt, o, 64);
+  if (lane == 0) atomicAdd(s_dcnt, dcnt);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T.tile_dsum[gt] = s;
+    T.tile_dcnt[gt] = *s_dcnt;
+    T.cnt1[gt] = rb;
+  }
+}
+
+// Per tile (EXT consecutive tiles per block, all of one problem; EXT = 0 takes
+// the count from ext_rt): count / sum above the digit-1 bucket, candidates
+// inside it, and the block's digit-2 histogram as one row of hist2_rows.
+// The tile being processed is held as |a|^2 and only the tile in flight as
+// amplitudes (block barriers wait on LDS only).
 template <bool BATCH, int EXT>
 __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
-                                                 int64_t tiles_end) {
+                                                 int64_t tiles_end, int ext_rt) {
   __shared__ unsigned int s_hist[HIST_BINS];
-  __shared__ int s_scan[SEL_ITEMS * WAVES];
+  __shared__ int s_wcnt[SEL_ITEMS * WAVES];
   __shared__ double s_red[WAVES];
   __shared__ int s_dcnt;
-  const int64_t gt0 = (int64_t)blockIdx.x * EXT;
+  const int ext = EXT > 0 ? EXT : ext_rt;
+  const int64_t gt0 = (int64_t)blockIdx.x * ext;
   const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt0);
   const SelState* sel = P.sel;
   if (sel->keepall) {
     // every entry is kept: per-tile kept count = tile count, no candidates
-    for (int sub = threadIdx.x; sub < EXT; sub += NT) {
+    for (int sub = threadIdx.x; sub < ext; sub += NT) {
       const int64_t gt = gt0 + sub;
       if (gt < tiles_end) {
         T.tile_dcnt[gt] = tile_count(P, gt);
@@ -278,98 +360,20 @@ __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int np
   }
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
   const unsigned int b1 = sel->b1;
-  // amplitudes of the current tile; the next tile's are loaded (clamped to a
-  // valid slot) before the current one is processed, so one tile's loads are
-  // always in flight (block barriers wait on LDS only)
-  double vr[SEL_ITEMS], vi[SEL_ITEMS];
-  int cnt = gt0 < tiles_end ? tile_count(P, gt0) : 0;
-  if (gt0 < tiles_end) {
-    const int64_t base0 = tile_base(P, gt0);
-#pragma unroll
-    for (int it = 0; it < SEL_ITEMS; ++it) {
-      const int j = it * NT + threadIdx.x;
-      const double2 v = P.a[base0 + (j < cnt ? j : 0)];
-      vr[it] = v.x;
-      vi[it] = v.y;
-    }
-  }
-  for (int sub = 0; sub < EXT; ++sub) {
+  // the next tile's amplitudes are loaded before the current tile is
+  // processed, so one tile's loads are always in flight
+  double nr[SEL_ITEMS], ni[SEL_ITEMS];
+  int cnt = 0;
+  if (gt0 < tiles_end) ex1_load(P, gt0, nr, ni, cnt);
+  for (int sub = 0; sub < ext; ++sub) {
     const int64_t gt = gt0 + sub;
     if (gt >= tiles_end) break;  // uniform
-    const bool more = sub + 1 < EXT && gt + 1 < tiles_end;  // uniform
-    double nr[SEL_ITEMS], ni[SEL_ITEMS];
-    int ncnt = 0;
-    if (more) {
-      ncnt = tile_count(P, gt + 1);
-      const int64_t nbase = tile_base(P, gt + 1);
+    double vp[SEL_ITEMS];
 #pragma unroll
-      for (int it = 0; it < SEL_ITEMS; ++it) {
-        const int j = it * NT + threadIdx.x;
-        const double2 v = P.a[nbase + (j < ncnt ? j : 0)];
-        nr[it] = v.x;
-        ni[it] = v.y;
-      }
-    }
-    if (threadIdx.x == 0) s_dcnt = 0;
-    __syncthreads();
-    const int64_t base = tile_base(P, gt);
-    const int64_t cbase = gt * SEL_TILE;
-    double dsum = 0.0;
-    int dcnt = 0;
-    int ci[SEL_ITEMS], coff[SEL_ITEMS];
-#pragma unroll
-    for (int it = 0; it < SEL_ITEMS; ++it) {
-      const int j = it * NT + threadIdx.x;
-      ci[it] = 0;
-      if (j < cnt) {
-        const double p = cabs2(make_double2(vr[it], vi[it]));
-        const unsigned long long bits = (unsigned long long)__double_as_longlong(p);
-        const unsigned int d = (unsigned int)(bits >> 52);
-        if (d > b1) {
-          dsum += p;
-          ++dcnt;
-        } else if (d == b1) {
-          ci[it] = 1;
-          atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
-        }
-      }
-    }
-    // candidates in input order (item-major), one scan for all items; their
-    // bit patterns are recomputed from the amplitudes (same operations, same
-    // rounding) instead of being held through the scan
-    int rb;
-    block_excl_scan_items<SEL_ITEMS>(ci, coff, rb, s_scan);
-#pragma unroll
-    for (int it = 0; it < SEL_ITEMS; ++it) {
-      const int j = it * NT + threadIdx.x;
-      if (j < cnt) {
-        const unsigned long long bits =
-            (unsigned long long)__double_as_longlong(cabs2(make_double2(vr[it], vi[it])));
-        if ((unsigned int)(bits >> 52) == b1) {
-          T.c1p[cbase + coff[it]] = bits;
-          T.c1i[cbase + coff[it]] = (unsigned int)(base + j);
-        }
-      }
-    }
-    const double s = block_sum(dsum, s_red);
-    // count above the bucket: wave sums, one LDS add per wave
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) dcnt += __shfl_xor(dcnt, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(&s_dcnt, dcnt);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      T.tile_dsum[gt] = s;
-      T.tile_dcnt[gt] = s_dcnt;
-      T.cnt1[gt] = rb;
-    }
-    if (more) {
-      cnt = ncnt;
-#pragma unroll
-      for (int it = 0; it < SEL_ITEMS; ++it) {
-        vr[it] = nr[it];
-        vi[it] = ni[it];
-      }
-    }
+    for (int it = 0; it < SEL_ITEMS; ++it) vp[it] = cabs2(make_double2(nr[it], ni[it]));
+    const int c = cnt;
+    if (sub + 1 < ext && gt + 1 < tiles_end) ex1_load(P, gt + 1, nr, ni, cnt);
+    ex1_tile(P, T, gt, c, vp, b1, s_hist, s_wcnt, s_red, &s_dcnt);
   }
   __syncthreads();
   // counts <= EXT * SEL_TILE fit 16 bits; k_hist2_reduce sums the rows (about
@@ -1157,15 +1161,15 @@ static void select_sequence(const SelProblem* probs, int nprob, const SelProblem
   hipLaunchKernelGGL((k_find1<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
   const int ext = BATCH ? 1 : extract1_tiles(tiles_end);
   const int64_t rows = (tiles_end + ext - 1) / ext;
-  if (ext == EX1_TILES_LARGE)
-    hipLaunchKernelGGL((k_extract1<BATCH, EX1_TILES_LARGE>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob,
-                       one, T, tiles_end);
+  if (ext == 1)
+    hipLaunchKernelGGL((k_extract1<BATCH, 1>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one, T,
+                       tiles_end, 1);
   else if (ext == EX1_TILES)
     hipLaunchKernelGGL((k_extract1<BATCH, EX1_TILES>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one,
-                       T, tiles_end);
+                       T, tiles_end, ext);
   else
-    hipLaunchKernelGGL((k_extract1<BATCH, 1>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one, T,
-                       tiles_end);
+    hipLaunchKernelGGL((k_extract1<BATCH, 0>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one, T,
+                       tiles_end, ext);
   hipLaunchKernelGGL((k_hist2_reduce<BATCH>),
                      dim3(HIST_BINS / (4 * NT), (unsigned)((max_rows + H2_ROWS - 1) / H2_ROWS), (unsigned)nprob),
                      dim3(NT), 0, s, probs, one, T);
@@ -1203,9 +1207,40 @@ static void select_sequence(const SelProblem* probs, int nprob, const SelProblem
   }
 }
 
-// Tiles per k_extract1 block of a single problem: 16 from 16384 tiles on (a
-// quarter of the digit-2 rows for k_hist2_reduce to sum), else 4.
-int extract1_tiles(int64_t ntiles) { return ntiles >= 16384 ? EX1_TILES_LARGE : EX1_TILES; }
+// Tiles per k_extract1 block of a single problem.  From 16384 tiles on, the
+// count in [EX1_TILES_MIN, EX1_TILES_MAX] whose blocks best fill whole rounds
+// of the blocks the device holds at once (the last round of a launch otherwise
+// runs partly empty; ties go to the larger count: fewer digit-2 rows for
+// k_hist2_reduce to sum); else 4.
+int extract1_tiles(int64_t ntiles) {
+  if (ntiles < 16384) return EX1_TILES;
+  static int slots = 0;  // resident k_extract1 blocks on the device (same value from every thread)
+  if (slots == 0) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(&k_extract1<false, 0>), NT,
+                                                     0) != hipSuccess ||
+        cus * per <= 0) {
+      (void)hipGetLastError();
+      cus = 256;
+      per = 4;
+    }
+    slots = cus * per;
+  }
+  int best = EX1_TILES_MAX;
+  double best_eff = 0.0;
+  for (int e = EX1_TILES_MAX; e >= EX1_TILES_MIN; --e) {
+    const int64_t rows = (ntiles + e - 1) / e;
+    const int64_t rounds = (rows + slots - 1) / slots;
+    const double eff = (double)rows / (double)(rounds * slots);
+    if (eff > best_eff + 1e-3) {
+      best_eff = eff;
+      best = e;
+    }
+  }
+  return best;
+}
 
 int select_rows(int64_t ntiles, bool batch) {
   const int ext = batch ? 1 : extract1_tiles(ntiles);

@@ -8,7 +8,10 @@ Against per-trial results the reference stored (tests/golden/reference_results):
                            values tie exactly at the truncation thresholds; the
                            reference breaks those ties with np.argpartition, this
                            build by the smaller key, so single trials differ and
-                           the means must agree.
+                           the means must agree;
+  phase_diagram_extended   N=16 RFIM and QAOA exact-state PR and entropy per
+                           trial; the RFIM rows of run_trial_timed (K=4096,
+                           default BASS arguments) as 20-trial means.
 Against the oracle on the same circuits:
   run_sweep_full on brickwork circuits (Haar 2-qubit gates: no |a|^2 ties, so
   the trajectories do not hinge on last-bit differences), truncating at two
@@ -111,3 +114,36 @@ def test_trial_timed_and_comparison(gpu_ctx):
     # no truncation (k >= 2^N): both simulators are exact
     c2 = R.run_comparison(6, 64, 4, 2, bass_kwargs=KW)
     assert abs(c2["f_fixed_median"] - 1) < 1e-10 and abs(c2["f_bass_median"] - 1) < 1e-10
+
+
+def test_phase_diagram_rows(gpu_ctx):
+    """PhaseDiagram.ipynb c2-c4 (N=16, K=4096, seed fi*1000 + t): the device's
+    exact states reproduce the stored exact PR and half-chain entropy of the
+    RFIM (fi=0) and QAOA (fi=4) rows per trial; run_trial_timed with default
+    BASS arguments reproduces the RFIM rows' 20-trial means (the trials
+    truncate through exact |a|^2 ties, see the module docstring)."""
+    print("[case] harness phase-diagram rows N=16 K=4096", flush=True)
+    R = pkg_module("experimentation.runner")
+    d = np.load(os.path.join(REF_RESULTS, "phase_diagram_extended.npz"), allow_pickle=False)
+    N, K, T = 16, 4096, 20
+    fams = {0: lambda n, rng: R.make_rfim(n, rng, layers=5, W=2.0, dt=0.2),
+            4: lambda n, rng: R.make_qaoa(n, 3, rng)}
+    got = []
+    for fi, gen in fams.items():
+        for t in range(T):
+            seed = fi * 1000 + t
+            sv = R.exact_statevector(N, gen(N, np.random.default_rng(seed)))
+            p = np.abs(sv) ** 2
+            psi = sv.reshape(2 ** (N // 2), -1)
+            ev = np.linalg.eigvalsh(psi @ psi.conj().T)
+            ev = ev[ev > 1e-15]
+            assert abs(1.0 / np.sum(p * p) / 2 ** N - d["pr_exact"][fi, t]) < 1e-12, (fi, t)
+            assert abs(float(-np.sum(ev * np.log2(ev))) / (N // 2) - d["entropy_exact"][fi, t]) < 1e-12, (fi, t)
+            if fi == 0:
+                r = R.run_trial_timed(gen, N, K, seed=seed, bass_kwargs={}, _exact_sv=sv)
+                got.append((r["f_fixed"], r["f_bass"], r["pr_fixed"] / 2 ** N, r["pr_bass"] / 2 ** N))
+    got = np.array(got)
+    ref = np.stack([d["fidelity_fixed"][0], d["fidelity_bass"][0], d["pr_fixed"][0], d["pr_bass"][0]], 1)
+    assert np.all(np.abs(got[:, :2].mean(0) - ref[:, :2].mean(0)) < 2e-3), (got.mean(0), ref.mean(0))
+    assert np.all(np.abs(got[:, 2:].mean(0) / ref[:, 2:].mean(0) - 1) < 1.5e-2), (got.mean(0), ref.mean(0))
+    assert np.abs(got[:, :2] - ref[:, :2]).max() < 1e-2

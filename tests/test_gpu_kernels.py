@@ -238,10 +238,12 @@ def test_truncate_renorm_stats(gpu_ctx):
     assert abs(st.sum4 - np.sum(np.abs(ga) ** 4)) < 1e-12 * st.sum4
 
 
-@pytest.mark.parametrize("n,k", [(9_000_001, 4_000_000)])
+@pytest.mark.parametrize("n,k", [(9_000_001, 4_000_000), (34_000_001, 15_000_000)])
 def test_truncate_chunked(gpu_ctx, n, k):
     """More than 4096 tiles of 2048 entries: the per-tile bookkeeping (kept
-    offsets, kept / input / output sums) reduces by chunk in two levels."""
+    offsets, kept / input / output sums) reduces by chunk in two levels.  From
+    16384 tiles on (the second case) the first candidate pass takes a tile
+    count per block chosen to fill whole rounds of resident blocks."""
     print(f"[case] truncate chunked n={n} k={k}", flush=True)
     ops = pkg_module("_ops")
     dev = pkg_module("_device")
