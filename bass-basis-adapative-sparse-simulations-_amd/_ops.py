@@ -146,6 +146,41 @@ def key_map_arg(pos):
     return km
 
 
+def truncate_launch(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False, key_map=None):
+    """The launches of truncate() without waiting for the device: returns the
+    handle truncate_finish() takes.  Nothing may launch a stats-writing
+    operation on the context in between."""
+    n = 0 if n is None else n
+    # the select reads whole tiles of SEG slots (and hist_tiles of per-tile sums)
+    tiles = hist_tiles if segmented else -(-int(n) // SEG)
+    ctx.ensure_ws(max(n, tiles * SEG, hist_tiles * SEG))
+    dst.ensure(k)
+    if key_map is None:
+        km, kp, kb = None, None, 0
+    else:
+        km = np.ascontiguousarray(key_map, dtype=np.int32)
+        kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
+    flags = 1 if segmented else 0
+    args = (src.xp, src.ap, n, k, dst.xp, dst.ap, dst.cap, 1 if renorm else 0, hist_tiles, flags, kp, kb)
+    rc = ctx.lib.bass_truncate(*args, ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_truncate")
+    return args, km
+
+
+def truncate_finish(ctx, handle):
+    """Stats of a truncate_launch() (waits for the device)."""
+    args, _km = handle
+    st = ctx.read_stats("bass_truncate")
+    if st.flags & 2:
+        # very many entries tie at the threshold: repeat with the multi-block path
+        a = list(args)
+        a[9] |= 2
+        rc = ctx.lib.bass_truncate(*a, ctx.wsp, ctx.ws_bytes, ctx.stream)
+        L.check(rc, "bass_truncate")
+        st = ctx.read_stats("bass_truncate")
+    return st
+
+
 def truncate(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False, key_map=None):
     """Top-k of src[:n] (n > k) into dst[:k].  Returns stats (kept, total, sum2, sum4, norm).
 
@@ -153,28 +188,7 @@ def truncate(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False, ke
     <= k entries, which are then all kept, in order, without renormalisation
     (stats.flags bit 0 clear); stats.n_in is the input count.
     key_map: stored-bit -> qubit map (key_map_arg) when src holds stored words."""
-    n = 0 if n is None else n
-    # the select reads whole tiles of SEG slots (and hist_tiles of per-tile sums)
-    tiles = hist_tiles if segmented else -(-int(n) // SEG)
-    ctx.ensure_ws(max(n, tiles * SEG, hist_tiles * SEG))
-    dst.ensure(k)
-    if key_map is None:
-        kp, kb = None, 0
-    else:
-        km = np.ascontiguousarray(key_map, dtype=np.int32)
-        kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
-    flags = 1 if segmented else 0
-    rc = ctx.lib.bass_truncate(src.xp, src.ap, n, k, dst.xp, dst.ap, dst.cap, 1 if renorm else 0,
-                               hist_tiles, flags, kp, kb, ctx.wsp, ctx.ws_bytes, ctx.stream)
-    L.check(rc, "bass_truncate")
-    st = ctx.read_stats("bass_truncate")
-    if st.flags & 2:
-        # very many entries tie at the threshold: repeat with the multi-block path
-        rc = ctx.lib.bass_truncate(src.xp, src.ap, n, k, dst.xp, dst.ap, dst.cap, 1 if renorm else 0,
-                                   hist_tiles, flags | 2, kp, kb, ctx.wsp, ctx.ws_bytes, ctx.stream)
-        L.check(rc, "bass_truncate")
-        st = ctx.read_stats("bass_truncate")
-    return st
+    return truncate_finish(ctx, truncate_launch(ctx, src, n, k, dst, renorm, hist_tiles, segmented, key_map))
 
 
 def diag2q(ctx, buf, n, q1, q2, diag):

@@ -410,6 +410,8 @@ class BASS:
             self._count("rdm")
             pr = self._pr(state)
             improved = False
+            # the pass's rotations, all from the RDMs at its start
+            plan = []
             for j in range(self.N):
                 rho = all_rho[j]
                 if self.use_fast_eigh:
@@ -421,24 +423,24 @@ class BASS:
                         V[:, 1] *= -1
                 if abs(V[0, 1]) + abs(V[1, 0]) < 1e-10:
                     continue
-                # tentative: rotate -> truncate to k -> renormalise -> PR
-                rot = state._rot
-                jb = int(state.pos[j])
-                _, rht = _ops.lift(ctx, state._buf, state.nnz, jb, rot, G=V.conj().T, nq=1, mode=L.MODE_ROT,
-                                   count=False)
-                trial_pos = lifted_positions(state.pos, jb, 0)
+                plan.append((j, V, np.ascontiguousarray(V.conj().T)))
+            # tentative: rotate -> truncate to k -> renormalise -> PR.  Trial
+            # i + 1 is launched as soon as trial i is decided, its arguments
+            # for either outcome prepared while trial i runs; trial i's
+            # bookkeeping then overlaps trial i + 1 on the device.
+            trial = self._rot_trial_launch(state, cand, self._rot_trial_args(state.pos, plan[0])) if plan else None
+            for i, (j, V, _) in enumerate(plan):
+                handle, trial_pos, n_in = trial
+                nxt = plan[i + 1] if i + 1 < len(plan) else None
+                if nxt is not None:
+                    args_acc = self._rot_trial_args(trial_pos, nxt)
+                    args_rej = self._rot_trial_args(state.pos, nxt)
                 # truncate to k, or (<= k entries) keep all; one stats read
-                st = _ops.truncate(ctx, rot, None, self.k, cand, renorm=True, hist_tiles=rht, segmented=True,
-                                   key_map=_ops.key_map_arg(trial_pos))
+                st = _ops.truncate_finish(ctx, handle)
                 m = st.n_in
-                self._count("rotate")
-                self._count("amp_updates", m)
-                self._bytes("rot1q", state.nnz, m)
                 if st.flags & 1:
-                    self._bytes("select", m, self.k)
                     nn, frac, s4 = self.k, st.kept, st.sum4
                 else:
-                    self._count("bytes_select", 48 * m)
                     s2 = st.sum2
                     frac = s2
                     norm = math.sqrt(s2)
@@ -448,18 +450,28 @@ class BASS:
                     nn = m
                 pr_new = 1.0 / s4 if s4 > 1e-60 else float("inf")
                 reject = pr_new >= pr * (1.0 - self.accept_rtol)
+                if not reject:
+                    # accept: the tentative buffer becomes the state
+                    state._buf, state._tmp = cand, state._buf
+                    cand = state._tmp
+                    state.nnz = nn
+                    state.pos = trial_pos
+                    state._sum2, state._sum4 = None, s4
+                    state._hist_tiles = 0
+                if nxt is not None:
+                    trial = self._rot_trial_launch(state, cand, args_rej if reject else args_acc)
+                self._count("rotate")
+                self._count("amp_updates", m)
+                self._bytes("rot1q", n_in, m)
+                if st.flags & 1:
+                    self._bytes("select", m, self.k)
+                else:
+                    self._count("bytes_select", 48 * m)
                 if self.trace is not None:
                     self.trace.append(("rot", j, not reject, pr, pr_new, frac))
                 if reject:
                     self._count("reject")
                     continue
-                # accept: the tentative buffer becomes the state
-                state._buf, state._tmp = cand, state._buf
-                cand = state._tmp
-                state.nnz = nn
-                state.pos = trial_pos
-                state._sum2, state._sum4 = None, s4
-                state._hist_tiles = 0
                 self.U[j] = self.U[j] @ V
                 pr = pr_new
                 self._gamma_running *= np.sqrt(frac if frac > 1e-30 else 0)
@@ -473,6 +485,26 @@ class BASS:
             self._optimize_2q_pass(state, [(i, i + 1) for i in range(1, self.N - 1, 2)])
         self._truncate_state(state)
         self._renorm(state)
+
+    def _rot_trial_args(self, pos, item):
+        """Launch arguments of the rotation trial item = (qubit, V, V^dagger)
+        on a state whose qubit q sits at stored bit pos[q]."""
+        j, _, Vh = item
+        jb = int(pos[j])
+        trial_pos = lifted_positions(pos, jb, 0)
+        return jb, Vh, trial_pos, _ops.key_map_arg(trial_pos)
+
+    def _rot_trial_launch(self, state, cand, args):
+        """Rotation of the state's stored bit jb by V^dagger into state._rot,
+        then its truncate to k (renormalised) into cand; launched, not waited
+        for.  Returns (truncate handle, positions after the rotation, input
+        count)."""
+        jb, Vh, trial_pos, km = args
+        ctx = self._ctx
+        _, rht = _ops.lift(ctx, state._buf, state.nnz, jb, state._rot, G=Vh, nq=1, mode=L.MODE_ROT, count=False)
+        handle = _ops.truncate_launch(ctx, state._rot, None, self.k, cand, renorm=True, hist_tiles=rht,
+                                      segmented=True, key_map=km)
+        return handle, trial_pos, state.nnz
 
     def _spare(self, state, name, n):
         buf = getattr(state, name, None)
