@@ -475,6 +475,78 @@ __global__ __launch_bounds__(NT) void k_extract2(const SelProblem* probs, int np
   }
 }
 
+// The same for one problem: each wave takes tpw consecutive tiles and buffers
+// its candidates in LDS, reserving space in the list with one atomic per
+// buffer flush (usually one per wave) instead of one per 64 candidates with a
+// match: the list's single counter, not the candidate reads, bounds the
+// per-64 form when most tiles hold a few matches.
+constexpr int EX2_CAP = 512;  // buffered candidates per wave
+__global__ __launch_bounds__(NT) void k_extract2_agg(SelProblem P, SelTiles T, int64_t tiles_end, int tpw) {
+  __shared__ unsigned long long s_bits[WAVES][EX2_CAP];
+  __shared__ unsigned int s_idx[WAVES][EX2_CAP];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t t0 = ((int64_t)blockIdx.x * WAVES + w) * tpw;
+  if (t0 >= tiles_end) return;  // uniform per wave; no block barriers below
+  SelState* sel = P.sel;
+  if (sel->keepall) return;
+  const unsigned int b2 = sel->b2;
+  const int64_t t1 = t0 + tpw < tiles_end ? t0 + tpw : tiles_end;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  int nbuf = 0;  // wave-uniform
+  constexpr int U = 8;  // candidate loads in flight per lane (clamped to a valid slot)
+  for (int64_t gt = t0; gt < t1; ++gt) {
+    const int cnt = T.cnt1[gt];
+    const int64_t base = gt * SEL_TILE;
+    for (int j0 = 0; j0 < cnt; j0 += 64 * U) {
+      unsigned long long bv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + u * 64 + lane;
+        bv[u] = T.c1p[base + (j < cnt ? j : 0)];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + u * 64 + lane;
+        const unsigned long long bits = bv[u];
+        const bool take = j < cnt && ((unsigned int)(bits >> 41) & (HIST_BINS - 1)) == b2;
+        const unsigned long long bal = __ballot(take);
+        if (!bal) continue;  // uniform
+        const int c = __popcll(bal);
+        if (nbuf + c > EX2_CAP) {
+          // flush: reserve nbuf slots, copy the buffer out (wave-local LDS:
+          // the wave's own writes are visible to its later reads)
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          unsigned int wb = 0;
+          if (lane == 0) wb = atomicAdd(&sel->cand2, (unsigned int)nbuf);
+          wb = __shfl(wb, 0, 64);
+          for (int i = lane; i < nbuf; i += 64) {
+            P.c2p[wb + i] = s_bits[w][i];
+            P.c2i[wb + i] = s_idx[w][i];
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          nbuf = 0;
+        }
+        if (take) {
+          const int r = nbuf + __popcll(bal & below);
+          s_bits[w][r] = bits;
+          s_idx[w][r] = T.c1i[base + j];
+        }
+        nbuf += c;
+      }
+    }
+  }
+  if (nbuf > 0) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    unsigned int wb = 0;
+    if (lane == 0) wb = atomicAdd(&sel->cand2, (unsigned int)nbuf);
+    wb = __shfl(wb, 0, 64);
+    for (int i = lane; i < nbuf; i += 64) {
+      P.c2p[wb + i] = s_bits[w][i];
+      P.c2i[wb + i] = s_idx[w][i];
+    }
+  }
+}
+
 // Resolve the remaining bits of the threshold from the (small) digit-2
 // candidate list, then the key tie-break among entries equal to it.  One block
 // of NTB threads per problem; the logical keys of the tied entries are
@@ -1175,7 +1247,15 @@ static void select_sequence(const SelProblem* probs, int nprob, const SelProblem
                      dim3(NT), 0, s, probs, one, T);
   hipLaunchKernelGGL((k_find2<BATCH>), dim3((unsigned)nprob), dim3(NT), 0, s, probs, one);
   const unsigned wave_tiles = (unsigned)((tiles_end + WAVES - 1) / WAVES);
-  hipLaunchKernelGGL((k_extract2<BATCH>), dim3(wave_tiles), dim3(NT), 0, s, probs, nprob, one, T, tiles_end);
+  if (BATCH) {
+    hipLaunchKernelGGL((k_extract2<BATCH>), dim3(wave_tiles), dim3(NT), 0, s, probs, nprob, one, T, tiles_end);
+  } else {
+    // 8 tiles per wave from 16384 tiles on (every wave still resident at once)
+    const int tpw = tiles_end >= 16384 ? 8 : 1;
+    const int64_t waves = (tiles_end + tpw - 1) / tpw;
+    hipLaunchKernelGGL(k_extract2_agg, dim3((unsigned)((waves + WAVES - 1) / WAVES)), dim3(NT), 0, s, one, T,
+                       tiles_end, tpw);
+  }
   if (!wide) {
     hipLaunchKernelGGL((k_finish<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one);
   } else {
