@@ -457,18 +457,32 @@ static int lift_impl(const uint64_t* x, const double* a, int64_t n, int nq, int 
   const int tk = ktimer_begin(names[mode][nq - 1], s);
   GateMat gm;
   memset(&gm, 0, sizeof(gm));
+  // cleared before the gate kernel accumulates: the digit-1 histogram (gates)
+  // and the first 48 bytes of the stats block (err untouched); by the lift's
+  // first boundary kernel when it runs, else here
+  LiftZero zero;
   if (mode != MODE_MOVE) {
     const int M = 1 << nq;
     for (int e = 0; e < M * M; ++e) gm.m[e] = make_double2(G[2 * e], G[2 * e + 1]);
-    zero2(at<void>(ws, L.hist1), HIST_BINS * 4, st, 48, s);  // histogram + stats (err untouched)
+    zero.p = at<unsigned long long>(ws, L.hist1);
+    zero.wp = HIST_BINS * 4 / 8;
+    zero.q = (unsigned long long*)st;
+    zero.wq = 48 / 8;
   } else {
-    fill(st, 48, 0ull, s);
+    zero.p = (unsigned long long*)st;
+    zero.wp = 48 / 8;
+  }
+  if (n <= 0) {
+    if (zero.q)
+      zero2(zero.p, (size_t)zero.wp * 8, zero.q, (size_t)zero.wq * 8, s);
+    else
+      fill(zero.p, (size_t)zero.wp * 8, 0ull, s);
   }
   if (n > 0) {
     int* seg = at<int>(ws, L.seg_cnt);
     launch_lift(nq, mode, x, (const double2*)a, n, j, gm, xo, (double2*)ao, at<LiftBound>(ws, L.lift_b),
                 at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), seg, s,
-                counted ? at<int64_t>(ws, L.count_slot) : nullptr);
+                counted ? at<int64_t>(ws, L.count_slot) : nullptr, zero);
     if (mode != MODE_MOVE)
       launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), seg_chunk_cnt(ws, L), seg_chunk_off(ws, L), st, s);
   }

@@ -12,6 +12,15 @@ struct GateMat {
   double2 m[16];  // row-major (2^QB x 2^QB), index (out, in) with in = (b_q0 << 1) | b_q1
 };
 
+// Two word ranges a lift clears in the first kernel it launches (block 0),
+// before its gate kernel accumulates into them: the digit-1 histogram and the
+// stats block.  p == nullptr: nothing to clear.
+struct LiftZero {
+  unsigned long long* p = nullptr;
+  unsigned long long* q = nullptr;
+  int wp = 0, wq = 0;
+};
+
 // Top-k selection bookkeeping, device resident.
 struct SelState {
   unsigned long long need;       // elements still to take inside the current bucket
@@ -86,7 +95,7 @@ int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, int njobs, int64_
 int lift_tile_entries(int qb, int mode);
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
-                hipStream_t s, const int64_t* n_dev = nullptr);
+                hipStream_t s, const int64_t* n_dev = nullptr, const LiftZero& zero = LiftZero());
 
 // One state of a batched elementwise operation (see launch_ew_batch):
 // op 0 diagonal 2-qubit gate a *= d[(b_j1 << 1) | b_j2], op 1 a /= div,

@@ -122,11 +122,23 @@ __device__ __forceinline__ LiftBound lift_boundary(const uint64_t* __restrict__ 
   return r;
 }
 
+// Block (0, 0) clears the lift's zero ranges (every thread, before any exit).
+__device__ __forceinline__ void lift_zero(const LiftZero& z) {
+  if (z.p == nullptr || blockIdx.x != 0 || blockIdx.y != 0) return;
+  for (int i = threadIdx.x; i < z.wp + z.wq; i += NT) {
+    if (i < z.wp)
+      z.p[i] = 0ull;
+    else
+      z.q[i - z.wp] = 0ull;
+  }
+}
+
 // Tile boundaries of the lift of bit j0 + blockIdx.y (one bit per grid row;
 // row r writes bounds[r * nbound + u]).
 __global__ __launch_bounds__(NT) void k_lift_part(const uint64_t* __restrict__ w, int64_t n, int j0, int t,
                                                    int tm, int64_t nbound, LiftBound* bounds,
-                                                   const int64_t* __restrict__ n_dev) {
+                                                   const int64_t* __restrict__ n_dev, LiftZero zero) {
+  lift_zero(zero);
   n = dev_count(n_dev, n);
   const int lane = threadIdx.x & 63;
   const int j = j0 + (int)blockIdx.y;
@@ -189,7 +201,8 @@ __host__ __device__ inline int64_t lift_coarse_count(int64_t nbound) { return (n
 
 __global__ __launch_bounds__(NT) void k_lift_part_coarse(const uint64_t* __restrict__ w, int64_t n, int j0, int t,
                                                           int tm, int64_t nbound, LiftBound* coarse,
-                                                          const int64_t* __restrict__ n_dev) {
+                                                          const int64_t* __restrict__ n_dev, LiftZero zero) {
+  lift_zero(zero);
   n = dev_count(n_dev, n);
   const int lane = threadIdx.x & 63;
   const int j = j0 + (int)blockIdx.y;
@@ -231,15 +244,16 @@ __global__ __launch_bounds__(NT) void k_lift_part_fine(const uint64_t* __restric
 // coarse pass uses `coarse` (lift_coarse_count(nbound) per row).  Small
 // lifts take the direct pass.
 static void lift_partition(const uint64_t* x, int64_t n, int j0, int rows, int t, int tm, int64_t nbound,
-                           LiftBound* bounds, LiftBound* coarse, hipStream_t s, const int64_t* n_dev = nullptr) {
+                           LiftBound* bounds, LiftBound* coarse, hipStream_t s, const int64_t* n_dev = nullptr,
+                           const LiftZero& zero = LiftZero()) {
   if (nbound <= 4 * LIFT_CS || coarse == nullptr) {
     hipLaunchKernelGGL(k_lift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES), (unsigned)rows), dim3(NT), 0, s,
-                       x, n, j0, t, tm, nbound, bounds, n_dev);
+                       x, n, j0, t, tm, nbound, bounds, n_dev, zero);
     return;
   }
   const int64_t nc = lift_coarse_count(nbound);
   hipLaunchKernelGGL(k_lift_part_coarse, dim3((unsigned)((nc + WAVES - 1) / WAVES), (unsigned)rows), dim3(NT), 0, s,
-                     x, n, j0, t, tm, nbound, coarse, n_dev);
+                     x, n, j0, t, tm, nbound, coarse, n_dev, zero);
   hipLaunchKernelGGL(k_lift_part_fine, dim3((unsigned)((nbound + WAVES - 1) / WAVES), (unsigned)rows), dim3(NT), 0,
                      s, x, n, j0, t, tm, nbound, (const LiftBound*)coarse, bounds, n_dev);
 }
@@ -1099,14 +1113,14 @@ int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, int njobs, int64_
 
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
-                hipStream_t s, const int64_t* n_dev) {
+                hipStream_t s, const int64_t* n_dev, const LiftZero& zero) {
   if (n <= 0) return 0;
   const int tm = lift_tile_entries(qb, mode);
   const int t = (mode == MODE_MOVE) ? 0 : qb - 1;
   const int64_t tiles = (n + tm - 1) / tm;
   const int64_t nbound = tiles + 1;
   // the coarse boundaries follow the nbound boundaries (lift_bounds_entries)
-  lift_partition(x, n, j, 1, t, tm, nbound, bounds, bounds + nbound, s, n_dev);
+  lift_partition(x, n, j, 1, t, tm, nbound, bounds, bounds + nbound, s, n_dev, zero);
   const dim3 grid((unsigned)tiles);
   if (mode == MODE_MOVE) {
     hipLaunchKernelGGL(k_lift_move<1024>, grid, dim3(NT), 0, s, x, a, n, j, bounds, xo, ao, n_dev);
