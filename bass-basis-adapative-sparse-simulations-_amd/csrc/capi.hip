@@ -533,7 +533,8 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   WS_REQUIRE(need_ws > psum_slots ? need_ws : psum_slots);
   hipStream_t s = (hipStream_t)stream;
   OpStats* st = at<OpStats>(ws, L.st);
-  fill(at<void>(ws, L.hist2), L.hdrB_end - L.hist2, 0ull, s);
+  // header B (digit-2 and wide histograms, select state, tickets) is cleared by
+  // k_find1, the select's first kernel after the histogram pass
   if (hist_tiles <= 0) fill(at<void>(ws, L.hist1), HIST_BINS * 4, 0ull, s);
   char* big = at<char>(ws, L.big);
   const size_t p8 = al((size_t)8 * slots), p4 = al((size_t)4 * slots);
@@ -564,6 +565,8 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   P.c2p = (unsigned long long*)(big + p8 + p4);
   P.c2i = (unsigned int*)(big + 2 * p8 + p4);
   P.tk = (uint64_t*)(big + 2 * p8 + 2 * p4);
+  P.zero_hdr = at<unsigned long long>(ws, L.hist2);
+  P.zero_words = (int64_t)((L.hdrB_end - L.hist2) / 8);
   P.km = km;
   const SelTiles T = tiles_view(ws, L, (unsigned long long*)big, (unsigned int*)(big + p8));
   const int tk = ktimer_begin("select", s);

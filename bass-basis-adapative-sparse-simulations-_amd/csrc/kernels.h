@@ -166,6 +166,8 @@ struct SelProblem {
   unsigned long long* c2p;  // >= ntiles * SEG_ENTRIES slots
   unsigned int* c2i;
   uint64_t* tk;             // >= ntiles * SEG_ENTRIES
+  unsigned long long* zero_hdr;  // cleared by k_find1 before it starts (zero_words words), or nullptr
+  int64_t zero_words;
   KeyMap km;             // ties are broken by the logical key
 };
 

@@ -217,6 +217,11 @@ __global__ __launch_bounds__(NTB) void k_find1(const SelProblem* probs, SelProbl
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_above;
   const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
+  // the select's zeroed header (digit-2 and wide histograms, state, tickets),
+  // when the caller leaves it to this first kernel; the barrier below orders
+  // it before this kernel's own writes to the state
+  if (P.zero_hdr)
+    for (int64_t i = threadIdx.x; i < P.zero_words; i += NTB) P.zero_hdr[i] = 0ull;
   if (threadIdx.x == 0) {
     s_b = 0xFFFFFFFFu;
     s_above = 0;
