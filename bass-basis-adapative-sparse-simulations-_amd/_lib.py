@@ -23,6 +23,7 @@ BASS_E_DEVICE = -5
 MODE_GATE = 0
 MODE_ROT = 1
 MODE_MOVE = 2
+LIFT_NO_SCAN = 16  # bass_lift mode flag: no output-count scan (a segmented truncate reads the output next)
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64

@@ -65,7 +65,10 @@ def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE, count=True):
     else:
         buf, gp = L.mat_arg(G)
     st = ctypes.c_int64(0)
-    rc = ctx.lib.bass_lift(src.xp, src.ap, n, nq, int(j), gp, mode, dst.xp, dst.ap, dst.cap, ctypes.byref(st),
+    # count=False: a segmented truncate reads the output next, which needs no
+    # scan of its tile counts
+    fl = mode | (L.LIFT_NO_SCAN if (not count and mode != L.MODE_MOVE) else 0)
+    rc = ctx.lib.bass_lift(src.xp, src.ap, n, nq, int(j), gp, fl, dst.xp, dst.ap, dst.cap, ctypes.byref(st),
                            ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_lift")
     if mode == L.MODE_MOVE:
@@ -87,7 +90,9 @@ def lift_counted(ctx, src, n_max, j, dst, G=None, nq=1, mode=L.MODE_GATE):
     dst.ensure(max(need, 1))
     buf, gp = (None, None) if mode == L.MODE_MOVE else L.mat_arg(G)
     st = ctypes.c_int64(0)
-    rc = ctx.lib.bass_lift_counted(src.xp, src.ap, n_max, nq, int(j), gp, mode, dst.xp, dst.ap, dst.cap,
+    # a counted truncate reads a gate's output next: no scan of its tile counts
+    fl = mode if mode == L.MODE_MOVE else mode | L.LIFT_NO_SCAN
+    rc = ctx.lib.bass_lift_counted(src.xp, src.ap, n_max, nq, int(j), gp, fl, dst.xp, dst.ap, dst.cap,
                                    ctypes.byref(st), ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_lift_counted")
     return int(st.value)

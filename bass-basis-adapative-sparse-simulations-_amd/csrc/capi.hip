@@ -439,6 +439,9 @@ int bass_counted_read(void* ws, size_t ws_bytes, bass_stats* out, int64_t max_en
 static int lift_impl(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G, int mode,
                      uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws, size_t ws_bytes,
                      void* stream, bool counted) {
+  // mode bit 4: no scan of the output's tile counts (a segmented truncate reads them)
+  const bool no_scan = (mode & 16) != 0;
+  mode &= ~16;
   if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_lift: nq must be 1 or 2");
   if (mode != MODE_GATE && mode != MODE_ROT && mode != MODE_MOVE) return fail(BASS_E_ARG, "bass_lift: bad mode");
   if (mode == MODE_MOVE && nq != 1) return fail(BASS_E_ARG, "bass_lift: a move lifts one bit");
@@ -483,7 +486,7 @@ static int lift_impl(const uint64_t* x, const double* a, int64_t n, int nq, int 
     launch_lift(nq, mode, x, (const double2*)a, n, j, gm, xo, (double2*)ao, at<LiftBound>(ws, L.lift_b),
                 at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), seg, s,
                 counted ? at<int64_t>(ws, L.count_slot) : nullptr, zero);
-    if (mode != MODE_MOVE)
+    if (mode != MODE_MOVE && !no_scan)
       launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), seg_chunk_cnt(ws, L), seg_chunk_off(ws, L), st, s);
   }
   ktimer_end(tk, s);

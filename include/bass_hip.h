@@ -126,6 +126,9 @@ int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t
  * a group's products are summed in stored order.  Replaces, inside the
  * driver, the gate step of src/simulation/bass_simulator.py:861-896 and the
  * tentative rotation of :768-783 (no reference counterpart as an entry point).
+ * mode | 16 (gate or rotation): skip the closing scan of the per-tile output
+ * counts, so stats.nnz and the segment offsets bass_densify needs are not
+ * produced; for an output that a bass_truncate(in_flags bit 0) reads next.
  * Returns BASS_E_OVERFLOW if cap_out is below the output size. */
 int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G,
               int mode, uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws,
