@@ -140,7 +140,8 @@ int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, cons
  * follow n_max, tiles past the actual count come out empty (a move copies
  * the actual count and leaves the slot valid).  With counted truncations this
  * runs a gate loop without a host round trip per gate (the driver reads the
- * op log at its decision points).  No counterpart in the reference. */
+ * op log at its decision points).  mode | 16 as in bass_lift.  No counterpart
+ * in the reference. */
 int bass_lift_counted(const uint64_t* x, const double* a, int64_t n_max, int nq, int j, const double* G,
                       int mode, uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws,
                       size_t ws_bytes, void* stream);

@@ -81,6 +81,11 @@ def test_argument_errors_without_gpu(lib):
     assert rc == L.BASS_E_ARG and "move" in L.last_error()
     rc = lib.bass_lift(None, None, 3000, 1, 3, G, 0, None, None, 2048, ctypes.byref(ht), None, 0, None)
     assert rc == L.BASS_E_OVERFLOW
+    # the no-scan flag (mode bit 4) leaves the mode check intact
+    rc = lib.bass_lift(None, None, 10, 1, 3, G, 3 | L.LIFT_NO_SCAN, None, None, 40, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_ARG and "mode" in L.last_error()
+    rc = lib.bass_lift(None, None, 3000, 1, 3, G, L.LIFT_NO_SCAN, None, None, 2048, ctypes.byref(ht), None, 0, None)
+    assert rc == L.BASS_E_OVERFLOW
     rc = lib.bass_radix_sort(None, None, 10, 0, None, 0, None)
     assert rc == L.BASS_E_ARG
     rc = lib.bass_lift_counted(None, None, 10, 3, 3, G, 0, None, None, 40, ctypes.byref(ht), None, 0, None)
