@@ -30,5 +30,21 @@ Parity pins (what checks each function; the tests named live in ``tests/``):
 * BASS / FixedBasis drivers end to end: the reference-held per-trial fidelity and
   participation ratio in ``tests/results/fidelity_pr_raw.npz`` (RFIM rows) and
   ``nscaling_*.npz`` (``test_oracle_fixtures.py``);
+* RFIM and QAOA generators plus the dense simulator: the exact-state PR and
+  half-chain entropy of every trial of ``phase_diagram_extended.npz``; the RFIM
+  driver rows there as 20-trial means (``test_oracle_fixtures.py``);
 * random-k truncation (LCG partial Fisher-Yates): closed-form LCG sequence.
+
+Parity unpinned (no reference-held result or closed form decides them):
+
+* the reference's order among exactly tied |a|^2 (``np.argpartition`` inside
+  and outside Numba): ``argpartition_selector`` reproduces numpy's choice on
+  this host only, and numpy's own choice differs between its SIMD dispatch
+  paths; single trials that truncate through ties are compared as means;
+* the 2-qubit brick-wall pass end to end (its kernels, RDM and eigensolver are
+  pinned; the reference's QAOA results with it depend on tie order);
+* the brickwork / Haar generators' matrices (scipy ``unitary_group`` at a
+  different version than the reference's) and the FixedBasis random-k draw
+  (``np.random.choice`` on the legacy generator; its only stored results,
+  ``schmidt_benchmark.npz``, come from scipy-generated circuits).
 """
