@@ -131,6 +131,39 @@ def apply_1qubit_rotation_sparse(x, alpha, nnz, qubit, U):
     return m
 
 
+def compute_2qubit_rdm(x, alpha, nnz, q1, q2):
+    """bass_simulator.py:203-234: 4x4 reduced density matrix of the pair (q1, q2),
+    row / column index (b_q1 << 1) | b_q2, summed over the other bits (keys
+    distinct, as in a SparseState).  On the device: a copy of the state sorted
+    by key (a key-ordered state with every qubit at its own bit), q2's bit
+    moved to bit 0, then the ordered pair RDM of bits (position of q1, 0).
+    Group products are summed in a fixed order, not the reference's argsort
+    order (results agree to rounding)."""
+    q1, q2 = int(q1), int(q2)
+    if q1 == q2 or not (0 <= q1 <= 62 and 0 <= q2 <= 62):
+        raise ValueError("compute_2qubit_rdm: need two distinct qubits in [0, 62]")
+    ctx = context()
+    src, _, _ = _pair(x, alpha, ctx)
+    n = int(nnz)
+    if n <= 0:
+        return np.zeros((4, 4), dtype=np.complex128)
+    buf = Buffers(n, ctx.device)
+    buf.x[:n].copy_(src.x[:n])
+    buf.a[:n].copy_(src.a[:n])
+    lo = int(buf.x[:n].min().item())
+    key_bits = 64 if lo < 0 else max(1, int(buf.x[:n].max().item()).bit_length())
+    key_bits = max(key_bits, q1 + 1, q2 + 1)
+    ctx.ensure_ws(n)
+    L.check(ctx.lib.bass_radix_sort(buf.xp, buf.ap, n, key_bits, ctx.wsp, ctx.ws_bytes, ctx.stream),
+            "bass_radix_sort")
+    pos = np.arange(key_bits)
+    if q2 != 0:
+        mv = Buffers(n, ctx.device)
+        _ops.lift(ctx, buf, n, q2, mv, mode=L.MODE_MOVE)
+        buf, pos = mv, lifted_positions(pos, q2, 0)
+    return _ops.rdm2q(ctx, buf, n, int(pos[q1]))
+
+
 def apply_2qubit_rotation_sparse(x, alpha, nnz, q1, q2, U4):
     """bass_simulator.py:237-289: 4x4 unitary on the pair (absent members read
     as 0, |a| <= 1e-30 dropped), in place; buffers must hold 4 * nnz."""

@@ -286,3 +286,26 @@ def test_rdm2q_ordered(gpu_ctx, n, nbits):
         ref = OK.compute_2qubit_rdm(x, a, n, f, s)
         np.testing.assert_allclose(rho, ref, rtol=0, atol=1e-13 * max(1.0, np.abs(ref).max()))
         assert np.allclose(rho, rho.conj().T, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("n,nbits", [(1, 4), (9, 4), (5000, 13), (200000, 30)])
+def test_compute_2qubit_rdm_module(gpu_ctx, n, nbits):
+    """The module-level compute_2qubit_rdm (an unordered state, as the
+    reference's kernel takes it) against the oracle: pairs with qubit 0 in
+    either place and both orders, device tensors and NumPy arrays."""
+    B = pkg_module("simulation.bass_simulator")
+    rng = np.random.default_rng(7 * n + nbits)
+    x, a = random_state(n, nbits, seed=n + 5)
+    perm = rng.permutation(n)
+    x, a = x[perm], a[perm]  # input order is not key order
+    pairs = [(0, nbits - 1), (nbits - 1, 0), tuple(int(v) for v in rng.choice(nbits, size=2, replace=False))]
+    xt = torch.from_numpy(x.view(np.int64)).to(gpu_ctx.device)
+    at = torch.from_numpy(a).to(gpu_ctx.device)
+    for q1, q2 in pairs:
+        print(f"[case] compute_2qubit_rdm n={n} nbits={nbits} pair=({q1},{q2})", flush=True)
+        ref = OK.compute_2qubit_rdm(x, a, n, q1, q2)
+        tol = 1e-13 * max(1.0, np.abs(ref).max())
+        np.testing.assert_allclose(B.compute_2qubit_rdm(xt, at, n, q1, q2), ref, rtol=0, atol=tol)
+        np.testing.assert_allclose(B.compute_2qubit_rdm(x, a, n, q1, q2), ref, rtol=0, atol=tol)
+    # the inputs are left as they were
+    assert np.array_equal(xt.cpu().numpy().view(np.uint64), x)
