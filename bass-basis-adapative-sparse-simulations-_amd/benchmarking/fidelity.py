@@ -38,3 +38,11 @@ def fidelity_bass(sim, state, exact_sv):
         np.ascontiguousarray(exact_sv, dtype=np.complex128))
     ex = ex.to(device=psi.device, dtype=torch.complex128)
     return float(abs(_ops.dense_vdot(sim._ctx, ex, psi)) ** 2)
+
+
+def estimate_cross_entropy_fidelity(sparse_state, exact_state, num_samples=1000):
+    """fidelity.py:92-102: cross-entropy benchmark XEB = 2^N <P_exact(x)> - 1
+    over num_samples keys x sampled from the sparse state (its sample())."""
+    keys = np.asarray(sparse_state.sample(num_samples), dtype=np.uint64).astype(np.int64)
+    p = np.abs(np.asarray(exact_state)[keys]) ** 2
+    return float(2 ** sparse_state.N * np.mean(p) - 1)

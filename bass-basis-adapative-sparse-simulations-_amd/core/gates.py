@@ -122,3 +122,21 @@ class HaarSingleQubitGate(SingleQubitGate):
 
     def __init__(self, qubit, rng):
         super().__init__(qubit, haar_unitary_2x2(rng))
+
+
+def haar_random_unitary(dimension, seed=None):
+    """gates.py:152-164: Haar-random unitary of the given dimension from scipy's
+    unitary_group on numpy's legacy global generator, which a seed reseeds
+    first (as the reference does, so seeded draws match it)."""
+    from scipy.stats import unitary_group
+
+    if seed is not None:
+        np.random.seed(seed)
+    return unitary_group.rvs(dimension)
+
+
+class RandomTwoQubitGate(TwoQubitGate):
+    """gates.py:142-149: Haar-random 2-qubit gate, haar_random_unitary(4, seed)."""
+
+    def __init__(self, qubit1, qubit2, seed=None):
+        super().__init__(qubit1, qubit2, haar_random_unitary(4, seed))

@@ -85,6 +85,16 @@ def make_brickwork_2d(rows, cols, depth, rng):
     return gates
 
 
+def make_haar(N, depth, rng):
+    """runner.py:162-165: random-pairing Haar circuit; one seed drawn from rng
+    reseeds numpy's legacy global generator, which generate_random_circuit
+    draws from."""
+    from ..utils.random_circuits import generate_random_circuit
+
+    np.random.seed(int(rng.integers(0, 2 ** 31)))
+    return generate_random_circuit(N, depth)
+
+
 def make_qft(N, rng=None):
     """runner.py:168-185: H on qubit i, then controlled phases pi / 2^(j-i) to
     the next four qubits."""

@@ -9,7 +9,7 @@ reference's gate classes.
 
 import numpy as np
 
-from ..core.gates import CZGate, HaarSingleQubitGate, RXGate, RZGate, RZZGate
+from ..core.gates import CZGate, HaarSingleQubitGate, RandomTwoQubitGate, RXGate, RZGate, RZZGate
 
 SEED_BASE = 20260520
 
@@ -85,3 +85,54 @@ def tfim_config_circuit(num_qubits=20, steps=40, J=1.0, h=1.0, dt=0.05):
     """Uniform open-chain TFIM Trotter circuit (reference gate conventions):
     per step N-1 RZZ(J dt) then N RX(h dt)."""
     return generate_tfim_circuit(num_qubits, steps, J=J, h=h, dt=dt, rng=np.random.default_rng(0))
+
+
+def generate_random_circuit(num_qubits, num_layers, gate_type="haar"):
+    """random_circuits.py:60-97: per layer, a random pairing of the qubits
+    (numpy's legacy global shuffle) with a Haar-random 2-qubit gate on each of
+    the num_qubits // 2 pairs (drawn from the same global generator)."""
+    if gate_type != "haar":
+        raise NotImplementedError(f"Gate type {gate_type} not implemented")
+    circuit = []
+    for _ in range(num_layers):
+        order = list(range(num_qubits))
+        np.random.shuffle(order)
+        for p in range(num_qubits // 2):
+            circuit.append(RandomTwoQubitGate(order[2 * p], order[2 * p + 1]))
+    return circuit
+
+
+def generate_rfitm_circuit(num_qubits, num_layers, J=1.0, h=1.0, dt=0.1, sigma_h=0.1, rng=None):
+    """random_circuits.py:168-209: random-field transverse Ising Trotter circuit.
+    Site fields h_j ~ N(h, sigma_h^2), one draw per qubit per circuit from rng
+    (default_rng(0) when None); per layer RZZ(J dt) on the open chain, then
+    RX(h_j dt) on every qubit."""
+    if rng is None:
+        rng = np.random.default_rng(0)
+    fields = rng.normal(loc=h, scale=sigma_h, size=num_qubits)
+    circuit = []
+    for _ in range(num_layers):
+        circuit.extend(RZZGate(q, q + 1, J * dt) for q in range(num_qubits - 1))
+        circuit.extend(RXGate(q, float(fields[q]) * dt) for q in range(num_qubits))
+    return circuit
+
+
+def estimate_circuit_depth(circuit, num_qubits):
+    """random_circuits.py:352-364: longest chain of gates sharing qubits (each
+    gate starts after the latest gate on any of its qubits)."""
+    t = [0] * num_qubits
+    for g in circuit:
+        start = max(t[q] for q in g.qubits)
+        for q in g.qubits:
+            t[q] = start + 1
+    return max(t) if t else 0
+
+
+def circuit_info(circuit):
+    """random_circuits.py:334-349: gate count, qubit count (highest qubit + 1)
+    and depth."""
+    used = set()
+    for g in circuit:
+        used.update(g.qubits)
+    n = max(used) + 1 if used else 0
+    return {"num_gates": len(circuit), "num_qubits": n, "depth": estimate_circuit_depth(circuit, n)}
