@@ -29,10 +29,14 @@ def _circuits():
     out.append(("rfim-random-h", 10, rc.generate_tfim_circuit(10, 6, J=1.0, h=1.2, dt=0.2,
                                                                rng=np.random.default_rng(3),
                                                                disorder_strength_h=0.7), None))
+    # random pairing (runner.py:162-165): 2-qubit gates on arbitrary, mostly
+    # distant qubit pairs
+    out.append(("haar-random-pairing", 10, pkg_module("experimentation.runner").make_haar(
+        10, 4, np.random.default_rng(11)), None))
     return out
 
 
-@pytest.mark.parametrize("idx", [0, 1, 2])
+@pytest.mark.parametrize("idx", [0, 1, 2, 3])
 @pytest.mark.parametrize("K", [64, 256])
 def test_bass_end_to_end(gpu_ctx, idx, K):
     name, N, circ, ends = _circuits()[idx]
