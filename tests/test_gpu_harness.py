@@ -147,3 +147,39 @@ def test_phase_diagram_rows(gpu_ctx):
     assert np.all(np.abs(got[:, :2].mean(0) - ref[:, :2].mean(0)) < 2e-3), (got.mean(0), ref.mean(0))
     assert np.all(np.abs(got[:, 2:].mean(0) / ref[:, 2:].mean(0) - 1) < 1.5e-2), (got.mean(0), ref.mean(0))
     assert np.abs(got[:, :2] - ref[:, :2]).max() < 1e-2
+
+
+@pytest.mark.parametrize("fi", [1, 2, 3])
+def test_phase_diagram_haar_rows_device(gpu_ctx, fi):
+    """PhaseDiagram.ipynb rows built from Haar 2-qubit gates (1 brickwork 1D,
+    2 brickwork 2D, 3 random pairing; N=16, K=4096, default arguments with the
+    reference's accept rule): the device drivers reproduce the stored F and PR
+    per trial (1e-9); BASS rows are compared on every trial whose rotation
+    decisions stand clear of PR-neutral relabelings (decided by summation
+    order in either code), at least 14 of 20."""
+    print(f"[case] harness phase-diagram Haar family {fi} N=16 K=4096", flush=True)
+    R = pkg_module("experimentation.runner")
+    B = pkg_module("simulation.bass_simulator")
+    S = pkg_module("simulation.simulator")
+    F = pkg_module("benchmarking.fidelity")
+    d = np.load(os.path.join(REF_RESULTS, "phase_diagram_extended.npz"), allow_pickle=False)
+    N, K, T = 16, 4096, 20
+    gen = {1: lambda rng: R.make_brickwork(N, 6, rng),
+           2: lambda rng: R.make_brickwork_2d(4, 4, 4, rng),
+           3: lambda rng: R.make_haar(N, 3, rng)}[fi]
+    near = 0
+    for t in range(T):
+        circ = gen(np.random.default_rng(fi * 1000 + t))
+        sv = R.exact_statevector(N, circ)
+        fst = S.FixedBasisSimulator(N, K).simulate(circ)
+        assert abs(F.compute_fidelity(fst, sv) - d["fidelity_fixed"][fi, t]) < 1e-9, t
+        assert abs(F.compute_participation_ratio(fst) / 2 ** N / d["pr_fixed"][fi, t] - 1) < 1e-9, t
+        trace = []
+        sim = B.BASS(N, K, device="cuda:0", trace=trace, accept_rtol=0.0)
+        st = sim.simulate(circ)
+        if any(e[0] == "rot" and abs(e[4] - e[3]) <= 1e-12 * abs(e[3]) for e in trace):
+            near += 1
+            continue
+        assert abs(F.fidelity_bass(sim, st, sv) - d["fidelity_bass"][fi, t]) < 1e-9, t
+        assert abs(F.compute_participation_ratio(st) / 2 ** N / d["pr_bass"][fi, t] - 1) < 1e-9, t
+    assert T - near >= 14, near
