@@ -183,3 +183,42 @@ def test_phase_diagram_haar_rows_device(gpu_ctx, fi):
         assert abs(F.fidelity_bass(sim, st, sv) - d["fidelity_bass"][fi, t]) < 1e-9, t
         assert abs(F.compute_participation_ratio(st) / 2 ** N / d["pr_bass"][fi, t] - 1) < 1e-9, t
     assert T - near >= 14, near
+
+
+@pytest.mark.parametrize("fam", ["brickwork", "brickwork_2d"])
+def test_fidelity_pr_brickwork_rows_device(gpu_ctx, fam):
+    """fidelity_pr_raw.npz Haar rows (N=20, seed 20260520e6 + t,
+    optimize_every=5, buffer_factor=1, the reference's accept rule) on the
+    device drivers: per trial and budget the fixed-basis F and PR (1e-9
+    relative), BASS's on every run whose rotation decisions stand clear of a
+    PR-neutral relabeling (a run with one may take the other branch; most still
+    match)."""
+    print(f"[case] harness fidelity_pr {fam} N=20", flush=True)
+    R = pkg_module("experimentation.runner")
+    B = pkg_module("simulation.bass_simulator")
+    S = pkg_module("simulation.simulator")
+    F = pkg_module("benchmarking.fidelity")
+    d = np.load(os.path.join(REF_RESULTS, "fidelity_pr_raw.npz"), allow_pickle=False)
+    N = 20
+    gen = (lambda rng: R.make_brickwork(N, 6, rng)) if fam == "brickwork" else \
+        (lambda rng: R.make_brickwork_2d(4, 5, 4, rng))
+    ks = [100, 200, 500, 1000, 2000, 5000, 10000, 20000, 50000]
+    compared = 0
+    for t in range(3):
+        circ = gen(np.random.default_rng(SEED_BASE * 10 ** 6 + t))
+        sv = R.exact_statevector(N, circ)
+        for ki in (0, 3, 6):
+            k = ks[ki]
+            fst = S.FixedBasisSimulator(N, k).simulate(circ)
+            assert abs(F.compute_fidelity(fst, sv) / d[fam + "_f_fixed"][t, ki] - 1) < 1e-9, (t, k)
+            assert abs(F.compute_participation_ratio(fst) / d[fam + "_pr_fixed"][t, ki] - 1) < 1e-9, (t, k)
+            trace = []
+            sim = B.BASS(N, k, device="cuda:0", trace=trace, accept_rtol=0.0, **KW)
+            st = sim.simulate(circ)
+            near = any(e[0] == "rot" and abs(e[4] - e[3]) <= 1e-12 * abs(e[3]) for e in trace)
+            ok = (abs(F.fidelity_bass(sim, st, sv) / d[fam + "_f_bass"][t, ki] - 1) < 1e-9
+                  and abs(F.compute_participation_ratio(st) / d[fam + "_pr_bass"][t, ki] - 1) < 1e-9)
+            # a run with a PR-neutral rotation decision may follow the other branch
+            assert ok or near, (t, k)
+            compared += ok
+    assert compared >= 4, compared

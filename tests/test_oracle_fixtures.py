@@ -228,3 +228,39 @@ def test_phase_diagram_haar_rows(fi):
         pr_b = OE.compute_participation_ratio(bs.alpha[: bs.nnz]) / 2 ** PD_N
         assert abs(pr_b / d["pr_bass"][fi, t] - 1) < 1e-10, t
     assert PD_T - near >= 14, near
+
+
+@pytest.mark.parametrize("fam", ["brickwork", "brickwork_2d"])
+def test_fidelity_pr_brickwork_rows(fam):
+    """fidelity_pr_raw.npz Haar rows (Fidelity_Pr.ipynb: N=20, brickwork depth
+    6 or 4 x 5 depth 4, seed 20260520e6 + t, optimize_every=5,
+    buffer_factor=1): per trial and budget, the fixed-basis F and PR (1e-10
+    relative) and BASS's on every run whose rotation decisions stand clear of
+    a PR-neutral relabeling (a run with one may take the other branch; most
+    still match)."""
+    d = _load("fidelity_pr_raw.npz")
+    R = pkg_module("experimentation.runner")
+    N = 20
+    gen = (lambda rng: R.make_brickwork(N, 6, rng)) if fam == "brickwork" else \
+        (lambda rng: R.make_brickwork_2d(4, 5, 4, rng))
+    ks = [100, 200, 500, 1000, 2000, 5000, 10000, 20000, 50000]
+    compared = 0
+    for t in range(2):
+        circ = gen(np.random.default_rng(SEED_BASE * 10 ** 6 + t))
+        sv = OE.exact_statevector(N, circ)
+        for ki in (0, 3):
+            k = ks[ki]
+            fs = OD.FixedBasisSimulator(N, k).simulate(circ, seed=0)
+            f = OE.compute_fidelity(fs.x[: fs.nnz], fs.alpha[: fs.nnz], sv)
+            assert abs(f / d[fam + "_f_fixed"][t, ki] - 1) < 1e-10, (t, k)
+            assert abs(OE.compute_participation_ratio(fs.alpha[: fs.nnz]) / d[fam + "_pr_fixed"][t, ki] - 1) < 1e-10
+            trace = []
+            b = OD.BASS(N, k, optimize_every=5, buffer_factor=1, trace=trace)
+            bs = b.simulate(circ)
+            fb = OE.fidelity_bass(N, bs.x[: bs.nnz], bs.alpha[: bs.nnz], b.U, sv)
+            pb = OE.compute_participation_ratio(bs.alpha[: bs.nnz])
+            ok = abs(fb / d[fam + "_f_bass"][t, ki] - 1) < 1e-10 and abs(pb / d[fam + "_pr_bass"][t, ki] - 1) < 1e-10
+            # a run with a PR-neutral rotation decision may follow the other branch
+            assert ok or near_tie_rotation(trace), (t, k)
+            compared += ok
+    assert compared >= 2, compared
