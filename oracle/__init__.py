@@ -33,8 +33,9 @@ Parity pins (what checks each function; the tests named live in ``tests/``):
 * RFIM and QAOA generators plus the dense simulator: the exact-state PR and
   half-chain entropy of every trial of ``phase_diagram_extended.npz``; the RFIM
   driver rows there as 20-trial means; its Haar families (brickwork 1D / 2D,
-  random pairing; tie-free) per trial for both drivers, optimiser included
-  (``test_oracle_fixtures.py``);
+  random pairing; tie-free) per trial for both drivers, optimiser included;
+  the 2-qubit brick-wall pass end to end against ``gamma_diagnostics.npz``'s
+  tie-free rows (``test_oracle_fixtures.py``);
 * random-k truncation (LCG partial Fisher-Yates): closed-form LCG sequence.
 
 Parity unpinned (no reference-held result or closed form decides them):
@@ -43,8 +44,6 @@ Parity unpinned (no reference-held result or closed form decides them):
   and outside Numba): ``argpartition_selector`` reproduces numpy's choice on
   this host only, and numpy's own choice differs between its SIMD dispatch
   paths; single trials that truncate through ties are compared as means;
-* the 2-qubit brick-wall pass end to end (its kernels, RDM and eigensolver are
-  pinned; the reference's QAOA results with it depend on tie order);
 * the FixedBasis random-k draw (``np.random.choice`` on the legacy generator;
   its only stored results, ``schmidt_benchmark.npz``, were not replayed).
 """

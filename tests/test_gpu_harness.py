@@ -222,3 +222,34 @@ def test_fidelity_pr_brickwork_rows_device(gpu_ctx, fam):
             assert ok or near, (t, k)
             compared += ok
     assert compared >= 4, compared
+
+
+@pytest.mark.parametrize("fi", [0, 1])
+def test_gamma_diagnostics_2q_pass_rows_device(gpu_ctx, fi):
+    """gamma_diagnostics.npz tie-free families (fi = 0 random-pairing Haar
+    depth 3, 1 brickwork depth 5; use_2qubit_rotations=True, the reference's
+    accept rule) on the device driver, N = 14 and 16: per run the stored
+    gamma^2 and true F (1e-9), the 2-qubit brick-wall pass included; a run with
+    a PR-neutral decision may take the other branch."""
+    print(f"[case] harness gamma diagnostics family {fi}", flush=True)
+    R = pkg_module("experimentation.runner")
+    B = pkg_module("simulation.bass_simulator")
+    F = pkg_module("benchmarking.fidelity")
+    d = np.load(os.path.join(REF_RESULTS, "gamma_diagnostics.npz"), allow_pickle=False)
+    matched = 0
+    for ni, N in ((0, 14), (1, 16)):
+        gen = (lambda rng: R.make_haar(N, 3, rng)) if fi == 0 else (lambda rng: R.make_brickwork(N, 5, rng))
+        for t in range(3):
+            circ = gen(np.random.default_rng(42 * 10 ** 6 + ni * 1000 + fi * 100 + t))
+            sv = R.exact_statevector(N, circ)
+            for ki, K in ((0, 512), (1, 1024)):
+                trace = []
+                sim = B.BASS(N, K, device="cuda:0", trace=trace, accept_rtol=0.0, use_2qubit_rotations=True)
+                st = sim.simulate(circ)
+                assert any(e[0] == "rot2" for e in trace)
+                near = any(e[0] in ("rot", "rot2") and abs(e[4] - e[3]) <= 1e-12 * abs(e[3]) for e in trace)
+                ok = (abs(st.gamma ** 2 - d["gamma_sq"][fi, ni, ki, t]) < 1e-9
+                      and abs(F.fidelity_bass(sim, st, sv) - d["true_f"][fi, ni, ki, t]) < 1e-9)
+                assert ok or near, (N, t, K)
+                matched += ok
+    assert matched >= 9, matched

@@ -191,9 +191,10 @@ def _pd_haar_families():
 
 
 def near_tie_rotation(trace, rel=1e-12):
-    """A rotation decision whose PR change is within rounding: the rotation
-    only relabels the state, and accept / reject follows summation order."""
-    return any(e[0] == "rot" and abs(e[4] - e[3]) <= rel * abs(e[3]) for e in trace)
+    """A rotation decision (1- or 2-qubit) whose PR change is within rounding:
+    the rotation only relabels the state, and accept / reject follows
+    summation order."""
+    return any(e[0] in ("rot", "rot2") and abs(e[4] - e[3]) <= rel * abs(e[3]) for e in trace)
 
 
 @pytest.mark.parametrize("fi", [1, 2, 3])
@@ -264,3 +265,31 @@ def test_fidelity_pr_brickwork_rows(fam):
             assert ok or near_tie_rotation(trace), (t, k)
             compared += ok
     assert compared >= 2, compared
+
+
+@pytest.mark.parametrize("fi", [0, 1])
+def test_gamma_diagnostics_2q_pass_rows(fi):
+    """gamma_diagnostics.npz (GammaOverstimationDiagnostics.ipynb: BASS with
+    use_2qubit_rotations=True, seed 42e6 + ni*1000 + fi*100 + t) on its
+    tie-free families, fi = 0 random-pairing Haar (depth 3) and 1 brickwork
+    (depth 5), N = 14: per run the stored gamma^2 and true F (1e-10), the
+    2-qubit brick-wall pass included; a run with a PR-neutral decision may
+    take the other branch."""
+    d = _load("gamma_diagnostics.npz")
+    R = pkg_module("experimentation.runner")
+    N, ni = 14, 0
+    gen = (lambda rng: R.make_haar(N, 3, rng)) if fi == 0 else (lambda rng: R.make_brickwork(N, 5, rng))
+    matched = 0
+    for t in range(4):
+        circ = gen(np.random.default_rng(42 * 10 ** 6 + ni * 1000 + fi * 100 + t))
+        sv = OE.exact_statevector(N, circ)
+        for ki, K in ((0, 512), (1, 1024)):
+            trace = []
+            b = OD.BASS(N, K, use_2qubit_rotations=True, trace=trace)
+            bs = b.simulate(circ)
+            assert any(e[0] == "rot2" for e in trace)
+            f = OE.fidelity_bass(N, bs.x[: bs.nnz], bs.alpha[: bs.nnz], b.U, sv)
+            ok = abs(bs.gamma ** 2 - d["gamma_sq"][fi, ni, ki, t]) < 1e-10 and abs(f - d["true_f"][fi, ni, ki, t]) < 1e-10
+            assert ok or near_tie_rotation(trace), (t, K)
+            matched += ok
+    assert matched >= 6, matched
