@@ -36,11 +36,18 @@ struct SelState {
   unsigned long long kp;         // wide path: resolved high bits of the key threshold
   unsigned int tcount;           // wide path: tied candidates collected
   unsigned int tie;              // wide path: the key tie-break is needed
+  // staged finish (k_finish_staged): block tickets, finished blocks and the
+  // published flag of every stage, and an abort word for error exits
+  unsigned int stg_ticket;
+  unsigned int stg_abort;
+  unsigned int stg_done[12];
+  unsigned int stg_flag[12];
 };
 
-// k_finish resolves at most FIN_MAX digit-2 candidates in one block; beyond
-// that the call reports OpStats.flags bit 1 and is repeated with the wide
-// (multi-block) path.
+// One block resolves at most FIN_MAX digit-2 candidates.  Beyond that a single
+// problem's select runs the staged multi-block stages of k_finish_staged in
+// the same launch; a batch's k_finish reports OpStats.flags bit 1 and the call
+// is repeated with the wide (multi-launch) path.
 constexpr unsigned int FIN_MAX = 1u << 16;
 constexpr int WIDE_LEVELS = 10;  // 4 |a|^2 digits + 6 key digits
 

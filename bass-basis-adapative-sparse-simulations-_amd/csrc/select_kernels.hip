@@ -552,33 +552,32 @@ __global__ __launch_bounds__(NT) void k_extract2_agg(SelProblem P, SelTiles T, i
   }
 }
 
+// Digits of the multi-block resolutions (staged finish, wide path): |a|^2
+// digits at bits 30..40, 19..29, 8..18, 0..7; then, if entries equal to the
+// threshold must be split, a radix select of the smallest logical keys over
+// the collected tied keys: bits 53..63, 42..52, 31..41, 20..30, 9..19, 0..8.
+__device__ __constant__ int kWideShift[WIDE_LEVELS] = {30, 19, 8, 0, 53, 42, 31, 20, 9, 0};
+__device__ __constant__ int kWideWidth[WIDE_LEVELS] = {11, 11, 11, 8, 11, 11, 11, 11, 11, 9};
+
 // Resolve the remaining bits of the threshold from the (small) digit-2
 // candidate list, then the key tie-break among entries equal to it.  One block
 // of NTB threads per problem; the logical keys of the tied entries are
 // gathered once into LDS when they fit (FIN_TIES), else re-read per level.
 constexpr int FIN_TIES = 2048;
-template <bool BATCH>
-__global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProblem one) {
-  __shared__ unsigned int s_hist[HIST_BINS];
-  __shared__ int s_scan[WAVESB];
-  __shared__ unsigned int s_b;
-  __shared__ unsigned long long s_ab;
-  __shared__ uint64_t s_keys[FIN_TIES];
-  __shared__ unsigned int s_nk;
-  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
+struct FinLds {
+  unsigned int hist[HIST_BINS];
+  int scan[WAVESB];
+  unsigned int b;
+  unsigned long long ab;
+  uint64_t keys[FIN_TIES];
+  unsigned int nk;
+};
+
+// The single-block resolution of problem P (every thread of the block calls it).
+__device__ __forceinline__ void finish_block(const SelProblem& P, FinLds& S) {
   SelState* sel = P.sel;
   OpStats* st = P.st;
-  if (sel->keepall) return;  // pstar / keystar set by k_find1
   const unsigned int m2 = sel->cand2;
-  if (m2 > FIN_MAX && !P.finish_any) {
-    // too many for one block: keep nothing, ask for the wide path
-    if (threadIdx.x == 0) {
-      sel->pstar = ~0ull;
-      sel->keystar = 0;
-      atomicOr(&st->flags, 2);
-    }
-    return;
-  }
   const unsigned long long* c2p = P.c2p;
   const unsigned int* c2i = P.c2i;
   unsigned long long need = sel->need;
@@ -588,27 +587,27 @@ __global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProb
   unsigned int cnt_eq = 0;
   for (int lv = 0; lv < 4; ++lv) {
     const int sh = shifts[lv], wd = widths[lv];
-    for (int b = threadIdx.x; b < HIST_BINS; b += NTB) s_hist[b] = 0;
+    for (int b = threadIdx.x; b < HIST_BINS; b += NTB) S.hist[b] = 0;
     if (threadIdx.x == 0) {
-      s_b = 0xFFFFFFFFu;
-      s_ab = 0;
+      S.b = 0xFFFFFFFFu;
+      S.ab = 0;
     }
     __syncthreads();
     const unsigned long long hi = prefix >> (sh + wd);
     for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
       const unsigned long long bits = c2p[j];
       if ((bits >> (sh + wd)) == hi)
-        atomicAdd(&s_hist[(unsigned int)(bits >> sh) & ((1u << wd) - 1u)], 1u);
+        atomicAdd(&S.hist[(unsigned int)(bits >> sh) & ((1u << wd) - 1u)], 1u);
     }
     __syncthreads();
-    find_from_top<NTB>(s_hist, need, &s_b, &s_ab, s_scan);
-    if (s_b == 0xFFFFFFFFu) {
+    find_from_top<NTB>(S.hist, need, &S.b, &S.ab, S.scan);
+    if (S.b == 0xFFFFFFFFu) {
       if (threadIdx.x == 0) set_err(&st->err, ERR_SELECT);
       return;
     }
-    prefix |= (unsigned long long)s_b << sh;
-    need -= s_ab;
-    cnt_eq = s_hist[s_b];
+    prefix |= (unsigned long long)S.b << sh;
+    need -= S.ab;
+    cnt_eq = S.hist[S.b];
     __syncthreads();
   }
   const unsigned long long pstar = prefix;
@@ -617,46 +616,46 @@ __global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProb
     // need-th smallest key among entries whose pattern equals pstar
     const bool in_lds = cnt_eq <= (unsigned int)FIN_TIES;
     if (in_lds) {
-      if (threadIdx.x == 0) s_nk = 0;
+      if (threadIdx.x == 0) S.nk = 0;
       __syncthreads();
       for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
         if (c2p[j] != pstar) continue;
-        const unsigned int slot = atomicAdd(&s_nk, 1u);
-        if (slot < (unsigned int)FIN_TIES) s_keys[slot] = logical_key(P.x[c2i[j]], P.km);
+        const unsigned int slot = atomicAdd(&S.nk, 1u);
+        if (slot < (unsigned int)FIN_TIES) S.keys[slot] = logical_key(P.x[c2i[j]], P.km);
       }
       __syncthreads();
     }
-    const unsigned int nk = in_lds ? s_nk : 0u;
+    const unsigned int nk = in_lds ? S.nk : 0u;
     unsigned long long kp = 0;
     for (int sh = 56; sh >= 0; sh -= 8) {
-      for (int b = threadIdx.x; b < HIST_BINS; b += NTB) s_hist[b] = 0;
+      for (int b = threadIdx.x; b < HIST_BINS; b += NTB) S.hist[b] = 0;
       if (threadIdx.x == 0) {
-        s_b = 0xFFFFFFFFu;
-        s_ab = 0;
+        S.b = 0xFFFFFFFFu;
+        S.ab = 0;
       }
       __syncthreads();
       if (in_lds) {
         for (unsigned int e = threadIdx.x; e < nk; e += NTB) {
-          const uint64_t key = s_keys[e];
+          const uint64_t key = S.keys[e];
           const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
-          if (match) atomicAdd(&s_hist[(unsigned int)(key >> sh) & 255u], 1u);
+          if (match) atomicAdd(&S.hist[(unsigned int)(key >> sh) & 255u], 1u);
         }
       } else {
         for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
           if (c2p[j] != pstar) continue;
           const uint64_t key = logical_key(P.x[c2i[j]], P.km);
           const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
-          if (match) atomicAdd(&s_hist[(unsigned int)(key >> sh) & 255u], 1u);
+          if (match) atomicAdd(&S.hist[(unsigned int)(key >> sh) & 255u], 1u);
         }
       }
       __syncthreads();
-      find_from_bottom<NTB>(s_hist, need, &s_b, &s_ab, s_scan);
-      if (s_b == 0xFFFFFFFFu) {
+      find_from_bottom<NTB>(S.hist, need, &S.b, &S.ab, S.scan);
+      if (S.b == 0xFFFFFFFFu) {
         if (threadIdx.x == 0) set_err(&st->err, ERR_SELECT);
         return;
       }
-      kp |= (unsigned long long)s_b << sh;
-      need -= s_ab;
+      kp |= (unsigned long long)S.b << sh;
+      need -= S.ab;
       __syncthreads();
     }
     keystar = kp;
@@ -667,16 +666,244 @@ __global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProb
   }
 }
 
+template <bool BATCH>
+__global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProblem one) {
+  __shared__ FinLds S;
+  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
+  SelState* sel = P.sel;
+  if (sel->keepall) return;  // pstar / keystar set by k_find1
+  if (sel->cand2 > FIN_MAX && !P.finish_any) {
+    // too many for one block: keep nothing, ask for the wide path
+    if (threadIdx.x == 0) {
+      sel->pstar = ~0ull;
+      sel->keystar = 0;
+      atomicOr(&P.st->flags, 2);
+    }
+    return;
+  }
+  finish_block(P, S);
+}
+
+// ------------------------------------------------------ staged finish ----
+// One problem, any digit-2 candidate count, one launch.  Up to FIN_MAX
+// candidates block 0 runs finish_block and the other blocks exit.  Beyond
+// that (massive |a|^2 ties at the threshold) the blocks run STG_STAGES stages
+// of STG_BLOCKS blocks each:
+//   stages 0-3: histograms of the |a|^2 digits at bits 30..40, 19..29, 8..18,
+//               0..7 over the candidates matching the resolved prefix;
+//   stage 4:    the logical keys of the candidates equal to the threshold,
+//               appended to P.tk (only when the key tie-break is needed);
+//   stages 5-10: histograms of the key digits at bits 53..63, 42..52, 31..41,
+//               20..30, 9..19, 0..8 over the collected keys.
+// Stage s uses its own zeroed histogram (hist_w + HIST_BINS * level).  The
+// last block of a stage to finish (a per-stage counter) resolves its digit,
+// publishes the result in the select state and sets the stage's flag; blocks
+// of stage s + 1 wait for that flag before they read anything.  A block's
+// stage comes from an atomic ticket in dispatch order, so every block of stage
+// s had started before any block of stage s + 1 waits on it, and stage-s
+// blocks wait only on stage s - 1: every wait ends.  A spin bound turns a
+// broken invariant into an error bit instead of a hang.  Data crossing blocks
+// inside the launch (histogram atomics, the collected keys, the published
+// digits) is ordered by agent-scope release fences before each counter
+// increment / flag store and acquire fences after each counter / flag read.
+constexpr int STG_BLOCKS = 64;
+constexpr int STG_STAGES = 11;
+constexpr int STG_COLLECT = 4;
+constexpr unsigned int STG_SPIN = 1u << 24;
+constexpr int STG_U = 8;  // candidate loads in flight per thread
+static_assert(STG_STAGES <= 12, "stage flags of SelState");
+
+__device__ __forceinline__ unsigned int ld_acq_u32(unsigned int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Publish the end of stages [from, STG_STAGES) at once (error exits and a
+// threshold without a key tie-break release every later waiter).
+__device__ __forceinline__ void stg_release_from(SelState* sel, int from) {
+  __threadfence();
+  for (int s = from; s < STG_STAGES; ++s)
+    __hip_atomic_store(&sel->stg_flag[s], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
+  __shared__ FinLds S;
+  __shared__ unsigned int s_t, s_flag, s_base;
+  SelState* sel = P.sel;
+  OpStats* st = P.st;
+  if (sel->keepall) return;  // pstar / keystar set by k_find1
+  const unsigned int m2 = sel->cand2;
+  if (m2 <= FIN_MAX) {
+    if (blockIdx.x == 0) finish_block(P, S);
+    return;
+  }
+  const int tid = threadIdx.x;
+  if (tid == 0) s_t = atomicAdd(&sel->stg_ticket, 1u);
+  __syncthreads();
+  const unsigned int ticket = s_t;
+  const int stage = (int)(ticket / STG_BLOCKS), part = (int)(ticket % STG_BLOCKS);
+  if (stage >= STG_STAGES) return;  // the grid holds exactly STG_STAGES * STG_BLOCKS blocks
+  if (stage > 0) {
+    if (tid == 0) {
+      unsigned int spins = 0, ok = 1;
+      while (ld_acq_u32(&sel->stg_flag[stage - 1]) == 0u) {
+        if (++spins > STG_SPIN) {
+          set_err(&st->err, ERR_LOOKBACK);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(4);
+      }
+      s_flag = ok && ld_acq_u32(&sel->stg_abort) == 0u;
+    }
+    __syncthreads();
+    if (!s_flag) return;
+    __threadfence();  // acquire what the earlier stages published
+  }
+  const unsigned long long prefix = __hip_atomic_load(&sel->prefix, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long need = __hip_atomic_load(&sel->need, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long kp = __hip_atomic_load(&sel->kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long pstar = __hip_atomic_load(&sel->pstar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned int tie = ld_acq_u32(&sel->tie);
+  const unsigned int tcount = ld_acq_u32(&sel->tcount);
+  if (stage >= STG_COLLECT && !tie) return;  // released by stage 3 without a tie-break
+  const int lvh = stage < STG_COLLECT ? stage : stage - 1;  // wide level / histogram
+  unsigned int* hist = P.hist_w + (size_t)lvh * HIST_BINS;
+  const unsigned int stride = (unsigned int)STG_BLOCKS * NTB;
+  if (stage != STG_COLLECT) {
+    for (int b = tid; b < HIST_BINS; b += NTB) S.hist[b] = 0;
+    __syncthreads();
+    const int sh = kWideShift[lvh], wd = kWideWidth[lvh];
+    const unsigned int mask = (1u << wd) - 1u;
+    if (stage < STG_COLLECT) {
+      const unsigned long long hi = prefix >> (sh + wd);
+      for (unsigned int j0 = (unsigned int)part * NTB * STG_U; j0 < m2; j0 += stride * STG_U) {
+        unsigned long long bv[STG_U];
+#pragma unroll
+        for (int u = 0; u < STG_U; ++u) {
+          const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+          bv[u] = P.c2p[j < m2 ? j : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < STG_U; ++u) {
+          const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+          if (j < m2 && (bv[u] >> (sh + wd)) == hi) atomicAdd(&S.hist[(unsigned int)(bv[u] >> sh) & mask], 1u);
+        }
+      }
+    } else {
+      const int top = sh + wd;  // 64 at the first key level
+      for (unsigned int j0 = (unsigned int)part * NTB * STG_U; j0 < tcount; j0 += stride * STG_U) {
+        uint64_t kv[STG_U];
+#pragma unroll
+        for (int u = 0; u < STG_U; ++u) {
+          const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+          kv[u] = P.tk[j < tcount ? j : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < STG_U; ++u) {
+          const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+          if (j < tcount && (top >= 64 || (kv[u] >> top) == (kp >> top)))
+            atomicAdd(&S.hist[(unsigned int)(kv[u] >> sh) & mask], 1u);
+        }
+      }
+    }
+    __syncthreads();
+    for (int b = tid; b < HIST_BINS; b += NTB)
+      if (S.hist[b]) atomicAdd(&hist[b], S.hist[b]);
+  } else {
+    // tied candidates -> logical keys; one list reservation per round of
+    // NTB * STG_U candidates (a block scan of the per-thread counts)
+    for (unsigned int j0 = (unsigned int)part * NTB * STG_U; j0 < m2; j0 += stride * STG_U) {
+      bool take[STG_U];
+      int c = 0;
+#pragma unroll
+      for (int u = 0; u < STG_U; ++u) {
+        const unsigned int j = j0 + (unsigned int)(tid * STG_U + u);
+        take[u] = j < m2 && P.c2p[j] == pstar;
+        c += take[u] ? 1 : 0;
+      }
+      int tot;
+      int off = block_excl_scan<NTB>(c, tot, S.scan);
+      if (tid == 0) s_base = tot > 0 ? atomicAdd(&sel->tcount, (unsigned int)tot) : 0u;
+      __syncthreads();
+      const unsigned int base = s_base;
+#pragma unroll
+      for (int u = 0; u < STG_U; ++u) {
+        if (take[u]) {
+          const unsigned int j = j0 + (unsigned int)(tid * STG_U + u);
+          P.tk[base + (unsigned int)off] = logical_key(P.x[P.c2i[j]], P.km);
+          ++off;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // this block's share is done: release it, count it; the last block goes on
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    __threadfence();
+    s_flag = atomicAdd(&sel->stg_done[stage], 1u) == (unsigned int)(STG_BLOCKS - 1);
+  }
+  __syncthreads();
+  if (!s_flag) return;
+  __threadfence();  // acquire every block's share of the stage
+  if (stage == STG_COLLECT) {
+    if (tid == 0) {
+      __threadfence();
+      __hip_atomic_store(&sel->stg_flag[stage], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return;
+  }
+  for (int b = tid; b < HIST_BINS; b += NTB) S.hist[b] = ld_acq_u32(&hist[b]);
+  if (tid == 0) {
+    S.b = 0xFFFFFFFFu;
+    S.ab = 0;
+  }
+  __syncthreads();
+  if (stage < STG_COLLECT)
+    find_from_top<NTB>(S.hist, need, &S.b, &S.ab, S.scan);
+  else
+    find_from_bottom<NTB>(S.hist, need, &S.b, &S.ab, S.scan);
+  if (tid != 0) return;
+  const unsigned int b = S.b;
+  if (b == 0xFFFFFFFFu) {
+    set_err(&st->err, ERR_SELECT);
+    __hip_atomic_store(&sel->stg_abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    stg_release_from(sel, stage);
+    return;
+  }
+  const int sh = kWideShift[lvh];
+  const unsigned long long nneed = need - S.ab;
+  sel->need = nneed;
+  if (stage < STG_COLLECT) {
+    const unsigned long long np = prefix | ((unsigned long long)b << sh);
+    sel->prefix = np;
+    if (stage == STG_COLLECT - 1) {
+      const unsigned int tied = nneed < (unsigned long long)S.hist[b] ? 1u : 0u;
+      sel->pstar = np;
+      sel->keystar = ~0ull;
+      sel->kp = 0;
+      sel->tie = tied;
+      if (!tied) {
+        stg_release_from(sel, stage);
+        return;
+      }
+    }
+  } else {
+    const unsigned long long nkp = kp | ((unsigned long long)b << sh);
+    sel->kp = nkp;
+    if (stage == STG_STAGES - 1) sel->keystar = nkp;
+  }
+  __threadfence();
+  __hip_atomic_store(&sel->stg_flag[stage], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // ------------------------------------------------------------- wide path --
-// Multi-block resolution of the digit-2 candidates (P.c2p / P.c2i, count
-// sel->cand2), for inputs with very many equal |a|^2 near the threshold.
-// |a|^2 digits: bits 30..40, 19..29, 8..18, 0..7; then, if entries equal to
-// the threshold must be split, a radix select of the smallest logical keys
-// over the collected tied keys: bits 53..63, 42..52, 31..41, 20..30, 9..19, 0..8.
-// One histogram per problem (P.hist_w): each find clears it for the next level.
+// Multi-launch resolution of the digit-2 candidates (P.c2p / P.c2i, count
+// sel->cand2) for a batch of problems whose threshold buckets hold very many
+// equal |a|^2 (the levels of kWideShift / kWideWidth).  One histogram per
+// problem (P.hist_w): each find clears it for the next level.
 constexpr int WIDE_BLOCKS = 256;
-__device__ __constant__ int kWideShift[WIDE_LEVELS] = {30, 19, 8, 0, 53, 42, 31, 20, 9, 0};
-__device__ __constant__ int kWideWidth[WIDE_LEVELS] = {11, 11, 11, 8, 11, 11, 11, 11, 11, 9};
 
 __device__ __forceinline__ void flush_hist(unsigned int* s_hist, unsigned int* hist) {
   __syncthreads();
@@ -1261,7 +1488,9 @@ static void select_sequence(const SelProblem* probs, int nprob, const SelProblem
     hipLaunchKernelGGL(k_extract2_agg, dim3((unsigned)((waves + WAVES - 1) / WAVES)), dim3(NT), 0, s, one, T,
                        tiles_end, tpw);
   }
-  if (!wide) {
+  if (!wide && !BATCH) {
+    hipLaunchKernelGGL(k_finish_staged, dim3(STG_STAGES * STG_BLOCKS), dim3(NTB), 0, s, one);
+  } else if (!wide) {
     hipLaunchKernelGGL((k_finish<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one);
   } else {
     const dim3 wg(WIDE_BLOCKS, (unsigned)nprob);

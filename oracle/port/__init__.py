@@ -18,6 +18,8 @@ def load():
         P = ctypes.c_void_p
         L.cpu_step_1q.restype = ctypes.c_int64
         L.cpu_step_1q.argtypes = [P, P, ctypes.c_int64, ctypes.c_int, P, ctypes.c_int64, P, P, P, P]
+        L.cpu_gate_1q.restype = ctypes.c_int64
+        L.cpu_gate_1q.argtypes = [P, P, ctypes.c_int64, ctypes.c_int, P, P, P]
         L.cpu_radix_sort.restype = None
         L.cpu_radix_sort.argtypes = [P, P, ctypes.c_int64]
         _LIB = L
@@ -35,6 +37,16 @@ def step_1q(x, a, n, q, G, k, tx, ta):
     m = L.cpu_step_1q(x.ctypes.data, a.ctypes.data, int(n), int(q), g.ctypes.data, int(k), tx.ctypes.data,
                       ta.ctypes.data, ctypes.byref(gf), ctypes.byref(mg))
     return int(m), gf.value, int(mg.value)
+
+
+def gate_1q(x, a, n, q, G, xo, ao):
+    """Emit, 16-bit radix sort and merge of a 1-qubit gate on x[:n] / a[:n]
+    into xo / ao (room for 2n entries); no truncation.  Returns the merged
+    count."""
+    L = load()
+    g = np.ascontiguousarray(np.asarray(G, dtype=np.complex128)).view(np.float64)
+    return int(L.cpu_gate_1q(x.ctypes.data, a.ctypes.data, int(n), int(q), g.ctypes.data, xo.ctypes.data,
+                             ao.ctypes.data))
 
 
 def radix_sort(x, a, n):

@@ -149,6 +149,17 @@ int64_t cpu_step_1q(uint64_t* x, double* a, int64_t n, int q, const double* G, i
   return m;
 }
 
+// The gate part of cpu_step_1q alone: emit, sort and merge of (x, a) of n
+// entries into (xo, ao) (room for 2n entries each), no truncation and no
+// renormalisation.  Returns the merged count.
+int64_t cpu_gate_1q(const uint64_t* x, const double* a, int64_t n, int q, const double* G, uint64_t* xo, double* ao) {
+  static thread_local std::vector<uint64_t> sk;
+  static thread_local std::vector<double> sv;
+  const int64_t m = emit(x, a, n, q, G, xo, ao);
+  radix_sort(xo, ao, m, sk, sv);
+  return merge(xo, ao, m);
+}
+
 // The reference's radix_sort_uint64 alone (src/core/sparse_state.py:150-172),
 // in place on (keys, a: re, im interleaved).
 void cpu_radix_sort(uint64_t* keys, double* a, int64_t n) {

@@ -309,3 +309,54 @@ def test_compute_2qubit_rdm_module(gpu_ctx, n, nbits):
         np.testing.assert_allclose(B.compute_2qubit_rdm(x, a, n, q1, q2), ref, rtol=0, atol=tol)
     # the inputs are left as they were
     assert np.array_equal(xt.cpu().numpy().view(np.uint64), x)
+
+
+@pytest.mark.parametrize("n,k,ordered", [(300000, 123457, False), (200000, 150001, True), (2100000, 1048577, True)])
+def test_truncate_counted_massive_ties(gpu_ctx, n, k, ordered):
+    """Counted truncation (the driver's gate -> truncate pair with the count
+    kept on the device) of a gate output whose threshold bucket holds far more
+    than FIN_MAX = 65536 exactly tied entries: the threshold and the key
+    tie-break are resolved inside the truncate's own launches (multi-block
+    stages), with no hand-off to the host.  Against the key-tie-break oracle:
+    identical kept keys (bit for bit: every tie is exact), amplitudes within
+    1e-12 after the renormalisation; the op log records the truncation and the
+    state-count slot holds k."""
+    ops, dev, SS, L = _mods()
+    print(f"[case] counted massive ties n={n} k={k} ordered={ordered}", flush=True)
+    nbits = 40
+    rng = np.random.default_rng(n + 17)
+    x = np.unique(rng.integers(0, 2 ** nbits, size=n + 1000, dtype=np.uint64))[:n]
+    vals = np.array([0.5, 0.25 + 0.25j, 0.125], dtype=np.complex128)
+    a = vals[rng.integers(0, 3, size=n)]
+    a[rng.permutation(n)[: n // 2]] = vals[1]  # the threshold lands inside this tie class
+    pos = rng.permutation(nbits) if ordered else np.arange(nbits)
+    w, aw = _ordered(x, a, pos)
+    q = int(rng.integers(nbits))
+    j = int(pos[q])
+    # diag(1, i): every output is its input times 1 or i, exactly (ties stay exact)
+    G = np.diag([1.0, 1j]).astype(np.complex128)
+    src = _upload(dev, gpu_ctx, w, aw)
+    seg = dev.Buffers(1, gpu_ctx.device)
+    dst = dev.Buffers(k, gpu_ctx.device)
+    ops.counted_begin(gpu_ctx, n, n)
+    tiles = ops.lift_counted(gpu_ctx, src, n, j, seg, G=G, nq=1)
+    npos = SS.lifted_positions(pos, j, 0)
+    ops.truncate_counted(gpu_ctx, seg, k, dst, tiles, key_map=ops.key_map_arg(npos))
+    log = ops.counted_read(gpu_ctx, 8)
+    assert len(log) == 1
+    st = log[0]
+    assert st.err == 0 and st.nnz == k and st.n_in == n and (st.flags & 1) and not (st.flags & 2)
+    gw = dev.to_host_keys(dst.x, k)
+    assert np.all(gw[1:] > gw[:-1]), "truncation must preserve order"
+    gx = SS.stored_to_keys(gw, npos)
+    ga = dev.to_host_amps(dst.a, k)
+    # oracle: the same diagonal gate (exact), key-tie-break top-k, renormalised
+    ox = x.copy()
+    bit = (ox >> np.uint64(q)) & np.uint64(1)
+    oa = np.where(bit == 1, a * 1j, a)
+    OK.top_k_truncate_kernel_opt(ox, oa, n, k, selector=OD.key_tiebreak_selector)
+    kx, ka = ox[:k].copy(), oa[:k].copy()
+    kept = np.sum(np.abs(ka) ** 2)
+    assert abs(st.kept - kept) <= 1e-12 * kept
+    assert set(gx.tolist()) == set(kx.tolist())
+    assert_same_entries(gx, ga, kx, ka / np.sqrt(kept), rtol=1e-12)
