@@ -336,6 +336,12 @@ class SparseState:
             if method == "top-k":
                 self.truncate_top_k()
             else:
+                if self.dedup == "table":
+                    # random-k draws entries by array index: put them in the
+                    # order the reference's sort-and-merge gate leaves them
+                    # (ascending key, sparse_state.py:276-285) before the draw
+                    radix_sort_uint64(self._buf.x, self._buf.a, self.nnz, key_bits=max(1, self.N))
+                    self._hist_tiles = 0
                 self.truncate_random_k(seed)
 
     def apply_gate(self, gate, method="top-k", seed="None"):

@@ -79,6 +79,7 @@ struct Layout {
   size_t lb_gate, lb1, sort_status;
   size_t seg_cnt, soff, cnt1, tile_dcnt, ksum, kcnt, koff, lift_b, hist2_rows, chunks;
   size_t count_slot, oplog;  // counted mode: state entry count; op log (counter + entries)
+  size_t ktab;               // key-map byte tables of the select (KTAB_WORDS words)
   size_t big;
   size_t total;
 };
@@ -124,6 +125,7 @@ static Layout layout(int64_t S) {
   L.chunks = o; o += al((size_t)8 * 8 * (sel_chunks(stiles) + 1));
   L.count_slot = o; o += al(8);
   L.oplog = o; o += al(8 + (size_t)OPLOG_CAP * sizeof(OpStats));
+  L.ktab = o; o += al((size_t)KTAB_WORDS * 8);
   L.big = o;
   // big scratch union:
   //   gate group table: cap <= 4 n_in slots x <= 32 B, n_in <= S/4 (2q) or S/2 (1q) -> <= 32 S
@@ -571,6 +573,7 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   P.zero_hdr = at<unsigned long long>(ws, L.hist2);
   P.zero_words = (int64_t)((L.hdrB_end - L.hist2) / 8);
   P.km = km;
+  P.ktab = km.nbits ? at<uint64_t>(ws, L.ktab) : nullptr;
   const SelTiles T = tiles_view(ws, L, (unsigned long long*)big, (unsigned int*)(big + p8));
   const int tk = ktimer_begin("select", s);
   launch_select(P, T, hist_tiles > 0, wide, s);

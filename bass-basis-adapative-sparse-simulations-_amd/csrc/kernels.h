@@ -176,7 +176,12 @@ struct SelProblem {
   unsigned long long* zero_hdr;  // cleared by k_find1 before it starts (zero_words words), or nullptr
   int64_t zero_words;
   KeyMap km;             // ties are broken by the logical key
+  // byte tables of km (8 x 256 words: entry 256 b + v holds the logical bits
+  // of stored byte b = v), written by k_find1 when km.nbits > 0, or null
+  // (the map is then applied bit by bit)
+  uint64_t* ktab;
 };
+constexpr int KTAB_WORDS = 8 * 256;
 
 // Per-tile scratch shared by all problems of a launch (indexed by global tile;
 // c1p / c1i by global slot).

@@ -48,6 +48,41 @@ __device__ __forceinline__ uint64_t logical_key(uint64_t w, const KeyMap& km) {
   return r;
 }
 
+// The same through byte tables (tab: KTAB_WORDS words, global or LDS).
+__device__ __forceinline__ uint64_t logical_key_tab(uint64_t w, const uint64_t* tab) {
+  uint64_t r = 0;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) r |= tab[b * 256 + (int)((w >> (8 * b)) & 255u)];
+  return r;
+}
+
+// Logical key of stored word w of problem P.
+__device__ __forceinline__ uint64_t lkey(const SelProblem& P, uint64_t w) {
+  if (P.km.nbits == 0) return w;
+  if (P.ktab) return logical_key_tab(w, P.ktab);
+  return logical_key(w, P.km);
+}
+
+// Byte tables of P.km into P.ktab (the threads of one block share the
+// entries; the map is staged in s_q, 64 bytes of LDS).  Uniform branch.
+template <int NTHR>
+__device__ __forceinline__ void build_ktab(const SelProblem& P, unsigned char* s_q) {
+  if (!P.ktab || P.km.nbits == 0) return;
+  const int nb = P.km.nbits;
+  if (threadIdx.x < 64) s_q[threadIdx.x] = threadIdx.x < nb ? P.km.q[threadIdx.x] : 64;
+  __syncthreads();
+  for (int e = threadIdx.x; e < KTAB_WORDS; e += NTHR) {
+    const int b = e >> 8, v = e & 255;
+    uint64_t r = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int q = s_q[8 * b + i];
+      if (((v >> i) & 1) && q < 64) r |= 1ull << q;
+    }
+    P.ktab[e] = r;
+  }
+}
+
 // Find, in a 2048-bin histogram, the bucket holding the need-th largest element.
 // Block of NTHR threads, HIST_BINS/NTHR bins per thread.  Results in *b and
 // *above (count in higher buckets).  lds_scan: NTHR/64 ints.
@@ -216,12 +251,14 @@ __global__ __launch_bounds__(NTB) void k_find1(const SelProblem* probs, SelProbl
   __shared__ double s_red[WAVESB];
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_above;
+  __shared__ unsigned char s_q[64];
   const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
   // the select's zeroed header (digit-2 and wide histograms, state, tickets),
   // when the caller leaves it to this first kernel; the barrier below orders
   // it before this kernel's own writes to the state
   if (P.zero_hdr)
     for (int64_t i = threadIdx.x; i < P.zero_words; i += NTB) P.zero_hdr[i] = 0ull;
+  build_ktab<NTB>(P, s_q);
   if (threadIdx.x == 0) {
     s_b = 0xFFFFFFFFu;
     s_above = 0;
@@ -621,7 +658,7 @@ __device__ __forceinline__ void finish_block(const SelProblem& P, FinLds& S) {
       for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
         if (c2p[j] != pstar) continue;
         const unsigned int slot = atomicAdd(&S.nk, 1u);
-        if (slot < (unsigned int)FIN_TIES) S.keys[slot] = logical_key(P.x[c2i[j]], P.km);
+        if (slot < (unsigned int)FIN_TIES) S.keys[slot] = lkey(P, P.x[c2i[j]]);
       }
       __syncthreads();
     }
@@ -643,7 +680,7 @@ __device__ __forceinline__ void finish_block(const SelProblem& P, FinLds& S) {
       } else {
         for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
           if (c2p[j] != pstar) continue;
-          const uint64_t key = logical_key(P.x[c2i[j]], P.km);
+          const uint64_t key = lkey(P, P.x[c2i[j]]);
           const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
           if (match) atomicAdd(&S.hist[(unsigned int)(key >> sh) & 255u], 1u);
         }
@@ -693,8 +730,10 @@ __global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProb
 //               0..7 over the candidates matching the resolved prefix;
 //   stage 4:    the logical keys of the candidates equal to the threshold,
 //               appended to P.tk (only when the key tie-break is needed);
-//   stages 5-10: histograms of the key digits at bits 53..63, 42..52, 31..41,
-//               20..30, 9..19, 0..8 over the collected keys.
+//   stages 5-10: histograms of the key digits, from the top of the key width
+//               (KeyMap bits, else 64) down, in the fewest levels of at most 11
+//               bits, widths balanced (56-bit keys: 6 levels of 10, 10, 10,
+//               10, 10, 6 bits); stages past the last level end at once.
 // Stage s uses its own zeroed histogram (hist_w + HIST_BINS * level).  The
 // last block of a stage to finish (a per-stage counter) resolves its digit,
 // publishes the result in the select state and sets the stage's flag; blocks
@@ -706,12 +745,15 @@ __global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProb
 // inside the launch (histogram atomics, the collected keys, the published
 // digits) is ordered by agent-scope release fences before each counter
 // increment / flag store and acquire fences after each counter / flag read.
-constexpr int STG_BLOCKS = 64;
+// Tied candidates land in one histogram bin: a wave whose active lanes share
+// a bin adds them with one LDS atomic.
+constexpr int STG_BLOCKS = 256;
 constexpr int STG_STAGES = 11;
 constexpr int STG_COLLECT = 4;
+constexpr int STG_KEY_LEVELS = 6;
 constexpr unsigned int STG_SPIN = 1u << 24;
 constexpr int STG_U = 8;  // candidate loads in flight per thread
-static_assert(STG_STAGES <= 12, "stage flags of SelState");
+static_assert(STG_STAGES <= 12 && STG_STAGES == STG_COLLECT + 1 + STG_KEY_LEVELS, "stage flags of SelState");
 
 __device__ __forceinline__ unsigned int ld_acq_u32(unsigned int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -725,8 +767,36 @@ __device__ __forceinline__ void stg_release_from(SelState* sel, int from) {
     __hip_atomic_store(&sel->stg_flag[s], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Key digit `lv` (0 = most significant) of keys of kb bits: bits [lo, hi).
+__device__ __forceinline__ void stg_key_digit(int kb, int lv, int& lo, int& hi) {
+  const int nlev = (kb + 10) / 11;
+  const int w = (kb + nlev - 1) / nlev;
+  hi = kb - lv * w;
+  lo = hi - w > 0 ? hi - w : 0;
+  if (hi < 0) hi = 0;
+}
+
+// s_hist[bin] += 1 for the active lanes of the wave: the lanes sharing the
+// first remaining lane's bin are added by one atomic, for up to 4 distinct
+// bins; lanes left after that add one each.
+__device__ __forceinline__ void stg_hist_add(unsigned int* s_hist, unsigned int bin, bool act) {
+  unsigned long long am = __ballot(act);
+  const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (!am) return;  // uniform
+    const int first = __ffsll((long long)am) - 1;
+    const unsigned int b0 = (unsigned int)__shfl((int)bin, first, 64);
+    const unsigned long long same = __ballot(((am >> lane) & 1ull) && bin == b0);
+    if (lane == first) atomicAdd(&s_hist[b0], (unsigned int)__popcll(same));
+    am &= ~same;
+  }
+  if ((am >> lane) & 1ull) atomicAdd(&s_hist[bin], 1u);
+}
+
 __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
   __shared__ FinLds S;
+  __shared__ uint64_t s_ktab[KTAB_WORDS];
   __shared__ unsigned int s_t, s_flag, s_base;
   SelState* sel = P.sel;
   OpStats* st = P.st;
@@ -765,17 +835,28 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
   const unsigned long long pstar = __hip_atomic_load(&sel->pstar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const unsigned int tie = ld_acq_u32(&sel->tie);
   const unsigned int tcount = ld_acq_u32(&sel->tcount);
-  if (stage >= STG_COLLECT && !tie) return;  // released by stage 3 without a tie-break
-  const int lvh = stage < STG_COLLECT ? stage : stage - 1;  // wide level / histogram
+  // released by stage 3 without a tie-break, or by the last key level
+  if (stage >= STG_COLLECT && (!tie || ld_acq_u32(&sel->stg_flag[stage]))) return;
+  const int kb = P.km.nbits ? P.km.nbits : 64;
+  const int lvh = stage < STG_COLLECT ? stage : stage - 1;  // histogram
+  int sh, wd;
+  if (stage < STG_COLLECT) {
+    sh = kWideShift[lvh];
+    wd = kWideWidth[lvh];
+  } else {
+    int hi;
+    stg_key_digit(kb, stage - STG_COLLECT - 1, sh, hi);
+    wd = hi - sh;
+  }
   unsigned int* hist = P.hist_w + (size_t)lvh * HIST_BINS;
   const unsigned int stride = (unsigned int)STG_BLOCKS * NTB;
   if (stage != STG_COLLECT) {
     for (int b = tid; b < HIST_BINS; b += NTB) S.hist[b] = 0;
     __syncthreads();
-    const int sh = kWideShift[lvh], wd = kWideWidth[lvh];
     const unsigned int mask = (1u << wd) - 1u;
+    const int top = sh + wd;  // may be 64 at the first key level
     if (stage < STG_COLLECT) {
-      const unsigned long long hi = prefix >> (sh + wd);
+      const unsigned long long hi = prefix >> top;
       for (unsigned int j0 = (unsigned int)part * NTB * STG_U; j0 < m2; j0 += stride * STG_U) {
         unsigned long long bv[STG_U];
 #pragma unroll
@@ -786,11 +867,10 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
 #pragma unroll
         for (int u = 0; u < STG_U; ++u) {
           const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
-          if (j < m2 && (bv[u] >> (sh + wd)) == hi) atomicAdd(&S.hist[(unsigned int)(bv[u] >> sh) & mask], 1u);
+          stg_hist_add(S.hist, (unsigned int)(bv[u] >> sh) & mask, j < m2 && (bv[u] >> top) == hi);
         }
       }
     } else {
-      const int top = sh + wd;  // 64 at the first key level
       for (unsigned int j0 = (unsigned int)part * NTB * STG_U; j0 < tcount; j0 += stride * STG_U) {
         uint64_t kv[STG_U];
 #pragma unroll
@@ -801,8 +881,8 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
 #pragma unroll
         for (int u = 0; u < STG_U; ++u) {
           const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
-          if (j < tcount && (top >= 64 || (kv[u] >> top) == (kp >> top)))
-            atomicAdd(&S.hist[(unsigned int)(kv[u] >> sh) & mask], 1u);
+          stg_hist_add(S.hist, (unsigned int)(kv[u] >> sh) & mask,
+                       j < tcount && (top >= 64 || (kv[u] >> top) == (kp >> top)));
         }
       }
     }
@@ -810,16 +890,25 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
     for (int b = tid; b < HIST_BINS; b += NTB)
       if (S.hist[b]) atomicAdd(&hist[b], S.hist[b]);
   } else {
-    // tied candidates -> logical keys; one list reservation per round of
-    // NTB * STG_U candidates (a block scan of the per-thread counts)
+    // tied candidates -> logical keys (in any order); one list reservation
+    // per round of NTB * STG_U candidates (a block scan of the per-thread
+    // counts); the key map's byte tables staged in LDS
+    const bool tab = P.km.nbits > 0 && P.ktab;
+    if (tab)
+      for (int e = tid; e < KTAB_WORDS; e += NTB) s_ktab[e] = P.ktab[e];
+    __syncthreads();
     for (unsigned int j0 = (unsigned int)part * NTB * STG_U; j0 < m2; j0 += stride * STG_U) {
-      bool take[STG_U];
+      unsigned long long bv[STG_U];
+#pragma unroll
+      for (int u = 0; u < STG_U; ++u) {
+        const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+        bv[u] = P.c2p[j < m2 ? j : 0];
+      }
       int c = 0;
 #pragma unroll
       for (int u = 0; u < STG_U; ++u) {
-        const unsigned int j = j0 + (unsigned int)(tid * STG_U + u);
-        take[u] = j < m2 && P.c2p[j] == pstar;
-        c += take[u] ? 1 : 0;
+        const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+        c += (j < m2 && bv[u] == pstar) ? 1 : 0;
       }
       int tot;
       int off = block_excl_scan<NTB>(c, tot, S.scan);
@@ -828,9 +917,10 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
       const unsigned int base = s_base;
 #pragma unroll
       for (int u = 0; u < STG_U; ++u) {
-        if (take[u]) {
-          const unsigned int j = j0 + (unsigned int)(tid * STG_U + u);
-          P.tk[base + (unsigned int)off] = logical_key(P.x[P.c2i[j]], P.km);
+        const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+        if (j < m2 && bv[u] == pstar) {
+          const uint64_t w = P.x[P.c2i[j]];
+          P.tk[base + (unsigned int)off] = tab ? logical_key_tab(w, s_ktab) : logical_key(w, P.km);
           ++off;
         }
       }
@@ -872,7 +962,6 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
     stg_release_from(sel, stage);
     return;
   }
-  const int sh = kWideShift[lvh];
   const unsigned long long nneed = need - S.ab;
   sel->need = nneed;
   if (stage < STG_COLLECT) {
@@ -892,7 +981,12 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
   } else {
     const unsigned long long nkp = kp | ((unsigned long long)b << sh);
     sel->kp = nkp;
-    if (stage == STG_STAGES - 1) sel->keystar = nkp;
+    if (sh == 0) {
+      // the last key level: the need-th smallest tied key; later stages end
+      sel->keystar = nkp;
+      stg_release_from(sel, stage);
+      return;
+    }
   }
   __threadfence();
   __hip_atomic_store(&sel->stg_flag[stage], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -993,7 +1087,7 @@ __global__ __launch_bounds__(NT) void k_wide_collect(const SelProblem* probs, Se
     const int leader = __ffsll((long long)bal) - 1;
     if (lane == leader) wb = atomicAdd(&sel->tcount, (unsigned int)__popcll(bal));
     wb = __shfl(wb, leader, 64);
-    if (take) P.tk[wb + __popcll(bal & ((1ull << lane) - 1ull))] = logical_key(P.x[P.c2i[j]], P.km);
+    if (take) P.tk[wb + __popcll(bal & ((1ull << lane) - 1ull))] = lkey(P, P.x[P.c2i[j]]);
   }
 }
 
@@ -1018,8 +1112,8 @@ __global__ __launch_bounds__(NT) void k_wide_key_hist(const SelProblem* probs, S
 }
 
 __device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t w, unsigned long long pstar,
-                                          unsigned long long keystar, const KeyMap& km) {
-  return bits > pstar || (bits == pstar && logical_key(w, km) <= keystar);
+                                          unsigned long long keystar, const SelProblem& P) {
+  return bits > pstar || (bits == pstar && lkey(P, w) <= keystar);
 }
 
 // Per tile: sum and count of the kept candidates (lane-strided, then a
@@ -1050,7 +1144,7 @@ __global__ __launch_bounds__(NT) void k_keptsum(const SelProblem* probs, int npr
       if (j >= cnt) continue;
       const unsigned long long bits = bv[u];
       bool keep = bits > pstar;
-      if (bits == pstar) keep = logical_key(P.x[T.c1i[base + j]], P.km) <= keystar;
+      if (bits == pstar) keep = lkey(P, P.x[T.c1i[base + j]]) <= keystar;
       if (keep) {
         s += __longlong_as_double((long long)bits);
         ++c;
@@ -1225,7 +1319,7 @@ __global__ __launch_bounds__(NT) void k_compact(const SelProblem* probs, int npr
     if (j < cnt) {
       kk[it] = P.x[base + j];
       vv[it] = P.a[base + j];
-      keep[it] = kept_pred(pbits_of(vv[it]), kk[it], pstar, keystar, P.km);
+      keep[it] = kept_pred(pbits_of(vv[it]), kk[it], pstar, keystar, P);
     }
   }
   int64_t rb = T.koff[gt] + (P.chunked ? T.ch_off[(gt - P.tile0) / SEL_CHUNK] : 0);

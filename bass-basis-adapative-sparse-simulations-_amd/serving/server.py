@@ -145,6 +145,23 @@ def _request_cost(req):
     return float(1 << req.log2k) * req.n * req.depth
 
 
+class RejectedRequest(ValueError):
+    """A request that no amount of waiting would admit (its state does not fit
+    a slab, or its footprint exceeds the HBM budget)."""
+
+
+def check_admissible(req, budget, slab_entries=None):
+    """Raise RejectedRequest unless `req` fits an empty server: 1..63
+    qubits, footprint within the budget and (engines) K within a slab."""
+    if not 1 <= req.n <= 63:
+        raise RejectedRequest(f"request {req.rid}: {req.n} qubits (1 to 63 supported)")
+    fp = footprint_bytes(req)
+    if fp > budget:
+        raise RejectedRequest(f"request {req.rid}: footprint {fp:.3e} B exceeds the HBM budget {budget:.3e} B")
+    if slab_entries is not None and (1 << req.log2k) > slab_entries:
+        raise RejectedRequest(f"request {req.rid}: K = 2^{req.log2k} exceeds the slab ({slab_entries} entries)")
+
+
 class _EngineLoop:
     """Admission + ticking of one BatchEngine (FIFO, HBM budget)."""
 
@@ -153,9 +170,13 @@ class _EngineLoop:
         self.budget = float(budget)
         self.pending = collections.deque()
         self.used = 0.0
+        self.peak = 0.0
         self.inflight = {}  # rid -> (request, footprint, start, gates)
 
     def add(self, req):
+        """Queue a request; RejectedRequest if it could never be admitted
+        (FIFO admission would otherwise wait on it forever)."""
+        check_admissible(req, self.budget, self.eng.slab_entries)
         circ, kw = build_circuit(req)
         self.pending.append((req, circ, kw))
 
@@ -166,9 +187,12 @@ class _EngineLoop:
             if self.eng.active and self.used + fp > self.budget:
                 return
             if not self.eng.admit(r.n, 1 << r.log2k, circ, kw.get("optimize_at"), tag=r.rid):
+                if not self.eng.active:
+                    raise RuntimeError(f"request {r.rid} does not fit the empty engine")
                 return
             self.pending.popleft()
             self.used += fp
+            self.peak = max(self.peak, self.used)
             self.inflight[r.rid] = (r, fp, time.monotonic(), len(circ))
 
     def tick(self):
@@ -305,6 +329,8 @@ class BatchServer:
         import torch
 
         reqs = sorted(requests, key=lambda r: r.arrival)
+        for r in reqs:
+            check_admissible(r, self._cfg["budget"], self._cfg["slab_entries"])
         by_id = {r.rid: r for r in reqs}
         recs = {}
         torch.cuda.synchronize(self.device)
@@ -396,6 +422,8 @@ class BassServer:
         record per request: arrival, start, end (monotonic seconds), latency,
         worker and result."""
         reqs = sorted(requests, key=lambda r: r.arrival)
+        for r in reqs:
+            check_admissible(r, self.hbm_budget)
         t_base = time.monotonic()
         pending = list(reqs)
         idle = list(range(self.workers))

@@ -272,16 +272,20 @@ class BASS:
     run it at layer boundaries of circuits whose layers differ in length);
     ``trace`` -- optional list receiving the decision log ("rot" / "trunc"
     records); ``accept_rtol`` -- a basis rotation is accepted only when it
-    lowers PR by more than this fraction (the reference accepts any decrease,
-    :785-795; rotations that merely relabel the state leave PR unchanged up to
-    rounding, and the reference's choice there depends on summation order, so
-    they are rejected deterministically; 0 restores the literal rule).
+    lowers PR by more than this fraction, a 2-qubit pair kept only when PR
+    does not grow beyond pr_before * (1 - accept_rtol).  The default 0 is the
+    reference's literal rule (any decrease, :785-795; revert iff
+    pr_after > pr_before, :714-722).  Rotations that merely relabel the state
+    leave PR unchanged up to rounding, where the choice depends on summation
+    order; a small positive value (1e-10) rejects them deterministically,
+    which is what comparisons against another implementation decision by
+    decision use.
     """
 
     def __init__(self, num_qubits, k, optimize_every=5, buffer_factor=2, truncate_every=1,
                  max_nnz_factor=8, use_2qubit_rotations=False, verbose=False, use_fast_eigh=True,
                  use_diag_gate=True, use_kron_cache=True, pr_opt_threshold=0.90, device=None,
-                 optimize_at=None, trace=None, accept_rtol=1e-10, device_counts=True):
+                 optimize_at=None, trace=None, accept_rtol=0.0, device_counts=True):
         self.N = num_qubits
         self.k = k
         self.optimize_every = optimize_every

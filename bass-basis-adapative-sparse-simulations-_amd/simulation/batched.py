@@ -108,7 +108,7 @@ class BatchEngine:
     """
 
     def __init__(self, max_circuits, slab_entries, device=None, optimize_every=5, pr_opt_threshold=0.90,
-                 use_diag_gate=True, accept_rtol=1e-10, trace=False):
+                 use_diag_gate=True, accept_rtol=0.0, trace=False):
         self._ctx = context(device)
         C = int(max_circuits)
         self.C = C

@@ -46,7 +46,7 @@ def test_bass_end_to_end(gpu_ctx, idx, K):
     if ends is not None:
         kw = dict(optimize_at=ends)
     td, to = [], []
-    sim = B.BASS(N, K, device="cuda:0", trace=td, **kw)
+    sim = B.BASS(N, K, device="cuda:0", trace=td, accept_rtol=1e-10, **kw)
     st = sim.simulate(circ)
     ref = OD.BASS(N, K, selector=OD.key_tiebreak_selector, trace=to, accept_rtol=sim.accept_rtol, **kw)
     rs = ref.simulate(circ)
@@ -108,7 +108,7 @@ def test_bass_2qubit_pass(gpu_ctx, idx):
     if ends is not None:
         kw = dict(optimize_at=ends, use_2qubit_rotations=True)
     td, to = [], []
-    sim = B.BASS(N, K, device="cuda:0", trace=td, **kw)
+    sim = B.BASS(N, K, device="cuda:0", trace=td, accept_rtol=1e-10, **kw)
     st = sim.simulate(circ)
     ref = OD.BASS(N, K, selector=OD.key_tiebreak_selector, trace=to, accept_rtol=sim.accept_rtol, **kw)
     rs = ref.simulate(circ)
@@ -156,7 +156,7 @@ def test_tfim_config0_lockstep(gpu_ctx):
     circ = rc.generate_tfim_circuit(N, 40, J=1.0, h=1.0, dt=0.05, rng=np.random.default_rng(0))
     assert len(circ) == 1560
     td = []
-    sim = B.BASS(N, K, device="cuda:0", trace=td, optimize_every=39)
+    sim = B.BASS(N, K, device="cuda:0", trace=td, optimize_every=39, accept_rtol=1e-10)
     sim.begin(len(circ))
     ref = OD.BASS(N, K, selector=OD.key_tiebreak_selector, optimize_every=39)
     internal_k = min(max(2 * K, 8 * K), 2 ** N)

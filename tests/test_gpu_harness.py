@@ -77,7 +77,8 @@ def test_sweep_full_brickwork_vs_oracle(gpu_ctx, tmp_path):
         return R.make_brickwork(n, depth, rng)
 
     def compute():
-        res = R.run_sweep_full(gen, N, ks, T, 7, bass_kwargs=KW, verbose=False, n_boot=200)
+        res = R.run_sweep_full(gen, N, ks, T, 7, bass_kwargs={**KW, "accept_rtol": 1e-10}, verbose=False,
+                                n_boot=200)
         return None, R.sweep_arrays(res)
 
     meta = {"N": N, "depth": depth, "k_values": ks, "n_trials": T, "seed_base": 7, "bass_kw": KW}

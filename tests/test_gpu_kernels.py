@@ -439,3 +439,33 @@ def test_sparse_state_2q_paths(gpu_ctx, dedup, n, q1, q2):
         assert_same_entries(gx, ga, o.x[: o.nnz], o.alpha[: o.nnz],
                             exact=(k >= 4 * n and exact()), rtol=1e-13)
         assert abs(st.gamma - o.gamma) <= 1e-13
+
+
+@pytest.mark.parametrize("dedup", ["sort", "table"])
+def test_sparse_state_random_k_sequence(gpu_ctx, dedup):
+    """SparseState.apply_gate(..., "random-k", seed) over a sequence of 1- and
+    2-qubit gates against the oracle's SparseState: the reference draws the
+    kept entries by array index from its sort-and-merge output (ascending
+    keys, src/core/sparse_state.py:268-301, :303-377, :402-415), so the
+    device's state must hold the same keys in the same order after every
+    gate, whichever way it deduplicates; amplitudes and gamma within 1e-12."""
+    print(f"[case] sparse state random-k sequence dedup={dedup}", flush=True)
+    rng = np.random.default_rng(77)
+    N, k = 12, 150
+    st = ss().SparseState(N, k, device="cuda:0", dedup=dedup)
+    o = OD.SparseState(N, k)
+    for i in range(24):
+        if i % 4 == 3:
+            q1, q2 = (int(v) for v in rng.choice(N, size=2, replace=False))
+            g = OD._G((q1, q2), random_unitary(4, rng))
+        else:
+            g = OD._G((int(rng.integers(N)),), random_unitary(2, rng))
+        seed = 1000 + 7 * i
+        st.apply_gate(g, "random-k", seed)
+        o.apply_gate(g, "random-k", seed)
+        assert st.nnz == o.nnz
+        gk = st.keys()
+        assert np.array_equal(gk, o.x[: o.nnz]), f"gate {i}: keys or their order differ"
+        np.testing.assert_allclose(st.alpha[: st.nnz], o.alpha[: o.nnz], rtol=1e-12, atol=1e-15)
+        assert abs(st.gamma - o.gamma) <= 1e-12 * o.gamma
+    assert o.nnz == k

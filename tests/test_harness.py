@@ -131,3 +131,33 @@ def test_run_or_load_cache(tmp_path):
     _, a3 = R.run_or_load("arr", {"x": 1}, lambda: (None, {"a": np.ones(3)}), d)
     _, a4 = R.run_or_load("arr", {"x": 1}, lambda: (_ for _ in ()).throw(AssertionError("recomputed")), d)
     assert np.array_equal(a3["a"], a4["a"]) and not os.path.exists(R.cache_paths("arr", d)[0])
+
+
+def test_server_rejects_requests_that_never_fit():
+    """A request whose K exceeds the engine's slab, whose footprint exceeds the
+    HBM budget, or whose qubit count is out of range is rejected when it is
+    queued (FIFO admission would otherwise wait on it forever); requests that
+    fit are queued."""
+    S = pkg_module("serving.server")
+
+    class _Engine:  # what _EngineLoop reads of a BatchEngine, without a device
+        slab_entries = 1 << 16
+        active = 0
+
+        def admit(self, *a, **k):
+            return True
+
+    loop = S._EngineLoop(_Engine(), budget=200e9)
+    ok = S.CircuitRequest(rid=0, kind="tfim", n=12, depth=2, log2k=10)
+    loop.add(ok)
+    assert len(loop.pending) == 1
+    with pytest.raises(S.RejectedRequest):
+        loop.add(S.CircuitRequest(rid=1, kind="tfim", n=12, depth=2, log2k=17))  # K > slab
+    with pytest.raises(S.RejectedRequest):
+        loop.add(S.CircuitRequest(rid=2, kind="tfim", n=70, depth=2, log2k=10))  # qubits
+    small = S._EngineLoop(_Engine(), budget=S.footprint_bytes(ok) - 1)
+    with pytest.raises(S.RejectedRequest):
+        small.add(ok)  # footprint > budget
+    assert len(loop.pending) == 1
+    loop.admit()
+    assert not loop.pending and loop.used == S.footprint_bytes(ok) and loop.peak == loop.used
