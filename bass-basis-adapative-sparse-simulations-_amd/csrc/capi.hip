@@ -269,6 +269,44 @@ static int hip_check(const char* where) {
   return BASS_OK;
 }
 
+// ---------------------------------------------------- no-allocation call ----
+// bass_truncate_noalloc works in `scratch`, a static device array of this
+// library (one copy per GPU, held until the process ends) laid out like a
+// workspace of NOALLOC_MAX_ENTRIES entries, instead of a caller workspace.
+// Host code uses the array only through the address hipGetSymbolAddress
+// returns on each device (the array's name on the host is a stand-in, not a
+// device address).
+constexpr int64_t NOALLOC_MAX_ENTRIES = (int64_t)1 << 20;
+constexpr size_t NOALLOC_SCRATCH_BYTES = bass::WS_FIXED + bass::WS_PER * (size_t)NOALLOC_MAX_ENTRIES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace bass {
+__global__ void k_copy_stats(const OpStats* src, OpStats* dst) { *dst = *src; }
+}  // namespace bass
+
+// The device address of `scratch` on the current device (looked up once per
+// device, then cached).
+static int noalloc_scratch(void** out) {
+  static std::mutex mu;
+  static void* addr[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+    return fail(BASS_E_HIP, "bass_truncate_noalloc: no current device");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(BASS_E_HIP, "bass_truncate_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+    if (bass::ws_capacity(NOALLOC_SCRATCH_BYTES) < NOALLOC_MAX_ENTRIES)
+      return fail(BASS_E_WORKSPACE, "bass_truncate_noalloc: scratch layout does not hold %lld entries",
+                  (long long)NOALLOC_MAX_ENTRIES);
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return BASS_OK;
+}
+
 #define WS_REQUIRE(need)                                                                       \
   int64_t S = ws_capacity(ws_bytes);                                                           \
   if (!ws || S < (need))                                                                       \
@@ -506,9 +544,20 @@ int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t
   return hip_check("bass_densify");
 }
 
+static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
+                         int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags, const int32_t* key_map,
+                         int key_bits, void* ws, size_t ws_bytes, void* stream, bool timed = true);
+
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
                   int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags, const int32_t* key_map,
                   int key_bits, void* ws, size_t ws_bytes, void* stream) {
+  return truncate_impl(x, a, n, k, xo, ao, cap_out, renorm, hist_tiles, in_flags, key_map, key_bits, ws, ws_bytes,
+                       stream);
+}
+
+static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
+                         int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags, const int32_t* key_map,
+                         int key_bits, void* ws, size_t ws_bytes, void* stream, bool timed) {
   const bool segmented = (in_flags & 1) != 0;
   const bool wide = (in_flags & 2) != 0;
   const bool counted = (in_flags & 4) != 0;
@@ -575,7 +624,7 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
   P.km = km;
   P.ktab = km.nbits ? at<uint64_t>(ws, L.ktab) : nullptr;
   const SelTiles T = tiles_view(ws, L, (unsigned long long*)big, (unsigned int*)(big + p8));
-  const int tk = ktimer_begin("select", s);
+  const int tk = timed ? ktimer_begin("select", s) : -1;
   launch_select(P, T, hist_tiles > 0, wide, s);
   ktimer_end(tk, s);
   if (counted)
@@ -583,6 +632,27 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
                        at<unsigned long long>(ws, L.oplog), (OpStats*)(at<char>(ws, L.oplog) + 8),
                        at<int64_t>(ws, L.count_slot));
   return hip_check("bass_truncate");
+}
+
+int bass_truncate_noalloc(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
+                          int64_t cap_out, int renorm, const int32_t* key_map, int key_bits, bass_stats* stats_out,
+                          void* stream) {
+  if (n > NOALLOC_MAX_ENTRIES)
+    return fail(BASS_E_ARG, "bass_truncate_noalloc: n = %lld exceeds the maximum of %lld entries", (long long)n,
+                (long long)NOALLOC_MAX_ENTRIES);
+  if (!stats_out) return fail(BASS_E_ARG, "bass_truncate_noalloc: stats_out is null");
+  void* ws = nullptr;
+  const int rc0 = noalloc_scratch(&ws);
+  if (rc0) return rc0;
+  hipStream_t s = (hipStream_t)stream;
+  const Layout L = layout(ws_capacity(NOALLOC_SCRATCH_BYTES));
+  // the array keeps the previous call's contents: a fresh stats block
+  fill(at<void>(ws, L.st), sizeof(OpStats), 0ull, s);
+  const int rc = truncate_impl(x, a, n, k, xo, ao, cap_out, renorm, 0, 0, key_map, key_bits, ws,
+                               NOALLOC_SCRATCH_BYTES, stream, false);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_copy_stats, dim3(1), dim3(1), 0, s, at<const OpStats>(ws, L.st), (OpStats*)stats_out);
+  return hip_check("bass_truncate_noalloc");
 }
 
 int bass_diag2q(const uint64_t* x, double* a, int64_t n, int q1, int q2, const double* diag, void* stream) {

@@ -187,6 +187,22 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
                   double* ao, int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags,
                   const int32_t* key_map, int key_bits, void* ws, size_t ws_bytes, void* stream);
 
+/* bass_truncate of a dense input (hist_tiles = 0, in_flags = 0) that
+ * allocates nothing and takes no workspace: its scratch (select histograms,
+ * candidate lists, per-tile sums, stats) is a static device array of the
+ * library, one per GPU, sized for n <= 2^20 entries (72 MiB, held until the
+ * process ends).  n above that fails with BASS_E_ARG.  stats_out: a device
+ * pointer (caller-owned) that receives the call's bass_stats, written in
+ * stream order; nothing is read back to the host.  Calls on one GPU share
+ * the array, so calls that overlap on one GPU (different streams or
+ * threads) race; each call clears what it needs zeroed.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+int bass_truncate_noalloc(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo,
+                          double* ao, int64_t cap_out, int renorm, const int32_t* key_map, int key_bits,
+                          bass_stats* stats_out, void* stream);
+
 /* In place: a[i] *= diag[(b_q1 << 1) | b_q2]; diag = 4 complex (8 doubles).
  * Replaces apply_2qubit_diagonal_kernel (src/core/sparse_state.py:178-190). */
 int bass_diag2q(const uint64_t* x, double* a, int64_t n, int q1, int q2, const double* diag,
