@@ -173,13 +173,13 @@ int bass_counted_read(void* ws, size_t ws_bytes, bass_stats* out, int64_t max_en
  * (the count comes from the device) and the input may hold <= k entries:
  * then all are kept, in order and not renormalised, and stats.flags bit 0 is
  * clear -- the gate / truncate pair needs no host round trip in between.
- * in_flags bit 1 (wide): resolve the threshold with multi-block passes.  A
- * call without it whose threshold bucket holds more than 65536 entries
- * (massive |a|^2 ties) keeps nothing and sets stats.flags bit 1; repeat it
- * with bit 1 set (the ctypes layer does).
- * in_flags bit 2 (counted): resolve any threshold bucket in the call itself
- * (one block; never asks for a repeat), then append the stats block to the
- * workspace's op log and set its state-count slot to the output count, for
+ * Any threshold bucket is resolved inside the call: up to 65536 candidates
+ * by one block, more (massive |a|^2 ties) by multi-block stages of the same
+ * launch; the call never asks for a repeat.
+ * in_flags bit 1 (wide): resolve the threshold with the multi-launch passes
+ * instead (kept for callers that request them; same result).
+ * in_flags bit 2 (counted): then append the stats block to the workspace's op
+ * log and set its state-count slot to the output count, for
  * bass_lift_counted and bass_counted_read.
  * key_map (key_bits entries, or NULL): stored bit p of x holds qubit
  * key_map[p]; ties are then ordered by the key mapped back. */
