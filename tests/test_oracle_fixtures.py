@@ -293,3 +293,41 @@ def test_gamma_diagnostics_2q_pass_rows(fi):
             assert ok or near_tie_rotation(trace), (t, K)
             matched += ok
     assert matched >= 6, matched
+
+
+SCHMIDT_FAMILIES = ("haar_L3", "haar_L5", "brickwork_L3", "brickwork_L5")
+
+
+def schmidt_circuit(ci, t):
+    """SchmidtWeightedTruncation.ipynb c1 / c4: N=16, family ci (Haar random
+    pairing or brickwork of Haar 2-qubit gates, depth 3 or 5), trial t, seed
+    20260520 * 10^6 + ci * 10^4 + t."""
+    R = pkg_module("experimentation.runner")
+    rng = np.random.default_rng(SEED_BASE * 10 ** 6 + ci * 10_000 + t)
+    depth = 3 if ci % 2 == 0 else 5
+    return R.make_haar(16, depth, rng) if ci < 2 else R.make_brickwork(16, depth, rng)
+
+
+@pytest.mark.parametrize("ci", [0, 1, 2, 3])
+def test_schmidt_benchmark_topk_rows(ci):
+    """schmidt_benchmark.npz, method "top-k" (SchmidtWeightedTruncation.ipynb
+    c2 TruncationSimulator: every gate through the reference's emit / radix
+    sort / merge kernels, truncation to the K=8192 largest |a|^2 with
+    renormalisation and gamma *= sqrt(kept) -- the SparseState top-k path,
+    src/core/sparse_state.py:268-400): per trial the stored fidelity against
+    the exact state and gamma^2, to 1e-10.  Haar gates: no |a|^2 ties, so
+    every trial is a pin.  The first 25 trials of each family here; the device
+    test replays all 100."""
+    d = _load("schmidt_benchmark.npz")
+    meta = json.load(open(os.path.join(REF_RESULTS, "schmidt_benchmark_meta.json")))
+    assert meta["circuits"][ci] == SCHMIDT_FAMILIES[ci] and meta["methods"][0] == "top-k"
+    N, K = meta["N"], meta["K"]
+    for t in range(25):
+        circ = schmidt_circuit(ci, t)
+        sv = OE.exact_statevector(N, circ)
+        s = OD.SparseState(N, K)
+        for g in circ:
+            s.apply_gate(OD._G(g.qubits, g.matrix), "top-k")
+        f = OE.compute_fidelity(s.x[: s.nnz], s.alpha[: s.nnz], sv)
+        assert abs(f - d["fidelity"][ci, 0, t]) < 1e-10, t
+        assert abs(s.gamma ** 2 - d["gamma_sq"][ci, 0, t]) < 1e-10, t
