@@ -44,12 +44,24 @@ class CircuitRequest:
     arrival: float = 0.0  # seconds after the stream starts
 
 
+FOOTPRINT_PER_ENTRY = 24 * 8 + 64 * 4  # bytes per state entry (K) of footprint_bytes
+FOOTPRINT_FIXED = 8 << 20
+
+
 def footprint_bytes(req):
     """HBM a request holds while it runs: state buffer (K), gate output (<= 4K
     slots, 2K for 1q), rotation buffer (2K), 24 B each, plus the workspace
     (64 B per entry of the largest operation, 4K, and 8 MB fixed)."""
     K = 1 << req.log2k
-    return 24 * (K + 4 * K + 2 * K + K) + 64 * 4 * K + (8 << 20)
+    return FOOTPRINT_PER_ENTRY * K + FOOTPRINT_FIXED
+
+
+def slab_entries_for(budget):
+    """Slab size (entries) of an engine with this HBM budget: large enough
+    that the budget, not the slab, bounds admission (a request of K entries
+    counts FOOTPRINT_PER_ENTRY * K bytes against the budget but takes K slab
+    entries)."""
+    return max(1 << 16, int(budget // FOOTPRINT_PER_ENTRY))
 
 
 def make_batched_requests(count=512, seed=20260520):
@@ -242,7 +254,7 @@ def _engine_worker(wid, device, cfg, inq, outq):
                     outq.put(("done", wid, rec))
             elif closing and not loop.pending:
                 break
-        outq.put(("counters", wid, dict(loop.eng.counters)))
+        outq.put(("counters", wid, {**loop.eng.counters, "peak_footprint_bytes": loop.peak}))
     except Exception:  # reported to the dispatcher, which raises
         outq.put(("error", wid, traceback.format_exc()))
 
@@ -255,17 +267,24 @@ class BatchServer:
     engine's host work overlaps the others' kernels) and a dispatcher that
     sends each arrival to the worker with the least outstanding work.
     max_circuits / slab_entries (24 B entries, two slabs) are per engine;
-    hbm_budget is shared evenly.  Engine keywords (optimize_every, ...) pass
-    through.  Use as a context manager."""
+    hbm_budget is shared evenly; slab_entries=None sizes the slabs from it
+    (slab_entries_for), so admission is bound by the budget and the slots.
+    Engine keywords (optimize_every, ...) pass through.  After a run,
+    peak_footprint_bytes is the sum over engines of their peak admitted
+    footprint.  Use as a context manager."""
 
-    def __init__(self, workers=1, max_circuits=512, slab_entries=1 << 27, hbm_budget=200e9, device="cuda:0",
+    def __init__(self, workers=1, max_circuits=512, slab_entries=None, hbm_budget=200e9, device="cuda:0",
                  **kw):
         self.workers = int(workers)
         self.hbm_budget = float(hbm_budget)
         self.device = device
         self.counters = {}
+        self.peak_footprint_bytes = 0.0
+        budget = self.hbm_budget / max(1, self.workers)
+        if slab_entries is None:
+            slab_entries = slab_entries_for(budget)
         self._cfg = {"max_circuits": int(max_circuits), "slab_entries": int(slab_entries),
-                     "budget": self.hbm_budget / max(1, self.workers), "kw": kw}
+                     "budget": budget, "kw": kw}
         self._loop = None
         self._procs = []
         self._inqs = []
@@ -297,6 +316,7 @@ class BatchServer:
     def __exit__(self, *exc):
         if self._loop is not None:
             self.counters = dict(self._loop.eng.counters)
+            self.peak_footprint_bytes = self._loop.peak
             return False
         for q in self._inqs:
             q.put(None)
@@ -308,6 +328,7 @@ class BatchServer:
             except queue.Empty:
                 break
             if kind == "counters":
+                self.peak_footprint_bytes += msg.pop("peak_footprint_bytes", 0.0)
                 tot.update(msg)
                 left -= 1
         self.counters = dict(tot)

@@ -427,6 +427,10 @@ class BatchEngine:
         order = np.argsort(trivial, axis=1, kind="stable")
         self.ochain[cs] = order
         self.optr[cs] = 0
+        # PR at the pass start from the state's moments, recomputed after an
+        # accepted rotation (as the single-circuit driver's _pr does: its sum
+        # of |a|^2 is then unknown), so the accept rule compares the same value
+        self._ensure_sums(cs)
         self.opr[cs] = self._pr(cs)
         self.oimproved[cs] = False
         self.ostage[cs] = 1

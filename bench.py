@@ -330,19 +330,20 @@ def bench_sort(args):
 
 
 def bench_served(args):
-    """Config 4: Poisson stream (--rate) of 50/50 TFIM and brickwork circuits,
-    HBM-budget admission (200 GB), p50/p99 latency.  --server engine: one
-    continuous batch (BatchServer); pool: worker processes running one circuit
-    each (BassServer).  Baseline: a --single-sample subset of the same requests
-    one at a time through the plain driver, as circuits/s."""
+    """Config 4: Poisson stream (--rate, or a sweep over --rates) of 50/50 TFIM
+    and brickwork circuits, HBM-budget admission (200 GB), p50/p99 latency.
+    --server engine: continuous batching (BatchServer, slabs sized from the
+    budget); pool: worker processes running one circuit each (BassServer).
+    Baseline: a --single-sample subset of the same requests one at a time
+    through the plain driver, as circuits/s."""
     import torch
 
     S = importlib.import_module(PKG + ".serving.server")
     importlib.import_module(PKG + "._lib").load()
-    reqs = S.make_served_requests(args.circuits, args.rate)
+    rates = [float(r) for r in args.rates.split(",")] if args.rates else [args.rate]
     single_rate = None
     if args.single_sample:
-        sample = _spread(reqs, args.single_sample)
+        sample = _spread(S.make_served_requests(args.circuits, rates[0]), args.single_sample)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for r in sample:
@@ -350,28 +351,43 @@ def bench_served(args):
         t_single = time.perf_counter() - t0
         single_rate = len(sample) / t_single
         log(f"one at a time: {len(sample)} circuits in {t_single:.2f} s = {single_rate:.2f} circuits/s")
-    prog = lambda d, t: log(f"served {d}/{t}") if d % 32 == 0 else None  # noqa: E731
-    if args.server == "engine":
-        with S.BatchServer(workers=args.workers, max_circuits=args.max_circuits, hbm_budget=200e9) as srv:
-            recs = srv.run(reqs, progress=prog)
-        summ = S.summarize(recs, amp_updates=srv.counters.get("amp_updates", 0))
-        cfg = {"server": "continuous batch", "engines": args.workers, "max_circuits_per_engine": args.max_circuits}
-    else:
-        with S.BassServer(workers=args.workers, hbm_budget=200e9) as srv:
-            recs = srv.run(reqs, progress=prog)
-        summ = S.summarize(recs)
-        cfg = {"server": "worker pool", "workers": args.workers}
+    sweep = []
+    for rate in rates:
+        reqs = S.make_served_requests(args.circuits, rate)
+        prog = lambda d, t: log(f"served {d}/{t}") if d % 64 == 0 else None  # noqa: E731
+        peak = None
+        if args.server == "engine":
+            with S.BatchServer(workers=args.workers, max_circuits=args.max_circuits, hbm_budget=200e9) as srv:
+                recs = srv.run(reqs, progress=prog)
+            summ = S.summarize(recs, amp_updates=srv.counters.get("amp_updates", 0))
+            peak = srv.peak_footprint_bytes
+            cfg = {"server": "continuous batch", "engines": args.workers,
+                   "max_circuits_per_engine": args.max_circuits, "slab_entries_per_engine": srv._cfg["slab_entries"]}
+        else:
+            with S.BassServer(workers=args.workers, hbm_budget=200e9) as srv:
+                recs = srv.run(reqs, progress=prog)
+            summ = S.summarize(recs)
+            cfg = {"server": "worker pool", "workers": args.workers}
+        row = {"rate_req_s": rate, "circuits_per_s": summ["circuits_per_s"], "p50_latency_s": summ["p50_latency_s"],
+               "p99_latency_s": summ["p99_latency_s"], "mean_service_s": summ["mean_service_s"],
+               "wall_s": summ["wall_s"], "amp_updates": summ["amp_updates"], "peak_footprint_bytes": peak}
+        log(f"rate {rate:g} req/s: {summ['circuits_per_s']:.2f} circuits/s, p50 {summ['p50_latency_s']:.2f} s, "
+            f"p99 {summ['p99_latency_s']:.2f} s, peak footprint {peak}")
+        sweep.append(row)
+    head = sweep[0]
     out = {
-        "metric": "served circuits/s", "value": summ["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1,
-        "steps": 1, "warmup": 0, "ms_per_step": summ["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "metric": "served circuits/s", "value": head["circuits_per_s"], "unit": "circuits/s", "n_gpus": 1,
+        "steps": 1, "warmup": 0, "ms_per_step": head["wall_s"] * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "complex128", "data": "synthetic",
-        "config": {"workload": f"served Poisson {args.rate:g} req/s, {args.circuits} requests, 50/50 TFIM "
+        "config": {"workload": f"served Poisson {rates[0]:g} req/s, {args.circuits} requests, 50/50 TFIM "
                                f"(K=2^U{{12..20}}) / brickwork n~U[24,44] depth~U[10,30]",
                    "hbm_budget_bytes": 200e9, **cfg},
-        "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"],
-        "mean_service_s": summ["mean_service_s"], "amp_updates": summ["amp_updates"],
+        "p50_latency_s": head["p50_latency_s"], "p99_latency_s": head["p99_latency_s"],
+        "mean_service_s": head["mean_service_s"], "amp_updates": head["amp_updates"],
+        "peak_footprint_bytes": head["peak_footprint_bytes"],
         "one_at_a_time_circuits_per_s": single_rate,
-        "speedup_vs_one_at_a_time": (summ["circuits_per_s"] / single_rate) if single_rate else None,
+        "speedup_vs_one_at_a_time": (head["circuits_per_s"] / single_rate) if single_rate else None,
+        "sweep": sweep,
     }
     print(json.dumps(out), flush=True)
 
@@ -510,6 +526,7 @@ def main():
     ap.add_argument("--single-sample", type=int, default=24,
                     help="batched / served: circuits timed one at a time for the baseline")
     ap.add_argument("--rate", type=float, default=200.0, help="served: Poisson arrival rate (req/s)")
+    ap.add_argument("--rates", type=str, default=None, help="served: comma-separated arrival rates to sweep")
     ap.add_argument("--server", choices=["engine", "pool"], default="engine", help="served: server kind")
     ap.add_argument("--max-circuits", type=int, default=512, help="served (engine): circuits running at once")
     ap.add_argument("--workers", type=int, default=None,

@@ -125,3 +125,65 @@ def test_batch_server_two_engines(gpu_ctx):
         res = Srv.run_request(q, "cuda:0")
         assert res["nnz"] == r["nnz"] and res["gamma2"] == r["gamma2"]
     assert srv.counters["amp_updates"] > 0
+
+
+@pytest.mark.timeout(900)
+def test_batched_config3_sample(gpu_ctx):
+    """32 circuits of the configured batched stream (make_batched_requests:
+    TFIM with n ~ U[16,48], steps ~ U[10,60], h ~ U[0.2,2], K = 2^U{10..18};
+    every 16th of the 512) run as one batch: each circuit equals the
+    single-circuit driver bit for bit (keys, amplitudes, gamma, frame)."""
+    print("[case] batched config-3 sample: 32 of 512 circuits", flush=True)
+    Srv = pkg_module("serving.server")
+    Bm = pkg_module("simulation.batched")
+    S = pkg_module("simulation.bass_simulator")
+    reqs = Srv.make_batched_requests(512)[::16]
+    specs = []
+    for r in reqs:
+        circ, kw = Srv.build_circuit(r)
+        specs.append((r.n, 1 << r.log2k, circ, kw.get("optimize_at")))
+    print(f"  n {min(s[0] for s in specs)}..{max(s[0] for s in specs)}, K 2^{min(r.log2k for r in reqs)}.."
+          f"2^{max(r.log2k for r in reqs)}, {sum(len(s[2]) for s in specs)} gates", flush=True)
+    res = Bm.BatchedBASS(specs, device="cuda:0").run()
+    for c, (n, k, circ, oat) in enumerate(specs):
+        sim = S.BASS(n, k, device="cuda:0", **({"optimize_at": oat} if oat else {}))
+        st = sim.simulate(circ)
+        keys, amps = st.keys(), st.alpha[: st.nnz]
+        r = res[c]
+        assert r["nnz"] == st.nnz and r["gamma"] == st.gamma, c
+        o1, o2 = np.argsort(r["keys"]), np.argsort(keys)
+        assert np.array_equal(r["keys"][o1], keys[o2]), c
+        assert np.array_equal(r["amps"][o1].view(np.float64), amps[o2].view(np.float64)), c
+        assert all(np.array_equal(r["U"][j], sim.U[j]) for j in range(n)), c
+
+
+@pytest.mark.timeout(900)
+def test_served_sample_alone_and_in_mix(gpu_ctx):
+    """16 requests of the configured served stream (make_served_requests: 50/50
+    TFIM and brickwork-Haar, n ~ U[16,48] / U[24,44], K = 2^U{12..16} here),
+    arriving while others run (their arrival times compressed 100-fold), served
+    by two engine processes: each result equals the same request run alone
+    through the plain driver -- the same entry count, gamma^2 within 1e-12
+    relative (equal bit for bit unless two reductions of different tile
+    shapes round one sum apart)."""
+    print("[case] served sample: 16 requests, alone vs in the mix", flush=True)
+    Srv = pkg_module("serving.server")
+    pool = [r for r in Srv.make_served_requests(512, 200.0) if r.log2k <= 16]
+    reqs = pool[:: max(1, len(pool) // 16)][:16]
+    for r in reqs:
+        r.arrival /= 100.0
+    assert {r.kind for r in reqs} == {"tfim", "brickwork"}
+    with Srv.BatchServer(workers=2, max_circuits=8, slab_entries=1 << 20, device="cuda:0") as srv:
+        recs = srv.run(reqs)
+    assert sorted(r["rid"] for r in recs) == sorted(q.rid for q in reqs)
+    by = {r["rid"]: r for r in recs}
+    exact = 0
+    for q in reqs:
+        alone = Srv.run_request(q, "cuda:0")
+        m = by[q.rid]
+        assert alone["nnz"] == m["nnz"], q.rid
+        assert abs(alone["gamma2"] - m["gamma2"]) <= 1e-12 * alone["gamma2"], (q.rid, alone["gamma2"], m["gamma2"])
+        exact += alone["gamma2"] == m["gamma2"]
+    summ = Srv.summarize(recs)
+    print(f"  {exact} of {len(reqs)} bit for bit; p50 {summ['p50_latency_s']:.2f} s, "
+          f"p99 {summ['p99_latency_s']:.2f} s", flush=True)
