@@ -85,6 +85,7 @@ SIGNATURES = {
     "bass_ht_lookup": (_I, [_P, _P, _I, _P, _I64, _P, _P]),
     "bass_radix_sort": (_I, [_P, _P, _I64, _I, _P, _SZ, _P]),
     "bass_sort_digit_bits": (_I, [_I]),
+    "bass_tune": (_I, [_I, _I, _I]),
     "bass_sort_plan": (_I, [_I, _P]),
     "bass_merge_sorted": (_I, [_P, _P, _I64, _P, _P, _P, _SZ, _P]),
     "bass_dense_scatter": (_I, [_P, _P, _I64, _P, _I, _P, _I, _P]),
@@ -133,6 +134,12 @@ def load():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
+    # the committed per-size-class tuning table, if any (layout only, never results)
+    from . import tuning
+
+    table = tuning.load_table()
+    if table is not None:
+        tuning.apply(L, table)
     _lib = L
     return L
 

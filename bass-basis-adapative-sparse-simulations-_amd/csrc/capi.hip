@@ -269,6 +269,31 @@ static int hip_check(const char* where) {
   return BASS_OK;
 }
 
+// ------------------------------------------------------- size-class tuning ----
+// Per size class of a call's input count n (class 0: n < 2^14, 1: < 2^18,
+// 2: < 2^22, 3: larger), set with bass_tune (typically from the autotuner's
+// table): the hash/group-table capacity factor (capacity = next power of two
+// >= factor * n: factor 2 keeps the load factor <= 1/2, 4 <= 1/4), the items
+// per thread of the emission kernels (the block's tile = 256 x items entries)
+// and the radix digit width.  0 = the built-in choice.  None of them changes a
+// result.
+enum { TUNE_TABLE_FACTOR = 0, TUNE_EMIT_ITEMS_1Q = 1, TUNE_EMIT_ITEMS_2Q = 2, TUNE_SORT_DIGITS = 3, TUNE_KNOBS = 4 };
+constexpr int TUNE_CLASSES = 4;
+static int g_tune[TUNE_KNOBS][TUNE_CLASSES] = {};
+static std::mutex g_tune_mu;
+
+static int tune_class(int64_t n) { return n < (1 << 14) ? 0 : n < (1 << 18) ? 1 : n < (1 << 22) ? 2 : 3; }
+static int tuned(int knob, int64_t n) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return g_tune[knob][tune_class(n)];
+}
+static int table_log2cap(int64_t n) {
+  const int f = tuned(TUNE_TABLE_FACTOR, n);
+  int l = 0;
+  while ((1ll << l) < (f == 4 ? 4 : 2) * n) ++l;
+  return l < 6 ? 6 : l;
+}
+
 // ---------------------------------------------------- no-allocation call ----
 // bass_truncate_noalloc works in `scratch`, a static device array of this
 // library (one copy per GPU, held until the process ends) laid out like a
@@ -378,7 +403,9 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
   if (n < 0 || n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_gate: bad n %lld", (long long)n);
   if (mode == MODE_ROT && !dedup) return fail(BASS_E_ARG, "bass_gate: rotation requires dedup");
   const int M = 1 << nq;
-  const int64_t tiles = n > 0 ? (n + emit_tile_entries(nq) - 1) / emit_tile_entries(nq) : 0;
+  // tuned tile size for dense outputs; a segmented output keeps the default
+  const int items = segmented ? 0 : tuned(nq == 1 ? TUNE_EMIT_ITEMS_1Q : TUNE_EMIT_ITEMS_2Q, n);
+  const int64_t tiles = n > 0 ? (n + emit_tile_entries(nq, items) - 1) / emit_tile_entries(nq, items) : 0;
   const int64_t need_out = segmented ? tiles * SEG_ENTRIES : (int64_t)M * n;
   if (segmented && cap_out < need_out)
     return fail(BASS_E_OVERFLOW, "bass_gate: segmented output needs %lld slots", (long long)need_out);
@@ -399,8 +426,7 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
   int log2cap = 0;
   uint64_t* gslot = nullptr;
   if (dedup && n > 0) {
-    log2cap = log2_ceil(2 * n);
-    if (log2cap < 6) log2cap = 6;
+    log2cap = table_log2cap(n);
     const size_t cap = (size_t)1 << log2cap;
     const size_t bytes = cap * (size_t)group_slot_bytes(nq);
     if (bytes > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_gate: group table does not fit");
@@ -412,7 +438,7 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
   if (n > 0) {
     launch_emit(nq, mode, dedup != 0, x, (const double2*)a, n, q0, nq == 2 ? q1 : 0, gslot, log2cap, gm, xo,
                 (double2*)ao, cap_out, at<unsigned long long>(ws, L.lb_gate), at<unsigned int>(ws, L.tick_gate),
-                at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), &st->nnz, seg, &st->err, s);
+                at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), &st->nnz, seg, &st->err, s, items);
     if (segmented)
       launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), seg_chunk_cnt(ws, L), seg_chunk_off(ws, L), st, s);
     if (hist_tiles) *hist_tiles = tiles;
@@ -694,8 +720,8 @@ int bass_rdm1q(const uint64_t* x, const double* a, int64_t n, int N, double* rho
   if (n <= 0) {
     fill(rho, (size_t)N * 4 * 8, 0ull, s);
   } else {
-    int log2cap = log2_ceil(2 * n);
-    if (log2cap < 6) log2cap = 6;
+    int log2cap = table_log2cap(n);
+    
     const size_t cap = (size_t)1 << log2cap;
     if (cap * 12 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_rdm1q: table does not fit");
     uint64_t* tk = at<uint64_t>(ws, L.big);
@@ -1180,10 +1206,28 @@ int bass_radix_sort(uint64_t* keys, double* vals, int64_t n, int key_bits, void*
   // histograms, tickets and the first pass's look-back words start at zero
   // (contiguous in the layout)
   fill(at<void>(ws, L.sort_hist), (L.sort_status - L.sort_hist) + (size_t)sweep_tiles(n) * 1024 * 4, 0ull, s);
+  // an explicit process-wide width (bass_sort_digit_bits) wins over the table
+  const int db = sort_digit_bits_get();
   launch_radix_sort(keys, (double2*)vals, n, key_bits, tk, tv, tk2, tv2, at<unsigned int>(ws, L.sort_hist),
-                    at<unsigned int>(ws, L.sort_status), at<unsigned int>(ws, L.sort_tick), &st->err, s);
+                    at<unsigned int>(ws, L.sort_status), at<unsigned int>(ws, L.sort_tick), &st->err, s,
+                    db ? 0 : tuned(TUNE_SORT_DIGITS, n));
   ktimer_end(tm, s);
   return hip_check("bass_radix_sort");
+}
+
+int bass_tune(int knob, int size_class, int value) {
+  if (knob < 0 || knob >= TUNE_KNOBS) return fail(BASS_E_ARG, "bass_tune: unknown knob %d", knob);
+  if (size_class < 0 || size_class >= TUNE_CLASSES) return fail(BASS_E_ARG, "bass_tune: bad size class %d", size_class);
+  bool ok = value == 0;
+  if (knob == TUNE_TABLE_FACTOR) ok |= value == 2 || value == 4;
+  if (knob == TUNE_EMIT_ITEMS_1Q) ok |= emit_items_ok(1, value);
+  if (knob == TUNE_EMIT_ITEMS_2Q) ok |= emit_items_ok(2, value);
+  if (knob == TUNE_SORT_DIGITS) ok |= value == 8 || value == 10;
+  if (!ok) return fail(BASS_E_ARG, "bass_tune: value %d not allowed for knob %d", value, knob);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const int old = g_tune[knob][size_class];
+  g_tune[knob][size_class] = value;
+  return old;
 }
 
 int bass_sort_digit_bits(int bits) {

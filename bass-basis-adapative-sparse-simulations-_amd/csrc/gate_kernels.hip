@@ -308,38 +308,64 @@ int launch_group_build(int qb, const uint64_t* x, int64_t n, int q0, int q1, uin
   return (int)hipGetLastError();
 }
 
-int emit_tile_entries(int qb) { return NT * (qb == 1 ? 4 : 2); }
+// Input entries per emission tile: NT x items, items per thread 2, 4 (default)
+// or 8 for 1 qubit and 1, 2 (default) or 4 for 2 qubits.  A segmented output
+// needs the default (a tile then fills SEG_ENTRIES output slots).
+int emit_default_items(int qb) { return qb == 1 ? 4 : 2; }
+bool emit_items_ok(int qb, int items) {
+  return qb == 1 ? (items == 2 || items == 4 || items == 8) : (items == 1 || items == 2 || items == 4);
+}
+int emit_tile_entries(int qb, int items) { return NT * (items > 0 ? items : emit_default_items(qb)); }
+
+template <int QB, int ITEMS>
+static void emit_variant(int mode, bool dedup, dim3 grid, hipStream_t s, const uint64_t* x, const double2* a,
+                         int64_t n, int q0, int q1, const uint64_t* gslot, int shift, uint64_t capmask,
+                         const GateMat& G, uint64_t* xo, double2* ao, int64_t cap_out,
+                         unsigned long long* lb_status, unsigned int* ticket, unsigned int* hist,
+                         double* tile_psum, int64_t* n_out, int* seg_cnt, int* err) {
+#define BASS_EMIT(MODE_, DEDUP_)                                                                     \
+  hipLaunchKernelGGL((k_emit<QB, MODE_, DEDUP_, ITEMS>), grid, dim3(NT), 0, s, x, a, n, q0, q1,     \
+                     gslot, shift, capmask, G, xo, ao, cap_out, lb_status, ticket, hist, tile_psum,  \
+                     n_out, seg_cnt, err)
+  if (!dedup)
+    BASS_EMIT(MODE_GATE, false);
+  else if (mode == MODE_GATE)
+    BASS_EMIT(MODE_GATE, true);
+  else
+    BASS_EMIT(MODE_ROT, true);
+#undef BASS_EMIT
+}
 
 int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* a, int64_t n, int q0,
                 int q1, const uint64_t* gslot, int log2cap, const GateMat& G,
                 uint64_t* xo, double2* ao, int64_t cap_out, unsigned long long* lb_status,
                 unsigned int* ticket, unsigned int* hist, double* tile_psum, int64_t* n_out, int* seg_cnt,
-                int* err, hipStream_t s) {
+                int* err, hipStream_t s, int items) {
   if (n <= 0) return 0;
+  if (items <= 0 || seg_cnt) items = emit_default_items(qb);
   const int shift = 64 - log2cap;
   const uint64_t capmask = log2cap > 0 ? ((1ull << log2cap) - 1) : 0;
-  const int tile = emit_tile_entries(qb);
+  const int tile = emit_tile_entries(qb, items);
   const dim3 grid((unsigned)((n + tile - 1) / tile));
-#define BASS_EMIT(QB_, MODE_, DEDUP_, ITEMS_)                                                        \
-  hipLaunchKernelGGL((k_emit<QB_, MODE_, DEDUP_, ITEMS_>), grid, dim3(NT), 0, s, x, a, n, q0, q1,    \
-                     gslot, shift, capmask, G, xo, ao, cap_out, lb_status, ticket, hist,             \
-                     tile_psum, n_out, seg_cnt, err)
+#define BASS_EV(QB_, IT_)                                                                           \
+  emit_variant<QB_, IT_>(mode, dedup, grid, s, x, a, n, q0, q1, gslot, shift, capmask, G, xo, ao,  \
+                         cap_out, lb_status, ticket, hist, tile_psum, n_out, seg_cnt, err)
   if (qb == 1) {
-    if (!dedup)
-      BASS_EMIT(1, MODE_GATE, false, 4);
-    else if (mode == MODE_GATE)
-      BASS_EMIT(1, MODE_GATE, true, 4);
+    if (items == 2)
+      BASS_EV(1, 2);
+    else if (items == 8)
+      BASS_EV(1, 8);
     else
-      BASS_EMIT(1, MODE_ROT, true, 4);
+      BASS_EV(1, 4);
   } else {
-    if (!dedup)
-      BASS_EMIT(2, MODE_GATE, false, 2);
-    else if (mode == MODE_GATE)
-      BASS_EMIT(2, MODE_GATE, true, 2);
+    if (items == 1)
+      BASS_EV(2, 1);
+    else if (items == 4)
+      BASS_EV(2, 4);
     else
-      BASS_EMIT(2, MODE_ROT, true, 2);
+      BASS_EV(2, 2);
   }
-#undef BASS_EMIT
+#undef BASS_EV
   return (int)hipGetLastError();
 }
 

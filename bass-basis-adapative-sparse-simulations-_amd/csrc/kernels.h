@@ -67,12 +67,14 @@ struct OpStats {
 int group_slot_bytes(int qb);
 int launch_group_build(int qb, const uint64_t* x, int64_t n, int q0, int q1, uint64_t* gslot, int log2cap,
                        int* err, hipStream_t s);
-int emit_tile_entries(int qb);
+int emit_default_items(int qb);
+bool emit_items_ok(int qb, int items);
+int emit_tile_entries(int qb, int items = 0);
 int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* a, int64_t n, int q0,
                 int q1, const uint64_t* gslot, int log2cap, const GateMat& G,
                 uint64_t* xo, double2* ao, int64_t cap_out, unsigned long long* lb_status,
                 unsigned int* ticket, unsigned int* hist, double* tile_psum, int64_t* n_out, int* seg_cnt,
-                int* err, hipStream_t s);
+                int* err, hipStream_t s, int items = 0);
 // Every emission tile covers SEG_ENTRIES output slots (1q: 1024 inputs x 2, 2q: 512 x 4).
 constexpr int SEG_ENTRIES = 2048;
 int launch_diag2q(const uint64_t* x, double2* a, int64_t n, int q1, int q2, const double2* d,
@@ -239,13 +241,15 @@ constexpr int64_t SORT_MAX_N = (1ll << 30) - 1;  // 32-bit positions, 30-bit loo
 // (tk, tv), (tk2, tv2): two scratch record buffers of n entries; hist:
 // SORT_MAX_PASSES x 1024 words, tickets: SORT_MAX_PASSES words, both zero on
 // entry; status: sort_status_bytes(n), its first sweep_tiles(n) x 1024 words zero.
+// digit_bits: 8 or 10, or 0 for the process-wide choice (sort_digit_bits_set)
 int launch_radix_sort(uint64_t* keys, double2* vals, int64_t n, int key_bits, uint64_t* tk, double2* tv,
                       uint64_t* tk2, double2* tv2, unsigned int* hist, unsigned int* status,
-                      unsigned int* tickets, int* err, hipStream_t s);
+                      unsigned int* tickets, int* err, hipStream_t s, int digit_bits = 0);
 int sweep_tiles(int64_t n);
 size_t sort_status_bytes(int64_t n);
 int sort_plan(int key_bits, int* passes);
 int sort_digit_bits_set(int bits);
+int sort_digit_bits_get();
 int launch_merge(const uint64_t* x, const double2* a, int64_t n, uint64_t* xo, double2* ao,
                  unsigned long long* lb, unsigned int* ticket, int64_t* n_out, int* err, hipStream_t s);
 

@@ -328,6 +328,8 @@ __global__ __launch_bounds__(NTHR) void k_sort_sweep(const uint64_t* __restrict_
 
 static int g_sort_digit_bits = 0;  // 0: automatic
 
+int sort_digit_bits_get() { return g_sort_digit_bits; }
+
 int sort_digit_bits_set(int bits) {
   const int old = g_sort_digit_bits;
   g_sort_digit_bits = bits;
@@ -399,10 +401,14 @@ static void run_sort(uint64_t* keys, double2* vals, int64_t n, int key_bits, int
 // first tiles x 2^RB words (cleared pass by pass).
 int launch_radix_sort(uint64_t* keys, double2* vals, int64_t n, int key_bits, uint64_t* tk, double2* tv,
                       uint64_t* tk2, double2* tv2, unsigned int* hist, unsigned int* status, unsigned int* ticket,
-                      int* err, hipStream_t s) {
+                      int* err, hipStream_t s, int digit_bits) {
   if (n <= 1) return 0;
   int passes = 0;
-  const int rb = sort_plan(key_bits, &passes);
+  int rb = sort_plan(key_bits, &passes);
+  if (digit_bits == 8 || digit_bits == 10) {
+    rb = digit_bits;
+    passes = (key_bits + rb - 1) / rb;
+  }
   if (rb == 10)
     run_sort<10>(keys, vals, n, key_bits, passes, tk, tv, tk2, tv2, hist, status, ticket, err, s);
   else

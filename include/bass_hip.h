@@ -324,6 +324,14 @@ int bass_radix_sort(uint64_t* keys, double* vals, int64_t n, int key_bits, void*
  * previous setting.  Process-wide tuning knob; no counterpart in the reference. */
 int bass_sort_digit_bits(int bits);
 
+/* Per-size-class launch tuning (no counterpart in the reference): knob 0 =
+ * hash / group-table capacity factor (2 or 4), 1 = items per thread of 1-qubit
+ * emission blocks (2, 4, 8), 2 = the same for 2-qubit gates (1, 2, 4), 3 =
+ * radix digit width (8 or 10); size_class 0..3 for input counts < 2^14,
+ * < 2^18, < 2^22 and larger; value 0 = the built-in choice.  No knob changes a
+ * result.  Returns the previous value, or BASS_E_ARG.  Process-wide. */
+int bass_tune(int knob, int size_class, int value);
+
 /* The plan bass_radix_sort follows for keys of key_bits bits: returns the
  * digit width (8 or 10) and stores the number of digit passes in *passes (an
  * odd count adds one copy back into keys / vals).  No counterpart in the

@@ -479,3 +479,54 @@ def test_sparse_state_random_k_sequence(gpu_ctx, dedup):
             np.testing.assert_allclose(ga[og], o.alpha[: o.nnz][oo], rtol=1e-12, atol=1e-15)
         assert abs(st.gamma - o.gamma) <= 1e-12 * o.gamma
     assert o.nnz == k and draws >= 10
+
+
+def test_tuning_knobs_keep_results(gpu_ctx):
+    """Every value of every tuning knob (table capacity factor, emission tile
+    size, radix digit width; bass_tune) leaves the results bit for bit
+    unchanged: a sequence of 1- and 2-qubit gates with truncations through
+    SparseState (both dedup modes) at the size class of its state."""
+    print("[case] tuning knobs keep results", flush=True)
+    L = pkg_module("_lib")
+    T = pkg_module("tuning")
+    lib = L.load()
+    rng = np.random.default_rng(31)
+    N, n0, k = 30, 30000, 20000
+    x0, a0 = random_state(n0, N, seed=9)
+    gates = []
+    for i in range(6):
+        if i % 3 == 2:
+            q1, q2 = (int(v) for v in rng.choice(N, size=2, replace=False))
+            gates.append(OD._G((q1, q2), random_unitary(4, rng)))
+        else:
+            gates.append(OD._G((int(rng.integers(N)),), random_unitary(2, rng)))
+
+    def run(dedup):
+        st = ss().SparseState(N, k, device="cuda:0", dedup=dedup, capacity=4 * n0)
+        st._buf.ensure(n0)
+        st._buf.x[:n0] = torch.from_numpy(x0.view(np.int64)).to("cuda:0")
+        st._buf.a[:n0] = torch.from_numpy(a0).to("cuda:0")
+        st.nnz = n0
+        for g in gates:
+            st.apply_gate(g, "top-k")
+        return st.keys().copy(), st.alpha[: st.nnz].copy(), st.gamma
+
+    saved = {name: [lib.bass_tune(knob, c, 0) for c in range(4)] for name, knob in T.KNOBS.items()}
+    try:
+        ref = {d: run(d) for d in ("table", "sort")}
+        for name, knob in T.KNOBS.items():
+            for v in T.VALUES[name]:
+                for c in range(4):
+                    lib.bass_tune(knob, c, v)
+                for d in ("table", "sort"):
+                    kx, ka, gm = run(d)
+                    rk, ra, rg = ref[d]
+                    assert np.array_equal(kx, rk) and np.array_equal(ka.view(np.float64), ra.view(np.float64)), \
+                        (name, v, d)
+                    assert gm == rg
+                for c in range(4):
+                    lib.bass_tune(knob, c, 0)
+    finally:
+        for name, knob in T.KNOBS.items():
+            for c, v in enumerate(saved[name]):
+                lib.bass_tune(knob, c, v)
