@@ -235,6 +235,21 @@ class SparseState:
         # key-ordered layout (set by the BASS driver): entries sorted by stored
         # word, qubit q at bit pos[q]; None = keys stored as is
         self.pos = None
+        # optional operation counters (records produced, algorithmic bytes)
+        self.counters = None
+
+    def _count_gate(self, op, n_in, n_out):
+        if self.counters is not None:
+            c = self.counters
+            c["amp_updates"] = c.get("amp_updates", 0) + n_out
+            c["bytes_" + op] = c.get("bytes_" + op, 0) + 24 * (n_in + n_out)
+            c[op] = c.get(op, 0) + 1
+
+    def _count_select(self, n_in, k):
+        if self.counters is not None:
+            c = self.counters
+            c["bytes_select"] = c.get("bytes_select", 0) + 40 * n_in + 24 * k
+            c["truncate"] = c.get("truncate", 0) + 1
 
     # host views ------------------------------------------------------------
     @property
@@ -299,6 +314,7 @@ class SparseState:
     def apply_single_qubit_gate(self, gate, method, seed):
         """sparse_state.py:268-301: gate, deduplicate, truncate to k if needed."""
         q = gate.qubits[0]
+        n_in = self.nnz
         if self.dedup == "sort":
             n, _ = _ops.gate(self._ctx, self._buf, self.nnz, gate.matrix, (q,), self._tmp, dedup=False)
             radix_sort_uint64(self._tmp.x, self._tmp.a, n, key_bits=max(1, self.N))
@@ -307,6 +323,7 @@ class SparseState:
         else:
             n, ht = _ops.gate(self._ctx, self._buf, self.nnz, gate.matrix, (q,), self._tmp, dedup=True)
             self._hist_tiles = ht
+        self._count_gate("gate1q", n_in, n)
         self._swap()
         self._tmp.ensure(4 * max(n, 1))
         self.nnz = n
@@ -317,6 +334,7 @@ class SparseState:
         (stable radix sort), merge equal keys (dedup="sort", the reference's
         path); dedup="table" deduplicates through the group table instead
         (same entries and sums, no sort)."""
+        n_in = self.nnz
         q1, q2 = gate.qubits
         if self.dedup == "sort":
             n, _ = _ops.gate(self._ctx, self._buf, self.nnz, gate.matrix, (q1, q2), self._tmp, dedup=False)
@@ -325,6 +343,7 @@ class SparseState:
             ht = 0
         else:
             n, ht = _ops.gate(self._ctx, self._buf, self.nnz, gate.matrix, (q1, q2), self._tmp, dedup=True)
+        self._count_gate("gate2q", n_in, n)
         self._hist_tiles = ht
         self._swap()
         self._tmp.ensure(4 * max(n, 1))
@@ -363,6 +382,7 @@ class SparseState:
         st = _ops.truncate(self._ctx, self._buf, self.nnz, self.k, self._tmp, renorm=True,
                            hist_tiles=self._hist_tiles)
         self._swap()
+        self._count_select(self.nnz, self.k)
         self.nnz = self.k
         self.gamma *= np.sqrt(st.kept)
         self._hist_tiles = 0

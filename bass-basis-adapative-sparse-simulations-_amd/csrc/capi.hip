@@ -287,11 +287,16 @@ static int tuned(int knob, int64_t n) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   return g_tune[knob][tune_class(n)];
 }
-static int table_log2cap(int64_t n) {
-  const int f = tuned(TUNE_TABLE_FACTOR, n);
-  int l = 0;
-  while ((1ll << l) < (f == 4 ? 4 : 2) * n) ++l;
-  return l < 6 ? 6 : l;
+// Table capacity (log2) for n keys of slot_bytes each: the tuned factor when
+// the table fits in `room` bytes of workspace, else the default factor 2.
+static int table_log2cap(int64_t n, size_t slot_bytes, size_t room) {
+  auto lg = [n](int f) {
+    int l = 0;
+    while ((1ll << l) < f * n) ++l;
+    return l < 6 ? 6 : l;
+  };
+  const int l = lg(tuned(TUNE_TABLE_FACTOR, n) == 4 ? 4 : 2);
+  return ((size_t)1 << l) * slot_bytes <= room ? l : lg(2);
 }
 
 // ---------------------------------------------------- no-allocation call ----
@@ -426,7 +431,7 @@ int bass_gate(const uint64_t* x, const double* a, int64_t n, int nq, int q0, int
   int log2cap = 0;
   uint64_t* gslot = nullptr;
   if (dedup && n > 0) {
-    log2cap = table_log2cap(n);
+    log2cap = table_log2cap(n, (size_t)group_slot_bytes(nq), L.total - L.big);
     const size_t cap = (size_t)1 << log2cap;
     const size_t bytes = cap * (size_t)group_slot_bytes(nq);
     if (bytes > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_gate: group table does not fit");
@@ -720,7 +725,7 @@ int bass_rdm1q(const uint64_t* x, const double* a, int64_t n, int N, double* rho
   if (n <= 0) {
     fill(rho, (size_t)N * 4 * 8, 0ull, s);
   } else {
-    int log2cap = table_log2cap(n);
+    int log2cap = table_log2cap(n, 12, L.total - L.big);
     
     const size_t cap = (size_t)1 << log2cap;
     if (cap * 12 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_rdm1q: table does not fit");

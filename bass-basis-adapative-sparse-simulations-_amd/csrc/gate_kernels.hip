@@ -54,6 +54,12 @@ struct GSlot {
   static constexpr int WORDS = QB == 1 ? 2 : 4;  // u64 words per slot
 };
 
+// Insert: lanes of a wave whose entries share a rest key (members of one
+// group) are matched by ballots first -- one round per distinct key of the
+// wave, the lowest unmatched lane's key each round -- and only the lowest lane
+// of each match probes and CASes the table; the others take its slot by a
+// shuffle and write their member index there.  Colliding inserts of one group
+// inside a wave thus cost one probe sequence instead of up to 2^QB.
 template <int QB>
 __global__ __launch_bounds__(NT) void k_group_build(const uint64_t* __restrict__ x, int64_t n, int q0,
                                                      int q1, uint64_t* __restrict__ gslot, int shift,
@@ -61,26 +67,45 @@ __global__ __launch_bounds__(NT) void k_group_build(const uint64_t* __restrict__
   constexpr int M = 1 << QB;
   constexpr int W = GSlot<QB>::WORDS;
   const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
-  if (i >= n) return;
-  const uint64_t key = x[i];
-  const uint64_t r = key & ~out_bits<QB>(M - 1, q0, q1);
-  const int c = member_index<QB>(key, q0, q1);
-  uint64_t h = hslot(r, shift);
-  for (uint64_t probe = 0; probe <= capmask; ++probe) {
-    uint64_t* slot = gslot + h * W;
-    uint64_t cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (cur == EMPTY) {
-      unsigned long long prev = atomicCAS((unsigned long long*)slot, (unsigned long long)EMPTY,
-                                          (unsigned long long)r);
-      cur = (prev == (unsigned long long)EMPTY) ? r : (uint64_t)prev;
-    }
-    if (cur == r) {
-      reinterpret_cast<int32_t*>(slot + 1)[c] = (int32_t)i;
-      return;
-    }
-    h = (h + 1) & capmask;
+  const bool valid = i < n;  // every lane takes part in the wave-level match
+  const int lane = threadIdx.x & 63;
+  uint64_t r = 0;
+  int c = 0;
+  if (valid) {
+    const uint64_t key = x[i];
+    r = key & ~out_bits<QB>(M - 1, q0, q1);
+    c = member_index<QB>(key, q0, q1);
   }
-  set_err(err, ERR_PROBE);
+  int leader = lane;
+  unsigned long long pend = __ballot(valid);
+  while (pend) {  // wave-uniform
+    const int f = __ffsll((long long)pend) - 1;
+    const uint64_t rf = (uint64_t)__shfl((long long)r, f, 64);
+    const unsigned long long same = __ballot(valid && r == rf) & pend;
+    if ((same >> lane) & 1ull) leader = f;
+    pend &= ~same;
+  }
+  long long hs = -1;  // slot of the group (leaders), -1: none
+  if (valid && leader == lane) {
+    uint64_t h = hslot(r, shift);
+    for (uint64_t probe = 0; probe <= capmask; ++probe) {
+      uint64_t* slot = gslot + h * W;
+      uint64_t cur = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cur == EMPTY) {
+        unsigned long long prev = atomicCAS((unsigned long long*)slot, (unsigned long long)EMPTY,
+                                            (unsigned long long)r);
+        cur = (prev == (unsigned long long)EMPTY) ? r : (uint64_t)prev;
+      }
+      if (cur == r) {
+        hs = (long long)h;
+        break;
+      }
+      h = (h + 1) & capmask;
+    }
+    if (hs < 0) set_err(err, ERR_PROBE);
+  }
+  hs = __shfl(hs, leader, 64);
+  if (valid && hs >= 0) reinterpret_cast<int32_t*>(gslot + (uint64_t)hs * W + 1)[c] = (int32_t)i;
 }
 
 template <int QB>

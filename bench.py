@@ -392,6 +392,72 @@ def bench_served(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_fixed(args):
+    """FixedBasisSimulator (the reference's fixed-basis driver: group-table
+    deduplicated 1- and 2-qubit gates in input order, top-k with the 2-qubit
+    gates' gamma rule) on the brickwork circuit of the plain benchmark, n=40,
+    K=2^22: --warmup layers untimed, --steps layers timed, per-operation
+    HIP-event timing and algorithmic bytes (gate 24 (n_in + n_out), select
+    40 n + 24 k)."""
+    import numpy as np
+    import torch
+
+    lib = importlib.import_module(PKG + "._lib")
+    rc = importlib.import_module(PKG + ".utils.random_circuits")
+    Fs = importlib.import_module(PKG + ".simulation.simulator")
+    lib.load()
+    N, K = args.qubits or 40, 1 << (args.log2k or 22)
+    total = 30
+    if args.warmup + args.steps > total:
+        raise SystemExit(f"--warmup + --steps exceeds the circuit's {total} layers")
+    circ, ends = rc.brickwork_haar_circuit(N, total, np.random.default_rng(SEED))
+    starts = [0] + [e + 1 for e in ends[:-1]]
+    sim = Fs.FixedBasisSimulator(N, K, verbose=False, device="cuda:0")
+    sim.begin(len(circ), seed=0)
+    log(f"fixed-basis brickwork n={N} K=2^{K.bit_length() - 1}; timing layers {args.warmup}..")
+    for layer in range(args.warmup):
+        for i in range(starts[layer], ends[layer] + 1):
+            sim.step(i, circ[i])
+    torch.cuda.synchronize()
+    c0 = dict(sim.counters)
+    lib.timing(True)
+    t0 = time.perf_counter()
+    for layer in range(args.warmup, args.warmup + args.steps):
+        for i in range(starts[layer], ends[layer] + 1):
+            sim.step(i, circ[i])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    c1 = dict(sim.counters)
+    delta = {k: c1.get(k, 0) - c0.get(k, 0) for k in c1}
+    kernels = {}
+    for op in ["gate1q", "gate2q", "select"]:
+        cnt, ms = lib.timing_read(op)
+        if cnt == 0:
+            continue
+        byt = delta.get("bytes_" + op, 0)
+        gbs = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        kernels[op] = {"launches": cnt, "avg_ms": ms / cnt, "total_ms": ms, "bytes_per_launch": byt / cnt,
+                       "achieved_GBps": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+    lib.timing(False)
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    d = kernels[dom]
+    out = {
+        "metric": "sparse amp-updates/s", "value": delta.get("amp_updates", 0) / dt, "unit": "amp-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "complex128",
+        "data": "synthetic",
+        "config": {"workload": f"fixed-basis (hash-table gates) brickwork-haar n={N} layers={total} "
+                               f"K=2^{K.bit_length() - 1} (step = 1 layer)",
+                   "gate_counts": {k: v for k, v in delta.items() if not k.startswith("bytes_")}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": d["achieved_GBps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": d["frac_hbm"], "traffic": None, "bytes_per_launch": d["bytes_per_launch"],
+                     "avg_ms": d["avg_ms"]},
+        "kernels": kernels,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def cpu_model():
     """The host CPU's model name (from /proc/cpuinfo), or None."""
     try:
@@ -518,7 +584,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed steps (circuit layers)")
     ap.add_argument("--warmup", type=int, default=4, help="untimed steps before them")
-    ap.add_argument("--workload", choices=["kicked-ising", "brickwork", "batched", "served", "tfim20", "sort"],
+    ap.add_argument("--workload", choices=["kicked-ising", "brickwork", "batched", "served", "tfim20", "sort",
+                                           "fixed-brickwork"],
                     default="kicked-ising")
     ap.add_argument("--no-secondary", action="store_true",
                     help="kicked-ising: skip the brickwork n=40 K=2^22 line reported beside it")
@@ -564,6 +631,8 @@ def main():
         return bench_tfim20(args)
     if args.workload == "sort":
         return bench_sort(args)
+    if args.workload == "fixed-brickwork":
+        return bench_fixed(args)
     if args.workload == "served":
         if args.workers is None:
             args.workers = 4 if args.server == "engine" else 8
