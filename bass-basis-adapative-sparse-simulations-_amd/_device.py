@@ -44,9 +44,15 @@ class Context:
             return ctypes.c_void_p(self._raw_stream(self._dev_index))
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
-    def ensure_ws(self, max_entries):
+    def ensure_ws(self, max_entries, keep=False):
+        """Grow the workspace to hold calls of max_entries entries.  keep=True:
+        the workspace holds a preceding call's results that this call reads (a
+        gate's histogram and per-tile sums), so it must not be replaced."""
         need = self._ws_fixed + self._ws_per * max(1, int(max_entries))
         if need > self.ws_bytes:
+            if keep:
+                raise RuntimeError(f"workspace of {self.ws_bytes} bytes holds the preceding gate's histogram; "
+                                   f"this call needs {need}: size it before the gate")
             grow = max(need, int(self.ws_bytes * 1.5))
             self.ws = None
             # zeroed: the stats block keeps device error bits between calls

@@ -27,7 +27,9 @@ def gate(ctx, src, n, G, qubits, dst, mode=L.MODE_GATE, dedup=True, segmented=Fa
     nq = len(qubits)
     M = 1 << nq
     need = _tiles(n, nq) * SEG if segmented else M * n
-    ctx.ensure_ws(max(need, n, 1))
+    # a truncate of the output reads this call's histogram from the workspace
+    # and needs whole SEG-slot tiles of it: size for that now
+    ctx.ensure_ws(max(-(-need // SEG) * SEG, n, 1))
     dst.ensure(max(need, 1))
     buf, gp = L.mat_arg(G)
     q0 = int(qubits[0])
@@ -156,9 +158,12 @@ def truncate_launch(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=Fa
     handle truncate_finish() takes.  Nothing may launch a stats-writing
     operation on the context in between."""
     n = 0 if n is None else n
-    # the select reads whole tiles of SEG slots (and hist_tiles of per-tile sums)
+    # the select reads whole tiles of SEG slots, and hist_tiles per-tile sums
+    # of the producing gate (segmented: SEG slots a tile; dense: emission tiles
+    # of >= 512 slots, of which a workspace of S entries keeps S // 512 + 2)
     tiles = hist_tiles if segmented else -(-int(n) // SEG)
-    ctx.ensure_ws(max(n, tiles * SEG, hist_tiles * SEG))
+    psum = 0 if segmented else max(hist_tiles - 2, 0) * 512
+    ctx.ensure_ws(max(n, tiles * SEG, psum), keep=hist_tiles > 0)
     dst.ensure(k)
     if key_map is None:
         km, kp, kb = None, None, 0

@@ -84,6 +84,10 @@ struct Layout {
   size_t total;
 };
 
+// Entries of each per-chunk array: chunks of the largest tile count a call
+// reduces (emission tiles, >= 512 output slots each).
+static size_t chunk_slots(int64_t S) { return (size_t)sel_chunks(S / 512 + 2) + 1; }
+
 static Layout layout(int64_t S) {
   Layout L;
   L.S = S;
@@ -121,8 +125,9 @@ static Layout layout(int64_t S) {
   L.lift_b = o; o += al((size_t)lift_bounds_entries(etiles + 1) * sizeof(LiftBound));
   L.hist2_rows = o; o += al((size_t)stiles * HIST_BINS * 2);
   // per-chunk values of the chunked reductions: 6 arrays for a select, 2 for
-  // a gate's segment scan (read later by bass_densify)
-  L.chunks = o; o += al((size_t)8 * 8 * (sel_chunks(stiles) + 1));
+  // a gate's segment scan (read later by bass_densify); a select reading a
+  // dense gate's per-tile sums spans up to etiles tiles
+  L.chunks = o; o += al((size_t)8 * 8 * chunk_slots(S));
   L.count_slot = o; o += al(8);
   L.oplog = o; o += al(8 + (size_t)OPLOG_CAP * sizeof(OpStats));
   L.ktab = o; o += al((size_t)KTAB_WORDS * 8);
@@ -156,11 +161,9 @@ static int log2_ceil(int64_t v) {
 
 // chunk counts / offsets of a gate's segment scan (after the select's six arrays)
 static int64_t* seg_chunk_cnt(void* ws, const Layout& L) {
-  return at<int64_t>(ws, L.chunks) + 6 * ((size_t)sel_chunks(L.S / 2048 + 2) + 1);
+  return at<int64_t>(ws, L.chunks) + 6 * chunk_slots(L.S);
 }
-static int64_t* seg_chunk_off(void* ws, const Layout& L) {
-  return seg_chunk_cnt(ws, L) + ((size_t)sel_chunks(L.S / 2048 + 2) + 1);
-}
+static int64_t* seg_chunk_off(void* ws, const Layout& L) { return seg_chunk_cnt(ws, L) + chunk_slots(L.S); }
 
 static SelTiles tiles_view(void* ws, const Layout& L, unsigned long long* c1p, unsigned int* c1i) {
   SelTiles T;
@@ -176,7 +179,7 @@ static SelTiles tiles_view(void* ws, const Layout& L, unsigned long long* c1p, u
   T.tile_s2 = at<double>(ws, L.tile_s2);
   T.tile_s4 = at<double>(ws, L.tile_s4);
   T.hist2_rows = at<unsigned short>(ws, L.hist2_rows);
-  const size_t nc = (size_t)sel_chunks(L.S / 2048 + 2) + 1;
+  const size_t nc = chunk_slots(L.S);
   int64_t* ch = at<int64_t>(ws, L.chunks);
   T.ch_cnt = ch;
   T.ch_off = ch + nc;
@@ -275,8 +278,10 @@ static int hip_check(const char* where) {
 // table): the hash/group-table capacity factor (capacity = next power of two
 // >= factor * n: factor 2 keeps the load factor <= 1/2, 4 <= 1/4), the items
 // per thread of the emission kernels (the block's tile = 256 x items entries)
-// and the radix digit width.  0 = the built-in choice.  None of them changes a
-// result.
+// and the radix digit width.  0 = the built-in choice.  Keys, amplitudes and
+// kept sums do not depend on them; the emission tile size groups a dense gate
+// output's input total (per-tile sums, then tiles in order), so the total may
+// differ in its last bits.
 enum { TUNE_TABLE_FACTOR = 0, TUNE_EMIT_ITEMS_1Q = 1, TUNE_EMIT_ITEMS_2Q = 2, TUNE_SORT_DIGITS = 3, TUNE_KNOBS = 4 };
 constexpr int TUNE_CLASSES = 4;
 static int g_tune[TUNE_KNOBS][TUNE_CLASSES] = {};
@@ -613,7 +618,10 @@ static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t 
   }
   const int64_t tiles = segmented ? hist_tiles : sel_tiles(n);
   const int64_t slots = tiles * SEG_ENTRIES;
-  const int64_t psum_slots = hist_tiles > 0 ? hist_tiles * SEG_ENTRIES : 0;  // per-tile sums read
+  // per-tile sums read: a segmented input's tiles hold SEG_ENTRIES slots, a
+  // dense gate output's emission tiles >= 512 (the layout keeps S / 512 + 2)
+  const int64_t psum_slots = hist_tiles <= 0 ? 0 : segmented ? hist_tiles * SEG_ENTRIES
+                                                             : (hist_tiles > 2 ? (hist_tiles - 2) * 512 : 0);
   const int64_t need_ws = slots > n ? slots : n;
   WS_REQUIRE(need_ws > psum_slots ? need_ws : psum_slots);
   hipStream_t s = (hipStream_t)stream;

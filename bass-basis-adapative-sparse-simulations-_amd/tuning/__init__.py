@@ -1,8 +1,11 @@
 """Per-size-class tuning of the kernel launches (no counterpart in the reference).
 
 The library keeps, per size class of a call's input count n (0: n < 2^14,
-1: < 2^18, 2: < 2^22, 3: larger), four knobs that change how work is laid out
-but never a result:
+1: < 2^18, 2: < 2^22, 3: larger), four knobs that change how work is laid
+out.  Keys, amplitudes and kept sums never depend on them; the emission tile
+size groups the input total of a dense gate output (per-tile sums, then tiles
+in order), so a truncation's total -- and a kept/total ratio built from it --
+may move in its last bits:
 
   table_factor   hash / group-table capacity = next power of two >= factor * n
                  (2: load factor <= 1/2, 4: <= 1/4)      bass_gate, bass_rdm1q

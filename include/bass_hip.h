@@ -328,8 +328,11 @@ int bass_sort_digit_bits(int bits);
  * hash / group-table capacity factor (2 or 4), 1 = items per thread of 1-qubit
  * emission blocks (2, 4, 8), 2 = the same for 2-qubit gates (1, 2, 4), 3 =
  * radix digit width (8 or 10); size_class 0..3 for input counts < 2^14,
- * < 2^18, < 2^22 and larger; value 0 = the built-in choice.  No knob changes a
- * result.  Returns the previous value, or BASS_E_ARG.  Process-wide. */
+ * < 2^18, < 2^22 and larger; value 0 = the built-in choice.  Output keys,
+ * amplitudes and kept sums do not depend on any knob; the emission tile size
+ * sets the grouping of a dense gate output's input total (summed per tile,
+ * then over tiles in order), which may move its last bits.  Returns the
+ * previous value, or BASS_E_ARG.  Process-wide. */
 int bass_tune(int knob, int size_class, int value);
 
 /* The plan bass_radix_sort follows for keys of key_bits bits: returns the

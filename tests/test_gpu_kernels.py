@@ -530,3 +530,43 @@ def test_tuning_knobs_keep_results(gpu_ctx):
         for name, knob in T.KNOBS.items():
             for c, v in enumerate(saved[name]):
                 lib.bass_tune(knob, c, v)
+
+
+def test_fresh_workspace_gate_then_truncate(gpu_ctx):
+    """A 2-qubit gate whose tiles are smaller than the select's (emit_items_2q
+    = 1: 256-entry tiles, twice the default count) followed by the truncate
+    that reads its histogram, on a context whose workspace starts empty: the
+    truncate finds the workspace the gate sized (it must not be replaced
+    between the two) and every tile setting gives the default's keys and
+    amplitudes bit for bit (the fixed-basis driver, N=20, k=10000, a 6-layer
+    brickwork); its gamma multiplies kept/total ratios whose input totals are
+    summed per emission tile, so the tile size may move gamma by a few ulps
+    (1e-14 relative here)."""
+    print("[case] fresh workspace: gate then truncate, every 2q tile setting", flush=True)
+    L = pkg_module("_lib")
+    T = pkg_module("tuning")
+    D = pkg_module("_device")
+    R = pkg_module("experimentation.runner")
+    S = pkg_module("simulation.simulator")
+    lib = L.load()
+    circ = R.make_brickwork(20, 6, np.random.default_rng(20260520 * 10 ** 6))
+    knob = T.KNOBS["emit_items_2q"]
+    saved = [lib.bass_tune(knob, c, 0) for c in range(4)]
+    key = str(torch.device("cuda:0"))
+    shared = D._CONTEXTS.pop(key)
+    try:
+        out = []
+        for v in (0,) + T.VALUES["emit_items_2q"]:
+            for c in range(4):
+                lib.bass_tune(knob, c, v)
+            D._CONTEXTS.pop(key, None)  # a new context: its workspace grows from nothing
+            st = S.FixedBasisSimulator(20, 10000, verbose=False, device="cuda:0").simulate(circ, seed=1)
+            out.append((st.keys().copy(), st.alpha[: st.nnz].copy(), st.gamma))
+        for v, (kx, ka, gm) in zip(T.VALUES["emit_items_2q"], out[1:]):
+            assert np.array_equal(kx, out[0][0]), v
+            assert np.array_equal(ka.view(np.float64), out[0][1].view(np.float64)), v
+            assert abs(gm - out[0][2]) <= 1e-14 * out[0][2], v
+    finally:
+        D._CONTEXTS[key] = shared
+        for c, v in enumerate(saved):
+            lib.bass_tune(knob, c, v)
