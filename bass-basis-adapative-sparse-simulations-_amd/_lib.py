@@ -70,6 +70,7 @@ SIGNATURES = {
     "bass_lift_counted": (_I, [_P, _P, _I64, _I, _I, _P, _I, _P, _P, _I64, ctypes.POINTER(_I64), _P, _SZ, _P]),
     "bass_set_count": (_I, [_P, _SZ, _I64, _P]),
     "bass_diag2q_counted": (_I, [_P, _P, _I64, _I, _I, _P, _P, _SZ, _P]),
+    "bass_diag2q_run": (_I, [_P, _P, _I64, _I, _P, _P, _I, _P, _SZ, _P]),
     "bass_counted_read": (_I, [_P, _SZ, _P, _I64, _P, _P]),
     "bass_truncate": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _I64, _I, _P, _I, _P, _SZ, _P]),
     "bass_truncate_noalloc": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _P, _I, _P, _P]),

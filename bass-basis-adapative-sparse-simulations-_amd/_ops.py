@@ -124,6 +124,23 @@ def diag2q_counted(ctx, buf, n_max, q1, q2, diag):
     L.check(rc, "bass_diag2q_counted")
 
 
+DIAG_RUN_MAX = 32  # gates per launch of bass_diag2q_run
+
+
+def diag2q_run(ctx, buf, n, gates, counted=False):
+    """Diagonal 2-qubit gates [(q1, q2, diag[4]), ...] applied in order in one
+    pass per DIAG_RUN_MAX gates (bit-identical to one diag2q() per gate).
+    counted=True: n bounds the count read from the state-count slot."""
+    if not gates:
+        return
+    qs = np.ascontiguousarray([[int(g[0]), int(g[1])] for g in gates], dtype=np.int32)
+    ds = np.ascontiguousarray([np.asarray(g[2], dtype=np.complex128) for g in gates]).view(np.float64)
+    rc = ctx.lib.bass_diag2q_run(buf.xp, buf.ap, int(n), len(gates), qs.ctypes.data_as(ctypes.c_void_p),
+                                 ds.ctypes.data_as(ctypes.c_void_p), 1 if counted else 0, ctx.wsp, ctx.ws_bytes,
+                                 ctx.stream)
+    L.check(rc, "bass_diag2q_run")
+
+
 def counted_begin(ctx, n, n_max):
     """Start a run of counted gates on a state of n entries (<= n_max): size the
     workspace for every operation of the run first (the largest is a 2-qubit

@@ -486,6 +486,38 @@ int bass_diag2q_counted(const uint64_t* x, double* a, int64_t n_max, int q1, int
   return hip_check("bass_diag2q_counted");
 }
 
+int bass_diag2q_run(const uint64_t* x, double* a, int64_t n, int ngates, const int* qubits, const double* diags,
+                    int counted, void* ws, size_t ws_bytes, void* stream) {
+  if (n < 0 || n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_diag2q_run: bad n %lld", (long long)n);
+  if (ngates < 0 || (ngates > 0 && (!qubits || !diags))) return fail(BASS_E_ARG, "bass_diag2q_run: bad gate list");
+  for (int g = 0; g < ngates; ++g) {
+    const int q1 = qubits[2 * g], q2 = qubits[2 * g + 1];
+    if (q1 < 0 || q1 > 62 || q2 < 0 || q2 > 62 || q1 == q2)
+      return fail(BASS_E_ARG, "bass_diag2q_run: gate %d: bad qubits %d %d", g, q1, q2);
+  }
+  const int64_t* n_dev = nullptr;
+  if (counted) {
+    WS_REQUIRE(0);
+    n_dev = at<int64_t>(ws, L.count_slot);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int tk = ktimer_begin("diag2q", s);
+  for (int g0 = 0; g0 < ngates; g0 += DIAG_RUN_MAX) {
+    DiagRun R;
+    memset(&R, 0, sizeof(R));
+    R.ngates = ngates - g0 < DIAG_RUN_MAX ? ngates - g0 : DIAG_RUN_MAX;
+    for (int g = 0; g < R.ngates; ++g) {
+      R.q[g][0] = (unsigned char)qubits[2 * (g0 + g)];
+      R.q[g][1] = (unsigned char)qubits[2 * (g0 + g) + 1];
+      for (int e = 0; e < 4; ++e)
+        R.d[g][e] = make_double2(diags[8 * (g0 + g) + 2 * e], diags[8 * (g0 + g) + 2 * e + 1]);
+    }
+    launch_diag2q_run(x, (double2*)a, n, R, s, n_dev);
+  }
+  ktimer_end(tk, s);
+  return hip_check("bass_diag2q_run");
+}
+
 int bass_set_count(void* ws, size_t ws_bytes, int64_t n, void* stream) {
   if (n < 0) return fail(BASS_E_ARG, "bass_set_count: bad n %lld", (long long)n);
   WS_REQUIRE(0);

@@ -316,6 +316,47 @@ __global__ __launch_bounds__(NT) void k_diag2q(const uint64_t* __restrict__ x, d
   }
 }
 
+// A run of diagonal 2-qubit gates in one pass: per entry, gate g = 0, 1, ...
+// in order multiplies the amplitude by its diagonal entry -- the products of
+// k_diag2q, in the same order, so the result equals one launch per gate bit
+// for bit, from one read and one write of the state.  The gate tables sit in
+// LDS (16 B per entry, indexed by the entry's two bits); each thread keeps
+// DIAG_ITEMS entries in flight.
+constexpr int DIAG_ITEMS = 4;
+__global__ __launch_bounds__(NT) void k_diag2q_run(const uint64_t* __restrict__ x, double2* __restrict__ a,
+                                                    int64_t n, DiagRun R, const int64_t* __restrict__ n_dev) {
+  __shared__ double2 s_d[DIAG_RUN_MAX * 4];
+  for (int e = threadIdx.x; e < R.ngates * 4; e += NT) s_d[e] = R.d[e >> 2][e & 3];
+  __syncthreads();
+  n = dev_count(n_dev, n);
+  const int64_t per = (int64_t)NT * DIAG_ITEMS;
+  for (int64_t b = (int64_t)blockIdx.x * per; b < n; b += (int64_t)gridDim.x * per) {
+    uint64_t key[DIAG_ITEMS];
+    double2 v[DIAG_ITEMS];
+#pragma unroll
+    for (int it = 0; it < DIAG_ITEMS; ++it) {
+      const int64_t i = b + it * NT + threadIdx.x;
+      if (i < n) {
+        key[it] = x[i];
+        v[it] = a[i];
+      }
+    }
+    for (int g = 0; g < R.ngates; ++g) {
+      const int q1 = R.q[g][0], q2 = R.q[g][1];
+#pragma unroll
+      for (int it = 0; it < DIAG_ITEMS; ++it) {
+        const int idx = (int)((((key[it] >> q1) & 1ull) << 1) | ((key[it] >> q2) & 1ull));
+        v[it] = cmul(v[it], s_d[g * 4 + idx]);
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < DIAG_ITEMS; ++it) {
+      const int64_t i = b + it * NT + threadIdx.x;
+      if (i < n) a[i] = v[it];
+    }
+  }
+}
+
 // ----------------------------------------------------------- host launch ---
 
 int group_slot_bytes(int qb) { return qb == 1 ? 8 * GSlot<1>::WORDS : 8 * GSlot<2>::WORDS; }
@@ -391,6 +432,15 @@ int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* 
       BASS_EV(2, 2);
   }
 #undef BASS_EV
+  return (int)hipGetLastError();
+}
+
+int launch_diag2q_run(const uint64_t* x, double2* a, int64_t n, const DiagRun& R, hipStream_t s,
+                      const int64_t* n_dev) {
+  if (n <= 0 || R.ngates <= 0) return 0;
+  int64_t blocks = (n + (int64_t)NT * DIAG_ITEMS - 1) / ((int64_t)NT * DIAG_ITEMS);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_diag2q_run, dim3((unsigned)blocks), dim3(NT), 0, s, x, a, n, R, n_dev);
   return (int)hipGetLastError();
 }
 

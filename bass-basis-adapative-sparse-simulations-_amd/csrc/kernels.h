@@ -79,6 +79,16 @@ int launch_emit(int qb, int mode, bool dedup, const uint64_t* x, const double2* 
 constexpr int SEG_ENTRIES = 2048;
 int launch_diag2q(const uint64_t* x, double2* a, int64_t n, int q1, int q2, const double2* d,
                   hipStream_t s, const int64_t* n_dev = nullptr);
+// A run of up to DIAG_RUN_MAX diagonal 2-qubit gates (kernel argument: 2 KB
+// of tables), applied in order in one pass.
+constexpr int DIAG_RUN_MAX = 32;
+struct DiagRun {
+  double2 d[DIAG_RUN_MAX][4];
+  unsigned char q[DIAG_RUN_MAX][2];
+  int ngates;
+};
+int launch_diag2q_run(const uint64_t* x, double2* a, int64_t n, const DiagRun& R, hipStream_t s,
+                      const int64_t* n_dev = nullptr);
 
 // Key-ordered state: merge split of the run around one tile boundary.
 struct LiftBound {

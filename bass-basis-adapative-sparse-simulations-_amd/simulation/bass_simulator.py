@@ -314,6 +314,10 @@ class BASS:
         self.device_counts = bool(device_counts)
         self._cpending = []
         self._cstart_n = 0
+        # counted diagonal 2-qubit gates not yet launched: (bit1, bit2, diag);
+        # a run of them goes to the device as one pass (bass_diag2q_run) when
+        # the next other operation or a read of the state comes
+        self._drun = []
 
     @property
     def counters(self):
@@ -325,9 +329,17 @@ class BASS:
     def _count(self, key, v=1):
         self._counters[key] = self._counters.get(key, 0) + v
 
+    def _diag_flush(self):
+        """Launch the pending run of diagonal gates (one pass per
+        DIAG_RUN_MAX gates, the same products as one launch per gate)."""
+        run, self._drun = self._drun, []
+        for c in range(0, len(run), _ops.DIAG_RUN_MAX):
+            _ops.diag2q_run(self._ctx, self.state._buf, self.k, run[c: c + _ops.DIAG_RUN_MAX], counted=True)
+
     def _counted_flush(self):
         """Read the op log of the pending counted gates and book them exactly as
         the per-gate path (_fast_truncate) does, in order."""
+        self._diag_flush()
         if not self._cpending:
             return
         state, pend = self.state, self._cpending
@@ -344,8 +356,10 @@ class BASS:
         for i, op, moved in pend:
             if op == "diag2q":
                 # in place, count unchanged; moments recomputed on demand, as after
-                # a per-gate diagonal gate
-                self._bytes("diag2q", n_prev, n_prev)
+                # a per-gate diagonal gate.  One pass of the state per launch of a
+                # run (`moved` marks a gate that starts one).
+                if moved:
+                    self._bytes("diag2q", n_prev, n_prev)
                 self._count("amp_updates", n_prev)
                 sums = (None, None)
                 continue
@@ -661,6 +675,7 @@ class BASS:
         self._pr_at_last_opt = 0.0
         self._counters = {"amp_updates": 0}
         self._cpending = []
+        self._drun = []
         self._hard_cap = min(self.max_nnz_factor * self.k, 2 ** self.N)
         internal_k = min(max(max(self.buffer_factor, 2) * self.k, self._hard_cap), 2 ** self.N)
         # buffers start at the truncation size and grow on demand (internal_k
@@ -687,6 +702,7 @@ class BASS:
             _ops.counted_begin(ctx, state.nnz, self.k)
         km = None
         if gate.n_qubits == 1:
+            self._diag_flush()
             q = gate.qubits[0]
             G = self.U[q].conj().T @ gate.matrix @ self.U[q]
             jb = int(state.pos[q])
@@ -699,13 +715,14 @@ class BASS:
             Ut = self._get_kron(q1, q2)
             G = Ut.conj().T @ gate.matrix @ Ut
             if self.use_diag_gate and _is_gate_diagonal(G):
-                _ops.diag2q_counted(ctx, state._buf, self.k, int(state.pos[q1]), int(state.pos[q2]),
-                                    np.diag(G).astype(np.complex128))
+                # deferred: consecutive diagonal gates run as one pass
+                self._drun.append((int(state.pos[q1]), int(state.pos[q2]), np.diag(G).astype(np.complex128)))
                 self._count("diag2q")
-                self._cpending.append((i, "diag2q", False))
+                self._cpending.append((i, "diag2q", (len(self._drun) - 1) % _ops.DIAG_RUN_MAX == 0))
                 state.nnz = None
                 state._sum2 = state._sum4 = None
                 return
+            self._diag_flush()
             # as _lift_2q: bring q2 to bit 0 (a move), lift q1's bit to bit 1 with the gate
             if state.pos[q1] == 0 and state.pos[q2] != 0:
                 G = G[np.ix_(_SWAP_IDX, _SWAP_IDX)]

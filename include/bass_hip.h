@@ -152,6 +152,19 @@ int bass_lift_counted(const uint64_t* x, const double* a, int64_t n_max, int nq,
 int bass_diag2q_counted(const uint64_t* x, double* a, int64_t n_max, int q1, int q2, const double* diag,
                         void* ws, size_t ws_bytes, void* stream);
 
+/* ngates diagonal 2-qubit gates applied in order in one pass over the state:
+ * for g = 0 .. ngates-1, alpha[i] *= diags[g][(b_{qubits[2g]} << 1) |
+ * b_{qubits[2g+1]}] (diags: ngates x 4 complex128, interleaved re/im), the
+ * same products in the same order as ngates calls of bass_diag2q, so the
+ * result is bit-identical, from one read and one write of the state per 32
+ * gates instead of one per gate.  counted != 0: the entry count is the
+ * workspace's state-count slot clamped to n, as in bass_diag2q_counted
+ * (ws unused otherwise).  The reference applies the gates one call each
+ * (src/core/sparse_state.py:179-190 from src/simulation/bass_simulator.py:871-880);
+ * this fuses a run of them and has no counterpart there. */
+int bass_diag2q_run(const uint64_t* x, double* a, int64_t n, int ngates, const int* qubits, const double* diags,
+                    int counted, void* ws, size_t ws_bytes, void* stream);
+
 /* Write n into the workspace's state-count slot (asynchronous). */
 int bass_set_count(void* ws, size_t ws_bytes, int64_t n, void* stream);
 

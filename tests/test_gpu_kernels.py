@@ -570,3 +570,54 @@ def test_fresh_workspace_gate_then_truncate(gpu_ctx):
         D._CONTEXTS[key] = shared
         for c, v in enumerate(saved):
             lib.bass_tune(knob, c, v)
+
+
+@pytest.mark.parametrize("n,ngates,count", [(2000, 40, None), (300001, 45, None), (70000, 56, 69877),
+                                            (5000, 1, None), (100, 0, None)])
+def test_diagonal_run(gpu_ctx, n, ngates, count):
+    """bass_diag2q_run (a run of diagonal 2-qubit gates in one pass per 32
+    gates) against one bass_diag2q launch per gate on the same state: bit for
+    bit; with the count slot (counted) only its first `count` entries change.
+    At n=2000 also against the oracle's per-gate kernel
+    (src/core/sparse_state.py:179-190) applied gate by gate."""
+    print(f"[case] diag run n={n} gates={ngates} count={count}", flush=True)
+    D = pkg_module("_device")
+    O = pkg_module("_ops")
+    ctx = gpu_ctx
+    rng = np.random.default_rng(n + ngates)
+    x, a = random_state(n, 56, seed=n)
+    gates = []
+    for _ in range(ngates):
+        q1, q2 = (int(v) for v in rng.choice(56, size=2, replace=False))
+        d = rng.standard_normal(4) + 1j * rng.standard_normal(4)
+        if rng.random() < 0.5:
+            d = np.exp(1j * rng.uniform(-np.pi, np.pi, 4))
+        gates.append((q1, q2, d))
+
+    def load():
+        b = D.Buffers(n, "cuda:0")
+        b.x[:n] = torch.from_numpy(x.view(np.int64)).to("cuda:0")
+        b.a[:n] = torch.from_numpy(a).to("cuda:0")
+        return b
+
+    m = n if count is None else count
+    seq = load()
+    for q1, q2, d in gates:
+        O.diag2q(ctx, seq, m, q1, q2, d)
+    run = load()
+    if count is None:
+        O.diag2q_run(ctx, run, n, gates)
+    else:
+        O.counted_begin(ctx, count, n)
+        O.diag2q_run(ctx, run, n, gates, counted=True)
+    ga = run.a[:n].cpu().numpy()
+    sa = seq.a[:n].cpu().numpy()
+    assert np.array_equal(ga.view(np.float64), sa.view(np.float64))
+    assert np.array_equal(ga[m:].view(np.float64), a[m:].view(np.float64))
+    if n <= 2000:
+        oa = a.copy()
+        for q1, q2, d in gates:
+            OK.apply_2qubit_diagonal_kernel(x, oa, n, q1, q2, d)
+        np.testing.assert_allclose(ga, oa, rtol=1e-12, atol=0)
+        if exact():
+            assert np.array_equal(ga.view(np.float64), oa.view(np.float64))
