@@ -24,6 +24,8 @@ MODE_GATE = 0
 MODE_ROT = 1
 MODE_MOVE = 2
 LIFT_NO_SCAN = 16  # bass_lift mode flag: no output-count scan (a segmented truncate reads the output next)
+LIFT_SPEC = 32  # bass_lift mode flag: the truncate's digit-1 pass against the predicted bucket
+TRUNC_SPEC = 8  # bass_truncate in_flags bit: the input comes from a LIFT_SPEC lift issued just before
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64

@@ -80,11 +80,13 @@ def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE, count=True):
     return ctx.read_stats("bass_lift").nnz, int(st.value)
 
 
-def lift_counted(ctx, src, n_max, j, dst, G=None, nq=1, mode=L.MODE_GATE):
+def lift_counted(ctx, src, n_max, j, dst, G=None, nq=1, mode=L.MODE_GATE, spec=False):
     """lift() whose input count is the workspace's state-count slot (<= n_max),
     read on the device: nothing waits for the device.  Gates: returns the
     output's tile count (its segmented output goes to a counted truncate);
-    a move (MODE_MOVE) copies the state densely and returns 0."""
+    a move (MODE_MOVE) copies the state densely and returns 0.  spec=True
+    (gates): also the truncate's digit-1 pass against the last select's
+    bucket; pass spec=True to the truncate_counted that follows."""
     tm = 1024 if mode == L.MODE_MOVE else SEG // (1 << nq)
     tiles = (n_max + tm - 1) // tm
     need = n_max if mode == L.MODE_MOVE else tiles * SEG
@@ -93,17 +95,18 @@ def lift_counted(ctx, src, n_max, j, dst, G=None, nq=1, mode=L.MODE_GATE):
     buf, gp = (None, None) if mode == L.MODE_MOVE else L.mat_arg(G)
     st = ctypes.c_int64(0)
     # a counted truncate reads a gate's output next: no scan of its tile counts
-    fl = mode if mode == L.MODE_MOVE else mode | L.LIFT_NO_SCAN
+    fl = mode if mode == L.MODE_MOVE else mode | L.LIFT_NO_SCAN | (L.LIFT_SPEC if spec else 0)
     rc = ctx.lib.bass_lift_counted(src.xp, src.ap, n_max, nq, int(j), gp, fl, dst.xp, dst.ap, dst.cap,
                                    ctypes.byref(st), ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_lift_counted")
     return int(st.value)
 
 
-def truncate_counted(ctx, src, k, dst, hist_tiles, key_map=None):
+def truncate_counted(ctx, src, k, dst, hist_tiles, key_map=None, spec=False):
     """truncate(..., segmented=True) in counted mode: resolves every threshold
     itself, appends its stats to the op log and sets the state-count slot;
-    nothing waits for the device."""
+    nothing waits for the device.  spec=True: src is the output of a
+    lift_counted(spec=True) issued just before."""
     ctx.ensure_ws(max(hist_tiles * SEG, 1))
     dst.ensure(k)
     if key_map is None:
@@ -111,7 +114,8 @@ def truncate_counted(ctx, src, k, dst, hist_tiles, key_map=None):
     else:
         km = np.ascontiguousarray(key_map, dtype=np.int32)
         kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
-    rc = ctx.lib.bass_truncate(src.xp, src.ap, 0, k, dst.xp, dst.ap, dst.cap, 1, hist_tiles, 1 | 4, kp, kb,
+    fl = 1 | 4 | (L.TRUNC_SPEC if spec else 0)
+    rc = ctx.lib.bass_truncate(src.xp, src.ap, 0, k, dst.xp, dst.ap, dst.cap, 1, hist_tiles, fl, kp, kb,
                                ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_truncate")
 
@@ -201,7 +205,7 @@ def truncate_finish(ctx, handle):
     if st.flags & 2:
         # very many entries tie at the threshold: repeat with the multi-block path
         a = list(args)
-        a[9] |= 2
+        a[9] = (a[9] | 2) & ~L.TRUNC_SPEC
         rc = ctx.lib.bass_truncate(*a, ctx.wsp, ctx.ws_bytes, ctx.stream)
         L.check(rc, "bass_truncate")
         st = ctx.read_stats("bass_truncate")

@@ -80,6 +80,7 @@ struct Layout {
   size_t seg_cnt, soff, cnt1, tile_dcnt, ksum, kcnt, koff, lift_b, hist2_rows, chunks;
   size_t count_slot, oplog;  // counted mode: state entry count; op log (counter + entries)
   size_t ktab;               // key-map byte tables of the select (KTAB_WORDS words)
+  size_t spec;               // digit-1 bucket of the last select: a speculative lift's prediction
   size_t big;
   size_t total;
 };
@@ -131,6 +132,7 @@ static Layout layout(int64_t S) {
   L.count_slot = o; o += al(8);
   L.oplog = o; o += al(8 + (size_t)OPLOG_CAP * sizeof(OpStats));
   L.ktab = o; o += al((size_t)KTAB_WORDS * 8);
+  L.spec = o; o += al(8);
   L.big = o;
   // big scratch union:
   //   gate group table: cap <= 4 n_in slots x <= 32 B, n_in <= S/4 (2q) or S/2 (1q) -> <= 32 S
@@ -547,9 +549,12 @@ int bass_counted_read(void* ws, size_t ws_bytes, bass_stats* out, int64_t max_en
 static int lift_impl(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G, int mode,
                      uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws, size_t ws_bytes,
                      void* stream, bool counted) {
-  // mode bit 4: no scan of the output's tile counts (a segmented truncate reads them)
+  // mode bit 4: no scan of the output's tile counts (a segmented truncate reads them);
+  // bit 5: speculative digit-1 pass of the select that reads the output next
   const bool no_scan = (mode & 16) != 0;
-  mode &= ~16;
+  const bool spec = (mode & 32) != 0;
+  mode &= ~48;
+  if (spec && mode != MODE_GATE) return fail(BASS_E_ARG, "bass_lift: the speculative pass needs a gate");
   if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_lift: nq must be 1 or 2");
   if (mode != MODE_GATE && mode != MODE_ROT && mode != MODE_MOVE) return fail(BASS_E_ARG, "bass_lift: bad mode");
   if (mode == MODE_MOVE && nq != 1) return fail(BASS_E_ARG, "bass_lift: a move lifts one bit");
@@ -589,11 +594,27 @@ static int lift_impl(const uint64_t* x, const double* a, int64_t n, int nq, int 
     else
       fill(zero.p, (size_t)zero.wp * 8, 0ull, s);
   }
+  LiftSpec sp;
+  memset(&sp, 0, sizeof(sp));
+  if (spec) {
+    // the candidate arrays where bass_truncate places them for an input of
+    // `tiles` segments
+    const int64_t slots = tiles * SEG_ENTRIES;
+    const size_t p8 = al((size_t)8 * slots), p4 = al((size_t)4 * slots);
+    if (3 * p8 + 2 * p4 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_lift: candidates do not fit");
+    char* big = at<char>(ws, L.big);
+    sp.pred = at<unsigned int>(ws, L.spec);
+    sp.out.c1p = (unsigned long long*)big;
+    sp.out.c1i = (unsigned int*)(big + p8);
+    sp.out.cnt1 = at<int>(ws, L.cnt1);
+    sp.out.tile_dsum = at<double>(ws, L.tile_dsum);
+    sp.out.tile_dcnt = at<int>(ws, L.tile_dcnt);
+  }
   if (n > 0) {
     int* seg = at<int>(ws, L.seg_cnt);
     launch_lift(nq, mode, x, (const double2*)a, n, j, gm, xo, (double2*)ao, at<LiftBound>(ws, L.lift_b),
                 at<unsigned int>(ws, L.hist1), at<double>(ws, L.tile_psum), seg, s,
-                counted ? at<int64_t>(ws, L.count_slot) : nullptr, zero);
+                counted ? at<int64_t>(ws, L.count_slot) : nullptr, zero, spec ? &sp : nullptr);
     if (mode != MODE_MOVE && !no_scan)
       launch_seg_scan(seg, tiles, at<int64_t>(ws, L.soff), seg_chunk_cnt(ws, L), seg_chunk_off(ws, L), st, s);
   }
@@ -629,6 +650,8 @@ static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t 
   const bool segmented = (in_flags & 1) != 0;
   const bool wide = (in_flags & 2) != 0;
   const bool counted = (in_flags & 4) != 0;
+  const bool spec_in = (in_flags & 8) != 0;
+  if (spec_in && !segmented) return fail(BASS_E_ARG, "bass_truncate: a speculative lift's output is segmented");
   if (k < 1 || (!segmented && n <= k))
     return fail(BASS_E_ARG, "bass_truncate: need n > k >= 1 (n=%lld k=%lld)", (long long)n, (long long)k);
   if (cap_out < k) return fail(BASS_E_OVERFLOW, "bass_truncate: cap_out < k");
@@ -694,6 +717,8 @@ static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t 
   P.zero_words = (int64_t)((L.hdrB_end - L.hist2) / 8);
   P.km = km;
   P.ktab = km.nbits ? at<uint64_t>(ws, L.ktab) : nullptr;
+  P.spec_word = at<unsigned int>(ws, L.spec);
+  P.spec_in = spec_in && !wide ? 1 : 0;
   const SelTiles T = tiles_view(ws, L, (unsigned long long*)big, (unsigned int*)(big + p8));
   const int tk = timed ? ktimer_begin("select", s) : -1;
   launch_select(P, T, hist_tiles > 0, wide, s);

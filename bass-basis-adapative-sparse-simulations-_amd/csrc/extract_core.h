@@ -1,0 +1,100 @@
+// The digit-1 pass of the top-k select over one tile of SEG_ENTRIES slots:
+// count and sum of the entries above the digit-1 bucket, the candidates
+// inside it, and (optionally) their digit-2 counts.  Shared by the select's
+// k_extract1 (reading the tile from memory) and by a speculative gate lift
+// (k_lift_gate with a predicted bucket, reading the tile it has just produced
+// from LDS), so both write the same values in the same order.
+#pragma once
+#include "common.h"
+#include "kernels.h"
+
+namespace bass {
+
+constexpr int SEL_ITEMS = SEG_ENTRIES / NT;  // 8
+constexpr int SEL_TILE = SEG_ENTRIES;
+
+// s_hist[bin] += 1 for the active lanes of the wave: the lanes sharing the
+// first remaining lane's bin are added by one atomic, for up to 4 distinct
+// bins; lanes left after that add one each.
+__device__ __forceinline__ void wave_hist_add(unsigned int* s_hist, unsigned int bin, bool act) {
+  unsigned long long am = __ballot(act);
+  const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (!am) return;  // uniform
+    const int first = __ffsll((long long)am) - 1;
+    const unsigned int b0 = (unsigned int)__shfl((int)bin, first, 64);
+    const unsigned long long same = __ballot(((am >> lane) & 1ull) && bin == b0);
+    if (lane == first) atomicAdd(&s_hist[b0], (unsigned int)__popcll(same));
+    am &= ~same;
+  }
+  if ((am >> lane) & 1ull) atomicAdd(&s_hist[bin], 1u);
+}
+
+// One tile from its |a|^2 values vp (slot it * NT + threadIdx.x holds
+// vp[it]; slots >= cnt are ignored): count / sum above bucket b1 into
+// tile_dcnt[gt] / tile_dsum[gt], candidates inside it (in slot order:
+// item-major, then thread order) at c1p / c1i[gt * SEL_TILE + ...] (|a|^2 bit
+// pattern, input slot base + slot), their count into cnt1[gt], and with HIST
+// their digit-2 counts into s_hist.  Candidate flags are per-item wave ballots
+// (scalar registers); a candidate's offset is its item's and wave's base (one
+// wave scans the (item, wave) counts) plus its rank among the wave's set lanes.
+template <bool HIST>
+__device__ __forceinline__ void ex1_tile_core(const Ex1Out& O, int64_t gt, int64_t base, int cnt,
+                                              const double (&vp)[SEL_ITEMS], unsigned int b1,
+                                              unsigned int* s_hist, int* s_wcnt, double* s_red, int* s_dcnt) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) *s_dcnt = 0;
+  __syncthreads();
+  const int64_t cbase = gt * SEL_TILE;
+  double dsum = 0.0;
+  int dcnt = 0;
+  unsigned long long m[SEL_ITEMS];
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int j = it * NT + threadIdx.x;
+    bool cand = false;
+    if (j < cnt) {
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(vp[it]);
+      const unsigned int d = (unsigned int)(bits >> 52);
+      if (d > b1) {
+        dsum += vp[it];
+        ++dcnt;
+      } else if (d == b1) {
+        cand = true;
+        if constexpr (HIST) atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
+      }
+    }
+    m[it] = __ballot(cand);
+    if (lane == 0) s_wcnt[it * WAVES + w] = __popcll(m[it]);
+  }
+  __syncthreads();
+  const int cw = lane < SEL_ITEMS * WAVES ? s_wcnt[lane] : 0;
+  const int cx = wave_incl_scan(cw) - cw;
+  const int rb = __builtin_amdgcn_readlane(cx + cw, SEL_ITEMS * WAVES - 1);
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    const int o = __builtin_amdgcn_readlane(cx, it * WAVES + wu);
+    const unsigned long long mk = m[it];
+    if ((mk >> lane) & 1ull) {
+      const int off =
+          o + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mk, 0u));
+      O.c1p[cbase + off] = (unsigned long long)__double_as_longlong(vp[it]);
+      O.c1i[cbase + off] = (unsigned int)(base + it * NT + threadIdx.x);
+    }
+  }
+  const double s = block_sum(dsum, s_red);
+  // count above the bucket: wave sums, one LDS add per wave
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) dcnt += __shfl_xor(dcnt, o, 64);
+  if (lane == 0) atomicAdd(s_dcnt, dcnt);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    O.tile_dsum[gt] = s;
+    O.tile_dcnt[gt] = *s_dcnt;
+    O.cnt1[gt] = rb;
+  }
+}
+
+}  // namespace bass

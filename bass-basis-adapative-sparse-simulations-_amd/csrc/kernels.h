@@ -42,6 +42,9 @@ struct SelState {
   unsigned int stg_abort;
   unsigned int stg_done[12];
   unsigned int stg_flag[12];
+  // the speculative lift's digit-1 pass used this select's bucket: k_extract1
+  // only builds the digit-2 rows from its candidates
+  unsigned int spec_hit;
 };
 
 // One block resolves at most FIN_MAX digit-2 candidates.  Beyond that a single
@@ -112,9 +115,27 @@ int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, int njobs, int64_
                       uint64_t* xo, double2* ao, unsigned int* hist, double* tile_psum, int* seg_cnt,
                       hipStream_t s);
 int lift_tile_entries(int qb, int mode);
+// Outputs of the select's digit-1 pass per tile (ex1_tile_core).
+struct Ex1Out {
+  unsigned long long* c1p;
+  unsigned int* c1i;
+  int* cnt1;
+  double* tile_dsum;
+  int* tile_dcnt;
+};
+
+// A gate lift that also runs the select's digit-1 pass on each output tile,
+// against the bucket *pred (the previous select's), into the select's
+// candidate arrays; the select uses them when its bucket equals *pred.
+struct LiftSpec {
+  const unsigned int* pred;
+  Ex1Out out;
+};
+
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
-                hipStream_t s, const int64_t* n_dev = nullptr, const LiftZero& zero = LiftZero());
+                hipStream_t s, const int64_t* n_dev = nullptr, const LiftZero& zero = LiftZero(),
+                const LiftSpec* spec = nullptr);
 
 // One state of a batched elementwise operation (see launch_ew_batch):
 // op 0 diagonal 2-qubit gate a *= d[(b_j1 << 1) | b_j2], op 1 a /= div,
@@ -176,7 +197,7 @@ struct SelProblem {
   int renorm;
   int chunked;           // per-tile reductions run by chunk (single large problems)
   int finish_any;        // k_finish resolves any candidate count itself (no wide-path hand-off)
-  int pad;
+  int spec_in;           // the input comes from a speculative lift (LiftSpec) launched just before
   unsigned int* hist1;   // HIST_BINS, zeroed by the caller unless filled by the producing gate
   unsigned int* hist2;   // HIST_BINS, zeroed
   unsigned int* hist_w;  // HIST_BINS, zeroed (wide path)
@@ -192,6 +213,10 @@ struct SelProblem {
   // of stored byte b = v), written by k_find1 when km.nbits > 0, or null
   // (the map is then applied bit by bit)
   uint64_t* ktab;
+  // single problems: the digit-1 bucket a speculative lift predicts (the last
+  // select's bucket, 0xFFFFFFFF after a keep-all select); k_find1 compares
+  // and updates it.  Null: no speculation.
+  unsigned int* spec_word;
 };
 constexpr int KTAB_WORDS = 8 * 256;
 

@@ -34,6 +34,7 @@
 // input is sorted); rotations sum every member (absent = 0) and drop outputs
 // with |a| <= 1e-30.
 #include "common.h"
+#include "extract_core.h"
 #include "kernels.h"
 
 namespace bass {
@@ -474,13 +475,16 @@ __device__ __forceinline__ bool lift_group_gather(int m, int cnt, const uint64_t
 
 // One tile of a gate: local tile `tile` of the input (merged positions
 // [tile*TM, ...)), outputs to segment `otile` of (xo, ao), seg_cnt[otile],
-// tile_psum[otile]; digit-1 histogram into hist.
-template <int QB, int MODE>
+// tile_psum[otile]; digit-1 histogram into hist.  SPEC: also the select's
+// digit-1 pass on the tile (ex1_tile_core, bucket *S.pred) from its |a|^2
+// values staged in LDS in slot order.
+template <int QB, int MODE, bool SPEC = false>
 __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a,
                                                int64_t n, int j, const LiftBound& B0, const LiftBound& B1,
                                                const GateMat& G, int64_t tile, int64_t otile,
                                                uint64_t* __restrict__ xo, double2* __restrict__ ao,
-                                               unsigned int* hist, double* tile_psum, int* seg_cnt) {
+                                               unsigned int* hist, double* tile_psum, int* seg_cnt,
+                                               LiftSpec S = LiftSpec()) {
   constexpr int M = 1 << QB;
   constexpr int T = QB - 1;
   constexpr int TM = SEG_ENTRIES / M;
@@ -549,6 +553,13 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
   if (threadIdx.x == 0) seg_cnt[otile] = tile_total;
 
   double psum = 0.0;
+  // SPEC: |a|^2 of the outputs, staged in s_a (free: every gather ended
+  // before the offset scan's barrier) in slot order, CAP slots per pass; one
+  // pass (1 qubit) stages them here, two (2 qubits) keep them in pv
+  constexpr int CAP = 2 * TM;
+  constexpr bool ONE_PASS = CAP == SEG_ENTRIES;
+  double* s_p = reinterpret_cast<double*>(s_a);
+  double pv[ITEMS][M];
   const int64_t rb = otile * SEG_ENTRIES;
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
@@ -560,6 +571,8 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
         xo[pos] = okey[it][o];
         ao[pos] = v;
         const double p = cabs2(v);
+        if constexpr (SPEC && ONE_PASS) s_p[pos - rb] = p;
+        if constexpr (SPEC && !ONE_PASS) pv[it][o] = p;
         psum += p;
         const unsigned int bin = (unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1);
         if (bin < LBINS)
@@ -575,20 +588,54 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
   __syncthreads();
   for (int b = threadIdx.x; b < LBINS; b += NT)
     if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
+  if constexpr (SPEC) {
+    // the staged |a|^2 read back in the select's slot-to-thread order
+    __shared__ int s_wcnt[SEL_ITEMS * WAVES];
+    __shared__ int s_dcnt;
+    static_assert(SEG_ENTRIES % CAP == 0 && CAP % NT == 0, "passes cover whole items");
+    const unsigned int b1 = *S.pred;
+    double vp[SEL_ITEMS];
+#pragma unroll
+    for (int pass = 0; pass < SEG_ENTRIES / CAP; ++pass) {
+      const int lo = pass * CAP;
+      if constexpr (!ONE_PASS) {
+#pragma unroll
+        for (int it = 0; it < ITEMS; ++it) {
+          int pos = ooff[it];
+#pragma unroll
+          for (int o = 0; o < M; ++o) {
+            if ((omask[it] >> o) & 1u) {
+              if (pos >= lo && pos < lo + CAP) s_p[pos - lo] = pv[it][o];
+              ++pos;
+            }
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int it = lo / NT; it < (lo + CAP) / NT; ++it) {
+        const int jj = it * NT + threadIdx.x;
+        vp[it] = jj < tile_total ? s_p[jj - lo] : 0.0;
+      }
+      __syncthreads();
+    }
+    ex1_tile_core<false>(S.out, otile, rb, tile_total, vp, b1, nullptr, s_wcnt, s_red, &s_dcnt);
+  }
 }
 
-template <int QB, int MODE>
-__global__ __launch_bounds__(NT) void k_lift_gate(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+// (the speculative variant is held to the plain kernel's 5 waves per SIMD)
+template <int QB, int MODE, bool SPEC>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPEC ? 5 : 1))) void k_lift_gate(const uint64_t* __restrict__ w, const double2* __restrict__ a,
                                                    int64_t n, int j, const LiftBound* __restrict__ bounds,
                                                    GateMat G, uint64_t* __restrict__ xo, double2* __restrict__ ao,
                                                    unsigned int* hist, double* tile_psum, int* seg_cnt,
-                                                   const int64_t* __restrict__ n_dev) {
+                                                   const int64_t* __restrict__ n_dev, LiftSpec S) {
   // tiles past a counted input's end have empty bounds: they write a zero
   // count and sum
   n = dev_count(n_dev, n);
   const int64_t tile = blockIdx.x;
-  lift_gate_tile<QB, MODE>(w, a, n, j, bounds[tile], bounds[tile + 1], G, tile, tile, xo, ao, hist, tile_psum,
-                           seg_cnt);
+  lift_gate_tile<QB, MODE, SPEC>(w, a, n, j, bounds[tile], bounds[tile + 1], G, tile, tile, xo, ao, hist,
+                                 tile_psum, seg_cnt, S);
 }
 
 // ------------------------------------------------- two-qubit RDM ------
@@ -1113,7 +1160,7 @@ int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, int njobs, int64_
 
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
-                hipStream_t s, const int64_t* n_dev, const LiftZero& zero) {
+                hipStream_t s, const int64_t* n_dev, const LiftZero& zero, const LiftSpec* spec) {
   if (n <= 0) return 0;
   const int tm = lift_tile_entries(qb, mode);
   const int t = (mode == MODE_MOVE) ? 0 : qb - 1;
@@ -1124,20 +1171,27 @@ int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n
   const dim3 grid((unsigned)tiles);
   if (mode == MODE_MOVE) {
     hipLaunchKernelGGL(k_lift_move<1024>, grid, dim3(NT), 0, s, x, a, n, j, bounds, xo, ao, n_dev);
-  } else if (qb == 1) {
-    if (mode == MODE_GATE)
-      hipLaunchKernelGGL((k_lift_gate<1, MODE_GATE>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
-                         tile_psum, seg_cnt, n_dev);
-    else
-      hipLaunchKernelGGL((k_lift_gate<1, MODE_ROT>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
-                         tile_psum, seg_cnt, n_dev);
   } else {
-    if (mode == MODE_GATE)
-      hipLaunchKernelGGL((k_lift_gate<2, MODE_GATE>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
-                         tile_psum, seg_cnt, n_dev);
-    else
-      hipLaunchKernelGGL((k_lift_gate<2, MODE_ROT>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist,
-                         tile_psum, seg_cnt, n_dev);
+    const LiftSpec S = spec ? *spec : LiftSpec();
+#define BASS_LG(QB_, MODE_, SPEC_)                                                                          \
+  hipLaunchKernelGGL((k_lift_gate<QB_, MODE_, SPEC_>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist, \
+                     tile_psum, seg_cnt, n_dev, S)
+    if (qb == 1) {
+      if (mode == MODE_GATE && spec)
+        BASS_LG(1, MODE_GATE, true);
+      else if (mode == MODE_GATE)
+        BASS_LG(1, MODE_GATE, false);
+      else
+        BASS_LG(1, MODE_ROT, false);
+    } else {
+      if (mode == MODE_GATE && spec)
+        BASS_LG(2, MODE_GATE, true);
+      else if (mode == MODE_GATE)
+        BASS_LG(2, MODE_GATE, false);
+      else
+        BASS_LG(2, MODE_ROT, false);
+    }
+#undef BASS_LG
   }
   return (int)hipGetLastError();
 }

@@ -29,12 +29,11 @@
 #include <string.h>
 
 #include "common.h"
+#include "extract_core.h"
 #include "kernels.h"
 
 namespace bass {
 
-constexpr int SEL_ITEMS = SEG_ENTRIES / NT;  // 8
-constexpr int SEL_TILE = SEG_ENTRIES;
 
 __device__ __forceinline__ unsigned long long pbits_of(double2 v) {
   return (unsigned long long)__double_as_longlong(cabs2(v));
@@ -291,6 +290,16 @@ __global__ __launch_bounds__(NTB) void k_find1(const SelProblem* probs, SelProbl
       sel->prefix = (unsigned long long)s_b << 52;
       st->flags = 1;
     }
+    if (P.spec_word) {
+      // a speculative lift ran the digit-1 pass against *spec_word: usable
+      // when that was this bucket (stats flags bit 2); the bucket is the next
+      // lift's prediction
+      if (P.spec_in && !keepall && *P.spec_word == s_b) {
+        sel->spec_hit = 1;
+        st->flags |= 4;
+      }
+      *P.spec_word = keepall ? 0xFFFFFFFFu : s_b;
+    }
   }
 }
 
@@ -308,68 +317,13 @@ __device__ __forceinline__ void ex1_load(const SelProblem& P, int64_t gt, double
   }
 }
 
-// One tile of k_extract1 from its |a|^2 values vp: count / sum above the
-// digit-1 bucket b1, candidates inside it (in input order: item-major, then
-// thread order) and their digit-2 counts into s_hist.  Candidate flags are
-// per-item wave ballots (scalar registers); a candidate's offset is its
-// item's and wave's base (one wave scans the (item, wave) counts) plus its
-// rank among the wave's set lanes.
+// One tile of k_extract1 from its |a|^2 values vp (ex1_tile_core with the
+// digit-2 counts).
 __device__ __forceinline__ void ex1_tile(const SelProblem& P, const SelTiles& T, int64_t gt, int cnt,
                                          const double (&vp)[SEL_ITEMS], unsigned int b1, unsigned int* s_hist,
                                          int* s_wcnt, double* s_red, int* s_dcnt) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (threadIdx.x == 0) *s_dcnt = 0;
-  __syncthreads();
-  const int64_t base = tile_base(P, gt);
-  const int64_t cbase = gt * SEL_TILE;
-  double dsum = 0.0;
-  int dcnt = 0;
-  unsigned long long m[SEL_ITEMS];
-#pragma unroll
-  for (int it = 0; it < SEL_ITEMS; ++it) {
-    const int j = it * NT + threadIdx.x;
-    bool cand = false;
-    if (j < cnt) {
-      const unsigned long long bits = (unsigned long long)__double_as_longlong(vp[it]);
-      const unsigned int d = (unsigned int)(bits >> 52);
-      if (d > b1) {
-        dsum += vp[it];
-        ++dcnt;
-      } else if (d == b1) {
-        cand = true;
-        atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
-      }
-    }
-    m[it] = __ballot(cand);
-    if (lane == 0) s_wcnt[it * WAVES + w] = __popcll(m[it]);
-  }
-  __syncthreads();
-  const int cw = lane < SEL_ITEMS * WAVES ? s_wcnt[lane] : 0;
-  const int cx = wave_incl_scan(cw) - cw;
-  const int rb = __builtin_amdgcn_readlane(cx + cw, SEL_ITEMS * WAVES - 1);
-  const int wu = __builtin_amdgcn_readfirstlane(w);
-#pragma unroll
-  for (int it = 0; it < SEL_ITEMS; ++it) {
-    const int o = __builtin_amdgcn_readlane(cx, it * WAVES + wu);
-    const unsigned long long mk = m[it];
-    if ((mk >> lane) & 1ull) {
-      const int off =
-          o + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mk, 0u));
-      T.c1p[cbase + off] = (unsigned long long)__double_as_longlong(vp[it]);
-      T.c1i[cbase + off] = (unsigned int)(base + it * NT + threadIdx.x);
-    }
-  }
-  const double s = block_sum(dsum, s_red);
-  // count above the bucket: wave sums, one LDS add per wave
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) dcnt += __shfl_xor(dcnt, o, 64);
-  if (lane == 0) atomicAdd(s_dcnt, dcnt);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    T.tile_dsum[gt] = s;
-    T.tile_dcnt[gt] = *s_dcnt;
-    T.cnt1[gt] = rb;
-  }
+  const Ex1Out O{T.c1p, T.c1i, T.cnt1, T.tile_dsum, T.tile_dcnt};
+  ex1_tile_core<true>(O, gt, tile_base(P, gt), cnt, vp, b1, s_hist, s_wcnt, s_red, s_dcnt);
 }
 
 // Per tile (EXT consecutive tiles per block, all of one problem; EXT = 0 takes
@@ -401,6 +355,50 @@ __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int np
     return;
   }
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  if (!BATCH && sel->spec_hit) {
+    // the lift already wrote the per-tile counts, sums and candidates: only
+    // the row's digit-2 counts, from the candidates' |a|^2 patterns.  The
+    // candidates of the block's tiles are walked as one range (tile starts
+    // from a scan of their counts), 4 independent loads per thread at a time.
+    static_assert(EX1_TILES_MAX < 64, "one wave scans the tile counts");
+    __shared__ int s_pre[EX1_TILES_MAX + 2];
+    if (threadIdx.x < 64) {
+      const int64_t gt = gt0 + threadIdx.x;
+      const int c = ((int)threadIdx.x < ext && gt < tiles_end) ? T.cnt1[gt] : 0;
+      const int inc = wave_incl_scan(c);
+      if (threadIdx.x == 0) s_pre[0] = 0;
+      if ((int)threadIdx.x < ext) s_pre[threadIdx.x + 1] = inc;
+    }
+    __syncthreads();
+    const int total = s_pre[ext];
+    for (int f0 = 0; f0 < total; f0 += 4 * NT) {
+      unsigned long long pb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int f = f0 + u * NT + threadIdx.x;
+        pb[u] = 0;
+        if (f < total) {
+          int lo = 0, hi = ext - 1;  // the last tile t with s_pre[t] <= f
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_pre[mid] <= f)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+          pb[u] = T.c1p[(gt0 + lo) * SEL_TILE + (f - s_pre[lo])];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (f0 + u * NT + (int)threadIdx.x < total)
+          atomicAdd(&s_hist[(unsigned int)(pb[u] >> 41) & (HIST_BINS - 1)], 1u);
+    }
+    __syncthreads();
+    unsigned short* row = T.hist2_rows + (int64_t)blockIdx.x * HIST_BINS;
+    for (int b = threadIdx.x; b < HIST_BINS; b += NT) row[b] = (unsigned short)s_hist[b];
+    return;
+  }
   const unsigned int b1 = sel->b1;
   // the next tile's amplitudes are loaded before the current tile is
   // processed, so one tile's loads are always in flight
@@ -776,24 +774,6 @@ __device__ __forceinline__ void stg_key_digit(int kb, int lv, int& lo, int& hi) 
   if (hi < 0) hi = 0;
 }
 
-// s_hist[bin] += 1 for the active lanes of the wave: the lanes sharing the
-// first remaining lane's bin are added by one atomic, for up to 4 distinct
-// bins; lanes left after that add one each.
-__device__ __forceinline__ void stg_hist_add(unsigned int* s_hist, unsigned int bin, bool act) {
-  unsigned long long am = __ballot(act);
-  const int lane = (int)(threadIdx.x & 63);
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (!am) return;  // uniform
-    const int first = __ffsll((long long)am) - 1;
-    const unsigned int b0 = (unsigned int)__shfl((int)bin, first, 64);
-    const unsigned long long same = __ballot(((am >> lane) & 1ull) && bin == b0);
-    if (lane == first) atomicAdd(&s_hist[b0], (unsigned int)__popcll(same));
-    am &= ~same;
-  }
-  if ((am >> lane) & 1ull) atomicAdd(&s_hist[bin], 1u);
-}
-
 __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
   __shared__ FinLds S;
   __shared__ uint64_t s_ktab[KTAB_WORDS];
@@ -867,7 +847,7 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
 #pragma unroll
         for (int u = 0; u < STG_U; ++u) {
           const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
-          stg_hist_add(S.hist, (unsigned int)(bv[u] >> sh) & mask, j < m2 && (bv[u] >> top) == hi);
+          wave_hist_add(S.hist, (unsigned int)(bv[u] >> sh) & mask, j < m2 && (bv[u] >> top) == hi);
         }
       }
     } else {
@@ -881,7 +861,7 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
 #pragma unroll
         for (int u = 0; u < STG_U; ++u) {
           const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
-          stg_hist_add(S.hist, (unsigned int)(kv[u] >> sh) & mask,
+          wave_hist_add(S.hist, (unsigned int)(kv[u] >> sh) & mask,
                        j < tcount && (top >= 64 || (kv[u] >> top) == (kp >> top)));
         }
       }
@@ -1300,7 +1280,7 @@ __global__ __launch_bounds__(NTB) void k_sel_final(SelProblem P, SelTiles T, int
 
 template <bool BATCH>
 __global__ __launch_bounds__(NT) void k_compact(const SelProblem* probs, int nprob, SelProblem one, SelTiles T) {
-  __shared__ int s_scan[WAVES];
+  __shared__ int s_scan[SEL_ITEMS * WAVES];
   __shared__ double s_red[WAVES];
   const int64_t gt = blockIdx.x;
   const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt);
@@ -1322,16 +1302,30 @@ __global__ __launch_bounds__(NT) void k_compact(const SelProblem* probs, int npr
       keep[it] = kept_pred(pbits_of(vv[it]), kk[it], pstar, keystar, P);
     }
   }
-  int64_t rb = T.koff[gt] + (P.chunked ? T.ch_off[(gt - P.tile0) / SEL_CHUNK] : 0);
+  const int64_t rb = T.koff[gt] + (P.chunked ? T.ch_off[(gt - P.tile0) / SEL_CHUNK] : 0);
+  // kept-entry offsets in slot (item-major) order: per (item, wave) ballot
+  // counts, one wave scans them, a lane adds its rank among the set lanes
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long m[SEL_ITEMS];
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    m[it] = __ballot(keep[it]);
+    if (lane == 0) s_scan[it * WAVES + w] = __popcll(m[it]);
+  }
+  __syncthreads();
+  const int cw = lane < SEL_ITEMS * WAVES ? s_scan[lane] : 0;
+  const int cx = wave_incl_scan(cw) - cw;
+  const int wu = __builtin_amdgcn_readfirstlane(w);
   double s2 = 0.0, s4 = 0.0;
   bool fits = true;
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
-    int tot;
-    const int off = block_excl_scan(keep[it] ? 1 : 0, tot, s_scan);
+    const int o = __builtin_amdgcn_readlane(cx, it * WAVES + wu);
     if (keep[it]) {
       const double2 v = make_double2(vv[it].x * inv, vv[it].y * inv);
-      const int64_t pos = rb + off;
+      const unsigned long long mk = m[it];
+      const int64_t pos =
+          rb + o + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mk, 0u));
       if (pos < P.cap_out) {
         P.xo[pos] = kk[it];
         P.ao[pos] = v;
@@ -1342,7 +1336,6 @@ __global__ __launch_bounds__(NT) void k_compact(const SelProblem* probs, int npr
       s2 += p;
       s4 += p * p;
     }
-    rb += tot;
   }
   if (!fits) set_err(&P.st->err, ERR_OVERFLOW);
   const double t2 = block_sum(s2, s_red);

@@ -314,6 +314,10 @@ class BASS:
         self.device_counts = bool(device_counts)
         self._cpending = []
         self._cstart_n = 0
+        # counted gates: each gate's lift also runs the truncation's digit-1
+        # pass against the previous truncation's |a|^2 exponent bucket, which
+        # the truncation then skips when the bucket is unchanged (same results)
+        self.speculate = True
         # counted diagonal 2-qubit gates not yet launched: (bit1, bit2, diag);
         # a run of them goes to the device as one pass (bass_diag2q_run) when
         # the next other operation or a read of the state comes
@@ -373,11 +377,15 @@ class BASS:
             self._since_trunc += 1
             if r.flags & 1:
                 self._trunc_active = True
-                self._bytes("select", n, self.k)
+                # with the lift's digit-1 pass used, the select reads the
+                # amplitudes once (compaction) instead of twice
+                self._bytes("select_lifted" if r.flags & 4 else "select", n, self.k)
                 total, kept = r.total, r.kept
                 self._gamma_running *= kept / total if total > 1e-30 else 0
                 self._since_trunc = 0
                 self._count("truncate")
+                if r.flags & 4:  # the lift's digit-1 pass was used
+                    self._count("spec_hit")
             else:
                 self._count("bytes_select", 48 * n)
             n_prev = int(r.nnz)
@@ -387,9 +395,12 @@ class BASS:
 
     def _bytes(self, op, n_in, n_out):
         """Algorithmic HBM bytes of one device operation (24 B per key + complex128 record):
-        gate / rotation 24 (n_in + n_out); top-k select 40 n_in + 24 k; diagonal 40 n."""
+        gate / rotation 24 (n_in + n_out); top-k select 40 n_in + 24 k (24 n_in + 24 k
+        when the producing lift ran its digit-1 pass); diagonal 40 n."""
         if op == "select":
             b = 40 * n_in + 24 * n_out
+        elif op == "select_lifted":
+            op, b = "select", 24 * n_in + 24 * n_out
         elif op == "diag2q":
             b = 40 * n_in
         else:
@@ -706,7 +717,7 @@ class BASS:
             q = gate.qubits[0]
             G = self.U[q].conj().T @ gate.matrix @ self.U[q]
             jb = int(state.pos[q])
-            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=1)
+            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=1, spec=self.speculate)
             state.pos = lifted_positions(state.pos, jb, 0)
             self._count("gate1q")
             entry = (i, "gate1q", False)
@@ -734,12 +745,12 @@ class BASS:
                 state._buf, state._tmp = state._tmp, state._buf
                 state.pos = lifted_positions(state.pos, jb, 0)
             jb = int(state.pos[q1])
-            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=2)
+            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=2, spec=self.speculate)
             state.pos = lifted_positions(state.pos, jb, 1)
             self._count("gate2q")
             entry = (i, "gate2q", moved)
         km = _ops.key_map_arg(state.pos)
-        _ops.truncate_counted(ctx, state._tmp, self.k, state._buf, ht, key_map=km)
+        _ops.truncate_counted(ctx, state._tmp, self.k, state._buf, ht, key_map=km, spec=self.speculate)
         self._cpending.append(entry)
         state.nnz = None  # known at the next flush
         state._sum2 = state._sum4 = None

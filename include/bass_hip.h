@@ -129,6 +129,13 @@ int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t
  * mode | 16 (gate or rotation): skip the closing scan of the per-tile output
  * counts, so stats.nnz and the segment offsets bass_densify needs are not
  * produced; for an output that a bass_truncate(in_flags bit 0) reads next.
+ * mode | 32 (gate only): also run the truncate's digit-1 pass on every output
+ * tile (count and sum above one IEEE-exponent bucket of |a|^2, the candidates
+ * inside it) against the bucket of the workspace's last select, into that
+ * truncate's candidate arrays; a bass_truncate with in_flags bit 3 that reads
+ * this output next, on the same workspace, skips its own pass over the
+ * amplitudes when its bucket is the predicted one.  Results are the same
+ * either way (the pass writes what the truncate's would).
  * Returns BASS_E_OVERFLOW if cap_out is below the output size. */
 int bass_lift(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G,
               int mode, uint64_t* xo, double* ao, int64_t cap_out, int64_t* seg_tiles, void* ws,
@@ -194,6 +201,10 @@ int bass_counted_read(void* ws, size_t ws_bytes, bass_stats* out, int64_t max_en
  * in_flags bit 2 (counted): then append the stats block to the workspace's op
  * log and set its state-count slot to the output count, for
  * bass_lift_counted and bass_counted_read.
+ * in_flags bit 3 (with bit 0): (x, a) is the output of a bass_lift(mode | 32)
+ * issued just before on this workspace; its digit-1 pass is used when it
+ * predicted this call's bucket (stats.flags bit 2 set).  Every call records
+ * its bucket as the next lift's prediction.
  * key_map (key_bits entries, or NULL): stored bit p of x holds qubit
  * key_map[p]; ties are then ordered by the key mapped back. */
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo,

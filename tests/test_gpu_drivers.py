@@ -199,7 +199,10 @@ def test_tfim_config0_lockstep(gpu_ctx):
 def test_device_counts_match_host_counts(gpu_ctx, idx, K):
     """Counted 1-qubit gates (input count kept on the device, op log read at
     the driver's decision points) and the per-gate host round trip produce the
-    same state bit for bit, the same frame, gamma and counters."""
+    same state bit for bit, the same frame, gamma and operation counters.  The
+    byte counters of selects and diagonal gates follow the passes each mode
+    makes (the counted mode fuses runs of diagonal gates and lets the lift run
+    the select's digit-1 pass), so they may only be lower there."""
     name, N, circ, ends = _circuits()[idx]
     print(f"[case] device counts {name} N={N} K={K}", flush=True)
     B = pkg_module("simulation.bass_simulator")
@@ -216,5 +219,9 @@ def test_device_counts_match_host_counts(gpu_ctx, idx, K):
     (x0, a0, g0, u0, p0, c0), (x1, a1, g1, u1, p1, c1) = res
     assert np.array_equal(x0, x1) and np.array_equal(a0.view(np.float64), a1.view(np.float64))
     assert g0 == g1 and np.array_equal(p0, p1) and all(np.array_equal(a, b) for a, b in zip(u0, u1))
-    assert c0 == c1, (c0, c1)
+    by_pass = {"bytes_select", "bytes_diag2q", "spec_hit"}
+    assert {k: v for k, v in c0.items() if k not in by_pass} == {k: v for k, v in c1.items() if k not in by_pass}
+    for key in ("bytes_select", "bytes_diag2q"):
+        assert c1.get(key, 0) <= c0.get(key, 0), key
+    assert c0.get("spec_hit", 0) == 0
     assert c1.get("truncate", 0) > 0 and c1.get("gate1q", 0) > 0

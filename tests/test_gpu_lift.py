@@ -360,3 +360,89 @@ def test_truncate_counted_massive_ties(gpu_ctx, n, k, ordered):
     assert abs(st.kept - kept) <= 1e-12 * kept
     assert set(gx.tolist()) == set(kx.tolist())
     assert_same_entries(gx, ga, kx, ka / np.sqrt(kept), rtol=1e-12)
+
+
+def _counted_pair(ops, dev, ctx, w, aw, n, j, G, nq, k, km, spec):
+    """One counted lift -> truncate of (w, aw); returns (words, amps, stats)."""
+    src = _upload(dev, ctx, w, aw)
+    seg = dev.Buffers(1, ctx.device)
+    dst = dev.Buffers(k, ctx.device)
+    ops.counted_begin(ctx, n, n)
+    tiles = ops.lift_counted(ctx, src, n, j, seg, G=G, nq=nq, spec=spec)
+    ops.truncate_counted(ctx, seg, k, dst, tiles, key_map=km, spec=spec)
+    log = ops.counted_read(ctx, 4)
+    assert len(log) == 1 and log[0].err == 0
+    m = int(log[0].nnz)
+    return dev.to_host_keys(dst.x, m).copy(), dev.to_host_amps(dst.a, m).copy(), log[0]
+
+
+@pytest.mark.parametrize("n,nq,k", [(70000, 1, 50000), (300000, 1, 123457), (60000, 2, 90001), (5000, 2, 30000)])
+def test_speculative_digit1_pass(gpu_ctx, n, nq, k):
+    """A gate lift that runs the truncate's digit-1 pass against the last
+    select's bucket (bass_lift mode | 32, bass_truncate in_flags bit 3) gives
+    the plain pair's output and stats bit for bit: when the prediction holds
+    (the same input again: stats flags bit 2 set), when it does not (a select
+    on the state scaled by 2^-8 first: bit 2 clear), and when the truncate
+    keeps everything (k >= output count)."""
+    ops, dev, SS, L = _mods()
+    print(f"[case] speculative digit-1 pass n={n} nq={nq} k={k}", flush=True)
+    nbits = 36
+    rng = np.random.default_rng(n + nq)
+    x, a = random_state(n, nbits, seed=n + 3)
+    pos = rng.permutation(nbits)
+    w, aw = _ordered(x, a, pos)
+    G = random_unitary(1 << nq, rng)
+    if nq == 1:
+        j = int(pos[int(rng.integers(nbits))])
+        npos = SS.lifted_positions(pos, j, 0)
+    else:
+        # the 2-qubit lift needs its second qubit at bit 0
+        q2 = int(np.flatnonzero(pos == 0)[0])
+        q1 = int(rng.choice([q for q in range(nbits) if q != q2]))
+        j = int(pos[q1])
+        npos = SS.lifted_positions(pos, j, 1)
+    km = ops.key_map_arg(npos)
+    ref = _counted_pair(ops, dev, gpu_ctx, w, aw, n, j, G, nq, k, km, spec=False)
+    # prediction = the plain pair's bucket
+    hit = _counted_pair(ops, dev, gpu_ctx, w, aw, n, j, G, nq, k, km, spec=True)
+    # prediction from a state whose |a|^2 sit 16 exponents lower
+    _counted_pair(ops, dev, gpu_ctx, w, aw * 2.0 ** -8, n, j, G, nq, k, km, spec=False)
+    miss = _counted_pair(ops, dev, gpu_ctx, w, aw, n, j, G, nq, k, km, spec=True)
+    keepall = (ref[2].flags & 1) == 0
+    assert (hit[2].flags & 4) == (0 if keepall else 4)
+    assert (miss[2].flags & 4) == 0
+    for got in (hit, miss):
+        assert np.array_equal(got[0], ref[0])
+        assert np.array_equal(got[1].view(np.float64), ref[1].view(np.float64))
+        for f in ("nnz", "n_in", "total", "kept", "norm", "sum2", "sum4"):
+            assert getattr(got[2], f) == getattr(ref[2], f), f
+        assert (got[2].flags & 3) == (ref[2].flags & 3)
+
+
+@pytest.mark.parametrize("family", ["kicked", "brickwork"])
+def test_driver_speculation_changes_nothing(gpu_ctx, family):
+    """The counted driver with and without the speculative digit-1 pass:
+    identical keys, amplitudes, gamma and frames, and the speculation is used
+    on most truncations."""
+    print(f"[case] driver speculation on/off: {family}", flush=True)
+    rc = pkg_module("utils.random_circuits")
+    B = pkg_module("simulation.bass_simulator")
+    if family == "kicked":
+        N, K = 18, 1 << 14
+        circ, ends = rc.kicked_ising_circuit(N, 4)
+    else:
+        N, K = 16, 1 << 11
+        circ, ends = rc.brickwork_haar_circuit(N, 8, np.random.default_rng(77))
+    out = []
+    for spec in (False, True):
+        sim = B.BASS(N, K, device="cuda:0", optimize_at=ends, accept_rtol=1e-10)
+        sim.speculate = spec
+        st = sim.simulate(circ)
+        out.append((st.keys().copy(), st.alpha[: st.nnz].copy(), st.gamma, [u.copy() for u in sim.U],
+                    dict(sim.counters)))
+    (k0, a0, g0, u0, c0), (k1, a1, g1, u1, c1) = out
+    assert np.array_equal(k0, k1) and np.array_equal(a0.view(np.float64), a1.view(np.float64))
+    assert g0 == g1 and all(np.array_equal(p, q) for p, q in zip(u0, u1))
+    assert c0.get("spec_hit", 0) == 0
+    print(f"  truncations {c1.get('truncate', 0)}, speculation used {c1.get('spec_hit', 0)}", flush=True)
+    assert c1.get("spec_hit", 0) >= c1.get("truncate", 0) // 2
