@@ -51,13 +51,18 @@ __device__ __forceinline__ uint64_t hslot(uint64_t key, int shift) {
 __device__ __forceinline__ void set_err(int* err, int bit) { atomicOr(err, bit); }
 
 // ---------------------------------------------------------------- scans ----
+// Inclusive scan over the 64 lanes of a wave (all lanes active) with DPP
+// moves: shifts by 1, 2, 4, 8 inside each 16-lane row, then row 0's and
+// row 1's last lanes broadcast into the rows after them (row_bcast:15 into
+// rows 1 and 3, row_bcast:31 into rows 2 and 3).  Lanes a move does not
+// reach add 0.  No LDS traffic, unlike lane shuffles.
 __device__ __forceinline__ int wave_incl_scan(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    int t = __shfl_up(v, d, 64);
-    if (lane >= d) v += t;
-  }
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return v;
 }
 
