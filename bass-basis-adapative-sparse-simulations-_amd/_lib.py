@@ -56,8 +56,12 @@ class BassStats(ctypes.Structure):
 # filled with vector assignments
 JOB_DTYPE = np.dtype([("x", "<u8"), ("a", "<u8"), ("xo", "<u8"), ("ao", "<u8"), ("n", "<i8"), ("k", "<i8"),
                       ("cap_out", "<i8"), ("j", "<i4"), ("j2", "<i4"), ("renorm", "<i4"), ("key_bits", "<i4"),
-                      ("g", "<f8", (32,)), ("key_map", "<i4", (64,)), ("scale", "<f8")])
-assert JOB_DTYPE.itemsize == 592
+                      ("g", "<f8", (32,)), ("key_map", "<i4", (64,)), ("scale", "<f8"),
+                      ("cnt_in", "<u8"), ("cnt_out", "<u8")])
+assert JOB_DTYPE.itemsize == 608
+
+# bass_bplan_run operations
+BOP_GATE1, BOP_GATE2, BOP_ROT1, BOP_ROT2, BOP_MOVE, BOP_DIAG2Q, BOP_SCALE, BOP_MOMENTS, BOP_RDM1Q = range(9)
 
 # name -> (restype, argtypes); mirrors include/bass_hip.h
 SIGNATURES = {
@@ -101,6 +105,10 @@ SIGNATURES = {
     "bass_batch_moments": (_I, [_P, _I, _P, _SZ, _P]),
     "bass_batch_rdm1q": (_I, [_P, _I, _P, _P, _SZ, _P]),
     "bass_batch_stats": (_I, [_P, _SZ, _I, _P, _P]),
+    "bass_bplan_ws_bytes": (_SZ, [_I, _I64]),
+    "bass_bplan_create": (_I, [_I, _I64, _P, _P, _P, _SZ, _I, ctypes.POINTER(_P)]),
+    "bass_bplan_destroy": (_I, [_P]),
+    "bass_bplan_run": (_I, [_P, _I, _P, _I, _P, _P]),
     "bass_timing": (_I, [_I]),
     "bass_timing_read": (_I, [ctypes.c_char_p, ctypes.POINTER(_I64), ctypes.POINTER(_D)]),
 }

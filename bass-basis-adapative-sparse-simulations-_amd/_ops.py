@@ -343,17 +343,7 @@ def batch_step(ctx, jobs, nq, mode, tiles):
     rc = ctx.lib.bass_batch_step(jp, len(jobs), nq, mode, 0, tiles.xp, tiles.ap, tiles.cap, ctx.wsp, ctx.ws_bytes,
                                  ctx.stream)
     L.check(rc, "bass_batch_step")
-    st = batch_stats(ctx, len(jobs))
-    wide = np.flatnonzero(st["flags"] & 2)
-    if len(wide):
-        # very many entries tie at some thresholds: those jobs' states are
-        # untouched (nothing was kept); repeat them with the multi-block path
-        sub = np.ascontiguousarray(jobs[wide])
-        rc = ctx.lib.bass_batch_step(sub.ctypes.data_as(ctypes.c_void_p), len(sub), nq, mode, 1, tiles.xp, tiles.ap,
-                                     tiles.cap, ctx.wsp, ctx.ws_bytes, ctx.stream)
-        L.check(rc, "bass_batch_step")
-        st[wide] = batch_stats(ctx, len(sub))
-    return st
+    return batch_stats(ctx, len(jobs))
 
 
 def batch_move(ctx, jobs):
@@ -397,3 +387,57 @@ def batch_rdm1q(ctx, jobs):
     rho[:, :, 0, 1] = out[:, :, 2] + 1j * out[:, :, 3]
     rho[:, :, 1, 0] = out[:, :, 2] - 1j * out[:, :, 3]
     return rho
+
+
+class BatchPlan:
+    """A batch plan (bass_bplan_*): its own workspace and gate-output tiles
+    for up to jobs_cap states and tiles_cap tiles, fixed launch shapes, and
+    with graphs=True each operation's launch sequence captured once in a HIP
+    graph.  run() is asynchronous; its results land in the pinned host array
+    `out` once the stream has passed the call."""
+
+    def __init__(self, ctx, jobs_cap, tiles_cap, graphs=True):
+        from ._device import Buffers
+
+        self.ctx = ctx
+        self.jobs_cap = int(jobs_cap)
+        self.tiles_cap = int(tiles_cap)
+        nb = int(ctx.lib.bass_bplan_ws_bytes(self.jobs_cap, self.tiles_cap))
+        if nb <= 0:
+            raise ValueError("bad batch plan capacities")
+        self.ws = torch.zeros(nb, dtype=torch.uint8, device=ctx.device)
+        self.tiles = Buffers(self.tiles_cap * SEG, ctx.device)
+        h = ctypes.c_void_p()
+        rc = ctx.lib.bass_bplan_create(self.jobs_cap, self.tiles_cap, self.tiles.xp, self.tiles.ap,
+                                       L.ptr(self.ws), nb, 1 if graphs else 0, ctypes.byref(h))
+        L.check(rc, "bass_bplan_create")
+        self._h = h
+        self._lib = ctx.lib
+
+    def run(self, op, jobs, out=None):
+        ptr = out.ctypes.data_as(ctypes.c_void_p) if out is not None else None
+        rc = self._lib.bass_bplan_run(self._h, int(op), jobs.ctypes.data_as(ctypes.c_void_p), len(jobs), ptr,
+                                      self.ctx.stream)
+        L.check(rc, "bass_bplan_run")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.bass_bplan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pinned_array(shape, dtype):
+    """A numpy array over pinned host memory (asynchronous device copies)."""
+    import torch
+
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    t = torch.empty(max(n, 1), dtype=torch.uint8, pin_memory=True)
+    arr = t.numpy()[:n].view(dt).reshape(shape)
+    return arr, t

@@ -353,7 +353,7 @@ static int noalloc_scratch(void** out) {
 
 extern "C" {
 
-int bass_abi_version(void) { return 2; }
+int bass_abi_version(void) { return 3; }
 
 const char* bass_last_error(void) { return g_err; }
 
@@ -927,27 +927,73 @@ int bass_dense_vdot(const double* u, const double* v, int64_t len, double* out, 
 }
 
 // ----------------------------------------------------------- batches ----
-// Per-job region at the start of the workspace's big area (offsets depend on
-// njobs only), then the select's candidate arrays (32 B per tile slot), then
-// scratch of the RDM / moments calls.
-struct BatchRegion {
-  size_t st, sel, hist1, hist2, histw, ljobs, probs, rjobs, ewjobs, end;
+// Regions of a batch launch sequence, from a base offset, for jc jobs and tc
+// output tiles: per-job stats first (bass_batch_stats reads them at the base
+// whatever the rest), select states, histograms, RDM results, the staged
+// tables (each header directly followed by its jobs: one copy), per-tile
+// scratch, lift boundaries, and the select's per-slot candidate arrays (also
+// the scratch of the RDM and moment calls).
+struct BLayout {
+  size_t st, sel, hist1, hist2, rho, hdr_l, ljobs, probs, hdr_e, ewjobs, rjobs;
+  size_t tile_psum, seg_cnt, cnt1, tile_dsum, tile_dcnt, ksum, kcnt, koff, tile_s2, tile_s4, hist2_rows, bounds;
+  size_t c1p, c1i, c2p, c2i, tk, end;
+  int jc;
+  int64_t tc;
 };
 
-static BatchRegion batch_region(const Layout& L, int njobs) {
-  BatchRegion R;
-  size_t o = L.big;
-  R.st = o; o += al((size_t)njobs * sizeof(OpStats));
-  R.sel = o; o += al((size_t)njobs * sizeof(SelState));
-  R.hist1 = o; o += al((size_t)njobs * HIST_BINS * 4);
-  R.hist2 = o; o += al((size_t)njobs * HIST_BINS * 4);
-  R.histw = o; o += al((size_t)njobs * HIST_BINS * 4);
-  R.ljobs = o; o += al((size_t)njobs * sizeof(LiftJob));
-  R.probs = o; o += al((size_t)njobs * sizeof(SelProblem));
-  R.rjobs = o; o += al((size_t)njobs * sizeof(RdmJob));
-  R.ewjobs = o; o += al((size_t)njobs * sizeof(EwJob));
-  R.end = o;
-  return R;
+static BLayout blayout(size_t base, int jc, int64_t tc) {
+  BLayout B;
+  B.jc = jc;
+  B.tc = tc;
+  const size_t t = (size_t)(tc > 0 ? tc : 0);
+  const size_t slots = t * SEG_ENTRIES;
+  size_t o = base;
+  B.st = o; o += al((size_t)jc * sizeof(OpStats));
+  B.sel = o; o += al((size_t)jc * sizeof(SelState));
+  B.hist1 = o; o += al((size_t)jc * HIST_BINS * 4);
+  B.hist2 = o; o += al((size_t)jc * HIST_BINS * 4);
+  B.rho = o; o += al((size_t)jc * 64 * 4 * 8);
+  B.hdr_l = o; B.ljobs = o + sizeof(BatchHdr); o = al(B.ljobs + (size_t)jc * sizeof(LiftJob));
+  B.probs = o; o += al((size_t)jc * sizeof(SelProblem));
+  B.hdr_e = o; B.ewjobs = o + sizeof(BatchHdr); o = al(B.ewjobs + (size_t)jc * sizeof(EwJob));
+  B.rjobs = o; o += al((size_t)jc * sizeof(RdmJob));
+  B.tile_psum = o; o += al(t * 8);
+  B.seg_cnt = o; o += al(t * 4);
+  B.cnt1 = o; o += al(t * 4);
+  B.tile_dsum = o; o += al(t * 8);
+  B.tile_dcnt = o; o += al(t * 4);
+  B.ksum = o; o += al(t * 8);
+  B.kcnt = o; o += al(t * 4);
+  B.koff = o; o += al(t * 8);
+  B.tile_s2 = o; o += al(t * 8);
+  B.tile_s4 = o; o += al(t * 8);
+  B.hist2_rows = o; o += al(t * HIST_BINS * 2);
+  B.bounds = o; o += al((t + (size_t)jc + 1) * sizeof(LiftBound));
+  B.c1p = o; o += al(slots * 8);
+  B.c1i = o; o += al(slots * 4);
+  B.c2p = o; o += al(slots * 8);
+  B.c2i = o; o += al(slots * 4);
+  B.tk = o; o += al(slots * 8);
+  B.end = o;
+  return B;
+}
+
+static SelTiles btiles(void* ws, const BLayout& B) {
+  SelTiles T;
+  memset(&T, 0, sizeof(T));
+  T.tile_psum = at<double>(ws, B.tile_psum);
+  T.c1p = at<unsigned long long>(ws, B.c1p);
+  T.c1i = at<unsigned int>(ws, B.c1i);
+  T.cnt1 = at<int>(ws, B.cnt1);
+  T.tile_dsum = at<double>(ws, B.tile_dsum);
+  T.tile_dcnt = at<int>(ws, B.tile_dcnt);
+  T.ksum = at<double>(ws, B.ksum);
+  T.kcnt = at<int>(ws, B.kcnt);
+  T.koff = at<int64_t>(ws, B.koff);
+  T.tile_s2 = at<double>(ws, B.tile_s2);
+  T.tile_s4 = at<double>(ws, B.tile_s4);
+  T.hist2_rows = at<unsigned short>(ws, B.hist2_rows);
+  return T;
 }
 
 // Pinned staging ring for host -> device job tables: a slot is reused only
@@ -1012,120 +1058,409 @@ static int check_job(const bass_job& J, int c, bool need_k, int jmin, const char
   return BASS_OK;
 }
 
+// Host tables of a batch lift (step: with select problems; move: without).
+// The staged blob is the header followed by the lift jobs.  *tbound: output
+// tiles for the jobs' n (a job with a count cell gives its count's bound).
+struct BTables {
+  std::vector<char> hl;  // BatchHdr + LiftJob[njobs]
+  std::vector<SelProblem> pr;
+};
+
+static int build_lift_tables(const bass_job* jobs, int njobs, int nq, int mode, void* ws, const BLayout& B,
+                             uint64_t* tiles_x, double* tiles_a, BTables& X, int64_t* tbound, const char* who) {
+  const bool step = mode != MODE_MOVE;
+  const int tm = lift_tile_entries(nq, mode);
+  X.hl.assign(sizeof(BatchHdr) + (size_t)njobs * sizeof(LiftJob), 0);
+  BatchHdr* h = (BatchHdr*)X.hl.data();
+  LiftJob* lj = (LiftJob*)(X.hl.data() + sizeof(BatchHdr));
+  if (step) X.pr.assign(njobs, SelProblem());
+  int64_t T = 0;
+  for (int c = 0; c < njobs; ++c) {
+    const bass_job& J = jobs[c];
+    KeyMap km;
+    const int rc = check_job(J, c, step, step ? nq - 1 : 0, who, step ? &km : nullptr);
+    if (rc) return rc;
+    if (!step && J.cap_out < J.n) return fail(BASS_E_OVERFLOW, "%s: job %d: cap_out < n", who, c);
+    LiftJob& L = lj[c];
+    L.x = J.x;
+    L.a = (const double2*)J.a;
+    L.xo = J.xo;
+    L.ao = (double2*)J.ao;
+    L.n = J.n;
+    L.cnt_in = J.cnt_in;
+    L.cnt_out = step ? nullptr : J.cnt_out;
+    L.j = J.j;
+    if (step) {
+      const int M = 1 << nq;
+      for (int e = 0; e < M * M; ++e) L.g.m[e] = make_double2(J.g[2 * e], J.g[2 * e + 1]);
+      SelProblem& P = X.pr[c];
+      memset(&P, 0, sizeof(P));
+      P.x = tiles_x;
+      P.a = (const double2*)tiles_a;
+      P.seg_cnt = at<int>(ws, B.seg_cnt);
+      P.k = (unsigned long long)J.k;
+      P.xo = J.xo;
+      P.ao = (double2*)J.ao;
+      P.cap_out = J.cap_out;
+      P.renorm = J.renorm;
+      P.finish_any = 1;
+      P.hist1 = at<unsigned int>(ws, B.hist1) + (size_t)c * HIST_BINS;
+      P.hist2 = at<unsigned int>(ws, B.hist2) + (size_t)c * HIST_BINS;
+      P.sel = at<SelState>(ws, B.sel) + c;
+      P.st = at<OpStats>(ws, B.st) + c;
+      P.km = km;
+      P.cnt_out = J.cnt_out;
+    }
+    T += (J.n + tm - 1) / tm;
+  }
+  h->njobs = njobs;
+  h->cap_tiles = B.tc;
+  *tbound = T;
+  if (T > B.tc) return fail(BASS_E_OVERFLOW, "%s: jobs need %lld tiles, the buffers hold %lld", who, (long long)T,
+                            (long long)B.tc);
+  return BASS_OK;
+}
+
+// Launch sequences.  No argument depends on a count: tiles / jobs are bounds
+// that size the persistent grids.
+static void seq_lift(void* ws, const BLayout& B, int nq, int mode, uint64_t* tiles_x, double* tiles_a, int64_t tiles,
+                     int jobs, hipStream_t s) {
+  const int tm = lift_tile_entries(nq, mode);
+  LiftJob* lj = at<LiftJob>(ws, B.ljobs);
+  BatchHdr* hdr = at<BatchHdr>(ws, B.hdr_l);
+  if (mode == MODE_MOVE) {
+    launch_batch_prep(lj, nullptr, hdr, tm, SelBase{nullptr, nullptr, nullptr},
+                      BatchZero{nullptr, nullptr, nullptr, nullptr}, s);
+    launch_lift_batch(1, MODE_MOVE, lj, hdr, tiles, jobs, at<LiftBound>(ws, B.bounds), nullptr, nullptr, nullptr,
+                      nullptr, nullptr, s);
+    return;
+  }
+  SelProblem* pr = at<SelProblem>(ws, B.probs);
+  const SelBase sb{at<unsigned long long>(ws, B.c2p), at<unsigned int>(ws, B.c2i), at<uint64_t>(ws, B.tk)};
+  const BatchZero z{at<OpStats>(ws, B.st), at<SelState>(ws, B.sel), at<unsigned int>(ws, B.hist1),
+                    at<unsigned int>(ws, B.hist2)};
+  launch_batch_prep(lj, pr, hdr, tm, sb, z, s);
+  launch_lift_batch(nq, mode, lj, hdr, tiles, jobs, at<LiftBound>(ws, B.bounds), tiles_x, (double2*)tiles_a,
+                    at<unsigned int>(ws, B.hist1), at<double>(ws, B.tile_psum), at<int>(ws, B.seg_cnt), s);
+  launch_select_batch(pr, hdr, btiles(ws, B), tiles, jobs, s);
+}
+
+// Elementwise tables: header + EwJob[njobs]; *ptiles: moment tiles.
+static int build_ew_tables(int op, const bass_job* jobs, int njobs, std::vector<char>& he, int64_t* ptiles,
+                           const char* who) {
+  he.assign(sizeof(BatchHdr) + (size_t)njobs * sizeof(EwJob), 0);
+  BatchHdr* h = (BatchHdr*)he.data();
+  EwJob* ej = (EwJob*)(he.data() + sizeof(BatchHdr));
+  int64_t ntiles = 0;
+  for (int c = 0; c < njobs; ++c) {
+    const bass_job& J = jobs[c];
+    if (J.n < 0) return fail(BASS_E_ARG, "%s: job %d: bad n", who, c);
+    if (op == 0 && (J.j < 0 || J.j > 62 || J.j2 < 0 || J.j2 > 62 || J.j == J.j2))
+      return fail(BASS_E_ARG, "%s: job %d: bad bits", who, c);
+    if (op == 2 && J.cnt_in) return fail(BASS_E_ARG, "%s: job %d: moments take the staged count", who, c);
+    EwJob& E = ej[c];
+    E.x = J.x;
+    E.a = (double2*)J.a;
+    E.n = J.n;
+    E.cnt_in = J.cnt_in;
+    E.j1 = J.j;
+    E.j2 = J.j2;
+    for (int e = 0; e < 4; ++e) E.d[e] = make_double2(J.g[2 * e], J.g[2 * e + 1]);
+    E.div = J.scale;
+    E.toff = ntiles;
+    ntiles += (J.n + SEG_ENTRIES - 1) / SEG_ENTRIES;
+  }
+  h->njobs = njobs;
+  *ptiles = ntiles;
+  return BASS_OK;
+}
+
+// RDM tables (staged counts) and scratch needs.
+static int build_rdm_tables(const bass_job* jobs, int njobs, std::vector<RdmJob>& rj, int64_t* nb_, int64_t* np_,
+                            int64_t* nd_) {
+  const int tm = rdm_tile_entries();
+  rj.assign(njobs, RdmJob());
+  int64_t nb = 0, np = 0, nd = 0;
+  for (int c = 0; c < njobs; ++c) {
+    const bass_job& J = jobs[c];
+    if (J.n < 0 || J.n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_batch_rdm1q: job %d: bad n", c);
+    if (J.key_bits < 1 || J.key_bits > 63) return fail(BASS_E_ARG, "bass_batch_rdm1q: job %d: bad key_bits", c);
+    if (J.cnt_in) return fail(BASS_E_ARG, "bass_batch_rdm1q: job %d: RDMs take the staged count", c);
+    RdmJob& R = rj[c];
+    memset(&R, 0, sizeof(R));
+    R.x = J.x;
+    R.a = (const double2*)J.a;
+    R.n = J.n;
+    R.nbits = J.key_bits;
+    R.ptiles = (J.n + tm - 1) / tm;
+    R.dtiles = (J.n + SEG_ENTRIES - 1) / SEG_ENTRIES;
+    R.bound_off = nb;
+    R.ptile_off = np;
+    R.dtile_off = nd;
+    nb += (int64_t)J.key_bits * (R.ptiles + 1);
+    np += (int64_t)J.key_bits * R.ptiles;
+    nd += R.dtiles;
+  }
+  *nb_ = nb;
+  *np_ = np;
+  *nd_ = nd;
+  return BASS_OK;
+}
+
+static int d2h_async(void* host, const void* dev, size_t bytes, hipStream_t s, const char* who) {
+  if (!host || !bytes) return BASS_OK;
+  if (hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s) != hipSuccess)
+    return fail(BASS_E_HIP, "%s: result copy: %s", who, hipGetErrorString(hipGetLastError()));
+  return BASS_OK;
+}
+
+// ------------------------------------------------------------ batch plan ----
+// A plan fixes the workspace regions for jc jobs and tc tiles, so every call's
+// launch sequence has one shape; with graphs on, each sequence is captured
+// once into a HIP graph (on a private stream) and replayed on the caller's
+// stream after the call's tables are staged.
+enum { BOP_GATE1 = 0, BOP_GATE2, BOP_ROT1, BOP_ROT2, BOP_MOVE, BOP_DIAG2Q, BOP_SCALE, BOP_MOMENTS, BOP_RDM1Q,
+       BOP_COUNT };
+// Graphs per size class of an operation: tile and job bounds rounded up to
+// powers of 4 (grids near the work of the call; blocks past it exit).
+constexpr int BCLS_T = 12, BCLS_J = 8;
+static int bclass(int64_t v, int ncls) {
+  int c = 0;
+  int64_t b = 1;
+  while (b < v && c < ncls - 1) {
+    b <<= 2;
+    ++c;
+  }
+  return c;
+}
+static int64_t bclass_bound(int c) { return (int64_t)1 << (2 * c); }
+
+struct BPlan {
+  int jc = 0;
+  int64_t tc = 0;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  uint64_t* tiles_x = nullptr;
+  double* tiles_a = nullptr;
+  BLayout B;
+  bool graphs = false;
+  int dev = 0;
+  hipStream_t cap = nullptr;
+  hipGraphExec_t exec[BOP_COUNT][BCLS_T][BCLS_J] = {};
+  BTables X;
+  std::vector<char> he;
+  std::vector<RdmJob> rj;
+};
+
+// The fixed sequence of op (tiles / jobs: bounds sizing the persistent grids).
+static void plan_sequence(BPlan* P, int op, int64_t tiles, int jobs, hipStream_t s) {
+  switch (op) {
+    case BOP_GATE1: seq_lift(P->ws, P->B, 1, MODE_GATE, P->tiles_x, P->tiles_a, tiles, jobs, s); break;
+    case BOP_GATE2: seq_lift(P->ws, P->B, 2, MODE_GATE, P->tiles_x, P->tiles_a, tiles, jobs, s); break;
+    case BOP_ROT1: seq_lift(P->ws, P->B, 1, MODE_ROT, P->tiles_x, P->tiles_a, tiles, jobs, s); break;
+    case BOP_ROT2: seq_lift(P->ws, P->B, 2, MODE_ROT, P->tiles_x, P->tiles_a, tiles, jobs, s); break;
+    case BOP_MOVE: seq_lift(P->ws, P->B, 1, MODE_MOVE, nullptr, nullptr, tiles, jobs, s); break;
+    case BOP_DIAG2Q:
+    case BOP_SCALE:
+      launch_ew_batch(op == BOP_DIAG2Q ? 0 : 1, at<EwJob>(P->ws, P->B.ewjobs), at<BatchHdr>(P->ws, P->B.hdr_e), jobs,
+                      0, nullptr, at<OpStats>(P->ws, P->B.st), s);
+      break;
+    default: break;
+  }
+}
+
+static int plan_launch(BPlan* P, int op, int64_t tiles, int jobs, hipStream_t s) {
+  if (!P->graphs) {
+    plan_sequence(P, op, tiles, jobs, s);
+    return hip_check("bass_bplan_run");
+  }
+  // the size class of the call's bounds (the last class holds any size: its
+  // grids are the plan's capacity)
+  const int ct = bclass(tiles, BCLS_T), cj = bclass(jobs, BCLS_J);
+  hipGraphExec_t& ex = P->exec[op][ct][cj];
+  if (!ex) {
+    // capture once per class, with the persistent grids of its bounds
+    const int64_t bt = ct == BCLS_T - 1 ? P->tc : bclass_bound(ct);
+    const int bj = cj == BCLS_J - 1 ? P->jc : (int)bclass_bound(cj);
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return fail(BASS_E_HIP, "bass_bplan_run: begin capture: %s", hipGetErrorString(hipGetLastError()));
+    plan_sequence(P, op, bt > P->tc ? P->tc : bt, bj > P->jc ? P->jc : bj, P->cap);
+    const hipError_t e1 = hipStreamEndCapture(P->cap, &g);
+    if (e1 != hipSuccess || !g) return fail(BASS_E_HIP, "bass_bplan_run: end capture: %s", hipGetErrorString(e1));
+    const hipError_t e2 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+      ex = nullptr;
+      return fail(BASS_E_HIP, "bass_bplan_run: graph instantiation: %s", hipGetErrorString(e2));
+    }
+  }
+  if (hipGraphLaunch(ex, s) != hipSuccess)
+    return fail(BASS_E_HIP, "bass_bplan_run: graph launch: %s", hipGetErrorString(hipGetLastError()));
+  return BASS_OK;
+}
+
+size_t bass_bplan_ws_bytes(int jobs_cap, int64_t tiles_cap) {
+  if (jobs_cap < 1 || tiles_cap < 1) return 0;
+  return blayout(0, jobs_cap, tiles_cap).end;
+}
+
+int bass_bplan_create(int jobs_cap, int64_t tiles_cap, uint64_t* tiles_x, double* tiles_a, void* ws, size_t ws_bytes,
+                      int flags, void** plan) {
+  if (!plan) return fail(BASS_E_ARG, "bass_bplan_create: null plan");
+  *plan = nullptr;
+  if (jobs_cap < 1 || jobs_cap > (1 << 20) || tiles_cap < 1)
+    return fail(BASS_E_ARG, "bass_bplan_create: bad capacities");
+  if (!ws || !tiles_x || !tiles_a) return fail(BASS_E_ARG, "bass_bplan_create: null buffer");
+  const BLayout B = blayout(0, jobs_cap, tiles_cap);
+  if (B.end > ws_bytes)
+    return fail(BASS_E_WORKSPACE, "bass_bplan_create: workspace of %zu bytes, plan needs %zu", ws_bytes, B.end);
+  BPlan* P = new BPlan();
+  P->jc = jobs_cap;
+  P->tc = tiles_cap;
+  P->ws = ws;
+  P->ws_bytes = ws_bytes;
+  P->tiles_x = tiles_x;
+  P->tiles_a = tiles_a;
+  P->B = B;
+  P->graphs = (flags & 1) != 0;
+  (void)hipGetDevice(&P->dev);
+  if (P->graphs && hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking) != hipSuccess) {
+    delete P;
+    return fail(BASS_E_HIP, "bass_bplan_create: capture stream: %s", hipGetErrorString(hipGetLastError()));
+  }
+  *plan = P;
+  return BASS_OK;
+}
+
+int bass_bplan_destroy(void* plan) {
+  BPlan* P = (BPlan*)plan;
+  if (!P) return BASS_OK;
+  for (int i = 0; i < BOP_COUNT; ++i)
+    for (int a = 0; a < BCLS_T; ++a)
+      for (int b = 0; b < BCLS_J; ++b)
+        if (P->exec[i][a][b]) (void)hipGraphExecDestroy(P->exec[i][a][b]);
+  if (P->cap) (void)hipStreamDestroy(P->cap);
+  delete P;
+  return BASS_OK;
+}
+
+int bass_bplan_run(void* plan, int op, const bass_job* jobs, int njobs, void* out_host, void* stream) {
+  BPlan* P = (BPlan*)plan;
+  if (!P) return fail(BASS_E_ARG, "bass_bplan_run: null plan");
+  if (op < 0 || op >= BOP_COUNT) return fail(BASS_E_ARG, "bass_bplan_run: bad op %d", op);
+  if (njobs < 1 || njobs > P->jc) return fail(BASS_E_ARG, "bass_bplan_run: %d jobs, plan holds %d", njobs, P->jc);
+  hipStream_t s = (hipStream_t)stream;
+  void* ws = P->ws;
+  const BLayout& B = P->B;
+  static const char* names[BOP_COUNT] = {"batch_gate1q", "batch_gate2q", "batch_rot1q", "batch_rot2q", "batch_move",
+                                         "batch_diag2q", "batch_scale", "batch_moments", "batch_rdm"};
+  int rc = BASS_OK;
+  if (op <= BOP_MOVE) {
+    const int nq = (op == BOP_GATE2 || op == BOP_ROT2) ? 2 : 1;
+    const int mode = op == BOP_MOVE ? MODE_MOVE : ((op == BOP_ROT1 || op == BOP_ROT2) ? MODE_ROT : MODE_GATE);
+    int64_t T = 0;
+    rc = build_lift_tables(jobs, njobs, nq, mode, ws, B, P->tiles_x, P->tiles_a, P->X, &T, "bass_bplan_run");
+    if (rc) return rc;
+    const int tk = ktimer_begin(names[op], s);
+    rc = stage_copy(at<void>(ws, B.hdr_l), P->X.hl.data(), P->X.hl.size(), s);
+    if (!rc && mode != MODE_MOVE)
+      rc = stage_copy(at<void>(ws, B.probs), P->X.pr.data(), sizeof(SelProblem) * njobs, s);
+    if (!rc) rc = plan_launch(P, op, T, njobs, s);
+    ktimer_end(tk, s);
+    if (!rc && mode != MODE_MOVE) rc = d2h_async(out_host, at<void>(ws, B.st), sizeof(bass_stats) * njobs, s, names[op]);
+    return rc;
+  }
+  if (op == BOP_DIAG2Q || op == BOP_SCALE || op == BOP_MOMENTS) {
+    const int eop = op == BOP_DIAG2Q ? 0 : (op == BOP_SCALE ? 1 : 2);
+    int64_t pt = 0;
+    rc = build_ew_tables(eop, jobs, njobs, P->he, &pt, "bass_bplan_run");
+    if (rc) return rc;
+    if (eop == 2 && (size_t)pt * 16 + 256 > B.end - B.c1p)
+      return fail(BASS_E_WORKSPACE, "bass_bplan_run: moments of %lld tiles do not fit", (long long)pt);
+    const int tk = ktimer_begin(names[op], s);
+    rc = stage_copy(at<void>(ws, B.hdr_e), P->he.data(), P->he.size(), s);
+    if (!rc && eop < 2) {
+      rc = plan_launch(P, op, 1, njobs, s);
+    } else if (!rc) {
+      fill(at<void>(ws, B.st), sizeof(OpStats) * njobs, 0ull, s);
+      launch_ew_batch(2, at<EwJob>(ws, B.ewjobs), at<BatchHdr>(ws, B.hdr_e), njobs, pt, at<double>(ws, B.c1p),
+                      at<OpStats>(ws, B.st), s);
+      rc = hip_check("bass_bplan_run");
+      if (!rc) rc = d2h_async(out_host, at<void>(ws, B.st), sizeof(bass_stats) * njobs, s, names[op]);
+    }
+    ktimer_end(tk, s);
+    return rc;
+  }
+  // RDM1Q: staged counts, result rho (njobs x 64 x 4 doubles) copied to out_host
+  int64_t nb = 0, np = 0, nd = 0;
+  rc = build_rdm_tables(jobs, njobs, P->rj, &nb, &np, &nd);
+  if (rc) return rc;
+  if (rdm_batch_scratch(nb, np, nd) > B.end - B.c1p)
+    return fail(BASS_E_WORKSPACE, "bass_bplan_run: RDM scratch does not fit");
+  const size_t rho_bytes = (size_t)njobs * 64 * 4 * 8;
+  double* rho = at<double>(ws, B.rho);
+  const int tk = ktimer_begin(names[op], s);
+  fill(rho, rho_bytes, 0ull, s);
+  rc = stage_copy(at<void>(ws, B.rjobs), P->rj.data(), sizeof(RdmJob) * njobs, s);
+  if (!rc) {
+    launch_rdm_batch(at<RdmJob>(ws, B.rjobs), njobs, nb, np, nd, at<void>(ws, B.c1p), rho, s);
+    rc = hip_check("bass_bplan_run");
+  }
+  ktimer_end(tk, s);
+  if (!rc) rc = d2h_async(out_host, rho, rho_bytes, s, names[op]);
+  return rc;
+}
+
+// ---------------------------------------------- batch calls without a plan --
+// Laid out per call from the workspace's big area for the call's own counts
+// (direct launches, exact grids).
 int bass_batch_step(const bass_job* jobs, int njobs, int nq, int mode, int wide, uint64_t* tiles_x, double* tiles_a,
                     int64_t tiles_cap, void* ws, size_t ws_bytes, void* stream) {
+  (void)wide;  // every threshold is resolved in the call
   if (njobs < 1) return fail(BASS_E_ARG, "bass_batch_step: no jobs");
   if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_batch_step: nq must be 1 or 2");
   if (mode != MODE_GATE && mode != MODE_ROT) return fail(BASS_E_ARG, "bass_batch_step: bad mode");
   const int tm = lift_tile_entries(nq, mode);
-  std::vector<LiftJob> lj(njobs);
-  std::vector<SelProblem> pr(njobs);
-  std::vector<KeyMap> kms(njobs);
-  int64_t T = 0, max_rows = 1;
-  for (int c = 0; c < njobs; ++c) {
-    const int rc = check_job(jobs[c], c, true, nq - 1, "bass_batch_step", &kms[c]);
-    if (rc) return rc;
-    const int64_t nt = (jobs[c].n + tm - 1) / tm;
-    memset(&lj[c], 0, sizeof(LiftJob));
-    lj[c].x = jobs[c].x;
-    lj[c].a = (const double2*)jobs[c].a;
-    lj[c].n = jobs[c].n;
-    lj[c].tile0 = T;
-    lj[c].j = jobs[c].j;
-    const int M = 1 << nq;
-    for (int e = 0; e < M * M; ++e) lj[c].g.m[e] = make_double2(jobs[c].g[2 * e], jobs[c].g[2 * e + 1]);
-    pr[c].ntiles = nt;
-    T += nt;
-    if (nt > max_rows) max_rows = nt;
-  }
+  int64_t T = 0;
+  for (int c = 0; c < njobs; ++c) T += (jobs[c].n > 0 ? (jobs[c].n + tm - 1) / tm : 0);
   const int64_t slots = T * SEG_ENTRIES;
   if (tiles_cap < slots)
     return fail(BASS_E_OVERFLOW, "bass_batch_step: tiles need %lld slots", (long long)slots);
   WS_REQUIRE(slots > njobs ? slots : njobs);
-  const BatchRegion R = batch_region(L, njobs);
-  const size_t p8 = al((size_t)8 * slots), p4 = al((size_t)4 * slots);
-  if (R.end + 3 * p8 + 2 * p4 > L.total) return fail(BASS_E_WORKSPACE, "bass_batch_step: batch does not fit");
+  const BLayout B = blayout(L.big, njobs, T > 0 ? T : 1);
+  if (B.end > L.total) return fail(BASS_E_WORKSPACE, "bass_batch_step: batch does not fit");
+  thread_local BTables X;
+  int64_t Tb = 0;
+  int rc = build_lift_tables(jobs, njobs, nq, mode, ws, B, tiles_x, tiles_a, X, &Tb, "bass_batch_step");
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  char* c1p = at<char>(ws, R.end);
-  char* c1i = c1p + p8;
-  char* c2p = c1i + p4;
-  char* c2i = c2p + p8;
-  char* tkb = c2i + p4;
-  for (int c = 0; c < njobs; ++c) {
-    SelProblem& P = pr[c];
-    const int64_t nt = P.ntiles;
-    memset(&P, 0, sizeof(P));
-    P.x = tiles_x;
-    P.a = (const double2*)tiles_a;
-    P.seg_cnt = at<int>(ws, L.seg_cnt);
-    P.tile0 = lj[c].tile0;
-    P.ntiles = nt;
-    P.row0 = lj[c].tile0;
-    P.nrows = nt;
-    P.npsum = nt;
-    P.k = (unsigned long long)jobs[c].k;
-    P.xo = jobs[c].xo;
-    P.ao = (double2*)jobs[c].ao;
-    P.cap_out = jobs[c].cap_out;
-    P.renorm = jobs[c].renorm;
-    P.hist1 = at<unsigned int>(ws, R.hist1) + (size_t)c * HIST_BINS;
-    P.hist2 = at<unsigned int>(ws, R.hist2) + (size_t)c * HIST_BINS;
-    P.hist_w = at<unsigned int>(ws, R.histw) + (size_t)c * HIST_BINS;
-    P.sel = at<SelState>(ws, R.sel) + c;
-    P.st = at<OpStats>(ws, R.st) + c;
-    P.c2p = (unsigned long long*)c2p + P.tile0 * SEG_ENTRIES;
-    P.c2i = (unsigned int*)c2i + P.tile0 * SEG_ENTRIES;
-    P.tk = (uint64_t*)tkb + P.tile0 * SEG_ENTRIES;
-    P.km = kms[c];
-  }
   static const char* names[2][2] = {{"batch_gate1q", "batch_gate2q"}, {"batch_rot1q", "batch_rot2q"}};
   const int tk = ktimer_begin(names[mode][nq - 1], s);
-  // per-job headers: stats, selection state, histograms
-  fill(at<void>(ws, R.st), R.ljobs - R.st, 0ull, s);
-  int rc = stage_copy(at<void>(ws, R.ljobs), lj.data(), sizeof(LiftJob) * njobs, s);
-  if (!rc) rc = stage_copy(at<void>(ws, R.probs), pr.data(), sizeof(SelProblem) * njobs, s);
+  rc = stage_copy(at<void>(ws, B.hdr_l), X.hl.data(), X.hl.size(), s);
+  if (!rc) rc = stage_copy(at<void>(ws, B.probs), X.pr.data(), sizeof(SelProblem) * njobs, s);
   if (rc) return rc;
-  if (T > 0) {
-    launch_lift_batch(nq, mode, at<LiftJob>(ws, R.ljobs), njobs, T, at<LiftBound>(ws, L.lift_b), tiles_x,
-                      (double2*)tiles_a, at<unsigned int>(ws, R.hist1), at<double>(ws, L.tile_psum),
-                      at<int>(ws, L.seg_cnt), s);
-  }
-  launch_select_batch(at<SelProblem>(ws, R.probs), njobs,
-                      tiles_view(ws, L, (unsigned long long*)c1p, (unsigned int*)c1i), T, max_rows, true,
-                      wide != 0, s);
+  seq_lift(ws, B, nq, mode, tiles_x, tiles_a, T, njobs, s);
   ktimer_end(tk, s);
   return hip_check("bass_batch_step");
 }
 
 int bass_batch_move(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream) {
   if (njobs < 1) return fail(BASS_E_ARG, "bass_batch_move: no jobs");
-  std::vector<LiftJob> lj(njobs);
-  int64_t T = 0, nmax = 1;
-  for (int c = 0; c < njobs; ++c) {
-    const int rc = check_job(jobs[c], c, false, 0, "bass_batch_move", nullptr);
-    if (rc) return rc;
-    if (jobs[c].cap_out < jobs[c].n) return fail(BASS_E_OVERFLOW, "bass_batch_move: job %d: cap_out < n", c);
-    memset(&lj[c], 0, sizeof(LiftJob));
-    lj[c].x = jobs[c].x;
-    lj[c].a = (const double2*)jobs[c].a;
-    lj[c].xo = jobs[c].xo;
-    lj[c].ao = (double2*)jobs[c].ao;
-    lj[c].n = jobs[c].n;
-    lj[c].tile0 = T;
-    lj[c].j = jobs[c].j;
-    T += (jobs[c].n + 1023) / 1024;
-    if (jobs[c].n > nmax) nmax = jobs[c].n;
-  }
+  int64_t T = 0;
+  for (int c = 0; c < njobs; ++c) T += (jobs[c].n > 0 ? (jobs[c].n + 1023) / 1024 : 0);
   WS_REQUIRE(T * 1024 + njobs);
-  const BatchRegion R = batch_region(L, njobs);
-  if (R.end > L.total) return fail(BASS_E_WORKSPACE, "bass_batch_move: batch does not fit");
+  const BLayout B = blayout(L.big, njobs, T > 0 ? T : 1);
+  if (B.end > L.total) return fail(BASS_E_WORKSPACE, "bass_batch_move: batch does not fit");
+  thread_local BTables X;
+  int64_t Tb = 0;
+  int rc = build_lift_tables(jobs, njobs, 1, MODE_MOVE, ws, B, nullptr, nullptr, X, &Tb, "bass_batch_move");
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const int tk = ktimer_begin("batch_move", s);
-  int rc = stage_copy(at<void>(ws, R.ljobs), lj.data(), sizeof(LiftJob) * njobs, s);
+  rc = stage_copy(at<void>(ws, B.hdr_l), X.hl.data(), X.hl.size(), s);
   if (rc) return rc;
-  if (T > 0)
-    launch_lift_batch(1, MODE_MOVE, at<LiftJob>(ws, R.ljobs), njobs, T, at<LiftBound>(ws, L.lift_b), nullptr,
-                      nullptr, nullptr, nullptr, nullptr, s);
+  seq_lift(ws, B, 1, MODE_MOVE, nullptr, nullptr, T, njobs, s);
   ktimer_end(tk, s);
   return hip_check("bass_batch_move");
 }
@@ -1133,32 +1468,19 @@ int bass_batch_move(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, 
 static int batch_ew(int op, const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, void* stream,
                     const char* who) {
   if (njobs < 1) return fail(BASS_E_ARG, "%s: no jobs", who);
-  std::vector<EwJob> ej(njobs);
-  int64_t ntiles = 0;
-  for (int c = 0; c < njobs; ++c) {
-    const bass_job& J = jobs[c];
-    if (J.n < 0) return fail(BASS_E_ARG, "%s: job %d: bad n", who, c);
-    if (op == 0 && (J.j < 0 || J.j > 62 || J.j2 < 0 || J.j2 > 62 || J.j == J.j2))
-      return fail(BASS_E_ARG, "%s: job %d: bad bits", who, c);
-    memset(&ej[c], 0, sizeof(EwJob));
-    ej[c].x = J.x;
-    ej[c].a = (double2*)J.a;
-    ej[c].n = J.n;
-    ej[c].j1 = J.j;
-    ej[c].j2 = J.j2;
-    for (int e = 0; e < 4; ++e) ej[c].d[e] = make_double2(J.g[2 * e], J.g[2 * e + 1]);
-    ej[c].div = J.scale;
-    ej[c].toff = ntiles;
-    ntiles += (J.n + SEG_ENTRIES - 1) / SEG_ENTRIES;
-  }
-  WS_REQUIRE(0);
-  const BatchRegion R = batch_region(L, njobs);
-  if (R.end + ew_batch_scratch(ntiles) > L.total) return fail(BASS_E_WORKSPACE, "%s: batch does not fit", who);
-  hipStream_t s = (hipStream_t)stream;
-  if (op == 2) fill(at<void>(ws, R.st), (size_t)njobs * sizeof(OpStats), 0ull, s);
-  int rc = stage_copy(at<void>(ws, R.ewjobs), ej.data(), sizeof(EwJob) * njobs, s);
+  thread_local std::vector<char> he;
+  int64_t pt = 0;
+  int rc = build_ew_tables(op, jobs, njobs, he, &pt, who);
   if (rc) return rc;
-  launch_ew_batch(op, at<EwJob>(ws, R.ewjobs), njobs, ntiles, at<double>(ws, R.end), at<OpStats>(ws, R.st), s);
+  WS_REQUIRE(0);
+  const BLayout B = blayout(L.big, njobs, 0);
+  if (B.end + ew_batch_scratch(pt) > L.total) return fail(BASS_E_WORKSPACE, "%s: batch does not fit", who);
+  hipStream_t s = (hipStream_t)stream;
+  if (op == 2) fill(at<void>(ws, B.st), (size_t)njobs * sizeof(OpStats), 0ull, s);
+  rc = stage_copy(at<void>(ws, B.hdr_e), he.data(), he.size(), s);
+  if (rc) return rc;
+  launch_ew_batch(op, at<EwJob>(ws, B.ewjobs), at<BatchHdr>(ws, B.hdr_e), njobs, pt, at<double>(ws, B.end),
+                  at<OpStats>(ws, B.st), s);
   return hip_check(who);
 }
 
@@ -1174,39 +1496,22 @@ int bass_batch_moments(const bass_job* jobs, int njobs, void* ws, size_t ws_byte
 
 int bass_batch_rdm1q(const bass_job* jobs, int njobs, double* rho_out, void* ws, size_t ws_bytes, void* stream) {
   if (njobs < 1) return fail(BASS_E_ARG, "bass_batch_rdm1q: no jobs");
-  const int tm = rdm_tile_entries();
-  std::vector<RdmJob> rj(njobs);
+  thread_local std::vector<RdmJob> rj;
   int64_t nb = 0, np = 0, nd = 0;
-  for (int c = 0; c < njobs; ++c) {
-    const bass_job& J = jobs[c];
-    if (J.n < 0 || J.n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_batch_rdm1q: job %d: bad n", c);
-    if (J.key_bits < 1 || J.key_bits > 63) return fail(BASS_E_ARG, "bass_batch_rdm1q: job %d: bad key_bits", c);
-    memset(&rj[c], 0, sizeof(RdmJob));
-    rj[c].x = J.x;
-    rj[c].a = (const double2*)J.a;
-    rj[c].n = J.n;
-    rj[c].nbits = J.key_bits;
-    rj[c].ptiles = (J.n + tm - 1) / tm;
-    rj[c].dtiles = (J.n + SEG_ENTRIES - 1) / SEG_ENTRIES;
-    rj[c].bound_off = nb;
-    rj[c].ptile_off = np;
-    rj[c].dtile_off = nd;
-    nb += (int64_t)J.key_bits * (rj[c].ptiles + 1);
-    np += (int64_t)J.key_bits * rj[c].ptiles;
-    nd += rj[c].dtiles;
-  }
+  int rc = build_rdm_tables(jobs, njobs, rj, &nb, &np, &nd);
+  if (rc) return rc;
   WS_REQUIRE(0);
-  const BatchRegion R = batch_region(L, njobs);
+  const BLayout B = blayout(L.big, njobs, 0);
   const size_t rho_bytes = (size_t)njobs * 64 * 4 * 8;
-  if (R.end + al(rho_bytes) + rdm_batch_scratch(nb, np, nd) > L.total)
+  if (B.end + rdm_batch_scratch(nb, np, nd) > L.total)
     return fail(BASS_E_WORKSPACE, "bass_batch_rdm1q: batch does not fit");
   hipStream_t s = (hipStream_t)stream;
-  double* rho = at<double>(ws, R.end);
+  double* rho = at<double>(ws, B.rho);
   const int tm2 = ktimer_begin("batch_rdm", s);
   fill(rho, rho_bytes, 0ull, s);
-  int rc = stage_copy(at<void>(ws, R.rjobs), rj.data(), sizeof(RdmJob) * njobs, s);
+  rc = stage_copy(at<void>(ws, B.rjobs), rj.data(), sizeof(RdmJob) * njobs, s);
   if (rc) return rc;
-  launch_rdm_batch(at<RdmJob>(ws, R.rjobs), njobs, nb, np, nd, at<void>(ws, R.end + al(rho_bytes)), rho, s);
+  launch_rdm_batch(at<RdmJob>(ws, B.rjobs), njobs, nb, np, nd, at<void>(ws, B.end), rho, s);
   ktimer_end(tm2, s);
   rc = hip_check("bass_batch_rdm1q");
   if (rc) return rc;
@@ -1219,9 +1524,9 @@ int bass_batch_rdm1q(const bass_job* jobs, int njobs, double* rho_out, void* ws,
 int bass_batch_stats(void* ws, size_t ws_bytes, int njobs, bass_stats* out, void* stream) {
   if (njobs < 1) return fail(BASS_E_ARG, "bass_batch_stats: no jobs");
   WS_REQUIRE(0);
-  const BatchRegion R = batch_region(L, njobs);
+  if (L.big + (size_t)njobs * sizeof(OpStats) > L.total) return fail(BASS_E_ARG, "bass_batch_stats: too many jobs");
   hipStream_t s = (hipStream_t)stream;
-  OpStats* st = at<OpStats>(ws, R.st);
+  OpStats* st = at<OpStats>(ws, L.big);  // the stats region of every batch layout starts at the big area
   if (hipMemcpyAsync(out, st, sizeof(bass_stats) * njobs, hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return fail(BASS_E_HIP, "bass_batch_stats: %s", hipGetErrorString(hipGetLastError()));

@@ -49,8 +49,8 @@ struct SelState {
 
 // One block resolves at most FIN_MAX digit-2 candidates.  Beyond that a single
 // problem's select runs the staged multi-block stages of k_finish_staged in
-// the same launch; a batch's k_finish reports OpStats.flags bit 1 and the call
-// is repeated with the wide (multi-launch) path.
+// the same launch; the problems of a batch (at most 4 x 2^20 inputs each)
+// are resolved by their own block whatever the count.
 constexpr unsigned int FIN_MAX = 1u << 16;
 constexpr int WIDE_LEVELS = 10;  // 4 |a|^2 digits + 6 key digits
 
@@ -99,21 +99,67 @@ struct LiftBound {
   int a, b;        // A / B cursors of the merge at the boundary
   int pad0, pad1, pad2;
 };
+// Header of a batch launch sequence (device memory).  The host stages njobs
+// and cap_tiles; the sequence's prep kernel fills the rest from the jobs'
+// input counts, so no launch argument depends on a count: every batch kernel
+// runs a persistent grid over the totals here, and one sequence can be
+// captured in a graph and replayed whatever the counts.
+struct BatchHdr {
+  int njobs;          // staged: jobs in the table
+  int nrun;           // prep: jobs the kernels process (0 after an overflow)
+  int64_t cap_tiles;  // staged: output tiles the tile buffers hold
+  int64_t ntiles;     // prep: output tiles over all jobs
+  int64_t nbound;     // prep: lift boundaries, ntiles + njobs
+  int64_t max_rows;   // prep: largest tile count of one job
+  int64_t pad[3];
+};
+// Persistent grids of the batch kernels: at most the blocks of the kernel the
+// device holds at once (so every block starts at once and the work is shared
+// evenly), at most `bound` (virtual blocks of work, when known), at least 1.
+int resident_blocks(const void* kernel, int threads);
+inline unsigned persist_grid(const void* kernel, int threads, int64_t bound) {
+  const int64_t r = resident_blocks(kernel, threads);
+  const int64_t g = bound < r ? bound : r;
+  return (unsigned)(g < 1 ? 1 : g);
+}
+
 // One state of a batched lift (see launch_lift_batch).
 struct LiftJob {
   const uint64_t* x;  // input (sorted by stored word)
   const double2* a;
   uint64_t* xo;       // move: dense output of this job
   double2* ao;
-  int64_t n;
-  int64_t tile0;      // first global output tile
+  int64_t n;          // input entries (the prep kernel's copy of *cnt_in when set)
+  int64_t tile0;      // first global output tile (prep kernel)
+  const int64_t* cnt_in;  // device count cell of the input, or null (n as staged)
+  int64_t* cnt_out;       // move: count cell of the output (set to n), or null
   int j;              // bit to lift
   int pad;
   GateMat g;
 };
-int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, int njobs, int64_t ntiles_total, LiftBound* bounds,
-                      uint64_t* xo, double2* ao, unsigned int* hist, double* tile_psum, int* seg_cnt,
-                      hipStream_t s);
+// Tile-slot bases of the select's per-problem candidate arrays (the prep
+// kernel points each problem at its own tiles).
+struct SelBase {
+  unsigned long long* c2p;
+  unsigned int* c2i;
+  uint64_t* tk;
+};
+struct SelProblem;
+// Zeroed per-job regions of a batch step (cleared by its prep kernel).
+struct BatchZero {
+  OpStats* st;
+  SelState* sel;
+  unsigned int* hist1;
+  unsigned int* hist2;
+};
+// Prep of a lift / step sequence: counts, tile geometry, header, zeroing
+// (probs == nullptr: a move).  grid_zero blocks clear the histograms.
+int launch_batch_prep(LiftJob* d_jobs, SelProblem* d_probs, BatchHdr* hdr, int tm, const SelBase& sb,
+                      const BatchZero& z, hipStream_t s);
+// tiles / jobs: bounds of the tile and job counts (size the persistent grids)
+int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, const BatchHdr* hdr, int64_t tiles, int jobs,
+                      LiftBound* bounds, uint64_t* xo, double2* ao, unsigned int* hist, double* tile_psum,
+                      int* seg_cnt, hipStream_t s);
 int lift_tile_entries(int qb, int mode);
 // Outputs of the select's digit-1 pass per tile (ex1_tile_core).
 struct Ex1Out {
@@ -144,13 +190,16 @@ struct EwJob {
   const uint64_t* x;
   double2* a;
   int64_t n;
+  const int64_t* cnt_in;  // device count cell of the state, or null (n as staged)
   int j1, j2;
   double2 d[4];
   double div;
   int64_t toff;  // moments: first of the job's SEG_ENTRIES tiles in the scratch arrays
 };
-int launch_ew_batch(int op, const EwJob* d_jobs, int njobs, int64_t ntiles_total, double* part, OpStats* st,
-                    hipStream_t s);
+// hdr->njobs jobs (staged); jobs: a bound of that count (sizes the grid);
+// part_tiles: moment tiles of all jobs (part holds 2 x part_tiles doubles)
+int launch_ew_batch(int op, const EwJob* d_jobs, const BatchHdr* hdr, int jobs, int64_t part_tiles,
+                    double* part, OpStats* st, hipStream_t s);
 size_t ew_batch_scratch(int64_t ntiles_total);
 
 // One state of a batched RDM computation (see launch_rdm_batch).
@@ -217,6 +266,8 @@ struct SelProblem {
   // select's bucket, 0xFFFFFFFF after a keep-all select); k_find1 compares
   // and updates it.  Null: no speculation.
   unsigned int* spec_word;
+  // batch: count cell set to the output count (k_sums_finalize), or null
+  int64_t* cnt_out;
 };
 constexpr int KTAB_WORDS = 8 * 256;
 
@@ -252,8 +303,11 @@ inline int64_t sel_chunks(int64_t ntiles) { return (ntiles + SEL_CHUNK - 1) / SE
 
 int select_rows(int64_t ntiles, bool batch);
 int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s);
-int launch_select_batch(const SelProblem* d_probs, int nprob, const SelTiles& T, int64_t tiles_end,
-                        int64_t max_rows, bool hist_ready, bool wide, hipStream_t s);
+// A batch of problems with the geometry in hdr (prep kernel); tiles / jobs:
+// bounds of the tile and problem counts (size the persistent grids).  Every
+// threshold is resolved in the sequence (one block per problem).
+int launch_select_batch(const SelProblem* d_probs, const BatchHdr* hdr, const SelTiles& T, int64_t tiles,
+                        int jobs, hipStream_t s);
 int64_t sel_tiles(int64_t n);
 int launch_seg_scan(const int* seg_cnt, int64_t ntiles, int64_t* soff, int64_t* ch_cnt, int64_t* ch_off,
                     OpStats* st, hipStream_t s);

@@ -766,9 +766,11 @@ __global__ __launch_bounds__(NT) void k_lift_move(const uint64_t* __restrict__ w
 }
 
 // ------------------------------------------------------------- batches ----
-// A batch of independent states (one job each) in one launch: job c owns
-// output tiles [tile0_c, tile0_c + ntiles_c) and boundaries [b0_c, b0_c +
+// A batch of independent states (one job each) in one launch sequence: job c
+// owns output tiles [tile0_c, tile0_c + ntiles_c) and boundaries [b0_c, b0_c +
 // ntiles_c] (b0_c = tile0_c + c); blocks find their job by binary search.
+// k_bprep fixes that geometry on the device (BatchHdr): every kernel after it
+// walks the totals of the header with a persistent grid of any size.
 // Job of a block-uniform tile index.  The result is made explicitly
 // wave-uniform so the job's fields (pointers, gate matrix) are read with
 // scalar loads instead of occupying vector registers.
@@ -784,49 +786,136 @@ __device__ __forceinline__ int job_of_tile(const LiftJob* jobs, int njobs, int64
   return __builtin_amdgcn_readfirstlane(lo);
 }
 
-__global__ __launch_bounds__(NT) void k_blift_part(const LiftJob* __restrict__ jobs, int njobs, int t, int tm,
-                                                    int64_t nbound, LiftBound* bounds) {
-  const int lane = threadIdx.x & 63;
-  const int64_t gb = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
-  if (gb >= nbound) return;  // uniform per wave
-  // job: the last one with b0 = tile0 + index <= gb
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].tile0 + mid <= gb)
-      lo = mid;
-    else
-      hi = mid - 1;
+// Prep of a batch lift / step.  Block 0: per job the input count (its count
+// cell, else the staged n), tiles = ceil(n / tm), tile0 by an exclusive scan
+// in job order, the select problem's tile geometry and candidate arrays, the
+// header totals; it also clears the jobs' stats and select states.  Other
+// blocks clear the jobs' digit-1 / digit-2 histograms.  A total above the
+// staged tile capacity runs nothing (nrun = 0) and reports ERR_OVERFLOW in
+// every job's stats.
+__global__ __launch_bounds__(NTB) void k_bprep(LiftJob* __restrict__ jobs, SelProblem* __restrict__ probs,
+                                               BatchHdr* hdr, int tm, SelBase sb, BatchZero z) {
+  const int njobs = hdr->njobs;
+  if (blockIdx.x > 0) {
+    if (!probs) return;
+    const int64_t words = (int64_t)njobs * HIST_BINS;  // per histogram array
+    const int64_t stride = (int64_t)(gridDim.x - 1) * NTB;
+    for (int64_t i = (int64_t)(blockIdx.x - 1) * NTB + threadIdx.x; i < words; i += stride) {
+      z.hist1[i] = 0u;
+      z.hist2[i] = 0u;
+    }
+    return;
   }
-  lo = __builtin_amdgcn_readfirstlane(lo);
-  const LiftJob& J = jobs[lo];
-  const int64_t u = gb - (J.tile0 + lo);
-  const LiftBound r = lift_boundary(J.x, J.n, J.j, t, u * (int64_t)tm);
-  if (lane == 0) bounds[gb] = r;
+  __shared__ int s_scan[WAVESB];
+  __shared__ int s_max;
+  if (probs) {
+    const int64_t stw = (int64_t)njobs * (int64_t)(sizeof(OpStats) / 8);
+    const int64_t slw = (int64_t)njobs * (int64_t)(sizeof(SelState) / 4);
+    for (int64_t i = threadIdx.x; i < stw; i += NTB) ((unsigned long long*)z.st)[i] = 0ull;
+    for (int64_t i = threadIdx.x; i < slw; i += NTB) ((unsigned int*)z.sel)[i] = 0u;
+  }
+  if (threadIdx.x == 0) s_max = 0;
+  __syncthreads();
+  int64_t carry = 0;
+  for (int c0 = 0; c0 < njobs; c0 += NTB) {
+    const int c = c0 + (int)threadIdx.x;
+    int64_t n = 0;
+    int tiles = 0;
+    if (c < njobs) {
+      LiftJob& J = jobs[c];
+      n = J.cnt_in ? *J.cnt_in : J.n;
+      if (n < 0) n = 0;
+      J.n = n;
+      tiles = (int)((n + tm - 1) / tm);
+      if (!probs && J.cnt_out) *J.cnt_out = n;  // a move keeps the count
+      atomicMax(&s_max, tiles);
+    }
+    int tot;
+    const int pre = block_excl_scan<NTB>(tiles, tot, s_scan);
+    if (c < njobs) {
+      const int64_t t0 = carry + pre;
+      jobs[c].tile0 = t0;
+      if (probs) {
+        SelProblem& P = probs[c];
+        P.tile0 = t0;
+        P.ntiles = tiles;
+        P.row0 = t0;
+        P.nrows = tiles;
+        P.npsum = tiles;
+        P.c2p = sb.c2p + t0 * SEG_ENTRIES;
+        P.c2i = sb.c2i + t0 * SEG_ENTRIES;
+        P.tk = sb.tk + t0 * SEG_ENTRIES;
+      }
+    }
+    carry += tot;
+  }
+  __syncthreads();
+  const bool over = carry > hdr->cap_tiles;
+  if (over && probs)
+    for (int c = threadIdx.x; c < njobs; c += NTB) z.st[c].err = ERR_OVERFLOW;
+  if (threadIdx.x == 0) {
+    hdr->nrun = over ? 0 : njobs;
+    hdr->ntiles = over ? 0 : carry;
+    hdr->nbound = over ? 0 : carry + njobs;
+    hdr->max_rows = over ? 0 : s_max;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_blift_part(const LiftJob* __restrict__ jobs, const BatchHdr* __restrict__ hdr,
+                                                    int t, int tm, LiftBound* bounds) {
+  const int lane = threadIdx.x & 63;
+  const int njobs = hdr->nrun;
+  const int64_t nbound = hdr->nbound;
+  // one wave per boundary (uniform per wave)
+  for (int64_t gb = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6); gb < nbound;
+       gb += (int64_t)gridDim.x * WAVES) {
+    // job: the last one with b0 = tile0 + index <= gb
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].tile0 + mid <= gb)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const LiftJob& J = jobs[lo];
+    const int64_t u = gb - (J.tile0 + lo);
+    const LiftBound r = lift_boundary(J.x, J.n, J.j, t, u * (int64_t)tm);
+    if (lane == 0) bounds[gb] = r;
+  }
 }
 
 template <int QB, int MODE>
-__global__ __launch_bounds__(NT) void k_blift_gate(const LiftJob* __restrict__ jobs, int njobs,
+__global__ __launch_bounds__(NT) void k_blift_gate(const LiftJob* __restrict__ jobs, const BatchHdr* __restrict__ hdr,
                                                     const LiftBound* __restrict__ bounds, uint64_t* __restrict__ xo,
                                                     double2* __restrict__ ao, unsigned int* hist, double* tile_psum,
                                                     int* seg_cnt) {
-  const int64_t gt = blockIdx.x;
-  const int c = job_of_tile(jobs, njobs, gt);
-  const LiftJob& J = jobs[c];
-  const int64_t u = gt - J.tile0;
-  const int64_t b = J.tile0 + c + u;
-  lift_gate_tile<QB, MODE>(J.x, J.a, J.n, J.j, bounds[b], bounds[b + 1], J.g, u, gt, xo, ao,
-                           hist + (int64_t)c * HIST_BINS, tile_psum, seg_cnt);
+  const int njobs = hdr->nrun;
+  const int64_t ntiles = hdr->ntiles;
+  for (int64_t gt = blockIdx.x; gt < ntiles; gt += gridDim.x) {
+    const int c = job_of_tile(jobs, njobs, gt);
+    const LiftJob& J = jobs[c];
+    const int64_t u = gt - J.tile0;
+    const int64_t b = J.tile0 + c + u;
+    lift_gate_tile<QB, MODE>(J.x, J.a, J.n, J.j, bounds[b], bounds[b + 1], J.g, u, gt, xo, ao,
+                             hist + (int64_t)c * HIST_BINS, tile_psum, seg_cnt);
+    __syncthreads();  // the next tile reuses the LDS
+  }
 }
 
-__global__ __launch_bounds__(NT) void k_blift_move(const LiftJob* __restrict__ jobs, int njobs,
+__global__ __launch_bounds__(NT) void k_blift_move(const LiftJob* __restrict__ jobs, const BatchHdr* __restrict__ hdr,
                                                     const LiftBound* __restrict__ bounds) {
-  const int64_t gt = blockIdx.x;
-  const int c = job_of_tile(jobs, njobs, gt);
-  const LiftJob& J = jobs[c];
-  const int64_t u = gt - J.tile0;
-  const int64_t b = J.tile0 + c + u;
-  lift_move_tile<1024>(J.x, J.a, J.n, J.j, bounds[b], bounds[b + 1], u, J.xo, J.ao);
+  const int njobs = hdr->nrun;
+  const int64_t ntiles = hdr->ntiles;
+  for (int64_t gt = blockIdx.x; gt < ntiles; gt += gridDim.x) {
+    const int c = job_of_tile(jobs, njobs, gt);
+    const LiftJob& J = jobs[c];
+    const int64_t u = gt - J.tile0;
+    const int64_t b = J.tile0 + c + u;
+    lift_move_tile<1024>(J.x, J.a, J.n, J.j, bounds[b], bounds[b + 1], u, J.xo, J.ao);
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------ single-qubit RDMs ------
@@ -1126,35 +1215,40 @@ int64_t lift_bounds_entries(int64_t nbound) { return nbound + lift_coarse_count(
 
 int lift_tile_entries(int qb, int mode) { return mode == MODE_MOVE ? 1024 : SEG_ENTRIES / (1 << qb); }
 
-// Batched lift: jobs (device) with tile0 assigned in order; ntiles_total output
-// tiles, njobs + ntiles_total boundaries.
-int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, int njobs, int64_t ntiles_total, LiftBound* bounds,
-                      uint64_t* xo, double2* ao, unsigned int* hist, double* tile_psum, int* seg_cnt,
-                      hipStream_t s) {
-  if (njobs <= 0 || ntiles_total <= 0) return 0;
+// Batched lift: geometry in hdr (k_bprep first), `grid` persistent blocks.
+int launch_batch_prep(LiftJob* d_jobs, SelProblem* d_probs, BatchHdr* hdr, int tm, const SelBase& sb,
+                      const BatchZero& z, hipStream_t s) {
+  hipLaunchKernelGGL(k_bprep, dim3(d_probs ? 65 : 1), dim3(NTB), 0, s, d_jobs, d_probs, hdr, tm, sb, z);
+  return (int)hipGetLastError();
+}
+
+int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, const BatchHdr* hdr, int64_t tiles, int jobs,
+                      LiftBound* bounds, uint64_t* xo, double2* ao, unsigned int* hist, double* tile_psum,
+                      int* seg_cnt, hipStream_t s) {
   const int tm = lift_tile_entries(qb, mode);
   const int t = (mode == MODE_MOVE) ? 0 : qb - 1;
-  const int64_t nbound = ntiles_total + njobs;
-  hipLaunchKernelGGL(k_blift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES)), dim3(NT), 0, s, d_jobs, njobs, t,
-                     tm, nbound, bounds);
-  const dim3 grid((unsigned)ntiles_total);
+  // boundaries: one wave each, WAVES per block
+  const dim3 gp(persist_grid(reinterpret_cast<const void*>(&k_blift_part), NT, (tiles + jobs + WAVES) / WAVES));
+  hipLaunchKernelGGL(k_blift_part, gp, dim3(NT), 0, s, d_jobs, hdr, t, tm, bounds);
+#define BASS_BLG(QB_, MODE_)                                                                                  \
+  hipLaunchKernelGGL((k_blift_gate<QB_, MODE_>),                                                               \
+                     dim3(persist_grid(reinterpret_cast<const void*>(&k_blift_gate<QB_, MODE_>), NT, tiles)), \
+                     dim3(NT), 0, s, d_jobs, hdr, bounds, xo, ao, hist, tile_psum, seg_cnt)
   if (mode == MODE_MOVE) {
-    hipLaunchKernelGGL(k_blift_move, grid, dim3(NT), 0, s, d_jobs, njobs, bounds);
+    hipLaunchKernelGGL(k_blift_move, dim3(persist_grid(reinterpret_cast<const void*>(&k_blift_move), NT, tiles)),
+                       dim3(NT), 0, s, d_jobs, hdr, bounds);
   } else if (qb == 1) {
     if (mode == MODE_GATE)
-      hipLaunchKernelGGL((k_blift_gate<1, MODE_GATE>), grid, dim3(NT), 0, s, d_jobs, njobs, bounds, xo, ao, hist,
-                         tile_psum, seg_cnt);
+      BASS_BLG(1, MODE_GATE);
     else
-      hipLaunchKernelGGL((k_blift_gate<1, MODE_ROT>), grid, dim3(NT), 0, s, d_jobs, njobs, bounds, xo, ao, hist,
-                         tile_psum, seg_cnt);
+      BASS_BLG(1, MODE_ROT);
   } else {
     if (mode == MODE_GATE)
-      hipLaunchKernelGGL((k_blift_gate<2, MODE_GATE>), grid, dim3(NT), 0, s, d_jobs, njobs, bounds, xo, ao, hist,
-                         tile_psum, seg_cnt);
+      BASS_BLG(2, MODE_GATE);
     else
-      hipLaunchKernelGGL((k_blift_gate<2, MODE_ROT>), grid, dim3(NT), 0, s, d_jobs, njobs, bounds, xo, ao, hist,
-                         tile_psum, seg_cnt);
+      BASS_BLG(2, MODE_ROT);
   }
+#undef BASS_BLG
   return (int)hipGetLastError();
 }
 

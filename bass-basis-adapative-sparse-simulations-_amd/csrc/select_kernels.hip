@@ -28,6 +28,9 @@
 // Everything else touches only candidates or per-tile scalars.
 #include <string.h>
 
+#include <mutex>
+#include <unordered_map>
+
 #include "common.h"
 #include "extract_core.h"
 #include "kernels.h"
@@ -244,14 +247,12 @@ __global__ __launch_bounds__(NT) void k_hist_psum(const SelProblem* probs, int n
 }
 
 // Digit 1: find the threshold bucket; also reduce the input's sum |a|^2.
-template <bool BATCH>
-__global__ __launch_bounds__(NTB) void k_find1(const SelProblem* probs, SelProblem one, SelTiles T) {
+__device__ __forceinline__ void find1_body(const SelProblem& P, const SelTiles& T) {
   __shared__ int s_scan[WAVESB];
   __shared__ double s_red[WAVESB];
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_above;
   __shared__ unsigned char s_q[64];
-  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
   // the select's zeroed header (digit-2 and wide histograms, state, tickets),
   // when the caller leaves it to this first kernel; the barrier below orders
   // it before this kernel's own writes to the state
@@ -303,6 +304,29 @@ __global__ __launch_bounds__(NTB) void k_find1(const SelProblem* probs, SelProbl
   }
 }
 
+// Per-problem kernels of a batch walk the problems with a persistent grid
+// (BatchHdr::nrun); a single problem is its block's.  The barrier after each
+// problem keeps the next one's LDS writes behind this one's reads.
+#define BASS_PER_PROBLEM(BATCH, call_batch, call_one)               \
+  do {                                                              \
+    if constexpr (BATCH) {                                          \
+      const int np_ = hdr->nrun;                                    \
+      for (int c_ = blockIdx.x; c_ < np_; c_ += gridDim.x) {        \
+        const SelProblem& P_ = probs[c_];                           \
+        call_batch;                                                 \
+        __syncthreads();                                            \
+      }                                                             \
+    } else {                                                        \
+      call_one;                                                     \
+    }                                                               \
+  } while (0)
+
+template <bool BATCH>
+__global__ __launch_bounds__(NTB) void k_find1(const SelProblem* probs, SelProblem one, SelTiles T,
+                                               const BatchHdr* hdr) {
+  BASS_PER_PROBLEM(BATCH, find1_body(P_, T), find1_body(one, T));
+}
+
 // Amplitudes of tile gt (slots past its count read its first entry, unused).
 __device__ __forceinline__ void ex1_load(const SelProblem& P, int64_t gt, double (&re)[SEL_ITEMS],
                                          double (&im)[SEL_ITEMS], int& cnt) {
@@ -332,14 +356,14 @@ __device__ __forceinline__ void ex1_tile(const SelProblem& P, const SelTiles& T,
 // The tile being processed is held as |a|^2 and only the tile in flight as
 // amplitudes (block barriers wait on LDS only).
 template <bool BATCH, int EXT>
-__global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
-                                                 int64_t tiles_end, int ext_rt) {
+__device__ __forceinline__ void extract1_body(const SelProblem* probs, int nprob, const SelProblem& one,
+                                              const SelTiles& T, int64_t tiles_end, int ext_rt, int64_t vb) {
   __shared__ unsigned int s_hist[HIST_BINS];
   __shared__ int s_wcnt[SEL_ITEMS * WAVES];
   __shared__ double s_red[WAVES];
   __shared__ int s_dcnt;
   const int ext = EXT > 0 ? EXT : ext_rt;
-  const int64_t gt0 = (int64_t)blockIdx.x * ext;
+  const int64_t gt0 = vb * ext;
   const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt0);
   const SelState* sel = P.sel;
   if (sel->keepall) {
@@ -395,7 +419,7 @@ __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int np
           atomicAdd(&s_hist[(unsigned int)(pb[u] >> 41) & (HIST_BINS - 1)], 1u);
     }
     __syncthreads();
-    unsigned short* row = T.hist2_rows + (int64_t)blockIdx.x * HIST_BINS;
+    unsigned short* row = T.hist2_rows + vb * HIST_BINS;
     for (int b = threadIdx.x; b < HIST_BINS; b += NT) row[b] = (unsigned short)s_hist[b];
     return;
   }
@@ -418,19 +442,34 @@ __global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int np
   __syncthreads();
   // counts <= EXT * SEL_TILE fit 16 bits; k_hist2_reduce sums the rows (about
   // half of a row's bins are nonzero, too many for per-bin global atomics)
-  unsigned short* row = T.hist2_rows + (int64_t)blockIdx.x * HIST_BINS;
+  unsigned short* row = T.hist2_rows + vb * HIST_BINS;
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) row[b] = (unsigned short)s_hist[b];
+}
+
+// Per-tile kernels of a batch walk the header's tiles (or waves of WAVES
+// tiles) with a persistent grid; a single problem's block is its index.
+template <bool BATCH, int EXT>
+__global__ __launch_bounds__(NT) void k_extract1(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
+                                                 int64_t tiles_end, int ext_rt, const BatchHdr* hdr) {
+  if constexpr (BATCH) {
+    const int np = hdr->nrun;
+    const int64_t end = hdr->ntiles;  // one tile per row (EXT = 1)
+    for (int64_t vb = blockIdx.x; vb < end; vb += gridDim.x) {
+      extract1_body<BATCH, EXT>(probs, np, one, T, end, ext_rt, vb);
+      __syncthreads();
+    }
+  } else {
+    extract1_body<BATCH, EXT>(probs, nprob, one, T, tiles_end, ext_rt, blockIdx.x);
+  }
 }
 
 // Column sums of a problem's digit-2 rows: thread = 4 adjacent bins (one
 // 8-byte load per row), block = H2_ROWS rows; blockIdx.z = problem.
 constexpr int H2_ROWS = 32;
-template <bool BATCH>
-__global__ __launch_bounds__(NT) void k_hist2_reduce(const SelProblem* probs, SelProblem one, SelTiles T) {
-  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.z);
+__device__ __forceinline__ void hist2_reduce_body(const SelProblem& P, const SelTiles& T, int bx, int64_t by) {
   if (P.sel->keepall) return;
-  const int b = (blockIdx.x * NT + threadIdx.x) * 4;
-  const int64_t r0 = (int64_t)blockIdx.y * H2_ROWS;
+  const int b = (bx * NT + threadIdx.x) * 4;
+  const int64_t r0 = by * H2_ROWS;
   if (r0 >= P.nrows) return;
   const int64_t r1 = r0 + H2_ROWS < P.nrows ? r0 + H2_ROWS : P.nrows;
   unsigned int s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -448,12 +487,29 @@ __global__ __launch_bounds__(NT) void k_hist2_reduce(const SelProblem* probs, Se
   if (s3) atomicAdd(&P.hist2[b + 3], s3);
 }
 
+// A batch flattens (bin group, row block, problem) into one persistent index.
 template <bool BATCH>
-__global__ __launch_bounds__(NT) void k_find2(const SelProblem* probs, SelProblem one) {
+__global__ __launch_bounds__(NT) void k_hist2_reduce(const SelProblem* probs, SelProblem one, SelTiles T,
+                                                     const BatchHdr* hdr) {
+  if constexpr (BATCH) {
+    constexpr int BX = HIST_BINS / (4 * NT);
+    const int64_t ry = (hdr->max_rows + H2_ROWS - 1) / H2_ROWS;
+    const int64_t total = (int64_t)BX * ry * hdr->nrun;
+    for (int64_t v = blockIdx.x; v < total; v += gridDim.x) {
+      const int bx = (int)(v % BX);
+      const int64_t by = (v / BX) % ry;
+      const int64_t c = v / (BX * ry);
+      hist2_reduce_body(probs[c], T, bx, by);
+    }
+  } else {
+    hist2_reduce_body(one, T, blockIdx.x, blockIdx.y);
+  }
+}
+
+__device__ __forceinline__ void find2_body(const SelProblem& P) {
   __shared__ int s_scan[WAVES];
   __shared__ unsigned int s_b;
   __shared__ unsigned long long s_above;
-  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
   SelState* sel = P.sel;
   if (threadIdx.x == 0) {
     s_b = 0xFFFFFFFFu;
@@ -471,13 +527,18 @@ __global__ __launch_bounds__(NT) void k_find2(const SelProblem* probs, SelProble
   }
 }
 
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_find2(const SelProblem* probs, SelProblem one, const BatchHdr* hdr) {
+  BASS_PER_PROBLEM(BATCH, find2_body(P_), find2_body(one));
+}
+
 // Candidates of the digit-2 bucket, appended to the problem's list in any
 // order (the threshold does not depend on their order).  One wave per tile (a
 // tile holds a few hundred candidates; whole blocks would mostly wait).
 template <bool BATCH>
-__global__ __launch_bounds__(NT) void k_extract2(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
-                                                 int64_t tiles_end) {
-  const int64_t gt = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+__device__ __forceinline__ void extract2_body(const SelProblem* probs, int nprob, const SelProblem& one,
+                                              const SelTiles& T, int64_t tiles_end, int64_t vb) {
+  const int64_t gt = vb * WAVES + (threadIdx.x >> 6);
   if (gt >= tiles_end) return;  // uniform per wave
   const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt);
   SelState* sel = P.sel;
@@ -512,6 +573,20 @@ __global__ __launch_bounds__(NT) void k_extract2(const SelProblem* probs, int np
         }
       }
     }
+  }
+}
+
+// Batch form: the header's tiles, WAVES per virtual block (no block barriers:
+// a wave leaves a tile past the end on its own).
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_extract2(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
+                                                 int64_t tiles_end, const BatchHdr* hdr) {
+  if constexpr (BATCH) {
+    const int np = hdr->nrun;
+    const int64_t end = hdr->ntiles;
+    for (int64_t vb = blockIdx.x; vb * WAVES < end; vb += gridDim.x) extract2_body<BATCH>(probs, np, one, T, end, vb);
+  } else {
+    extract2_body<BATCH>(probs, nprob, one, T, tiles_end, blockIdx.x);
   }
 }
 
@@ -594,11 +669,16 @@ __global__ __launch_bounds__(NT) void k_extract2_agg(SelProblem P, SelTiles T, i
 __device__ __constant__ int kWideShift[WIDE_LEVELS] = {30, 19, 8, 0, 53, 42, 31, 20, 9, 0};
 __device__ __constant__ int kWideWidth[WIDE_LEVELS] = {11, 11, 11, 8, 11, 11, 11, 11, 11, 9};
 
-// Resolve the remaining bits of the threshold from the (small) digit-2
-// candidate list, then the key tie-break among entries equal to it.  One block
-// of NTB threads per problem; the logical keys of the tied entries are
-// gathered once into LDS when they fit (FIN_TIES), else re-read per level.
+// Resolve the remaining bits of the threshold from the digit-2 candidate
+// list, then the key tie-break among entries equal to it.  One block of NTB
+// threads per problem, FIN_U candidate loads in flight per thread, histogram
+// increments of a wave's lanes that share a bin (exact ties) as one LDS
+// atomic.  The logical keys of the tied entries are gathered once: into LDS
+// when they fit (FIN_TIES), else into the problem's key list P.tk (through the
+// key map's byte tables, built in LDS), whose radix passes take key digits of
+// at most 11 bits sized to the key width.
 constexpr int FIN_TIES = 2048;
+constexpr int FIN_U = 8;
 struct FinLds {
   unsigned int hist[HIST_BINS];
   int scan[WAVESB];
@@ -606,7 +686,22 @@ struct FinLds {
   unsigned long long ab;
   uint64_t keys[FIN_TIES];
   unsigned int nk;
+  uint64_t ktab[KTAB_WORDS];  // byte tables of the key map (the tied-key list path)
 };
+
+// Byte tables of km into LDS tab (the block's threads; km.nbits > 0).
+__device__ __forceinline__ void ktab_lds(const KeyMap& km, uint64_t* tab) {
+  for (int e = threadIdx.x; e < KTAB_WORDS; e += NTB) {
+    const int bb = e >> 8, v = e & 255;
+    uint64_t r = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int p = 8 * bb + i;
+      if (((v >> i) & 1) && p < km.nbits) r |= 1ull << km.q[p];
+    }
+    tab[e] = r;
+  }
+}
 
 // The single-block resolution of problem P (every thread of the block calls it).
 __device__ __forceinline__ void finish_block(const SelProblem& P, FinLds& S) {
@@ -615,6 +710,7 @@ __device__ __forceinline__ void finish_block(const SelProblem& P, FinLds& S) {
   const unsigned int m2 = sel->cand2;
   const unsigned long long* c2p = P.c2p;
   const unsigned int* c2i = P.c2i;
+  const int tid = threadIdx.x;
   unsigned long long need = sel->need;
   unsigned long long prefix = sel->prefix;
   const int shifts[4] = {30, 19, 8, 0};
@@ -622,22 +718,31 @@ __device__ __forceinline__ void finish_block(const SelProblem& P, FinLds& S) {
   unsigned int cnt_eq = 0;
   for (int lv = 0; lv < 4; ++lv) {
     const int sh = shifts[lv], wd = widths[lv];
-    for (int b = threadIdx.x; b < HIST_BINS; b += NTB) S.hist[b] = 0;
-    if (threadIdx.x == 0) {
+    for (int b = tid; b < HIST_BINS; b += NTB) S.hist[b] = 0;
+    if (tid == 0) {
       S.b = 0xFFFFFFFFu;
       S.ab = 0;
     }
     __syncthreads();
     const unsigned long long hi = prefix >> (sh + wd);
-    for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
-      const unsigned long long bits = c2p[j];
-      if ((bits >> (sh + wd)) == hi)
-        atomicAdd(&S.hist[(unsigned int)(bits >> sh) & ((1u << wd) - 1u)], 1u);
+    const unsigned int mask = (1u << wd) - 1u;
+    for (unsigned int j0 = 0; j0 < m2; j0 += NTB * FIN_U) {
+      unsigned long long bv[FIN_U];
+#pragma unroll
+      for (int u = 0; u < FIN_U; ++u) {
+        const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+        bv[u] = c2p[j < m2 ? j : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < FIN_U; ++u) {
+        const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+        wave_hist_add(S.hist, (unsigned int)(bv[u] >> sh) & mask, j < m2 && (bv[u] >> (sh + wd)) == hi);
+      }
     }
     __syncthreads();
     find_from_top<NTB>(S.hist, need, &S.b, &S.ab, S.scan);
     if (S.b == 0xFFFFFFFFu) {
-      if (threadIdx.x == 0) set_err(&st->err, ERR_SELECT);
+      if (tid == 0) set_err(&st->err, ERR_SELECT);
       return;
     }
     prefix |= (unsigned long long)S.b << sh;
@@ -648,63 +753,128 @@ __device__ __forceinline__ void finish_block(const SelProblem& P, FinLds& S) {
   const unsigned long long pstar = prefix;
   unsigned long long keystar = ~0ull;
   if (need < cnt_eq) {
-    // need-th smallest key among entries whose pattern equals pstar
+    // need-th smallest logical key among entries whose pattern equals pstar
     const bool in_lds = cnt_eq <= (unsigned int)FIN_TIES;
-    if (in_lds) {
-      if (threadIdx.x == 0) S.nk = 0;
-      __syncthreads();
-      for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
-        if (c2p[j] != pstar) continue;
-        const unsigned int slot = atomicAdd(&S.nk, 1u);
-        if (slot < (unsigned int)FIN_TIES) S.keys[slot] = lkey(P, P.x[c2i[j]]);
+    const bool tab = !in_lds && P.km.nbits > 0;
+    if (tab) ktab_lds(P.km, S.ktab);
+    if (tid == 0) S.nk = 0;
+    __syncthreads();
+    unsigned int base = 0;  // list path: keys written so far (the same in every thread)
+    for (unsigned int j0 = 0; j0 < m2; j0 += NTB * FIN_U) {
+      unsigned long long bv[FIN_U];
+#pragma unroll
+      for (int u = 0; u < FIN_U; ++u) {
+        const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+        bv[u] = c2p[j < m2 ? j : 0];
       }
-      __syncthreads();
-    }
-    const unsigned int nk = in_lds ? S.nk : 0u;
-    unsigned long long kp = 0;
-    for (int sh = 56; sh >= 0; sh -= 8) {
-      for (int b = threadIdx.x; b < HIST_BINS; b += NTB) S.hist[b] = 0;
-      if (threadIdx.x == 0) {
-        S.b = 0xFFFFFFFFu;
-        S.ab = 0;
-      }
-      __syncthreads();
       if (in_lds) {
-        for (unsigned int e = threadIdx.x; e < nk; e += NTB) {
+#pragma unroll
+        for (int u = 0; u < FIN_U; ++u) {
+          const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+          if (j < m2 && bv[u] == pstar) {
+            const unsigned int slot = atomicAdd(&S.nk, 1u);
+            if (slot < (unsigned int)FIN_TIES) S.keys[slot] = lkey(P, P.x[c2i[j]]);
+          }
+        }
+      } else {
+        // the round's tied candidates at consecutive list slots (a block scan)
+        int c = 0;
+#pragma unroll
+        for (int u = 0; u < FIN_U; ++u) {
+          const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+          c += (j < m2 && bv[u] == pstar) ? 1 : 0;
+        }
+        int tot;
+        const int pre = block_excl_scan<NTB>(c, tot, S.scan);
+        unsigned int off = base + (unsigned int)pre;
+#pragma unroll
+        for (int u = 0; u < FIN_U; ++u) {
+          const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+          if (j < m2 && bv[u] == pstar) {
+            const uint64_t w = P.x[c2i[j]];
+            P.tk[off++] = tab ? logical_key_tab(w, S.ktab) : w;
+          }
+        }
+        base += (unsigned int)tot;
+      }
+    }
+    __syncthreads();
+    const unsigned int nk = in_lds ? (S.nk < (unsigned int)FIN_TIES ? S.nk : (unsigned int)FIN_TIES) : base;
+    unsigned long long kp = 0;
+    if (in_lds) {
+      for (int sh = 56; sh >= 0; sh -= 8) {
+        for (int b = tid; b < HIST_BINS; b += NTB) S.hist[b] = 0;
+        if (tid == 0) {
+          S.b = 0xFFFFFFFFu;
+          S.ab = 0;
+        }
+        __syncthreads();
+        for (unsigned int e = tid; e < nk; e += NTB) {
           const uint64_t key = S.keys[e];
           const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
           if (match) atomicAdd(&S.hist[(unsigned int)(key >> sh) & 255u], 1u);
         }
-      } else {
-        for (unsigned int j = threadIdx.x; j < m2; j += NTB) {
-          if (c2p[j] != pstar) continue;
-          const uint64_t key = lkey(P, P.x[c2i[j]]);
-          const bool match = (sh == 56) ? true : ((key >> (sh + 8)) == (kp >> (sh + 8)));
-          if (match) atomicAdd(&S.hist[(unsigned int)(key >> sh) & 255u], 1u);
+        __syncthreads();
+        find_from_bottom<NTB>(S.hist, need, &S.b, &S.ab, S.scan);
+        if (S.b == 0xFFFFFFFFu) {
+          if (tid == 0) set_err(&st->err, ERR_SELECT);
+          return;
         }
+        kp |= (unsigned long long)S.b << sh;
+        need -= S.ab;
+        __syncthreads();
       }
-      __syncthreads();
-      find_from_bottom<NTB>(S.hist, need, &S.b, &S.ab, S.scan);
-      if (S.b == 0xFFFFFFFFu) {
-        if (threadIdx.x == 0) set_err(&st->err, ERR_SELECT);
-        return;
+    } else {
+      // the list's keys have kb significant bits: the fewest levels of at
+      // most 11 bits, from the top
+      const int kb = P.km.nbits ? P.km.nbits : 64;
+      const int nlev = (kb + 10) / 11;
+      const int w = (kb + nlev - 1) / nlev;
+      for (int lv = 0; lv < nlev; ++lv) {
+        const int hi = kb - lv * w;
+        const int lo = hi - w > 0 ? hi - w : 0;
+        const unsigned int mask = (1u << (hi - lo)) - 1u;
+        for (int b = tid; b < HIST_BINS; b += NTB) S.hist[b] = 0;
+        if (tid == 0) {
+          S.b = 0xFFFFFFFFu;
+          S.ab = 0;
+        }
+        __syncthreads();
+        for (unsigned int j0 = 0; j0 < nk; j0 += NTB * FIN_U) {
+          uint64_t kv[FIN_U];
+#pragma unroll
+          for (int u = 0; u < FIN_U; ++u) {
+            const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+            kv[u] = P.tk[j < nk ? j : 0];
+          }
+#pragma unroll
+          for (int u = 0; u < FIN_U; ++u) {
+            const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
+            wave_hist_add(S.hist, (unsigned int)(kv[u] >> lo) & mask,
+                          j < nk && (hi >= 64 || (kv[u] >> hi) == (kp >> hi)));
+          }
+        }
+        __syncthreads();
+        find_from_bottom<NTB>(S.hist, need, &S.b, &S.ab, S.scan);
+        if (S.b == 0xFFFFFFFFu) {
+          if (tid == 0) set_err(&st->err, ERR_SELECT);
+          return;
+        }
+        kp |= (unsigned long long)S.b << lo;
+        need -= S.ab;
+        __syncthreads();
       }
-      kp |= (unsigned long long)S.b << sh;
-      need -= S.ab;
-      __syncthreads();
     }
     keystar = kp;
   }
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     sel->pstar = pstar;
     sel->keystar = keystar;
   }
 }
 
-template <bool BATCH>
-__global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProblem one) {
+__device__ __forceinline__ void finish_body(const SelProblem& P) {
   __shared__ FinLds S;
-  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
   SelState* sel = P.sel;
   if (sel->keepall) return;  // pstar / keystar set by k_find1
   if (sel->cand2 > FIN_MAX && !P.finish_any) {
@@ -717,6 +887,11 @@ __global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProb
     return;
   }
   finish_block(P, S);
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(NTB) void k_finish(const SelProblem* probs, SelProblem one, const BatchHdr* hdr) {
+  BASS_PER_PROBLEM(BATCH, finish_body(P_), finish_body(one));
 }
 
 // ------------------------------------------------------ staged finish ----
@@ -776,7 +951,6 @@ __device__ __forceinline__ void stg_key_digit(int kb, int lv, int& lo, int& hi) 
 
 __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
   __shared__ FinLds S;
-  __shared__ uint64_t s_ktab[KTAB_WORDS];
   __shared__ unsigned int s_t, s_flag, s_base;
   SelState* sel = P.sel;
   OpStats* st = P.st;
@@ -875,7 +1049,7 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
     // counts); the key map's byte tables staged in LDS
     const bool tab = P.km.nbits > 0 && P.ktab;
     if (tab)
-      for (int e = tid; e < KTAB_WORDS; e += NTB) s_ktab[e] = P.ktab[e];
+      for (int e = tid; e < KTAB_WORDS; e += NTB) S.ktab[e] = P.ktab[e];
     __syncthreads();
     for (unsigned int j0 = (unsigned int)part * NTB * STG_U; j0 < m2; j0 += stride * STG_U) {
       unsigned long long bv[STG_U];
@@ -900,7 +1074,7 @@ __global__ __launch_bounds__(NTB) void k_finish_staged(SelProblem P) {
         const unsigned int j = j0 + (unsigned int)(u * NTB + tid);
         if (j < m2 && bv[u] == pstar) {
           const uint64_t w = P.x[P.c2i[j]];
-          P.tk[base + (unsigned int)off] = tab ? logical_key_tab(w, s_ktab) : logical_key(w, P.km);
+          P.tk[base + (unsigned int)off] = tab ? logical_key_tab(w, S.ktab) : logical_key(w, P.km);
           ++off;
         }
       }
@@ -1099,9 +1273,9 @@ __device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t w, u
 // Per tile: sum and count of the kept candidates (lane-strided, then a
 // fixed shuffle tree).  One wave per tile.
 template <bool BATCH>
-__global__ __launch_bounds__(NT) void k_keptsum(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
-                                                int64_t tiles_end) {
-  const int64_t gt = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+__device__ __forceinline__ void keptsum_body(const SelProblem* probs, int nprob, const SelProblem& one,
+                                             const SelTiles& T, int64_t tiles_end, int64_t vb) {
+  const int64_t gt = vb * WAVES + (threadIdx.x >> 6);
   if (gt >= tiles_end) return;  // uniform per wave
   const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt);
   const unsigned long long pstar = P.sel->pstar, keystar = P.sel->keystar;
@@ -1139,15 +1313,25 @@ __global__ __launch_bounds__(NT) void k_keptsum(const SelProblem* probs, int npr
   }
 }
 
+template <bool BATCH>
+__global__ __launch_bounds__(NT) void k_keptsum(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
+                                                int64_t tiles_end, const BatchHdr* hdr) {
+  if constexpr (BATCH) {
+    const int np = hdr->nrun;
+    const int64_t end = hdr->ntiles;
+    for (int64_t vb = blockIdx.x; vb * WAVES < end; vb += gridDim.x) keptsum_body<BATCH>(probs, np, one, T, end, vb);
+  } else {
+    keptsum_body<BATCH>(probs, nprob, one, T, tiles_end, blockIdx.x);
+  }
+}
+
 // kept = sum over the problem's tiles in a fixed order; kept-entry offsets
 // per tile (exclusive scan).  Thread i owns FIN_CH consecutive tiles per
 // round; its loads are issued together.
 constexpr int FIN_CH = 4;
-template <bool BATCH>
-__global__ __launch_bounds__(NTB) void k_sel_finalize(const SelProblem* probs, SelProblem one, SelTiles T) {
+__device__ __forceinline__ void sel_finalize_body(const SelProblem& P, const SelTiles& T) {
   __shared__ double s_red[WAVESB];
   __shared__ int s_scan[WAVESB];
-  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
   const int64_t t_end = P.tile0 + P.ntiles;
   double s = 0.0;
   int64_t carry = 0;
@@ -1195,6 +1379,12 @@ __global__ __launch_bounds__(NTB) void k_sel_finalize(const SelProblem* probs, S
       st->norm = (P.renorm && norm > 1e-30) ? norm : 1.0;
     }
   }
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(NTB) void k_sel_finalize(const SelProblem* probs, SelProblem one, SelTiles T,
+                                                      const BatchHdr* hdr) {
+  BASS_PER_PROBLEM(BATCH, sel_finalize_body(P_, T), sel_finalize_body(one, T));
 }
 
 // Chunked form of k_sel_finalize for one large problem, first level: block c
@@ -1279,10 +1469,10 @@ __global__ __launch_bounds__(NTB) void k_sel_final(SelProblem P, SelTiles T, int
 }
 
 template <bool BATCH>
-__global__ __launch_bounds__(NT) void k_compact(const SelProblem* probs, int nprob, SelProblem one, SelTiles T) {
+__device__ __forceinline__ void compact_body(const SelProblem* probs, int nprob, const SelProblem& one,
+                                             const SelTiles& T, int64_t gt) {
   __shared__ int s_scan[SEL_ITEMS * WAVES];
   __shared__ double s_red[WAVES];
-  const int64_t gt = blockIdx.x;
   const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt);
   const unsigned long long pstar = P.sel->pstar, keystar = P.sel->keystar;
   // numpy divides complex128 by a real norm as a product with 1/norm
@@ -1346,17 +1536,38 @@ __global__ __launch_bounds__(NT) void k_compact(const SelProblem* probs, int npr
   }
 }
 
-// Fixed-order reduction of a problem's per-tile sums into its stats.
 template <bool BATCH>
-__global__ __launch_bounds__(NTB) void k_sums_finalize(const SelProblem* probs, SelProblem one, SelTiles T) {
+__global__ __launch_bounds__(NT) void k_compact(const SelProblem* probs, int nprob, SelProblem one, SelTiles T,
+                                                const BatchHdr* hdr) {
+  if constexpr (BATCH) {
+    const int np = hdr->nrun;
+    const int64_t end = hdr->ntiles;
+    for (int64_t gt = blockIdx.x; gt < end; gt += gridDim.x) {
+      compact_body<BATCH>(probs, np, one, T, gt);
+      __syncthreads();
+    }
+  } else {
+    compact_body<BATCH>(probs, nprob, one, T, blockIdx.x);
+  }
+}
+
+// Fixed-order reduction of a problem's per-tile sums into its stats; a batch
+// problem with a count cell gets its output count there.
+__device__ __forceinline__ void sums_finalize_body(const SelProblem& P, const SelTiles& T) {
   __shared__ double s_red[WAVESB];
-  const SelProblem& P = block_problem<BATCH>(probs, one, blockIdx.x);
   const double s2 = block_sum_array<NTB>(T.tile_s2 + P.tile0, P.ntiles, s_red);
   const double s4 = block_sum_array<NTB>(T.tile_s4 + P.tile0, P.ntiles, s_red);
   if (threadIdx.x == 0) {
     P.st->sum2 = s2;
     P.st->sum4 = s4;
+    if (P.cnt_out) *P.cnt_out = P.st->nnz;
   }
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(NTB) void k_sums_finalize(const SelProblem* probs, SelProblem one, SelTiles T,
+                                                       const BatchHdr* hdr) {
+  BASS_PER_PROBLEM(BATCH, sums_finalize_body(P_, T), sums_finalize_body(one, T));
 }
 
 // Chunked form of k_sums_finalize: per-chunk sums of the output's |a|^2 and
@@ -1541,70 +1752,64 @@ __global__ __launch_bounds__(NT) void k_gather(const uint64_t* __restrict__ x, c
 
 // ----------------------------------------------------------- host side ----
 
-// The select sequence for nprob problems covering tiles [0, tiles_end).
-// BATCH = false: one problem by value (probs unused); extract1 groups 4 tiles
-// per block.
-template <bool BATCH>
-static void select_sequence(const SelProblem* probs, int nprob, const SelProblem& one, const SelTiles& T,
-                            int64_t tiles_end, int64_t max_rows, bool hist_ready, bool wide, hipStream_t s) {
+// The select of one problem (tiles_end = its tiles; extract1 groups several
+// tiles per block).
+static void select_one(const SelProblem& one, const SelTiles& T, int64_t tiles_end, bool hist_ready, bool wide,
+                       hipStream_t s) {
+  const SelProblem* probs = nullptr;
+  const BatchHdr* hdr = nullptr;
+  const int nprob = 1;
   if (!hist_ready)
-    hipLaunchKernelGGL((k_hist_psum<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
-  hipLaunchKernelGGL((k_find1<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
-  const int ext = BATCH ? 1 : extract1_tiles(tiles_end);
+    hipLaunchKernelGGL((k_hist_psum<false>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
+  hipLaunchKernelGGL((k_find1<false>), dim3(1), dim3(NTB), 0, s, probs, one, T, hdr);
+  const int ext = extract1_tiles(tiles_end);
   const int64_t rows = (tiles_end + ext - 1) / ext;
-  if (ext == 1)
-    hipLaunchKernelGGL((k_extract1<BATCH, 1>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one, T,
-                       tiles_end, 1);
-  else if (ext == EX1_TILES)
-    hipLaunchKernelGGL((k_extract1<BATCH, EX1_TILES>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one,
-                       T, tiles_end, ext);
+  if (ext == EX1_TILES)
+    hipLaunchKernelGGL((k_extract1<false, EX1_TILES>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one,
+                       T, tiles_end, ext, hdr);
   else
-    hipLaunchKernelGGL((k_extract1<BATCH, 0>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one, T,
-                       tiles_end, ext);
-  hipLaunchKernelGGL((k_hist2_reduce<BATCH>),
-                     dim3(HIST_BINS / (4 * NT), (unsigned)((max_rows + H2_ROWS - 1) / H2_ROWS), (unsigned)nprob),
-                     dim3(NT), 0, s, probs, one, T);
-  hipLaunchKernelGGL((k_find2<BATCH>), dim3((unsigned)nprob), dim3(NT), 0, s, probs, one);
+    hipLaunchKernelGGL((k_extract1<false, 0>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one, T,
+                       tiles_end, ext, hdr);
+  hipLaunchKernelGGL((k_hist2_reduce<false>),
+                     dim3(HIST_BINS / (4 * NT), (unsigned)((one.nrows + H2_ROWS - 1) / H2_ROWS), 1u), dim3(NT), 0,
+                     s, probs, one, T, hdr);
+  hipLaunchKernelGGL((k_find2<false>), dim3(1), dim3(NT), 0, s, probs, one, hdr);
   const unsigned wave_tiles = (unsigned)((tiles_end + WAVES - 1) / WAVES);
-  if (BATCH) {
-    hipLaunchKernelGGL((k_extract2<BATCH>), dim3(wave_tiles), dim3(NT), 0, s, probs, nprob, one, T, tiles_end);
-  } else {
+  {
     // 8 tiles per wave from 16384 tiles on (every wave still resident at once)
     const int tpw = tiles_end >= 16384 ? 8 : 1;
     const int64_t waves = (tiles_end + tpw - 1) / tpw;
     hipLaunchKernelGGL(k_extract2_agg, dim3((unsigned)((waves + WAVES - 1) / WAVES)), dim3(NT), 0, s, one, T,
                        tiles_end, tpw);
   }
-  if (!wide && !BATCH) {
+  if (!wide) {
     hipLaunchKernelGGL(k_finish_staged, dim3(STG_STAGES * STG_BLOCKS), dim3(NTB), 0, s, one);
-  } else if (!wide) {
-    hipLaunchKernelGGL((k_finish<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one);
   } else {
-    const dim3 wg(WIDE_BLOCKS, (unsigned)nprob);
+    const dim3 wg(WIDE_BLOCKS, 1u);
     for (int lv = 0; lv < 4; ++lv) {
-      hipLaunchKernelGGL((k_wide_hist<BATCH>), wg, dim3(NT), 0, s, probs, one, lv);
-      hipLaunchKernelGGL((k_wide_find<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, lv);
+      hipLaunchKernelGGL((k_wide_hist<false>), wg, dim3(NT), 0, s, probs, one, lv);
+      hipLaunchKernelGGL((k_wide_find<false>), dim3(1), dim3(NTB), 0, s, probs, one, lv);
     }
-    hipLaunchKernelGGL((k_wide_collect<BATCH>), wg, dim3(NT), 0, s, probs, one);
+    hipLaunchKernelGGL((k_wide_collect<false>), wg, dim3(NT), 0, s, probs, one);
     for (int lv = 4; lv < WIDE_LEVELS; ++lv) {
-      hipLaunchKernelGGL((k_wide_key_hist<BATCH>), wg, dim3(NT), 0, s, probs, one, lv);
-      hipLaunchKernelGGL((k_wide_find<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, lv);
+      hipLaunchKernelGGL((k_wide_key_hist<false>), wg, dim3(NT), 0, s, probs, one, lv);
+      hipLaunchKernelGGL((k_wide_find<false>), dim3(1), dim3(NTB), 0, s, probs, one, lv);
     }
   }
-  hipLaunchKernelGGL((k_keptsum<BATCH>), dim3(wave_tiles), dim3(NT), 0, s, probs, nprob, one, T, tiles_end);
+  hipLaunchKernelGGL((k_keptsum<false>), dim3(wave_tiles), dim3(NT), 0, s, probs, nprob, one, T, tiles_end, hdr);
   const int64_t nch = sel_chunks(one.npsum > one.ntiles ? one.npsum : one.ntiles);
-  if (!BATCH && one.chunked) {
+  if (one.chunked) {
     hipLaunchKernelGGL(k_sel_chunks, dim3((unsigned)nch), dim3(NTB), 0, s, one, T);
     hipLaunchKernelGGL(k_sel_final, dim3(1), dim3(NTB), 0, s, one, T, nch);
   } else {
-    hipLaunchKernelGGL((k_sel_finalize<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
+    hipLaunchKernelGGL((k_sel_finalize<false>), dim3(1), dim3(NTB), 0, s, probs, one, T, hdr);
   }
-  hipLaunchKernelGGL((k_compact<BATCH>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
-  if (!BATCH && one.chunked) {
+  hipLaunchKernelGGL((k_compact<false>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T, hdr);
+  if (one.chunked) {
     hipLaunchKernelGGL(k_sums_chunks, dim3((unsigned)sel_chunks(one.ntiles)), dim3(NTB), 0, s, one, T);
     hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(NTB), 0, s, one, T, sel_chunks(one.ntiles));
   } else {
-    hipLaunchKernelGGL((k_sums_finalize<BATCH>), dim3((unsigned)nprob), dim3(NTB), 0, s, probs, one, T);
+    hipLaunchKernelGGL((k_sums_finalize<false>), dim3(1), dim3(NTB), 0, s, probs, one, T, hdr);
   }
 }
 
@@ -1649,19 +1854,61 @@ int select_rows(int64_t ntiles, bool batch) {
 }
 
 int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s) {
-  select_sequence<false>(nullptr, 1, P, T, P.ntiles, P.nrows, hist_ready, wide, s);
+  select_one(P, T, P.ntiles, hist_ready, wide, s);
   return (int)hipGetLastError();
 }
 
-int launch_select_batch(const SelProblem* d_probs, int nprob, const SelTiles& T, int64_t tiles_end,
-                        int64_t max_rows, bool hist_ready, bool wide, hipStream_t s) {
+// The select of a batch of problems (digit-1 histograms filled by the gate
+// that produced the tiles), geometry from hdr; persistent grids sized by the
+// tile / problem bounds.
+int launch_select_batch(const SelProblem* d_probs, const BatchHdr* hdr, const SelTiles& T, int64_t tiles, int jobs,
+                        hipStream_t s) {
   SelProblem none;
   memset(&none, 0, sizeof(none));
-  select_sequence<true>(d_probs, nprob, none, T, tiles_end, max_rows, hist_ready, wide, s);
+  const int64_t wtiles = (tiles + WAVES - 1) / WAVES;
+  const int64_t h2 = (int64_t)(HIST_BINS / (4 * NT)) * (tiles / H2_ROWS + jobs + 1);
+#define BASS_PG(fn, thr, bound) dim3(persist_grid(reinterpret_cast<const void*>(&fn), thr, bound))
+  hipLaunchKernelGGL((k_find1<true>), BASS_PG(k_find1<true>, NTB, jobs), dim3(NTB), 0, s, d_probs, none, T, hdr);
+  hipLaunchKernelGGL((k_extract1<true, 1>), BASS_PG((k_extract1<true, 1>), NT, tiles), dim3(NT), 0, s, d_probs, 0,
+                     none, T, (int64_t)0, 1, hdr);
+  hipLaunchKernelGGL((k_hist2_reduce<true>), BASS_PG(k_hist2_reduce<true>, NT, h2), dim3(NT), 0, s, d_probs, none, T,
+                     hdr);
+  hipLaunchKernelGGL((k_find2<true>), BASS_PG(k_find2<true>, NT, jobs), dim3(NT), 0, s, d_probs, none, hdr);
+  hipLaunchKernelGGL((k_extract2<true>), BASS_PG(k_extract2<true>, NT, wtiles), dim3(NT), 0, s, d_probs, 0, none, T,
+                     (int64_t)0, hdr);
+  hipLaunchKernelGGL((k_finish<true>), BASS_PG(k_finish<true>, NTB, jobs), dim3(NTB), 0, s, d_probs, none, hdr);
+  hipLaunchKernelGGL((k_keptsum<true>), BASS_PG(k_keptsum<true>, NT, wtiles), dim3(NT), 0, s, d_probs, 0, none, T,
+                     (int64_t)0, hdr);
+  hipLaunchKernelGGL((k_sel_finalize<true>), BASS_PG(k_sel_finalize<true>, NTB, jobs), dim3(NTB), 0, s, d_probs,
+                     none, T, hdr);
+  hipLaunchKernelGGL((k_compact<true>), BASS_PG(k_compact<true>, NT, tiles), dim3(NT), 0, s, d_probs, 0, none, T,
+                     hdr);
+  hipLaunchKernelGGL((k_sums_finalize<true>), BASS_PG(k_sums_finalize<true>, NTB, jobs), dim3(NTB), 0, s, d_probs,
+                     none, T, hdr);
+#undef BASS_PG
   return (int)hipGetLastError();
 }
 
-
+// Blocks of a kernel the device holds at once (occupancy x compute units),
+// measured once per kernel.
+int resident_blocks(const void* kernel, int threads) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(kernel);
+  if (it != cache.end()) return it->second;
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || cus * per <= 0) {
+    (void)hipGetLastError();
+    cus = 256;
+    per = 2;
+  }
+  const int r = cus * per;
+  cache[kernel] = r;
+  return r;
+}
 
 int64_t sel_tiles(int64_t n) { return (n + SEL_TILE - 1) / SEL_TILE; }
 

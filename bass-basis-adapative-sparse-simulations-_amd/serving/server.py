@@ -174,6 +174,18 @@ def check_admissible(req, budget, slab_entries=None):
         raise RejectedRequest(f"request {req.rid}: K = 2^{req.log2k} exceeds the slab ({slab_entries} entries)")
 
 
+def _make_engine(cfg, device):
+    """The engine of a server config: PipelinedBatchEngine (HIP graphs unless
+    cfg["graphs"] is false) or, with cfg["pipelined"] false, BatchEngine."""
+    import importlib
+
+    Bm = importlib.import_module(PKG + ".simulation.batched")
+    if cfg.get("pipelined", True):
+        return Bm.PipelinedBatchEngine(cfg["max_circuits"], cfg["slab_entries"], device=device,
+                                       graphs=cfg.get("graphs", True), **cfg["kw"])
+    return Bm.BatchEngine(cfg["max_circuits"], cfg["slab_entries"], device=device, **cfg["kw"])
+
+
 class _EngineLoop:
     """Admission + ticking of one BatchEngine (FIFO, HBM budget)."""
 
@@ -231,9 +243,7 @@ def _engine_worker(wid, device, cfg, inq, outq):
 
         importlib.import_module(PKG + "._lib").load()
         torch.cuda.set_device(device)
-        Bm = importlib.import_module(PKG + ".simulation.batched")
-        loop = _EngineLoop(Bm.BatchEngine(cfg["max_circuits"], cfg["slab_entries"], device=device, **cfg["kw"]),
-                           cfg["budget"])
+        loop = _EngineLoop(_make_engine(cfg, device), cfg["budget"])
         outq.put(("ready", wid, None))
         closing = False
         while True:
@@ -274,7 +284,7 @@ class BatchServer:
     footprint.  Use as a context manager."""
 
     def __init__(self, workers=1, max_circuits=512, slab_entries=None, hbm_budget=200e9, device="cuda:0",
-                 **kw):
+                 pipelined=True, graphs=True, **kw):
         self.workers = int(workers)
         self.hbm_budget = float(hbm_budget)
         self.device = device
@@ -284,18 +294,14 @@ class BatchServer:
         if slab_entries is None:
             slab_entries = slab_entries_for(budget)
         self._cfg = {"max_circuits": int(max_circuits), "slab_entries": int(slab_entries),
-                     "budget": budget, "kw": kw}
+                     "budget": budget, "kw": kw, "pipelined": bool(pipelined), "graphs": bool(graphs)}
         self._loop = None
         self._procs = []
         self._inqs = []
 
     def __enter__(self):
         if self.workers <= 1:
-            import importlib
-
-            Bm = importlib.import_module(PKG + ".simulation.batched")
-            self._loop = _EngineLoop(Bm.BatchEngine(self._cfg["max_circuits"], self._cfg["slab_entries"],
-                                                    device=self.device, **self._cfg["kw"]), self._cfg["budget"])
+            self._loop = _EngineLoop(_make_engine(self._cfg, self.device), self._cfg["budget"])
             return self
         ctx = mp.get_context("spawn")
         self._outq = ctx.Queue()

@@ -192,6 +192,7 @@ class BatchEngine:
         if st is None:
             return False
         c = int(slots[0])
+        self._last_admitted = c
         M = len(circuit)
         self.alive[c] = True
         self.N[c], self.k[c], self.off[c] = num_qubits, k, st
@@ -524,14 +525,18 @@ class BatchEngine:
 class BatchedBASS:
     """A fixed batch: every circuit starts at once (specs: list of
     (num_qubits, k, circuit, optimize_at or None)); run() returns the results
-    in spec order.  Keywords as BatchEngine."""
+    in spec order.  pipelined: PipelinedBatchEngine (graphs: its HIP graphs)
+    or BatchEngine; other keywords as BatchEngine."""
 
-    def __init__(self, specs, device=None, trace=False, **kw):
+    def __init__(self, specs, device=None, trace=False, pipelined=True, graphs=True, **kw):
         if kw.pop("truncate_every", 1) != 1:
             raise NotImplementedError("the batched driver implements truncate_every = 1")
         self.specs = list(specs)
-        self.engine = BatchEngine(max(1, len(self.specs)), max(1, sum(int(s[1]) for s in self.specs)),
-                                  device=device, trace=trace, **kw)
+        C, slab = max(1, len(self.specs)), max(1, sum(int(s[1]) for s in self.specs))
+        if pipelined:
+            self.engine = PipelinedBatchEngine(C, slab, device=device, trace=trace, graphs=graphs, **kw)
+        else:
+            self.engine = BatchEngine(C, slab, device=device, trace=trace, **kw)
         self.traces = None
 
     @property
@@ -549,3 +554,532 @@ class BatchedBASS:
         if self.engine.trace:
             self.traces = [r.pop("trace") for r in out]
         return out
+
+
+# next action of a circuit in the pipelined engine
+_A_GATE, _A_WAIT, _A_RDM, _A_TRIAL, _A_KEEP, _A_SUMS, _A_FIN, _A_DONE = range(8)
+# what a moments result leads to
+_C_CHK, _C_RDM, _C_FIN = range(1, 4)
+
+
+class PipelinedBatchEngine(BatchEngine):
+    """BatchEngine whose ticks overlap with the device.
+
+    State sizes live in device count cells (one per circuit and slab): a run
+    of gates reads and writes them on the device, so it needs no host read.
+    tick() launches the next operation of every circuit that can go on, then
+    waits for the previous tick's device work and applies its results (stats
+    copied asynchronously into pinned host memory).  A circuit whose next
+    operation depends on a result (a check point, the RDMs of a pass, a
+    rotation trial's decision, moments) sits out the ticks until that result
+    is in.  Per circuit the device operations, their order and every decision
+    are those of BatchEngine, so the results are identical.
+
+    The operations run through a batch plan (_ops.BatchPlan): fixed launch
+    shapes whose geometry is derived on the device, and with graphs=True each
+    operation's launch sequence is captured once in a HIP graph and replayed.
+    """
+
+    def __init__(self, max_circuits, slab_entries, device=None, graphs=True, **kw):
+        super().__init__(max_circuits, slab_entries, device=device, **kw)
+        import torch
+
+        C = self.C
+        self.graphs = bool(graphs)
+        self.plan = _ops.BatchPlan(self._ctx, C, self.slab_entries // 512 + C + 1, graphs=self.graphs)
+        self.tiles = None
+        self._cnt = torch.zeros((2, C), dtype=torch.int64, device=self._ctx.device)
+        self._cbase = np.uint64(self._cnt.data_ptr())
+        self.act = np.full(C, _A_DONE, dtype=np.int64)
+        self.after = np.zeros(C, dtype=np.int64)
+        # the pending rotation trial of each circuit
+        self.tj = np.zeros(C, dtype=np.int64)
+        self.tV = np.zeros((C, 2, 2), dtype=np.complex128)
+        self.tpos = np.zeros((C, NMAX), dtype=np.int64)
+        self.tm = np.zeros(C, dtype=np.int64)
+        self.ttr = np.zeros(C, dtype=bool)
+        self.tfrac = np.zeros(C)
+        self._keep_sum2 = np.zeros(C)
+        # gates of the admitted circuits, flattened: [gbase_c, gbase_c + len_c)
+        self._gcap = 0
+        self._gtop = 0
+        self.gbase = np.zeros(C, dtype=np.int64)
+        self._g_nq = np.zeros(0, dtype=np.int8)
+        self._g_q = np.zeros((0, 2), dtype=np.int64)
+        self._g_m = np.zeros((0, 4, 4), dtype=np.complex128)
+        self._g_chk = np.zeros(0, dtype=bool)  # an optimiser check point follows the gate
+        # two tick slots of pinned result buffers (stats rows, RDM rows)
+        self._stats = [_ops.pinned_array((8 * C,), _ops.STATS_DTYPE) for _ in range(2)]
+        self._rho = [_ops.pinned_array((C, 64, 4), np.float64) for _ in range(2)]
+        self._srow = [0, 0]
+        self._records = [[], []]
+        self._events = [None, None]
+        self._tickno = 0
+        self._finishing = []  # (event, [(c, tag, meta, x_pinned, a_pinned)])
+
+    # ---------------------------------------------------------- admission --
+    def _store_gates(self, circuit, new):
+        """Append circuit's gates (circuit slot `new`, not stored yet) and its
+        check points."""
+        M = len(circuit)
+        nqs = np.fromiter((g.n_qubits for g in circuit), dtype=np.int8, count=M)
+        if np.any((nqs != 1) & (nqs != 2)):
+            raise ValueError("only 1- and 2-qubit gates are supported")
+        if self._gtop + M > self._gcap:
+            # compact the stored circuits' gates, then grow
+            live = np.flatnonzero(self.alive)
+            live = live[live != new]
+            keep = [(c, int(self.gbase[c]), int(self.lens[c])) for c in live]
+            need = sum(m for _, _, m in keep) + M
+            cap = max(1 << 16, 2 * need)
+            nq = np.zeros(cap, dtype=np.int8)
+            q = np.zeros((cap, 2), dtype=np.int64)
+            m = np.zeros((cap, 4, 4), dtype=np.complex128)
+            ck = np.zeros(cap, dtype=bool)
+            top = 0
+            for c, b, ln in keep:
+                nq[top: top + ln] = self._g_nq[b: b + ln]
+                q[top: top + ln] = self._g_q[b: b + ln]
+                m[top: top + ln] = self._g_m[b: b + ln]
+                ck[top: top + ln] = self._g_chk[b: b + ln]
+                self.gbase[c] = top
+                top += ln
+            self._g_nq, self._g_q, self._g_m, self._g_chk = nq, q, m, ck
+            self._gcap, self._gtop = cap, top
+        b = self._gtop
+        self._g_nq[b: b + M] = nqs
+        i1 = np.flatnonzero(nqs == 1)
+        i2 = np.flatnonzero(nqs == 2)
+        if len(i1):
+            self._g_q[b + i1, 0] = np.fromiter((circuit[i].qubits[0] for i in i1), dtype=np.int64, count=len(i1))
+            self._g_m[b + i1, :2, :2] = np.array([circuit[i].matrix for i in i1], dtype=np.complex128)
+        if len(i2):
+            self._g_q[b + i2] = np.array([circuit[i].qubits[:2] for i in i2], dtype=np.int64)
+            self._g_m[b + i2] = np.array([circuit[i].matrix for i in i2], dtype=np.complex128)
+        self._g_chk[b: b + M] = False
+        pts = np.fromiter(self.chk[new], dtype=np.int64, count=len(self.chk[new]))
+        self._g_chk[b + pts] = True
+        self._gtop = b + M
+        return b
+
+    def admit(self, num_qubits, k, circuit, optimize_at=None, tag=None):
+        circuit = list(circuit)
+        if not super().admit(num_qubits, k, circuit, optimize_at, tag):
+            return False
+        c = self._last_admitted
+        self.gbase[c] = self._store_gates(circuit, c)
+        self.circuits[c] = None  # the flattened copy is used
+        self.act[c] = _A_GATE if self.lens[c] > 0 else _A_DONE
+        self._cnt[0, c] = 1
+        if self.lens[c] == 0:
+            self._finish(np.array([c]))
+        return True
+
+    # ------------------------------------------------------------ helpers --
+    def _jobs_dev(self, cs, out_other=False):
+        """Jobs whose counts are the device cells (n = the bound k)."""
+        J = self._jobs(cs, out_other)
+        cur = self.cur[cs].astype(np.uint64)
+        oth = cur ^ np.uint64(1) if out_other else cur
+        C = np.uint64(self.C)
+        csu = cs.astype(np.uint64)
+        J["cnt_in"] = self._cbase + np.uint64(8) * (cur * C + csu)
+        J["cnt_out"] = self._cbase + np.uint64(8) * (oth * C + csu)
+        J["n"] = self.k[cs]
+        return J
+
+    def _cells(self, J, cs, out_other):
+        cur = self.cur[cs].astype(np.uint64)
+        oth = cur ^ np.uint64(1) if out_other else cur
+        C = np.uint64(self.C)
+        csu = cs.astype(np.uint64)
+        J["cnt_out"] = self._cbase + np.uint64(8) * (oth * C + csu)
+
+    def _run(self, op, J, kind, cs, rows=True, extra=None):
+        slot = self._tickno & 1
+        out = None
+        if kind == "rdm":
+            out = self._rho[slot][0][: len(cs)]
+        elif rows:
+            r0 = self._srow[slot]
+            out = self._stats[slot][0][r0: r0 + len(cs)]
+            self._srow[slot] = r0 + len(cs)
+        self.plan.run(op, J, out)
+        if kind is not None:
+            self._records[slot].append((kind, cs, out, extra))
+
+    # ------------------------------------------------------------- issue ----
+    def _issue_gates(self, g):
+        ii = self.gidx[g]
+        idx = self.gbase[g] + ii
+        nq = self._g_nq[idx]
+        # after this gate: 2 = the circuit's last, 1 = a check point, 0 = go on
+        last = ii + 1 >= self.lens[g]
+        post = np.where(last, 2, np.where(self._g_chk[idx], 1, 0))
+        one = nq == 1
+        if (~one).any():
+            self._issue_2q(ii[~one], g[~one], idx[~one], post[~one])
+        if one.any():
+            cs, gi = g[one], idx[one]
+            q = self._g_q[gi, 0]
+            gm = self._g_m[gi, :2, :2]
+            U = self.U[cs, q]
+            G = _dagger(U) @ gm @ U
+            jb = self.pos[cs, q]
+            self.pos[cs] = _lift_rows(self.pos[cs], jb, 0)
+            J = self._jobs_dev(cs)
+            J["j"] = jb
+            J["g"][:, :8] = np.ascontiguousarray(G).reshape(len(cs), 4).view(np.float64)
+            J["key_map"] = _key_maps(self.pos[cs])
+            self._run(L.BOP_GATE1, J, "gate", cs, extra=(ii[one], post[one]))
+            self._count("gate1q", len(cs))
+        self.gidx[g] += 1
+        self.act[g] = np.where(post > 0, _A_WAIT, _A_GATE)
+
+    def _issue_2q(self, ii, cs, gi, post):
+        q1 = self._g_q[gi, 0].copy()
+        q2 = self._g_q[gi, 1].copy()
+        gm = self._g_m[gi]
+        Ut = _kron2(self.U[cs, q1], self.U[cs, q2])
+        G = _dagger(Ut) @ gm @ Ut
+        off = G.copy()
+        off[:, np.arange(4), np.arange(4)] = 0
+        diag = self.use_diag_gate & (np.abs(off).max(axis=(1, 2)) < 1e-12)
+        if diag.any():
+            d = cs[diag]
+            J = self._jobs_dev(d)
+            J["j"] = self.pos[d, q1[diag]]
+            J["j2"] = self.pos[d, q2[diag]]
+            J["g"][:, :8] = np.ascontiguousarray(np.diagonal(G[diag], axis1=1, axis2=2)).view(np.float64)
+            self._run(L.BOP_DIAG2Q, J, "diag", d, rows=False, extra=(ii[diag], post[diag]))
+            self._count("diag2q", len(d))
+        gen = ~diag
+        if not gen.any():
+            return
+        cs, q1, q2, G, ii, post = cs[gen], q1[gen], q2[gen], G[gen], ii[gen], post[gen]
+        sw = (self.pos[cs, q1] == 0) & (self.pos[cs, q2] != 0)
+        if sw.any():
+            G[sw] = G[sw][:, _SWAP_IDX][:, :, _SWAP_IDX]
+            q1[sw], q2[sw] = q2[sw], q1[sw].copy()
+        mv = self.pos[cs, q2] != 0
+        if mv.any():
+            m = cs[mv]
+            J = self._jobs_dev(m, out_other=True)
+            jb = self.pos[m, q2[mv]]
+            J["j"] = jb
+            self._run(L.BOP_MOVE, J, None, m, rows=False)
+            self.pos[m] = _lift_rows(self.pos[m], jb, 0)
+            self.cur[m] ^= 1
+        jb = self.pos[cs, q1]
+        self.pos[cs] = _lift_rows(self.pos[cs], jb, 1)
+        J = self._jobs_dev(cs)
+        J["j"] = jb
+        J["g"][:, :32] = np.ascontiguousarray(G).reshape(len(cs), 16).view(np.float64)
+        J["key_map"] = _key_maps(self.pos[cs])
+        self._run(L.BOP_GATE2, J, "gate", cs, extra=(ii, post))
+        self._count("gate2q", len(cs))
+
+    def _issue_sums(self, cs):
+        self._run(L.BOP_MOMENTS, self._jobs(cs), "sums", cs)
+
+    def _issue_rdm(self, cs):
+        self._run(L.BOP_RDM1Q, self._jobs(cs), "rdm", cs)
+        self._count("rdm", len(cs))
+
+    def _issue_trials(self, trial):
+        j = self.ochain[trial, self.optr[trial]]
+        self.optr[trial] += 1
+        Vj = self.oV[trial, j]
+        jb = self.pos[trial, j]
+        tpos = _lift_rows(self.pos[trial], jb, 0)
+        self.tj[trial], self.tV[trial], self.tpos[trial] = j, Vj, tpos
+        J = self._jobs(trial, out_other=True)
+        self._cells(J, trial, True)
+        J["j"] = jb
+        J["g"][:, :8] = np.ascontiguousarray(_dagger(Vj)).reshape(len(trial), 4).view(np.float64)
+        J["key_map"] = _key_maps(tpos)
+        self._run(L.BOP_ROT1, J, "trial", trial)
+        self._count("rotate", len(trial))
+
+    def _issue_keep(self, cs):
+        # kept whole: renormalise the candidates, then their moments
+        J = self._jobs(cs, out_other=True)
+        J["x"], J["a"] = J["xo"], J["ao"]
+        J["n"] = self.tm[cs]
+        norm = np.sqrt(self._keep_sum2[cs])
+        J["scale"] = np.where(norm > 1e-30, norm, 1.0)
+        self._run(L.BOP_SCALE, J, None, cs, rows=False)
+        self._run(L.BOP_MOMENTS, J, "keep", cs)
+
+    def _issue_fin(self, cs):
+        """Final renormalisation (:551-554), then the moments _opt_leave reads."""
+        norm = np.sqrt(self.sum2[cs])
+        do = norm > 1e-30
+        if do.any():
+            J = self._jobs(cs[do])
+            J["scale"] = norm[do]
+            self._run(L.BOP_SCALE, J, None, cs[do], rows=False)
+        self.sum4[cs] = np.nan
+        self._run(L.BOP_MOMENTS, self._jobs(cs), "leave", cs)
+
+    # ------------------------------------------------------------ results --
+    def _process(self, slot):
+        chk, done = [], []
+        for kind, cs, out, extra in self._records[slot]:
+            if kind in ("gate", "diag"):
+                chk.append(cs[extra[1] == 1])
+                done.append(cs[extra[1] == 2])
+            if out is not None and kind != "rdm":
+                bad = np.flatnonzero(out["err"] != 0)
+                if len(bad):
+                    raise L.BassError(f"batch {kind}: circuit {int(cs[bad[0]])}: device error bits "
+                                      f"0x{int(out['err'][bad[0]]):x}")
+            if kind == "gate":
+                self._apply_trunc_stats(extra[0], cs, out)
+            elif kind == "diag":
+                self.sum4[cs] = np.nan
+                self._count("amp_updates", self.nnz[cs].sum())
+            elif kind == "sums":
+                self.sum2[cs], self.sum4[cs] = out["sum2"], out["sum4"]
+                a = self.after[cs]
+                self._checkpoint2(cs[a == _C_CHK])
+                self._rdm2(cs[a == _C_RDM])
+                self.act[cs[a == _C_FIN]] = _A_FIN
+            elif kind == "rdm":
+                self._rdm_result(cs, out)
+            elif kind == "trial":
+                self._trial_result(cs, out)
+            elif kind == "keep":
+                self._decide(cs, out["sum4"].copy())
+            elif kind == "leave":
+                self.sum2[cs], self.sum4[cs] = out["sum2"], out["sum4"]
+                self.pr_last[cs] = self._pr(cs)
+                self.phase[cs] = 0
+                self.act[cs] = _A_GATE
+        self._records[slot] = []
+        self._srow[slot] = 0
+        if chk:
+            self._checkpoint(np.concatenate(chk))
+            self._finish(np.concatenate(done))
+
+    def _checkpoint(self, ch):
+        """Check point after a gate (BatchEngine._gate_step)."""
+        if not len(ch):
+            return
+        ta_mask = self.trunc_active[ch]
+        self.trunc_active[ch] = False
+        self.act[ch[~ta_mask]] = _A_GATE
+        ta = ch[ta_mask]
+        if not len(ta):
+            return
+        need = np.isnan(self.sum4[ta]) | np.isnan(self.sum2[ta])
+        self.act[ta[need]] = _A_SUMS
+        self.after[ta[need]] = _C_CHK
+        self._checkpoint2(ta[~need])
+
+    def _checkpoint2(self, ta):
+        if not len(ta):
+            return
+        pr = self._pr(ta)
+        trig = np.ones(len(ta), bool) if self.pr_opt_threshold >= 1.0 else pr > self.pr_last[ta] / self.pr_opt_threshold
+        self.act[ta[~trig]] = _A_GATE
+        opt = ta[trig]
+        if not len(opt):
+            return
+        enter = ~(self._pr(opt) < self.k[opt] / 4)
+        e = opt[enter]
+        self.phase[e] = 1
+        self.ostage[e] = 0
+        self.opass[e] = 0
+        self.oentered[opt] = enter
+        self.act[e] = _A_RDM
+        skip = opt[~enter]
+        if len(skip):
+            # _opt_leave: the sums are known here
+            self.pr_last[skip] = self._pr(skip)
+            self.phase[skip] = 0
+            self.act[skip] = _A_GATE
+
+    def _rdm_result(self, cs, out):
+        rho_st = np.zeros((len(cs), 64, 2, 2), dtype=np.complex128)
+        rho_st[:, :, 0, 0] = out[:, :, 0]
+        rho_st[:, :, 1, 1] = out[:, :, 1]
+        rho_st[:, :, 0, 1] = out[:, :, 2] + 1j * out[:, :, 3]
+        rho_st[:, :, 1, 0] = out[:, :, 2] - 1j * out[:, :, 3]
+        P = np.minimum(self.pos[cs], NMAX - 1)
+        rho = np.take_along_axis(rho_st, P[:, :, None, None], axis=1)
+        V, trivial = _eigh2x2_rows(rho)
+        trivial |= np.arange(NMAX)[None, :] >= self.N[cs][:, None]
+        self.oV[cs] = V
+        self.oclen[cs] = (~trivial).sum(axis=1)
+        self.ochain[cs] = np.argsort(trivial, axis=1, kind="stable")
+        self.optr[cs] = 0
+        need = np.isnan(self.sum4[cs]) | np.isnan(self.sum2[cs])
+        self.act[cs[need]] = _A_SUMS
+        self.after[cs[need]] = _C_RDM
+        self._rdm2(cs[~need])
+
+    def _rdm2(self, cs):
+        if not len(cs):
+            return
+        self.opr[cs] = self._pr(cs)
+        self.oimproved[cs] = False
+        self.ostage[cs] = 1
+        self.act[cs] = _A_TRIAL
+
+    def _trial_result(self, cs, st):
+        m = st["n_in"].copy()
+        tr = (st["flags"] & 1) != 0
+        self._count("amp_updates", m.sum())
+        self.tm[cs] = m
+        self.ttr[cs] = tr
+        self.tfrac[cs] = np.where(tr, st["kept"], st["sum2"])
+        self._keep_sum2[cs] = st["sum2"]
+        keep = cs[~tr]
+        self.act[keep] = _A_KEEP
+        if tr.any():
+            self._decide(cs[tr], st["sum4"][tr].copy())
+
+    def _decide(self, trial, s4):
+        """Accept / reject of rotation trials (BatchEngine._opt_trial)."""
+        m, tr, frac = self.tm[trial], self.ttr[trial], self.tfrac[trial]
+        j, Vj = self.tj[trial], self.tV[trial]
+        pr_new = np.where(s4 > 1e-60, 1.0 / np.where(s4 > 1e-60, s4, 1.0), np.inf)
+        pr = self.opr[trial]
+        reject = pr_new >= pr * (1.0 - self.accept_rtol)
+        if self.traces is not None:
+            for r, c in enumerate(trial):
+                self.traces[c].append(("rot", int(j[r]), bool(not reject[r]), float(pr[r]),
+                                       float(pr_new[r]), float(frac[r])))
+        acc = np.flatnonzero(~reject)
+        self._count("reject", int(reject.sum()))
+        self._count("accept", len(acc))
+        self.act[trial] = _A_TRIAL
+        if not len(acc):
+            return
+        a = trial[acc]
+        ja = j[acc]
+        self.cur[a] ^= 1
+        self.nnz[a] = np.where(tr[acc], self.k[a], m[acc])
+        self.pos[a] = self.tpos[a]
+        self.sum2[a], self.sum4[a] = np.nan, s4[acc]
+        self.U[a, ja] = self.U[a, ja] @ Vj[acc]
+        self.opr[a] = pr_new[acc]
+        f = frac[acc]
+        self.gamma_run[a] *= np.sqrt(np.where(f > 1e-30, f, 0.0))
+        self.oimproved[a] = True
+
+    def _finish(self, cs):
+        """Last gate done: copy the state to pinned host memory (returned once
+        the copy is complete) and release the slab region."""
+        if not len(cs):
+            return
+        import torch
+
+        items = []
+        for c in cs:
+            c = int(c)
+            n = int(self.nnz[c])
+            buf = self.slabs[int(self.cur[c])]
+            o = int(self.off[c])
+            xp = torch.empty(max(n, 1), dtype=torch.int64, pin_memory=True)
+            ap = torch.empty(max(n, 1), dtype=torch.complex128, pin_memory=True)
+            if n:
+                xp[:n].copy_(buf.x[o: o + n], non_blocking=True)
+                ap[:n].copy_(buf.a[o: o + n], non_blocking=True)
+            meta = {"nnz": n, "gamma": float(np.sqrt(max(self.gamma_run[c], 0))),
+                    "U": [self.U[c, j].copy() for j in range(self.N[c])], "pos": self.pos[c, : self.N[c]].copy()}
+            if self.traces is not None:
+                meta["trace"] = self.traces[c]
+            items.append((c, self.tags[c], meta, xp, ap))
+            self.act[c] = _A_DONE
+            self._release(o, int(self.k[c]))
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self._ctx.device))
+        self._finishing.append((ev, items))
+
+    def _collect(self, force=False):
+        out = []
+        keep = []
+        for ev, items in self._finishing:
+            if force:
+                ev.synchronize()
+            if force or ev.query():
+                for c, tag, meta, xp, ap in items:
+                    n = meta["nnz"]
+                    w = xp[:n].numpy().view(np.uint64)
+                    res = {"keys": stored_to_keys(w, meta.pop("pos")), "amps": ap[:n].numpy().copy(),
+                           "nnz": n, "gamma": meta["gamma"], "U": meta["U"]}
+                    if "trace" in meta:
+                        res["trace"] = meta["trace"]
+                    out.append((tag, res))
+                    self.alive[c] = False
+            else:
+                keep.append((ev, items))
+        self._finishing = keep
+        return out
+
+    # --------------------------------------------------------------- tick --
+    def tick(self):
+        """Launch one operation of every circuit that can go on, then apply
+        the previous tick's results.  Returns [(tag, result)] of circuits
+        whose final state has reached the host."""
+        import torch
+
+        slot = self._tickno & 1
+        live = self.alive
+        # pass ends (host only): another pass after an improvement, else finish
+        tr = np.flatnonzero(live & (self.act == _A_TRIAL))
+        if len(tr):
+            end = tr[self.optr[tr] >= self.oclen[tr]]
+            if len(end):
+                again = self.oimproved[end] & (self.opass[end] < 2)
+                a = end[again]
+                self.opass[a] += 1
+                self.ostage[a] = 0
+                self.act[a] = _A_RDM
+                f = end[~again]
+                self.ostage[f] = 2
+                need = np.isnan(self.sum4[f]) | np.isnan(self.sum2[f])
+                self.act[f[need]] = _A_SUMS
+                self.after[f[need]] = _C_FIN
+                self.act[f[~need]] = _A_FIN
+        g = np.flatnonzero(live & (self.act == _A_GATE))
+        if len(g):
+            self._issue_gates(g)
+        s = np.flatnonzero(live & (self.act == _A_SUMS))
+        if len(s):
+            self.act[s] = _A_WAIT
+            self._issue_sums(s)
+        r = np.flatnonzero(live & (self.act == _A_RDM))
+        if len(r):
+            self.act[r] = _A_WAIT
+            self._issue_rdm(r)
+        t = np.flatnonzero(live & (self.act == _A_TRIAL))
+        if len(t):
+            self.act[t] = _A_WAIT
+            self._issue_trials(t)
+        kp = np.flatnonzero(live & (self.act == _A_KEEP))
+        if len(kp):
+            self.act[kp] = _A_WAIT
+            self._issue_keep(kp)
+        fn = np.flatnonzero(live & (self.act == _A_FIN))
+        if len(fn):
+            self.act[fn] = _A_WAIT
+            self._issue_fin(fn)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self._ctx.device))
+        self._events[slot] = ev
+        prev = slot ^ 1
+        if self._events[prev] is not None:
+            self._events[prev].synchronize()
+            self._events[prev] = None
+            self._process(prev)
+        self._tickno += 1
+        # nothing launched and nothing in flight: the last results are in
+        idle = not (len(g) or len(s) or len(r) or len(t) or len(kp) or len(fn))
+        if idle:
+            ev.synchronize()
+            self._events[slot] = None
+            self._process(slot)
+        return self._collect(force=idle)

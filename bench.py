@@ -134,9 +134,10 @@ def bench_batched(args):
         single_rate = len(sample) / t_single
         log(f"one at a time: {len(sample)} circuits in {t_single:.2f} s = {single_rate:.2f} circuits/s")
     # the batch
+    mode = {"graphs": {}, "pipelined": {"graphs": False}, "sync": {"pipelined": False}}[args.batch_engine]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    bat = Bm.BatchedBASS(specs, device="cuda:0")
+    bat = Bm.BatchedBASS(specs, device="cuda:0", **mode)
     res = bat.run()
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
@@ -149,7 +150,7 @@ def bench_batched(args):
         "config": {"workload": f"batched TFIM circuits={len(specs)} n~U[16,48] steps~U[10,60] h~U[0.2,2] "
                                f"K=2^U{{10..18}} (step = the whole batch)",
                    "circuits": len(specs), "gates": int(sum(len(s[2]) for s in specs)),
-                   "amp_updates": int(bat.counters.get("amp_updates", 0))},
+                   "amp_updates": int(bat.counters.get("amp_updates", 0)), "engine": args.batch_engine},
         "one_at_a_time": {"value": single_rate, "unit": "circuits/s", "sample": len(sample),
                           "seconds": t_single, "driver": "plain BASS, one circuit after another, "
                                                           "every (circuits/sample)-th circuit of the stream"},
@@ -595,6 +596,9 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0, help="served: Poisson arrival rate (req/s)")
     ap.add_argument("--rates", type=str, default=None, help="served: comma-separated arrival rates to sweep")
     ap.add_argument("--server", choices=["engine", "pool"], default="engine", help="served: server kind")
+    ap.add_argument("--batch-engine", choices=["graphs", "pipelined", "sync"], default="graphs",
+                    help="batched / served: pipelined engine with HIP graphs, pipelined with direct launches, "
+                         "or the synchronous engine")
     ap.add_argument("--max-circuits", type=int, default=512, help="served (engine): circuits running at once")
     ap.add_argument("--workers", type=int, default=None,
                     help="served: worker processes (engine: one batch engine each, default 4; pool: default 8)")

@@ -69,6 +69,9 @@ typedef struct bass_job {
   double g[32];       /* gate matrix (re, im) row-major, 2x2 or 4x4; diagonal gate: 4 values */
   int32_t key_map[64];/* stored bit -> qubit */
   double scale;       /* bass_batch_scale divisor */
+  const int64_t* cnt_in; /* device count cell of the input, or NULL: n entries.  When set,
+                            the call reads the count on the device (n bounds it) */
+  int64_t* cnt_out;   /* device count cell the call sets to its output count, or NULL */
 } bass_job;
 
 int bass_abi_version(void);
@@ -296,7 +299,10 @@ int bass_dense_vdot(const double* u, const double* v, int64_t len, double* out, 
 
 /* ---- batches: many independent key-ordered states per launch ----------
  * Each call runs one launch sequence for all njobs states (per-job scalars via
- * bass_batch_stats).  Job tables are staged through pinned host buffers.
+ * bass_batch_stats).  Job tables are staged through pinned host buffers.  A
+ * job's count cells (cnt_in / cnt_out) keep state sizes on the device, so
+ * consecutive gates need no host read; the geometry of every launch is then
+ * derived on the device.
  *
  * bass_batch_step: for every job, the gate (mode 0) or rotation (mode 1) of
  * bass_lift on (x, a, n, j, g), then bass_truncate of its output into (xo, ao)
@@ -327,6 +333,40 @@ int bass_batch_rdm1q(const bass_job* jobs, int njobs, double* rho_out, void* ws,
 /* Synchronous: the per-job stats of the last batch call (njobs entries);
  * clears device error bits.  BASS_E_DEVICE if any job reports one. */
 int bass_batch_stats(void* ws, size_t ws_bytes, int njobs, bass_stats* out, void* stream);
+
+/* ---- batch plans: fixed launch shapes, captured in HIP graphs ---------
+ * No counterpart in the reference, which runs circuits one after another
+ * (src/experimentation/runner.py:697-874 calling
+ * src/simulation/bass_simulator.py:847-929); the operations per circuit are
+ * those of the bass_batch_* calls above.
+ *
+ * A plan fixes the workspace regions for up to jobs_cap states and tiles_cap
+ * gate-output tiles (tiles_x / tiles_a hold tiles_cap * 2048 slots), so each
+ * operation's launch sequence has one shape: a prep kernel reads the input
+ * counts (count cells or staged n) and lays out the tiles on the device, and
+ * every kernel after it runs a persistent grid over the totals it wrote.
+ * flags bit 0: capture each sequence once into a HIP graph and replay it. */
+enum {
+  BASS_BOP_GATE1 = 0, /* bass_batch_step nq=1 mode=0 */
+  BASS_BOP_GATE2 = 1, /* nq=2 mode=0 */
+  BASS_BOP_ROT1 = 2,  /* nq=1 mode=1 */
+  BASS_BOP_ROT2 = 3,  /* nq=2 mode=1 */
+  BASS_BOP_MOVE = 4,  /* bass_batch_move */
+  BASS_BOP_DIAG2Q = 5,
+  BASS_BOP_SCALE = 6,
+  BASS_BOP_MOMENTS = 7, /* staged counts only */
+  BASS_BOP_RDM1Q = 8    /* staged counts only */
+};
+size_t bass_bplan_ws_bytes(int jobs_cap, int64_t tiles_cap);
+int bass_bplan_create(int jobs_cap, int64_t tiles_cap, uint64_t* tiles_x, double* tiles_a, void* ws,
+                      size_t ws_bytes, int flags, void** plan);
+int bass_bplan_destroy(void* plan);
+/* One batch operation, asynchronous on the stream.  out_host (NULL: none),
+ * written when the stream reaches the end of the call: steps and moments,
+ * njobs bass_stats; RDM1Q, njobs x 64 x 4 doubles as bass_batch_rdm1q.
+ * Pinned host memory keeps the copy asynchronous.  The caller checks the
+ * stats' err fields. */
+int bass_bplan_run(void* plan, int op, const bass_job* jobs, int njobs, void* out_host, void* stream);
 
 int bass_ht_clear(uint64_t* ht_keys, int32_t* ht_vals, int log2cap, void* stream);
 int bass_ht_insert(uint64_t* ht_keys, int32_t* ht_vals, int log2cap, const uint64_t* keys,

@@ -148,3 +148,28 @@ def test_no_gpu_means_loud_failure():
     dev = pkg_module("_device")
     with pytest.raises(pkg_module("_lib").BassError):
         dev.context("cuda:0")
+
+
+def test_job_table_layout():
+    """bass_job as the ctypes table sees it: 608 bytes, the count cells last."""
+    L = pkg_module("_lib")
+    assert L.JOB_DTYPE.itemsize == 608
+    assert L.JOB_DTYPE.fields["cnt_in"][1] == 592 and L.JOB_DTYPE.fields["cnt_out"][1] == 600
+
+
+def test_batch_plan_arguments_without_gpu(lib):
+    L = pkg_module("_lib")
+    a = lib.bass_bplan_ws_bytes(8, 16)
+    b = lib.bass_bplan_ws_bytes(8, 32)
+    c = lib.bass_bplan_ws_bytes(16, 32)
+    assert 0 < a < b < c
+    assert b - a >= 16 * 2048 * 32  # the select's candidate arrays per tile slot
+    assert lib.bass_bplan_ws_bytes(0, 16) == 0
+    h = ctypes.c_void_p()
+    rc = lib.bass_bplan_create(0, 16, None, None, None, 0, 0, ctypes.byref(h))
+    assert rc == L.BASS_E_ARG and "capacities" in L.last_error()
+    rc = lib.bass_bplan_create(8, 16, None, None, None, 0, 0, ctypes.byref(h))
+    assert rc == L.BASS_E_ARG and "null" in L.last_error()
+    rc = lib.bass_bplan_run(None, 0, None, 1, None, None)
+    assert rc == L.BASS_E_ARG and "plan" in L.last_error()
+    assert lib.bass_bplan_destroy(None) == L.BASS_OK

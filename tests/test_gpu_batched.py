@@ -23,12 +23,18 @@ def _specs():
     return out
 
 
-def test_batched_matches_single(gpu_ctx):
-    print("[case] batched vs single", flush=True)
+# engine modes: pipelined with HIP graphs (the default), pipelined with direct
+# launches, the synchronous engine
+MODES = {"graphs": {}, "pipelined": {"graphs": False}, "sync": {"pipelined": False}}
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_batched_matches_single(gpu_ctx, mode):
+    print(f"[case] batched vs single ({mode})", flush=True)
     Bm = pkg_module("simulation.batched")
     S = pkg_module("simulation.bass_simulator")
     specs = _specs()
-    bat = Bm.BatchedBASS(specs, device="cuda:0", trace=True)
+    bat = Bm.BatchedBASS(specs, device="cuda:0", trace=True, **MODES[mode])
     res = bat.run()
     for c, (n, k, circ, ends) in enumerate(specs):
         tr = []
@@ -58,16 +64,18 @@ def _single(spec):
     return st.keys(), st.alpha[: st.nnz].copy(), st.nnz, st.gamma
 
 
-def test_engine_staggered_admission(gpu_ctx):
+@pytest.mark.parametrize("pipelined", [False, True])
+def test_engine_staggered_admission(gpu_ctx, pipelined):
     """Circuits joining a running batch between ticks, into slots and slab
     regions released by finished ones (two slots, a slab that fits the two
     largest states only), give the single-circuit results bit for bit."""
-    print("[case] engine staggered admission", flush=True)
+    print(f"[case] engine staggered admission (pipelined={pipelined})", flush=True)
     Bm = pkg_module("simulation.batched")
     specs = _specs()
     want = [_single(s) for s in specs]
     ks = sorted(s[1] for s in specs)
-    eng = Bm.BatchEngine(2, ks[-1] + ks[-2], device="cuda:0")
+    cls = Bm.PipelinedBatchEngine if pipelined else Bm.BatchEngine
+    eng = cls(2, ks[-1] + ks[-2], device="cuda:0")
     queue = list(range(len(specs)))
     got = {}
     ticks = 0
@@ -131,8 +139,10 @@ def test_batch_server_two_engines(gpu_ctx):
 def test_batched_config3_sample(gpu_ctx):
     """32 circuits of the configured batched stream (make_batched_requests:
     TFIM with n ~ U[16,48], steps ~ U[10,60], h ~ U[0.2,2], K = 2^U{10..18};
-    every 16th of the 512) run as one batch: each circuit equals the
-    single-circuit driver bit for bit (keys, amplitudes, gamma, frame)."""
+    every 16th of the 512) run as one batch, with the engine's launch
+    sequences replayed from HIP graphs and launched directly: each circuit
+    equals the single-circuit driver bit for bit (keys, amplitudes, gamma,
+    frame) both ways."""
     print("[case] batched config-3 sample: 32 of 512 circuits", flush=True)
     Srv = pkg_module("serving.server")
     Bm = pkg_module("simulation.batched")
@@ -144,17 +154,18 @@ def test_batched_config3_sample(gpu_ctx):
         specs.append((r.n, 1 << r.log2k, circ, kw.get("optimize_at")))
     print(f"  n {min(s[0] for s in specs)}..{max(s[0] for s in specs)}, K 2^{min(r.log2k for r in reqs)}.."
           f"2^{max(r.log2k for r in reqs)}, {sum(len(s[2]) for s in specs)} gates", flush=True)
-    res = Bm.BatchedBASS(specs, device="cuda:0").run()
+    runs = {g: Bm.BatchedBASS(specs, device="cuda:0", graphs=g).run() for g in (True, False)}
     for c, (n, k, circ, oat) in enumerate(specs):
         sim = S.BASS(n, k, device="cuda:0", **({"optimize_at": oat} if oat else {}))
         st = sim.simulate(circ)
         keys, amps = st.keys(), st.alpha[: st.nnz]
-        r = res[c]
-        assert r["nnz"] == st.nnz and r["gamma"] == st.gamma, c
-        o1, o2 = np.argsort(r["keys"]), np.argsort(keys)
-        assert np.array_equal(r["keys"][o1], keys[o2]), c
-        assert np.array_equal(r["amps"][o1].view(np.float64), amps[o2].view(np.float64)), c
-        assert all(np.array_equal(r["U"][j], sim.U[j]) for j in range(n)), c
+        for g, res in runs.items():
+            r = res[c]
+            assert r["nnz"] == st.nnz and r["gamma"] == st.gamma, (c, g)
+            o1, o2 = np.argsort(r["keys"]), np.argsort(keys)
+            assert np.array_equal(r["keys"][o1], keys[o2]), (c, g)
+            assert np.array_equal(r["amps"][o1].view(np.float64), amps[o2].view(np.float64)), (c, g)
+            assert all(np.array_equal(r["U"][j], sim.U[j]) for j in range(n)), (c, g)
 
 
 @pytest.mark.timeout(900)
