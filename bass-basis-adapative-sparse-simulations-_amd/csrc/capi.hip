@@ -997,20 +997,23 @@ static SelTiles btiles(void* ws, const BLayout& B) {
 }
 
 // Pinned staging ring for host -> device job tables: a slot is reused only
-// after the copy that last read it has completed.
+// after the copy that last read it has completed.  32 slots: a pipelined
+// engine stages up to ~12 tables per tick and runs a tick ahead of the device,
+// so reusing a slot must not wait on the tick in flight.
+constexpr int STAGE_SLOTS = 32;
 struct StageSlot {
   void* host = nullptr;
   size_t cap = 0;
   hipEvent_t done = nullptr;
 };
 static std::mutex g_stage_mu;
-static StageSlot g_stage[8];
+static StageSlot g_stage[STAGE_SLOTS];
 static int g_stage_next = 0;
 
 static int stage_copy(void* dev, const void* src, size_t bytes, hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_stage_mu);
   StageSlot& S = g_stage[g_stage_next];
-  g_stage_next = (g_stage_next + 1) % 8;
+  g_stage_next = (g_stage_next + 1) % STAGE_SLOTS;
   if (S.done) (void)hipEventSynchronize(S.done);
   if (S.cap < bytes) {
     if (S.host) (void)hipHostFree(S.host);

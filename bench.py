@@ -134,15 +134,21 @@ def bench_batched(args):
         single_rate = len(sample) / t_single
         log(f"one at a time: {len(sample)} circuits in {t_single:.2f} s = {single_rate:.2f} circuits/s")
     # the batch
-    mode = {"graphs": {}, "pipelined": {"graphs": False}, "sync": {"pipelined": False}}[args.batch_engine]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    bat = Bm.BatchedBASS(specs, device="cuda:0", **mode)
-    res = bat.run()
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
+    modes = {"graphs": {}, "pipelined": {"graphs": False}, "sync": {"pipelined": False}}
+    engines = {}
+    order = [args.batch_engine] + ([m for m in modes if m != args.batch_engine] if args.all_engines else [])
+    for name in order:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = Bm.BatchedBASS(specs, device="cuda:0", **modes[name])
+        b.run()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        engines[name] = {"circuits_per_s": len(specs) / t, "seconds": t}
+        log(f"batched ({name}): {len(specs)} circuits in {t:.2f} s = {len(specs) / t:.2f} circuits/s")
+        if name == args.batch_engine:
+            bat, t_batch = b, t
     rate = len(specs) / t_batch
-    log(f"batched: {len(specs)} circuits in {t_batch:.2f} s = {rate:.2f} circuits/s")
     out = {
         "metric": "batched circuits/s", "value": rate, "unit": "circuits/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -156,6 +162,9 @@ def bench_batched(args):
                                                           "every (circuits/sample)-th circuit of the stream"},
         "speedup_vs_one_at_a_time": (rate / single_rate) if single_rate else None,
         "amp_updates_per_s": bat.counters.get("amp_updates", 0) / t_batch,
+        # the same batch through each engine (graphs: pipelined with replayed
+        # HIP graphs; pipelined: direct launches; sync: a host read per tick)
+        "engines": engines,
     }
     print(json.dumps(out), flush=True)
 
@@ -358,11 +367,13 @@ def bench_served(args):
         prog = lambda d, t: log(f"served {d}/{t}") if d % 64 == 0 else None  # noqa: E731
         peak = None
         if args.server == "engine":
-            with S.BatchServer(workers=args.workers, max_circuits=args.max_circuits, hbm_budget=200e9) as srv:
+            ekw = {"graphs": {}, "pipelined": {"graphs": False}, "sync": {"pipelined": False}}[args.batch_engine]
+            with S.BatchServer(workers=args.workers, max_circuits=args.max_circuits, hbm_budget=200e9,
+                               **ekw) as srv:
                 recs = srv.run(reqs, progress=prog)
             summ = S.summarize(recs, amp_updates=srv.counters.get("amp_updates", 0))
             peak = srv.peak_footprint_bytes
-            cfg = {"server": "continuous batch", "engines": args.workers,
+            cfg = {"server": "continuous batch", "engine": args.batch_engine, "engines": args.workers,
                    "max_circuits_per_engine": args.max_circuits, "slab_entries_per_engine": srv._cfg["slab_entries"]}
         else:
             with S.BassServer(workers=args.workers, hbm_budget=200e9) as srv:
@@ -600,8 +611,10 @@ def main():
                     help="batched / served: pipelined engine with HIP graphs, pipelined with direct launches, "
                          "or the synchronous engine")
     ap.add_argument("--max-circuits", type=int, default=512, help="served (engine): circuits running at once")
+    ap.add_argument("--all-engines", action="store_true",
+                    help="batched: also time the batch through the other two engines")
     ap.add_argument("--workers", type=int, default=None,
-                    help="served: worker processes (engine: one batch engine each, default 4; pool: default 8)")
+                    help="served: worker processes (engine: one batch engine each, default 2; pool: default 8)")
     ap.add_argument("--qubits", type=int, default=None)
     ap.add_argument("--log2k", type=int, default=None)
     ap.add_argument("--calibrate-copy", type=float, default=None, metavar="MIB")
@@ -639,7 +652,7 @@ def main():
         return bench_fixed(args)
     if args.workload == "served":
         if args.workers is None:
-            args.workers = 4 if args.server == "engine" else 8
+            args.workers = 2 if args.server == "engine" else 8
         return bench_served(args)
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
