@@ -57,11 +57,11 @@ class BassStats(ctypes.Structure):
 JOB_DTYPE = np.dtype([("x", "<u8"), ("a", "<u8"), ("xo", "<u8"), ("ao", "<u8"), ("n", "<i8"), ("k", "<i8"),
                       ("cap_out", "<i8"), ("j", "<i4"), ("j2", "<i4"), ("renorm", "<i4"), ("key_bits", "<i4"),
                       ("g", "<f8", (32,)), ("key_map", "<i4", (64,)), ("scale", "<f8"),
-                      ("cnt_in", "<u8"), ("cnt_out", "<u8")])
-assert JOB_DTYPE.itemsize == 608
+                      ("cnt_in", "<u8"), ("cnt_out", "<u8"), ("nq", "<i4"), ("pad", "<i4")])
+assert JOB_DTYPE.itemsize == 616
 
 # bass_bplan_run operations
-BOP_GATE1, BOP_GATE2, BOP_ROT1, BOP_ROT2, BOP_MOVE, BOP_DIAG2Q, BOP_SCALE, BOP_MOMENTS, BOP_RDM1Q = range(9)
+BOP_GATE1, BOP_GATE2, BOP_ROT1, BOP_ROT2, BOP_MOVE, BOP_DIAG2Q, BOP_SCALE, BOP_MOMENTS, BOP_RDM1Q, BOP_GATE12 = range(10)
 
 # name -> (restype, argtypes); mirrors include/bass_hip.h
 SIGNATURES = {

@@ -1069,10 +1069,13 @@ struct BTables {
   std::vector<SelProblem> pr;
 };
 
+// nq: gate qubits of every job, or 0 for each job's own (bass_job.nq); the
+// kinds present are returned in *qmask (bit nq - 1).
 static int build_lift_tables(const bass_job* jobs, int njobs, int nq, int mode, void* ws, const BLayout& B,
-                             uint64_t* tiles_x, double* tiles_a, BTables& X, int64_t* tbound, const char* who) {
+                             uint64_t* tiles_x, double* tiles_a, BTables& X, int64_t* tbound, int* qmask,
+                             const char* who) {
   const bool step = mode != MODE_MOVE;
-  const int tm = lift_tile_entries(nq, mode);
+  *qmask = 0;
   X.hl.assign(sizeof(BatchHdr) + (size_t)njobs * sizeof(LiftJob), 0);
   BatchHdr* h = (BatchHdr*)X.hl.data();
   LiftJob* lj = (LiftJob*)(X.hl.data() + sizeof(BatchHdr));
@@ -1080,9 +1083,13 @@ static int build_lift_tables(const bass_job* jobs, int njobs, int nq, int mode, 
   int64_t T = 0;
   for (int c = 0; c < njobs; ++c) {
     const bass_job& J = jobs[c];
+    const int qb = step ? (nq ? nq : J.nq) : 0;
+    if (step && qb != 1 && qb != 2) return fail(BASS_E_ARG, "%s: job %d: gate qubits must be 1 or 2", who, c);
+    if (step && mode == MODE_ROT && nq == 0) return fail(BASS_E_ARG, "%s: rotations take one kind per call", who);
     KeyMap km;
-    const int rc = check_job(J, c, step, step ? nq - 1 : 0, who, step ? &km : nullptr);
+    const int rc = check_job(J, c, step, step ? qb - 1 : 0, who, step ? &km : nullptr);
     if (rc) return rc;
+    if (step) *qmask |= 1 << (qb - 1);
     if (!step && J.cap_out < J.n) return fail(BASS_E_OVERFLOW, "%s: job %d: cap_out < n", who, c);
     LiftJob& L = lj[c];
     L.x = J.x;
@@ -1093,8 +1100,9 @@ static int build_lift_tables(const bass_job* jobs, int njobs, int nq, int mode, 
     L.cnt_in = J.cnt_in;
     L.cnt_out = step ? nullptr : J.cnt_out;
     L.j = J.j;
+    L.qb = qb;
     if (step) {
-      const int M = 1 << nq;
+      const int M = 1 << qb;
       for (int e = 0; e < M * M; ++e) L.g.m[e] = make_double2(J.g[2 * e], J.g[2 * e + 1]);
       SelProblem& P = X.pr[c];
       memset(&P, 0, sizeof(P));
@@ -1114,6 +1122,7 @@ static int build_lift_tables(const bass_job* jobs, int njobs, int nq, int mode, 
       P.km = km;
       P.cnt_out = J.cnt_out;
     }
+    const int tm = lift_job_tm(qb);
     T += (J.n + tm - 1) / tm;
   }
   h->njobs = njobs;
@@ -1126,15 +1135,15 @@ static int build_lift_tables(const bass_job* jobs, int njobs, int nq, int mode, 
 
 // Launch sequences.  No argument depends on a count: tiles / jobs are bounds
 // that size the persistent grids.
-static void seq_lift(void* ws, const BLayout& B, int nq, int mode, uint64_t* tiles_x, double* tiles_a, int64_t tiles,
-                     int jobs, hipStream_t s) {
-  const int tm = lift_tile_entries(nq, mode);
+// qmask: gate kinds of the jobs (bit 0 one qubit, bit 1 two), 0 for a move.
+static void seq_lift(void* ws, const BLayout& B, int qmask, int mode, uint64_t* tiles_x, double* tiles_a,
+                     int64_t tiles, int jobs, hipStream_t s) {
   LiftJob* lj = at<LiftJob>(ws, B.ljobs);
   BatchHdr* hdr = at<BatchHdr>(ws, B.hdr_l);
   if (mode == MODE_MOVE) {
-    launch_batch_prep(lj, nullptr, hdr, tm, SelBase{nullptr, nullptr, nullptr},
+    launch_batch_prep(lj, nullptr, hdr, SelBase{nullptr, nullptr, nullptr},
                       BatchZero{nullptr, nullptr, nullptr, nullptr}, s);
-    launch_lift_batch(1, MODE_MOVE, lj, hdr, tiles, jobs, at<LiftBound>(ws, B.bounds), nullptr, nullptr, nullptr,
+    launch_lift_batch(0, MODE_MOVE, lj, hdr, tiles, jobs, at<LiftBound>(ws, B.bounds), nullptr, nullptr, nullptr,
                       nullptr, nullptr, s);
     return;
   }
@@ -1142,8 +1151,8 @@ static void seq_lift(void* ws, const BLayout& B, int nq, int mode, uint64_t* til
   const SelBase sb{at<unsigned long long>(ws, B.c2p), at<unsigned int>(ws, B.c2i), at<uint64_t>(ws, B.tk)};
   const BatchZero z{at<OpStats>(ws, B.st), at<SelState>(ws, B.sel), at<unsigned int>(ws, B.hist1),
                     at<unsigned int>(ws, B.hist2)};
-  launch_batch_prep(lj, pr, hdr, tm, sb, z, s);
-  launch_lift_batch(nq, mode, lj, hdr, tiles, jobs, at<LiftBound>(ws, B.bounds), tiles_x, (double2*)tiles_a,
+  launch_batch_prep(lj, pr, hdr, sb, z, s);
+  launch_lift_batch(qmask, mode, lj, hdr, tiles, jobs, at<LiftBound>(ws, B.bounds), tiles_x, (double2*)tiles_a,
                     at<unsigned int>(ws, B.hist1), at<double>(ws, B.tile_psum), at<int>(ws, B.seg_cnt), s);
   launch_select_batch(pr, hdr, btiles(ws, B), tiles, jobs, s);
 }
@@ -1223,7 +1232,7 @@ static int d2h_async(void* host, const void* dev, size_t bytes, hipStream_t s, c
 // once into a HIP graph (on a private stream) and replayed on the caller's
 // stream after the call's tables are staged.
 enum { BOP_GATE1 = 0, BOP_GATE2, BOP_ROT1, BOP_ROT2, BOP_MOVE, BOP_DIAG2Q, BOP_SCALE, BOP_MOMENTS, BOP_RDM1Q,
-       BOP_COUNT };
+       BOP_GATE12, BOP_COUNT };
 // Graphs per size class of an operation: tile and job bounds rounded up to
 // powers of 4 (grids near the work of the call; blocks past it exit).
 constexpr int BCLS_T = 12, BCLS_J = 8;
@@ -1260,9 +1269,10 @@ static void plan_sequence(BPlan* P, int op, int64_t tiles, int jobs, hipStream_t
   switch (op) {
     case BOP_GATE1: seq_lift(P->ws, P->B, 1, MODE_GATE, P->tiles_x, P->tiles_a, tiles, jobs, s); break;
     case BOP_GATE2: seq_lift(P->ws, P->B, 2, MODE_GATE, P->tiles_x, P->tiles_a, tiles, jobs, s); break;
+    case BOP_GATE12: seq_lift(P->ws, P->B, 3, MODE_GATE, P->tiles_x, P->tiles_a, tiles, jobs, s); break;
     case BOP_ROT1: seq_lift(P->ws, P->B, 1, MODE_ROT, P->tiles_x, P->tiles_a, tiles, jobs, s); break;
     case BOP_ROT2: seq_lift(P->ws, P->B, 2, MODE_ROT, P->tiles_x, P->tiles_a, tiles, jobs, s); break;
-    case BOP_MOVE: seq_lift(P->ws, P->B, 1, MODE_MOVE, nullptr, nullptr, tiles, jobs, s); break;
+    case BOP_MOVE: seq_lift(P->ws, P->B, 0, MODE_MOVE, nullptr, nullptr, tiles, jobs, s); break;
     case BOP_DIAG2Q:
     case BOP_SCALE:
       launch_ew_batch(op == BOP_DIAG2Q ? 0 : 1, at<EwJob>(P->ws, P->B.ewjobs), at<BatchHdr>(P->ws, P->B.hdr_e), jobs,
@@ -1357,13 +1367,14 @@ int bass_bplan_run(void* plan, int op, const bass_job* jobs, int njobs, void* ou
   void* ws = P->ws;
   const BLayout& B = P->B;
   static const char* names[BOP_COUNT] = {"batch_gate1q", "batch_gate2q", "batch_rot1q", "batch_rot2q", "batch_move",
-                                         "batch_diag2q", "batch_scale", "batch_moments", "batch_rdm"};
+                                         "batch_diag2q", "batch_scale", "batch_moments", "batch_rdm", "batch_gate12"};
   int rc = BASS_OK;
-  if (op <= BOP_MOVE) {
-    const int nq = (op == BOP_GATE2 || op == BOP_ROT2) ? 2 : 1;
+  if (op <= BOP_MOVE || op == BOP_GATE12) {
+    const int nq = op == BOP_GATE12 ? 0 : ((op == BOP_GATE2 || op == BOP_ROT2) ? 2 : 1);
     const int mode = op == BOP_MOVE ? MODE_MOVE : ((op == BOP_ROT1 || op == BOP_ROT2) ? MODE_ROT : MODE_GATE);
     int64_t T = 0;
-    rc = build_lift_tables(jobs, njobs, nq, mode, ws, B, P->tiles_x, P->tiles_a, P->X, &T, "bass_bplan_run");
+    int qmask = 0;
+    rc = build_lift_tables(jobs, njobs, nq, mode, ws, B, P->tiles_x, P->tiles_a, P->X, &T, &qmask, "bass_bplan_run");
     if (rc) return rc;
     const int tk = ktimer_begin(names[op], s);
     rc = stage_copy(at<void>(ws, B.hdr_l), P->X.hl.data(), P->X.hl.size(), s);
@@ -1435,7 +1446,8 @@ int bass_batch_step(const bass_job* jobs, int njobs, int nq, int mode, int wide,
   if (B.end > L.total) return fail(BASS_E_WORKSPACE, "bass_batch_step: batch does not fit");
   thread_local BTables X;
   int64_t Tb = 0;
-  int rc = build_lift_tables(jobs, njobs, nq, mode, ws, B, tiles_x, tiles_a, X, &Tb, "bass_batch_step");
+  int qmask = 0;
+  int rc = build_lift_tables(jobs, njobs, nq, mode, ws, B, tiles_x, tiles_a, X, &Tb, &qmask, "bass_batch_step");
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   static const char* names[2][2] = {{"batch_gate1q", "batch_gate2q"}, {"batch_rot1q", "batch_rot2q"}};
@@ -1443,7 +1455,7 @@ int bass_batch_step(const bass_job* jobs, int njobs, int nq, int mode, int wide,
   rc = stage_copy(at<void>(ws, B.hdr_l), X.hl.data(), X.hl.size(), s);
   if (!rc) rc = stage_copy(at<void>(ws, B.probs), X.pr.data(), sizeof(SelProblem) * njobs, s);
   if (rc) return rc;
-  seq_lift(ws, B, nq, mode, tiles_x, tiles_a, T, njobs, s);
+  seq_lift(ws, B, qmask, mode, tiles_x, tiles_a, T, njobs, s);
   ktimer_end(tk, s);
   return hip_check("bass_batch_step");
 }
@@ -1457,13 +1469,14 @@ int bass_batch_move(const bass_job* jobs, int njobs, void* ws, size_t ws_bytes, 
   if (B.end > L.total) return fail(BASS_E_WORKSPACE, "bass_batch_move: batch does not fit");
   thread_local BTables X;
   int64_t Tb = 0;
-  int rc = build_lift_tables(jobs, njobs, 1, MODE_MOVE, ws, B, nullptr, nullptr, X, &Tb, "bass_batch_move");
+  int qmask = 0;
+  int rc = build_lift_tables(jobs, njobs, 1, MODE_MOVE, ws, B, nullptr, nullptr, X, &Tb, &qmask, "bass_batch_move");
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   const int tk = ktimer_begin("batch_move", s);
   rc = stage_copy(at<void>(ws, B.hdr_l), X.hl.data(), X.hl.size(), s);
   if (rc) return rc;
-  seq_lift(ws, B, 1, MODE_MOVE, nullptr, nullptr, T, njobs, s);
+  seq_lift(ws, B, 0, MODE_MOVE, nullptr, nullptr, T, njobs, s);
   ktimer_end(tk, s);
   return hip_check("bass_batch_move");
 }

@@ -134,9 +134,12 @@ struct LiftJob {
   const int64_t* cnt_in;  // device count cell of the input, or null (n as staged)
   int64_t* cnt_out;       // move: count cell of the output (set to n), or null
   int j;              // bit to lift
-  int pad;
+  int qb;             // gate qubits (1, 2), or 0 for a move: tile size and lifted-to bit
   GateMat g;
 };
+// Tile entries and the bit lifted to, per job kind (LiftJob::qb).
+__host__ __device__ inline int lift_job_tm(int qb) { return qb == 0 ? 1024 : SEG_ENTRIES >> qb; }
+__host__ __device__ inline int lift_job_t(int qb) { return qb == 2 ? 1 : 0; }
 // Tile-slot bases of the select's per-problem candidate arrays (the prep
 // kernel points each problem at its own tiles).
 struct SelBase {
@@ -152,12 +155,13 @@ struct BatchZero {
   unsigned int* hist1;
   unsigned int* hist2;
 };
-// Prep of a lift / step sequence: counts, tile geometry, header, zeroing
-// (probs == nullptr: a move).  grid_zero blocks clear the histograms.
-int launch_batch_prep(LiftJob* d_jobs, SelProblem* d_probs, BatchHdr* hdr, int tm, const SelBase& sb,
-                      const BatchZero& z, hipStream_t s);
-// tiles / jobs: bounds of the tile and job counts (size the persistent grids)
-int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, const BatchHdr* hdr, int64_t tiles, int jobs,
+// Prep of a lift / step sequence: counts, tile geometry (tiles of each job's
+// kind), header, zeroing (probs == nullptr: a move).
+int launch_batch_prep(LiftJob* d_jobs, SelProblem* d_probs, BatchHdr* hdr, const SelBase& sb, const BatchZero& z,
+                      hipStream_t s);
+// qmask: gate kinds present (bit 0: 1-qubit jobs, bit 1: 2-qubit jobs; 0 = a
+// move); tiles / jobs: bounds of the tile and job counts (size the grids)
+int launch_lift_batch(int qmask, int mode, const LiftJob* d_jobs, const BatchHdr* hdr, int64_t tiles, int jobs,
                       LiftBound* bounds, uint64_t* xo, double2* ao, unsigned int* hist, double* tile_psum,
                       int* seg_cnt, hipStream_t s);
 int lift_tile_entries(int qb, int mode);

@@ -794,7 +794,7 @@ __device__ __forceinline__ int job_of_tile(const LiftJob* jobs, int njobs, int64
 // staged tile capacity runs nothing (nrun = 0) and reports ERR_OVERFLOW in
 // every job's stats.
 __global__ __launch_bounds__(NTB) void k_bprep(LiftJob* __restrict__ jobs, SelProblem* __restrict__ probs,
-                                               BatchHdr* hdr, int tm, SelBase sb, BatchZero z) {
+                                               BatchHdr* hdr, SelBase sb, BatchZero z) {
   const int njobs = hdr->njobs;
   if (blockIdx.x > 0) {
     if (!probs) return;
@@ -826,6 +826,7 @@ __global__ __launch_bounds__(NTB) void k_bprep(LiftJob* __restrict__ jobs, SelPr
       n = J.cnt_in ? *J.cnt_in : J.n;
       if (n < 0) n = 0;
       J.n = n;
+      const int tm = lift_job_tm(J.qb);
       tiles = (int)((n + tm - 1) / tm);
       if (!probs && J.cnt_out) *J.cnt_out = n;  // a move keeps the count
       atomicMax(&s_max, tiles);
@@ -862,7 +863,7 @@ __global__ __launch_bounds__(NTB) void k_bprep(LiftJob* __restrict__ jobs, SelPr
 }
 
 __global__ __launch_bounds__(NT) void k_blift_part(const LiftJob* __restrict__ jobs, const BatchHdr* __restrict__ hdr,
-                                                    int t, int tm, LiftBound* bounds) {
+                                                    LiftBound* bounds) {
   const int lane = threadIdx.x & 63;
   const int njobs = hdr->nrun;
   const int64_t nbound = hdr->nbound;
@@ -881,7 +882,7 @@ __global__ __launch_bounds__(NT) void k_blift_part(const LiftJob* __restrict__ j
     lo = __builtin_amdgcn_readfirstlane(lo);
     const LiftJob& J = jobs[lo];
     const int64_t u = gb - (J.tile0 + lo);
-    const LiftBound r = lift_boundary(J.x, J.n, J.j, t, u * (int64_t)tm);
+    const LiftBound r = lift_boundary(J.x, J.n, J.j, lift_job_t(J.qb), u * (int64_t)lift_job_tm(J.qb));
     if (lane == 0) bounds[gb] = r;
   }
 }
@@ -896,6 +897,7 @@ __global__ __launch_bounds__(NT) void k_blift_gate(const LiftJob* __restrict__ j
   for (int64_t gt = blockIdx.x; gt < ntiles; gt += gridDim.x) {
     const int c = job_of_tile(jobs, njobs, gt);
     const LiftJob& J = jobs[c];
+    if (J.qb != QB) continue;  // another kind's tile (uniform per block)
     const int64_t u = gt - J.tile0;
     const int64_t b = J.tile0 + c + u;
     lift_gate_tile<QB, MODE>(J.x, J.a, J.n, J.j, bounds[b], bounds[b + 1], J.g, u, gt, xo, ao,
@@ -1215,21 +1217,19 @@ int64_t lift_bounds_entries(int64_t nbound) { return nbound + lift_coarse_count(
 
 int lift_tile_entries(int qb, int mode) { return mode == MODE_MOVE ? 1024 : SEG_ENTRIES / (1 << qb); }
 
-// Batched lift: geometry in hdr (k_bprep first), `grid` persistent blocks.
-int launch_batch_prep(LiftJob* d_jobs, SelProblem* d_probs, BatchHdr* hdr, int tm, const SelBase& sb,
-                      const BatchZero& z, hipStream_t s) {
-  hipLaunchKernelGGL(k_bprep, dim3(d_probs ? 65 : 1), dim3(NTB), 0, s, d_jobs, d_probs, hdr, tm, sb, z);
+// Batched lift: geometry in hdr (k_bprep first), persistent grids.
+int launch_batch_prep(LiftJob* d_jobs, SelProblem* d_probs, BatchHdr* hdr, const SelBase& sb, const BatchZero& z,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_bprep, dim3(d_probs ? 65 : 1), dim3(NTB), 0, s, d_jobs, d_probs, hdr, sb, z);
   return (int)hipGetLastError();
 }
 
-int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, const BatchHdr* hdr, int64_t tiles, int jobs,
+int launch_lift_batch(int qmask, int mode, const LiftJob* d_jobs, const BatchHdr* hdr, int64_t tiles, int jobs,
                       LiftBound* bounds, uint64_t* xo, double2* ao, unsigned int* hist, double* tile_psum,
                       int* seg_cnt, hipStream_t s) {
-  const int tm = lift_tile_entries(qb, mode);
-  const int t = (mode == MODE_MOVE) ? 0 : qb - 1;
   // boundaries: one wave each, WAVES per block
   const dim3 gp(persist_grid(reinterpret_cast<const void*>(&k_blift_part), NT, (tiles + jobs + WAVES) / WAVES));
-  hipLaunchKernelGGL(k_blift_part, gp, dim3(NT), 0, s, d_jobs, hdr, t, tm, bounds);
+  hipLaunchKernelGGL(k_blift_part, gp, dim3(NT), 0, s, d_jobs, hdr, bounds);
 #define BASS_BLG(QB_, MODE_)                                                                                  \
   hipLaunchKernelGGL((k_blift_gate<QB_, MODE_>),                                                               \
                      dim3(persist_grid(reinterpret_cast<const void*>(&k_blift_gate<QB_, MODE_>), NT, tiles)), \
@@ -1237,16 +1237,12 @@ int launch_lift_batch(int qb, int mode, const LiftJob* d_jobs, const BatchHdr* h
   if (mode == MODE_MOVE) {
     hipLaunchKernelGGL(k_blift_move, dim3(persist_grid(reinterpret_cast<const void*>(&k_blift_move), NT, tiles)),
                        dim3(NT), 0, s, d_jobs, hdr, bounds);
-  } else if (qb == 1) {
-    if (mode == MODE_GATE)
-      BASS_BLG(1, MODE_GATE);
-    else
-      BASS_BLG(1, MODE_ROT);
+  } else if (mode == MODE_GATE) {
+    if (qmask & 1) BASS_BLG(1, MODE_GATE);
+    if (qmask & 2) BASS_BLG(2, MODE_GATE);
   } else {
-    if (mode == MODE_GATE)
-      BASS_BLG(2, MODE_GATE);
-    else
-      BASS_BLG(2, MODE_ROT);
+    if (qmask & 1) BASS_BLG(1, MODE_ROT);
+    if (qmask & 2) BASS_BLG(2, MODE_ROT);
   }
 #undef BASS_BLG
   return (int)hipGetLastError();

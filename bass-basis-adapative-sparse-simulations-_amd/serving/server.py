@@ -279,7 +279,9 @@ class BatchServer:
     max_circuits / slab_entries (24 B entries, two slabs) are per engine;
     hbm_budget is shared evenly; slab_entries=None sizes the slabs from it
     (slab_entries_for), so admission is bound by the budget and the slots.
-    Engine keywords (optimize_every, ...) pass through.  After a run,
+    pipelined / graphs choose the engine (PipelinedBatchEngine with or without
+    HIP graph replay, or BatchEngine); engine keywords (optimize_every, ...)
+    pass through.  After a run,
     peak_footprint_bytes is the sum over engines of their peak admitted
     footprint.  Use as a context manager."""
 

@@ -717,8 +717,13 @@ class PipelinedBatchEngine(BatchEngine):
         last = ii + 1 >= self.lens[g]
         post = np.where(last, 2, np.where(self._g_chk[idx], 1, 0))
         one = nq == 1
+        # the general gates of both widths go into one step (one lift pass per
+        # width, one select over all of them)
+        parts = []
         if (~one).any():
-            self._issue_2q(ii[~one], g[~one], idx[~one], post[~one])
+            p2 = self._issue_2q(ii[~one], g[~one], idx[~one], post[~one])
+            if p2 is not None:
+                parts.append(p2)
         if one.any():
             cs, gi = g[one], idx[one]
             q = self._g_q[gi, 0]
@@ -729,14 +734,25 @@ class PipelinedBatchEngine(BatchEngine):
             self.pos[cs] = _lift_rows(self.pos[cs], jb, 0)
             J = self._jobs_dev(cs)
             J["j"] = jb
+            J["nq"] = 1
             J["g"][:, :8] = np.ascontiguousarray(G).reshape(len(cs), 4).view(np.float64)
             J["key_map"] = _key_maps(self.pos[cs])
-            self._run(L.BOP_GATE1, J, "gate", cs, extra=(ii[one], post[one]))
+            parts.append((cs, J, ii[one], post[one]))
             self._count("gate1q", len(cs))
+        if len(parts) == 1:
+            cs, J, pi, pp = parts[0]
+            self._run(L.BOP_GATE1 if J["nq"][0] == 1 else L.BOP_GATE2, J, "gate", cs, extra=(pi, pp))
+        elif parts:
+            cs = np.concatenate([p[0] for p in parts])
+            J = np.concatenate([p[1] for p in parts])
+            self._run(L.BOP_GATE12, J, "gate", cs, extra=(np.concatenate([p[2] for p in parts]),
+                                                          np.concatenate([p[3] for p in parts])))
         self.gidx[g] += 1
         self.act[g] = np.where(post > 0, _A_WAIT, _A_GATE)
 
     def _issue_2q(self, ii, cs, gi, post):
+        """Diagonal gates (launched) and moves (launched); returns the general
+        gates' step table (cs, jobs, gate indices, post) or None."""
         q1 = self._g_q[gi, 0].copy()
         q2 = self._g_q[gi, 1].copy()
         gm = self._g_m[gi]
@@ -755,7 +771,7 @@ class PipelinedBatchEngine(BatchEngine):
             self._count("diag2q", len(d))
         gen = ~diag
         if not gen.any():
-            return
+            return None
         cs, q1, q2, G, ii, post = cs[gen], q1[gen], q2[gen], G[gen], ii[gen], post[gen]
         sw = (self.pos[cs, q1] == 0) & (self.pos[cs, q2] != 0)
         if sw.any():
@@ -774,10 +790,11 @@ class PipelinedBatchEngine(BatchEngine):
         self.pos[cs] = _lift_rows(self.pos[cs], jb, 1)
         J = self._jobs_dev(cs)
         J["j"] = jb
+        J["nq"] = 2
         J["g"][:, :32] = np.ascontiguousarray(G).reshape(len(cs), 16).view(np.float64)
         J["key_map"] = _key_maps(self.pos[cs])
-        self._run(L.BOP_GATE2, J, "gate", cs, extra=(ii, post))
         self._count("gate2q", len(cs))
+        return cs, J, ii, post
 
     def _issue_sums(self, cs):
         self._run(L.BOP_MOMENTS, self._jobs(cs), "sums", cs)

@@ -72,6 +72,8 @@ typedef struct bass_job {
   const int64_t* cnt_in; /* device count cell of the input, or NULL: n entries.  When set,
                             the call reads the count on the device (n bounds it) */
   int64_t* cnt_out;   /* device count cell the call sets to its output count, or NULL */
+  int32_t nq;         /* BASS_BOP_GATE12: this job's gate qubits (1 or 2) */
+  int32_t pad;
 } bass_job;
 
 int bass_abi_version(void);
@@ -355,7 +357,8 @@ enum {
   BASS_BOP_DIAG2Q = 5,
   BASS_BOP_SCALE = 6,
   BASS_BOP_MOMENTS = 7, /* staged counts only */
-  BASS_BOP_RDM1Q = 8    /* staged counts only */
+  BASS_BOP_RDM1Q = 8,   /* staged counts only */
+  BASS_BOP_GATE12 = 9   /* gates of 1 and 2 qubits (bass_job.nq) in one lift pass pair and one select */
 };
 size_t bass_bplan_ws_bytes(int jobs_cap, int64_t tiles_cap);
 int bass_bplan_create(int jobs_cap, int64_t tiles_cap, uint64_t* tiles_x, double* tiles_a, void* ws,

@@ -151,10 +151,12 @@ def test_no_gpu_means_loud_failure():
 
 
 def test_job_table_layout():
-    """bass_job as the ctypes table sees it: 608 bytes, the count cells last."""
+    """bass_job as the ctypes table sees it: 616 bytes, the count cells and the
+    per-job gate width last."""
     L = pkg_module("_lib")
-    assert L.JOB_DTYPE.itemsize == 608
+    assert L.JOB_DTYPE.itemsize == 616
     assert L.JOB_DTYPE.fields["cnt_in"][1] == 592 and L.JOB_DTYPE.fields["cnt_out"][1] == 600
+    assert L.JOB_DTYPE.fields["nq"][1] == 608
 
 
 def test_batch_plan_arguments_without_gpu(lib):
@@ -173,3 +175,30 @@ def test_batch_plan_arguments_without_gpu(lib):
     rc = lib.bass_bplan_run(None, 0, None, 1, None, None)
     assert rc == L.BASS_E_ARG and "plan" in L.last_error()
     assert lib.bass_bplan_destroy(None) == L.BASS_OK
+
+
+def test_batch_plan_table_checks_without_gpu(lib):
+    """bass_bplan_run validates its job table on the host before any device
+    work: tile capacity, gate widths of a mixed step."""
+    L = pkg_module("_lib")
+    O = pkg_module("_ops")
+    fake = ctypes.c_void_p(1 << 20)  # never dereferenced: the calls fail on the host
+    h = ctypes.c_void_p()
+    nb = lib.bass_bplan_ws_bytes(4, 2)
+    rc = lib.bass_bplan_create(4, 2, fake, fake, fake, nb, 0, ctypes.byref(h))
+    assert rc == L.BASS_OK and h.value
+    try:
+        J = O.new_jobs(2)
+        J["x"] = J["a"] = J["xo"] = J["ao"] = 1 << 20
+        J["k"] = J["cap_out"] = 4096
+        J["n"] = 4096  # 4 one-qubit tiles per job: more than the plan's 2
+        rc = lib.bass_bplan_run(h, L.BOP_GATE1, J.ctypes.data_as(ctypes.c_void_p), 2, None, None)
+        assert rc == L.BASS_E_OVERFLOW and "tiles" in L.last_error()
+        J["n"] = 1
+        J["nq"] = [1, 3]
+        rc = lib.bass_bplan_run(h, L.BOP_GATE12, J.ctypes.data_as(ctypes.c_void_p), 2, None, None)
+        assert rc == L.BASS_E_ARG and "gate qubits" in L.last_error()
+        rc = lib.bass_bplan_run(h, L.BOP_GATE1, J.ctypes.data_as(ctypes.c_void_p), 5, None, None)
+        assert rc == L.BASS_E_ARG and "plan holds" in L.last_error()
+    finally:
+        lib.bass_bplan_destroy(h)
