@@ -50,12 +50,14 @@ def densify(ctx, src, seg_tiles, n, dst):
     L.check(rc, "bass_densify")
 
 
-def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE, count=True):
+def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE, count=True, spec=False):
     """Key-ordered gate (mode GATE / ROT) or bit move (MODE_MOVE) on src[:n].
 
     Gates: returns (n_out, seg_tiles) with the output segmented in dst; with
     count=False n_out is None and nothing waits for the device (a following
-    truncate(..., segmented=True) reports it as stats.n_in).
+    truncate(..., segmented=True) reports it as stats.n_in).  spec=True (with
+    count=False): also the truncate's digit-1 pass against the last select's
+    bucket; pass spec=True to the truncate_launch that follows.
     Move: returns (n, 0) with the output dense in dst."""
     tm = 1024 if mode == L.MODE_MOVE else SEG // (1 << nq)
     tiles = (n + tm - 1) // tm
@@ -70,6 +72,8 @@ def lift(ctx, src, n, j, dst, G=None, nq=1, mode=L.MODE_GATE, count=True):
     # count=False: a segmented truncate reads the output next, which needs no
     # scan of its tile counts
     fl = mode | (L.LIFT_NO_SCAN if (not count and mode != L.MODE_MOVE) else 0)
+    if spec and mode != L.MODE_MOVE:
+        fl |= L.LIFT_SPEC
     rc = ctx.lib.bass_lift(src.xp, src.ap, n, nq, int(j), gp, fl, dst.xp, dst.ap, dst.cap, ctypes.byref(st),
                            ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_lift")
@@ -174,10 +178,11 @@ def key_map_arg(pos):
     return km
 
 
-def truncate_launch(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False, key_map=None):
+def truncate_launch(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=False, key_map=None, spec=False):
     """The launches of truncate() without waiting for the device: returns the
     handle truncate_finish() takes.  Nothing may launch a stats-writing
-    operation on the context in between."""
+    operation on the context in between.  spec=True: src is the output of a
+    lift(..., spec=True) issued just before."""
     n = 0 if n is None else n
     # the select reads whole tiles of SEG slots, and hist_tiles per-tile sums
     # of the producing gate (segmented: SEG slots a tile; dense: emission tiles
@@ -191,7 +196,7 @@ def truncate_launch(ctx, src, n, k, dst, renorm=True, hist_tiles=0, segmented=Fa
     else:
         km = np.ascontiguousarray(key_map, dtype=np.int32)
         kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
-    flags = 1 if segmented else 0
+    flags = (1 if segmented else 0) | (L.TRUNC_SPEC if spec else 0)
     args = (src.xp, src.ap, n, k, dst.xp, dst.ap, dst.cap, 1 if renorm else 0, hist_tiles, flags, kp, kb)
     rc = ctx.lib.bass_truncate(*args, ctx.wsp, ctx.ws_bytes, ctx.stream)
     L.check(rc, "bass_truncate")

@@ -554,7 +554,7 @@ static int lift_impl(const uint64_t* x, const double* a, int64_t n, int nq, int 
   const bool no_scan = (mode & 16) != 0;
   const bool spec = (mode & 32) != 0;
   mode &= ~48;
-  if (spec && mode != MODE_GATE) return fail(BASS_E_ARG, "bass_lift: the speculative pass needs a gate");
+  if (spec && mode == MODE_MOVE) return fail(BASS_E_ARG, "bass_lift: the speculative pass needs a gate or rotation");
   if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_lift: nq must be 1 or 2");
   if (mode != MODE_GATE && mode != MODE_ROT && mode != MODE_MOVE) return fail(BASS_E_ARG, "bass_lift: bad mode");
   if (mode == MODE_MOVE && nq != 1) return fail(BASS_E_ARG, "bass_lift: a move lifts one bit");

@@ -1271,6 +1271,8 @@ int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n
         BASS_LG(1, MODE_GATE, true);
       else if (mode == MODE_GATE)
         BASS_LG(1, MODE_GATE, false);
+      else if (spec)
+        BASS_LG(1, MODE_ROT, true);
       else
         BASS_LG(1, MODE_ROT, false);
     } else {
@@ -1278,6 +1280,8 @@ int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n
         BASS_LG(2, MODE_GATE, true);
       else if (mode == MODE_GATE)
         BASS_LG(2, MODE_GATE, false);
+      else if (spec)
+        BASS_LG(2, MODE_ROT, true);
       else
         BASS_LG(2, MODE_ROT, false);
     }

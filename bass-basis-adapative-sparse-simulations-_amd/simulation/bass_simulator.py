@@ -563,9 +563,12 @@ class BASS:
         count)."""
         jb, Vh, trial_pos, km = args
         ctx = self._ctx
-        _, rht = _ops.lift(ctx, state._buf, state.nnz, jb, state._rot, G=Vh, nq=1, mode=L.MODE_ROT, count=False)
+        # the speculative digit-1 pass against the last select's bucket (a
+        # trial's threshold is near the state's)
+        _, rht = _ops.lift(ctx, state._buf, state.nnz, jb, state._rot, G=Vh, nq=1, mode=L.MODE_ROT, count=False,
+                           spec=self.speculate)
         handle = _ops.truncate_launch(ctx, state._rot, None, self.k, cand, renorm=True, hist_tiles=rht,
-                                      segmented=True, key_map=km)
+                                      segmented=True, key_map=km, spec=self.speculate)
         return handle, trial_pos, state.nnz
 
     def _spare(self, state, name, n):

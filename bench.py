@@ -102,7 +102,7 @@ def _spread(items, m):
     return items[::step][:m]
 
 
-def bench_batched(args):
+def bench_batched(args, emit=True):
     """Config 3: independent TFIM circuits (n ~ U[16,48], steps ~ U[10,60],
     h ~ U[0.2,2], J = 1, dt = 0.05, K = 2^U{10..18}) run as one batch; baseline: the same circuits one at a time through the plain driver
     (a --single-sample subset, timed, as circuits/s)."""
@@ -166,7 +166,9 @@ def bench_batched(args):
         # HIP graphs; pipelined: direct launches; sync: a host read per tick)
         "engines": engines,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_tfim20(args):
@@ -339,7 +341,7 @@ def bench_sort(args):
     print(json.dumps(out), flush=True)
 
 
-def bench_served(args):
+def bench_served(args, emit=True):
     """Config 4: Poisson stream (--rate, or a sweep over --rates) of 50/50 TFIM
     and brickwork circuits, HBM-budget admission (200 GB), p50/p99 latency.
     --server engine: continuous batching (BatchServer, slabs sized from the
@@ -401,7 +403,9 @@ def bench_served(args):
         "speedup_vs_one_at_a_time": (head["circuits_per_s"] / single_rate) if single_rate else None,
         "sweep": sweep,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def bench_fixed(args):
@@ -594,13 +598,16 @@ def cpu_sample(sim, circ, workload, N, K, seconds):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10, help="timed steps (circuit layers)")
-    ap.add_argument("--warmup", type=int, default=4, help="untimed steps before them")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (circuit layers); default: the rest of the circuit after the warm-up "
+                         "(kicked Ising 56 of 60, brickwork 25 of 30)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps before them (default 4; brickwork 5)")
     ap.add_argument("--workload", choices=["kicked-ising", "brickwork", "batched", "served", "tfim20", "sort",
                                            "fixed-brickwork"],
                     default="kicked-ising")
     ap.add_argument("--no-secondary", action="store_true",
-                    help="kicked-ising: skip the brickwork n=40 K=2^22 line reported beside it")
+                    help="kicked-ising: skip the lines reported beside it (brickwork n=40 K=2^22, the batched and "
+                         "served configurations)")
     ap.add_argument("--circuits", type=int, default=512, help="batched / served: circuits in the stream")
     ap.add_argument("--single-sample", type=int, default=24,
                     help="batched / served: circuits timed one at a time for the baseline")
@@ -623,6 +630,9 @@ def main():
                     help="sort: force the digit width (0 = automatic)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
+    if args.calibrate_copy is not None or args.workload not in ("kicked-ising", "brickwork"):
+        args.warmup = 4 if args.warmup is None else args.warmup
+        args.steps = 10 if args.steps is None else args.steps
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs a sparse state on a single MI355X "
               "(independent circuits would shard by circuit across GPUs); use --gpus 1", file=sys.stderr)
@@ -659,8 +669,12 @@ def main():
     importlib.import_module(PKG + "._lib").load()
     if args.workload == "brickwork":
         N, K = args.qubits or 40, 1 << (args.log2k or 22)
+        args.warmup = 5 if args.warmup is None else args.warmup
+        args.steps = 25 if args.steps is None else args.steps
     else:
         N, K = args.qubits or 56, 1 << (args.log2k or 26)
+        args.warmup = 4 if args.warmup is None else args.warmup
+        args.steps = 56 if args.steps is None else args.steps
     res = run_plain(args.workload, N, K, args.warmup, args.steps)
     sim, circ = res.pop("_sim"), res.pop("_circ")
     cpu = None if args.no_cpu else cpu_sample(sim, circ, args.workload, N, K, args.cpu_seconds)
@@ -674,6 +688,16 @@ def main():
         secondary = {"metric": "sparse amp-updates/s", "value": sec["value"], "unit": "amp-updates/s",
                      "ms_per_step": sec["ms_per_step"], "steps": 25, "warmup": 5, "config": sec["config"],
                      "roofline": sec["roofline"], "kernels": sec["kernels"]}
+    batched = served = None
+    if args.workload == "kicked-ising" and not args.no_secondary:
+        # the batched (512 TFIM circuits) and served (Poisson 200 req/s)
+        # configurations, each against the same circuits one at a time
+        torch.cuda.empty_cache()
+        batched = bench_batched(args, emit=False)
+        torch.cuda.empty_cache()
+        if args.workers is None:
+            args.workers = 2
+        served = bench_served(args, emit=False)
     out = {
         "metric": "sparse amp-updates/s",
         "value": res["value"],
@@ -693,6 +717,8 @@ def main():
         "warmup_step_ms": res["warmup_step_ms"],
         "cpu_baseline": cpu,
         "secondary": secondary,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 
