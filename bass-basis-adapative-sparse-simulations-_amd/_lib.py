@@ -26,6 +26,8 @@ MODE_MOVE = 2
 LIFT_NO_SCAN = 16  # bass_lift mode flag: no output-count scan (a segmented truncate reads the output next)
 LIFT_SPEC = 32  # bass_lift mode flag: the truncate's digit-1 pass against the predicted bucket
 TRUNC_SPEC = 8  # bass_truncate in_flags bit: the input comes from a LIFT_SPEC lift issued just before
+GT_COUNTED = 1  # bass_gate_truncate flag: the input count from the state-count slot
+GT_SPEC = 2  # bass_gate_truncate flag: the speculative digit-1 pass
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -80,6 +82,7 @@ SIGNATURES = {
     "bass_counted_read": (_I, [_P, _SZ, _P, _I64, _P, _P]),
     "bass_truncate": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _I64, _I, _P, _I, _P, _SZ, _P]),
     "bass_truncate_noalloc": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _P, _I, _P, _P]),
+    "bass_gate_truncate": (_I, [_P, _P, _I64, _I, _I, _P, _I, _I64, _P, _P, _I64, _I, _I, _P, _I, _P, _SZ, _P]),
     "bass_diag2q": (_I, [_P, _P, _I64, _I, _I, _P, _P]),
     "bass_moments": (_I, [_P, _I64, _P, _SZ, _P]),
     "bass_scale": (_I, [_P, _I64, _D, _P]),

@@ -124,6 +124,32 @@ def truncate_counted(ctx, src, k, dst, hist_tiles, key_map=None, spec=False):
     L.check(rc, "bass_truncate")
 
 
+def gate_truncate(ctx, src, n, j, dst, k, G, nq=1, mode=L.MODE_GATE, renorm=True, counted=False, spec=False,
+                  key_map=None):
+    """A gate or rotation lift of src[:n] followed by the top-k select of its
+    output into dst (<= k entries), the output never stored
+    (bass_gate_truncate): bit for bit lift(..., count=False, spec) +
+    truncate_launch(..., segmented=True, spec), or with counted=True (n bounds
+    the count in the state-count slot) lift_counted + truncate_counted.
+    dst must not share memory with src.  Nothing waits for the device; a
+    non-counted call's stats come from truncate_finish(ctx, handle)."""
+    tm = SEG // (1 << nq)
+    tiles = (n + tm - 1) // tm
+    ctx.ensure_ws(max(tiles * SEG, n, 1))
+    dst.ensure(k)
+    buf, gp = L.mat_arg(G)
+    if key_map is None:
+        km, kp, kb = None, None, 0
+    else:
+        km = np.ascontiguousarray(key_map, dtype=np.int32)
+        kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
+    fl = (L.GT_COUNTED if counted else 0) | (L.GT_SPEC if spec else 0)
+    rc = ctx.lib.bass_gate_truncate(src.xp, src.ap, int(n), int(nq), int(j), gp, int(mode), int(k), dst.xp, dst.ap,
+                                    dst.cap, 1 if renorm else 0, fl, kp, kb, ctx.wsp, ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_gate_truncate")
+    return None, (km, buf)
+
+
 def diag2q_counted(ctx, buf, n_max, q1, q2, diag):
     """diag2q() on the state's count from the state-count slot (<= n_max)."""
     d = np.ascontiguousarray(np.asarray(diag, dtype=np.complex128)).view(np.float64)
@@ -209,6 +235,8 @@ def truncate_finish(ctx, handle):
     st = ctx.read_stats("bass_truncate")
     if st.flags & 2:
         # very many entries tie at the threshold: repeat with the multi-block path
+        if args is None:  # a fused gate select resolves every threshold itself
+            raise L.BassError("bass_gate_truncate: threshold left unresolved")
         a = list(args)
         a[9] = (a[9] | 2) & ~L.TRUNC_SPEC
         rc = ctx.lib.bass_truncate(*a, ctx.wsp, ctx.ws_bytes, ctx.stream)

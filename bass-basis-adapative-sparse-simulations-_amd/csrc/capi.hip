@@ -169,6 +169,7 @@ static int64_t* seg_chunk_off(void* ws, const Layout& L) { return seg_chunk_cnt(
 
 static SelTiles tiles_view(void* ws, const Layout& L, unsigned long long* c1p, unsigned int* c1i) {
   SelTiles T;
+  memset(&T, 0, sizeof(T));
   T.tile_psum = at<double>(ws, L.tile_psum);
   T.c1p = c1p;
   T.c1i = c1i;
@@ -191,6 +192,32 @@ static SelTiles tiles_view(void* ws, const Layout& L, unsigned long long* c1p, u
   T.ch_s4 = (double*)(ch + 5 * nc);
   return T;
 }
+
+// Candidate arrays of a select over `slots` input slots, in the workspace's
+// big region: c1p, c1i, c2p, c2i, tk, and (fused gate selects) c1k.
+struct CandRegion {
+  unsigned long long* c1p;
+  unsigned int* c1i;
+  unsigned long long* c2p;
+  unsigned int* c2i;
+  uint64_t* tk;
+  unsigned long long* c1k;
+  size_t bytes;
+};
+static CandRegion cand_region(void* ws, const Layout& L, int64_t slots) {
+  char* big = at<char>(ws, L.big);
+  const size_t p8 = al((size_t)8 * slots), p4 = al((size_t)4 * slots);
+  CandRegion c;
+  c.c1p = (unsigned long long*)big;
+  c.c1i = (unsigned int*)(big + p8);
+  c.c2p = (unsigned long long*)(big + p8 + p4);
+  c.c2i = (unsigned int*)(big + 2 * p8 + p4);
+  c.tk = (uint64_t*)(big + 2 * p8 + 2 * p4);
+  c.c1k = (unsigned long long*)(big + 3 * p8 + 2 * p4);
+  c.bytes = 4 * p8 + 2 * p4;
+  return c;
+}
+
 
 // ------------------------------------------------------------- timing ----
 struct TimerSlot {
@@ -341,6 +368,21 @@ static int noalloc_scratch(void** out) {
     addr[dev] = p;
   }
   *out = addr[dev];
+  return BASS_OK;
+}
+
+static int parse_key_map(const int32_t* key_map, int key_bits, KeyMap& km, const char* who) {
+  memset(&km, 0, sizeof(km));
+  if (!key_map) return BASS_OK;
+  if (key_bits < 1 || key_bits > 63) return fail(BASS_E_ARG, "%s: key_bits out of range", who);
+  uint64_t seen = 0;
+  for (int p = 0; p < key_bits; ++p) {
+    const int q = key_map[p];
+    if (q < 0 || q >= key_bits || ((seen >> q) & 1ull)) return fail(BASS_E_ARG, "%s: key_map is not a permutation", who);
+    seen |= 1ull << q;
+    km.q[p] = (unsigned char)q;
+  }
+  km.nbits = key_bits;
   return BASS_OK;
 }
 
@@ -600,12 +642,12 @@ static int lift_impl(const uint64_t* x, const double* a, int64_t n, int nq, int 
     // the candidate arrays where bass_truncate places them for an input of
     // `tiles` segments
     const int64_t slots = tiles * SEG_ENTRIES;
-    const size_t p8 = al((size_t)8 * slots), p4 = al((size_t)4 * slots);
-    if (3 * p8 + 2 * p4 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_lift: candidates do not fit");
-    char* big = at<char>(ws, L.big);
+    const CandRegion C = cand_region(ws, L, slots);
+    if (C.bytes - al((size_t)8 * slots) > L.total - L.big)
+      return fail(BASS_E_WORKSPACE, "bass_lift: candidates do not fit");
     sp.pred = at<unsigned int>(ws, L.spec);
-    sp.out.c1p = (unsigned long long*)big;
-    sp.out.c1i = (unsigned int*)(big + p8);
+    sp.out.c1p = C.c1p;
+    sp.out.c1i = C.c1i;
     sp.out.cnt1 = at<int>(ws, L.cnt1);
     sp.out.tile_dsum = at<double>(ws, L.tile_dsum);
     sp.out.tile_dcnt = at<int>(ws, L.tile_dcnt);
@@ -635,7 +677,8 @@ int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t
 
 static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
                          int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags, const int32_t* key_map,
-                         int key_bits, void* ws, size_t ws_bytes, void* stream, bool timed = true);
+                         int key_bits, void* ws, size_t ws_bytes, void* stream, bool timed = true,
+                         const LiftRecompute* fused = nullptr);
 
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
                   int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags, const int32_t* key_map,
@@ -644,9 +687,13 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
                        stream);
 }
 
+// fused: the input is the unstored output of a fused lift (k_lift_gate with
+// FUSE) on this workspace; x is ignored (the candidates' keys are read from
+// the candidate region), and the select recomputes the lift's tiles.
 static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
                          int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags, const int32_t* key_map,
-                         int key_bits, void* ws, size_t ws_bytes, void* stream, bool timed) {
+                         int key_bits, void* ws, size_t ws_bytes, void* stream, bool timed,
+                         const LiftRecompute* fused) {
   const bool segmented = (in_flags & 1) != 0;
   const bool wide = (in_flags & 2) != 0;
   const bool counted = (in_flags & 4) != 0;
@@ -658,19 +705,8 @@ static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t 
   if (n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_truncate: n too large");
   if (segmented && hist_tiles <= 0) return fail(BASS_E_ARG, "bass_truncate: segmented input needs hist_tiles");
   KeyMap km;
-  memset(&km, 0, sizeof(km));
-  if (key_map) {
-    if (key_bits < 1 || key_bits > 63) return fail(BASS_E_ARG, "bass_truncate: key_bits out of range");
-    uint64_t seen = 0;
-    for (int p = 0; p < key_bits; ++p) {
-      const int q = key_map[p];
-      if (q < 0 || q >= key_bits || ((seen >> q) & 1ull))
-        return fail(BASS_E_ARG, "bass_truncate: key_map is not a permutation");
-      seen |= 1ull << q;
-      km.q[p] = (unsigned char)q;
-    }
-    km.nbits = key_bits;
-  }
+  const int krc = parse_key_map(key_map, key_bits, km, "bass_truncate");
+  if (krc != BASS_OK) return krc;
   const int64_t tiles = segmented ? hist_tiles : sel_tiles(n);
   const int64_t slots = tiles * SEG_ENTRIES;
   // per-tile sums read: a segmented input's tiles hold SEG_ENTRIES slots, a
@@ -684,13 +720,14 @@ static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t 
   // header B (digit-2 and wide histograms, select state, tickets) is cleared by
   // k_find1, the select's first kernel after the histogram pass
   if (hist_tiles <= 0) fill(at<void>(ws, L.hist1), HIST_BINS * 4, 0ull, s);
-  char* big = at<char>(ws, L.big);
-  const size_t p8 = al((size_t)8 * slots), p4 = al((size_t)4 * slots);
-  if (3 * p8 + 2 * p4 > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_truncate: candidates do not fit");
+  const CandRegion C = cand_region(ws, L, slots);
+  const size_t cbytes = fused ? C.bytes : C.bytes - al((size_t)8 * slots);
+  if (cbytes > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_truncate: candidates do not fit");
   SelProblem P;
   memset(&P, 0, sizeof(P));
-  P.x = x;
-  P.a = (const double2*)a;
+  P.x = fused ? (const uint64_t*)C.c1k : x;
+  P.a = fused ? nullptr : (const double2*)a;
+  P.fused = fused ? 1 : 0;
   P.seg_cnt = segmented ? at<int>(ws, L.seg_cnt) : nullptr;
   P.n_dense = n;
   P.tile0 = 0;
@@ -699,7 +736,7 @@ static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t 
   P.nrows = select_rows(tiles, false);
   P.npsum = hist_tiles > 0 ? hist_tiles : tiles;
   P.chunked = sel_chunks(P.npsum > tiles ? P.npsum : tiles) > 1 ? 1 : 0;
-  P.finish_any = counted ? 1 : 0;
+  P.finish_any = (counted || fused) ? 1 : 0;
   P.k = (unsigned long long)k;
   P.xo = xo;
   P.ao = (double2*)ao;
@@ -710,24 +747,104 @@ static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t 
   P.hist_w = at<unsigned int>(ws, L.hist_w);
   P.sel = at<SelState>(ws, L.sel);
   P.st = st;
-  P.c2p = (unsigned long long*)(big + p8 + p4);
-  P.c2i = (unsigned int*)(big + 2 * p8 + p4);
-  P.tk = (uint64_t*)(big + 2 * p8 + 2 * p4);
+  P.c2p = C.c2p;
+  P.c2i = C.c2i;
+  P.tk = C.tk;
   P.zero_hdr = at<unsigned long long>(ws, L.hist2);
   P.zero_words = (int64_t)((L.hdrB_end - L.hist2) / 8);
   P.km = km;
   P.ktab = km.nbits ? at<uint64_t>(ws, L.ktab) : nullptr;
   P.spec_word = at<unsigned int>(ws, L.spec);
   P.spec_in = spec_in && !wide ? 1 : 0;
-  const SelTiles T = tiles_view(ws, L, (unsigned long long*)big, (unsigned int*)(big + p8));
+  SelTiles T = tiles_view(ws, L, C.c1p, C.c1i);
+  if (fused) T.c1k = C.c1k;
   const int tk = timed ? ktimer_begin("select", s) : -1;
-  launch_select(P, T, hist_tiles > 0, wide, s);
+  launch_select(P, T, hist_tiles > 0, wide, s, fused);
   ktimer_end(tk, s);
   if (counted)
     hipLaunchKernelGGL(k_oplog_append, dim3(1), dim3(1), 0, s, (const OpStats*)st,
                        at<unsigned long long>(ws, L.oplog), (OpStats*)(at<char>(ws, L.oplog) + 8),
                        at<int64_t>(ws, L.count_slot));
   return hip_check("bass_truncate");
+}
+
+int bass_gate_truncate(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G, int mode,
+                       int64_t k, uint64_t* xo, double* ao, int64_t cap_out, int renorm, int flags,
+                       const int32_t* key_map, int key_bits, void* ws, size_t ws_bytes, void* stream) {
+  const bool counted = (flags & 1) != 0;
+  const bool spec = (flags & 2) != 0;
+  if (flags & ~3) return fail(BASS_E_ARG, "bass_gate_truncate: unknown flags 0x%x", flags);
+  if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_gate_truncate: nq must be 1 or 2");
+  if (mode != MODE_GATE && mode != MODE_ROT)
+    return fail(BASS_E_ARG, "bass_gate_truncate: mode must be 0 (gate) or 1 (rotation)");
+  if (j < nq - 1 || j > 62) return fail(BASS_E_ARG, "bass_gate_truncate: bad bit %d for %d qubit(s)", j, nq);
+  if (n < 1 || n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_gate_truncate: bad n %lld", (long long)n);
+  if (k < 1) return fail(BASS_E_ARG, "bass_gate_truncate: need k >= 1");
+  if (cap_out < k) return fail(BASS_E_OVERFLOW, "bass_gate_truncate: cap_out < k");
+  if (!x || !a || !xo || !ao || !G) return fail(BASS_E_ARG, "bass_gate_truncate: null pointer");
+  {
+    // the compaction reads the input while it writes the output
+    const char *xb = (const char*)x, *ab = (const char*)a, *xob = (const char*)xo, *aob = (const char*)ao;
+    const size_t ni = (size_t)n, no = (size_t)cap_out;
+    auto overlap = [](const char* p, size_t pn, const char* q, size_t qn) { return p < q + qn && q < p + pn; };
+    if (overlap(xb, 8 * ni, xob, 8 * no) || overlap(xb, 8 * ni, aob, 16 * no) || overlap(ab, 16 * ni, xob, 8 * no) ||
+        overlap(ab, 16 * ni, aob, 16 * no))
+      return fail(BASS_E_ARG, "bass_gate_truncate: the output overlaps the input");
+  }
+  KeyMap km;
+  const int krc = parse_key_map(key_map, key_bits, km, "bass_gate_truncate");
+  if (krc != BASS_OK) return krc;
+  const int tm = SEG_ENTRIES >> nq;
+  const int64_t tiles = (n + tm - 1) / tm;
+  const int64_t slots = tiles * SEG_ENTRIES;
+  WS_REQUIRE(slots > n ? slots : n);
+  const CandRegion C = cand_region(ws, L, slots);
+  if (C.bytes > L.total - L.big) return fail(BASS_E_WORKSPACE, "bass_gate_truncate: candidates do not fit");
+  hipStream_t s = (hipStream_t)stream;
+  OpStats* st = at<OpStats>(ws, L.st);
+  GateMat gm;
+  memset(&gm, 0, sizeof(gm));
+  const int M = 1 << nq;
+  for (int e = 0; e < M * M; ++e) gm.m[e] = make_double2(G[2 * e], G[2 * e + 1]);
+  // cleared by the lift's first boundary kernel: the digit-1 histogram and
+  // the first 48 bytes of the stats block (err untouched)
+  LiftZero zero;
+  zero.p = at<unsigned long long>(ws, L.hist1);
+  zero.wp = HIST_BINS * 4 / 8;
+  zero.q = (unsigned long long*)st;
+  zero.wq = 48 / 8;
+  LiftSpec sp;
+  memset(&sp, 0, sizeof(sp));
+  if (spec) {
+    sp.pred = at<unsigned int>(ws, L.spec);
+    sp.out.c1p = C.c1p;
+    sp.out.c1i = C.c1i;
+    sp.out.cnt1 = at<int>(ws, L.cnt1);
+    sp.out.tile_dsum = at<double>(ws, L.tile_dsum);
+    sp.out.tile_dcnt = at<int>(ws, L.tile_dcnt);
+    sp.out.c1k = C.c1k;
+  }
+  const int64_t* n_dev = counted ? at<int64_t>(ws, L.count_slot) : nullptr;
+  LiftBound* bounds = at<LiftBound>(ws, L.lift_b);
+  static const char* names[2][2] = {{"gate1q", "gate2q"}, {"rot1q", "rot2q"}};
+  const int tk = ktimer_begin(names[mode][nq - 1], s);
+  launch_lift(nq, mode, x, (const double2*)a, n, j, gm, nullptr, nullptr, bounds, at<unsigned int>(ws, L.hist1),
+              at<double>(ws, L.tile_psum), at<int>(ws, L.seg_cnt), s, n_dev, zero, spec ? &sp : nullptr, true);
+  ktimer_end(tk, s);
+  const int hrc = hip_check("bass_gate_truncate");
+  if (hrc != BASS_OK) return hrc;
+  LiftRecompute R;
+  R.qb = nq;
+  R.mode = mode;
+  R.x = x;
+  R.a = (const double2*)a;
+  R.n = n;
+  R.n_dev = n_dev;
+  R.j = j;
+  R.bounds = bounds;
+  R.G = gm;
+  return truncate_impl(nullptr, nullptr, 0, k, xo, ao, cap_out, renorm, tiles, 1 | (counted ? 4 : 0) | (spec ? 8 : 0),
+                       key_map, key_bits, ws, ws_bytes, stream, true, &R);
 }
 
 int bass_truncate_noalloc(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,

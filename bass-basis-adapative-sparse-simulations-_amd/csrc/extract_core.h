@@ -13,6 +13,38 @@ namespace bass {
 constexpr int SEL_ITEMS = SEG_ENTRIES / NT;  // 8
 constexpr int SEL_TILE = SEG_ENTRIES;
 
+__device__ __forceinline__ unsigned long long pbits_of(double2 v) {
+  return (unsigned long long)__double_as_longlong(cabs2(v));
+}
+
+// Logical key of a stored word (ties are ordered by it).
+__device__ __forceinline__ uint64_t logical_key(uint64_t w, const KeyMap& km) {
+  if (km.nbits == 0) return w;
+  uint64_t r = 0;
+  for (int p = 0; p < km.nbits; ++p) r |= ((w >> p) & 1ull) << km.q[p];
+  return r;
+}
+
+// The same through byte tables (tab: KTAB_WORDS words, global or LDS).
+__device__ __forceinline__ uint64_t logical_key_tab(uint64_t w, const uint64_t* tab) {
+  uint64_t r = 0;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) r |= tab[b * 256 + (int)((w >> (8 * b)) & 255u)];
+  return r;
+}
+
+// Logical key of stored word w of problem P.
+__device__ __forceinline__ uint64_t lkey(const SelProblem& P, uint64_t w) {
+  if (P.km.nbits == 0) return w;
+  if (P.ktab) return logical_key_tab(w, P.ktab);
+  return logical_key(w, P.km);
+}
+
+__device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t w, unsigned long long pstar,
+                                          unsigned long long keystar, const SelProblem& P) {
+  return bits > pstar || (bits == pstar && lkey(P, w) <= keystar);
+}
+
 // s_hist[bin] += 1 for the active lanes of the wave: the lanes sharing the
 // first remaining lane's bin are added by one atomic, for up to 4 distinct
 // bins; lanes left after that add one each.
@@ -39,10 +71,13 @@ __device__ __forceinline__ void wave_hist_add(unsigned int* s_hist, unsigned int
 // their digit-2 counts into s_hist.  Candidate flags are per-item wave ballots
 // (scalar registers); a candidate's offset is its item's and wave's base (one
 // wave scans the (item, wave) counts) plus its rank among the wave's set lanes.
-template <bool HIST>
+// KEYS: vk holds the slots' stored words in the same order; candidates get
+// their word in c1k and their own list index in c1i.
+template <bool HIST, bool KEYS = false>
 __device__ __forceinline__ void ex1_tile_core(const Ex1Out& O, int64_t gt, int64_t base, int cnt,
                                               const double (&vp)[SEL_ITEMS], unsigned int b1,
-                                              unsigned int* s_hist, int* s_wcnt, double* s_red, int* s_dcnt) {
+                                              unsigned int* s_hist, int* s_wcnt, double* s_red, int* s_dcnt,
+                                              const uint64_t* vk = nullptr) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (threadIdx.x == 0) *s_dcnt = 0;
   __syncthreads();
@@ -81,7 +116,12 @@ __device__ __forceinline__ void ex1_tile_core(const Ex1Out& O, int64_t gt, int64
       const int off =
           o + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mk, 0u));
       O.c1p[cbase + off] = (unsigned long long)__double_as_longlong(vp[it]);
-      O.c1i[cbase + off] = (unsigned int)(base + it * NT + threadIdx.x);
+      if constexpr (KEYS) {
+        O.c1k[cbase + off] = vk[it];
+        O.c1i[cbase + off] = (unsigned int)(cbase + off);
+      } else {
+        O.c1i[cbase + off] = (unsigned int)(base + it * NT + threadIdx.x);
+      }
     }
   }
   const double s = block_sum(dsum, s_red);

@@ -172,6 +172,10 @@ struct Ex1Out {
   int* cnt1;
   double* tile_dsum;
   int* tile_dcnt;
+  // keys of the candidates (a lift whose output is not stored): c1k[i] is the
+  // stored word of candidate i and c1i[i] = i, so a select reading keys as
+  // x[c1i[i]] reads them from x = c1k
+  unsigned long long* c1k;
 };
 
 // A gate lift that also runs the select's digit-1 pass on each output tile,
@@ -185,7 +189,7 @@ struct LiftSpec {
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
                 hipStream_t s, const int64_t* n_dev = nullptr, const LiftZero& zero = LiftZero(),
-                const LiftSpec* spec = nullptr);
+                const LiftSpec* spec = nullptr, bool fuse = false);
 
 // One state of a batched elementwise operation (see launch_ew_batch):
 // op 0 diagonal 2-qubit gate a *= d[(b_j1 << 1) | b_j2], op 1 a /= div,
@@ -251,6 +255,10 @@ struct SelProblem {
   int chunked;           // per-tile reductions run by chunk (single large problems)
   int finish_any;        // k_finish resolves any candidate count itself (no wide-path hand-off)
   int spec_in;           // the input comes from a speculative lift (LiftSpec) launched just before
+  // the input is a gate lift's output that was not stored (fused gate select):
+  // x is the candidate key list (SelTiles::c1k), a is unused; the digit-1 pass
+  // after a missed speculation and the compaction recompute the lift's tiles
+  int fused;
   unsigned int* hist1;   // HIST_BINS, zeroed by the caller unless filled by the producing gate
   unsigned int* hist2;   // HIST_BINS, zeroed
   unsigned int* hist_w;  // HIST_BINS, zeroed (wide path)
@@ -290,6 +298,7 @@ struct SelTiles {
   double* tile_s2;
   double* tile_s4;
   unsigned short* hist2_rows;
+  unsigned long long* c1k;  // candidate keys of a fused gate select (Ex1Out::c1k), else null
   // per chunk of SEL_CHUNK tiles (chunked problems): kept count, kept-entry
   // offset, kept / input / output sums
   int64_t* ch_cnt;
@@ -306,7 +315,25 @@ constexpr int64_t SEL_CHUNK = 4096;
 inline int64_t sel_chunks(int64_t ntiles) { return (ntiles + SEL_CHUNK - 1) / SEL_CHUNK; }
 
 int select_rows(int64_t ntiles, bool batch);
-int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s);
+// The lift a fused gate select recomputes: its input state, the bit it lifts,
+// its tile boundaries (still in the workspace from the lift) and matrix.
+struct LiftRecompute {
+  int qb, mode;
+  const uint64_t* x;
+  const double2* a;
+  int64_t n;
+  const int64_t* n_dev;  // counted input: the count, or null
+  int j;
+  const LiftBound* bounds;
+  GateMat G;
+};
+// Fused select kernels over the lift's tiles: which = 0 the digit-1 pass when
+// the speculation missed (ext tiles per digit-2 row), 1 the compaction.
+int launch_lift_recompute(int which, const LiftRecompute& R, const SelProblem& P, const SelTiles& T, int ext,
+                          hipStream_t s);
+int extract1_tiles(int64_t ntiles);
+int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s,
+                  const LiftRecompute* fused = nullptr);
 // A batch of problems with the geometry in hdr (prep kernel); tiles / jobs:
 // bounds of the tile and problem counts (size the persistent grids).  Every
 // threshold is resolved in the sequence (one block per problem).

@@ -405,7 +405,7 @@ __device__ __forceinline__ void lift_halo_setup(const uint64_t* __restrict__ w, 
   H.prev[1] = B0.b > B0.sm ? lift_group_of<QB>(lift_key(w[B0.b - 1], j, T)) : NONE;
   // members of the tile's last group merged after the tile
   int nh = 0;
-  const uint64_t glast = lift_group_of<QB>(s_t[cnt - 1]);
+  const uint64_t glast = cnt > 0 ? lift_group_of<QB>(s_t[cnt - 1]) : NONE;
   const int64_t cand[3] = {B1.a < B1.sm ? (int64_t)B1.a : -1, B1.b < B1.s1 ? (int64_t)B1.b : -1,
                            B1.b + 1 < B1.s1 ? (int64_t)B1.b + 1 : -1};
   for (int c = 0; c < 3; ++c) {
@@ -473,50 +473,59 @@ __device__ __forceinline__ bool lift_group_gather(int m, int cnt, const uint64_t
   return true;
 }
 
-// One tile of a gate: local tile `tile` of the input (merged positions
-// [tile*TM, ...)), outputs to segment `otile` of (xo, ao), seg_cnt[otile],
-// tile_psum[otile]; digit-1 histogram into hist.  SPEC: also the select's
-// digit-1 pass on the tile (ex1_tile_core, bucket *S.pred) from its |a|^2
-// values staged in LDS in slot order.
-template <int QB, int MODE, bool SPEC = false>
-__device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a,
-                                               int64_t n, int j, const LiftBound& B0, const LiftBound& B1,
-                                               const GateMat& G, int64_t tile, int64_t otile,
-                                               uint64_t* __restrict__ xo, double2* __restrict__ ao,
-                                               unsigned int* hist, double* tile_psum, int* seg_cnt,
-                                               LiftSpec S = LiftSpec()) {
+// LDS of one lift tile (QB gate bits, TM = SEG_ENTRIES >> QB merged input
+// positions): merge ranks, staged lifted keys and amplitudes, scan and
+// reduction scratch, the halo.
+template <int QB>
+struct LiftTileLds {
+  static constexpr int TM = SEG_ENTRIES >> QB;
+  static constexpr int ITEMS = TM / NT;
+  unsigned short rank[TM];
+  uint64_t t[TM];
+  double2 a[TM];
+  int scan[ITEMS * WAVES];
+  double red[WAVES];
+  LiftHalo halo;
+};
+
+// A lift tile's outputs in registers.  Item it (merged position it * NT +
+// threadIdx.x, when it heads its group) holds the M = 2^QB outputs of its
+// group; output o exists when bit o of mask[it] is set.  The existing outputs
+// of item it take the tile's output slots off[it], off[it] + 1, ... in o order
+// (slots run item-major, then thread order); total = the tile's output count.
+template <int QB>
+struct LiftTileOut {
+  static constexpr int M = 1 << QB;
+  static constexpr int ITEMS = (SEG_ENTRIES >> QB) / NT;
+  uint64_t key[ITEMS][M];
+  double re[ITEMS][M], im[ITEMS][M];
+  unsigned int mask[ITEMS];
+  int off[ITEMS];
+  int total;
+};
+
+// Stages, merges and gates local tile `tile` of the input (merged positions
+// [tile * TM, ...)) into Q.  Ends with a barrier; L.t and L.a are free after it.
+template <int QB, int MODE>
+__device__ __forceinline__ void lift_tile_compute(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                                  int64_t n, int j, const LiftBound& B0, const LiftBound& B1,
+                                                  const GateMat& G, int64_t tile, LiftTileLds<QB>& L,
+                                                  LiftTileOut<QB>& Q) {
   constexpr int M = 1 << QB;
   constexpr int T = QB - 1;
   constexpr int TM = SEG_ENTRIES / M;
   constexpr int ITEMS = TM / NT;
-  __shared__ unsigned short s_rank[TM];
-  __shared__ uint64_t s_t[TM];
-  __shared__ double2 s_a[TM];
-  // digit-1 bins below 1024 (|a|^2 < 2, every normalised state) count in LDS;
-  // higher bins go straight to the global histogram
-  constexpr int LBINS = HIST_BINS / 2;
-  __shared__ unsigned int s_hist[LBINS];
-  __shared__ int s_scan[ITEMS * WAVES];
-  __shared__ double s_red[WAVES];
-  __shared__ LiftHalo s_halo;
-
-  for (int b = threadIdx.x; b < LBINS; b += NT) s_hist[b] = 0;
-  const int cnt = lift_stage<TM>(w, a, n, j, T, B0, B1, tile, s_rank, s_scan, s_t, s_a);
-  lift_halo_setup<QB>(w, a, j, B0, B1, s_t, cnt, s_halo);
+  const int cnt = lift_stage<TM>(w, a, n, j, T, B0, B1, tile, L.rank, L.scan, L.t, L.a);
+  lift_halo_setup<QB>(w, a, j, B0, B1, L.t, cnt, L.halo);
   __syncthreads();
-
-  uint64_t okey[ITEMS][M];
-  double ore[ITEMS][M], oim[ITEMS][M];
-  unsigned int omask[ITEMS];
-  int ooff[ITEMS];
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
     const int m = it * NT + threadIdx.x;
     unsigned int mask = 0;
     double avr[M], avi[M];
     unsigned int pres;
-    if (m < cnt && lift_group_gather<QB>(m, cnt, s_t, s_a, s_halo, avr, avi, pres)) {
-      const uint64_t base = s_t[m] & ~(uint64_t)(M - 1);
+    if (m < cnt && lift_group_gather<QB>(m, cnt, L.t, L.a, L.halo, avr, avi, pres)) {
+      const uint64_t base = L.t[m] & ~(uint64_t)(M - 1);
 #pragma unroll
       for (int o = 0; o < M; ++o) {
         double2 acc = make_double2(0.0, 0.0);
@@ -536,40 +545,107 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
           for (int c = 1; c < M; ++c) acc = cadd(acc, cmul(G.m[o * M + c], make_double2(avr[c], avi[c])));
           any = hypot(acc.x, acc.y) > 1e-30;
         }
-        okey[it][o] = base | (uint64_t)o;
-        ore[it][o] = acc.x;
-        oim[it][o] = acc.y;
+        Q.key[it][o] = base | (uint64_t)o;
+        Q.re[it][o] = acc.x;
+        Q.im[it][o] = acc.y;
         if (any) mask |= 1u << o;
       }
     }
-    omask[it] = mask;
+    Q.mask[it] = mask;
   }
   // output offsets in merged (item-major) order: one scan for all items
   int ocnt[ITEMS];
 #pragma unroll
-  for (int it = 0; it < ITEMS; ++it) ocnt[it] = __popc(omask[it]);
-  int tile_total;
-  block_excl_scan_items<ITEMS>(ocnt, ooff, tile_total, s_scan);
-  if (threadIdx.x == 0) seg_cnt[otile] = tile_total;
+  for (int it = 0; it < ITEMS; ++it) ocnt[it] = __popc(Q.mask[it]);
+  block_excl_scan_items<ITEMS>(ocnt, Q.off, Q.total, L.scan);
+}
+
+// Per-output values v[it][o] (existing outputs only) read back in slot order:
+// out[s] = the value of slot s * NT + threadIdx.x, 0 past the tile's outputs.
+// buf: LDS for CAP = 2 * TM values of 8 bytes (the staged amplitudes' space,
+// free once the outputs are computed); SEG_ENTRIES / CAP passes, each ending
+// with a barrier.
+template <int QB, class V>
+__device__ __forceinline__ void lift_to_slots(const LiftTileOut<QB>& Q,
+                                              const V (&v)[LiftTileOut<QB>::ITEMS][LiftTileOut<QB>::M], V* buf,
+                                              V (&out)[SEL_ITEMS]) {
+  constexpr int M = 1 << QB;
+  constexpr int TM = SEG_ENTRIES / M;
+  constexpr int ITEMS = TM / NT;
+  constexpr int CAP = 2 * TM;
+  static_assert(SEG_ENTRIES % CAP == 0 && CAP % NT == 0, "passes cover whole items");
+#pragma unroll
+  for (int pass = 0; pass < SEG_ENTRIES / CAP; ++pass) {
+    const int lo = pass * CAP;
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it) {
+      int pos = Q.off[it];
+#pragma unroll
+      for (int o = 0; o < M; ++o) {
+        if ((Q.mask[it] >> o) & 1u) {
+          if (pos >= lo && pos < lo + CAP) buf[pos - lo] = v[it][o];
+          ++pos;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = lo / NT; it < (lo + CAP) / NT; ++it) {
+      const int jj = it * NT + threadIdx.x;
+      out[it] = jj < Q.total ? buf[jj - lo] : V(0);
+    }
+    __syncthreads();
+  }
+}
+
+// One tile of a gate: local tile `tile` of the input (merged positions
+// [tile*TM, ...)), outputs to segment `otile` of (xo, ao), seg_cnt[otile],
+// tile_psum[otile]; digit-1 histogram into hist.  SPEC: also the select's
+// digit-1 pass on the tile (ex1_tile_core, bucket *S.pred) from its |a|^2
+// values staged in LDS in slot order.  FUSE: the outputs are not stored (a
+// fused select recomputes them); the speculative pass keeps the candidates'
+// keys (S.out.c1k).
+template <int QB, int MODE, bool SPEC = false, bool FUSE = false>
+__device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a,
+                                               int64_t n, int j, const LiftBound& B0, const LiftBound& B1,
+                                               const GateMat& G, int64_t tile, int64_t otile,
+                                               uint64_t* __restrict__ xo, double2* __restrict__ ao,
+                                               unsigned int* hist, double* tile_psum, int* seg_cnt,
+                                               LiftSpec S = LiftSpec()) {
+  constexpr int M = 1 << QB;
+  constexpr int TM = SEG_ENTRIES / M;
+  constexpr int ITEMS = TM / NT;
+  __shared__ LiftTileLds<QB> L;
+  // digit-1 bins below 1024 (|a|^2 < 2, every normalised state) count in LDS;
+  // higher bins go straight to the global histogram
+  constexpr int LBINS = HIST_BINS / 2;
+  __shared__ unsigned int s_hist[LBINS];
+
+  for (int b = threadIdx.x; b < LBINS; b += NT) s_hist[b] = 0;
+  LiftTileOut<QB> Q;
+  lift_tile_compute<QB, MODE>(w, a, n, j, B0, B1, G, tile, L, Q);
+  if (threadIdx.x == 0) seg_cnt[otile] = Q.total;
 
   double psum = 0.0;
-  // SPEC: |a|^2 of the outputs, staged in s_a (free: every gather ended
+  // SPEC: |a|^2 of the outputs, staged in L.a (free: every gather ended
   // before the offset scan's barrier) in slot order, CAP slots per pass; one
   // pass (1 qubit) stages them here, two (2 qubits) keep them in pv
   constexpr int CAP = 2 * TM;
   constexpr bool ONE_PASS = CAP == SEG_ENTRIES;
-  double* s_p = reinterpret_cast<double*>(s_a);
+  double* s_p = reinterpret_cast<double*>(L.a);
   double pv[ITEMS][M];
   const int64_t rb = otile * SEG_ENTRIES;
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
-    int64_t pos = rb + ooff[it];
+    int64_t pos = rb + Q.off[it];
 #pragma unroll
     for (int o = 0; o < M; ++o) {
-      if ((omask[it] >> o) & 1u) {
-        const double2 v = make_double2(ore[it][o], oim[it][o]);
-        xo[pos] = okey[it][o];
-        ao[pos] = v;
+      if ((Q.mask[it] >> o) & 1u) {
+        const double2 v = make_double2(Q.re[it][o], Q.im[it][o]);
+        if constexpr (!FUSE) {
+          xo[pos] = Q.key[it][o];
+          ao[pos] = v;
+        }
         const double p = cabs2(v);
         if constexpr (SPEC && ONE_PASS) s_p[pos - rb] = p;
         if constexpr (SPEC && !ONE_PASS) pv[it][o] = p;
@@ -583,7 +659,7 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
       }
     }
   }
-  const double s = block_sum(psum, s_red);
+  const double s = block_sum(psum, L.red);
   if (threadIdx.x == 0) tile_psum[otile] = s;
   __syncthreads();
   for (int b = threadIdx.x; b < LBINS; b += NT)
@@ -592,50 +668,165 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
     // the staged |a|^2 read back in the select's slot-to-thread order
     __shared__ int s_wcnt[SEL_ITEMS * WAVES];
     __shared__ int s_dcnt;
-    static_assert(SEG_ENTRIES % CAP == 0 && CAP % NT == 0, "passes cover whole items");
     const unsigned int b1 = *S.pred;
     double vp[SEL_ITEMS];
-#pragma unroll
-    for (int pass = 0; pass < SEG_ENTRIES / CAP; ++pass) {
-      const int lo = pass * CAP;
-      if constexpr (!ONE_PASS) {
-#pragma unroll
-        for (int it = 0; it < ITEMS; ++it) {
-          int pos = ooff[it];
-#pragma unroll
-          for (int o = 0; o < M; ++o) {
-            if ((omask[it] >> o) & 1u) {
-              if (pos >= lo && pos < lo + CAP) s_p[pos - lo] = pv[it][o];
-              ++pos;
-            }
-          }
-        }
-      }
+    if constexpr (ONE_PASS) {
       __syncthreads();
 #pragma unroll
-      for (int it = lo / NT; it < (lo + CAP) / NT; ++it) {
+      for (int it = 0; it < SEL_ITEMS; ++it) {
         const int jj = it * NT + threadIdx.x;
-        vp[it] = jj < tile_total ? s_p[jj - lo] : 0.0;
+        vp[it] = jj < Q.total ? s_p[jj] : 0.0;
       }
       __syncthreads();
+    } else {
+      lift_to_slots<QB>(Q, pv, s_p, vp);
     }
-    ex1_tile_core<false>(S.out, otile, rb, tile_total, vp, b1, nullptr, s_wcnt, s_red, &s_dcnt);
+    if constexpr (FUSE) {
+      uint64_t vk[SEL_ITEMS];
+      lift_to_slots<QB>(Q, Q.key, reinterpret_cast<uint64_t*>(L.a), vk);
+      ex1_tile_core<false, true>(S.out, otile, rb, Q.total, vp, b1, nullptr, s_wcnt, L.red, &s_dcnt, vk);
+    } else {
+      ex1_tile_core<false>(S.out, otile, rb, Q.total, vp, b1, nullptr, s_wcnt, L.red, &s_dcnt);
+    }
   }
 }
 
-// (the speculative variant is held to the plain kernel's 5 waves per SIMD)
-template <int QB, int MODE, bool SPEC>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPEC ? 5 : 1))) void k_lift_gate(const uint64_t* __restrict__ w, const double2* __restrict__ a,
-                                                   int64_t n, int j, const LiftBound* __restrict__ bounds,
-                                                   GateMat G, uint64_t* __restrict__ xo, double2* __restrict__ ao,
-                                                   unsigned int* hist, double* tile_psum, int* seg_cnt,
-                                                   const int64_t* __restrict__ n_dev, LiftSpec S) {
+// (the speculative variants are held to the plain kernel's 5 waves per SIMD)
+template <int QB, int MODE, bool SPEC, bool FUSE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPEC ? 5 : 1))) void k_lift_gate(
+    const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int j,
+    const LiftBound* __restrict__ bounds, GateMat G, uint64_t* __restrict__ xo, double2* __restrict__ ao,
+    unsigned int* hist, double* tile_psum, int* seg_cnt, const int64_t* __restrict__ n_dev, LiftSpec S) {
   // tiles past a counted input's end have empty bounds: they write a zero
   // count and sum
   n = dev_count(n_dev, n);
   const int64_t tile = blockIdx.x;
-  lift_gate_tile<QB, MODE, SPEC>(w, a, n, j, bounds[tile], bounds[tile + 1], G, tile, tile, xo, ao, hist,
-                                 tile_psum, seg_cnt, S);
+  lift_gate_tile<QB, MODE, SPEC, FUSE>(w, a, n, j, bounds[tile], bounds[tile + 1], G, tile, tile, xo, ao, hist,
+                                       tile_psum, seg_cnt, S);
+}
+
+// ------------------------------------------------- fused gate select ------
+// A gate whose output goes straight into a top-k select need not store it.
+// The lift (k_lift_gate with FUSE) keeps what the select reads before its
+// compaction -- the digit-1 histogram, per-tile |a|^2 sums and output counts,
+// and from the speculative pass the candidates with their keys -- and the
+// select recomputes the lift's tiles where it needs the outputs themselves:
+// k_lift_ex1 (the digit-1 pass, when the speculation missed) and
+// k_lift_compact (the compaction).  A recomputed tile is the lift's tile bit
+// for bit (same code on the same input), so the fused select writes what
+// the stored path writes, for about half the memory traffic.
+
+// The select's digit-1 pass when the speculative bucket was not the select's:
+// block vb recomputes tiles [vb * ext, +ext) and runs ex1_tile_core on each
+// (keys into c1k), its digit-2 counts into row vb (as k_extract1's rows).
+template <int QB, int MODE>
+__global__ __launch_bounds__(NT) void k_lift_ex1(LiftRecompute R, SelProblem P, SelTiles T, int ext) {
+  constexpr int M = 1 << QB;
+  constexpr int ITEMS = (SEG_ENTRIES >> QB) / NT;
+  const SelState* sel = P.sel;
+  if (sel->keepall || sel->spec_hit) return;
+  __shared__ LiftTileLds<QB> L;
+  __shared__ unsigned int s_hist[HIST_BINS];
+  __shared__ int s_wcnt[SEL_ITEMS * WAVES];
+  __shared__ int s_dcnt;
+  const int64_t n = dev_count(R.n_dev, R.n);
+  const unsigned int b1 = sel->b1;
+  const Ex1Out O{T.c1p, T.c1i, T.cnt1, T.tile_dsum, T.tile_dcnt, T.c1k};
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  const int64_t vb = blockIdx.x;
+  for (int sub = 0; sub < ext; ++sub) {
+    const int64_t gt = vb * ext + sub;
+    if (gt >= P.ntiles) break;  // uniform
+    LiftTileOut<QB> Q;
+    lift_tile_compute<QB, MODE>(R.x, R.a, n, R.j, R.bounds[gt], R.bounds[gt + 1], R.G, gt, L, Q);
+    double pv[ITEMS][M];
+#pragma unroll
+    for (int it = 0; it < ITEMS; ++it)
+#pragma unroll
+      for (int o = 0; o < M; ++o)
+        pv[it][o] = ((Q.mask[it] >> o) & 1u) ? cabs2(make_double2(Q.re[it][o], Q.im[it][o])) : 0.0;
+    double vp[SEL_ITEMS];
+    uint64_t vk[SEL_ITEMS];
+    lift_to_slots<QB>(Q, pv, reinterpret_cast<double*>(L.a), vp);
+    lift_to_slots<QB>(Q, Q.key, reinterpret_cast<uint64_t*>(L.a), vk);
+    ex1_tile_core<true, true>(O, gt, gt * SEG_ENTRIES, Q.total, vp, b1, s_hist, s_wcnt, L.red, &s_dcnt, vk);
+  }
+  __syncthreads();
+  unsigned short* row = T.hist2_rows + vb * HIST_BINS;
+  for (int b = threadIdx.x; b < HIST_BINS; b += NT) row[b] = (unsigned short)s_hist[b];
+}
+
+// The compaction: block gt recomputes tile gt and writes its kept outputs
+// (kept_pred against the resolved threshold, as k_compact) from the tile's
+// kept offset on, in slot order, divided by the norm; the tile's sums of the
+// written |a|^2 and |a|^4 follow k_compact's order (slot it * NT + thread,
+// items in order; a slot not kept adds +0.0 to a sum of non-negative terms,
+// which leaves it unchanged).
+template <int QB, int MODE>
+__global__ __launch_bounds__(NT) void k_lift_compact(LiftRecompute R, SelProblem P, SelTiles T) {
+  constexpr int M = 1 << QB;
+  constexpr int ITEMS = (SEG_ENTRIES >> QB) / NT;
+  __shared__ LiftTileLds<QB> L;
+  const int64_t gt = blockIdx.x;
+  const int64_t n = dev_count(R.n_dev, R.n);
+  LiftTileOut<QB> Q;
+  lift_tile_compute<QB, MODE>(R.x, R.a, n, R.j, R.bounds[gt], R.bounds[gt + 1], R.G, gt, L, Q);
+  const unsigned long long pstar = P.sel->pstar, keystar = P.sel->keystar;
+  // numpy divides complex128 by a real norm as a product with 1/norm
+  const double inv = 1.0 / P.st->norm;
+  unsigned int km[ITEMS];
+  int kc[ITEMS], ko[ITEMS];
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    unsigned int mk = 0;
+#pragma unroll
+    for (int o = 0; o < M; ++o) {
+      if (((Q.mask[it] >> o) & 1u) &&
+          kept_pred(pbits_of(make_double2(Q.re[it][o], Q.im[it][o])), Q.key[it][o], pstar, keystar, P))
+        mk |= 1u << o;
+    }
+    km[it] = mk;
+    kc[it] = __popc(mk);
+  }
+  int ktot;
+  block_excl_scan_items<ITEMS>(kc, ko, ktot, L.scan);
+  const int64_t rb = T.koff[gt] + (P.chunked ? T.ch_off[(gt - P.tile0) / SEL_CHUNK] : 0);
+  double p2[ITEMS][M];
+  bool fits = true;
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    int64_t pos = rb + ko[it];
+#pragma unroll
+    for (int o = 0; o < M; ++o) {
+      p2[it][o] = 0.0;
+      if ((km[it] >> o) & 1u) {
+        const double2 v = make_double2(Q.re[it][o] * inv, Q.im[it][o] * inv);
+        if (pos < P.cap_out) {
+          P.xo[pos] = Q.key[it][o];
+          P.ao[pos] = v;
+        } else {
+          fits = false;
+        }
+        p2[it][o] = cabs2(v);
+        ++pos;
+      }
+    }
+  }
+  if (!fits) set_err(&P.st->err, ERR_OVERFLOW);
+  double vp[SEL_ITEMS];
+  lift_to_slots<QB>(Q, p2, reinterpret_cast<double*>(L.a), vp);
+  double s2 = 0.0, s4 = 0.0;
+#pragma unroll
+  for (int it = 0; it < SEL_ITEMS; ++it) {
+    s2 += vp[it];
+    s4 += vp[it] * vp[it];
+  }
+  const double t2 = block_sum(s2, L.red);
+  const double t4 = block_sum(s4, L.red);
+  if (threadIdx.x == 0) {
+    T.tile_s2[gt] = t2;
+    T.tile_s4[gt] = t4;
+  }
 }
 
 // ------------------------------------------------- two-qubit RDM ------
@@ -1248,9 +1439,29 @@ int launch_lift_batch(int qmask, int mode, const LiftJob* d_jobs, const BatchHdr
   return (int)hipGetLastError();
 }
 
+// One gate kernel instance (QB, MODE) with the SPEC / FUSE variant chosen at run time.
+template <int QB, int MODE>
+static void launch_gate_kernel(bool spec, bool fuse, dim3 grid, hipStream_t s, const uint64_t* x, const double2* a,
+                               int64_t n, int j, const LiftBound* bounds, const GateMat& G, uint64_t* xo,
+                               double2* ao, unsigned int* hist, double* tile_psum, int* seg_cnt,
+                               const int64_t* n_dev, const LiftSpec& S) {
+#define BASS_LG(SPEC_, FUSE_)                                                                               \
+  hipLaunchKernelGGL((k_lift_gate<QB, MODE, SPEC_, FUSE_>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, \
+                     hist, tile_psum, seg_cnt, n_dev, S)
+  if (spec && fuse)
+    BASS_LG(true, true);
+  else if (spec)
+    BASS_LG(true, false);
+  else if (fuse)
+    BASS_LG(false, true);
+  else
+    BASS_LG(false, false);
+#undef BASS_LG
+}
+
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
-                hipStream_t s, const int64_t* n_dev, const LiftZero& zero, const LiftSpec* spec) {
+                hipStream_t s, const int64_t* n_dev, const LiftZero& zero, const LiftSpec* spec, bool fuse) {
   if (n <= 0) return 0;
   const int tm = lift_tile_entries(qb, mode);
   const int t = (mode == MODE_MOVE) ? 0 : qb - 1;
@@ -1263,30 +1474,44 @@ int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n
     hipLaunchKernelGGL(k_lift_move<1024>, grid, dim3(NT), 0, s, x, a, n, j, bounds, xo, ao, n_dev);
   } else {
     const LiftSpec S = spec ? *spec : LiftSpec();
-#define BASS_LG(QB_, MODE_, SPEC_)                                                                          \
-  hipLaunchKernelGGL((k_lift_gate<QB_, MODE_, SPEC_>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, hist, \
-                     tile_psum, seg_cnt, n_dev, S)
-    if (qb == 1) {
-      if (mode == MODE_GATE && spec)
-        BASS_LG(1, MODE_GATE, true);
-      else if (mode == MODE_GATE)
-        BASS_LG(1, MODE_GATE, false);
-      else if (spec)
-        BASS_LG(1, MODE_ROT, true);
-      else
-        BASS_LG(1, MODE_ROT, false);
-    } else {
-      if (mode == MODE_GATE && spec)
-        BASS_LG(2, MODE_GATE, true);
-      else if (mode == MODE_GATE)
-        BASS_LG(2, MODE_GATE, false);
-      else if (spec)
-        BASS_LG(2, MODE_ROT, true);
-      else
-        BASS_LG(2, MODE_ROT, false);
-    }
-#undef BASS_LG
+    const bool sp = spec != nullptr;
+    if (qb == 1 && mode == MODE_GATE)
+      launch_gate_kernel<1, MODE_GATE>(sp, fuse, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
+                                       n_dev, S);
+    else if (qb == 1)
+      launch_gate_kernel<1, MODE_ROT>(sp, fuse, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
+                                      n_dev, S);
+    else if (mode == MODE_GATE)
+      launch_gate_kernel<2, MODE_GATE>(sp, fuse, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
+                                       n_dev, S);
+    else
+      launch_gate_kernel<2, MODE_ROT>(sp, fuse, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
+                                      n_dev, S);
   }
+  return (int)hipGetLastError();
+}
+
+int launch_lift_recompute(int which, const LiftRecompute& R, const SelProblem& P, const SelTiles& T, int ext,
+                          hipStream_t s) {
+  const int64_t tiles = P.ntiles;
+  if (tiles <= 0) return 0;
+#define BASS_LR(QB_, MODE_)                                                                                     \
+  do {                                                                                                         \
+    if (which == 0)                                                                                            \
+      hipLaunchKernelGGL((k_lift_ex1<QB_, MODE_>), dim3((unsigned)((tiles + ext - 1) / ext)), dim3(NT), 0, s, R, \
+                         P, T, ext);                                                                           \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_lift_compact<QB_, MODE_>), dim3((unsigned)tiles), dim3(NT), 0, s, R, P, T);        \
+  } while (0)
+  if (R.qb == 1 && R.mode == MODE_GATE)
+    BASS_LR(1, MODE_GATE);
+  else if (R.qb == 1)
+    BASS_LR(1, MODE_ROT);
+  else if (R.mode == MODE_GATE)
+    BASS_LR(2, MODE_GATE);
+  else
+    BASS_LR(2, MODE_ROT);
+#undef BASS_LR
   return (int)hipGetLastError();
 }
 

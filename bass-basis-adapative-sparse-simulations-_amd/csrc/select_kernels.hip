@@ -38,33 +38,6 @@
 namespace bass {
 
 
-__device__ __forceinline__ unsigned long long pbits_of(double2 v) {
-  return (unsigned long long)__double_as_longlong(cabs2(v));
-}
-
-// Logical key of a stored word (ties are ordered by it).
-__device__ __forceinline__ uint64_t logical_key(uint64_t w, const KeyMap& km) {
-  if (km.nbits == 0) return w;
-  uint64_t r = 0;
-  for (int p = 0; p < km.nbits; ++p) r |= ((w >> p) & 1ull) << km.q[p];
-  return r;
-}
-
-// The same through byte tables (tab: KTAB_WORDS words, global or LDS).
-__device__ __forceinline__ uint64_t logical_key_tab(uint64_t w, const uint64_t* tab) {
-  uint64_t r = 0;
-#pragma unroll
-  for (int b = 0; b < 8; ++b) r |= tab[b * 256 + (int)((w >> (8 * b)) & 255u)];
-  return r;
-}
-
-// Logical key of stored word w of problem P.
-__device__ __forceinline__ uint64_t lkey(const SelProblem& P, uint64_t w) {
-  if (P.km.nbits == 0) return w;
-  if (P.ktab) return logical_key_tab(w, P.ktab);
-  return logical_key(w, P.km);
-}
-
 // Byte tables of P.km into P.ktab (the threads of one block share the
 // entries; the map is staged in s_q, 64 bytes of LDS).  Uniform branch.
 template <int NTHR>
@@ -164,7 +137,6 @@ __device__ double block_sum_array(const double* v, int64_t m, double* lds) {
 constexpr int EX1_TILES = 4;         // tiles per k_extract1 block of a single problem
 constexpr int EX1_TILES_MIN = 12;    // range for large problems (row counts <= 31 * SEL_TILE fit 16 bits)
 constexpr int EX1_TILES_MAX = 31;
-int extract1_tiles(int64_t ntiles);
 
 // ------------------------------------------------------ problem lookup ----
 // Problem of a global tile: the last problem whose tile0 <= gt (problems are
@@ -378,6 +350,9 @@ __device__ __forceinline__ void extract1_body(const SelProblem* probs, int nprob
     }
     return;
   }
+  // a fused select's missed speculation: the lift's tiles are recomputed for
+  // this pass (k_lift_ex1)
+  if (!BATCH && P.fused && !sel->spec_hit) return;
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
   if (!BATCH && sel->spec_hit) {
     // the lift already wrote the per-tile counts, sums and candidates: only
@@ -1265,10 +1240,6 @@ __global__ __launch_bounds__(NT) void k_wide_key_hist(const SelProblem* probs, S
   flush_hist(s_hist, P.hist_w);
 }
 
-__device__ __forceinline__ bool kept_pred(unsigned long long bits, uint64_t w, unsigned long long pstar,
-                                          unsigned long long keystar, const SelProblem& P) {
-  return bits > pstar || (bits == pstar && lkey(P, w) <= keystar);
-}
 
 // Per tile: sum and count of the kept candidates (lane-strided, then a
 // fixed shuffle tree).  One wave per tile.
@@ -1755,7 +1726,7 @@ __global__ __launch_bounds__(NT) void k_gather(const uint64_t* __restrict__ x, c
 // The select of one problem (tiles_end = its tiles; extract1 groups several
 // tiles per block).
 static void select_one(const SelProblem& one, const SelTiles& T, int64_t tiles_end, bool hist_ready, bool wide,
-                       hipStream_t s) {
+                       hipStream_t s, const LiftRecompute* fused) {
   const SelProblem* probs = nullptr;
   const BatchHdr* hdr = nullptr;
   const int nprob = 1;
@@ -1770,6 +1741,7 @@ static void select_one(const SelProblem& one, const SelTiles& T, int64_t tiles_e
   else
     hipLaunchKernelGGL((k_extract1<false, 0>), dim3((unsigned)rows), dim3(NT), 0, s, probs, nprob, one, T,
                        tiles_end, ext, hdr);
+  if (fused) launch_lift_recompute(0, *fused, one, T, ext, s);
   hipLaunchKernelGGL((k_hist2_reduce<false>),
                      dim3(HIST_BINS / (4 * NT), (unsigned)((one.nrows + H2_ROWS - 1) / H2_ROWS), 1u), dim3(NT), 0,
                      s, probs, one, T, hdr);
@@ -1804,7 +1776,10 @@ static void select_one(const SelProblem& one, const SelTiles& T, int64_t tiles_e
   } else {
     hipLaunchKernelGGL((k_sel_finalize<false>), dim3(1), dim3(NTB), 0, s, probs, one, T, hdr);
   }
-  hipLaunchKernelGGL((k_compact<false>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T, hdr);
+  if (fused)
+    launch_lift_recompute(1, *fused, one, T, 0, s);
+  else
+    hipLaunchKernelGGL((k_compact<false>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T, hdr);
   if (one.chunked) {
     hipLaunchKernelGGL(k_sums_chunks, dim3((unsigned)sel_chunks(one.ntiles)), dim3(NTB), 0, s, one, T);
     hipLaunchKernelGGL(k_sums_final, dim3(1), dim3(NTB), 0, s, one, T, sel_chunks(one.ntiles));
@@ -1853,8 +1828,9 @@ int select_rows(int64_t ntiles, bool batch) {
   return (int)((ntiles + ext - 1) / ext);
 }
 
-int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s) {
-  select_one(P, T, P.ntiles, hist_ready, wide, s);
+int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s,
+                  const LiftRecompute* fused) {
+  select_one(P, T, P.ntiles, hist_ready, wide, s, fused);
   return (int)hipGetLastError();
 }
 

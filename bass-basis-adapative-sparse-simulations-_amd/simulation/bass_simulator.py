@@ -318,6 +318,11 @@ class BASS:
         # pass against the previous truncation's |a|^2 exponent bucket, which
         # the truncation then skips when the bucket is unchanged (same results)
         self.speculate = True
+        # gates and rotation trials whose output goes straight into a top-k
+        # truncation run as one fused call (bass_gate_truncate): the gate's
+        # output is never stored, the truncation recomputes it where it needs
+        # it (same results as the stored lift + truncate, less memory traffic)
+        self.fused = True
         # counted diagonal 2-qubit gates not yet launched: (bit1, bit2, diag);
         # a run of them goes to the device as one pass (bass_diag2q_run) when
         # the next other operation or a read of the state comes
@@ -565,6 +570,10 @@ class BASS:
         ctx = self._ctx
         # the speculative digit-1 pass against the last select's bucket (a
         # trial's threshold is near the state's)
+        if self.fused:
+            handle = _ops.gate_truncate(ctx, state._buf, state.nnz, jb, cand, self.k, Vh, nq=1, mode=L.MODE_ROT,
+                                        renorm=True, spec=self.speculate, key_map=km)
+            return handle, trial_pos, state.nnz
         _, rht = _ops.lift(ctx, state._buf, state.nnz, jb, state._rot, G=Vh, nq=1, mode=L.MODE_ROT, count=False,
                            spec=self.speculate)
         handle = _ops.truncate_launch(ctx, state._rot, None, self.k, cand, renorm=True, hist_tiles=rht,
@@ -585,10 +594,15 @@ class BASS:
         Returns (nnz, sum |a|^2 before the renormalisation, sum |a|^4 after,
         positions)."""
         ctx = self._ctx
-        _, ht = _ops.lift(ctx, src, n, j, state._rot, G=V4, nq=2, mode=L.MODE_ROT, count=False)
         tpos = lifted_positions(pos, j, 1)
-        st = _ops.truncate(ctx, state._rot, None, self.k, dst, renorm=True, hist_tiles=ht, segmented=True,
-                           key_map=_ops.key_map_arg(tpos))
+        if self.fused:
+            st = _ops.truncate_finish(ctx, _ops.gate_truncate(ctx, src, n, j, dst, self.k, V4, nq=2,
+                                                              mode=L.MODE_ROT, renorm=True,
+                                                              key_map=_ops.key_map_arg(tpos)))
+        else:
+            _, ht = _ops.lift(ctx, src, n, j, state._rot, G=V4, nq=2, mode=L.MODE_ROT, count=False)
+            st = _ops.truncate(ctx, state._rot, None, self.k, dst, renorm=True, hist_tiles=ht, segmented=True,
+                               key_map=_ops.key_map_arg(tpos))
         m = st.n_in
         self._count("rotate2")
         self._count("amp_updates", m)
@@ -719,8 +733,7 @@ class BASS:
             self._diag_flush()
             q = gate.qubits[0]
             G = self.U[q].conj().T @ gate.matrix @ self.U[q]
-            jb = int(state.pos[q])
-            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=1, spec=self.speculate)
+            jb, nq = int(state.pos[q]), 1
             state.pos = lifted_positions(state.pos, jb, 0)
             self._count("gate1q")
             entry = (i, "gate1q", False)
@@ -747,13 +760,18 @@ class BASS:
                 _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, mode=L.MODE_MOVE)
                 state._buf, state._tmp = state._tmp, state._buf
                 state.pos = lifted_positions(state.pos, jb, 0)
-            jb = int(state.pos[q1])
-            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=2, spec=self.speculate)
+            jb, nq = int(state.pos[q1]), 2
             state.pos = lifted_positions(state.pos, jb, 1)
             self._count("gate2q")
             entry = (i, "gate2q", moved)
         km = _ops.key_map_arg(state.pos)
-        _ops.truncate_counted(ctx, state._tmp, self.k, state._buf, ht, key_map=km, spec=self.speculate)
+        if self.fused:
+            _ops.gate_truncate(ctx, state._buf, self.k, jb, state._tmp, self.k, G, nq=nq, counted=True,
+                               spec=self.speculate, key_map=km)
+            state._buf, state._tmp = state._tmp, state._buf
+        else:
+            ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=nq, spec=self.speculate)
+            _ops.truncate_counted(ctx, state._tmp, self.k, state._buf, ht, key_map=km, spec=self.speculate)
         self._cpending.append(entry)
         state.nnz = None  # known at the next flush
         state._sum2 = state._sum4 = None

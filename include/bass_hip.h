@@ -216,6 +216,32 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
                   double* ao, int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags,
                   const int32_t* key_map, int key_bits, void* ws, size_t ws_bytes, void* stream);
 
+/* A gate or rotation on a key-ordered state followed by the top-k truncation
+ * of its output, without storing that output: the results (xo[:nnz], ao,
+ * the stats block, the op log and state-count slot in counted mode, the
+ * next speculative prediction) are those of
+ *   bass_lift(x, a, n, nq, j, G, mode | 16 [| 32 with flags bit 1], tmp, ...)
+ *   bass_truncate(tmp, ..., k, xo, ao, cap_out, renorm, seg_tiles,
+ *                 1 [| 4 counted] [| 8 with flags bit 1], key_map, key_bits, ...)
+ * bit for bit.  The lift keeps only what the truncation reads before its
+ * compaction (digit-1 histogram, per-tile sums and counts, the candidates of
+ * the predicted |a|^2 bucket with their keys); the truncation recomputes the
+ * lift's tiles to write the kept entries (and for its digit-1 pass when the
+ * prediction missed), so the gate's 2^nq n outputs never go to memory.
+ * Replaces, inside the driver, a gate step of
+ * src/simulation/bass_simulator.py:861-896 with its truncation at :899-916,
+ * and a rotation trial's rotate + truncate at :768-790 (no reference
+ * counterpart as one entry point).
+ * mode: 0 gate, 1 rotation (nq and j as in bass_lift).  flags bit 0
+ * (counted): n bounds the count read from the state-count slot, as
+ * bass_lift_counted + a counted bass_truncate.  flags bit 1: the speculative
+ * digit-1 pass against the last select's bucket (as mode | 32 / in_flags 8).
+ * (xo, ao) must not overlap (x, a).  The workspace must hold the sizes that
+ * bass_lift + bass_truncate need (seg_tiles * 2048 slots). */
+int bass_gate_truncate(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G, int mode,
+                       int64_t k, uint64_t* xo, double* ao, int64_t cap_out, int renorm, int flags,
+                       const int32_t* key_map, int key_bits, void* ws, size_t ws_bytes, void* stream);
+
 /* bass_truncate of a dense input (hist_tiles = 0, in_flags = 0) that
  * allocates nothing and takes no workspace: its scratch (select histograms,
  * candidate lists, per-tile sums, stats) is a static device array of the
