@@ -318,11 +318,14 @@ class BASS:
         # pass against the previous truncation's |a|^2 exponent bucket, which
         # the truncation then skips when the bucket is unchanged (same results)
         self.speculate = True
-        # gates and rotation trials whose output goes straight into a top-k
-        # truncation run as one fused call (bass_gate_truncate): the gate's
-        # output is never stored, the truncation recomputes it where it needs
-        # it (same results as the stored lift + truncate, less memory traffic)
-        self.fused = True
+        # fused=True: gates and rotation trials whose output goes straight
+        # into a top-k truncation run as one call (bass_gate_truncate) that
+        # never stores the gate's output (the truncation recomputes it where
+        # it needs it): the same results as the stored lift + truncate with
+        # about half the memory traffic and no 2-4 k-entry output buffer, but
+        # the recomputation costs what the traffic saved, and more when the
+        # speculative bucket misses, so the stored pair stays the default
+        self.fused = False
         # counted diagonal 2-qubit gates not yet launched: (bit1, bit2, diag);
         # a run of them goes to the device as one pass (bass_diag2q_run) when
         # the next other operation or a read of the state comes
@@ -389,8 +392,10 @@ class BASS:
                 self._gamma_running *= kept / total if total > 1e-30 else 0
                 self._since_trunc = 0
                 self._count("truncate")
+                self._count("truncate_" + op)
                 if r.flags & 4:  # the lift's digit-1 pass was used
                     self._count("spec_hit")
+                    self._count("spec_hit_" + op)
             else:
                 self._count("bytes_select", 48 * n)
             n_prev = int(r.nnz)
@@ -506,6 +511,9 @@ class BASS:
                 st = _ops.truncate_finish(ctx, handle)
                 m = st.n_in
                 if st.flags & 1:
+                    self._count("truncate_rot1q")
+                    if st.flags & 4:
+                        self._count("spec_hit_rot1q")
                     nn, frac, s4 = self.k, st.kept, st.sum4
                 else:
                     s2 = st.sum2

@@ -125,19 +125,19 @@ def test_gate_truncate_matches_stored_pair(gpu_ctx, n, nq, mode, k, kind, count)
     src = _upload(dev, gpu_ctx, w, aw)
     ref = _pair(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, False, count)
     keepall = (ref[2].flags & 1) == 0
-    # prediction = the stored pair's bucket: the speculation holds
+    plain = _fused(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, False, count)
+    _same(plain, ref, "no speculation")
+    # prediction = this bucket: the speculation holds
     hit = _fused(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, True, count)
     assert (hit[2].flags & 4) == (0 if keepall else 4)
     _same(hit, ref, "hit")
     # prediction from a select 16 exponents lower: the digit-1 pass recomputed
     low = _upload(dev, gpu_ctx, w, aw * 2.0 ** -8)
-    _pair(ops, dev, gpu_ctx, low, n, j, G, nq, mode, k, km, False, count)
+    _fused(ops, dev, gpu_ctx, low, n, j, G, nq, mode, k, km, False, count)
     del low
     miss = _fused(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, True, count)
     assert (miss[2].flags & 4) == 0
     _same(miss, ref, "miss")
-    plain = _fused(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, False, count)
-    _same(plain, ref, "no speculation")
     print(f"  kept {ref[2].nnz} of {ref[2].n_in} (keep-all {keepall})", flush=True)
 
 
