@@ -891,6 +891,7 @@ class BASS:
             state.nnz = self.k
             self._since_trunc = 0
             self._count("truncate")
+            self._count("truncate_" + op)
         else:
             self._count("bytes_select", 48 * n)  # kept everything: a dense copy
             state.nnz = n
