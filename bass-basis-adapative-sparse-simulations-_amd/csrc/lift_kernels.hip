@@ -192,6 +192,13 @@ __device__ __forceinline__ LiftBound lift_boundary_between(const uint64_t* __res
   return r;
 }
 
+// Lifts of up to LIFT_ONE_MAX boundaries launch the boundary passes as one
+// kernel (k_lift_part_one): it saves a launch (about 3 us of a 100 us gate at
+// 2^22 entries) but searches each coarse boundary twice and holds 14 of its 16
+// waves while the coarse searches run, which costs more than the launch from
+// about 2^24 entries on (2^26: 17 us more per 1-qubit gate).
+constexpr int64_t LIFT_ONE_MAX = 16385;
+
 // Tile boundaries in two passes: every LIFT_CS-th boundary (and the last) by
 // full searches, then every boundary between its two coarse neighbours.  The
 // full 64-ary searches touch a cache line per probe; confining them cuts the
@@ -241,6 +248,35 @@ __global__ __launch_bounds__(NT) void k_lift_part_fine(const uint64_t* __restric
   if (lane == 0) bounds[u] = r;
 }
 
+// Both passes in one launch: block b (LIFT_CS waves) writes boundaries
+// [b * LIFT_CS, min((b + 1) * LIFT_CS, nbound - 1)) -- and the last block the
+// final one -- after its waves 0 and 1 have searched the two coarse
+// boundaries around them (each coarse boundary is searched by the two blocks
+// that share it); the same values as the two-launch form.
+__global__ __launch_bounds__(LIFT_CS * 64) void k_lift_part_one(const uint64_t* __restrict__ w, int64_t n, int j0,
+                                                                int t, int tm, int64_t nbound, LiftBound* bounds,
+                                                                const int64_t* __restrict__ n_dev, LiftZero zero) {
+  __shared__ LiftBound s_c[2];
+  lift_zero(zero);
+  n = dev_count(n_dev, n);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int j = j0 + (int)blockIdx.y;
+  bounds += (int64_t)blockIdx.y * nbound;
+  const int64_t u0 = (int64_t)blockIdx.x * LIFT_CS;
+  const int64_t u1 = u0 + LIFT_CS < nbound - 1 ? u0 + LIFT_CS : nbound - 1;
+  if (wv < 2) {
+    const LiftBound r = lift_boundary(w, n, j, t, (wv == 0 ? u0 : u1) * (int64_t)tm);
+    if (lane == 0) s_c[wv] = r;
+  }
+  __syncthreads();
+  const int64_t u = u0 + wv;
+  if (u < u1) {
+    const LiftBound r = wv == 0 ? s_c[0] : lift_boundary_between(w, n, j, t, u * (int64_t)tm, s_c[0], s_c[1]);
+    if (lane == 0) bounds[u] = r;
+  }
+  if (u1 == nbound - 1 && wv == 0 && lane == 0) bounds[u1] = s_c[1];
+}
+
 // Boundaries of rows j0 .. j0 + rows - 1 (nbound per row) into bounds; the
 // coarse pass uses `coarse` (lift_coarse_count(nbound) per row).  Small
 // lifts take the direct pass.
@@ -250,6 +286,11 @@ static void lift_partition(const uint64_t* x, int64_t n, int j0, int rows, int t
   if (nbound <= 4 * LIFT_CS || coarse == nullptr) {
     hipLaunchKernelGGL(k_lift_part, dim3((unsigned)((nbound + WAVES - 1) / WAVES), (unsigned)rows), dim3(NT), 0, s,
                        x, n, j0, t, tm, nbound, bounds, n_dev, zero);
+    return;
+  }
+  if (nbound <= LIFT_ONE_MAX) {
+    hipLaunchKernelGGL(k_lift_part_one, dim3((unsigned)((nbound - 1 + LIFT_CS - 1) / LIFT_CS), (unsigned)rows),
+                       dim3(LIFT_CS * 64), 0, s, x, n, j0, t, tm, nbound, bounds, n_dev, zero);
     return;
   }
   const int64_t nc = lift_coarse_count(nbound);
