@@ -221,7 +221,9 @@ def eigh_2x2(rho):
     a = float(rho[0, 0].real)
     d = float(rho[1, 1].real)
     b = complex(rho[0, 1])
-    b_sq = b.real ** 2 + b.imag ** 2
+    # squares as products: the reference's compiled x**2 is one rounded
+    # multiplication, which Python's float ** (C pow) is not always
+    b_sq = b.real * b.real + b.imag * b.imag
     tau = (d - a) * 0.5
     disc = math.sqrt(tau * tau + b_sq)
     evals = np.array([(a + d) * 0.5 + disc, (a + d) * 0.5 - disc])

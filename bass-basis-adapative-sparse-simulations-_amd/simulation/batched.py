@@ -70,7 +70,7 @@ def _eigh2x2_rows(rho):
     a = rho[..., 0, 0].real
     d = rho[..., 1, 1].real
     b = rho[..., 0, 1]
-    b_sq = b.real ** 2 + b.imag ** 2
+    b_sq = b.real * b.real + b.imag * b.imag
     tau = (d - a) * 0.5
     disc = np.sqrt(tau * tau + b_sq)
     td_p = tau + disc

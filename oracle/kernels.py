@@ -396,7 +396,9 @@ def eigh_2x2(rho):
     a = float(rho[0, 0].real)
     d = float(rho[1, 1].real)
     b = complex(rho[0, 1])
-    b_sq = b.real ** 2 + b.imag ** 2
+    # numba compiles b.real**2 to one rounded multiplication; Python's float
+    # ** calls C pow, which differs from it by an ulp for some inputs
+    b_sq = b.real * b.real + b.imag * b.imag
     tau = (d - a) * 0.5
     disc = math.sqrt(tau * tau + b_sq)
     lam0 = (a + d) * 0.5 + disc

@@ -61,6 +61,34 @@ def test_eigh_and_kron_match_oracle_bitwise():
         np.testing.assert_allclose(B._manual_kron_2x2(A, U), np.kron(A, U), atol=1e-15)
 
 
+def test_eigh_single_batch_oracle_bitwise_where_pow_rounds_differently():
+    """The single-circuit driver's eigh_2x2, the batch engine's row-wise form
+    and the oracle's agree bit for bit, also on off-diagonals whose square by
+    C pow (Python's float **) is an ulp off the correctly rounded product --
+    the reference's compiled b.real**2 -- which once made the two drivers take
+    different rotation decisions."""
+    from fractions import Fraction
+
+    B = pkg_module("simulation.bass_simulator")
+    Bm = pkg_module("simulation.batched")
+    rng = np.random.default_rng(0)
+    rhos = []
+    while len(rhos) < 12:
+        a = rng.random()
+        b = complex(rng.standard_normal(), rng.standard_normal()) * rng.random() * 0.5 * 10 ** rng.uniform(-6, 0)
+        if b.real ** 2 != b.real * b.real or b.imag ** 2 != b.imag * b.imag:
+            for v in (b.real, b.imag):
+                assert v * v == float(Fraction(v) * Fraction(v))  # the product rounds once, correctly
+            rhos.append(np.array([[a, b], [b.conjugate(), 1 - a]]))
+    Vb, _ = Bm._eigh2x2_rows(np.array(rhos))
+    for r, vb in zip(rhos, Vb):
+        e1, V1 = B.eigh_2x2(r)
+        e2, V2 = OK.eigh_2x2(r)
+        assert np.array_equal(e1.view(np.uint64), e2.view(np.uint64))
+        assert np.array_equal(V1.view(np.uint64), V2.view(np.uint64))
+        assert np.array_equal(V1.view(np.uint64), vb.view(np.uint64))
+
+
 def test_random_k_indices_match_oracle_lcg():
     T = pkg_module("core.truncation")
     for n, k, seed in [(10, 3, 1), (1000, 400, 12345), (50, 50 - 1, 7)]:
