@@ -354,6 +354,9 @@ __device__ __forceinline__ void extract1_body(const SelProblem* probs, int nprob
   // this pass (k_lift_ex1)
   if (!BATCH && P.fused && !sel->spec_hit) return;
   for (int b = threadIdx.x; b < HIST_BINS; b += NT) s_hist[b] = 0;
+  // every bin is cleared before any wave counts into it (a wave past its own
+  // bins could otherwise add to a bin another wave has yet to clear)
+  __syncthreads();
   if (!BATCH && sel->spec_hit) {
     // the lift already wrote the per-tile counts, sums and candidates: only
     // the row's digit-2 counts, from the candidates' |a|^2 patterns.  The
