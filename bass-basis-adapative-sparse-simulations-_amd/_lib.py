@@ -82,6 +82,8 @@ SIGNATURES = {
     "bass_counted_read": (_I, [_P, _SZ, _P, _I64, _P, _P]),
     "bass_truncate": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _I64, _I, _P, _I, _P, _SZ, _P]),
     "bass_truncate_noalloc": (_I, [_P, _P, _I64, _I64, _P, _P, _I64, _I, _P, _I, _P, _P]),
+    "bass_lift_truncate": (_I, [_P, _P, _I64, _I, _I, _P, _I, _I64, _P, _P, _I64, _P, _P, _I64, _I, _I, _P, _I, _P, _SZ,
+                                _P]),
     "bass_gate_truncate": (_I, [_P, _P, _I64, _I, _I, _P, _I, _I64, _P, _P, _I64, _I, _I, _P, _I, _P, _SZ, _P]),
     "bass_diag2q": (_I, [_P, _P, _I64, _I, _I, _P, _P]),
     "bass_moments": (_I, [_P, _I64, _P, _SZ, _P]),

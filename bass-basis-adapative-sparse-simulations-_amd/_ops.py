@@ -124,6 +124,36 @@ def truncate_counted(ctx, src, k, dst, hist_tiles, key_map=None, spec=False):
     L.check(rc, "bass_truncate")
 
 
+def lift_truncate(ctx, src, n, j, seg, dst, k, G, nq=1, mode=L.MODE_GATE, renorm=True, counted=False, spec=False,
+                  key_map=None):
+    """A gate or rotation lift of src[:n] into the segments seg, then the
+    top-k select of them into dst (<= k entries) (bass_lift_truncate): bit for
+    bit lift(..., count=False, spec) + truncate_launch(..., segmented=True,
+    spec), or with counted=True (n bounds the count in the state-count slot)
+    lift_counted + truncate_counted.  With spec the lift stores only the
+    outputs that can matter to the predicted threshold (and stores them all
+    again when the select finds it needs more).  dst may be src; seg may
+    share memory with neither.  Nothing waits for the device; a non-counted
+    call's stats come from truncate_finish(ctx, handle)."""
+    tm = SEG // (1 << nq)
+    tiles = (n + tm - 1) // tm
+    ctx.ensure_ws(max(tiles * SEG, n, 1))
+    seg.ensure(max(tiles * SEG, 1))
+    dst.ensure(k)
+    buf, gp = L.mat_arg(G)
+    if key_map is None:
+        km, kp, kb = None, None, 0
+    else:
+        km = np.ascontiguousarray(key_map, dtype=np.int32)
+        kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
+    fl = (L.GT_COUNTED if counted else 0) | (L.GT_SPEC if spec else 0)
+    rc = ctx.lib.bass_lift_truncate(src.xp, src.ap, int(n), int(nq), int(j), gp, int(mode), int(k), seg.xp, seg.ap,
+                                    seg.cap, dst.xp, dst.ap, dst.cap, 1 if renorm else 0, fl, kp, kb, ctx.wsp,
+                                    ctx.ws_bytes, ctx.stream)
+    L.check(rc, "bass_lift_truncate")
+    return None, (km, buf)
+
+
 def gate_truncate(ctx, src, n, j, dst, k, G, nq=1, mode=L.MODE_GATE, renorm=True, counted=False, spec=False,
                   key_map=None):
     """A gate or rotation lift of src[:n] followed by the top-k select of its

@@ -678,7 +678,7 @@ int bass_densify(const uint64_t* x, const double* a, int64_t seg_tiles, uint64_t
 static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
                          int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags, const int32_t* key_map,
                          int key_bits, void* ws, size_t ws_bytes, void* stream, bool timed = true,
-                         const LiftRecompute* fused = nullptr);
+                         const LiftRecompute* fused = nullptr, const LiftRecompute* relift = nullptr);
 
 int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
                   int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags, const int32_t* key_map,
@@ -693,7 +693,7 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
 static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,
                          int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags, const int32_t* key_map,
                          int key_bits, void* ws, size_t ws_bytes, void* stream, bool timed,
-                         const LiftRecompute* fused) {
+                         const LiftRecompute* fused, const LiftRecompute* relift) {
   const bool segmented = (in_flags & 1) != 0;
   const bool wide = (in_flags & 2) != 0;
   const bool counted = (in_flags & 4) != 0;
@@ -728,6 +728,7 @@ static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t 
   P.x = fused ? (const uint64_t*)C.c1k : x;
   P.a = fused ? nullptr : (const double2*)a;
   P.fused = fused ? 1 : 0;
+  P.pruned = relift ? 1 : 0;
   P.seg_cnt = segmented ? at<int>(ws, L.seg_cnt) : nullptr;
   P.n_dense = n;
   P.tile0 = 0;
@@ -759,7 +760,7 @@ static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t 
   SelTiles T = tiles_view(ws, L, C.c1p, C.c1i);
   if (fused) T.c1k = C.c1k;
   const int tk = timed ? ktimer_begin("select", s) : -1;
-  launch_select(P, T, hist_tiles > 0, wide, s, fused);
+  launch_select(P, T, hist_tiles > 0, wide, s, fused, relift);
   ktimer_end(tk, s);
   if (counted)
     hipLaunchKernelGGL(k_oplog_append, dim3(1), dim3(1), 0, s, (const OpStats*)st,
@@ -845,6 +846,96 @@ int bass_gate_truncate(const uint64_t* x, const double* a, int64_t n, int nq, in
   R.G = gm;
   return truncate_impl(nullptr, nullptr, 0, k, xo, ao, cap_out, renorm, tiles, 1 | (counted ? 4 : 0) | (spec ? 8 : 0),
                        key_map, key_bits, ws, ws_bytes, stream, true, &R);
+}
+
+int bass_lift_truncate(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G, int mode,
+                       int64_t k, uint64_t* seg_x, double* seg_a, int64_t seg_cap, uint64_t* xo, double* ao,
+                       int64_t cap_out, int renorm, int flags, const int32_t* key_map, int key_bits, void* ws,
+                       size_t ws_bytes, void* stream) {
+  const bool counted = (flags & 1) != 0;
+  const bool spec = (flags & 2) != 0;
+  if (flags & ~3) return fail(BASS_E_ARG, "bass_lift_truncate: unknown flags 0x%x", flags);
+  if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_lift_truncate: nq must be 1 or 2");
+  if (mode != MODE_GATE && mode != MODE_ROT)
+    return fail(BASS_E_ARG, "bass_lift_truncate: mode must be 0 (gate) or 1 (rotation)");
+  if (j < nq - 1 || j > 62) return fail(BASS_E_ARG, "bass_lift_truncate: bad bit %d for %d qubit(s)", j, nq);
+  if (n < 1 || n > 0x7FFFFFFFll) return fail(BASS_E_ARG, "bass_lift_truncate: bad n %lld", (long long)n);
+  if (k < 1) return fail(BASS_E_ARG, "bass_lift_truncate: need k >= 1");
+  if (cap_out < k) return fail(BASS_E_OVERFLOW, "bass_lift_truncate: cap_out < k");
+  if (!x || !a || !xo || !ao || !G || !seg_x || !seg_a) return fail(BASS_E_ARG, "bass_lift_truncate: null pointer");
+  const int tm = SEG_ENTRIES >> nq;
+  const int64_t tiles = (n + tm - 1) / tm;
+  const int64_t slots = tiles * SEG_ENTRIES;
+  if (seg_cap < slots) return fail(BASS_E_OVERFLOW, "bass_lift_truncate: segments need %lld slots", (long long)slots);
+  {
+    // the segments may share memory with neither the input (a relift reads
+    // it after the lift wrote them) nor the output (the compaction reads
+    // them while it writes); the output may be the input
+    const char *xb = (const char*)x, *ab = (const char*)a, *xob = (const char*)xo, *aob = (const char*)ao;
+    const char *sxb = (const char*)seg_x, *sab = (const char*)seg_a;
+    const size_t ni = (size_t)n, no = (size_t)cap_out, ns = (size_t)slots;
+    auto overlap = [](const char* p, size_t pn, const char* q, size_t qn) { return p < q + qn && q < p + pn; };
+    const char* seg[2] = {sxb, sab};
+    const size_t segn[2] = {8 * ns, 16 * ns};
+    const char* oth[4] = {xb, ab, xob, aob};
+    const size_t othn[4] = {8 * ni, 16 * ni, 8 * no, 16 * no};
+    for (int u = 0; u < 2; ++u)
+      for (int v = 0; v < 4; ++v)
+        if (overlap(seg[u], segn[u], oth[v], othn[v]))
+          return fail(BASS_E_ARG, "bass_lift_truncate: the segment buffers overlap the input or the output");
+    if (overlap(sxb, 8 * ns, sab, 16 * ns)) return fail(BASS_E_ARG, "bass_lift_truncate: segment buffers overlap");
+  }
+  KeyMap km;
+  const int krc = parse_key_map(key_map, key_bits, km, "bass_lift_truncate");
+  if (krc != BASS_OK) return krc;
+  WS_REQUIRE(slots > n ? slots : n);
+  const CandRegion C = cand_region(ws, L, slots);
+  if (C.bytes - al((size_t)8 * slots) > L.total - L.big)
+    return fail(BASS_E_WORKSPACE, "bass_lift_truncate: candidates do not fit");
+  hipStream_t s = (hipStream_t)stream;
+  OpStats* st = at<OpStats>(ws, L.st);
+  GateMat gm;
+  memset(&gm, 0, sizeof(gm));
+  const int M = 1 << nq;
+  for (int e = 0; e < M * M; ++e) gm.m[e] = make_double2(G[2 * e], G[2 * e + 1]);
+  LiftZero zero;
+  zero.p = at<unsigned long long>(ws, L.hist1);
+  zero.wp = HIST_BINS * 4 / 8;
+  zero.q = (unsigned long long*)st;
+  zero.wq = 48 / 8;
+  LiftSpec sp;
+  memset(&sp, 0, sizeof(sp));
+  if (spec) {
+    sp.pred = at<unsigned int>(ws, L.spec);
+    sp.out.c1p = C.c1p;
+    sp.out.c1i = C.c1i;
+    sp.out.cnt1 = at<int>(ws, L.cnt1);
+    sp.out.tile_dsum = at<double>(ws, L.tile_dsum);
+    sp.out.tile_dcnt = at<int>(ws, L.tile_dcnt);
+  }
+  const int64_t* n_dev = counted ? at<int64_t>(ws, L.count_slot) : nullptr;
+  LiftBound* bounds = at<LiftBound>(ws, L.lift_b);
+  static const char* names[2][2] = {{"gate1q", "gate2q"}, {"rot1q", "rot2q"}};
+  const int tk = ktimer_begin(names[mode][nq - 1], s);
+  // with the prediction the lift also prunes (stores only outputs that can
+  // matter to a threshold within PRUNE_MARGIN buckets below it)
+  launch_lift(nq, mode, x, (const double2*)a, n, j, gm, seg_x, (double2*)seg_a, bounds, at<unsigned int>(ws, L.hist1),
+              at<double>(ws, L.tile_psum), at<int>(ws, L.seg_cnt), s, n_dev, zero, spec ? &sp : nullptr, false, spec);
+  ktimer_end(tk, s);
+  const int hrc = hip_check("bass_lift_truncate");
+  if (hrc != BASS_OK) return hrc;
+  LiftRecompute R;
+  R.qb = nq;
+  R.mode = mode;
+  R.x = x;
+  R.a = (const double2*)a;
+  R.n = n;
+  R.n_dev = n_dev;
+  R.j = j;
+  R.bounds = bounds;
+  R.G = gm;
+  return truncate_impl(seg_x, seg_a, 0, k, xo, ao, cap_out, renorm, tiles, 1 | (counted ? 4 : 0) | (spec ? 8 : 0),
+                       key_map, key_bits, ws, ws_bytes, stream, true, nullptr, spec ? &R : nullptr);
 }
 
 int bass_truncate_noalloc(const uint64_t* x, const double* a, int64_t n, int64_t k, uint64_t* xo, double* ao,

@@ -63,51 +63,87 @@ __device__ __forceinline__ void wave_hist_add(unsigned int* s_hist, unsigned int
   if ((am >> lane) & 1ull) atomicAdd(&s_hist[bin], 1u);
 }
 
+// Fixed-order sum of values given by rank (r in [0, total), total the same
+// in every thread): thread t adds ranks t, t + NT, t + 2 NT, ... in
+// increasing order, so the result depends on the ranked sequence alone, not
+// on which thread held which value (a tile whose slots moved -- fewer of them,
+// the same entries in the same order -- sums to the same bits).  The values
+// go through buf (CAP doubles) in passes of CAP ranks; SQ: also the sum of
+// squares into *sq.  Returns the thread's partial sums (for block_sum).
+template <int R, int CAP, bool SQ = false>
+__device__ __forceinline__ double rank_ordered_sum(const double* val, const int* rank, const bool* ok, int total,
+                                                   double* buf, double* sq = nullptr) {
+  double acc = 0.0, acc2 = 0.0;
+  for (int lo = 0; lo < total; lo += CAP) {
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      if (ok[i] && rank[i] >= lo && rank[i] < lo + CAP) buf[rank[i] - lo] = val[i];
+    __syncthreads();
+    const int hi = total < lo + CAP ? total : lo + CAP;
+    for (int r = lo + (int)threadIdx.x; r < hi; r += NT) {
+      const double v = buf[r - lo];
+      acc += v;
+      if constexpr (SQ) acc2 += v * v;
+    }
+    __syncthreads();
+  }
+  if constexpr (SQ) *sq = acc2;
+  return acc;
+}
+
 // One tile from its |a|^2 values vp (slot it * NT + threadIdx.x holds
 // vp[it]; slots >= cnt are ignored): count / sum above bucket b1 into
-// tile_dcnt[gt] / tile_dsum[gt], candidates inside it (in slot order:
-// item-major, then thread order) at c1p / c1i[gt * SEL_TILE + ...] (|a|^2 bit
-// pattern, input slot base + slot), their count into cnt1[gt], and with HIST
-// their digit-2 counts into s_hist.  Candidate flags are per-item wave ballots
-// (scalar registers); a candidate's offset is its item's and wave's base (one
-// wave scans the (item, wave) counts) plus its rank among the wave's set lanes.
+// tile_dcnt[gt] / tile_dsum[gt] (the sum in the entries' order, by
+// rank_ordered_sum), candidates inside it (in slot order: item-major, then
+// thread order) at c1p / c1i[gt * SEL_TILE + ...] (|a|^2 bit pattern, input
+// slot base + slot), their count into cnt1[gt], and with HIST their digit-2
+// counts into s_hist.  Candidate and above flags are per-item wave ballots
+// (scalar registers); an entry's rank is its item's and wave's base (one wave
+// scans the (item, wave) counts) plus its rank among the wave's set lanes.
+// s_wcnt: 2 * SEL_ITEMS * WAVES ints; s_abv: CAPA doubles.
 // KEYS: vk holds the slots' stored words in the same order; candidates get
 // their word in c1k and their own list index in c1i.
-template <bool HIST, bool KEYS = false>
+template <bool HIST, bool KEYS = false, int CAPA = 1024>
 __device__ __forceinline__ void ex1_tile_core(const Ex1Out& O, int64_t gt, int64_t base, int cnt,
                                               const double (&vp)[SEL_ITEMS], unsigned int b1,
-                                              unsigned int* s_hist, int* s_wcnt, double* s_red, int* s_dcnt,
+                                              unsigned int* s_hist, int* s_wcnt, double* s_red, double* s_abv,
                                               const uint64_t* vk = nullptr) {
+  constexpr int IW = SEL_ITEMS * WAVES;
+  static_assert(IW <= 64, "one wave scans the (item, wave) counts");
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (threadIdx.x == 0) *s_dcnt = 0;
-  __syncthreads();
   const int64_t cbase = gt * SEL_TILE;
-  double dsum = 0.0;
-  int dcnt = 0;
-  unsigned long long m[SEL_ITEMS];
+  unsigned long long m[SEL_ITEMS], ma[SEL_ITEMS];
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
     const int j = it * NT + threadIdx.x;
-    bool cand = false;
+    bool cand = false, above = false;
     if (j < cnt) {
       const unsigned long long bits = (unsigned long long)__double_as_longlong(vp[it]);
       const unsigned int d = (unsigned int)(bits >> 52);
       if (d > b1) {
-        dsum += vp[it];
-        ++dcnt;
+        above = true;
       } else if (d == b1) {
         cand = true;
         if constexpr (HIST) atomicAdd(&s_hist[(unsigned int)(bits >> 41) & (HIST_BINS - 1)], 1u);
       }
     }
     m[it] = __ballot(cand);
-    if (lane == 0) s_wcnt[it * WAVES + w] = __popcll(m[it]);
+    ma[it] = __ballot(above);
+    if (lane == 0) {
+      s_wcnt[it * WAVES + w] = __popcll(m[it]);
+      s_wcnt[IW + it * WAVES + w] = __popcll(ma[it]);
+    }
   }
   __syncthreads();
-  const int cw = lane < SEL_ITEMS * WAVES ? s_wcnt[lane] : 0;
+  const int cw = lane < IW ? s_wcnt[lane] : 0;
   const int cx = wave_incl_scan(cw) - cw;
-  const int rb = __builtin_amdgcn_readlane(cx + cw, SEL_ITEMS * WAVES - 1);
+  const int rb = __builtin_amdgcn_readlane(cx + cw, IW - 1);
+  const int aw = lane < IW ? s_wcnt[IW + lane] : 0;
+  const int ax = wave_incl_scan(aw) - aw;
+  const int atot = __builtin_amdgcn_readlane(ax + aw, IW - 1);
   const int wu = __builtin_amdgcn_readfirstlane(w);
+  int ar[SEL_ITEMS];
+  bool aok[SEL_ITEMS];
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
     const int o = __builtin_amdgcn_readlane(cx, it * WAVES + wu);
@@ -123,16 +159,16 @@ __device__ __forceinline__ void ex1_tile_core(const Ex1Out& O, int64_t gt, int64
         O.c1i[cbase + off] = (unsigned int)(base + it * NT + threadIdx.x);
       }
     }
+    const unsigned long long mka = ma[it];
+    aok[it] = (mka >> lane) & 1ull;
+    ar[it] = __builtin_amdgcn_readlane(ax, it * WAVES + wu) +
+             (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mka >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mka, 0u));
   }
+  const double dsum = rank_ordered_sum<SEL_ITEMS, CAPA>(vp, ar, aok, atot, s_abv);
   const double s = block_sum(dsum, s_red);
-  // count above the bucket: wave sums, one LDS add per wave
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) dcnt += __shfl_xor(dcnt, o, 64);
-  if (lane == 0) atomicAdd(s_dcnt, dcnt);
-  __syncthreads();
   if (threadIdx.x == 0) {
     O.tile_dsum[gt] = s;
-    O.tile_dcnt[gt] = *s_dcnt;
+    O.tile_dcnt[gt] = atot;
     O.cnt1[gt] = rb;
   }
 }

@@ -45,7 +45,16 @@ struct SelState {
   // the speculative lift's digit-1 pass used this select's bucket: k_extract1
   // only builds the digit-2 rows from its candidates
   unsigned int spec_hit;
+  // a pruned lift dropped outputs this select needs (its bucket fell more than
+  // PRUNE_MARGIN below the prediction, or it keeps everything): k_lift_relift
+  // stores the whole output again before the select reads it
+  unsigned int relift;
 };
+
+// A pruned lift (bass_lift_truncate with the speculative pass) stores only
+// the outputs whose |a|^2 exponent bucket is at least the predicted threshold
+// bucket minus PRUNE_MARGIN; its histogram and sums still cover every output.
+constexpr unsigned int PRUNE_MARGIN = 1;
 
 // One block resolves at most FIN_MAX digit-2 candidates.  Beyond that a single
 // problem's select runs the staged multi-block stages of k_finish_staged in
@@ -189,7 +198,7 @@ struct LiftSpec {
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
                 hipStream_t s, const int64_t* n_dev = nullptr, const LiftZero& zero = LiftZero(),
-                const LiftSpec* spec = nullptr, bool fuse = false);
+                const LiftSpec* spec = nullptr, bool fuse = false, bool prune = false);
 
 // One state of a batched elementwise operation (see launch_ew_batch):
 // op 0 diagonal 2-qubit gate a *= d[(b_j1 << 1) | b_j2], op 1 a /= div,
@@ -259,6 +268,9 @@ struct SelProblem {
   // x is the candidate key list (SelTiles::c1k), a is unused; the digit-1 pass
   // after a missed speculation and the compaction recompute the lift's tiles
   int fused;
+  // the input comes from a pruned lift issued just before, whose prediction
+  // *spec_word still holds: k_find1 checks that nothing it dropped is needed
+  int pruned;
   unsigned int* hist1;   // HIST_BINS, zeroed by the caller unless filled by the producing gate
   unsigned int* hist2;   // HIST_BINS, zeroed
   unsigned int* hist_w;  // HIST_BINS, zeroed (wide path)
@@ -328,12 +340,14 @@ struct LiftRecompute {
   GateMat G;
 };
 // Fused select kernels over the lift's tiles: which = 0 the digit-1 pass when
-// the speculation missed (ext tiles per digit-2 row), 1 the compaction.
+// the speculation missed (ext tiles per digit-2 row), 1 the compaction; 2 the
+// relift of a pruned lift's select (the whole output into the select's input
+// tiles P.x / P.a / P.seg_cnt, when k_find1 asked for it).
 int launch_lift_recompute(int which, const LiftRecompute& R, const SelProblem& P, const SelTiles& T, int ext,
                           hipStream_t s);
 int extract1_tiles(int64_t ntiles);
 int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s,
-                  const LiftRecompute* fused = nullptr);
+                  const LiftRecompute* fused = nullptr, const LiftRecompute* relift = nullptr);
 // A batch of problems with the geometry in hdr (prep kernel); tiles / jobs:
 // bounds of the tile and problem counts (size the persistent grids).  Every
 // threshold is resolved in the sequence (one block per problem).

@@ -547,11 +547,16 @@ struct LiftTileOut {
 
 // Stages, merges and gates local tile `tile` of the input (merged positions
 // [tile * TM, ...)) into Q.  Ends with a barrier; L.t and L.a are free after it.
-template <int QB, int MODE>
+// PRUNE: every output's |a|^2 is added to psum (in the stored lift's order)
+// and to the digit-1 histogram (LDS bins below LBINS, else hist), and only
+// outputs in bucket b1p - PRUNE_MARGIN or above (all when b1p is 0xFFFFFFFF)
+// enter Q's mask and offsets.
+template <int QB, int MODE, bool PRUNE = false>
 __device__ __forceinline__ void lift_tile_compute(const uint64_t* __restrict__ w, const double2* __restrict__ a,
                                                   int64_t n, int j, const LiftBound& B0, const LiftBound& B1,
                                                   const GateMat& G, int64_t tile, LiftTileLds<QB>& L,
-                                                  LiftTileOut<QB>& Q) {
+                                                  LiftTileOut<QB>& Q, unsigned int b1p = 0, double* psum = nullptr,
+                                                  unsigned int* s_hist = nullptr, unsigned int* hist = nullptr) {
   constexpr int M = 1 << QB;
   constexpr int T = QB - 1;
   constexpr int TM = SEG_ENTRIES / M;
@@ -589,7 +594,20 @@ __device__ __forceinline__ void lift_tile_compute(const uint64_t* __restrict__ w
         Q.key[it][o] = base | (uint64_t)o;
         Q.re[it][o] = acc.x;
         Q.im[it][o] = acc.y;
-        if (any) mask |= 1u << o;
+        if constexpr (PRUNE) {
+          if (any) {
+            const double p = cabs2(acc);
+            *psum += p;
+            const unsigned int bin = (unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1);
+            if (bin < HIST_BINS / 2)
+              atomicAdd(&s_hist[bin], 1u);
+            else
+              atomicAdd(&hist[bin], 1u);
+            if (b1p == 0xFFFFFFFFu || bin + PRUNE_MARGIN >= b1p) mask |= 1u << o;
+          }
+        } else {
+          if (any) mask |= 1u << o;
+        }
       }
     }
     Q.mask[it] = mask;
@@ -645,8 +663,11 @@ __device__ __forceinline__ void lift_to_slots(const LiftTileOut<QB>& Q,
 // digit-1 pass on the tile (ex1_tile_core, bucket *S.pred) from its |a|^2
 // values staged in LDS in slot order.  FUSE: the outputs are not stored (a
 // fused select recomputes them); the speculative pass keeps the candidates'
-// keys (S.out.c1k).
-template <int QB, int MODE, bool SPEC = false, bool FUSE = false>
+// keys (S.out.c1k).  PRUNE 1: the histogram and |a|^2 sum over every output,
+// but only outputs in bucket *S.pred - PRUNE_MARGIN or above take a slot (in
+// the same order); PRUNE 2 (a relift): every output stored, no histogram or
+// sum (the pruned lift made them).
+template <int QB, int MODE, bool SPEC = false, bool FUSE = false, int PRUNE = 0>
 __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, const double2* __restrict__ a,
                                                int64_t n, int j, const LiftBound& B0, const LiftBound& B1,
                                                const GateMat& G, int64_t tile, int64_t otile,
@@ -664,10 +685,13 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
 
   for (int b = threadIdx.x; b < LBINS; b += NT) s_hist[b] = 0;
   LiftTileOut<QB> Q;
-  lift_tile_compute<QB, MODE>(w, a, n, j, B0, B1, G, tile, L, Q);
+  double psum = 0.0;
+  if constexpr (PRUNE == 1)
+    lift_tile_compute<QB, MODE, true>(w, a, n, j, B0, B1, G, tile, L, Q, *S.pred, &psum, s_hist, hist);
+  else
+    lift_tile_compute<QB, MODE>(w, a, n, j, B0, B1, G, tile, L, Q);
   if (threadIdx.x == 0) seg_cnt[otile] = Q.total;
 
-  double psum = 0.0;
   // SPEC: |a|^2 of the outputs, staged in L.a (free: every gather ended
   // before the offset scan's barrier) in slot order, CAP slots per pass; one
   // pass (1 qubit) stages them here, two (2 qubits) keep them in pv
@@ -690,25 +714,27 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
         const double p = cabs2(v);
         if constexpr (SPEC && ONE_PASS) s_p[pos - rb] = p;
         if constexpr (SPEC && !ONE_PASS) pv[it][o] = p;
-        psum += p;
-        const unsigned int bin = (unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1);
-        if (bin < LBINS)
-          atomicAdd(&s_hist[bin], 1u);
-        else
-          atomicAdd(&hist[bin], 1u);
+        if constexpr (PRUNE == 0) {
+          psum += p;
+          const unsigned int bin = (unsigned int)(__double_as_longlong(p) >> 52) & (HIST_BINS - 1);
+          if (bin < LBINS)
+            atomicAdd(&s_hist[bin], 1u);
+          else
+            atomicAdd(&hist[bin], 1u);
+        }
         ++pos;
       }
     }
   }
   const double s = block_sum(psum, L.red);
-  if (threadIdx.x == 0) tile_psum[otile] = s;
+  if (threadIdx.x == 0 && PRUNE != 2) tile_psum[otile] = s;
   __syncthreads();
-  for (int b = threadIdx.x; b < LBINS; b += NT)
-    if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
+  if constexpr (PRUNE != 2)
+    for (int b = threadIdx.x; b < LBINS; b += NT)
+      if (s_hist[b]) atomicAdd(&hist[b], s_hist[b]);
   if constexpr (SPEC) {
     // the staged |a|^2 read back in the select's slot-to-thread order
-    __shared__ int s_wcnt[SEL_ITEMS * WAVES];
-    __shared__ int s_dcnt;
+    __shared__ int s_wcnt[2 * SEL_ITEMS * WAVES];
     const unsigned int b1 = *S.pred;
     double vp[SEL_ITEMS];
     if constexpr (ONE_PASS) {
@@ -725,15 +751,15 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
     if constexpr (FUSE) {
       uint64_t vk[SEL_ITEMS];
       lift_to_slots<QB>(Q, Q.key, reinterpret_cast<uint64_t*>(L.a), vk);
-      ex1_tile_core<false, true>(S.out, otile, rb, Q.total, vp, b1, nullptr, s_wcnt, L.red, &s_dcnt, vk);
+      ex1_tile_core<false, true, CAP>(S.out, otile, rb, Q.total, vp, b1, nullptr, s_wcnt, L.red, s_p, vk);
     } else {
-      ex1_tile_core<false>(S.out, otile, rb, Q.total, vp, b1, nullptr, s_wcnt, L.red, &s_dcnt);
+      ex1_tile_core<false, false, CAP>(S.out, otile, rb, Q.total, vp, b1, nullptr, s_wcnt, L.red, s_p);
     }
   }
 }
 
 // (the speculative variants are held to the plain kernel's 5 waves per SIMD)
-template <int QB, int MODE, bool SPEC, bool FUSE>
+template <int QB, int MODE, bool SPEC, bool FUSE, int PRUNE = 0>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPEC ? 5 : 1))) void k_lift_gate(
     const uint64_t* __restrict__ w, const double2* __restrict__ a, int64_t n, int j,
     const LiftBound* __restrict__ bounds, GateMat G, uint64_t* __restrict__ xo, double2* __restrict__ ao,
@@ -742,8 +768,30 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SPEC ? 5 : 1
   // count and sum
   n = dev_count(n_dev, n);
   const int64_t tile = blockIdx.x;
-  lift_gate_tile<QB, MODE, SPEC, FUSE>(w, a, n, j, bounds[tile], bounds[tile + 1], G, tile, tile, xo, ao, hist,
-                                       tile_psum, seg_cnt, S);
+  lift_gate_tile<QB, MODE, SPEC, FUSE, PRUNE>(w, a, n, j, bounds[tile], bounds[tile + 1], G, tile, tile, xo, ao,
+                                              hist, tile_psum, seg_cnt, S);
+}
+
+// The relift of a pruned lift whose select needs outputs it dropped (flag set
+// by k_find1; otherwise every block returns at once): persistent blocks
+// recompute every tile and store all its outputs into the select's input
+// tiles, with the speculative digit-1 pass against the select's own bucket,
+// so the select goes on as after a held prediction.
+template <int QB, int MODE>
+__global__ __launch_bounds__(NT) void k_lift_relift(LiftRecompute R, SelProblem P, SelTiles T) {
+  if (!P.sel->relift) return;
+  const int64_t n = dev_count(R.n_dev, R.n);
+  LiftSpec S;
+  S.pred = &P.sel->b1;
+  S.out = Ex1Out{T.c1p, T.c1i, T.cnt1, T.tile_dsum, T.tile_dcnt, nullptr};
+  uint64_t* xo = const_cast<uint64_t*>(P.x);
+  double2* ao = const_cast<double2*>(P.a);
+  int* seg = const_cast<int*>(P.seg_cnt);
+  for (int64_t tile = blockIdx.x; tile < P.ntiles; tile += gridDim.x) {
+    lift_gate_tile<QB, MODE, true, false, 2>(R.x, R.a, n, R.j, R.bounds[tile], R.bounds[tile + 1], R.G, tile, tile,
+                                             xo, ao, nullptr, nullptr, seg, S);
+    __syncthreads();
+  }
 }
 
 // ------------------------------------------------- fused gate select ------
@@ -768,8 +816,7 @@ __global__ __launch_bounds__(NT) void k_lift_ex1(LiftRecompute R, SelProblem P, 
   if (sel->keepall || sel->spec_hit) return;
   __shared__ LiftTileLds<QB> L;
   __shared__ unsigned int s_hist[HIST_BINS];
-  __shared__ int s_wcnt[SEL_ITEMS * WAVES];
-  __shared__ int s_dcnt;
+  __shared__ int s_wcnt[2 * SEL_ITEMS * WAVES];
   const int64_t n = dev_count(R.n_dev, R.n);
   const unsigned int b1 = sel->b1;
   const Ex1Out O{T.c1p, T.c1i, T.cnt1, T.tile_dsum, T.tile_dcnt, T.c1k};
@@ -790,7 +837,8 @@ __global__ __launch_bounds__(NT) void k_lift_ex1(LiftRecompute R, SelProblem P, 
     uint64_t vk[SEL_ITEMS];
     lift_to_slots<QB>(Q, pv, reinterpret_cast<double*>(L.a), vp);
     lift_to_slots<QB>(Q, Q.key, reinterpret_cast<uint64_t*>(L.a), vk);
-    ex1_tile_core<true, true>(O, gt, gt * SEG_ENTRIES, Q.total, vp, b1, s_hist, s_wcnt, L.red, &s_dcnt, vk);
+    ex1_tile_core<true, true, 2 * (SEG_ENTRIES >> QB)>(O, gt, gt * SEG_ENTRIES, Q.total, vp, b1, s_hist, s_wcnt,
+                                                        L.red, reinterpret_cast<double*>(L.a), vk);
   }
   __syncthreads();
   unsigned short* row = T.hist2_rows + vb * HIST_BINS;
@@ -832,36 +880,38 @@ __global__ __launch_bounds__(NT) void k_lift_compact(LiftRecompute R, SelProblem
   int ktot;
   block_excl_scan_items<ITEMS>(kc, ko, ktot, L.scan);
   const int64_t rb = T.koff[gt] + (P.chunked ? T.ch_off[(gt - P.tile0) / SEL_CHUNK] : 0);
-  double p2[ITEMS][M];
+  // the written |a|^2 by kept rank (k_compact's sums, from the same entries)
+  double p2[ITEMS * M];
+  int kr[ITEMS * M];
+  bool kok[ITEMS * M];
   bool fits = true;
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
-    int64_t pos = rb + ko[it];
+    int r = ko[it];
 #pragma unroll
     for (int o = 0; o < M; ++o) {
-      p2[it][o] = 0.0;
-      if ((km[it] >> o) & 1u) {
+      const int e = it * M + o;
+      p2[e] = 0.0;
+      kr[e] = r;
+      kok[e] = (km[it] >> o) & 1u;
+      if (kok[e]) {
         const double2 v = make_double2(Q.re[it][o] * inv, Q.im[it][o] * inv);
+        const int64_t pos = rb + r;
         if (pos < P.cap_out) {
           P.xo[pos] = Q.key[it][o];
           P.ao[pos] = v;
         } else {
           fits = false;
         }
-        p2[it][o] = cabs2(v);
-        ++pos;
+        p2[e] = cabs2(v);
+        ++r;
       }
     }
   }
   if (!fits) set_err(&P.st->err, ERR_OVERFLOW);
-  double vp[SEL_ITEMS];
-  lift_to_slots<QB>(Q, p2, reinterpret_cast<double*>(L.a), vp);
-  double s2 = 0.0, s4 = 0.0;
-#pragma unroll
-  for (int it = 0; it < SEL_ITEMS; ++it) {
-    s2 += vp[it];
-    s4 += vp[it] * vp[it];
-  }
+  double s4;
+  const double s2 = rank_ordered_sum<ITEMS * M, 2 * (SEG_ENTRIES >> QB), true>(p2, kr, kok, ktot,
+                                                                            reinterpret_cast<double*>(L.a), &s4);
   const double t2 = block_sum(s2, L.red);
   const double t4 = block_sum(s4, L.red);
   if (threadIdx.x == 0) {
@@ -1482,14 +1532,17 @@ int launch_lift_batch(int qmask, int mode, const LiftJob* d_jobs, const BatchHdr
 
 // One gate kernel instance (QB, MODE) with the SPEC / FUSE variant chosen at run time.
 template <int QB, int MODE>
-static void launch_gate_kernel(bool spec, bool fuse, dim3 grid, hipStream_t s, const uint64_t* x, const double2* a,
-                               int64_t n, int j, const LiftBound* bounds, const GateMat& G, uint64_t* xo,
-                               double2* ao, unsigned int* hist, double* tile_psum, int* seg_cnt,
+static void launch_gate_kernel(bool spec, bool fuse, bool prune, dim3 grid, hipStream_t s, const uint64_t* x,
+                               const double2* a, int64_t n, int j, const LiftBound* bounds, const GateMat& G,
+                               uint64_t* xo, double2* ao, unsigned int* hist, double* tile_psum, int* seg_cnt,
                                const int64_t* n_dev, const LiftSpec& S) {
 #define BASS_LG(SPEC_, FUSE_)                                                                               \
   hipLaunchKernelGGL((k_lift_gate<QB, MODE, SPEC_, FUSE_>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao, \
                      hist, tile_psum, seg_cnt, n_dev, S)
-  if (spec && fuse)
+  if (prune && spec && !fuse)
+    hipLaunchKernelGGL((k_lift_gate<QB, MODE, true, false, 1>), grid, dim3(NT), 0, s, x, a, n, j, bounds, G, xo, ao,
+                       hist, tile_psum, seg_cnt, n_dev, S);
+  else if (spec && fuse)
     BASS_LG(true, true);
   else if (spec)
     BASS_LG(true, false);
@@ -1502,7 +1555,8 @@ static void launch_gate_kernel(bool spec, bool fuse, dim3 grid, hipStream_t s, c
 
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
                 uint64_t* xo, double2* ao, LiftBound* bounds, unsigned int* hist, double* tile_psum, int* seg_cnt,
-                hipStream_t s, const int64_t* n_dev, const LiftZero& zero, const LiftSpec* spec, bool fuse) {
+                hipStream_t s, const int64_t* n_dev, const LiftZero& zero, const LiftSpec* spec, bool fuse,
+                bool prune) {
   if (n <= 0) return 0;
   const int tm = lift_tile_entries(qb, mode);
   const int t = (mode == MODE_MOVE) ? 0 : qb - 1;
@@ -1517,16 +1571,16 @@ int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n
     const LiftSpec S = spec ? *spec : LiftSpec();
     const bool sp = spec != nullptr;
     if (qb == 1 && mode == MODE_GATE)
-      launch_gate_kernel<1, MODE_GATE>(sp, fuse, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
+      launch_gate_kernel<1, MODE_GATE>(sp, fuse, prune, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
                                        n_dev, S);
     else if (qb == 1)
-      launch_gate_kernel<1, MODE_ROT>(sp, fuse, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
+      launch_gate_kernel<1, MODE_ROT>(sp, fuse, prune, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
                                       n_dev, S);
     else if (mode == MODE_GATE)
-      launch_gate_kernel<2, MODE_GATE>(sp, fuse, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
+      launch_gate_kernel<2, MODE_GATE>(sp, fuse, prune, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
                                        n_dev, S);
     else
-      launch_gate_kernel<2, MODE_ROT>(sp, fuse, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
+      launch_gate_kernel<2, MODE_ROT>(sp, fuse, prune, grid, s, x, a, n, j, bounds, G, xo, ao, hist, tile_psum, seg_cnt,
                                       n_dev, S);
   }
   return (int)hipGetLastError();
@@ -1541,8 +1595,12 @@ int launch_lift_recompute(int which, const LiftRecompute& R, const SelProblem& P
     if (which == 0)                                                                                            \
       hipLaunchKernelGGL((k_lift_ex1<QB_, MODE_>), dim3((unsigned)((tiles + ext - 1) / ext)), dim3(NT), 0, s, R, \
                          P, T, ext);                                                                           \
-    else                                                                                                       \
+    else if (which == 1)                                                                                       \
       hipLaunchKernelGGL((k_lift_compact<QB_, MODE_>), dim3((unsigned)tiles), dim3(NT), 0, s, R, P, T);        \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_lift_relift<QB_, MODE_>),                                                          \
+                         dim3(persist_grid(reinterpret_cast<const void*>(&k_lift_relift<QB_, MODE_>), NT, tiles)), \
+                         dim3(NT), 0, s, R, P, T);                                                             \
   } while (0)
   if (R.qb == 1 && R.mode == MODE_GATE)
     BASS_LR(1, MODE_GATE);

@@ -134,6 +134,8 @@ __device__ double block_sum_array(const double* v, int64_t m, double* lds) {
   return block_sum<NTHR>(s, lds);
 }
 
+constexpr int CMP_BUF = 1024;        // k_compact's staging of the written |a|^2 (doubles; two passes a tile)
+constexpr int EX1_ABV = 1024;        // k_extract1's staging of above-bucket |a|^2 (doubles; two passes a tile)
 constexpr int EX1_TILES = 4;         // tiles per k_extract1 block of a single problem
 constexpr int EX1_TILES_MIN = 12;    // range for large problems (row counts <= 31 * SEL_TILE fit 16 bits)
 constexpr int EX1_TILES_MAX = 31;
@@ -235,12 +237,22 @@ __device__ __forceinline__ void find1_body(const SelProblem& P, const SelTiles& 
     s_b = 0xFFFFFFFFu;
     s_above = 0;
   }
+  // the bucket a speculative (and pruning) lift predicted, read before this
+  // kernel replaces it
+  const unsigned int pred = P.spec_word ? *P.spec_word : 0xFFFFFFFFu;
   __syncthreads();
-  // input count = sum of the digit-1 histogram
-  int cnt = 0;
-  for (int b = threadIdx.x; b < HIST_BINS; b += NTB) cnt += (int)P.hist1[b];
+  // input count = sum of the digit-1 histogram; a pruned lift's dropped
+  // outputs are those below bucket pred - PRUNE_MARGIN
+  int cnt = 0, drop = 0;
+  for (int b = threadIdx.x; b < HIST_BINS; b += NTB) {
+    const int h = (int)P.hist1[b];
+    cnt += h;
+    drop += (unsigned int)b + PRUNE_MARGIN < pred ? h : 0;
+  }
   int count;
   (void)block_excl_scan<NTB>(cnt, count, s_scan);
+  int dropped = 0;
+  if (P.pruned && pred != 0xFFFFFFFFu) (void)block_excl_scan<NTB>(drop, dropped, s_scan);
   const bool keepall = (unsigned long long)count <= P.k;
   if (!keepall) find_from_top<NTB>(P.hist1, P.k, &s_b, &s_above, s_scan);
   // a chunked problem sums its input |a|^2 by chunk (k_sel_chunks, k_sel_final)
@@ -267,9 +279,18 @@ __device__ __forceinline__ void find1_body(const SelProblem& P, const SelTiles& 
       // a speculative lift ran the digit-1 pass against *spec_word: usable
       // when that was this bucket (stats flags bit 2); the bucket is the next
       // lift's prediction
-      if (P.spec_in && !keepall && *P.spec_word == s_b) {
+      if (P.spec_in && !keepall && pred == s_b) {
         sel->spec_hit = 1;
         st->flags |= 4;
+      }
+      // a pruned lift dropped outputs this select needs (it keeps everything,
+      // or its bucket lies more than PRUNE_MARGIN below the prediction): the
+      // relift stores them all, with the digit-1 pass of this bucket (flags
+      // bit 3)
+      if (P.pruned && dropped > 0 && (keepall || s_b + PRUNE_MARGIN < pred)) {
+        sel->relift = 1;
+        if (!keepall) sel->spec_hit = 1;
+        st->flags |= 8;
       }
       *P.spec_word = keepall ? 0xFFFFFFFFu : s_b;
     }
@@ -317,9 +338,9 @@ __device__ __forceinline__ void ex1_load(const SelProblem& P, int64_t gt, double
 // digit-2 counts).
 __device__ __forceinline__ void ex1_tile(const SelProblem& P, const SelTiles& T, int64_t gt, int cnt,
                                          const double (&vp)[SEL_ITEMS], unsigned int b1, unsigned int* s_hist,
-                                         int* s_wcnt, double* s_red, int* s_dcnt) {
+                                         int* s_wcnt, double* s_red, double* s_abv) {
   const Ex1Out O{T.c1p, T.c1i, T.cnt1, T.tile_dsum, T.tile_dcnt};
-  ex1_tile_core<true>(O, gt, tile_base(P, gt), cnt, vp, b1, s_hist, s_wcnt, s_red, s_dcnt);
+  ex1_tile_core<true, false, EX1_ABV>(O, gt, tile_base(P, gt), cnt, vp, b1, s_hist, s_wcnt, s_red, s_abv);
 }
 
 // Per tile (EXT consecutive tiles per block, all of one problem; EXT = 0 takes
@@ -331,9 +352,9 @@ template <bool BATCH, int EXT>
 __device__ __forceinline__ void extract1_body(const SelProblem* probs, int nprob, const SelProblem& one,
                                               const SelTiles& T, int64_t tiles_end, int ext_rt, int64_t vb) {
   __shared__ unsigned int s_hist[HIST_BINS];
-  __shared__ int s_wcnt[SEL_ITEMS * WAVES];
+  __shared__ int s_wcnt[2 * SEL_ITEMS * WAVES];
   __shared__ double s_red[WAVES];
-  __shared__ int s_dcnt;
+  __shared__ double s_abv[EX1_ABV];
   const int ext = EXT > 0 ? EXT : ext_rt;
   const int64_t gt0 = vb * ext;
   const SelProblem& P = tile_problem<BATCH>(probs, nprob, one, gt0);
@@ -415,7 +436,7 @@ __device__ __forceinline__ void extract1_body(const SelProblem* probs, int nprob
     for (int it = 0; it < SEL_ITEMS; ++it) vp[it] = cabs2(make_double2(nr[it], ni[it]));
     const int c = cnt;
     if (sub + 1 < ext && gt + 1 < tiles_end) ex1_load(P, gt + 1, nr, ni, cnt);
-    ex1_tile(P, T, gt, c, vp, b1, s_hist, s_wcnt, s_red, &s_dcnt);
+    ex1_tile(P, T, gt, c, vp, b1, s_hist, s_wcnt, s_red, s_abv);
   }
   __syncthreads();
   // counts <= EXT * SEL_TILE fit 16 bits; k_hist2_reduce sums the rows (about
@@ -1479,29 +1500,35 @@ __device__ __forceinline__ void compact_body(const SelProblem* probs, int nprob,
   __syncthreads();
   const int cw = lane < SEL_ITEMS * WAVES ? s_scan[lane] : 0;
   const int cx = wave_incl_scan(cw) - cw;
+  const int ktot = __builtin_amdgcn_readlane(cx + cw, SEL_ITEMS * WAVES - 1);
   const int wu = __builtin_amdgcn_readfirstlane(w);
-  double s2 = 0.0, s4 = 0.0;
+  // |a|^2 of the written entries by kept rank: the tile's sums in the order of
+  // its kept entries, whichever slots held them
+  double pv[SEL_ITEMS];
+  int kr[SEL_ITEMS];
   bool fits = true;
 #pragma unroll
   for (int it = 0; it < SEL_ITEMS; ++it) {
-    const int o = __builtin_amdgcn_readlane(cx, it * WAVES + wu);
+    const unsigned long long mk = m[it];
+    kr[it] = __builtin_amdgcn_readlane(cx, it * WAVES + wu) +
+             (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mk, 0u));
+    pv[it] = 0.0;
     if (keep[it]) {
       const double2 v = make_double2(vv[it].x * inv, vv[it].y * inv);
-      const unsigned long long mk = m[it];
-      const int64_t pos =
-          rb + o + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mk, 0u));
+      const int64_t pos = rb + kr[it];
       if (pos < P.cap_out) {
         P.xo[pos] = kk[it];
         P.ao[pos] = v;
       } else {
         fits = false;
       }
-      const double p = cabs2(v);
-      s2 += p;
-      s4 += p * p;
+      pv[it] = cabs2(v);
     }
   }
   if (!fits) set_err(&P.st->err, ERR_OVERFLOW);
+  __shared__ double s_kbuf[CMP_BUF];
+  double s4;
+  const double s2 = rank_ordered_sum<SEL_ITEMS, CMP_BUF, true>(pv, kr, keep, ktot, s_kbuf, &s4);
   const double t2 = block_sum(s2, s_red);
   const double t4 = block_sum(s4, s_red);
   if (threadIdx.x == 0) {
@@ -1729,13 +1756,14 @@ __global__ __launch_bounds__(NT) void k_gather(const uint64_t* __restrict__ x, c
 // The select of one problem (tiles_end = its tiles; extract1 groups several
 // tiles per block).
 static void select_one(const SelProblem& one, const SelTiles& T, int64_t tiles_end, bool hist_ready, bool wide,
-                       hipStream_t s, const LiftRecompute* fused) {
+                       hipStream_t s, const LiftRecompute* fused, const LiftRecompute* relift) {
   const SelProblem* probs = nullptr;
   const BatchHdr* hdr = nullptr;
   const int nprob = 1;
   if (!hist_ready)
     hipLaunchKernelGGL((k_hist_psum<false>), dim3((unsigned)tiles_end), dim3(NT), 0, s, probs, nprob, one, T);
   hipLaunchKernelGGL((k_find1<false>), dim3(1), dim3(NTB), 0, s, probs, one, T, hdr);
+  if (relift) launch_lift_recompute(2, *relift, one, T, 0, s);
   const int ext = extract1_tiles(tiles_end);
   const int64_t rows = (tiles_end + ext - 1) / ext;
   if (ext == EX1_TILES)
@@ -1832,8 +1860,8 @@ int select_rows(int64_t ntiles, bool batch) {
 }
 
 int launch_select(const SelProblem& P, const SelTiles& T, bool hist_ready, bool wide, hipStream_t s,
-                  const LiftRecompute* fused) {
-  select_one(P, T, P.ntiles, hist_ready, wide, s, fused);
+                  const LiftRecompute* fused, const LiftRecompute* relift) {
+  select_one(P, T, P.ntiles, hist_ready, wide, s, fused, relift);
   return (int)hipGetLastError();
 }
 

@@ -328,6 +328,12 @@ class BASS:
         # the recomputation costs what the traffic saved, and more when the
         # speculative bucket misses, so the stored pair stays the default
         self.fused = False
+        # prune=True (with speculate): gate and trial lifts store only the
+        # outputs that can matter to the predicted threshold -- the bucket of
+        # the last select and the one below it, or higher -- and the select
+        # has them all stored again when it needs more (bass_lift_truncate;
+        # the same results as the stored pair)
+        self.prune = True
         # counted diagonal 2-qubit gates not yet launched: (bit1, bit2, diag);
         # a run of them goes to the device as one pass (bass_diag2q_run) when
         # the next other operation or a read of the state comes
@@ -398,6 +404,8 @@ class BASS:
                 if r.flags & 4:  # the lift's digit-1 pass was used
                     self._count("spec_hit")
                     self._count("spec_hit_" + op)
+                if r.flags & 8:  # a pruned lift's outputs stored again
+                    self._count("relift")
             else:
                 self._count("bytes_select", 48 * n)
             n_prev = int(r.nnz)
@@ -512,6 +520,8 @@ class BASS:
                 # truncate to k, or (<= k entries) keep all; one stats read
                 st = _ops.truncate_finish(ctx, handle)
                 m = st.n_in
+                if st.flags & 8:
+                    self._count("relift")
                 if st.flags & 1:
                     self._count("truncate_rot1q")
                     if st.flags & 4:
@@ -583,6 +593,10 @@ class BASS:
         if self.fused:
             handle = _ops.gate_truncate(ctx, state._buf, state.nnz, jb, cand, self.k, Vh, nq=1, mode=L.MODE_ROT,
                                         renorm=True, spec=self.speculate, key_map=km)
+            return handle, trial_pos, state.nnz
+        if self.prune and self.speculate:
+            handle = _ops.lift_truncate(ctx, state._buf, state.nnz, jb, state._rot, cand, self.k, Vh, nq=1,
+                                        mode=L.MODE_ROT, renorm=True, spec=True, key_map=km)
             return handle, trial_pos, state.nnz
         _, rht = _ops.lift(ctx, state._buf, state.nnz, jb, state._rot, G=Vh, nq=1, mode=L.MODE_ROT, count=False,
                            spec=self.speculate)
@@ -779,6 +793,9 @@ class BASS:
             _ops.gate_truncate(ctx, state._buf, self.k, jb, state._tmp, self.k, G, nq=nq, counted=True,
                                spec=self.speculate, key_map=km)
             state._buf, state._tmp = state._tmp, state._buf
+        elif self.prune and self.speculate:
+            _ops.lift_truncate(ctx, state._buf, self.k, jb, state._tmp, state._buf, self.k, G, nq=nq, counted=True,
+                               spec=True, key_map=km)
         else:
             ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=nq, spec=self.speculate)
             _ops.truncate_counted(ctx, state._tmp, self.k, state._buf, ht, key_map=km, spec=self.speculate)

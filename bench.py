@@ -485,7 +485,7 @@ def cpu_model():
     return None
 
 
-def run_plain(workload, N, K, warmup, steps, fused=False):
+def run_plain(workload, N, K, warmup, steps, fused=False, prune=True):
     """The plain driver on the full configured circuit (brickwork: 30 layers;
     kicked Ising: 60 steps): `warmup` steps untimed, then `steps` steps timed
     between two device synchronisations, with HIP-event timing of every
@@ -511,6 +511,7 @@ def run_plain(workload, N, K, warmup, steps, fused=False):
     # the bench's fixed accept rule (deterministic on PR-neutral rotations)
     sim = B.BASS(N, K, optimize_at=ends, device="cuda:0", accept_rtol=1e-10)
     sim.fused = fused
+    sim.prune = prune and not fused
     sim.begin(len(circ))
     log(f"workload {wname} gates={len(circ)}; timing steps {warmup}..{warmup + steps - 1}")
     wms = []
@@ -561,7 +562,8 @@ def run_plain(workload, N, K, warmup, steps, fused=False):
             "warmup_step_ms": wms,
             "config": {"workload": wname, "qubits": N, "k": K, "steps_timed": steps, "steps_untimed": warmup,
                        "optimize": "every step",
-                       "gate_select": "fused (output not stored)" if fused else "stored lift + truncate",
+                       "gate_select": ("fused (output not stored)" if fused else
+                                       "pruned lift + truncate" if prune else "stored lift + truncate"),
                        "gate_counts": {k: v for k, v in delta.items() if not k.startswith("bytes_")}},
             "_sim": sim, "_circ": circ}
 
@@ -633,7 +635,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fused", action="store_true",
                     help="kicked-ising / brickwork: run gates and rotation trials through the fused gate select "
-                         "(bass_gate_truncate, the gate output never stored) instead of the stored lift + truncate")
+                         "(bass_gate_truncate, the gate output never stored) instead of the pruned lift + truncate")
+    ap.add_argument("--stored", action="store_true",
+                    help="kicked-ising / brickwork: store every gate output before the select (bass_lift + "
+                         "bass_truncate) instead of only those the predicted threshold can keep "
+                         "(bass_lift_truncate)")
     args = ap.parse_args()
     if args.calibrate_copy is not None or args.workload not in ("kicked-ising", "brickwork"):
         args.warmup = 4 if args.warmup is None else args.warmup
@@ -680,7 +686,7 @@ def main():
         N, K = args.qubits or 56, 1 << (args.log2k or 26)
         args.warmup = 4 if args.warmup is None else args.warmup
         args.steps = 56 if args.steps is None else args.steps
-    res = run_plain(args.workload, N, K, args.warmup, args.steps, fused=args.fused)
+    res = run_plain(args.workload, N, K, args.warmup, args.steps, fused=args.fused, prune=not args.stored)
     sim, circ = res.pop("_sim"), res.pop("_circ")
     cpu = None if args.no_cpu else cpu_sample(sim, circ, args.workload, N, K, args.cpu_seconds)
     secondary = None
@@ -688,7 +694,7 @@ def main():
         # the brickwork configuration (n=40, K=2^22), reported beside the headline
         del sim
         torch.cuda.empty_cache()
-        sec = run_plain("brickwork", 40, 1 << 22, 5, 25, fused=args.fused)
+        sec = run_plain("brickwork", 40, 1 << 22, 5, 25, fused=args.fused, prune=not args.stored)
         sec.pop("_sim"), sec.pop("_circ")
         secondary = {"metric": "sparse amp-updates/s", "value": sec["value"], "unit": "amp-updates/s",
                      "ms_per_step": sec["ms_per_step"], "steps": 25, "warmup": 5, "config": sec["config"],

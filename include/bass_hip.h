@@ -216,6 +216,32 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
                   double* ao, int64_t cap_out, int renorm, int64_t hist_tiles, int in_flags,
                   const int32_t* key_map, int key_bits, void* ws, size_t ws_bytes, void* stream);
 
+/* bass_lift of (x, a) into the segments (seg_x, seg_a) followed by
+ * bass_truncate of them into (xo, ao), in one call: the results (xo[:nnz],
+ * ao, the stats block, the op log and state-count slot in counted mode, the
+ * next prediction) are bit for bit those of
+ *   bass_lift(x, a, n, nq, j, G, mode | 16 [| 32 with flags bit 1], seg_x, seg_a, seg_cap, ...)
+ *   bass_truncate(seg_x, seg_a, 0, k, xo, ao, cap_out, renorm, seg_tiles,
+ *                 1 [| 4 counted] [| 8 with flags bit 1], key_map, key_bits, ...).
+ * With flags bit 1 (the speculative digit-1 pass, against the bucket of the
+ * workspace's last select) the lift also prunes: it stores only the outputs
+ * whose |a|^2 lies in the predicted threshold's IEEE-exponent bucket, the one
+ * below it or higher -- the others cannot be kept or be candidates unless
+ * the threshold falls further -- while its histogram and sums still cover
+ * every output; when the select's bucket does fall further, or the select
+ * keeps everything and something was dropped, a relift stores every output
+ * before the select reads them (stats.flags bit 3).  Replaces, inside the
+ * driver, a gate step of src/simulation/bass_simulator.py:861-896 with its
+ * truncation at :899-916, and a rotation trial's rotate + truncate at
+ * :768-790 (no reference counterpart as one entry point).  flags bit 0:
+ * counted (n bounds the count read from the state-count slot).  (xo, ao) may
+ * be (x, a); the segments (seg_cap >= seg_tiles * 2048 slots) may overlap
+ * neither. */
+int bass_lift_truncate(const uint64_t* x, const double* a, int64_t n, int nq, int j, const double* G, int mode,
+                       int64_t k, uint64_t* seg_x, double* seg_a, int64_t seg_cap, uint64_t* xo, double* ao,
+                       int64_t cap_out, int renorm, int flags, const int32_t* key_map, int key_bits, void* ws,
+                       size_t ws_bytes, void* stream);
+
 /* A gate or rotation on a key-ordered state followed by the top-k truncation
  * of its output, without storing that output: the results (xo[:nnz], ao,
  * the stats block, the op log and state-count slot in counted mode, the

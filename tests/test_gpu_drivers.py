@@ -220,7 +220,10 @@ def test_device_counts_match_host_counts(gpu_ctx, idx, K):
     assert np.array_equal(x0, x1) and np.array_equal(a0.view(np.float64), a1.view(np.float64))
     assert g0 == g1 and np.array_equal(p0, p1) and all(np.array_equal(a, b) for a, b in zip(u0, u1))
     by_pass = {"bytes_select", "bytes_diag2q"}
-    same = lambda c: {k: v for k, v in c.items() if k not in by_pass and not k.startswith("spec_hit")}
+    # speculation hits and pruned-lift relifts count how each mode predicted
+    # thresholds, not what it computed
+    same = lambda c: {k: v for k, v in c.items()
+                      if k not in by_pass and not k.startswith("spec_hit") and k != "relift"}
     assert same(c0) == same(c1)
     for key in ("bytes_select", "bytes_diag2q"):
         assert c1.get(key, 0) <= c0.get(key, 0), key

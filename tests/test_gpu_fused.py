@@ -1,8 +1,9 @@
-"""The fused gate select (bass_gate_truncate) against the stored pair it
-replaces (bass_lift, then bass_truncate on the stored segmented output),
-through the C ABI: kept keys, amplitudes and stats bit for bit, with the
-speculative prediction holding, missing (the digit-1 pass recomputed from the
-lift's input) and switched off, in plain and counted mode."""
+"""The one-call gate selects against the stored pair they replace (bass_lift,
+then bass_truncate on the stored segmented output), through the C ABI: the
+fused select (bass_gate_truncate, the gate output never stored) and the
+pruned lift (bass_lift_truncate, only the outputs that can matter stored);
+kept keys, amplitudes and stats bit for bit, with the speculative prediction
+holding, missing and switched off, in plain and counted mode."""
 
 import numpy as np
 import pytest
@@ -190,3 +191,98 @@ def test_driver_fused_changes_nothing(gpu_ctx, family):
         assert g0 == g1 and all(np.array_equal(p, q) for p, q in zip(u0, u1))
         assert c0.get("accept", 0) == c1.get("accept", 0) and c0.get("reject", 0) == c1.get("reject", 0)
         print(f"  traced={traced}: {len(d0)} decisions, accepted {c1.get('accept', 0)}", flush=True)
+
+
+# ------------------------------------------------------------ pruned lift --
+def _pruned(ops, dev, ctx, src, n, j, G, nq, mode, k, km, spec, count):
+    seg = dev.Buffers(1, ctx.device)
+    dst = dev.Buffers(k, ctx.device)
+    if count is not None:
+        ops.counted_begin(ctx, count, n)
+        ops.lift_truncate(ctx, src, n, j, seg, dst, k, G, nq=nq, mode=mode, counted=True, spec=spec, key_map=km)
+        log = ops.counted_read(ctx, 4)
+        assert len(log) == 1 and log[0].err == 0
+        return _result(dev, dst, log[0])
+    h = ops.lift_truncate(ctx, src, n, j, seg, dst, k, G, nq=nq, mode=mode, spec=spec, key_map=km)
+    return _result(dev, dst, ops.truncate_finish(ctx, h))
+
+
+@pytest.mark.parametrize("n,nq,mode,k,kind,count", CASES)
+def test_lift_truncate_matches_stored_pair(gpu_ctx, n, nq, mode, k, kind, count):
+    """bass_lift_truncate (a lift that stores only the outputs that can
+    matter to the predicted threshold) against the stored pair, bit for bit:
+    the prediction exact, one or two buckets low, one bucket high (the
+    stored outputs a superset), far too high (the select needs dropped
+    outputs: relift, stats flags bit 3), switched off, and keep-all after a
+    truncating select (a relift whenever something was dropped)."""
+    ops, dev, SS, L = _mods()
+    print(f"[case] pruned lift n={n} nq={nq} mode={mode} k={k} {kind} count={count}", flush=True)
+    nbits = 40
+    rng = np.random.default_rng(n + 10 * nq + mode + 5)
+    x, a = random_state(n, nbits, seed=n + 9)
+    if kind == "ties":
+        a = np.exp(2j * np.pi * rng.random(n)) / np.sqrt(n)
+    pos = rng.permutation(nbits)
+    w = _stored(x, pos)
+    o = np.argsort(w)
+    w, aw = w[o], a[o]
+    if kind == "ties":
+        G = np.array([[0, 1], [1, 0]], complex) if nq == 1 else np.diag(np.exp(1j * rng.random(4)))
+    else:
+        G = random_unitary(1 << nq, rng)
+    if nq == 1:
+        j = int(pos[int(rng.integers(nbits))])
+        npos = SS.lifted_positions(pos, j, 0)
+    else:
+        q2 = int(np.flatnonzero(pos == 0)[0])
+        q1 = int(rng.choice([q for q in range(nbits) if q != q2]))
+        j = int(pos[q1])
+        npos = SS.lifted_positions(pos, j, 1)
+    km = ops.key_map_arg(npos)
+    src = _upload(dev, gpu_ctx, w, aw)
+    ref = _pair(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, False, count)
+    keepall = (ref[2].flags & 1) == 0
+    got = {"plain": _pruned(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, False, count)}
+    # predictions set by a stored select on the state scaled by s: |a|^2
+    # buckets shift by log2(s^2); the pruned call then predicts that bucket
+    for name, scale in [("exact", 1.0), ("one low", 2.0 ** -0.5), ("two low", 0.5), ("one high", 2.0 ** 0.5),
+                        ("far high", 16.0)]:
+        pre = _upload(dev, gpu_ctx, w, aw * scale)
+        _pair(ops, dev, gpu_ctx, pre, n, j, G, nq, mode, k, km, False, count)
+        del pre
+        got[name] = _pruned(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, True, count)
+    for name, g in got.items():
+        _same(g, ref, name)
+    if not keepall:
+        assert got["exact"][2].flags & 4 and not got["exact"][2].flags & 8
+        assert not got["one high"][2].flags & 8 and not got["two low"][2].flags & 8
+        assert got["far high"][2].flags & 8
+    print(f"  kept {ref[2].nnz} of {ref[2].n_in} (keep-all {keepall}); relifts "
+          f"{[n for n, g in got.items() if g[2].flags & 8]}", flush=True)
+
+
+@pytest.mark.parametrize("family", ["kicked", "brickwork"])
+def test_driver_prune_changes_nothing(gpu_ctx, family):
+    """The counted driver with its gate and trial lifts pruned and stored in
+    full: identical keys, amplitudes, gamma, frames and counters."""
+    print(f"[case] driver prune on/off: {family}", flush=True)
+    rc = pkg_module("utils.random_circuits")
+    B = pkg_module("simulation.bass_simulator")
+    if family == "kicked":
+        N, K = 18, 1 << 14
+        circ, ends = rc.kicked_ising_circuit(N, 6)
+    else:
+        N, K = 16, 1 << 11
+        circ, ends = rc.brickwork_haar_circuit(N, 8, np.random.default_rng(79))
+    out = []
+    for prune in (False, True):
+        sim = B.BASS(N, K, device="cuda:0", optimize_at=ends, accept_rtol=1e-10)
+        sim.prune = prune
+        st = sim.simulate(circ)
+        out.append((st.keys().copy(), st.alpha[: st.nnz].copy(), st.gamma, [u.copy() for u in sim.U],
+                    dict(sim.counters)))
+    (k0, a0, g0, u0, c0), (k1, a1, g1, u1, c1) = out
+    assert np.array_equal(k0, k1) and np.array_equal(a0.view(np.float64), a1.view(np.float64))
+    assert g0 == g1 and all(np.array_equal(p, q) for p, q in zip(u0, u1))
+    assert {k: v for k, v in c0.items() if k != "relift"} == {k: v for k, v in c1.items() if k != "relift"}
+    print(f"  truncations {c1.get('truncate', 0)}, relifts {c1.get('relift', 0)}", flush=True)
