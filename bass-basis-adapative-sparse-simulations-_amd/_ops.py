@@ -125,14 +125,16 @@ def truncate_counted(ctx, src, k, dst, hist_tiles, key_map=None, spec=False):
 
 
 def lift_truncate(ctx, src, n, j, seg, dst, k, G, nq=1, mode=L.MODE_GATE, renorm=True, counted=False, spec=False,
-                  key_map=None):
+                  key_map=None, margin=1):
     """A gate or rotation lift of src[:n] into the segments seg, then the
     top-k select of them into dst (<= k entries) (bass_lift_truncate): bit for
     bit lift(..., count=False, spec) + truncate_launch(..., segmented=True,
     spec), or with counted=True (n bounds the count in the state-count slot)
     lift_counted + truncate_counted.  With spec the lift stores only the
     outputs that can matter to the predicted threshold (and stores them all
-    again when the select finds it needs more).  dst may be src; seg may
+    again when the select finds it needs more); margin 1 keeps the binade
+    below the predicted bucket too, margin 0 only the bucket and above
+    (flags bit 2).  dst may be src; seg may
     share memory with neither.  Nothing waits for the device; a non-counted
     call's stats come from truncate_finish(ctx, handle)."""
     tm = SEG // (1 << nq)
@@ -146,7 +148,9 @@ def lift_truncate(ctx, src, n, j, seg, dst, k, G, nq=1, mode=L.MODE_GATE, renorm
     else:
         km = np.ascontiguousarray(key_map, dtype=np.int32)
         kp, kb = km.ctypes.data_as(ctypes.c_void_p), len(km)
-    fl = (L.GT_COUNTED if counted else 0) | (L.GT_SPEC if spec else 0)
+    if margin not in (0, 1):
+        raise ValueError("lift_truncate: margin must be 0 or 1")
+    fl = (L.GT_COUNTED if counted else 0) | (L.GT_SPEC if spec else 0) | (4 if margin == 0 else 0)
     rc = ctx.lib.bass_lift_truncate(src.xp, src.ap, int(n), int(nq), int(j), gp, int(mode), int(k), seg.xp, seg.ap,
                                     seg.cap, dst.xp, dst.ap, dst.cap, 1 if renorm else 0, fl, kp, kb, ctx.wsp,
                                     ctx.ws_bytes, ctx.stream)

@@ -728,7 +728,7 @@ static int truncate_impl(const uint64_t* x, const double* a, int64_t n, int64_t 
   P.x = fused ? (const uint64_t*)C.c1k : x;
   P.a = fused ? nullptr : (const double2*)a;
   P.fused = fused ? 1 : 0;
-  P.pruned = relift ? 1 : 0;
+  P.pruned = relift ? 1 + relift->margin : 0;
   P.seg_cnt = segmented ? at<int>(ws, L.seg_cnt) : nullptr;
   P.n_dense = n;
   P.tile0 = 0;
@@ -854,7 +854,10 @@ int bass_lift_truncate(const uint64_t* x, const double* a, int64_t n, int nq, in
                        size_t ws_bytes, void* stream) {
   const bool counted = (flags & 1) != 0;
   const bool spec = (flags & 2) != 0;
-  if (flags & ~3) return fail(BASS_E_ARG, "bass_lift_truncate: unknown flags 0x%x", flags);
+  // bit 2: prune to the predicted bucket itself (margin 0) instead of one
+  // binade below it
+  const unsigned int margin = (flags & 4) ? 0u : PRUNE_MARGIN;
+  if (flags & ~7) return fail(BASS_E_ARG, "bass_lift_truncate: unknown flags 0x%x", flags);
   if (nq != 1 && nq != 2) return fail(BASS_E_ARG, "bass_lift_truncate: nq must be 1 or 2");
   if (mode != MODE_GATE && mode != MODE_ROT)
     return fail(BASS_E_ARG, "bass_lift_truncate: mode must be 0 (gate) or 1 (rotation)");
@@ -912,6 +915,7 @@ int bass_lift_truncate(const uint64_t* x, const double* a, int64_t n, int nq, in
     sp.out.cnt1 = at<int>(ws, L.cnt1);
     sp.out.tile_dsum = at<double>(ws, L.tile_dsum);
     sp.out.tile_dcnt = at<int>(ws, L.tile_dcnt);
+    sp.margin = margin;
   }
   const int64_t* n_dev = counted ? at<int64_t>(ws, L.count_slot) : nullptr;
   LiftBound* bounds = at<LiftBound>(ws, L.lift_b);
@@ -934,6 +938,7 @@ int bass_lift_truncate(const uint64_t* x, const double* a, int64_t n, int nq, in
   R.j = j;
   R.bounds = bounds;
   R.G = gm;
+  R.margin = (int)margin;
   return truncate_impl(seg_x, seg_a, 0, k, xo, ao, cap_out, renorm, tiles, 1 | (counted ? 4 : 0) | (spec ? 8 : 0),
                        key_map, key_bits, ws, ws_bytes, stream, true, nullptr, spec ? &R : nullptr);
 }

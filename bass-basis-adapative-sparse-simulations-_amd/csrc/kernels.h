@@ -193,6 +193,7 @@ struct Ex1Out {
 struct LiftSpec {
   const unsigned int* pred;
   Ex1Out out;
+  unsigned int margin;  // a pruning lift keeps buckets >= *pred - margin (0 or PRUNE_MARGIN)
 };
 
 int launch_lift(int qb, int mode, const uint64_t* x, const double2* a, int64_t n, int j, const GateMat& G,
@@ -269,7 +270,9 @@ struct SelProblem {
   // after a missed speculation and the compaction recompute the lift's tiles
   int fused;
   // the input comes from a pruned lift issued just before, whose prediction
-  // *spec_word still holds: k_find1 checks that nothing it dropped is needed
+  // *spec_word still holds: k_find1 checks that nothing it dropped is needed.
+  // 0: not pruned; else 1 + the lift's margin (it dropped buckets below
+  // *spec_word - margin)
   int pruned;
   unsigned int* hist1;   // HIST_BINS, zeroed by the caller unless filled by the producing gate
   unsigned int* hist2;   // HIST_BINS, zeroed
@@ -338,6 +341,7 @@ struct LiftRecompute {
   int j;
   const LiftBound* bounds;
   GateMat G;
+  int margin;  // relift of a pruned lift: the margin it pruned with
 };
 // Fused select kernels over the lift's tiles: which = 0 the digit-1 pass when
 // the speculation missed (ext tiles per digit-2 row), 1 the compaction; 2 the

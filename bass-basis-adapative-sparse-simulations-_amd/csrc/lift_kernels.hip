@@ -549,13 +549,14 @@ struct LiftTileOut {
 // [tile * TM, ...)) into Q.  Ends with a barrier; L.t and L.a are free after it.
 // PRUNE: every output's |a|^2 is added to psum (in the stored lift's order)
 // and to the digit-1 histogram (LDS bins below LBINS, else hist), and only
-// outputs in bucket b1p - PRUNE_MARGIN or above (all when b1p is 0xFFFFFFFF)
+// outputs in bucket b1p - margin or above (all when b1p is 0xFFFFFFFF)
 // enter Q's mask and offsets.
 template <int QB, int MODE, bool PRUNE = false>
 __device__ __forceinline__ void lift_tile_compute(const uint64_t* __restrict__ w, const double2* __restrict__ a,
                                                   int64_t n, int j, const LiftBound& B0, const LiftBound& B1,
                                                   const GateMat& G, int64_t tile, LiftTileLds<QB>& L,
-                                                  LiftTileOut<QB>& Q, unsigned int b1p = 0, double* psum = nullptr,
+                                                  LiftTileOut<QB>& Q, unsigned int b1p = 0, unsigned int margin = 0,
+                                                  double* psum = nullptr,
                                                   unsigned int* s_hist = nullptr, unsigned int* hist = nullptr) {
   constexpr int M = 1 << QB;
   constexpr int T = QB - 1;
@@ -603,7 +604,7 @@ __device__ __forceinline__ void lift_tile_compute(const uint64_t* __restrict__ w
               atomicAdd(&s_hist[bin], 1u);
             else
               atomicAdd(&hist[bin], 1u);
-            if (b1p == 0xFFFFFFFFu || bin + PRUNE_MARGIN >= b1p) mask |= 1u << o;
+            if (b1p == 0xFFFFFFFFu || bin + margin >= b1p) mask |= 1u << o;
           }
         } else {
           if (any) mask |= 1u << o;
@@ -664,7 +665,7 @@ __device__ __forceinline__ void lift_to_slots(const LiftTileOut<QB>& Q,
 // values staged in LDS in slot order.  FUSE: the outputs are not stored (a
 // fused select recomputes them); the speculative pass keeps the candidates'
 // keys (S.out.c1k).  PRUNE 1: the histogram and |a|^2 sum over every output,
-// but only outputs in bucket *S.pred - PRUNE_MARGIN or above take a slot (in
+// but only outputs in bucket *S.pred - S.margin or above take a slot (in
 // the same order); PRUNE 2 (a relift): every output stored, no histogram or
 // sum (the pruned lift made them).
 template <int QB, int MODE, bool SPEC = false, bool FUSE = false, int PRUNE = 0>
@@ -687,7 +688,7 @@ __device__ __forceinline__ void lift_gate_tile(const uint64_t* __restrict__ w, c
   LiftTileOut<QB> Q;
   double psum = 0.0;
   if constexpr (PRUNE == 1)
-    lift_tile_compute<QB, MODE, true>(w, a, n, j, B0, B1, G, tile, L, Q, *S.pred, &psum, s_hist, hist);
+    lift_tile_compute<QB, MODE, true>(w, a, n, j, B0, B1, G, tile, L, Q, *S.pred, S.margin, &psum, s_hist, hist);
   else
     lift_tile_compute<QB, MODE>(w, a, n, j, B0, B1, G, tile, L, Q);
   if (threadIdx.x == 0) seg_cnt[otile] = Q.total;

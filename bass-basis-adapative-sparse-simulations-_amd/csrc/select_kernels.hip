@@ -240,14 +240,15 @@ __device__ __forceinline__ void find1_body(const SelProblem& P, const SelTiles& 
   // the bucket a speculative (and pruning) lift predicted, read before this
   // kernel replaces it
   const unsigned int pred = P.spec_word ? *P.spec_word : 0xFFFFFFFFu;
+  const unsigned int margin = P.pruned > 0 ? (unsigned int)(P.pruned - 1) : 0u;
   __syncthreads();
   // input count = sum of the digit-1 histogram; a pruned lift's dropped
-  // outputs are those below bucket pred - PRUNE_MARGIN
+  // outputs are those below bucket pred - margin
   int cnt = 0, drop = 0;
   for (int b = threadIdx.x; b < HIST_BINS; b += NTB) {
     const int h = (int)P.hist1[b];
     cnt += h;
-    drop += (unsigned int)b + PRUNE_MARGIN < pred ? h : 0;
+    drop += (unsigned int)b + margin < pred ? h : 0;
   }
   int count;
   (void)block_excl_scan<NTB>(cnt, count, s_scan);
@@ -284,10 +285,10 @@ __device__ __forceinline__ void find1_body(const SelProblem& P, const SelTiles& 
         st->flags |= 4;
       }
       // a pruned lift dropped outputs this select needs (it keeps everything,
-      // or its bucket lies more than PRUNE_MARGIN below the prediction): the
+      // or its bucket lies more than the margin below the prediction): the
       // relift stores them all, with the digit-1 pass of this bucket (flags
       // bit 3)
-      if (P.pruned && dropped > 0 && (keepall || s_b + PRUNE_MARGIN < pred)) {
+      if (P.pruned && dropped > 0 && (keepall || s_b + margin < pred)) {
         sel->relift = 1;
         if (!keepall) sel->spec_hit = 1;
         st->flags |= 8;

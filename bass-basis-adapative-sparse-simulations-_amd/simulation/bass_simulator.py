@@ -334,6 +334,10 @@ class BASS:
         # has them all stored again when it needs more (bass_lift_truncate;
         # the same results as the stored pair)
         self.prune = True
+        # binades below the predicted bucket a pruned lift still stores, per
+        # kind: 1-qubit gates rarely see the threshold fall (a relift when it
+        # does), rotation trials and 2-qubit gates often
+        self.prune_margin = {"gate1q": 0, "gate2q": 1, "rot1q": 1}
         # counted diagonal 2-qubit gates not yet launched: (bit1, bit2, diag);
         # a run of them goes to the device as one pass (bass_diag2q_run) when
         # the next other operation or a read of the state comes
@@ -596,6 +600,7 @@ class BASS:
             return handle, trial_pos, state.nnz
         if self.prune and self.speculate:
             handle = _ops.lift_truncate(ctx, state._buf, state.nnz, jb, state._rot, cand, self.k, Vh, nq=1,
+                                        margin=self.prune_margin["rot1q"],
                                         mode=L.MODE_ROT, renorm=True, spec=True, key_map=km)
             return handle, trial_pos, state.nnz
         _, rht = _ops.lift(ctx, state._buf, state.nnz, jb, state._rot, G=Vh, nq=1, mode=L.MODE_ROT, count=False,
@@ -795,6 +800,7 @@ class BASS:
             state._buf, state._tmp = state._tmp, state._buf
         elif self.prune and self.speculate:
             _ops.lift_truncate(ctx, state._buf, self.k, jb, state._tmp, state._buf, self.k, G, nq=nq, counted=True,
+                               margin=self.prune_margin["gate1q" if nq == 1 else "gate2q"],
                                spec=True, key_map=km)
         else:
             ht = _ops.lift_counted(ctx, state._buf, self.k, jb, state._tmp, G, nq=nq, spec=self.speculate)

@@ -228,7 +228,9 @@ int bass_truncate(const uint64_t* x, const double* a, int64_t n, int64_t k, uint
  * whose |a|^2 lies in the predicted threshold's IEEE-exponent bucket, the one
  * below it or higher -- the others cannot be kept or be candidates unless
  * the threshold falls further -- while its histogram and sums still cover
- * every output; when the select's bucket does fall further, or the select
+ * every output (flags bit 2: only the predicted bucket or higher -- cheaper
+ * when the threshold rarely falls, a relift whenever it does); when the
+ * select's bucket does fall further, or the select
  * keeps everything and something was dropped, a relift stores every output
  * before the select reads them (stats.flags bit 3).  Replaces, inside the
  * driver, a gate step of src/simulation/bass_simulator.py:861-896 with its

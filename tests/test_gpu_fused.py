@@ -194,16 +194,17 @@ def test_driver_fused_changes_nothing(gpu_ctx, family):
 
 
 # ------------------------------------------------------------ pruned lift --
-def _pruned(ops, dev, ctx, src, n, j, G, nq, mode, k, km, spec, count):
+def _pruned(ops, dev, ctx, src, n, j, G, nq, mode, k, km, spec, count, margin=1):
     seg = dev.Buffers(1, ctx.device)
     dst = dev.Buffers(k, ctx.device)
     if count is not None:
         ops.counted_begin(ctx, count, n)
-        ops.lift_truncate(ctx, src, n, j, seg, dst, k, G, nq=nq, mode=mode, counted=True, spec=spec, key_map=km)
+        ops.lift_truncate(ctx, src, n, j, seg, dst, k, G, nq=nq, mode=mode, counted=True, spec=spec, key_map=km,
+                          margin=margin)
         log = ops.counted_read(ctx, 4)
         assert len(log) == 1 and log[0].err == 0
         return _result(dev, dst, log[0])
-    h = ops.lift_truncate(ctx, src, n, j, seg, dst, k, G, nq=nq, mode=mode, spec=spec, key_map=km)
+    h = ops.lift_truncate(ctx, src, n, j, seg, dst, k, G, nq=nq, mode=mode, spec=spec, key_map=km, margin=margin)
     return _result(dev, dst, ops.truncate_finish(ctx, h))
 
 
@@ -214,7 +215,9 @@ def test_lift_truncate_matches_stored_pair(gpu_ctx, n, nq, mode, k, kind, count)
     the prediction exact, one or two buckets low, one bucket high (the
     stored outputs a superset), far too high (the select needs dropped
     outputs: relift, stats flags bit 3), switched off, and keep-all after a
-    truncating select (a relift whenever something was dropped)."""
+    truncating select (a relift whenever something was dropped); with the
+    default margin (the binade below the prediction stored too) and margin 0
+    (a prediction one bucket high then relifts)."""
     ops, dev, SS, L = _mods()
     print(f"[case] pruned lift n={n} nq={nq} mode={mode} k={k} {kind} count={count}", flush=True)
     nbits = 40
@@ -245,18 +248,22 @@ def test_lift_truncate_matches_stored_pair(gpu_ctx, n, nq, mode, k, kind, count)
     got = {"plain": _pruned(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, False, count)}
     # predictions set by a stored select on the state scaled by s: |a|^2
     # buckets shift by log2(s^2); the pruned call then predicts that bucket
-    for name, scale in [("exact", 1.0), ("one low", 2.0 ** -0.5), ("two low", 0.5), ("one high", 2.0 ** 0.5),
-                        ("far high", 16.0)]:
-        pre = _upload(dev, gpu_ctx, w, aw * scale)
-        _pair(ops, dev, gpu_ctx, pre, n, j, G, nq, mode, k, km, False, count)
-        del pre
-        got[name] = _pruned(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, True, count)
+    for margin in (1, 0):
+        for name, scale in [("exact", 1.0), ("one low", 2.0 ** -0.5), ("two low", 0.5), ("one high", 2.0 ** 0.5),
+                            ("far high", 16.0)]:
+            pre = _upload(dev, gpu_ctx, w, aw * scale)
+            _pair(ops, dev, gpu_ctx, pre, n, j, G, nq, mode, k, km, False, count)
+            del pre
+            got[f"{name} m{margin}"] = _pruned(ops, dev, gpu_ctx, src, n, j, G, nq, mode, k, km, True, count, margin)
     for name, g in got.items():
         _same(g, ref, name)
     if not keepall:
-        assert got["exact"][2].flags & 4 and not got["exact"][2].flags & 8
-        assert not got["one high"][2].flags & 8 and not got["two low"][2].flags & 8
-        assert got["far high"][2].flags & 8
+        for m in (1, 0):
+            assert got[f"exact m{m}"][2].flags & 4 and not got[f"exact m{m}"][2].flags & 8
+            assert not got[f"two low m{m}"][2].flags & 8
+            assert got[f"far high m{m}"][2].flags & 8
+        assert not got["one high m1"][2].flags & 8
+        assert got["one high m0"][2].flags & 8
     print(f"  kept {ref[2].nnz} of {ref[2].n_in} (keep-all {keepall}); relifts "
           f"{[n for n, g in got.items() if g[2].flags & 8]}", flush=True)
 
